@@ -1,0 +1,204 @@
+/*
+ * svdh: MI355X-native (gfx950) sparse SVD hot path of single-svdlib v1.0.9.
+ *
+ * C ABI. Every entry point returns an int32_t svdh_status; on failure
+ * svdh_last_error() returns a thread-local message. The ABI never aborts.
+ * A context owns one HIP stream and is not thread-safe: callers that share
+ * a context across threads must serialise calls (the Rust binding wraps it in
+ * a Mutex, which is what lets its operator type be Sync).
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   SMat<T> trait                          src/utils.rs:22-33
+ *   CsrMatrix as SMat                      src/lanczos/mod.rs:1506-1655
+ *   MaskedCSRMatrix                        src/lanczos/masked.rs:13-875
+ *   lanczos::{svd, svd_dim, svd_dim_seed, svd_las2}   src/lanczos/mod.rs:27-189
+ *   randomized::randomized_svd             src/randomized/mod.rs:24-199
+ *   SvdRec, Diagnostics                    src/utils.rs:43-79
+ *   SvdLibError                            src/error.rs:3-22
+ */
+#ifndef SVDH_H
+#define SVDH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 1..6 map one-to-one onto SvdLibError (src/error.rs:5-21). */
+typedef enum {
+  SVDH_OK = 0,
+  SVDH_ERR_IMTQLB = 1,  /* SvdLibError::ImtqlbError */
+  SVDH_ERR_STARTV = 2,  /* SvdLibError::StartvError */
+  SVDH_ERR_STPONE = 3,  /* SvdLibError::StponeError */
+  SVDH_ERR_IMTQL2 = 4,  /* SvdLibError::Imtql2Error */
+  SVDH_ERR_LAS2 = 5,    /* SvdLibError::Las2Error */
+  SVDH_ERR_NDARRAY = 6, /* SvdLibError::NDArrayError */
+  SVDH_ERR_SHAPE = 7,   /* shape mismatch (the reference panics via assert_eq!) */
+  SVDH_ERR_ALLOC = 8,   /* device or host allocation failed */
+  SVDH_ERR_HIP = 9,     /* HIP runtime error */
+  SVDH_ERR_ARG = 10,    /* invalid argument (null pointer, bad enum, nnz >= 2^31) */
+  SVDH_ERR_NO_DEVICE = 11
+} svdh_status;
+
+/* Element types at the boundary. */
+typedef enum { SVDH_F64 = 0, SVDH_F32 = 1, SVDH_BF16 = 2 } svdh_dtype;
+
+/* Arithmetic of a solve.
+ *  SVDH_PREC_F64:  T = f64. FP64 values, vectors, panels and recurrence.
+ *  SVDH_PREC_F32:  T = f32. LAS2: FP32 sparse values, FP64 vectors and
+ *                  recurrence, f32::EPSILON tolerances (the reference's
+ *                  svd_las2::<f32> control flow). rSVD: FP32 values and
+ *                  panels with FP32 accumulation, FP64 small dense algebra.
+ *  SVDH_PREC_BF16: rSVD only. BF16 values and BF16 input panels with FP32
+ *                  accumulation for the early power iterations, FP32 for the
+ *                  last ones (precision ladder), FP64 small dense algebra. */
+typedef enum { SVDH_PREC_F64 = 0, SVDH_PREC_F32 = 1, SVDH_PREC_BF16 = 2 } svdh_precision;
+
+/* randomized::PowerIterationNormalizer (src/randomized/mod.rs:14-19). */
+typedef enum { SVDH_NORM_QR = 0, SVDH_NORM_LU = 1, SVDH_NORM_NONE = 2 } svdh_normalizer;
+
+/* svdh_csr_create flags */
+#define SVDH_CSR_KEEP_F32 0x1u  /* also keep an FP32 copy of the values */
+#define SVDH_CSR_KEEP_BF16 0x2u /* also keep a BF16 copy of the values */
+
+typedef struct svdh_ctx svdh_ctx;
+typedef struct svdh_op svdh_op;
+
+/* Diagnostics, field for field (src/utils.rs:66-79). */
+typedef struct {
+  int64_t non_zero;
+  int64_t dimensions;
+  int64_t iterations;
+  int32_t transposed;
+  int64_t lanczos_steps;
+  int64_t ritz_values_stabilized;
+  int64_t significant_values;
+  int64_t singular_values;
+  double end_interval[2];
+  double kappa;
+  uint32_t random_seed;
+} svdh_diagnostics;
+
+/* SvdRec (src/utils.rs:43-50). Host arrays owned by the library, freed by
+ * svdh_result_free. u is u_rows x u_cols row-major, vt is vt_rows x vt_cols
+ * row-major, s has d entries. Layouts are the reference's: LAS2 returns
+ * u as d x m (rows are left vectors, src/lanczos/mod.rs:172); the randomized
+ * solver returns u as m x d (columns are left vectors,
+ * src/randomized/mod.rs:175-185). Values are FP64 regardless of precision. */
+typedef struct {
+  int64_t d;
+  int64_t u_rows, u_cols, vt_rows, vt_cols;
+  double *u;
+  double *s;
+  double *vt;
+  svdh_diagnostics diagnostics;
+  /* Extra (not in the reference): power iterations actually run, the
+   * residual estimate the adaptive randomized solver stopped on, and how
+   * many QR normalisations fell back from CholeskyQR2 to the eigen path. */
+  int64_t power_iters_run;
+  double residual_estimate;
+  int64_t qr_fallbacks;
+} svdh_result;
+
+/* ---- context ---------------------------------------------------------- */
+
+int32_t svdh_ctx_create(int32_t device, svdh_ctx **out);
+int32_t svdh_ctx_destroy(svdh_ctx *ctx);
+int32_t svdh_ctx_synchronize(svdh_ctx *ctx);
+const char *svdh_last_error(void);
+/* Kernel timing with HIP events on the context's stream. Kernel classes:
+ * 0 = SpMV, 1 = SpMM, 2 = dense (Gram / apply / GEMM), 3 = BLAS-1,
+ * 4 = basis GEMV (reorthogonalisation). total_ms and launches accumulate
+ * until svdh_ctx_profile is called again with enable = 1. */
+int32_t svdh_ctx_profile(svdh_ctx *ctx, int32_t enable);
+int32_t svdh_ctx_kernel_time(svdh_ctx *ctx, int32_t kernel_class, double *total_ms,
+                             int64_t *launches, double *algorithmic_bytes);
+
+/* ---- operators (SMat<T>) --------------------------------------------- */
+
+/* Upload a CSR matrix: rowptr[nrows+1] and colidx[nnz] as usize-like int64
+ * (nalgebra-sparse csr_data(), src/lanczos/mod.rs:1547), values of val_dtype.
+ * Indices are narrowed to int32; nnz >= 2^31 or an index out of range is
+ * SVDH_ERR_ARG. The transpose (CSC) is built once here. Replaces the
+ * nalgebra_sparse::CsrMatrix operand of SMat (src/lanczos/mod.rs:1506). */
+int32_t svdh_csr_create(svdh_ctx *ctx, int64_t nrows, int64_t ncols, int64_t nnz,
+                        const int64_t *rowptr, const int64_t *colidx, const void *vals,
+                        int32_t val_dtype, uint32_t flags, svdh_op **out);
+
+/* MaskedCSRMatrix::new(matrix, column_mask) (src/lanczos/masked.rs:21-46).
+ * mask has ncols(base) bytes, nonzero = column kept. The view borrows base:
+ * base must outlive it. Masked nnz is counted once here (masked.rs:89-102
+ * rescans on every call). */
+int32_t svdh_mask_create(svdh_ctx *ctx, const svdh_op *base, const uint8_t *mask,
+                         svdh_op **out);
+int32_t svdh_op_destroy(svdh_op *op);
+
+/* SMat::nrows / ncols / nnz (src/utils.rs:23-25). */
+int32_t svdh_op_shape(const svdh_op *op, int64_t *nrows, int64_t *ncols, int64_t *nnz);
+
+/* SMat::svd_opa (src/utils.rs:26): y = A x, or y = A^T x when transposed.
+ * y is overwritten. x and y are host pointers unless on_device != 0, in which
+ * case they are device pointers valid on the context's device. vec_dtype is
+ * SVDH_F64 or SVDH_F32. */
+int32_t svdh_spmv(svdh_ctx *ctx, const svdh_op *op, int32_t transposed, const void *x, void *y,
+                  int32_t vec_dtype, int32_t on_device);
+
+/* SMat::multiply_with_dense (src/utils.rs:28, masked.rs:265-471).
+ * transpose_self == 0: Y (nrows x l) = A X (ncols x l);
+ * transpose_self != 0: Y (ncols x l) = A^T X (nrows x l).
+ * Host panels are column-major (nalgebra DMatrix) with leading dimensions
+ * ldx, ldy. dtype is SVDH_F64 or SVDH_F32. */
+int32_t svdh_spmm(svdh_ctx *ctx, const svdh_op *op, int32_t transpose_self, const void *X,
+                  int64_t ldx, int64_t l, void *Y, int64_t ldy, int32_t dtype);
+
+/* SMat::multiply_transposed_by_dense (src/utils.rs:31, masked.rs:616-733):
+ * B (l x ncols, column-major, ldb) = Q^T A with Q (nrows x l, column-major). */
+int32_t svdh_spmm_qta(svdh_ctx *ctx, const svdh_op *op, const void *Q, int64_t ldq, int64_t l,
+                      void *B, int64_t ldb, int32_t dtype);
+
+/* SMat::compute_column_means (src/utils.rs:27, masked.rs:240-263). means has
+ * ncols entries of dtype (SVDH_F64 or SVDH_F32). */
+int32_t svdh_column_means(svdh_ctx *ctx, const svdh_op *op, void *means, int32_t dtype);
+
+/* ---- whole solves ----------------------------------------------------- */
+
+/* lanczos::svd_las2 (src/lanczos/mod.rs:104-189) with svd / svd_dim /
+ * svd_dim_seed as the callers' defaults. dimensions / iterations 0 = max.
+ * random_seed 0 draws a seed from the OS like the reference (mod.rs:116-119).
+ * start (nullable) is a start vector of the working dimension
+ * (min(nrows, ncols) side chosen by the transpose rule, mod.rs:143-145);
+ * when given it replaces the random start vector, which is the reference's
+ * own behaviour for a non-zero w0 (mod.rs:595-606). precision is
+ * SVDH_PREC_F64 or SVDH_PREC_F32. */
+int32_t svdh_las2(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t iterations,
+                  double endl, double endr, double kappa, uint32_t random_seed,
+                  const double *start, int32_t precision, svdh_result *out);
+
+/* Start vector the LAS2 driver draws for a seed (host only, no GPU):
+ * out[i] = 2 U_i - 1 with U_i = (mix64(key + (i+1) phi) >> 11) 2^-53,
+ * key = seed phi XOR 0x5356444C49423031, phi = 0x9E3779B97F4A7C15 and mix64
+ * the splitmix64 finaliser. */
+int32_t svdh_las2_start_vector(uint32_t seed, int64_t n, double *out);
+
+/* randomized::randomized_svd (src/randomized/mod.rs:24-199).
+ * has_seed == 0 is seed None (deterministic seed 0, randomized/mod.rs:304-307).
+ * omega (nullable) injects the ncols x (rank + oversample) Gaussian test
+ * matrix, column-major FP64; otherwise it is drawn on the device from a
+ * counter-based generator keyed by seed.
+ * tol <= 0: exactly n_power_iters power iterations (the reference).
+ * tol > 0: power iterations continue past n_power_iters, up to
+ * max_power_iters, until the Rayleigh-Ritz residual estimate
+ * max_i sqrt(|A v_i - s_i u_i|^2 + |A^T u_i - s_i v_i|^2) / s_i over the
+ * returned rank drops to tol (normaliser QR only). */
+int32_t svdh_rsvd(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, int64_t n_oversamples,
+                  int64_t n_power_iters, int32_t normalizer, int32_t mean_center,
+                  uint64_t seed, int32_t has_seed, const double *omega, int32_t precision,
+                  double tol, int64_t max_power_iters, svdh_result *out);
+
+void svdh_result_free(svdh_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

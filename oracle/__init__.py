@@ -1,0 +1,35 @@
+"""CPU oracle for the sparse-SVD hot path of single-svdlib v1.0.9.
+
+TEST INFRASTRUCTURE ONLY. This package is the checker of the HIP product
+path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+may import it, and only as the checker or the timed CPU baseline; the product
+(single-svdlib_amd/) never imports, links or calls it.
+
+Parity status of each function (pinned = checked by tests/ against a golden
+vector, a closed form or a library routine chosen so a plausible mistake fails):
+  opa / multiply_with_dense / multiply_transposed_by_dense / column_means
+      pinned: scipy.sparse FP64 products and dense numpy on small inputs
+  pythag      pinned: math.hypot
+  imtqlb, imtql2  pinned: scipy.linalg.eigh_tridiagonal (LAPACK)
+  las2        pinned: the reference's KATs (legacy.rs:137-283, 2184-2235),
+              numpy.linalg.svd on random sparse matrices, and invariants
+  start_vector pinned by its own closed form (the reference's ChaCha12
+              stream is parity unpinned: no Rust toolchain here)
+  randomized_svd (oracle/rsvd.py) pinned: numpy.linalg.svd truth for
+              rank-deficient/low-rank inputs and the Halko bound invariants;
+              nalgebra's Householder QR / SVD sign conventions are parity
+              unpinned (not in the container), so comparisons use subspace
+              angles and singular values.
+"""
+from .pyoracle import (  # noqa: F401
+    OracleOp,
+    las2,
+    imtqlb,
+    imtql2,
+    pythag,
+    start_vector,
+    determine_chunk_size,
+    F64_EPS,
+    F32_EPS,
+)
+from .rsvd import randomized_svd  # noqa: F401

@@ -1,0 +1,412 @@
+// C ABI of the svdh library (include/svdh.h): contexts, operators, the SMat
+// entry points and the whole-solve entry points. Every entry point converts
+// exceptions into a status code and a thread-local message; none aborts.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.h"
+
+namespace svdh {
+
+static thread_local std::string g_last_error;
+
+void fail(int32_t code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Error(code, buf);
+}
+
+void DevBuf::alloc(size_t n) {
+  release();
+  if (n == 0) n = 16;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    fail(SVDH_ERR_ALLOC, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+  }
+  bytes = n;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+void Ctx::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+
+hipEvent_t Ctx::get_event() {
+  if (!free_events.empty()) {
+    hipEvent_t e = free_events.back();
+    free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIP_CHECK(hipEventCreate(&e));
+  return e;
+}
+
+void Ctx::drain_profile() {
+  if (pending.empty()) return;
+  HIP_CHECK(hipStreamSynchronize(stream));
+  for (auto &p : pending) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    prof_ms[p.cls] += ms;
+    prof_n[p.cls] += 1;
+    prof_bytes[p.cls] += p.bytes;
+    free_events.push_back(p.a);
+    free_events.push_back(p.b);
+  }
+  pending.clear();
+}
+
+KTimer::KTimer(Ctx *ctx, int k, double algorithmic_bytes) : c(ctx), cls(k), bytes(algorithmic_bytes) {
+  if (!c->profile) return;
+  a = c->get_event();
+  HIP_CHECK(hipEventRecord(a, c->stream));
+}
+
+KTimer::~KTimer() {
+  if (!a) return;
+  hipEvent_t b = c->get_event();
+  (void)hipEventRecord(b, c->stream);
+  c->pending.push_back(Ctx::Pending{a, b, cls, bytes});
+  if (c->pending.size() > 4096) c->drain_profile();
+}
+
+svdh_op *csr_create(Ctx *c, int64_t nrows, int64_t ncols, int64_t nnz, const int64_t *rowptr, const int64_t *colidx,
+                    const void *vals, int32_t vdt, uint32_t flags);
+svdh_op *mask_create(Ctx *c, const svdh_op *base, const uint8_t *mask);
+int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
+                   double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out);
+int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_oversamples, int64_t n_power_iters,
+                   int32_t normalizer, int32_t mean_center, uint64_t seed, int32_t has_seed, const double *omega,
+                   int32_t precision, double tol, int64_t max_power_iters, svdh_result *out);
+
+template <typename F>
+static int32_t guarded(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return SVDH_OK;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return SVDH_ERR_ALLOC;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return SVDH_ERR_ARG;
+  }
+}
+
+static void check_ctx(svdh_ctx *ctx) {
+  if (!ctx) fail(SVDH_ERR_ARG, "null context");
+  HIP_CHECK(hipSetDevice(ctx->device));
+}
+
+static void check_op(svdh_ctx *ctx, const svdh_op *op) {
+  if (!op) fail(SVDH_ERR_ARG, "null operator");
+  if (op->ctx != ctx) fail(SVDH_ERR_ARG, "operator belongs to another context");
+}
+
+}  // namespace svdh
+
+using namespace svdh;
+
+extern "C" {
+
+const char *svdh_last_error(void) { return g_last_error.c_str(); }
+
+int32_t svdh_ctx_create(int32_t device, svdh_ctx **out) {
+  return guarded([&] {
+    if (!out) fail(SVDH_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+      (void)hipGetLastError();
+      fail(SVDH_ERR_NO_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= count) fail(SVDH_ERR_ARG, "device %d out of range (%d devices)", device, count);
+    HIP_CHECK(hipSetDevice(device));
+    auto c = new svdh_ctx();
+    try {
+      c->device = device;
+      HIP_CHECK(hipStreamCreate(&c->stream));
+      c->scal.alloc(S_NSLOT * sizeof(double));
+      HIP_CHECK(hipMemset(c->scal.p, 0, S_NSLOT * sizeof(double)));
+      HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_scal), S_NSLOT * sizeof(double), hipHostMallocDefault));
+      c->red_part.alloc(1 << 20);
+      c->red_count.alloc(256);
+      HIP_CHECK(hipMemset(c->red_count.p, 0, 256));
+      c->stage.alloc(1 << 16);
+    } catch (...) {
+      if (c->h_scal) (void)hipHostFree(c->h_scal);
+      if (c->stream) (void)hipStreamDestroy(c->stream);
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int32_t svdh_ctx_destroy(svdh_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+    if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int32_t svdh_ctx_synchronize(svdh_ctx *ctx) {
+  return guarded([&] {
+    check_ctx(ctx);
+    ctx->sync();
+  });
+}
+
+int32_t svdh_ctx_profile(svdh_ctx *ctx, int32_t enable) {
+  return guarded([&] {
+    check_ctx(ctx);
+    ctx->drain_profile();
+    for (int i = 0; i < KC_COUNT; ++i) {
+      ctx->prof_ms[i] = 0.0;
+      ctx->prof_n[i] = 0;
+      ctx->prof_bytes[i] = 0.0;
+    }
+    ctx->profile = enable != 0;
+  });
+}
+
+int32_t svdh_ctx_kernel_time(svdh_ctx *ctx, int32_t kernel_class, double *total_ms, int64_t *launches,
+                             double *algorithmic_bytes) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (kernel_class < 0 || kernel_class >= KC_COUNT) fail(SVDH_ERR_ARG, "bad kernel class");
+    ctx->drain_profile();
+    if (total_ms) *total_ms = ctx->prof_ms[kernel_class];
+    if (launches) *launches = ctx->prof_n[kernel_class];
+    if (algorithmic_bytes) *algorithmic_bytes = ctx->prof_bytes[kernel_class];
+  });
+}
+
+int32_t svdh_csr_create(svdh_ctx *ctx, int64_t nrows, int64_t ncols, int64_t nnz, const int64_t *rowptr,
+                        const int64_t *colidx, const void *vals, int32_t val_dtype, uint32_t flags, svdh_op **out) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!out) fail(SVDH_ERR_ARG, "null output pointer");
+    *out = csr_create(ctx, nrows, ncols, nnz, rowptr, colidx, vals, val_dtype, flags);
+  });
+}
+
+int32_t svdh_mask_create(svdh_ctx *ctx, const svdh_op *base, const uint8_t *mask, svdh_op **out) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, base);
+    if (!out) fail(SVDH_ERR_ARG, "null output pointer");
+    *out = mask_create(ctx, base, mask);
+  });
+}
+
+int32_t svdh_op_destroy(svdh_op *op) {
+  return guarded([&] {
+    if (!op) return;
+    (void)hipSetDevice(op->ctx->device);
+    (void)hipStreamSynchronize(op->ctx->stream);
+    delete op;
+  });
+}
+
+int32_t svdh_op_shape(const svdh_op *op, int64_t *nrows, int64_t *ncols, int64_t *nnz) {
+  return guarded([&] {
+    if (!op) fail(SVDH_ERR_ARG, "null operator");
+    if (nrows) *nrows = op->nrows;
+    if (ncols) *ncols = op->ncols;
+    if (nnz) *nnz = op->nnz;
+  });
+}
+
+int32_t svdh_spmv(svdh_ctx *ctx, const svdh_op *op_, int32_t transposed, const void *x, void *y, int32_t vec_dtype,
+                  int32_t on_device) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op_);
+    svdh_op *op = const_cast<svdh_op *>(op_);
+    if (vec_dtype != SVDH_F64 && vec_dtype != SVDH_F32) fail(SVDH_ERR_ARG, "vector dtype must be F64 or F32");
+    if (!x || !y) fail(SVDH_ERR_ARG, "null vector");
+    const int dir = transposed ? 1 : 0;
+    const int64_t nin = dir ? op->nrows : op->ncols;
+    const int64_t nout = dir ? op->ncols : op->nrows;
+    const size_t es = dtype_size(vec_dtype);
+    // Values: the operator's own dtype for F64 vectors; FP32 values for FP32
+    // vectors unless the matrix is stored in FP64 only (then FP64 values).
+    const svdh_op *own = op->base ? op->base : op;
+    int vdt = own->base_dtype;
+    if (vec_dtype == SVDH_F64 && vdt == SVDH_BF16) fail(SVDH_ERR_ARG, "BF16 operators take F32 vectors");
+    if (vdt == SVDH_BF16) fail(SVDH_ERR_ARG, "SpMV on BF16 values is not provided");
+    if (on_device) {
+      spmv(op, dir, vdt, x, y, vec_dtype);
+      ctx->sync();
+      return;
+    }
+    DevBuf dx(std::max<size_t>(nin * es, 16)), dy(std::max<size_t>(nout * es, 16));
+    HIP_CHECK(hipMemcpyAsync(dx.p, x, nin * es, hipMemcpyHostToDevice, ctx->stream));
+    spmv(op, dir, vdt, dx.p, dy.p, vec_dtype);
+    HIP_CHECK(hipMemcpyAsync(y, dy.p, nout * es, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
+}
+
+// Column-major host panel (rows x l, ld) -> row-major device panel (ld8) and back.
+static void to_rowmajor(const void *src, int64_t rows, int64_t l, int64_t ld, int dt, std::vector<unsigned char> &dst,
+                        int64_t ldr) {
+  const size_t es = dtype_size(dt);
+  dst.assign((size_t)rows * ldr * es, 0);
+  const unsigned char *s = static_cast<const unsigned char *>(src);
+  for (int64_t j = 0; j < l; ++j)
+    for (int64_t i = 0; i < rows; ++i) std::memcpy(&dst[(i * ldr + j) * es], s + (i + j * ld) * es, es);
+}
+
+static void to_colmajor(const std::vector<unsigned char> &src, int64_t rows, int64_t l, int64_t ldr, int dt, void *dst,
+                        int64_t ld) {
+  const size_t es = dtype_size(dt);
+  unsigned char *d = static_cast<unsigned char *>(dst);
+  for (int64_t j = 0; j < l; ++j)
+    for (int64_t i = 0; i < rows; ++i) std::memcpy(d + (i + j * ld) * es, &src[(i * ldr + j) * es], es);
+}
+
+static void host_spmm(svdh_ctx *ctx, svdh_op *op, int dir, const void *X, int64_t ldx, int64_t l, void *Y, int64_t ldy,
+                      int dt, bool transpose_out) {
+  const int64_t nin = dir ? op->nrows : op->ncols;
+  const int64_t nout = dir ? op->ncols : op->nrows;
+  if (dt != SVDH_F64 && dt != SVDH_F32) fail(SVDH_ERR_ARG, "panel dtype must be F64 or F32");
+  if (l < 0 || ldx < std::max<int64_t>(nin, 1) || (!transpose_out && ldy < std::max<int64_t>(nout, 1)) ||
+      (transpose_out && ldy < std::max<int64_t>(l, 1)))
+    fail(SVDH_ERR_SHAPE, "incompatible panel leading dimension");
+  if (!X || !Y) fail(SVDH_ERR_ARG, "null panel");
+  const svdh_op *own = op->base ? op->base : op;
+  if (own->base_dtype == SVDH_BF16) fail(SVDH_ERR_ARG, "BF16 operators are solver-only");
+  // FP64 panels use the stored values as they are; FP32 panels use FP32 values.
+  const int vuse = dt == SVDH_F64 ? own->base_dtype : SVDH_F32;
+  const int64_t ldr = (l + 7) / 8 * 8;
+  const size_t es = dtype_size(dt);
+  std::vector<unsigned char> hx, hy((size_t)nout * ldr * es);
+  to_rowmajor(X, nin, l, ldx, dt, hx, ldr);
+  DevBuf dx(std::max<size_t>(hx.size(), 16)), dy(std::max<size_t>(hy.size(), 16));
+  if (!hx.empty()) HIP_CHECK(hipMemcpyAsync(dx.p, hx.data(), hx.size(), hipMemcpyHostToDevice, ctx->stream));
+  if (l > 0) spmm(op, dir, vuse, dx.p, ldr, dt, dy.p, ldr, dt, l);
+  if (!hy.empty()) HIP_CHECK(hipMemcpyAsync(hy.data(), dy.p, hy.size(), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  if (!transpose_out) {
+    to_colmajor(hy, nout, l, ldr, dt, Y, ldy);
+  } else {
+    // B (l x nout, column-major ldy) = Y^T: element (c, i) at c + i*ldy.
+    unsigned char *d = static_cast<unsigned char *>(Y);
+    for (int64_t i = 0; i < nout; ++i)
+      for (int64_t c = 0; c < l; ++c) std::memcpy(d + (c + i * ldy) * es, &hy[(i * ldr + c) * es], es);
+  }
+}
+
+int32_t svdh_spmm(svdh_ctx *ctx, const svdh_op *op, int32_t transpose_self, const void *X, int64_t ldx, int64_t l,
+                  void *Y, int64_t ldy, int32_t dtype) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    host_spmm(ctx, const_cast<svdh_op *>(op), transpose_self ? 1 : 0, X, ldx, l, Y, ldy, dtype, false);
+  });
+}
+
+int32_t svdh_spmm_qta(svdh_ctx *ctx, const svdh_op *op, const void *Q, int64_t ldq, int64_t l, void *B, int64_t ldb,
+                      int32_t dtype) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    // Q^T A = (A^T Q)^T: the transposed SpMM, written transposed.
+    host_spmm(ctx, const_cast<svdh_op *>(op), 1, Q, ldq, l, B, ldb, dtype, true);
+  });
+}
+
+int32_t svdh_column_means(svdh_ctx *ctx, const svdh_op *op_, void *means, int32_t dtype) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op_);
+    if (!means) fail(SVDH_ERR_ARG, "null output");
+    if (dtype != SVDH_F64 && dtype != SVDH_F32) fail(SVDH_ERR_ARG, "dtype must be F64 or F32");
+    svdh_op *op = const_cast<svdh_op *>(op_);
+    // Column means = A^T 1 / nrows (masked.rs:240-263).
+    const svdh_op *own = op->base ? op->base : op;
+    if (own->base_dtype == SVDH_BF16) fail(SVDH_ERR_ARG, "column means need F32 or F64 values");
+    std::vector<double> ones(op->nrows, 1.0), sums(op->ncols, 0.0);
+    DevBuf dx(std::max<size_t>(op->nrows * 8, 16)), dy(std::max<size_t>(op->ncols * 8, 16));
+    HIP_CHECK(hipMemcpyAsync(dx.p, ones.data(), op->nrows * 8, hipMemcpyHostToDevice, ctx->stream));
+    spmv(op, 1, own->base_dtype, dx.p, dy.p, SVDH_F64);
+    HIP_CHECK(hipMemcpyAsync(sums.data(), dy.p, op->ncols * 8, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    const double r = op->nrows ? 1.0 / (double)op->nrows : 0.0;
+    for (int64_t j = 0; j < op->ncols; ++j) {
+      if (dtype == SVDH_F64)
+        static_cast<double *>(means)[j] = sums[j] * r;
+      else
+        static_cast<float *>(means)[j] = (float)(sums[j] * r);
+    }
+  });
+}
+
+int32_t svdh_las2(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
+                  double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out) {
+  if (out) std::memset(out, 0, sizeof *out);
+  int32_t st = guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    if (!out) fail(SVDH_ERR_ARG, "null result");
+    if (precision == SVDH_PREC_F32) ensure_values(const_cast<svdh_op *>(op), SVDH_F32);
+    las2_solve(ctx, const_cast<svdh_op *>(op), dimensions, iterations, endl, endr, kappa, random_seed, start,
+               precision, out);
+  });
+  if (st != SVDH_OK && out) svdh_result_free(out);
+  return st;
+}
+
+int32_t svdh_rsvd(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, int64_t n_oversamples,
+                  int64_t n_power_iters, int32_t normalizer, int32_t mean_center, uint64_t seed, int32_t has_seed,
+                  const double *omega, int32_t precision, double tol, int64_t max_power_iters, svdh_result *out) {
+  if (out) std::memset(out, 0, sizeof *out);
+  int32_t st = guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    if (!out) fail(SVDH_ERR_ARG, "null result");
+    rsvd_solve(ctx, const_cast<svdh_op *>(op), target_rank, n_oversamples, n_power_iters, normalizer, mean_center,
+               seed, has_seed, omega, precision, tol, max_power_iters, out);
+  });
+  if (st != SVDH_OK && out) svdh_result_free(out);
+  return st;
+}
+
+void svdh_result_free(svdh_result *res) {
+  if (!res) return;
+  std::free(res->u);
+  std::free(res->s);
+  std::free(res->vt);
+  res->u = res->s = res->vt = nullptr;
+}
+
+}  // extern "C"

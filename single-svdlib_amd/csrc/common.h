@@ -1,0 +1,218 @@
+// Internal declarations shared by the svdh library sources (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/svdh.h"
+
+namespace svdh {
+
+// Error carried from the point of failure to the C ABI, where it becomes a
+// status code plus the thread-local message.
+struct Error : std::runtime_error {
+  int32_t code;
+  Error(int32_t c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int32_t code, const char *fmt, ...);
+
+#define HIP_CHECK(expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      ::svdh::fail(SVDH_ERR_HIP, "%s failed at %s:%d: %s", #expr, __FILE__, __LINE__, \
+                   hipGetErrorString(e_));                                           \
+  } while (0)
+
+// Owning device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { alloc(n); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t n);
+  void ensure(size_t n) { if (n > bytes) alloc(n); }
+  void release();
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Kernel classes for event timing (svdh_ctx_kernel_time).
+enum KernelClass { KC_SPMV = 0, KC_SPMM = 1, KC_DENSE = 2, KC_BLAS1 = 3, KC_GEMV = 4, KC_COUNT = 5 };
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // Device scalar block written by reduction kernels, and its pinned mirror.
+  DevBuf scal;
+  double *h_scal = nullptr;
+  // Reduction partials and completion counters (self-resetting).
+  DevBuf red_part;
+  DevBuf red_count;
+  // Small host-to-device matrices (staged, then read by one kernel).
+  DevBuf stage;
+  // Profiling.
+  bool profile = false;
+  struct Pending { hipEvent_t a, b; int cls; double bytes; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> free_events;
+  double prof_ms[KC_COUNT] = {0};
+  int64_t prof_n[KC_COUNT] = {0};
+  double prof_bytes[KC_COUNT] = {0};
+
+  void sync();
+  hipEvent_t get_event();
+  void drain_profile();
+};
+
+// RAII helper: records events around the kernels launched in its scope when
+// profiling is on.
+struct KTimer {
+  Ctx *c;
+  int cls;
+  double bytes;
+  hipEvent_t a = nullptr;
+  KTimer(Ctx *ctx, int k, double algorithmic_bytes);
+  ~KTimer();
+};
+
+// Row schedule of one operator direction: output rows sorted by nnz, split
+// into groups-per-row bins, plus block-level chunks of long rows.
+struct BinDesc {
+  int32_t block_begin;  // first workgroup of the bin
+  int32_t row_begin;    // offset in the sorted row list
+  int32_t row_count;
+  int32_t lg;           // log2 lanes per row (SpMV bins)
+};
+
+constexpr int kMaxBins = 8;
+constexpr int kBlock = 256;
+
+struct Schedule {
+  bool built = false;
+  int nbins = 0;
+  BinDesc bins[kMaxBins];
+  int32_t long_block_begin = 0;  // first workgroup of the long-row chunks
+  int32_t nchunks = 0;           // number of long-row chunks (one workgroup each)
+  int32_t nlong_rows = 0;
+  int32_t total_blocks = 0;
+  DevBuf rows;      // int32 sorted output rows
+  DevBuf chunks;    // int32 x 5 per chunk: row, begin, end, first_slot, nchunks_of_row
+  DevBuf counters;  // int32 per long row
+  DevBuf partials;  // chunk partial results (grown on demand)
+};
+
+// One direction of an operator as a CSR: rows of this matrix are output rows.
+struct CsrView {
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  const int32_t *rowptr = nullptr;
+  const int32_t *colidx = nullptr;
+  const void *vals[3] = {nullptr, nullptr, nullptr};  // by svdh_dtype
+  const int32_t *rowmap = nullptr;  // output row -> source row (masked A^T)
+  const int32_t *colmap = nullptr;  // source column -> operand index or -1
+  std::vector<int32_t> *h_rowptr = nullptr;  // host row pointers (for schedules)
+  const std::vector<int32_t> *h_rowmap = nullptr;
+};
+
+}  // namespace svdh
+
+// The opaque ABI types.
+struct svdh_ctx : svdh::Ctx {};
+
+struct svdh_op {
+  svdh_ctx *ctx = nullptr;
+  const svdh_op *base = nullptr;  // masked view: the borrowed matrix
+  int64_t nrows = 0, ncols = 0, nnz = 0;
+  int32_t base_dtype = SVDH_F64;
+  // Owned storage (plain CSR only).
+  svdh::DevBuf rowptr, colidx, vals[3];
+  svdh::DevBuf t_rowptr, t_colidx, t_vals[3];  // the transpose (CSC of A)
+  std::vector<int32_t> h_rowptr, h_t_rowptr;
+  // Masked view storage.
+  svdh::DevBuf o2m, m2o;
+  std::vector<int32_t> h_m2o;
+  // Schedules: [direction][kind], direction 0 = A x, 1 = A^T x; kind 0 = SpMV, 1 = SpMM.
+  svdh::Schedule sched[2][2];
+  svdh::CsrView view(int direction) const;
+  const void *values(int direction, int dtype) const;
+};
+
+namespace svdh {
+
+// ---- sparse kernels (sparse.hip) -----------------------------------------
+// y = op_dir x for vectors; dtypes: values vdt, vectors xdt (F64 or F32).
+// Accumulation is FP64 when the vectors are FP64, FP32 otherwise.
+void spmv(svdh_op *op, int direction, int vdt, const void *x, void *y, int xdt);
+// Y = op_dir X for row-major panels of width l; X has leading dimension ldx
+// (elements) and dtype xdt (F64, F32 or BF16); Y has ldy and dtype ydt (F64 or
+// F32). Accumulation is FP64 for FP64 panels, FP32 otherwise.
+void spmm(svdh_op *op, int direction, int vdt, const void *X, int64_t ldx, int xdt, void *Y,
+          int64_t ldy, int ydt, int64_t l);
+void ensure_values(svdh_op *op, int dtype);
+
+// ---- vector / dense kernels (dense.hip) -----------------------------------
+// Scalar slots of Ctx::scal.
+enum Slot { S_A = 0, S_B = 1, S_C = 2, S_D = 3, S_E = 4, S_F = 5, S_NSLOT = 64 };
+// y[i] += coef * x[i] for all listed updates, then dot(u, v) -> scal[out_slot].
+// coef is  sign * scal[slot]  when slot >= 0, else the immediate imm.
+struct Axpy { double *y; const double *x; double imm; int slot; double sign; };
+void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *u,
+                    const double *v, int out_slot, double *copy_dst, const double *copy_src);
+void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double *z);
+void vec_copy(Ctx *c, int64_t n, const double *x, double *y);
+// Basis GEMV: cout[2k] = Q[:, c0:c0+k]^T [u v]  (k <= 4096)
+void basis_gemv_t(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *u,
+                  const double *v, double *cu, double *cv);
+// y1 -= Q c1, y2 -= Q c2 (Q is n x k column-major)
+void basis_gemv_n(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *c1,
+                  double *y1, const double *c2, double *y2);
+// C (n x d row-major, ldc) = Q (n x k col-major, ldq) * W (k x d row-major)
+void basis_combine(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *W,
+                   int64_t d, double *C, int64_t ldc);
+// Gram G (l x l, FP64, row-major) = Y^T Y for row-major Y (m x l, ldy) of dtype ydt.
+void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, double *G_host);
+// Y (m x lo) <- Y (m x li) * M (li x lo, FP64 row-major, host), in place or to Z.
+void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, int ydt,
+                 const double *M_host, void *Z, int64_t ldz, int zdt);
+// nr2[c] = |Y1 M1[:, c] + Y2 M2[:, c]|^2 for c < k (FP64); Y1, Y2 are m x w
+// row-major panels with leading dimension ld, M1/M2 are w x k host matrices.
+void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t k, const void *Y1, const void *Y2, int64_t ld, int dt,
+                    const double *M1, const double *M2, double *nr2_host);
+// Column L2 normalisation, columns with norm <= 1e-10 left as they are.
+void panel_normalize_columns(Ctx *c, int64_t m, int64_t l, void *Y, int64_t ldy, int ydt);
+// Standard normal panel (m x l row-major) from a counter-based generator.
+void panel_gaussian(Ctx *c, int64_t m, int64_t l, uint64_t seed, void *Y, int64_t ldy, int ydt);
+// Convert a panel between dtypes (same shape).
+void panel_convert(Ctx *c, int64_t m, int64_t l, const void *X, int64_t ldx, int xdt, void *Y,
+                   int64_t ldy, int ydt);
+// Transpose-copy between a host column-major panel and a device row-major panel.
+void panel_fill(Ctx *c, int64_t m, int64_t l, double v, void *Y, int64_t ldy, int ydt);
+
+// ---- host dense algebra (hostla.cpp) --------------------------------------
+double pythag(double a, double b, double eps);
+int imtqlb(int64_t n, double *d, double *e, double *bnd, int64_t max_iter, double eps);
+int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_iter, double eps);
+// Symmetric eigen-decomposition of an l x l row-major matrix: eigenvalues
+// ascending in w, eigenvectors as columns of V (row-major l x l).
+void sym_eig(int64_t l, const double *A, double *w, double *V);
+// Cholesky A = R^T R (upper R, row-major). Returns false on a non-positive pivot.
+bool cholesky_upper(int64_t l, const double *A, double *R);
+// Inverse of an upper-triangular matrix.
+void upper_inverse(int64_t l, const double *R, double *Rinv);
+
+size_t dtype_size(int dt);
+
+}  // namespace svdh

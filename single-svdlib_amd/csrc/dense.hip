@@ -1,0 +1,628 @@
+// Vector and dense kernels of the svdh library (gfx950).
+//
+//  - fused FP64 BLAS-1 for the Lanczos step (src/lanczos/mod.rs:321-424,
+//    674-758): y += a x updates followed by a dot product in one pass, the
+//    scalar kept on the device; reductions are two-level with a fixed block
+//    order, so results are run-to-run identical;
+//  - block GEMVs over the stored Lanczos basis for purge's classical
+//    Gram-Schmidt (mod.rs:760-813) and the Ritz back-transform (mod.rs:1150-1167);
+//  - tall-skinny Gram matrices, panel-times-small-matrix products, column
+//    normalisation and Gaussian panels for the randomized solver
+//    (src/randomized/mod.rs:24-313).
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace svdh {
+
+namespace {
+
+typedef uint16_t bf16_t;
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum in fixed order (wave sums, then waves in order); result in
+// thread 0.
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double w[kBlock / 64];
+  v = wave_sum_d(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) t += w[i];
+  }
+  return t;
+}
+
+// Second level of a grid reduction: block partial into part[blockIdx.x]; the
+// last block to arrive adds the partials in block order into *out.
+__device__ __forceinline__ void grid_finish(double t, double *part, int *count, double *out) {
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = t;
+    __threadfence();
+    last = atomicAdd(count, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    double s = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) s += __builtin_nontemporal_load(&part[i]);
+    // ordered: per-thread strided partial sums, then block_sum in fixed order
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+      *out = s;
+      *count = 0;
+    }
+  }
+}
+
+struct AxpyD {
+  double *y;
+  const double *x;
+  double imm;
+  int slot;
+  double sign;
+};
+
+struct FusedArgs {
+  int64_t n;
+  AxpyD up[2];
+  int nups;
+  const double *u;
+  const double *v;
+  int out_slot;
+  double *copy_dst;
+  const double *copy_src;
+  double *scal;
+  double *part;
+  int *count;
+};
+
+__global__ __launch_bounds__(kBlock) void k_fused_axpy_dot(FusedArgs a) {
+  double c0 = 0.0, c1 = 0.0;
+  if (a.nups > 0) c0 = a.up[0].slot >= 0 ? a.up[0].sign * a.scal[a.up[0].slot] : a.up[0].imm;
+  if (a.nups > 1) c1 = a.up[1].slot >= 0 ? a.up[1].sign * a.scal[a.up[1].slot] : a.up[1].imm;
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < a.n; i += stride) {
+    if (a.nups > 0) a.up[0].y[i] += c0 * a.up[0].x[i];
+    if (a.nups > 1) a.up[1].y[i] += c1 * a.up[1].x[i];
+    if (a.copy_dst) a.copy_dst[i] = a.copy_src[i];
+    if (a.out_slot >= 0) acc += a.u[i] * a.v[i];
+  }
+  if (a.out_slot < 0) return;
+  const double t = block_sum(acc);
+  grid_finish(t, a.part, a.count, &a.scal[a.out_slot]);
+}
+
+__global__ void k_scale2(int64_t n, double a, const double *x, double *y, double *z) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    y[i] = a * x[i];
+    z[i] *= a;
+  }
+}
+
+__global__ void k_copy(int64_t n, const double *x, double *y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = x[i];
+}
+
+// cu[j] = Q[:, j]^T u, cv[j] = Q[:, j]^T v; one workgroup per column.
+__global__ __launch_bounds__(kBlock) void k_gemv_t(int64_t n, const double *Q, int64_t ldq, const double *u,
+                                                  const double *v, double *cu, double *cv) {
+  const double *q = Q + (int64_t)blockIdx.x * ldq;
+  double su = 0.0, sv = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) {
+    const double qi = q[i];
+    su += qi * u[i];
+    sv += qi * v[i];
+  }
+  su = block_sum(su);
+  sv = block_sum(sv);
+  if (threadIdx.x == 0) {
+    cu[blockIdx.x] = su;
+    cv[blockIdx.x] = sv;
+  }
+}
+
+// y1 -= Q c1, y2 -= Q c2 for an n x k column-major basis.
+__global__ __launch_bounds__(kBlock) void k_gemv_n(int64_t n, const double *Q, int64_t ldq, int64_t k,
+                                                  const double *c1, double *y1, const double *c2, double *y2) {
+  __shared__ double s1[1024], s2[1024];
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  double a1 = 0.0, a2 = 0.0;
+  for (int64_t j0 = 0; j0 < k; j0 += 1024) {
+    const int64_t kk = min<int64_t>(1024, k - j0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kk; t += kBlock) {
+      s1[t] = c1[j0 + t];
+      s2[t] = c2[j0 + t];
+    }
+    __syncthreads();
+    if (i < n) {
+      const double *q = Q + j0 * ldq + i;
+      for (int64_t j = 0; j < kk; ++j) {
+        const double qv = q[j * ldq];
+        a1 += qv * s1[j];
+        a2 += qv * s2[j];
+      }
+    }
+  }
+  if (i < n) {
+    y1[i] -= a1;
+    y2[i] -= a2;
+  }
+}
+
+// C (n x d row-major) = Q (n x k col-major) W (k x d row-major); 16 outputs per thread.
+__global__ __launch_bounds__(kBlock) void k_combine(int64_t n, const double *Q, int64_t ldq, int64_t k,
+                                                   const double *W, int64_t d, int64_t c0, double *C, int64_t ldc) {
+  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t nd = min<int64_t>(16, d - c0);
+  double acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
+  for (int64_t j = 0; j < k; ++j) {
+    const double q = Q[j * ldq + i];
+    const double *w = W + j * d + c0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+      if (t < nd) acc[t] += q * w[t];
+  }
+  for (int t = 0; t < nd; ++t) C[i * ldc + c0 + t] = acc[t];
+}
+
+template <typename T> __device__ __forceinline__ double ldd(const T *p, int64_t i);
+template <> __device__ __forceinline__ double ldd<double>(const double *p, int64_t i) { return p[i]; }
+template <> __device__ __forceinline__ double ldd<float>(const float *p, int64_t i) { return (double)p[i]; }
+template <> __device__ __forceinline__ double ldd<bf16_t>(const bf16_t *p, int64_t i) {
+  return (double)__uint_as_float((uint32_t)p[i] << 16);
+}
+
+template <typename T> __device__ __forceinline__ void std_(T *p, int64_t i, double v);
+template <> __device__ __forceinline__ void std_<double>(double *p, int64_t i, double v) { p[i] = v; }
+template <> __device__ __forceinline__ void std_<float>(float *p, int64_t i, double v) { p[i] = (float)v; }
+template <> __device__ __forceinline__ void std_<bf16_t>(bf16_t *p, int64_t i, double v) {
+  float f = (float)v;
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  p[i] = (bf16_t)(u >> 16);
+}
+
+// Gram partials: each workgroup accumulates Y[rows]^T Y[rows] over a strided
+// set of 64-row slabs into part[blockIdx.x * L * L] (FP64). Threads own 4x4
+// tiles of the (padded) L x L result.
+constexpr int kGramSlab = 64;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gram_part(int64_t m, int l, const T *Y, int64_t ldy, double *part) {
+  extern __shared__ double slab[];  // kGramSlab x lp
+  const int lp = (l + 3) & ~3;
+  const int nt = lp / 4;  // tiles per dimension
+  const int ntiles = nt * nt;
+  double acc[4][16];
+  int tiles[4];
+  int ntl = 0;
+  for (int t = threadIdx.x; t < ntiles && ntl < 4; t += kBlock) tiles[ntl++] = t;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 16; ++b) acc[a][b] = 0.0;
+  for (int64_t r0 = (int64_t)blockIdx.x * kGramSlab; r0 < m; r0 += (int64_t)gridDim.x * kGramSlab) {
+    const int rows = (int)min<int64_t>(kGramSlab, m - r0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < kGramSlab * lp; e += kBlock) {
+      const int rr = e / lp, cc = e % lp;
+      slab[e] = (rr < rows && cc < l) ? ldd<T>(Y, (r0 + rr) * ldy + cc) : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      if (a >= ntl) continue;
+      const int ti = tiles[a] / nt, tj = tiles[a] % nt;
+      for (int rr = 0; rr < rows; ++rr) {
+        const double *row = slab + rr * lp;
+        double xi[4], xj[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          xi[q] = row[ti * 4 + q];
+          xj[q] = row[tj * 4 + q];
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[a][p * 4 + q] += xi[p] * xj[q];
+      }
+    }
+  }
+  double *out = part + (int64_t)blockIdx.x * l * l;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (a < ntl) {
+      const int ti = tiles[a] / nt, tj = tiles[a] % nt;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = ti * 4 + p, j = tj * 4 + q;
+          if (i < l && j < l) out[i * l + j] = acc[a][p * 4 + q];
+        }
+    }
+  }
+}
+
+__global__ void k_gram_reduce(int nb, int64_t ll, const double *part, double *G) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ll) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[(int64_t)b * ll + e];
+  G[e] = s;
+}
+
+// Z[i, 0:lo] = Y[i, 0:li] M (li x lo row-major FP64); one wavefront per row,
+// row held in registers so Z may alias Y.
+template <typename TY, typename TZ>
+__global__ __launch_bounds__(kBlock) void k_apply(int64_t m, int li, int lo, const TY *Y, int64_t ldy, const double *M,
+                                                 TZ *Z, int64_t ldz) {
+  const int64_t row = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= m) return;
+  double y0 = lane < li ? ldd<TY>(Y, row * ldy + lane) : 0.0;
+  double y1 = lane + 64 < li ? ldd<TY>(Y, row * ldy + lane + 64) : 0.0;
+  double y2 = lane + 128 < li ? ldd<TY>(Y, row * ldy + lane + 128) : 0.0;
+  double y3 = lane + 192 < li ? ldd<TY>(Y, row * ldy + lane + 192) : 0.0;
+  for (int c0 = 0; c0 < lo; c0 += 64) {
+    const int c = c0 + lane;
+    double acc = 0.0;
+    for (int j = 0; j < li; ++j) {
+      const double src = j < 64 ? y0 : j < 128 ? y1 : j < 192 ? y2 : y3;
+      const double yj = __shfl(src, j & 63, 64);
+      if (c < lo) acc += yj * M[(int64_t)j * lo + c];
+    }
+    if (c < lo) std_<TZ>(Z, row * ldz + c, acc);
+  }
+}
+
+// Residual column norms: one wavefront per row (rows strided over the grid),
+// lanes over output columns; per-workgroup column partials in FP64.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_resid(int64_t m, int w, int k, const T *Y1, const T *Y2, int64_t ld,
+                                                 const double *M1, const double *M2, double *part) {
+  __shared__ double red[kBlock / 64][128];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  double sq0 = 0.0, sq1 = 0.0;
+  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t row = blockIdx.x * (int64_t)(kBlock / 64) + wv; row < m; row += nw) {
+    const double a0 = lane < w ? ldd<T>(Y1, row * ld + lane) : 0.0;
+    const double a1 = lane + 64 < w ? ldd<T>(Y1, row * ld + lane + 64) : 0.0;
+    const double b0 = lane < w ? ldd<T>(Y2, row * ld + lane) : 0.0;
+    const double b1 = lane + 64 < w ? ldd<T>(Y2, row * ld + lane + 64) : 0.0;
+    double r0 = 0.0, r1 = 0.0;
+    for (int j = 0; j < w; ++j) {
+      const double y1 = __shfl(j < 64 ? a0 : a1, j & 63, 64);
+      const double y2 = __shfl(j < 64 ? b0 : b1, j & 63, 64);
+      if (lane < k) r0 += y1 * M1[(int64_t)j * k + lane] + y2 * M2[(int64_t)j * k + lane];
+      if (lane + 64 < k) r1 += y1 * M1[(int64_t)j * k + lane + 64] + y2 * M2[(int64_t)j * k + lane + 64];
+    }
+    sq0 += r0 * r0;
+    sq1 += r1 * r1;
+  }
+  red[wv][lane] = sq0;
+  red[wv][lane + 64] = sq1;
+  __syncthreads();
+  if (threadIdx.x < 128 && threadIdx.x < k) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; ++i) t += red[i][threadIdx.x];
+    part[(int64_t)blockIdx.x * k + threadIdx.x] = t;
+  }
+}
+
+// Column sums of squares: one workgroup per 64-row slab group, FP64.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_colsq_part(int64_t m, int l, const T *Y, int64_t ldy, double *part) {
+  double acc = 0.0;
+  const int c = threadIdx.x;
+  if (c < l)
+    for (int64_t r = blockIdx.x; r < m; r += gridDim.x) {
+      const double v = ldd<T>(Y, r * ldy + c);
+      acc += v * v;
+    }
+  if (c < l) part[(int64_t)blockIdx.x * l + c] = acc;
+}
+
+template <typename T>
+__global__ void k_colscale(int64_t m, int l, T *Y, int64_t ldy, const double *scale) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * l; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / l;
+    const int c = (int)(e % l);
+    std_<T>(Y, r * ldy + c, ldd<T>(Y, r * ldy + c) * scale[c]);
+  }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// Standard normals by Box-Muller on two counter-based uniforms; element (i, j)
+// of an m x l panel uses counter j*m + i (column-major, the fill order of
+// DMatrix::from_fn at src/randomized/mod.rs:310-312).
+template <typename T>
+__global__ void k_gauss(int64_t m, int l, uint64_t seed, T *Y, int64_t ldy) {
+  const uint64_t phi = 0x9E3779B97F4A7C15ULL;
+  const uint64_t key = (seed * phi) ^ 0x4F4D454741303031ULL;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * l; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % m;
+    const int64_t j = e / m;
+    const uint64_t cnt = (uint64_t)e;
+    const uint64_t a = mix64(key + (2 * cnt + 1) * phi);
+    const uint64_t b = mix64(key + (2 * cnt + 2) * phi);
+    const double u1 = ((double)(a >> 11) + 0.5) * 0x1.0p-53;
+    const double u2 = (double)(b >> 11) * 0x1.0p-53;
+    const double z = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+    std_<T>(Y, i * ldy + j, z);
+  }
+}
+
+template <typename S, typename D>
+__global__ void k_pconvert(int64_t m, int l, const S *X, int64_t ldx, D *Y, int64_t ldy) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * l; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / l;
+    const int c = (int)(e % l);
+    std_<D>(Y, r * ldy + c, ldd<S>(X, r * ldx + c));
+  }
+}
+
+template <typename T>
+__global__ void k_pfill(int64_t m, int l, double v, T *Y, int64_t ldy) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * l; e += (int64_t)gridDim.x * blockDim.x)
+    std_<T>(Y, (e / l) * ldy + (e % l), v);
+}
+
+int grid_of(int64_t n, int per = kBlock, int cap = 2048) {
+  int64_t g = (n + per - 1) / per;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+int fused_grid(int64_t n) { return grid_of(n, kBlock * 8, 512); }
+
+}  // namespace
+
+void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *u, const double *v,
+                    int out_slot, double *copy_dst, const double *copy_src) {
+  FusedArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.n = n;
+  a.nups = nups;
+  for (int i = 0; i < nups; ++i) {
+    a.up[i].y = ups[i].y;
+    a.up[i].x = ups[i].x;
+    a.up[i].imm = ups[i].imm;
+    a.up[i].slot = ups[i].slot;
+    a.up[i].sign = ups[i].sign;
+  }
+  a.u = u;
+  a.v = v;
+  a.out_slot = out_slot;
+  a.copy_dst = copy_dst;
+  a.copy_src = copy_src;
+  a.scal = c->scal.as<double>();
+  a.part = c->red_part.as<double>();
+  a.count = c->red_count.as<int>();
+  const int g = fused_grid(n);
+  KTimer t(c, KC_BLAS1, (double)n * 8.0 * (3 * nups + 2 + (copy_dst ? 2 : 0)));
+  hipLaunchKernelGGL(k_fused_axpy_dot, g, kBlock, 0, c->stream, a);
+  HIP_CHECK(hipGetLastError());
+}
+
+void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double *z) {
+  KTimer t(c, KC_BLAS1, (double)n * 32.0);
+  hipLaunchKernelGGL(k_scale2, grid_of(n), kBlock, 0, c->stream, n, a, x, y, z);
+  HIP_CHECK(hipGetLastError());
+}
+
+void vec_copy(Ctx *c, int64_t n, const double *x, double *y) {
+  HIP_CHECK(hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+}
+
+void basis_gemv_t(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *u, const double *v,
+                  double *cu, double *cv) {
+  if (k <= 0) return;
+  KTimer t(c, KC_GEMV, (double)n * k * 8.0 + 16.0 * n);
+  hipLaunchKernelGGL(k_gemv_t, (int)k, kBlock, 0, c->stream, n, Q, ldq, u, v, cu, cv);
+  HIP_CHECK(hipGetLastError());
+}
+
+void basis_gemv_n(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *c1, double *y1,
+                  const double *c2, double *y2) {
+  if (k <= 0) return;
+  KTimer t(c, KC_GEMV, (double)n * k * 8.0 + 32.0 * n);
+  hipLaunchKernelGGL(k_gemv_n, (int)((n + kBlock - 1) / kBlock), kBlock, 0, c->stream, n, Q, ldq, k, c1, y1, c2, y2);
+  HIP_CHECK(hipGetLastError());
+}
+
+void basis_combine(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *W, int64_t d,
+                   double *C, int64_t ldc) {
+  KTimer t(c, KC_DENSE, (double)n * k * 8.0 * ((d + 15) / 16) + (double)n * d * 8.0);
+  for (int64_t c0 = 0; c0 < d; c0 += 16)
+    hipLaunchKernelGGL(k_combine, (int)((n + kBlock - 1) / kBlock), kBlock, 0, c->stream, n, Q, ldq, k, W, d, c0, C,
+                       ldc);
+  HIP_CHECK(hipGetLastError());
+}
+
+void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, double *G_host) {
+  if (l > 128) fail(SVDH_ERR_ARG, "panel width %lld exceeds the Gram kernel limit 128", (long long)l);
+  const int nb = grid_of(m, kGramSlab, 256);
+  const int64_t ll = l * l;
+  const size_t need = (size_t)nb * ll * sizeof(double) + ll * sizeof(double);
+  c->red_part.ensure(std::max<size_t>(need, c->red_part.bytes));
+  double *part = c->red_part.as<double>();
+  double *G = part + (int64_t)nb * ll;
+  const int lp = (int)((l + 3) & ~3);
+  const size_t shm = (size_t)kGramSlab * lp * sizeof(double);
+  {
+    KTimer t(c, KC_DENSE, (double)m * l * dtype_size(ydt));
+    if (ydt == SVDH_F64)
+      hipLaunchKernelGGL(k_gram_part<double>, nb, kBlock, shm, c->stream, m, (int)l, (const double *)Y, ldy, part);
+    else if (ydt == SVDH_F32)
+      hipLaunchKernelGGL(k_gram_part<float>, nb, kBlock, shm, c->stream, m, (int)l, (const float *)Y, ldy, part);
+    else
+      hipLaunchKernelGGL(k_gram_part<bf16_t>, nb, kBlock, shm, c->stream, m, (int)l, (const bf16_t *)Y, ldy, part);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_gram_reduce, grid_of(ll), kBlock, 0, c->stream, nb, ll, (const double *)part, G);
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipMemcpyAsync(G_host, G, ll * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+template <typename TY, typename TZ>
+static void apply_t(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, const double *M, void *Z,
+                    int64_t ldz) {
+  const int64_t blocks = (m + (kBlock / 64) - 1) / (kBlock / 64);
+  hipLaunchKernelGGL((k_apply<TY, TZ>), (int)blocks, kBlock, 0, c->stream, m, (int)li, (int)lo, (const TY *)Y, ldy, M,
+                     (TZ *)Z, ldz);
+  HIP_CHECK(hipGetLastError());
+}
+
+void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, int ydt,
+                 const double *M_host, void *Z, int64_t ldz, int zdt) {
+  if (li > 256) fail(SVDH_ERR_ARG, "panel width %lld exceeds the apply kernel limit 256", (long long)li);
+  const size_t mb = (size_t)li * lo * sizeof(double);
+  c->stage.ensure(mb);
+  double *Md = c->stage.as<double>();
+  HIP_CHECK(hipMemcpyAsync(Md, M_host, mb, hipMemcpyHostToDevice, c->stream));
+  KTimer t(c, KC_DENSE, (double)m * (li * dtype_size(ydt) + lo * dtype_size(zdt)));
+  if (ydt == SVDH_F64 && zdt == SVDH_F64)
+    apply_t<double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+  else if (ydt == SVDH_F32 && zdt == SVDH_F32)
+    apply_t<float, float>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+  else if (ydt == SVDH_F32 && zdt == SVDH_F64)
+    apply_t<float, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+  else if (ydt == SVDH_F64 && zdt == SVDH_F32)
+    apply_t<double, float>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+  else if (ydt == SVDH_F32 && zdt == SVDH_BF16)
+    apply_t<float, bf16_t>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+  else
+    fail(SVDH_ERR_ARG, "unsupported apply dtypes %d -> %d", ydt, zdt);
+  // The staged matrix must stay valid until the kernel has read it.
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t k, const void *Y1, const void *Y2, int64_t ld, int dt,
+                    const double *M1, const double *M2, double *nr2_host) {
+  if (w > 128 || k > 128) fail(SVDH_ERR_ARG, "residual panel wider than 128");
+  const int nb = grid_of(m, kBlock / 64 * 8, 1024);
+  const size_t mb = (size_t)w * k * sizeof(double);
+  c->stage.ensure(2 * mb);
+  double *M1d = c->stage.as<double>();
+  double *M2d = M1d + w * k;
+  HIP_CHECK(hipMemcpyAsync(M1d, M1, mb, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(M2d, M2, mb, hipMemcpyHostToDevice, c->stream));
+  c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * k * sizeof(double)));
+  double *part = c->red_part.as<double>();
+  {
+    KTimer t(c, KC_DENSE, (double)m * 2 * w * dtype_size(dt));
+    if (dt == SVDH_F64)
+      hipLaunchKernelGGL(k_resid<double>, nb, kBlock, 0, c->stream, m, (int)w, (int)k, (const double *)Y1,
+                         (const double *)Y2, ld, M1d, M2d, part);
+    else
+      hipLaunchKernelGGL(k_resid<float>, nb, kBlock, 0, c->stream, m, (int)w, (int)k, (const float *)Y1,
+                         (const float *)Y2, ld, M1d, M2d, part);
+    HIP_CHECK(hipGetLastError());
+  }
+  std::vector<double> h((size_t)nb * k);
+  HIP_CHECK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (int64_t r = 0; r < k; ++r) {
+    double t = 0.0;
+    for (int b = 0; b < nb; ++b) t += h[(size_t)b * k + r];
+    nr2_host[r] = t;
+  }
+}
+
+void panel_normalize_columns(Ctx *c, int64_t m, int64_t l, void *Y, int64_t ldy, int ydt) {
+  if (l > kBlock) fail(SVDH_ERR_ARG, "panel width too large for column normalisation");
+  const int nb = grid_of(m, 64, 1024);
+  c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * l * sizeof(double)));
+  double *part = c->red_part.as<double>();
+  if (ydt == SVDH_F64)
+    hipLaunchKernelGGL(k_colsq_part<double>, nb, kBlock, 0, c->stream, m, (int)l, (const double *)Y, ldy, part);
+  else
+    hipLaunchKernelGGL(k_colsq_part<float>, nb, kBlock, 0, c->stream, m, (int)l, (const float *)Y, ldy, part);
+  HIP_CHECK(hipGetLastError());
+  std::vector<double> h((size_t)nb * l);
+  HIP_CHECK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  std::vector<double> scale(l, 1.0);
+  for (int64_t j = 0; j < l; ++j) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += h[(size_t)b * l + j];
+    const double nrm = std::sqrt(s);
+    if (nrm > 1e-10) scale[j] = 1.0 / nrm;  // src/randomized/mod.rs:279-283
+  }
+  c->stage.ensure(l * sizeof(double));
+  double *sd = c->stage.as<double>();
+  HIP_CHECK(hipMemcpyAsync(sd, scale.data(), l * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (ydt == SVDH_F64)
+    hipLaunchKernelGGL(k_colscale<double>, grid_of(m * l), kBlock, 0, c->stream, m, (int)l, (double *)Y, ldy, sd);
+  else
+    hipLaunchKernelGGL(k_colscale<float>, grid_of(m * l), kBlock, 0, c->stream, m, (int)l, (float *)Y, ldy, sd);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void panel_gaussian(Ctx *c, int64_t m, int64_t l, uint64_t seed, void *Y, int64_t ldy, int ydt) {
+  if (ydt == SVDH_F64)
+    hipLaunchKernelGGL(k_gauss<double>, grid_of(m * l), kBlock, 0, c->stream, m, (int)l, seed, (double *)Y, ldy);
+  else if (ydt == SVDH_F32)
+    hipLaunchKernelGGL(k_gauss<float>, grid_of(m * l), kBlock, 0, c->stream, m, (int)l, seed, (float *)Y, ldy);
+  else
+    hipLaunchKernelGGL(k_gauss<bf16_t>, grid_of(m * l), kBlock, 0, c->stream, m, (int)l, seed, (bf16_t *)Y, ldy);
+  HIP_CHECK(hipGetLastError());
+}
+
+void panel_convert(Ctx *c, int64_t m, int64_t l, const void *X, int64_t ldx, int xdt, void *Y, int64_t ldy,
+                   int ydt) {
+  const int g = grid_of(m * l);
+  if (xdt == SVDH_F64 && ydt == SVDH_F32)
+    hipLaunchKernelGGL((k_pconvert<double, float>), g, kBlock, 0, c->stream, m, (int)l, (const double *)X, ldx, (float *)Y, ldy);
+  else if (xdt == SVDH_F32 && ydt == SVDH_F64)
+    hipLaunchKernelGGL((k_pconvert<float, double>), g, kBlock, 0, c->stream, m, (int)l, (const float *)X, ldx, (double *)Y, ldy);
+  else if (xdt == SVDH_F32 && ydt == SVDH_BF16)
+    hipLaunchKernelGGL((k_pconvert<float, bf16_t>), g, kBlock, 0, c->stream, m, (int)l, (const float *)X, ldx, (bf16_t *)Y, ldy);
+  else if (xdt == SVDH_F64 && ydt == SVDH_BF16)
+    hipLaunchKernelGGL((k_pconvert<double, bf16_t>), g, kBlock, 0, c->stream, m, (int)l, (const double *)X, ldx, (bf16_t *)Y, ldy);
+  else if (xdt == SVDH_F64 && ydt == SVDH_F64)
+    hipLaunchKernelGGL((k_pconvert<double, double>), g, kBlock, 0, c->stream, m, (int)l, (const double *)X, ldx, (double *)Y, ldy);
+  else if (xdt == SVDH_F32 && ydt == SVDH_F32)
+    hipLaunchKernelGGL((k_pconvert<float, float>), g, kBlock, 0, c->stream, m, (int)l, (const float *)X, ldx, (float *)Y, ldy);
+  else
+    fail(SVDH_ERR_ARG, "unsupported panel conversion %d -> %d", xdt, ydt);
+  HIP_CHECK(hipGetLastError());
+}
+
+void panel_fill(Ctx *c, int64_t m, int64_t l, double v, void *Y, int64_t ldy, int ydt) {
+  const int g = grid_of(m * l);
+  if (ydt == SVDH_F64)
+    hipLaunchKernelGGL(k_pfill<double>, g, kBlock, 0, c->stream, m, (int)l, v, (double *)Y, ldy);
+  else if (ydt == SVDH_F32)
+    hipLaunchKernelGGL(k_pfill<float>, g, kBlock, 0, c->stream, m, (int)l, v, (float *)Y, ldy);
+  else
+    hipLaunchKernelGGL(k_pfill<bf16_t>, g, kBlock, 0, c->stream, m, (int)l, v, (bf16_t *)Y, ldy);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace svdh

@@ -1,0 +1,302 @@
+// Host FP64 dense algebra of the svdh library: the tridiagonal implicit-QL
+// solvers of LAS2 and the small symmetric eigen / Cholesky routines of the
+// randomized solver. These run on l x l or js x js matrices (l <= 256,
+// js = Lanczos steps + 1), next to the device work.
+//
+// imtqlb / imtql2 take the steps of src/lanczos/mod.rs:428-584 and 901-1028
+// (the reference's sparsity-dependent iteration caps, relaxed deflation and
+// forced-convergence behaviour), with src/legacy.rs's corrections:
+//   - pythag updates r *= (s/u)^2 (legacy.rs:882; mod.rs:373 drops the r);
+//   - imtqlb takes p = d[l] and f = bnd[l] afresh on every QL iteration
+//     (legacy.rs:990-991; mod.rs:455-457 reads them once per l).
+// Their control flow comes from SVDLIBC (Copyright (c) 2002, University of
+// Tennessee Research Foundation, BSD licence).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "common.h"
+
+namespace svdh {
+
+static inline bool near(double a, double b, double eps) { return std::fabs(b - a) < eps; }
+
+static inline double sign_of(double a, double b) {
+  // |a| with the sign convention of svd_fsign (mod.rs:353-358)
+  const bool same = (a >= 0.0 && b >= 0.0) || (a < 0.0 && b < 0.0);
+  return same ? a : -a;
+}
+
+double pythag(double a, double b, double eps) {
+  const double big = std::max(std::fabs(a), std::fabs(b));
+  if (!(big > 0.0)) return 0.0;
+  double p = big;
+  const double q = std::min(std::fabs(a), std::fabs(b)) / big;
+  double r = q * q;
+  for (double t = 4.0 + r; !near(t, 4.0, eps); t = 4.0 + r) {
+    const double s = r / t;
+    const double u = 1.0 + 2.0 * s;
+    p *= u;
+    const double w = s / u;
+    r *= w * w;
+  }
+  return p;
+}
+
+int imtqlb(int64_t n, double *d, double *e, double *bnd, int64_t max_iter, double eps) {
+  if (n == 1) return SVDH_OK;
+  const double size_factor = std::sqrt(static_cast<double>(n));
+  const int64_t last = n - 1;
+  bnd[0] = 1.0;
+  for (int64_t i = 1; i <= last; ++i) {
+    bnd[i] = 0.0;
+    e[i - 1] = e[i];
+  }
+  e[last] = 0.0;
+  bool warned = false;
+  for (int64_t l = 0; l <= last; ++l) {
+    for (int64_t it = 0; it <= max_iter;) {
+      // Find the first negligible sub-diagonal at or after l.
+      int64_t m = l;
+      for (; m < last; ++m) {
+        const double test = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        const double tol = eps * 100.0 * std::max(test, 1.0) * size_factor;
+        if (std::fabs(e[m]) <= tol) break;
+      }
+      double p = d[l];
+      double f = bnd[l];
+      if (m == l) {
+        // Insert the converged value into the ascending prefix d[0..l].
+        int64_t i = l;
+        while (i >= 1 && p < d[i - 1]) {
+          d[i] = d[i - 1];
+          bnd[i] = bnd[i - 1];
+          --i;
+        }
+        d[i] = p;
+        bnd[i] = f;
+        break;
+      }
+      if (it == max_iter) {
+        warned = true;
+        for (int64_t k = l; k <= m; ++k) bnd[k] = std::max(bnd[k], 0.1);
+        e[l] = 0.0;
+        break;
+      }
+      ++it;
+      // Implicit shift and the QL sweep from m-1 down to l.
+      double g = (d[l + 1] - p) / (2.0 * e[l]);
+      double r = pythag(g, 1.0, eps);
+      g = d[m] - p + e[l] / (g + sign_of(r, g));
+      double s = 1.0, c = 1.0;
+      p = 0.0;
+      int64_t i = m - 1;
+      bool underflow = false;
+      for (;;) {
+        f = s * e[i];
+        const double b = c * e[i];
+        r = pythag(f, g, eps);
+        e[i + 1] = r;
+        if (r < eps * 1000.0 * (std::fabs(f) + std::fabs(g))) {
+          underflow = true;
+          break;
+        }
+        if (std::fabs(r) < eps * 100.0) r = eps * 100.0 * sign_of(1.0, r);
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        f = bnd[i + 1];
+        bnd[i + 1] = s * bnd[i] + c * f;
+        bnd[i] = c * bnd[i] - s * f;
+        if (i == l) break;
+        --i;
+      }
+      if (underflow) {
+        d[i + 1] -= p;
+      } else {
+        d[l] -= p;
+        e[l] = g;
+      }
+      e[m] = 0.0;
+    }
+  }
+  if (warned)
+    std::fprintf(stderr,
+                 "Warning: imtqlb had some convergence issues but continued with best estimates. "
+                 "Results may have reduced accuracy.\n");
+  return SVDH_OK;
+}
+
+int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_iter, double eps) {
+  if (n == 1) return SVDH_OK;
+  const int64_t last = n - 1;
+  for (int64_t i = 1; i < n; ++i) e[i - 1] = e[i];
+  e[last] = 0.0;
+  for (int64_t l = 0; l < n; ++l) {
+    for (int64_t it = 0;; ++it) {
+      int64_t m = l;
+      for (; m < last; ++m) {
+        const double test = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (near(test, test + std::fabs(e[m]), eps)) break;
+      }
+      if (m == l) break;
+      if (it == max_iter)
+        fail(SVDH_ERR_IMTQL2, "imtql2 no convergence to an eigenvalue after %lld iterations",
+             (long long)max_iter);
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = pythag(g, 1.0, eps);
+      g = d[m] - d[l] + e[l] / (g + sign_of(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int64_t i = m - 1;
+      bool underflow = false;
+      for (;;) {
+        double f = s * e[i];
+        const double b = c * e[i];
+        r = pythag(f, g, eps);
+        e[i + 1] = r;
+        if (near(r, 0.0, eps)) {
+          underflow = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        // Rotate columns i and i+1 of every row of z (row-major nm x n).
+        for (int64_t row = 0; row < nm; ++row) {
+          double *zr = z + row * n;
+          const double zi1 = zr[i + 1];
+          zr[i + 1] = s * zr[i] + c * zi1;
+          zr[i] = c * zr[i] - s * zi1;
+        }
+        if (i == l) break;
+        --i;
+      }
+      if (underflow) {
+        d[i + 1] -= p;
+      } else {
+        d[l] -= p;
+        e[l] = g;
+      }
+      e[m] = 0.0;
+    }
+  }
+  // Selection sort into ascending order, swapping eigenvector columns.
+  for (int64_t i = 0; i + 1 < n; ++i) {
+    int64_t k = i;
+    double p = d[i];
+    for (int64_t j = i + 1; j < n; ++j)
+      if (d[j] < p) {
+        k = j;
+        p = d[j];
+      }
+    if (k != i) {
+      d[k] = d[i];
+      d[i] = p;
+      for (int64_t row = 0; row < nm; ++row) std::swap(z[row * n + i], z[row * n + k]);
+    }
+  }
+  return SVDH_OK;
+}
+
+// Symmetric eigen-decomposition of a small dense matrix: Householder
+// reduction to tridiagonal form with the reflections accumulated in V, then
+// implicit QL (imtql2 with f64 epsilon) applied to V's columns. The matrix
+// is scaled by its largest entry first so the absolute-epsilon deflation test
+// of imtql2 behaves the same for every scale.
+void sym_eig(int64_t l, const double *A, double *w, double *V) {
+  if (l == 0) return;
+  double scale = 0.0;
+  for (int64_t i = 0; i < l * l; ++i) scale = std::max(scale, std::fabs(A[i]));
+  if (!(scale > 0.0)) {
+    for (int64_t i = 0; i < l; ++i) w[i] = 0.0;
+    for (int64_t i = 0; i < l * l; ++i) V[i] = 0.0;
+    for (int64_t i = 0; i < l; ++i) V[i * l + i] = 1.0;
+    return;
+  }
+  std::vector<double> a(l * l);
+  for (int64_t i = 0; i < l * l; ++i) a[i] = A[i] / scale;
+  // V accumulates Q with A = Q T Q^T.
+  std::vector<double> q(l * l, 0.0);
+  for (int64_t i = 0; i < l; ++i) q[i * l + i] = 1.0;
+  std::vector<double> diag(l), off(l, 0.0), v(l), p(l), kv(l);
+  for (int64_t k = 0; k + 2 < l; ++k) {
+    // Householder vector for column k below the sub-diagonal.
+    double alpha = 0.0;
+    for (int64_t i = k + 1; i < l; ++i) alpha += a[i * l + k] * a[i * l + k];
+    alpha = std::sqrt(alpha);
+    if (alpha == 0.0) continue;
+    const double x0 = a[(k + 1) * l + k];
+    if (x0 > 0) alpha = -alpha;
+    for (int64_t i = 0; i < l; ++i) v[i] = 0.0;
+    v[k + 1] = x0 - alpha;
+    for (int64_t i = k + 2; i < l; ++i) v[i] = a[i * l + k];
+    double vn = 0.0;
+    for (int64_t i = k + 1; i < l; ++i) vn += v[i] * v[i];
+    if (vn == 0.0) continue;
+    const double beta = 2.0 / vn;
+    // A <- H A H with H = I - beta v v^T: p = beta A v, K = p - (beta/2)(v^T p) v,
+    // A <- A - v K^T - K v^T.
+    for (int64_t i = 0; i < l; ++i) {
+      double s = 0.0;
+      for (int64_t j = k + 1; j < l; ++j) s += a[i * l + j] * v[j];
+      p[i] = beta * s;
+    }
+    double vp = 0.0;
+    for (int64_t i = k + 1; i < l; ++i) vp += v[i] * p[i];
+    for (int64_t i = 0; i < l; ++i) kv[i] = p[i] - 0.5 * beta * vp * v[i];
+    for (int64_t i = 0; i < l; ++i)
+      for (int64_t j = 0; j < l; ++j) a[i * l + j] -= v[i] * kv[j] + kv[i] * v[j];
+    // Q <- Q H
+    for (int64_t i = 0; i < l; ++i) {
+      double s = 0.0;
+      for (int64_t j = k + 1; j < l; ++j) s += q[i * l + j] * v[j];
+      s *= beta;
+      for (int64_t j = k + 1; j < l; ++j) q[i * l + j] -= s * v[j];
+    }
+  }
+  for (int64_t i = 0; i < l; ++i) diag[i] = a[i * l + i];
+  for (int64_t i = 1; i < l; ++i) off[i] = a[i * l + i - 1];
+  imtql2(l, l, diag.data(), off.data(), q.data(), 100, 2.220446049250313e-16);
+  for (int64_t i = 0; i < l; ++i) w[i] = diag[i] * scale;
+  for (int64_t i = 0; i < l * l; ++i) V[i] = q[i];
+}
+
+bool cholesky_upper(int64_t l, const double *A, double *R) {
+  for (int64_t i = 0; i < l * l; ++i) R[i] = 0.0;
+  for (int64_t j = 0; j < l; ++j) {
+    double s = A[j * l + j];
+    for (int64_t k = 0; k < j; ++k) s -= R[k * l + j] * R[k * l + j];
+    if (!(s > 0.0)) return false;
+    const double rjj = std::sqrt(s);
+    R[j * l + j] = rjj;
+    for (int64_t i = j + 1; i < l; ++i) {
+      double t = A[j * l + i];
+      for (int64_t k = 0; k < j; ++k) t -= R[k * l + j] * R[k * l + i];
+      R[j * l + i] = t / rjj;
+    }
+  }
+  return true;
+}
+
+void upper_inverse(int64_t l, const double *R, double *Ri) {
+  for (int64_t i = 0; i < l * l; ++i) Ri[i] = 0.0;
+  for (int64_t j = l - 1; j >= 0; --j) {
+    Ri[j * l + j] = 1.0 / R[j * l + j];
+    for (int64_t i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int64_t k = i + 1; k <= j; ++k) s += R[i * l + k] * Ri[k * l + j];
+      Ri[i * l + j] = -s / R[i * l + i];
+    }
+  }
+}
+
+}  // namespace svdh

@@ -1,0 +1,521 @@
+// LAS2 (Lanczos with selective reorthogonalisation) on one MI355X.
+//
+// Control flow follows src/lanczos/mod.rs:104-1406 step for step (the
+// reference's sparsity-adaptive tolerances and iteration caps included);
+// numerical defects are corrected as src/legacy.rs does (pythag,
+// imtqlb's p/f refresh, Ritz-vector indexing s[(js-1-i)*js + k], largest-d
+// descending selection).
+//
+// Data placement: the working vectors w0..w4, the operator temp vector and
+// the Lanczos basis (Store, mod.rs:193-225: column i+2 holds q_i, columns 0
+// and 1 hold p_0, p_1) live in HBM as FP64. Every dot product of a step is a
+// fused "apply the pending axpy, then reduce" kernel whose result stays in a
+// device scalar slot, so the step's scalars travel to the host once per step
+// (one 32-byte read) for the host-side recurrences (ortbnd, error bounds,
+// tridiagonal QL), which are O(j) / O(j^2) scalar work.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "common.h"
+
+namespace svdh {
+
+namespace {
+
+constexpr int64_t kMaxLL = 2;
+
+struct Las2 {
+  Ctx *c;
+  svdh_op *op;
+  int vdt;          // sparse value dtype
+  double eps;       // epsilon of the reference's T
+  bool transposed;
+  int fwd, bwd;     // operator directions for "A x" and "A^T y" in working terms
+  int64_t nrows, ncols;  // working shape
+  int64_t iterations;
+  // device vectors
+  DevBuf wbuf[5], tempb, cbuf;
+  double *w[5];
+  double *temp;
+  // basis
+  DevBuf basis;
+  int64_t ldq = 0, cap = 0;
+  // host recurrences
+  std::vector<double> alf, eta, oldeta, bet, ritz, bnd, w5;
+
+  double *col(int64_t slot) const { return basis.as<double>() + slot * ldq; }
+  double *q(int64_t i) const { return col(i + kMaxLL); }
+  double *p(int64_t i) const { return col(i); }
+
+  void ensure_slots(int64_t slots) {
+    if (slots <= cap) return;
+    int64_t ncap = std::max<int64_t>(slots, cap * 2);
+    ncap = std::min<int64_t>(std::max(ncap, slots), iterations + kMaxLL + 2);
+    DevBuf nb((size_t)ncap * ldq * sizeof(double));
+    HIP_CHECK(hipMemsetAsync(nb.p, 0, (size_t)ncap * ldq * sizeof(double), c->stream));
+    if (cap) HIP_CHECK(hipMemcpyAsync(nb.p, basis.p, (size_t)cap * ldq * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    basis = std::move(nb);
+    cap = ncap;
+  }
+  void storq(int64_t i, const double *v) {
+    ensure_slots(i + kMaxLL + 1);
+    vec_copy(c, ncols, v, q(i));
+  }
+  void storp(int64_t i, const double *v) {
+    ensure_slots(i + 1);
+    vec_copy(c, ncols, v, p(i));
+  }
+
+  bool zero(double x) const { return std::fabs(x) < eps; }  // compare(x, 0), utils.rs:110-112
+
+  void opb(const double *x, double *y) {
+    spmv(op, fwd, vdt, x, temp, SVDH_F64);
+    spmv(op, bwd, vdt, temp, y, SVDH_F64);
+  }
+
+  // Read n scalar slots (synchronises the stream).
+  void read_slots(int n) {
+    HIP_CHECK(hipMemcpyAsync(c->h_scal, c->scal.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  double dot_now(const double *u, const double *v) {
+    fused_axpy_dot(c, ncols, nullptr, 0, u, v, S_A, nullptr, nullptr);
+    read_slots(1);
+    return c->h_scal[S_A];
+  }
+
+  double start_vec(uint32_t seed) {
+    std::vector<double> h(ncols);
+    svdh_las2_start_vector(seed, ncols, h.data());
+    HIP_CHECK(hipMemcpyAsync(w[0], h.data(), ncols * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return 0.0;
+  }
+
+  // Classical Gram-Schmidt of y against basis columns [slot0, slot0+k):
+  // coefficients from u (u unchanged), y -= Q (Q^T u).  c1/c2 on the device.
+  void cgs(int64_t slot0, int64_t k, const double *u, double *y) {
+    if (k <= 0) return;
+    cbuf.ensure((size_t)2 * k * sizeof(double));
+    double *cu = cbuf.as<double>();
+    double *cv = cu + k;
+    basis_gemv_t(c, ncols, col(slot0), ldq, k, u, u, cu, cv);
+    basis_gemv_n(c, ncols, col(slot0), ldq, k, cu, y, cv, temp_scratch());
+  }
+  // basis_gemv_n always updates two outputs; the second goes to a scratch
+  // vector when only one is wanted.
+  DevBuf scratchb;
+  double *temp_scratch() {
+    scratchb.ensure(ncols * sizeof(double));
+    return scratchb.as<double>();
+  }
+
+  // mod.rs:587-640
+  int startv(int64_t step, uint32_t seed, double *rnm_out) {
+    double rnm2 = dot_now(w[0], w[0]);
+    for (int id = 0; id < 3; ++id) {
+      if (id > 0 || step > 0 || zero(rnm2)) start_vec(seed);
+      vec_copy(c, ncols, w[0], w[3]);
+      opb(w[3], w[0]);
+      vec_copy(c, ncols, w[0], w[3]);
+      rnm2 = dot_now(w[3], w[3]);
+      if (rnm2 > 0.0) break;
+    }
+    if (rnm2 <= 0.0) fail(SVDH_ERR_STARTV, "rnm2 <= 0.0, rnm2 = %g", rnm2);
+    if (step > 0) {
+      cgs(kMaxLL, step, w[3], w[0]);
+      const double t = dot_now(w[4], w[0]);
+      Axpy up{w[0], w[2], -t, -1, 1.0};
+      fused_axpy_dot(c, ncols, &up, 1, w[0], w[0], S_A, w[3], w[0]);
+      read_slots(1);
+      const double d = c->h_scal[S_A];
+      rnm2 = d <= eps * rnm2 ? 0.0 : d;
+    }
+    *rnm_out = std::sqrt(rnm2);
+    return SVDH_OK;
+  }
+
+  // mod.rs:643-670
+  void stpone(uint32_t seed, double *rnm, double *tol) {
+    double r;
+    startv(0, seed, &r);
+    if (zero(r)) fail(SVDH_ERR_STPONE, "rnm == 0.0");
+    vec_scale2(c, ncols, 1.0 / r, w[0], w[1], w[3]);
+    opb(w[3], w[0]);
+    fused_axpy_dot(c, ncols, nullptr, 0, w[0], w[3], S_A, nullptr, nullptr);
+    Axpy u1{w[0], w[1], 0.0, S_A, -1.0};
+    fused_axpy_dot(c, ncols, &u1, 1, w[0], w[3], S_B, nullptr, nullptr);
+    Axpy u2{w[0], w[1], 0.0, S_B, -1.0};
+    fused_axpy_dot(c, ncols, &u2, 1, w[0], w[0], S_C, w[4], w[0]);
+    read_slots(3);
+    alf[0] = c->h_scal[S_A] + c->h_scal[S_B];
+    r = std::sqrt(c->h_scal[S_C]);
+    *rnm = r;
+    *tol = std::sqrt(eps) * (r + std::fabs(alf[0]));
+  }
+
+  // mod.rs:815-838
+  void ortbnd(int64_t step, double rnm, double eps1) {
+    if (step < 1) return;
+    if (!zero(rnm) && step > 1) {
+      oldeta[0] = (bet[1] * eta[1] + (alf[0] - alf[step]) * eta[0] - bet[step] * oldeta[0]) / rnm + eps1;
+      for (int64_t i = 1; i + 2 <= step; ++i)
+        oldeta[i] = (bet[i + 1] * eta[i + 1] + (alf[i] - alf[step]) * eta[i] + bet[i] * eta[i - 1] -
+                     bet[step] * oldeta[i]) /
+                        rnm +
+                    eps1;
+    }
+    oldeta[step - 1] = eps1;
+    std::swap(oldeta, eta);
+    eta[step] = eps1;
+  }
+
+  // mod.rs:760-813: up to two classical Gram-Schmidt passes of q (w1) and
+  // r (w0) against q_ll .. q_{step-1}, done as two block GEMVs per pass.
+  void purge(int64_t ll, int64_t step, double *rnm, double tol) {
+    if (step < ll + 2) return;
+    const double reps = std::sqrt(eps);
+    const double eps1 = eps * std::sqrt((double)ncols);
+    int64_t k = 0;
+    for (int64_t i = 1; i < step - (ll + 1); ++i)
+      if (std::fabs(eta[i]) > std::fabs(eta[k])) k = i;
+    k += ll;
+    if (!(std::fabs(eta[k]) > reps)) return;
+    const double reps1 = eps1 / reps;
+    const int64_t cnt = step - ll;
+    std::vector<double> hc(2 * cnt);
+    bool flag = true;
+    for (int it = 0; it < 2 && flag; ++it) {
+      if (!(*rnm > tol)) continue;
+      cbuf.ensure((size_t)2 * cnt * sizeof(double));
+      double *cq = cbuf.as<double>();
+      double *cr = cq + cnt;
+      basis_gemv_t(c, ncols, q(ll), ldq, cnt, w[3], w[4], cq, cr);
+      basis_gemv_n(c, ncols, q(ll), ldq, cnt, cq, w[1], cr, w[0]);
+      fused_axpy_dot(c, ncols, nullptr, 0, w[0], w[1], S_A, w[3], w[1]);
+      Axpy up{w[0], w[1], 0.0, S_A, -1.0};
+      fused_axpy_dot(c, ncols, &up, 1, w[0], w[0], S_B, w[4], w[0]);
+      HIP_CHECK(hipMemcpyAsync(hc.data(), cq, 2 * cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      read_slots(2);
+      double tq = 0.0, tr = 0.0;
+      for (int64_t i = 0; i < cnt; ++i) {
+        tq += std::fabs(hc[i]);
+        tr += std::fabs(hc[cnt + i]);
+      }
+      tr += std::fabs(c->h_scal[S_A]);
+      *rnm = std::sqrt(c->h_scal[S_B]);
+      if (tq <= reps1 && tr <= *rnm * reps1) flag = false;
+    }
+    for (int64_t i = ll; i <= step; ++i) {
+      eta[i] = eps1;
+      oldeta[i] = eps1;
+    }
+  }
+
+  // mod.rs:674-758
+  int64_t lanczos_step(int64_t first, int64_t last, int64_t *ll, bool *enough, double *rnm, double *tol) {
+    const double eps1 = eps * std::sqrt((double)ncols);
+    int64_t j = first;
+    for (; j < last; ++j) {
+      std::swap(w[1], w[2]);
+      std::swap(w[3], w[4]);
+      storq(j - 1, w[2]);
+      if (j - 1 < kMaxLL) storp(j - 1, w[4]);
+      bet[j] = *rnm;
+      if (zero(*rnm)) {
+        startv(j, 0u, rnm);
+        if (zero(*rnm)) *enough = true;
+      }
+      if (*enough) {
+        std::swap(w[1], w[2]);
+        break;
+      }
+      vec_scale2(c, ncols, 1.0 / *rnm, w[0], w[1], w[3]);
+      opb(w[3], w[0]);
+      Axpy u0{w[0], w[2], -*rnm, -1, 1.0};
+      fused_axpy_dot(c, ncols, &u0, 1, w[0], w[3], S_A, nullptr, nullptr);
+      if (j <= kMaxLL) {
+        read_slots(1);
+        if (std::fabs(alf[j - 1]) > 4.0 * std::fabs(c->h_scal[S_A])) *ll = j;
+      }
+      // Pending update: w0 -= alf_j w1, applied by the next fused kernel.
+      Axpy pend{w[0], w[1], 0.0, S_A, -1.0};
+      const int64_t lim = std::min(j - 1, *ll);
+      for (int64_t i = 0; i < lim; ++i) {
+        const int slot = 8 + (int)i;
+        fused_axpy_dot(c, ncols, &pend, 1, p(i), w[0], slot, nullptr, nullptr);
+        pend = Axpy{w[0], q(i), 0.0, slot, -1.0};
+        eta[i] = eps1;
+        oldeta[i] = eps1;
+      }
+      fused_axpy_dot(c, ncols, &pend, 1, w[0], w[4], S_B, nullptr, nullptr);
+      Axpy u2{w[0], w[2], 0.0, S_B, -1.0};
+      fused_axpy_dot(c, ncols, &u2, 1, w[0], w[3], S_C, nullptr, nullptr);
+      Axpy u3{w[0], w[1], 0.0, S_C, -1.0};
+      fused_axpy_dot(c, ncols, &u3, 1, w[0], w[0], S_E, w[4], w[0]);
+      read_slots(S_E + 1);
+      const double a = c->h_scal[S_A], t1 = c->h_scal[S_B], t2 = c->h_scal[S_C];
+      alf[j] = a;
+      if (bet[j] > 0.0) bet[j] += t1;
+      alf[j] += t2;
+      *rnm = std::sqrt(c->h_scal[S_E]);
+      const double anorm = bet[j] + std::fabs(alf[j]) + *rnm;
+      *tol = std::sqrt(eps) * anorm;
+      ortbnd(j, *rnm, eps1);
+      purge(*ll, j, rnm, *tol);
+      if (*rnm <= *tol) *rnm = 0.0;
+    }
+    return j;
+  }
+
+  double sparsity() const {
+    return 1.0 - (double)op->nnz / ((double)op->nrows * (double)op->ncols);
+  }
+
+  // mod.rs:840-899
+  int64_t error_bound(bool *enough, double endl, double endr, int64_t step, double tol) {
+    const double eps34 = std::pow(eps, 0.75);
+    int64_t mid = 0;
+    for (int64_t i = 1; i <= step; ++i)
+      if (std::fabs(bnd[i]) > std::fabs(bnd[mid])) mid = i;
+    for (int64_t i = ((step + 1) + (step - 1)) / 2; i > mid + 1; --i)
+      if (std::fabs(ritz[i - 1] - ritz[i]) < eps34 * std::fabs(ritz[i]) && bnd[i] > tol && bnd[i - 1] > tol) {
+        bnd[i - 1] = std::sqrt(bnd[i] * bnd[i] + bnd[i - 1] * bnd[i - 1]);
+        bnd[i] = 0.0;
+      }
+    for (int64_t i = ((step + 1) - (step - 1)) / 2; i + 1 < mid; ++i)
+      if (std::fabs(ritz[i + 1] - ritz[i]) < eps34 * std::fabs(ritz[i]) && bnd[i] > tol && bnd[i + 1] > tol) {
+        bnd[i + 1] = std::sqrt(bnd[i] * bnd[i] + bnd[i + 1] * bnd[i + 1]);
+        bnd[i] = 0.0;
+      }
+    int64_t neig = 0;
+    double gapl = ritz[step] - ritz[0];
+    for (int64_t i = 0; i <= step; ++i) {
+      double gap = gapl;
+      if (i < step) gapl = ritz[i + 1] - ritz[i];
+      gap = std::min(gap, gapl);
+      if (gap > bnd[i]) bnd[i] *= bnd[i] / gap;
+      if (bnd[i] <= 16.0 * eps * std::fabs(ritz[i])) {
+        ++neig;
+        if (!*enough) *enough = endl < ritz[i] && ritz[i] < endr;
+      }
+    }
+    return neig;
+  }
+
+  static void copy_reversed(int64_t n, int64_t off, const double *x, double *y) {
+    for (int64_t i = 0; i < n; ++i) y[off + n - 1 - i] = x[off + i];
+  }
+
+  // mod.rs:1263-1406
+  int64_t lanso(int64_t dim, double endl, double endr, uint32_t seed, int64_t *neig) {
+    const double sp = sparsity();
+    const int64_t max_imtqlb = sp > 0.999 ? 500 : sp > 0.99 ? 300 : sp > 0.9 ? 100 : 50;
+    const double aeps = sp > 0.99 ? eps * 100.0 : sp > 0.9 ? eps * 10.0 : eps;
+    double rnm, tol;
+    stpone(seed, &rnm, &tol);
+    const double eps1 = aeps * std::sqrt((double)ncols);
+    eta[0] = eps1;
+    oldeta[0] = eps1;
+    int64_t ll = 0, first = 1, j = 0, intro = 0;
+    int64_t last = std::min(iterations, std::max<int64_t>(dim, 8) + dim);
+    bool enough = false;
+    while (!enough) {
+      if (rnm <= tol) rnm = 0.0;
+      const int64_t steps = lanczos_step(first, last, &ll, &enough, &rnm, &tol);
+      j = enough ? steps - 1 : last - 1;
+      first = j + 1;
+      bet[first] = rnm;
+      int64_t l = 0;
+      for (int64_t it = 0; it < j && l <= j; ++it) {
+        int64_t i = l;
+        while (i <= j && !(std::fabs(bet[i + 1]) <= aeps)) ++i;
+        i = std::min(i, j);
+        const int64_t sz = i - l;
+        copy_reversed(sz + 1, l, alf.data(), ritz.data());
+        copy_reversed(sz, l + 1, bet.data(), w5.data());
+        imtqlb(sz + 1, ritz.data() + l, w5.data() + l, bnd.data() + l, max_imtqlb, eps);
+        for (int64_t m = l; m <= i; ++m) bnd[m] = rnm * std::fabs(bnd[m]);
+        l = i + 1;
+      }
+      // insertion sort of ritz (carrying bnd), mod.rs:297-307
+      for (int64_t i = 1; i <= j; ++i)
+        for (int64_t k = i; k >= 1 && ritz[k - 1] > ritz[k]; --k) {
+          std::swap(ritz[k - 1], ritz[k]);
+          std::swap(bnd[k - 1], bnd[k]);
+        }
+      if (j < 1) fail(SVDH_ERR_LAS2, "error_bound: expected 'step' to be non-zero");
+      *neig = error_bound(&enough, endl, endr, j, tol);
+      if (*neig < dim) {
+        if (*neig == 0) {
+          last = first + 9;
+          intro = first;
+        } else {
+          const int64_t extra = sp > 0.99 ? 5 : 0;
+          last = first + std::max<int64_t>(3, 1 + ((j - intro) * (dim - *neig)) / *neig) + extra;
+        }
+        last = std::min(last, iterations);
+      } else {
+        enough = true;
+      }
+      enough = enough || first >= iterations;
+    }
+    storq(j, w[1]);
+    return j;
+  }
+
+  // mod.rs:1056-1259 with the Ritz indexing and ordering of legacy.rs:1796-1835.
+  void ritvec(int64_t dims, double kappa, int64_t steps, int64_t neig, svdh_result *out, int64_t *nsig_out) {
+    const int64_t js = steps + 1;
+    const double sp = sparsity();
+    const double aeps = sp > 0.99 ? eps * 100.0 : sp > 0.9 ? eps * 10.0 : eps;
+    const int64_t max_imtql2 = sp > 0.999 ? 500 : sp > 0.99 ? 300 : sp > 0.9 ? 200 : 50;
+    std::vector<double> s(js * js, 0.0), ev(js), e(std::max<int64_t>(js, 1) + 1, 0.0);
+    for (int64_t i = 0; i < js; ++i) s[i * js + i] = 1.0;
+    copy_reversed(js, 0, alf.data(), ev.data());
+    copy_reversed(steps, 1, bet.data(), e.data());
+    imtql2(js, js, ev.data(), e.data(), s.data(), max_imtql2, eps);
+    double max_ev = 0.0;
+    for (double v : ev) max_ev = std::max(max_ev, std::fabs(v));
+    const double akappa = sp > 0.99 ? kappa * 10.0 : kappa;
+    std::vector<int64_t> sig;
+    for (int64_t k = 0; k < js; ++k)
+      if (bnd[k] <= akappa * std::max(std::fabs(ritz[k]), max_ev * aeps) && k + 1 > js - neig) sig.push_back(k);
+    const int64_t nsig = (int64_t)sig.size();
+    const int64_t d = std::min(dims, nsig);
+    *nsig_out = nsig;
+    out->d = d;
+    out->u_rows = d;
+    out->u_cols = transposed ? ncols : nrows;
+    out->vt_rows = d;
+    out->vt_cols = transposed ? nrows : ncols;
+    out->s = static_cast<double *>(std::calloc(std::max<int64_t>(d, 1), sizeof(double)));
+    out->u = static_cast<double *>(std::calloc(std::max<int64_t>(d * out->u_cols, 1), sizeof(double)));
+    out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(d * out->vt_cols, 1), sizeof(double)));
+    if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
+    if (d == 0) return;
+    // Coefficients W (js x d): column r is the eigenvector of T for the r-th
+    // largest significant Ritz value, in the basis q_0..q_{js-1}.
+    std::vector<double> W(js * d, 0.0);
+    for (int64_t r = 0; r < d; ++r) {
+      const int64_t k = sig[nsig - 1 - r];
+      for (int64_t i = 0; i < js; ++i) {
+        const double coef = s[(js - 1 - i) * js + k];
+        if (std::fabs(coef) > aeps) W[i * d + r] = coef;
+      }
+    }
+    const int64_t ldv = (d + 1) / 2 * 2;
+    DevBuf Wd(W.size() * sizeof(double));
+    HIP_CHECK(hipMemcpyAsync(Wd.p, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    DevBuf V((size_t)ncols * ldv * sizeof(double)), AV((size_t)nrows * ldv * sizeof(double)),
+        BV((size_t)ncols * ldv * sizeof(double));
+    HIP_CHECK(hipMemsetAsync(V.p, 0, V.bytes, c->stream));
+    basis_combine(c, ncols, q(0), ldq, js, Wd.as<double>(), d, V.as<double>(), ldv);
+    spmm(op, fwd, vdt, V.p, ldv, SVDH_F64, AV.p, ldv, SVDH_F64, d);
+    spmm(op, bwd, vdt, AV.p, ldv, SVDH_F64, BV.p, ldv, SVDH_F64, d);
+    std::vector<double> hv(ncols * ldv), hav(nrows * ldv), hbv(ncols * ldv);
+    HIP_CHECK(hipMemcpyAsync(hv.data(), V.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(hav.data(), AV.p, hav.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(hbv.data(), BV.p, hbv.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    // The working (possibly transposed) layout: Ut is d x nrows, Vt is d x ncols.
+    double *Ut = transposed ? out->vt : out->u;
+    double *Vt = transposed ? out->u : out->vt;
+    for (int64_t r = 0; r < d; ++r) {
+      double t = 0.0;
+      for (int64_t i = 0; i < ncols; ++i) t += hv[i * ldv + r] * hbv[i * ldv + r];
+      const double sval = std::sqrt(std::max(t, 0.0));
+      out->s[r] = sval;
+      const double scale = sval > aeps ? 1.0 / sval : 1.0 / std::max(sval, aeps);
+      for (int64_t i = 0; i < nrows; ++i) Ut[r * nrows + i] = hav[i * ldv + r] * scale;
+      for (int64_t i = 0; i < ncols; ++i) Vt[r * ncols + i] = hv[i * ldv + r];
+    }
+  }
+};
+
+}  // namespace
+
+int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
+                   double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out) {
+  if (precision != SVDH_PREC_F64 && precision != SVDH_PREC_F32)
+    fail(SVDH_ERR_ARG, "LAS2 precision must be F64 or F32");
+  if (op->nrows <= 0 || op->ncols <= 0) fail(SVDH_ERR_LAS2, "svd_las2: empty matrix");
+  if (random_seed == 0) random_seed = std::random_device{}();
+  const int64_t mn = std::min(op->nrows, op->ncols);
+  if (dimensions == 0 || dimensions > mn) dimensions = mn;
+  if (iterations == 0 || iterations > mn)
+    iterations = mn;
+  else if (iterations < dimensions)
+    iterations = dimensions;
+  if (dimensions < 2) fail(SVDH_ERR_LAS2, "svd_las2: insufficient dimensions: %lld", (long long)dimensions);
+
+  Las2 L;
+  L.c = ctx;
+  L.op = op;
+  L.vdt = precision == SVDH_PREC_F64 ? SVDH_F64 : SVDH_F32;
+  L.eps = precision == SVDH_PREC_F64 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+  L.transposed = (double)op->ncols >= (double)op->nrows * 1.2;
+  L.fwd = L.transposed ? 1 : 0;
+  L.bwd = 1 - L.fwd;
+  L.nrows = L.transposed ? op->ncols : op->nrows;
+  L.ncols = L.transposed ? op->nrows : op->ncols;
+  L.iterations = iterations;
+  const int64_t n = L.ncols;
+  for (int i = 0; i < 5; ++i) {
+    L.wbuf[i].alloc(n * sizeof(double));
+    L.w[i] = L.wbuf[i].as<double>();
+    HIP_CHECK(hipMemsetAsync(L.w[i], 0, n * sizeof(double), ctx->stream));
+  }
+  L.tempb.alloc(L.nrows * sizeof(double));
+  L.temp = L.tempb.as<double>();
+  L.ldq = (n + 31) / 32 * 32;
+  L.ensure_slots(std::min<int64_t>(iterations + kMaxLL + 2, 2 * dimensions + 32));
+  L.alf.assign(iterations + 1, 0.0);
+  L.eta.assign(iterations + 1, 0.0);
+  L.oldeta.assign(iterations + 1, 0.0);
+  L.bet.assign(iterations + 2, 0.0);
+  L.ritz.assign(iterations + 2, 0.0);
+  L.bnd.assign(iterations + 2, 1.7976931348623157e308);
+  L.w5.assign(iterations + 2, 0.0);
+  if (start) HIP_CHECK(hipMemcpyAsync(L.w[0], start, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+
+  int64_t neig = 0, nsig = 0;
+  const int64_t steps = L.lanso(dimensions, endl, endr, random_seed, &neig);
+  const double kap = std::max(std::fabs(kappa), std::pow(L.eps, 0.75));
+  L.ritvec(dimensions, kap, steps, neig, out, &nsig);
+  svdh_diagnostics &dg = out->diagnostics;
+  dg.non_zero = op->nnz;
+  dg.dimensions = dimensions;
+  dg.iterations = iterations;
+  dg.transposed = L.transposed ? 1 : 0;
+  dg.lanczos_steps = steps + 1;
+  dg.ritz_values_stabilized = neig;
+  dg.significant_values = out->d;
+  dg.singular_values = nsig;
+  dg.end_interval[0] = endl;
+  dg.end_interval[1] = endr;
+  dg.kappa = kap;
+  dg.random_seed = random_seed;
+  return SVDH_OK;
+}
+
+}  // namespace svdh
+
+extern "C" int32_t svdh_las2_start_vector(uint32_t seed, int64_t n, double *out) {
+  if (!out || n < 0) return SVDH_ERR_ARG;
+  const uint64_t phi = 0x9E3779B97F4A7C15ULL;
+  const uint64_t key = ((uint64_t)seed * phi) ^ 0x5356444C49423031ULL;
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t z = key + (uint64_t)(i + 1) * phi;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    out[i] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+  }
+  return SVDH_OK;
+}

@@ -1,0 +1,832 @@
+// Sparse operators of the svdh library: CSR upload, the transpose (CSC) and
+// masked views, nnz-sorted row schedules, SpMV and SpMM kernels for gfx950.
+//
+// Reference semantics:
+//   y = A x, y = A^T x          src/lanczos/mod.rs:1520-1603 (CsrMatrix::svd_opa)
+//   masked remap                src/lanczos/masked.rs:104-238
+//   Y = A X, Y = A^T X          src/lanczos/masked.rs:265-471 (multiply_with_dense)
+//
+// Design: every direction of an operator is a CSR whose rows are output
+// rows (A^T is the CSC built once at upload). Output rows are sorted by nnz
+// and cut into bins; a bin assigns 2^lg lanes to a row (SpMV) or a row group
+// of lanes across panel columns (SpMM). Rows longer than a threshold are cut
+// into fixed nnz chunks, one workgroup each; the last chunk of a row to finish
+// (atomic counter, self-resetting) adds the chunk partials in chunk order, so
+// results are deterministic and no output is written with atomics.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "common.h"
+
+namespace svdh {
+
+namespace {
+
+constexpr int kSpmvLong = 1024;   // SpMV rows above this nnz are chunked
+constexpr int kSpmvChunk = 4096;  // SpMV chunk nnz (256 threads x 16)
+constexpr int kSpmmLong = 768;    // SpMM rows above this nnz are chunked
+constexpr int kSpmmChunk = 1024;  // SpMM chunk nnz
+
+typedef uint16_t bf16_t;
+
+__device__ __forceinline__ float bf16_to_f32(bf16_t h) {
+  return __uint_as_float(static_cast<uint32_t>(h) << 16);
+}
+
+__host__ __device__ __forceinline__ bf16_t f32_to_bf16(float f) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t u = __float_as_uint(f);
+#else
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+#endif
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return static_cast<bf16_t>((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<bf16_t>(u >> 16);
+}
+
+template <typename T> __device__ __forceinline__ float to_f(T v);
+template <> __device__ __forceinline__ float to_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ float to_f<double>(double v) { return (float)v; }
+template <> __device__ __forceinline__ float to_f<bf16_t>(bf16_t v) { return bf16_to_f32(v); }
+
+template <typename Acc, typename T> __device__ __forceinline__ Acc ld_as(const T *p, int64_t i);
+template <> __device__ __forceinline__ double ld_as<double, double>(const double *p, int64_t i) { return p[i]; }
+template <> __device__ __forceinline__ double ld_as<double, float>(const float *p, int64_t i) { return (double)p[i]; }
+template <> __device__ __forceinline__ double ld_as<double, bf16_t>(const bf16_t *p, int64_t i) { return (double)bf16_to_f32(p[i]); }
+template <> __device__ __forceinline__ float ld_as<float, float>(const float *p, int64_t i) { return p[i]; }
+template <> __device__ __forceinline__ float ld_as<float, double>(const double *p, int64_t i) { return (float)p[i]; }
+template <> __device__ __forceinline__ float ld_as<float, bf16_t>(const bf16_t *p, int64_t i) { return bf16_to_f32(p[i]); }
+
+// Panel vector width in elements: 16 bytes per lane.
+template <typename X> struct PanelVec { static constexpr int N = 16 / sizeof(X); };
+
+template <typename Acc, typename X>
+__device__ __forceinline__ void ld_panel(const X *p, Acc (&out)[PanelVec<X>::N]);
+template <> __device__ __forceinline__ void ld_panel<double, double>(const double *p, double (&o)[2]) {
+  double2 v = *reinterpret_cast<const double2 *>(p);
+  o[0] = v.x; o[1] = v.y;
+}
+template <> __device__ __forceinline__ void ld_panel<float, float>(const float *p, float (&o)[4]) {
+  float4 v = *reinterpret_cast<const float4 *>(p);
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+template <> __device__ __forceinline__ void ld_panel<double, float>(const float *p, double (&o)[4]) {
+  float4 v = *reinterpret_cast<const float4 *>(p);
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+template <> __device__ __forceinline__ void ld_panel<float, bf16_t>(const bf16_t *p, float (&o)[8]) {
+  uint4 v = *reinterpret_cast<const uint4 *>(p);
+  uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    o[2 * i] = __uint_as_float(w[i] << 16);
+    o[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+  }
+}
+
+template <typename Y, typename Acc, int N>
+__device__ __forceinline__ void st_panel(Y *p, const Acc (&v)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) p[i] = static_cast<Y>(v[i]);
+}
+template <> __device__ __forceinline__ void st_panel<float, float, 4>(float *p, const float (&v)[4]) {
+  *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+template <> __device__ __forceinline__ void st_panel<float, float, 8>(float *p, const float (&v)[8]) {
+  reinterpret_cast<float4 *>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+  reinterpret_cast<float4 *>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+template <> __device__ __forceinline__ void st_panel<double, double, 2>(double *p, const double (&v)[2]) {
+  *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------------
+// SpMV
+// ------------------------------------------------------------------------
+
+template <typename V, typename X, typename Y>
+struct SpmvArgs {
+  const int32_t *rowptr;
+  const int32_t *colidx;
+  const V *vals;
+  const int32_t *rowmap;
+  const int32_t *colmap;
+  const int32_t *rows;
+  const int32_t *chunks;
+  int32_t *counters;
+  double *partials;
+  const X *x;
+  Y *y;
+  int32_t nbins;
+  int32_t long_block_begin;
+  BinDesc bins[kMaxBins];
+};
+
+template <int LG, bool MC, typename V, typename X, typename Y>
+__device__ __forceinline__ void spmv_group(const SpmvArgs<V, X, Y> &a, const BinDesc &d, int lb) {
+  constexpr int G = 1 << LG;
+  constexpr int RPB = kBlock >> LG;
+  const int gid = threadIdx.x >> LG;
+  const int lane = threadIdx.x & (G - 1);
+  const int idx = lb * RPB + gid;
+  const bool active = idx < d.row_count;
+  double acc = 0.0;
+  int r = 0;
+  if (active) {
+    r = a.rows[d.row_begin + idx];
+    const int src = a.rowmap ? a.rowmap[r] : r;
+    int k = a.rowptr[src] + lane;
+    const int e = a.rowptr[src + 1];
+    for (; k + 3 * G < e; k += 4 * G) {
+      int c[4];
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        c[u] = a.colidx[k + u * G];
+        v[u] = ld_as<double>(a.vals, k + u * G);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int cc = c[u];
+        if (MC) cc = a.colmap[cc];
+        if (!MC || cc >= 0) acc += v[u] * ld_as<double>(a.x, cc);
+      }
+    }
+    for (; k < e; k += G) {
+      int cc = a.colidx[k];
+      if (MC) cc = a.colmap[cc];
+      if (!MC || cc >= 0) acc += ld_as<double>(a.vals, k) * ld_as<double>(a.x, cc);
+    }
+  }
+  if (G > 1) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+  }
+  if (active && lane == 0) a.y[r] = static_cast<Y>(acc);
+}
+
+template <bool MC, typename V, typename X, typename Y>
+__device__ __forceinline__ void spmv_long(const SpmvArgs<V, X, Y> &a, int ch) {
+  __shared__ double red[kBlock / 64];
+  __shared__ int is_last;
+  const int *cd = a.chunks + 6 * ch;
+  const int r = cd[0], beg = cd[1], end = cd[2], first = cd[3], nch = cd[4], lid = cd[5];
+  double acc = 0.0;
+  int k = beg + threadIdx.x;
+  for (; k + 3 * kBlock < end; k += 4 * kBlock) {
+    int c[4];
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c[u] = a.colidx[k + u * kBlock];
+      v[u] = ld_as<double>(a.vals, k + u * kBlock);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int cc = c[u];
+      if (MC) cc = a.colmap[cc];
+      if (!MC || cc >= 0) acc += v[u] * ld_as<double>(a.x, cc);
+    }
+  }
+  for (; k < end; k += kBlock) {
+    int cc = a.colidx[k];
+    if (MC) cc = a.colmap[cc];
+    if (!MC || cc >= 0) acc += ld_as<double>(a.vals, k) * ld_as<double>(a.x, cc);
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+    if (nch == 1) {
+      a.y[r] = static_cast<Y>(t);
+      is_last = 0;
+    } else {
+      a.partials[ch] = t;
+      __threadfence();
+      const int prev = atomicAdd(&a.counters[lid], 1);
+      is_last = (prev == nch - 1);
+    }
+  }
+  __syncthreads();
+  if (is_last && threadIdx.x == 0) {
+    __threadfence();
+    double t = 0.0;
+    for (int i = 0; i < nch; ++i) t += __builtin_nontemporal_load(&a.partials[first + i]);
+    a.y[r] = static_cast<Y>(t);
+    a.counters[lid] = 0;
+  }
+}
+
+template <bool MC, typename V, typename X, typename Y>
+__global__ __launch_bounds__(kBlock) void k_spmv(SpmvArgs<V, X, Y> a) {
+  const int b = blockIdx.x;
+  if (b >= a.long_block_begin) {
+    spmv_long<MC>(a, b - a.long_block_begin);
+    return;
+  }
+  BinDesc d = a.bins[0];
+#pragma unroll
+  for (int i = 1; i < kMaxBins; ++i)
+    if (i < a.nbins && b >= a.bins[i].block_begin) d = a.bins[i];
+  const int lb = b - d.block_begin;
+  switch (d.lg) {
+    case 0: spmv_group<0, MC>(a, d, lb); break;
+    case 1: spmv_group<1, MC>(a, d, lb); break;
+    case 2: spmv_group<2, MC>(a, d, lb); break;
+    case 3: spmv_group<3, MC>(a, d, lb); break;
+    case 4: spmv_group<4, MC>(a, d, lb); break;
+    case 5: spmv_group<5, MC>(a, d, lb); break;
+    default: spmv_group<6, MC>(a, d, lb); break;
+  }
+}
+
+// ------------------------------------------------------------------------
+// SpMM: Y (rows x l, row-major, ldy) = A X (X row-major, ldx)
+// A row group of R = 2^LGR lanes owns one output row; lane t covers panel
+// columns [c0 + t*NV, c0 + (t+1)*NV) of each column pass.
+// ------------------------------------------------------------------------
+
+template <typename V, typename X, typename Y, typename Acc>
+struct SpmmArgs {
+  const int32_t *rowptr;
+  const int32_t *colidx;
+  const V *vals;
+  const int32_t *rowmap;
+  const int32_t *colmap;
+  const int32_t *rows;
+  const int32_t *chunks;
+  int32_t *counters;
+  Acc *partials;  // nchunks x ldp
+  const X *x;
+  Y *y;
+  int64_t ldx, ldy, ldp;
+  int32_t l;
+  int32_t row_count;
+  int32_t long_block_begin;
+};
+
+template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+__device__ __forceinline__ void spmm_rows(const SpmmArgs<V, X, Y, Acc> &a, int b) {
+  constexpr int R = 1 << LGR;
+  constexpr int NV = PanelVec<X>::N;
+  constexpr int RPB = kBlock >> LGR;
+  const int gid = threadIdx.x >> LGR;
+  const int lane = threadIdx.x & (R - 1);
+  const int idx = b * RPB + gid;
+  const bool active = idx < a.row_count;
+  int r = 0, k0 = 0, e = 0;
+  if (active) {
+    r = a.rows[idx];
+    const int src = a.rowmap ? a.rowmap[r] : r;
+    k0 = a.rowptr[src];
+    e = a.rowptr[src + 1];
+  }
+  for (int c0 = 0; c0 < a.l; c0 += R * NV) {
+    const int col = c0 + lane * NV;
+    const bool colok = col < a.l;
+    Acc acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
+    for (int base = k0; base < e; base += R) {
+      int c = -1;
+      Acc v = Acc(0);
+      const int kk = base + lane;
+      if (kk < e) {
+        c = a.colidx[kk];
+        if (MC) c = a.colmap[c];
+        v = ld_as<Acc>(a.vals, kk);
+      }
+      const int cnt = min(R, e - base);
+      for (int j = 0; j < cnt; ++j) {
+        const int cj = __shfl(c, j, R);
+        const Acc vj = __shfl(v, j, R);
+        if ((!MC || cj >= 0) && colok) {
+          Acc xv[NV];
+          ld_panel<Acc, X>(a.x + (int64_t)cj * a.ldx + col, xv);
+#pragma unroll
+          for (int i = 0; i < NV; ++i) acc[i] += vj * xv[i];
+        }
+      }
+    }
+    if (active && colok) st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
+  }
+}
+
+template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+__device__ __forceinline__ void spmm_long(const SpmmArgs<V, X, Y, Acc> &a, int ch) {
+  constexpr int R = 1 << LGR;
+  constexpr int NV = PanelVec<X>::N;
+  constexpr int NG = kBlock >> LGR;  // row groups per workgroup
+  __shared__ Acc red[kBlock * PanelVec<X>::N];
+  __shared__ int is_last;
+  const int *cd = a.chunks + 6 * ch;
+  const int r = cd[0], beg = cd[1], end = cd[2], first = cd[3], nch = cd[4], lid = cd[5];
+  const int gid = threadIdx.x >> LGR;
+  const int lane = threadIdx.x & (R - 1);
+  const int len = end - beg;
+  const int per = (len + NG - 1) / NG;
+  const int gb = beg + gid * per;
+  const int ge = min(end, gb + per);
+  for (int c0 = 0; c0 < a.l; c0 += R * NV) {
+    const int col = c0 + lane * NV;
+    const bool colok = col < a.l;
+    Acc acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
+    for (int base = gb; base < ge; base += R) {
+      int c = -1;
+      Acc v = Acc(0);
+      const int kk = base + lane;
+      if (kk < ge) {
+        c = a.colidx[kk];
+        if (MC) c = a.colmap[c];
+        v = ld_as<Acc>(a.vals, kk);
+      }
+      const int cnt = min(R, ge - base);
+      for (int j = 0; j < cnt; ++j) {
+        const int cj = __shfl(c, j, R);
+        const Acc vj = __shfl(v, j, R);
+        if ((!MC || cj >= 0) && colok) {
+          Acc xv[NV];
+          ld_panel<Acc, X>(a.x + (int64_t)cj * a.ldx + col, xv);
+#pragma unroll
+          for (int i = 0; i < NV; ++i) acc[i] += vj * xv[i];
+        }
+      }
+    }
+    // Reduce the NG group partials of this column pass in group order.
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[gid * (R * NV) + lane * NV + i] = acc[i];
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * NV; t += kBlock) {
+      const int cc = c0 + t;
+      if (cc < a.l) {
+        Acc s = Acc(0);
+        for (int g = 0; g < NG; ++g) s += red[g * (R * NV) + t];
+        if (nch == 1)
+          a.y[(int64_t)r * a.ldy + cc] = static_cast<Y>(s);
+        else
+          a.partials[(int64_t)ch * a.ldp + cc] = s;
+      }
+    }
+    __syncthreads();
+  }
+  if (nch == 1) return;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int prev = atomicAdd(&a.counters[lid], 1);
+    is_last = (prev == nch - 1);
+  }
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    for (int cc = threadIdx.x; cc < a.l; cc += kBlock) {
+      Acc s = Acc(0);
+      for (int i = 0; i < nch; ++i)
+        s += __builtin_nontemporal_load(&a.partials[(int64_t)(first + i) * a.ldp + cc]);
+      a.y[(int64_t)r * a.ldy + cc] = static_cast<Y>(s);
+    }
+    if (threadIdx.x == 0) a.counters[lid] = 0;
+  }
+}
+
+template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs<V, X, Y, Acc> a) {
+  const int b = blockIdx.x;
+  if (b >= a.long_block_begin)
+    spmm_long<LGR, MC>(a, b - a.long_block_begin);
+  else
+    spmm_rows<LGR, MC>(a, b);
+}
+
+// ------------------------------------------------------------------------
+// Value conversion and masked nnz
+// ------------------------------------------------------------------------
+
+template <typename S, typename D> __device__ __forceinline__ D cvt(S v) { return static_cast<D>(v); }
+template <> __device__ __forceinline__ bf16_t cvt<double, bf16_t>(double v) { return f32_to_bf16((float)v); }
+template <> __device__ __forceinline__ bf16_t cvt<float, bf16_t>(float v) { return f32_to_bf16(v); }
+
+template <typename S, typename D>
+__global__ void k_convert(const S *s, D *d, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = cvt<S, D>(s[i]);
+}
+
+int grid_for(int64_t n, int per_block = kBlock) {
+  int64_t g = (n + per_block - 1) / per_block;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 4096));
+}
+
+}  // namespace
+
+size_t dtype_size(int dt) {
+  switch (dt) {
+    case SVDH_F64: return 8;
+    case SVDH_F32: return 4;
+    case SVDH_BF16: return 2;
+  }
+  fail(SVDH_ERR_ARG, "unknown dtype %d", dt);
+}
+
+// ------------------------------------------------------------------------
+// Host side: operator construction
+// ------------------------------------------------------------------------
+
+static void convert_values(Ctx *c, const void *src, int sdt, void *dst, int ddt, int64_t n) {
+  if (n == 0) return;
+  const int g = grid_for(n);
+  if (sdt == SVDH_F64 && ddt == SVDH_F32)
+    hipLaunchKernelGGL((k_convert<double, float>), g, kBlock, 0, c->stream, (const double *)src, (float *)dst, n);
+  else if (sdt == SVDH_F32 && ddt == SVDH_F64)
+    hipLaunchKernelGGL((k_convert<float, double>), g, kBlock, 0, c->stream, (const float *)src, (double *)dst, n);
+  else if (sdt == SVDH_F64 && ddt == SVDH_BF16)
+    hipLaunchKernelGGL((k_convert<double, bf16_t>), g, kBlock, 0, c->stream, (const double *)src, (bf16_t *)dst, n);
+  else if (sdt == SVDH_F32 && ddt == SVDH_BF16)
+    hipLaunchKernelGGL((k_convert<float, bf16_t>), g, kBlock, 0, c->stream, (const float *)src, (bf16_t *)dst, n);
+  else
+    fail(SVDH_ERR_ARG, "unsupported value conversion %d -> %d", sdt, ddt);
+  HIP_CHECK(hipGetLastError());
+}
+
+void ensure_values(svdh_op *op, int dtype) {
+  svdh_op *owner = const_cast<svdh_op *>(op->base ? op->base : op);
+  if (owner->vals[dtype].p) return;
+  if (owner->base_dtype == SVDH_BF16)
+    fail(SVDH_ERR_ARG, "operator values are BF16; wider copies are not derivable");
+  const int64_t nnz = owner->nnz;
+  const size_t sz = dtype_size(dtype);
+  owner->vals[dtype].alloc(std::max<size_t>(nnz * sz, 16));
+  owner->t_vals[dtype].alloc(std::max<size_t>(nnz * sz, 16));
+  Ctx *c = owner->ctx;
+  convert_values(c, owner->vals[owner->base_dtype].p, owner->base_dtype, owner->vals[dtype].p, dtype, nnz);
+  convert_values(c, owner->t_vals[owner->base_dtype].p, owner->base_dtype, owner->t_vals[dtype].p, dtype, nnz);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace svdh
+
+using namespace svdh;
+
+svdh::CsrView svdh_op::view(int direction) const {
+  CsrView v;
+  const svdh_op *own = base ? base : this;
+  if (direction == 0) {
+    v.nrows = nrows;
+    v.ncols = ncols;
+    v.rowptr = own->rowptr.as<int32_t>();
+    v.colidx = own->colidx.as<int32_t>();
+    for (int d = 0; d < 3; ++d) v.vals[d] = own->vals[d].p;
+    v.colmap = base ? o2m.as<int32_t>() : nullptr;
+    v.h_rowptr = const_cast<std::vector<int32_t> *>(&own->h_rowptr);
+  } else {
+    v.nrows = ncols;
+    v.ncols = nrows;
+    v.rowptr = own->t_rowptr.as<int32_t>();
+    v.colidx = own->t_colidx.as<int32_t>();
+    for (int d = 0; d < 3; ++d) v.vals[d] = own->t_vals[d].p;
+    v.rowmap = base ? m2o.as<int32_t>() : nullptr;
+    v.h_rowptr = const_cast<std::vector<int32_t> *>(&own->h_t_rowptr);
+    v.h_rowmap = base ? &h_m2o : nullptr;
+  }
+  v.nnz = nnz;
+  return v;
+}
+
+namespace svdh {
+
+// Build (once) the schedule of one operator direction.
+//   kind 0 (SpMV): bins of 2^lg lanes per row, lg = ceil(log2(ceil(nnz/4)))
+//                  clamped to [0, 6]; rows above kSpmvLong nnz are chunked.
+//   kind 1 (SpMM): one bin of short rows, rows above kSpmmLong chunked.
+static Schedule &get_schedule(svdh_op *op, int direction, int kind) {
+  Schedule &s = op->sched[direction][kind];
+  if (s.built) return s;
+  CsrView v = op->view(direction);
+  const int64_t nout = v.nrows;
+  const std::vector<int32_t> &rp = *v.h_rowptr;
+  std::vector<int32_t> len(nout);
+  for (int64_t r = 0; r < nout; ++r) {
+    const int64_t src = v.h_rowmap ? (*v.h_rowmap)[r] : r;
+    len[r] = rp[src + 1] - rp[src];
+  }
+  std::vector<int32_t> order(nout);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return len[a] < len[b]; });
+  const int long_t = kind == 0 ? kSpmvLong : kSpmmLong;
+  const int chunk = kind == 0 ? kSpmvChunk : kSpmmChunk;
+  int64_t nshort = 0;
+  while (nshort < nout && len[order[nshort]] <= long_t) ++nshort;
+  // Long-row chunks.
+  std::vector<int32_t> ch;
+  int32_t nlong = 0;
+  for (int64_t i = nshort; i < nout; ++i) {
+    const int32_t r = order[i];
+    const int64_t src = v.h_rowmap ? (*v.h_rowmap)[r] : r;
+    const int32_t b = rp[src], e = rp[src + 1];
+    const int32_t n = (e - b + chunk - 1) / chunk;
+    const int32_t first = (int32_t)(ch.size() / 6);
+    for (int32_t k = 0; k < n; ++k) {
+      ch.push_back(r);
+      ch.push_back(b + k * chunk);
+      ch.push_back(std::min(e, b + (k + 1) * chunk));
+      ch.push_back(first);
+      ch.push_back(n);
+      ch.push_back(nlong);
+    }
+    ++nlong;
+  }
+  s.nchunks = (int32_t)(ch.size() / 6);
+  s.nlong_rows = nlong;
+  s.nbins = 0;
+  int32_t blocks = 0;
+  if (kind == 0) {
+    int64_t i = 0;
+    for (int lg = 0; lg <= 6 && i < nshort; ++lg) {
+      const int64_t lim = lg == 6 ? long_t : (4LL << lg);
+      int64_t j = i;
+      while (j < nshort && len[order[j]] <= lim) ++j;
+      if (j > i) {
+        BinDesc &d = s.bins[s.nbins++];
+        d.block_begin = blocks;
+        d.row_begin = (int32_t)i;
+        d.row_count = (int32_t)(j - i);
+        d.lg = lg;
+        blocks += (int32_t)((d.row_count + (kBlock >> lg) - 1) / (kBlock >> lg));
+      }
+      i = j;
+    }
+  } else {
+    BinDesc &d = s.bins[s.nbins++];
+    d.block_begin = 0;
+    d.row_begin = 0;
+    d.row_count = (int32_t)nshort;
+    d.lg = 0;
+  }
+  s.long_block_begin = blocks;
+  s.total_blocks = blocks + s.nchunks;
+  Ctx *c = op->ctx;
+  s.rows.alloc(std::max<int64_t>(nout, 1) * sizeof(int32_t));
+  if (nout) HIP_CHECK(hipMemcpy(s.rows.p, order.data(), nout * sizeof(int32_t), hipMemcpyHostToDevice));
+  s.chunks.alloc(std::max<size_t>(ch.size(), 6) * sizeof(int32_t));
+  if (!ch.empty()) HIP_CHECK(hipMemcpy(s.chunks.p, ch.data(), ch.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  s.counters.alloc(std::max<int32_t>(nlong, 1) * sizeof(int32_t));
+  HIP_CHECK(hipMemset(s.counters.p, 0, std::max<int32_t>(nlong, 1) * sizeof(int32_t)));
+  (void)c;
+  s.built = true;
+  return s;
+}
+
+template <typename V, typename X, typename Y>
+static void launch_spmv(svdh_op *op, int direction, const void *vals, const void *x, void *y) {
+  Ctx *c = op->ctx;
+  CsrView v = op->view(direction);
+  Schedule &s = get_schedule(op, direction, 0);
+  if (v.nrows == 0) return;
+  s.partials.ensure(std::max<int32_t>(s.nchunks, 1) * sizeof(double));
+  SpmvArgs<V, X, Y> a;
+  a.rowptr = v.rowptr;
+  a.colidx = v.colidx;
+  a.vals = static_cast<const V *>(vals);
+  a.rowmap = v.rowmap;
+  a.colmap = v.colmap;
+  a.rows = s.rows.as<int32_t>();
+  a.chunks = s.chunks.as<int32_t>();
+  a.counters = s.counters.as<int32_t>();
+  a.partials = s.partials.as<double>();
+  a.x = static_cast<const X *>(x);
+  a.y = static_cast<Y *>(y);
+  a.nbins = s.nbins;
+  a.long_block_begin = s.long_block_begin;
+  for (int i = 0; i < kMaxBins; ++i) a.bins[i] = i < s.nbins ? s.bins[i] : BinDesc{INT32_MAX, 0, 0, 0};
+  if (s.nbins == 0) a.bins[0] = BinDesc{INT32_MAX, 0, 0, 0};
+  const double bytes = (double)v.nnz * (sizeof(V) + 4) + (double)(v.nrows + 1) * 4 +
+                       (double)v.ncols * sizeof(X) + (double)v.nrows * sizeof(Y);
+  KTimer t(c, KC_SPMV, bytes);
+  if (v.colmap)
+    hipLaunchKernelGGL((k_spmv<true, V, X, Y>), s.total_blocks, kBlock, 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_spmv<false, V, X, Y>), s.total_blocks, kBlock, 0, c->stream, a);
+  HIP_CHECK(hipGetLastError());
+}
+
+void spmv(svdh_op *op, int direction, int vdt, const void *x, void *y, int xdt) {
+  ensure_values(op, vdt);
+  const void *vals = op->values(direction, vdt);
+  if (vdt == SVDH_F64 && xdt == SVDH_F64)
+    launch_spmv<double, double, double>(op, direction, vals, x, y);
+  else if (vdt == SVDH_F32 && xdt == SVDH_F64)
+    launch_spmv<float, double, double>(op, direction, vals, x, y);
+  else if (vdt == SVDH_F32 && xdt == SVDH_F32)
+    launch_spmv<float, float, float>(op, direction, vals, x, y);
+  else if (vdt == SVDH_F64 && xdt == SVDH_F32)
+    launch_spmv<double, float, float>(op, direction, vals, x, y);
+  else
+    fail(SVDH_ERR_ARG, "unsupported SpMV dtypes values=%d vectors=%d", vdt, xdt);
+}
+
+template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+static void spmm_launch_one(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, int blocks) {
+  hipLaunchKernelGGL((k_spmm<LGR, MC, V, X, Y, Acc>), blocks, kBlock, 0, c->stream, a);
+}
+
+template <bool MC, typename V, typename X, typename Y, typename Acc>
+static void spmm_dispatch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, int lgr, int blocks) {
+  switch (lgr) {
+    case 1: spmm_launch_one<1, MC>(c, a, blocks); break;
+    case 2: spmm_launch_one<2, MC>(c, a, blocks); break;
+    case 3: spmm_launch_one<3, MC>(c, a, blocks); break;
+    case 4: spmm_launch_one<4, MC>(c, a, blocks); break;
+    case 5: spmm_launch_one<5, MC>(c, a, blocks); break;
+    default: spmm_launch_one<6, MC>(c, a, blocks); break;
+  }
+}
+
+template <typename V, typename X, typename Y, typename Acc>
+static void launch_spmm(svdh_op *op, int direction, const void *vals, const void *X_, int64_t ldx,
+                        void *Y_, int64_t ldy, int64_t l) {
+  Ctx *c = op->ctx;
+  CsrView v = op->view(direction);
+  Schedule &s = get_schedule(op, direction, 1);
+  if (v.nrows == 0 || l == 0) return;
+  constexpr int NV = PanelVec<X>::N;
+  if (ldx % NV || ldy % NV) fail(SVDH_ERR_ARG, "panel leading dimensions must be multiples of %d", NV);
+  // lanes per row: enough to cover l with NV columns each, at least 2, at most 64
+  int lgr = 1;
+  while ((1 << lgr) * NV < l && lgr < 6) ++lgr;
+  const int rpb = kBlock >> lgr;
+  const int short_blocks = (s.bins[0].row_count + rpb - 1) / rpb;
+  const int64_t ldp = (l + 7) / 8 * 8;
+  s.partials.ensure((size_t)std::max<int32_t>(s.nchunks, 1) * ldp * sizeof(Acc));
+  SpmmArgs<V, X, Y, Acc> a;
+  a.rowptr = v.rowptr;
+  a.colidx = v.colidx;
+  a.vals = static_cast<const V *>(vals);
+  a.rowmap = v.rowmap;
+  a.colmap = v.colmap;
+  a.rows = s.rows.as<int32_t>();
+  a.chunks = s.chunks.as<int32_t>();
+  a.counters = s.counters.as<int32_t>();
+  a.partials = s.partials.as<Acc>();
+  a.x = static_cast<const X *>(X_);
+  a.y = static_cast<Y *>(Y_);
+  a.ldx = ldx;
+  a.ldy = ldy;
+  a.ldp = ldp;
+  a.l = (int32_t)l;
+  a.row_count = s.bins[0].row_count;
+  a.long_block_begin = short_blocks;
+  const int blocks = short_blocks + s.nchunks;
+  const double bytes = (double)v.nnz * (sizeof(V) + 4) + (double)(v.nrows + 1) * 4 +
+                       (double)v.ncols * l * sizeof(X) + (double)v.nrows * l * sizeof(Y);
+  KTimer t(c, KC_SPMM, bytes);
+  if (v.colmap)
+    spmm_dispatch<true>(c, a, lgr, blocks);
+  else
+    spmm_dispatch<false>(c, a, lgr, blocks);
+  HIP_CHECK(hipGetLastError());
+}
+
+void spmm(svdh_op *op, int direction, int vdt, const void *X, int64_t ldx, int xdt, void *Y,
+          int64_t ldy, int ydt, int64_t l) {
+  ensure_values(op, vdt);
+  const void *vals = op->values(direction, vdt);
+  if (vdt == SVDH_F64 && xdt == SVDH_F64 && ydt == SVDH_F64)
+    launch_spmm<double, double, double, double>(op, direction, vals, X, ldx, Y, ldy, l);
+  else if (vdt == SVDH_F32 && xdt == SVDH_F64 && ydt == SVDH_F64)
+    launch_spmm<float, double, double, double>(op, direction, vals, X, ldx, Y, ldy, l);
+  else if (vdt == SVDH_F32 && xdt == SVDH_F32 && ydt == SVDH_F32)
+    launch_spmm<float, float, float, float>(op, direction, vals, X, ldx, Y, ldy, l);
+  else if (vdt == SVDH_BF16 && xdt == SVDH_BF16 && ydt == SVDH_F32)
+    launch_spmm<bf16_t, bf16_t, float, float>(op, direction, vals, X, ldx, Y, ldy, l);
+  else if (vdt == SVDH_F32 && xdt == SVDH_BF16 && ydt == SVDH_F32)
+    launch_spmm<float, bf16_t, float, float>(op, direction, vals, X, ldx, Y, ldy, l);
+  else
+    fail(SVDH_ERR_ARG, "unsupported SpMM dtypes values=%d in=%d out=%d", vdt, xdt, ydt);
+}
+
+void panel_to_bf16(Ctx *c, const float *src, uint16_t *dst, int64_t n) {
+  convert_values(c, src, SVDH_F32, dst, SVDH_BF16, n);
+}
+
+}  // namespace svdh
+
+const void *svdh_op::values(int direction, int dtype) const {
+  const svdh_op *own = base ? base : this;
+  return direction == 0 ? own->vals[dtype].p : own->t_vals[dtype].p;
+}
+
+// ------------------------------------------------------------------------
+// Operator construction (called from api.cpp)
+// ------------------------------------------------------------------------
+
+namespace svdh {
+
+svdh_op *csr_create(Ctx *c, int64_t nrows, int64_t ncols, int64_t nnz, const int64_t *rowptr,
+                    const int64_t *colidx, const void *vals, int32_t vdt, uint32_t flags) {
+  if (nrows < 0 || ncols < 0 || nnz < 0) fail(SVDH_ERR_ARG, "negative shape");
+  if (nnz >= (int64_t)INT32_MAX || nrows >= (int64_t)INT32_MAX || ncols >= (int64_t)INT32_MAX)
+    fail(SVDH_ERR_ARG, "nnz=%lld or a dimension does not fit 32-bit indices", (long long)nnz);
+  if (!rowptr || (nnz > 0 && (!colidx || !vals))) fail(SVDH_ERR_ARG, "null CSR array");
+  if (vdt != SVDH_F64 && vdt != SVDH_F32 && vdt != SVDH_BF16) fail(SVDH_ERR_ARG, "bad value dtype");
+  if (rowptr[0] != 0 || rowptr[nrows] != nnz) fail(SVDH_ERR_ARG, "row pointers must start at 0 and end at nnz");
+  auto op = new svdh_op();
+  try {
+    op->ctx = static_cast<svdh_ctx *>(c);
+    op->nrows = nrows;
+    op->ncols = ncols;
+    op->nnz = nnz;
+    op->base_dtype = vdt;
+    const size_t vs = dtype_size(vdt);
+    op->h_rowptr.resize(nrows + 1);
+    std::vector<int32_t> ci(nnz);
+    std::vector<int32_t> tcount(ncols + 1, 0);
+    for (int64_t i = 0; i < nrows; ++i) {
+      if (rowptr[i + 1] < rowptr[i]) fail(SVDH_ERR_ARG, "row pointers decrease at row %lld", (long long)i);
+      op->h_rowptr[i] = (int32_t)rowptr[i];
+    }
+    op->h_rowptr[nrows] = (int32_t)nnz;
+    for (int64_t k = 0; k < nnz; ++k) {
+      const int64_t cc = colidx[k];
+      if (cc < 0 || cc >= ncols) fail(SVDH_ERR_ARG, "column index %lld out of range", (long long)cc);
+      ci[k] = (int32_t)cc;
+      ++tcount[cc + 1];
+    }
+    // Transpose by a stable counting sort on the column index.
+    op->h_t_rowptr.assign(ncols + 1, 0);
+    for (int64_t j = 0; j < ncols; ++j) op->h_t_rowptr[j + 1] = op->h_t_rowptr[j] + tcount[j + 1];
+    std::vector<int32_t> next(op->h_t_rowptr.begin(), op->h_t_rowptr.end() - 1);
+    std::vector<int32_t> tci(nnz);
+    std::vector<unsigned char> tv(nnz * vs);
+    const unsigned char *vb = static_cast<const unsigned char *>(vals);
+    for (int64_t i = 0; i < nrows; ++i)
+      for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        const int32_t pos = next[ci[k]]++;
+        tci[pos] = (int32_t)i;
+        std::memcpy(&tv[pos * vs], vb + k * vs, vs);
+      }
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+      b.alloc(std::max<size_t>(bytes, 16));
+      if (bytes) HIP_CHECK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    up(op->rowptr, op->h_rowptr.data(), (nrows + 1) * sizeof(int32_t));
+    up(op->colidx, ci.data(), nnz * sizeof(int32_t));
+    up(op->vals[vdt], vals, nnz * vs);
+    up(op->t_rowptr, op->h_t_rowptr.data(), (ncols + 1) * sizeof(int32_t));
+    up(op->t_colidx, tci.data(), nnz * sizeof(int32_t));
+    up(op->t_vals[vdt], tv.data(), nnz * vs);
+    if (flags & SVDH_CSR_KEEP_F32) ensure_values(op, SVDH_F32);
+    if (flags & SVDH_CSR_KEEP_BF16) ensure_values(op, SVDH_BF16);
+  } catch (...) {
+    delete op;
+    throw;
+  }
+  return op;
+}
+
+svdh_op *mask_create(Ctx *c, const svdh_op *base, const uint8_t *mask) {
+  if (!base || !mask) fail(SVDH_ERR_ARG, "null base or mask");
+  if (base->base) fail(SVDH_ERR_ARG, "a masked view cannot be masked again");
+  auto op = new svdh_op();
+  try {
+    op->ctx = static_cast<svdh_ctx *>(c);
+    op->base = base;
+    op->nrows = base->nrows;
+    op->base_dtype = base->base_dtype;
+    std::vector<int32_t> o2m(base->ncols, -1);
+    for (int64_t j = 0; j < base->ncols; ++j)
+      if (mask[j]) {
+        o2m[j] = (int32_t)op->h_m2o.size();
+        op->h_m2o.push_back((int32_t)j);
+      }
+    op->ncols = (int64_t)op->h_m2o.size();
+    int64_t nnz = 0;
+    for (int64_t j = 0; j < base->ncols; ++j)
+      if (mask[j]) nnz += base->h_t_rowptr[j + 1] - base->h_t_rowptr[j];
+    op->nnz = nnz;
+    op->o2m.alloc(std::max<size_t>(o2m.size() * 4, 16));
+    if (!o2m.empty()) HIP_CHECK(hipMemcpy(op->o2m.p, o2m.data(), o2m.size() * 4, hipMemcpyHostToDevice));
+    op->m2o.alloc(std::max<size_t>(op->h_m2o.size() * 4, 16));
+    if (!op->h_m2o.empty())
+      HIP_CHECK(hipMemcpy(op->m2o.p, op->h_m2o.data(), op->h_m2o.size() * 4, hipMemcpyHostToDevice));
+  } catch (...) {
+    delete op;
+    throw;
+  }
+  return op;
+}
+
+}  // namespace svdh

@@ -1,0 +1,12 @@
+"""single_svdlib: MI355X-native drop-in for the sparse-SVD hot path of
+single-svdlib v1.0.9 (module layout of src/lib.rs:1-9).
+
+    from single_svdlib import lanczos, randomized, CsrMatrix
+    a = CsrMatrix.from_scipy(scipy_csr)          # uploaded once to the GPU
+    rec = lanczos.svd_dim_seed(a, 10, 42)        # LAS2 on the GPU
+    rec = randomized.randomized_svd(a, 50, 10, 3, randomized.PowerIterationNormalizer.QR, False, 42, False)
+"""
+from . import error, lanczos, randomized  # noqa: F401
+from .utils import CsrMatrix, Diagnostics, SMat, SvdFloat, SvdRec, determine_chunk_size  # noqa: F401
+
+laczos = lanczos  # README.md:39 spells the module `laczos`

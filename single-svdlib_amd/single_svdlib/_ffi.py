@@ -1,0 +1,128 @@
+"""ctypes binding of libsvdh.so (include/svdh.h).
+
+The library is loaded from the package directory (single-svdlib_amd/libsvdh.so).
+There is no fallback: if the HIP library is missing or no GPU is present,
+the first call that needs it raises.
+"""
+import ctypes as C
+import os
+import threading
+
+from . import error
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libsvdh.so")
+
+SVDH_F64, SVDH_F32, SVDH_BF16 = 0, 1, 2
+PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
+NORM_QR, NORM_LU, NORM_NONE = 0, 1, 2
+KEEP_F32, KEEP_BF16 = 0x1, 0x2
+KC_SPMV, KC_SPMM, KC_DENSE, KC_BLAS1, KC_GEMV = 0, 1, 2, 3, 4
+
+
+class Diagnostics_(C.Structure):
+    _fields_ = [
+        ("non_zero", C.c_int64),
+        ("dimensions", C.c_int64),
+        ("iterations", C.c_int64),
+        ("transposed", C.c_int32),
+        ("lanczos_steps", C.c_int64),
+        ("ritz_values_stabilized", C.c_int64),
+        ("significant_values", C.c_int64),
+        ("singular_values", C.c_int64),
+        ("end_interval", C.c_double * 2),
+        ("kappa", C.c_double),
+        ("random_seed", C.c_uint32),
+    ]
+
+
+class Result_(C.Structure):
+    _fields_ = [
+        ("d", C.c_int64),
+        ("u_rows", C.c_int64),
+        ("u_cols", C.c_int64),
+        ("vt_rows", C.c_int64),
+        ("vt_cols", C.c_int64),
+        ("u", C.POINTER(C.c_double)),
+        ("s", C.POINTER(C.c_double)),
+        ("vt", C.POINTER(C.c_double)),
+        ("diagnostics", Diagnostics_),
+        ("power_iters_run", C.c_int64),
+        ("residual_estimate", C.c_double),
+        ("qr_fallbacks", C.c_int64),
+    ]
+
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int32
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+
+# (name, restype, argtypes) for every entry point declared in include/svdh.h
+SIGNATURES = [
+    ("svdh_ctx_create", _i32, [_i32, C.POINTER(_vp)]),
+    ("svdh_ctx_destroy", _i32, [_vp]),
+    ("svdh_ctx_synchronize", _i32, [_vp]),
+    ("svdh_last_error", C.c_char_p, []),
+    ("svdh_ctx_profile", _i32, [_vp, _i32]),
+    ("svdh_ctx_kernel_time", _i32, [_vp, _i32, _f64p, _i64p, _f64p]),
+    ("svdh_csr_create", _i32, [_vp, _i64, _i64, _i64, _i64p, _i64p, _vp, _i32, C.c_uint32, C.POINTER(_vp)]),
+    ("svdh_mask_create", _i32, [_vp, _vp, C.POINTER(C.c_uint8), C.POINTER(_vp)]),
+    ("svdh_op_destroy", _i32, [_vp]),
+    ("svdh_op_shape", _i32, [_vp, _i64p, _i64p, _i64p]),
+    ("svdh_spmv", _i32, [_vp, _vp, _i32, _vp, _vp, _i32, _i32]),
+    ("svdh_spmm", _i32, [_vp, _vp, _i32, _vp, _i64, _i64, _vp, _i64, _i32]),
+    ("svdh_spmm_qta", _i32, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _i32]),
+    ("svdh_column_means", _i32, [_vp, _vp, _vp, _i32]),
+    ("svdh_las2", _i32, [_vp, _vp, _i64, _i64, C.c_double, C.c_double, C.c_double, C.c_uint32, _f64p, _i32,
+                         C.POINTER(Result_)]),
+    ("svdh_las2_start_vector", _i32, [C.c_uint32, _i64, _f64p]),
+    ("svdh_rsvd", _i32, [_vp, _vp, _i64, _i64, _i64, _i32, _i32, C.c_uint64, _i32, _f64p, _i32, C.c_double, _i64,
+                         C.POINTER(Result_)]),
+    ("svdh_result_free", None, [C.POINTER(Result_)]),
+]
+
+_lib = None
+_lock = threading.Lock()
+_contexts = {}
+
+
+def lib():
+    """The loaded libsvdh.so (loaded on first use; raises if it is missing)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise ImportError(f"{LIB_PATH} is missing: build it with `python single-svdlib_amd/build.py`")
+                h = C.CDLL(LIB_PATH)
+                for name, res, args in SIGNATURES:
+                    f = getattr(h, name)
+                    f.restype = res
+                    f.argtypes = args
+                _lib = h
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        msg = lib().svdh_last_error().decode(errors="replace")
+        raise error.from_status(status, msg)
+
+
+def context(device=0):
+    """The process-wide context of a device (one HIP stream)."""
+    with _lock:
+        ctx = _contexts.get(device)
+    if ctx is None:
+        h = _vp()
+        check(lib().svdh_ctx_create(device, C.byref(h)))
+        with _lock:
+            _contexts.setdefault(device, h)
+            ctx = _contexts[device]
+    return ctx
+
+
+def ptr(a, t=_f64p):
+    return a.ctypes.data_as(t)

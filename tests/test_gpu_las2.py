@@ -1,0 +1,131 @@
+"""GPU LAS2 (lanczos::svd*) against the reference's known answers and the
+FP64 oracle on identical inputs and identical start vectors.
+
+Bars: FP64 singular values within 1e-10 relative of the oracle; singular
+vectors within a max principal angle of 1e-8 rad; diagnostics equal. FP32
+(svd_las2::<f32> control flow): singular values within 1e-5 relative of the
+FP64 truth.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from helpers import golden, max_principal_angle, random_csr, triplet_residual
+
+pytestmark = pytest.mark.gpu
+
+DIAG_KEYS = ("non_zero", "dimensions", "iterations", "lanczos_steps", "ritz_values_stabilized",
+             "significant_values", "singular_values", "random_seed")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import single_svdlib
+    from single_svdlib import _ffi
+    _ffi.context(0)
+    return single_svdlib
+
+
+def cfg1_matrix(seed=7):
+    """2,000 x 1,500, 15 distinct uniform columns per row, U(0,1) values."""
+    rng = np.random.default_rng(seed)
+    m, n, k = 2000, 1500, 15
+    cols = np.stack([rng.choice(n, k, replace=False) for _ in range(m)])
+    rows = np.repeat(np.arange(m), k)
+    return sp.csr_matrix((rng.uniform(size=m * k), (rows, cols.ravel())), shape=(m, n))
+
+
+def compare_with_oracle(a, rec, r, sv_tol=1e-10, angle_tol=1e-8, diag=True):
+    assert rec.d == r["d"]
+    assert np.allclose(rec.s, r["s"], rtol=sv_tol, atol=0), np.max(np.abs(rec.s - r["s"]) / r["s"])
+    if rec.d:
+        assert max_principal_angle(rec.vt.T, r["vt"].T) < angle_tol
+        assert max_principal_angle(rec.u.T, r["u"].T) < angle_tol
+    if diag:
+        for k in DIAG_KEYS:
+            assert getattr(rec.diagnostics, k) == r["diagnostics"][k], k
+        assert rec.diagnostics.transposed == bool(r["diagnostics"]["transposed"])
+
+
+def test_legacy_3x3_known_answers(lib):
+    g = golden("legacy_kats.json")
+    a = lib.CsrMatrix.from_scipy(sp.csc_matrix(np.array(g["matrix_3x3"])))
+    rec = lib.lanczos.svd_dim_seed(a, 0, 3141)
+    dump = g["svdlibrs_3x3_seed3141"]
+    assert rec.d == 3
+    assert np.allclose(rec.s, dump["s"], rtol=0, atol=1e-12)
+    for i in range(3):
+        for got, want in ((rec.u[i], np.array(dump["ut"][i])), (rec.vt[i], np.array(dump["vt"][i]))):
+            assert np.allclose(np.sign(got @ want) * got, want, atol=1e-11)
+    assert np.allclose(rec.u.T @ np.diag(rec.s) @ rec.vt, g["matrix_3x3"], atol=1e-12)
+    for k in DIAG_KEYS:
+        assert getattr(rec.diagnostics, k) == dump["diagnostics"][k], k
+
+
+def test_basic_2x2_and_identity(lib):
+    g = golden("legacy_kats.json")
+    rec = lib.lanczos.svd(lib.CsrMatrix.from_scipy(sp.coo_matrix(np.array(g["basic_2x2"]["matrix"]))))
+    assert rec.d == 2 and np.allclose(rec.s, g["basic_2x2"]["s"], atol=1e-12)
+    assert np.allclose(rec.u.T @ np.diag(rec.s) @ rec.vt, g["basic_2x2"]["matrix"], atol=1e-12)
+    rec = lib.lanczos.svd(lib.CsrMatrix.from_scipy(sp.identity(3, format="csc")))
+    assert rec.d == 1 and abs(rec.s[0] - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("shape,density,k,seed", [((60, 40), 0.2, 8, 42), ((40, 90), 0.15, 6, 7),
+                                                   ((300, 200), 0.05, 12, 1), ((800, 500), 0.01, 20, 3)])
+def test_matches_oracle(lib, shape, density, k, seed):
+    a = random_csr(*shape, density, seed=seed)
+    print(f"case shape={shape} k={k}", flush=True)
+    rec = lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(a), k, seed)
+    st, r = oracle.las2(oracle.OracleOp(a), k, 0, random_seed=seed)
+    assert st == 0
+    compare_with_oracle(a, rec, r)
+    s_true = np.linalg.svd(a.toarray(), compute_uv=False)
+    assert np.allclose(rec.s, s_true[: rec.d], rtol=1e-10)
+
+
+def test_cfg1_las2_f64(lib):
+    a = cfg1_matrix()
+    rec = lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(a), 10, 42)
+    st, r = oracle.las2(oracle.OracleOp(a, nthreads=4), 10, 0, random_seed=42)
+    assert st == 0
+    compare_with_oracle(a, rec, r)
+    assert triplet_residual(a, rec.u.T, rec.s, rec.vt) < 1e-8
+
+
+def test_injected_start_vector(lib):
+    a = random_csr(200, 150, 0.05, seed=11)
+    start = np.random.default_rng(0).uniform(-1, 1, 150)
+    rec = lib.lanczos.svd_las2(lib.CsrMatrix.from_scipy(a), 10, 0, [-1e-30, 1e-30], 1e-6, 99, start=start)
+    st, r = oracle.las2(oracle.OracleOp(a), 10, 0, random_seed=99, start=start)
+    compare_with_oracle(a, rec, r)
+
+
+def test_masked_equals_physical_subset(lib):
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(1400, 1000, 0.01, seed=42, low=0.1, high=10.0)
+    cols = np.sort(np.random.default_rng(1).choice(1000, 300, replace=False))
+    base = lib.CsrMatrix.from_scipy(a)
+    r1 = lib.lanczos.svd_dim_seed(MaskedCSRMatrix.with_columns(base, cols), 10, 5)
+    r2 = lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(a[:, cols]), 10, 5)
+    assert r1.d == r2.d
+    assert np.allclose(r1.s, r2.s, rtol=1e-10)
+
+
+def test_f32_mode(lib):
+    a = random_csr(500, 300, 0.03, seed=3).astype(np.float32)
+    rec = lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(a), 8, 9)
+    assert rec.s.dtype == np.float32
+    s_true = np.linalg.svd(a.astype(np.float64).toarray(), compute_uv=False)
+    assert np.allclose(rec.s, s_true[: rec.d], rtol=1e-5)
+    st, r = oracle.las2(oracle.OracleOp(a.astype(np.float64)), 8, 0, random_seed=9, eps=oracle.F32_EPS)
+    assert st == 0 and rec.d == r["d"]
+
+
+def test_error_paths(lib):
+    from single_svdlib import error
+    with pytest.raises(error.Las2Error):
+        lib.lanczos.svd(lib.CsrMatrix.from_scipy(sp.csr_matrix(np.ones((1, 5)))))
+    with pytest.raises(error.StartvError):
+        lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(sp.csr_matrix((6, 4))), 0, 1)

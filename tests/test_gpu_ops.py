@@ -1,0 +1,160 @@
+"""GPU parity of the SMat operators (libsvdh.so through the C ABI) against
+the FP64 oracle on identical inputs.
+
+Tolerance: the GPU sums each row in a different order than the oracle, so
+each output element must lie within  (nnz_row + 2) * eps * (|A| |x|)_i  of
+the oracle (the standard floating-point bound for a dot product), with eps of
+the accumulation type; FP32 outputs add one rounding of |y|.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from helpers import random_csr, skewed_csr
+
+pytestmark = pytest.mark.gpu
+
+EPS64 = np.finfo(np.float64).eps
+EPS32 = np.finfo(np.float32).eps
+
+
+@pytest.fixture(scope="module")
+def svdlib():
+    import single_svdlib
+    from single_svdlib import _ffi
+    _ffi.context(0)  # fails loudly without a GPU or without libsvdh.so
+    return single_svdlib
+
+
+def bound(a, x, eps):
+    a = sp.csr_matrix(a)
+    n = np.diff(a.indptr).astype(np.float64) + 2.0
+    if x.ndim == 2:
+        n = n[:, None]
+    return n * eps * (abs(a) @ np.abs(x))
+
+
+def check_close(got, want, b):
+    err = np.abs(np.asarray(got, dtype=np.float64) - want)
+    assert np.all(err <= b + 1e-300), float(np.max(err - b))
+
+
+MATRICES = {
+    "uniform": lambda: random_csr(700, 500, 0.02, seed=1),
+    "wide": lambda: random_csr(90, 2000, 0.05, seed=2),
+    "skewed": lambda: skewed_csr(6000, 2000, seed=3, mean_row=20, max_row=3000),
+    "empty_rows": lambda: random_csr(400, 300, 0.002, seed=4),
+    "single": lambda: sp.csr_matrix(np.array([[2.5]])),
+}
+
+
+@pytest.mark.parametrize("name", list(MATRICES))
+def test_spmv_f64(svdlib, name):
+    a = MATRICES[name]()
+    print(f"case {name} shape={a.shape} nnz={a.nnz}", flush=True)
+    op = svdlib.CsrMatrix.from_scipy(a)
+    ref = oracle.OracleOp(a)
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=a.shape[1])
+    y = np.empty(a.shape[0])
+    op.svd_opa(x, y, False)
+    check_close(y, ref.opa(x), bound(a, x, EPS64))
+    xt = rng.normal(size=a.shape[0])
+    yt = np.empty(a.shape[1])
+    op.svd_opa(xt, yt, True)
+    check_close(yt, ref.opa(xt, True), bound(a.T, xt, EPS64))
+    # deterministic: a second call gives identical bits
+    y2 = np.empty_like(yt)
+    op.svd_opa(xt, y2, True)
+    assert np.array_equal(yt, y2)
+
+
+@pytest.mark.parametrize("name", ["uniform", "skewed"])
+def test_spmv_f32(svdlib, name):
+    a = MATRICES[name]().astype(np.float32)
+    op = svdlib.CsrMatrix.from_scipy(a)
+    a64 = a.astype(np.float64)
+    ref = oracle.OracleOp(a64)
+    x = np.random.default_rng(1).normal(size=a.shape[1]).astype(np.float32)
+    y = np.empty(a.shape[0], dtype=np.float32)
+    op.svd_opa(x, y, False)
+    want = ref.opa(x.astype(np.float64))
+    check_close(y, want, bound(a64, x.astype(np.float64), EPS64) + EPS32 * np.abs(want))
+
+
+@pytest.mark.parametrize("name", ["uniform", "wide", "skewed", "empty_rows"])
+@pytest.mark.parametrize("l", [1, 7, 60])
+def test_spmm_f64(svdlib, name, l):
+    a = MATRICES[name]()
+    print(f"case {name} l={l}", flush=True)
+    op = svdlib.CsrMatrix.from_scipy(a)
+    ref = oracle.OracleOp(a)
+    rng = np.random.default_rng(l)
+    X = rng.normal(size=(a.shape[1], l))
+    Y = np.zeros((a.shape[0], l))
+    op.multiply_with_dense(X, Y, False)
+    check_close(Y, ref.multiply_with_dense(X), bound(a, X, EPS64))
+    Yin = rng.normal(size=(a.shape[0], l))
+    Z = np.zeros((a.shape[1], l))
+    op.multiply_with_dense(Yin, Z, True)
+    check_close(Z, ref.multiply_with_dense(Yin, True), bound(a.T, Yin, EPS64))
+    B = np.zeros((l, a.shape[1]))
+    op.multiply_transposed_by_dense(Yin, B)
+    check_close(B, ref.multiply_transposed_by_dense(Yin), bound(a.T, Yin, EPS64).T)
+
+
+@pytest.mark.parametrize("name", ["uniform", "skewed"])
+def test_spmm_f32(svdlib, name):
+    a = MATRICES[name]().astype(np.float32)
+    a64 = a.astype(np.float64)
+    op = svdlib.CsrMatrix.from_scipy(a)
+    ref = oracle.OracleOp(a64)
+    X = np.random.default_rng(2).normal(size=(a.shape[1], 60)).astype(np.float32)
+    Y = np.zeros((a.shape[0], 60), dtype=np.float32)
+    op.multiply_with_dense(X, Y, False)
+    want = ref.multiply_with_dense(X.astype(np.float64))
+    check_close(Y, want, bound(a64, X.astype(np.float64), EPS32) + EPS32 * np.abs(want))
+
+
+def test_masked_ops_equal_physical_subset(svdlib):
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = skewed_csr(5000, 1500, seed=8, mean_row=16, max_row=2000)
+    cols = np.sort(np.random.default_rng(3).choice(1500, 400, replace=False))
+    base = svdlib.CsrMatrix.from_scipy(a)
+    m = MaskedCSRMatrix.with_columns(base, cols)
+    sub = a[:, cols]
+    ref = oracle.OracleOp(a, columns=cols)
+    assert m.nrows() == 5000 and m.ncols() == 400 and m.nnz() == sub.nnz == ref.nnz()
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=400)
+    y = np.empty(5000)
+    m.svd_opa(x, y, False)
+    check_close(y, ref.opa(x), bound(sub, x, EPS64))
+    xt = rng.normal(size=5000)
+    yt = np.empty(400)
+    m.svd_opa(xt, yt, True)
+    check_close(yt, ref.opa(xt, True), bound(sub.T, xt, EPS64))
+    X = rng.normal(size=(400, 30))
+    Y = np.zeros((5000, 30))
+    m.multiply_with_dense(X, Y, False)
+    check_close(Y, ref.multiply_with_dense(X), bound(sub, X, EPS64))
+    Yin = rng.normal(size=(5000, 30))
+    Z = np.zeros((400, 30))
+    m.multiply_with_dense(Yin, Z, True)
+    check_close(Z, ref.multiply_with_dense(Yin, True), bound(sub.T, Yin, EPS64))
+    means = m.compute_column_means()
+    assert np.allclose(means, ref.column_means(), rtol=1e-12, atol=1e-14)
+    assert m.uses_all_columns() is False
+
+
+def test_empty_matrix_and_shape_errors(svdlib):
+    a = sp.csr_matrix((5, 3))
+    op = svdlib.CsrMatrix.from_scipy(a)
+    y = np.ones(5)
+    op.svd_opa(np.ones(3), y, False)
+    assert np.all(y == 0)
+    with pytest.raises(ValueError):
+        op.svd_opa(np.ones(4), y, False)
+    with pytest.raises(ValueError):
+        svdlib.CsrMatrix(np.array([0, 1]), np.array([7]), np.array([1.0]), (1, 3))

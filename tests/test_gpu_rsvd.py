@@ -1,0 +1,125 @@
+"""GPU randomized SVD (randomized::randomized_svd) against the FP64 oracle
+restatement with the same injected test matrix Omega, plus the reference's
+invariants (shapes, positive descending values, power iterations improve the
+fit, normalisers span the same space, masked == physical subset).
+
+Bars: FP64 singular values within 1e-10 relative of the oracle and a max
+principal angle of 1e-8 rad for U and V; FP32 within 1e-5 relative and
+1e-4 rad; BF16 ladder reaches the FP32 answer within 1e-5 in the adaptive mode.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from helpers import max_principal_angle, random_csr, skewed_csr, triplet_residual
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import single_svdlib
+    from single_svdlib import _ffi
+    _ffi.context(0)
+    return single_svdlib
+
+
+def lowrank_plus_noise(m, n, r, seed, noise=1e-3, density=0.2):
+    rng = np.random.default_rng(seed)
+    mask = sp.random(m, n, density=density, random_state=seed, format="csr")
+    dense = (rng.normal(size=(m, r)) * np.linspace(10, 1, r)) @ rng.normal(size=(r, n))
+    vals = dense[mask.nonzero()] + noise * rng.normal(size=mask.nnz)
+    return sp.csr_matrix((vals, mask.nonzero()), shape=(m, n))
+
+
+@pytest.mark.parametrize("dtype,sv_tol,ang_tol", [(np.float64, 1e-10, 1e-8), (np.float32, 1e-5, 1e-3)])
+@pytest.mark.parametrize("q", [0, 2])
+def test_matches_oracle_same_omega(lib, dtype, sv_tol, ang_tol, q):
+    a = random_csr(3000, 800, 0.01, seed=5)
+    rng = np.random.default_rng(1)
+    om = rng.normal(size=(800, 30))
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a.astype(dtype)), 20, 10, q,
+                                        lib.randomized.PowerIterationNormalizer.QR, False, 42, False, omega=om)
+    a64 = a.astype(dtype).astype(np.float64)
+    u, s, vt = oracle.randomized_svd(oracle.OracleOp(a64), 20, 10, q, "QR", omega=om)
+    assert rec.d == 20 and rec.u.shape == (3000, 20) and rec.vt.shape == (20, 800)
+    assert np.allclose(rec.s, s, rtol=sv_tol, atol=0), np.max(np.abs(rec.s - s) / s)
+    # well-separated leading triplets span the same subspaces
+    assert max_principal_angle(rec.u[:, :10], u[:, :10]) < ang_tol
+    assert max_principal_angle(rec.vt[:10].T, vt[:10].T) < ang_tol
+
+
+def test_shapes_order_positive(lib):
+    # randomized/mod.rs:651-690
+    a = random_csr(5000, 1000, 0.01, seed=42)
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a), 20, 10, 2,
+                                        lib.randomized.PowerIterationNormalizer.QR, False, 42, False)
+    assert rec.d == 20
+    assert rec.u.shape == (5000, 20) and rec.vt.shape == (20, 1000)
+    assert np.all(rec.s > 0) and np.all(np.diff(rec.s) <= 0)
+    dg = rec.diagnostics
+    assert (dg.lanczos_steps, dg.iterations, dg.dimensions, dg.random_seed) == (0, 2, 20, 42)
+
+
+def test_power_iterations_improve_fit(lib):
+    # randomized/mod.rs:692-777
+    rng = np.random.default_rng(987)
+    dense = rng.uniform(-1, 1, (100, 10)) @ rng.uniform(-1, 1, (10, 50)) + rng.uniform(-0.01, 0.01, (100, 50))
+    a = lib.CsrMatrix.from_scipy(sp.csr_matrix(dense))
+    errs = []
+    for q in (0, 1, 3, 5):
+        rec = lib.randomized.randomized_svd(a, 10, 5, q, lib.randomized.PowerIterationNormalizer.QR, False, 42, False)
+        errs.append(np.linalg.norm(dense - rec.recompose()) / np.linalg.norm(dense))
+    assert min(errs[1:]) < 0.9 * errs[0] or errs[0] < 1e-3
+
+
+@pytest.mark.parametrize("norm", ["LU", "NONE"])
+def test_normalizers(lib, norm):
+    a = random_csr(2000, 600, 0.02, seed=11)
+    om = np.random.default_rng(5).normal(size=(600, 15))
+    ref = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a), 10, 5, 2, "QR", False, 1, False, omega=om)
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a), 10, 5, 2, norm, False, 1, False, omega=om)
+    assert np.allclose(rec.s, ref.s, rtol=1e-8)
+
+
+def test_adaptive_reaches_tolerance(lib):
+    a = skewed_csr(20000, 3000, seed=21, mean_row=30, max_row=2000).astype(np.float32)
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a), 20, 10, 3, "QR", False, 7, False, tol=1e-5,
+                                        max_power_iters=300)
+    assert rec.extra["residual_estimate"] <= 1e-5
+    assert triplet_residual(a, rec.u, rec.s, rec.vt) < 3e-5
+
+
+def test_bf16_ladder_matches_f32(lib):
+    a = skewed_csr(20000, 3000, seed=22, mean_row=30, max_row=2000).astype(np.float32)
+    op = lib.CsrMatrix.from_scipy(a, keep_bf16=True)
+    r32 = lib.randomized.randomized_svd(op, 20, 10, 3, "QR", False, 7, False, tol=1e-5, max_power_iters=300)
+    r16 = lib.randomized.randomized_svd(op, 20, 10, 3, "QR", False, 7, False, tol=1e-5, max_power_iters=300,
+                                        precision="bf16")
+    assert r16.extra["residual_estimate"] <= 1e-5
+    assert np.allclose(r16.s, r32.s, rtol=1e-5)
+
+
+def test_masked_equals_physical_subset(lib):
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(3000, 1200, 0.01, seed=3)
+    cols = np.sort(np.random.default_rng(2).choice(1200, 500, replace=False))
+    om = np.random.default_rng(3).normal(size=(500, 25))
+    base = lib.CsrMatrix.from_scipy(a)
+    r1 = lib.randomized.randomized_svd(MaskedCSRMatrix.with_columns(base, cols), 15, 10, 2, "QR", False, 4, False,
+                                       omega=om)
+    r2 = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a[:, cols]), 15, 10, 2, "QR", False, 4, False,
+                                       omega=om)
+    assert np.allclose(r1.s, r2.s, rtol=1e-10)
+    assert max_principal_angle(r1.u[:, :8], r2.u[:, :8]) < 1e-8
+
+
+def test_rank_deficient_and_tiny(lib):
+    # l > rank: the panel Gram is singular; the eigen fallback keeps it usable
+    rng = np.random.default_rng(0)
+    dense = rng.normal(size=(30, 3)) @ rng.normal(size=(3, 10))
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(sp.csr_matrix(dense)), 3, 3, 2, "QR", False, 42,
+                                        False)
+    s_true = np.linalg.svd(dense, compute_uv=False)[:3]
+    assert np.allclose(rec.s, s_true, rtol=1e-8)

@@ -1,0 +1,366 @@
+"""Pin the CPU oracle (oracle/) before anything is compared with it.
+
+Checks: the reference's own known answers (tests/golden/legacy_kats.json,
+extracted from src/legacy.rs), library routines the algorithms reduce to
+(math.hypot, LAPACK eigh_tridiagonal and SVD, scipy.sparse products),
+invariants the reference tests assert (masked == physical subset, format
+independence, power iterations improve the fit), and planted-defect checks
+showing that each test would catch the reference defect it guards against.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+import oracle
+from helpers import golden, max_principal_angle, random_csr, triplet_residual
+
+EPS = np.finfo(np.float64).eps
+
+
+# ---------------------------------------------------------------- pythag (D1)
+
+def _pythag_reference_new(a, b):
+    """src/lanczos/mod.rs:361-380 as written (update r = (s/u)^2 drops r)."""
+    n = max(abs(a), abs(b))
+    if n <= 0:
+        return 0.0
+    p = n
+    r = (min(abs(a), abs(b)) / p) ** 2
+    t = 4.0 + r
+    for _ in range(200):
+        if abs(t - 4.0) < EPS:
+            break
+        s = r / t
+        u = 1.0 + 2.0 * s
+        p = p * u
+        r = (s / u) ** 2
+        t = 4.0 + r
+    return p
+
+
+def test_pythag_matches_hypot():
+    rng = np.random.default_rng(0)
+    vals = np.concatenate([rng.normal(size=500) * 10.0 ** rng.integers(-8, 8, 500), [0.0, 3.0, 4.0, 1e-300]])
+    for a, b in zip(vals, np.roll(vals, 7)):
+        h = math.hypot(a, b)
+        assert abs(oracle.pythag(a, b) - h) <= 4 * EPS * h
+
+
+def test_pythag_check_catches_d1():
+    # the same tolerance rejects the reference's current form
+    errs = [abs(_pythag_reference_new(a, b) - math.hypot(a, b)) / math.hypot(a, b)
+            for a, b in [(3.0, 4.0), (1.0, 1.0), (2.0, 7.0)]]
+    assert max(errs) > 1e-4
+
+
+# ------------------------------------------------------- tridiagonal QL solvers
+
+def _tridiag(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=n), rng.normal(size=n - 1)
+
+
+@pytest.mark.parametrize("n", [2, 3, 7, 20, 41])
+def test_imtqlb_eigenvalues_and_bounds(n):
+    d, off = _tridiag(n, n)
+    w, z = sla.eigh_tridiagonal(d, off)
+    ev, bnd = oracle.imtqlb(d, np.concatenate([[0.0], off]))
+    scale = np.abs(w).max()
+    assert np.max(np.abs(ev - w)) <= 1e-13 * scale
+    # bnd = first components of the eigenvectors (SVDLIBC's Ritz bound factor)
+    assert np.max(np.abs(np.abs(bnd) - np.abs(z[0, :]))) <= 1e-10
+
+
+def _imtqlb_stale_p(n, d, e, maxit=100):
+    """imtqlb with p = d[l], f = bnd[l] read once per l (mod.rs:455-457, D12)."""
+    d = list(d)
+    e = list(e)
+    bnd = [0.0] * n
+    sf = math.sqrt(n)
+    bnd[0] = 1.0
+    last = n - 1
+    for i in range(1, n):
+        e[i - 1] = e[i]
+    e[last] = 0.0
+    for l in range(n):
+        it = 0
+        p, f = d[l], bnd[l]
+        while it <= maxit:
+            m = l
+            while m < last:
+                test = abs(d[m]) + abs(d[m + 1])
+                if abs(e[m]) <= EPS * 100 * max(test, 1) * sf:
+                    break
+                m += 1
+            if m == l:
+                i = l
+                while i >= 1 and p < d[i - 1]:
+                    d[i] = d[i - 1]
+                    bnd[i] = bnd[i - 1]
+                    i -= 1
+                d[i] = p
+                bnd[i] = f
+                break
+            if it == maxit:
+                e[l] = 0.0
+                break
+            it += 1
+            g = (d[l + 1] - p) / (2 * e[l])
+            r = math.hypot(g, 1.0)
+            g = d[m] - p + e[l] / (g + (r if g >= 0 else -r))
+            s = c = 1.0
+            p = 0.0
+            i = m - 1
+            while True:
+                f = s * e[i]
+                b = c * e[i]
+                r = math.hypot(f, g)
+                e[i + 1] = r
+                s, c = f / r, g / r
+                g = d[i + 1] - p
+                r = (d[i] - g) * s + 2 * c * b
+                p = s * r
+                d[i + 1] = g + p
+                g = c * r - b
+                f = bnd[i + 1]
+                bnd[i + 1] = s * bnd[i] + c * f
+                bnd[i] = c * bnd[i] - s * f
+                if i == l:
+                    break
+                i -= 1
+            d[l] -= p
+            e[l] = g
+            e[m] = 0.0
+    return np.array(d)
+
+
+def test_imtqlb_check_catches_d12():
+    d, off = _tridiag(12, 3)
+    w = sla.eigh_tridiagonal(d, off, eigvals_only=True)
+    bad = np.sort(_imtqlb_stale_p(12, d, np.concatenate([[0.0], off])))
+    assert np.max(np.abs(bad - w)) > 1e-6 * np.abs(w).max()
+
+
+@pytest.mark.parametrize("n", [2, 5, 16, 60])
+def test_imtql2_eigenpairs(n):
+    d, off = _tridiag(n, 100 + n)
+    w, z = sla.eigh_tridiagonal(d, off)
+    ev, zz = oracle.imtql2(d, np.concatenate([[0.0], off]))
+    assert np.max(np.abs(ev - w)) <= 1e-13 * np.abs(w).max()
+    cos = np.abs(np.sum(zz * z, axis=0))
+    assert np.min(cos) >= 1 - 1e-10
+
+
+# ----------------------------------------------------------- SMat operators
+
+@pytest.mark.parametrize("shape,density", [((50, 30), 0.2), ((200, 700), 0.03), ((1, 5), 0.6), ((31, 1), 0.5)])
+@pytest.mark.parametrize("nthreads", [1, 3])
+def test_opa_matches_scipy(shape, density, nthreads):
+    a = random_csr(*shape, density, seed=shape[0])
+    op = oracle.OracleOp(a, nthreads=nthreads)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=shape[1])
+    y = rng.normal(size=shape[0])
+    assert np.allclose(op.opa(x), a @ x, rtol=1e-13, atol=1e-13)
+    assert np.allclose(op.opa(y, transposed=True), a.T @ y, rtol=1e-13, atol=1e-13)
+    assert op.nnz() == a.nnz
+
+
+def test_masked_ops_equal_physical_subset():
+    a = random_csr(120, 90, 0.1, seed=4)
+    cols = np.array([1, 3, 5, 7, 40, 41, 89])
+    op = oracle.OracleOp(a, columns=cols, nthreads=2)
+    sub = a[:, cols]
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=cols.size)
+    y = rng.normal(size=120)
+    assert op.ncols == cols.size and op.nnz() == sub.nnz
+    assert np.allclose(op.opa(x), sub @ x, rtol=1e-13, atol=1e-13)
+    assert np.allclose(op.opa(y, True), sub.T @ y, rtol=1e-13, atol=1e-13)
+    X = rng.normal(size=(cols.size, 6))
+    Y = rng.normal(size=(120, 6))
+    assert np.allclose(op.multiply_with_dense(X), sub @ X, rtol=1e-12, atol=1e-12)
+    assert np.allclose(op.multiply_with_dense(Y, True), sub.T @ Y, rtol=1e-12, atol=1e-12)
+    assert np.allclose(op.multiply_transposed_by_dense(Y), Y.T @ sub, rtol=1e-12, atol=1e-12)
+    assert np.allclose(op.column_means(), np.asarray(sub.mean(axis=0)).ravel(), rtol=1e-13, atol=1e-13)
+
+
+def test_transposed_panel_reads_column_major_rows():
+    # D5: the transposed product must read row i of a column-major panel as
+    # dense[(i, c)]; a row-major read of the same buffer gives another answer.
+    a = random_csr(40, 25, 0.3, seed=9)
+    op = oracle.OracleOp(a)
+    Y = np.random.default_rng(3).normal(size=(40, 5))
+    good = op.multiply_with_dense(Y, True)
+    assert np.allclose(good, a.T @ Y, rtol=1e-12)
+    scrambled = np.asfortranarray(Y).ravel(order="F").reshape(40, 5)  # row-major view of column-major data
+    assert not np.allclose(a.T @ scrambled, good)
+
+
+# ------------------------------------------------------------------ LAS2 KATs
+
+def test_las2_legacy_3x3_known_answers():
+    g = golden("legacy_kats.json")
+    a = sp.csc_matrix(np.array(g["matrix_3x3"]))
+    st, r = oracle.las2(oracle.OracleOp(a), 0, 0, random_seed=3141)
+    assert st == 0
+    dump = g["svdlibrs_3x3_seed3141"]
+    assert r["d"] == 3
+    assert np.allclose(r["s"], g["asserted_s_3x3"]["s"], rtol=0, atol=1e-12)
+    assert np.allclose(r["s"], dump["s"], rtol=0, atol=1e-12)
+    # vectors equal the documented ones up to a common sign per triplet
+    for i in range(3):
+        for got, want in ((r["u"][i], np.array(dump["ut"][i])), (r["vt"][i], np.array(dump["vt"][i]))):
+            sgn = np.sign(got @ want)
+            assert np.allclose(sgn * got, want, atol=1e-11)
+    rec = r["u"].T @ np.diag(r["s"]) @ r["vt"]
+    assert np.allclose(rec, g["matrix_3x3"], atol=1e-12)
+    dg = r["diagnostics"]
+    for key in ("non_zero", "dimensions", "iterations", "lanczos_steps", "ritz_values_stabilized",
+                "significant_values", "singular_values", "random_seed"):
+        assert dg[key] == dump["diagnostics"][key], key
+    assert bool(dg["transposed"]) == dump["diagnostics"]["transposed"]
+    assert dg["kappa"] == pytest.approx(dump["diagnostics"]["kappa"])
+
+
+def test_las2_basic_2x2_and_identity():
+    g = golden("legacy_kats.json")
+    st, r = oracle.las2(oracle.OracleOp(sp.csr_matrix(np.array(g["basic_2x2"]["matrix"]))), 0, 0, random_seed=1)
+    assert st == 0 and r["d"] == 2
+    assert np.allclose(r["s"], g["basic_2x2"]["s"], atol=1e-12)
+    assert np.allclose(r["u"].T @ np.diag(r["s"]) @ r["vt"], g["basic_2x2"]["matrix"], atol=1e-12)
+    st, r = oracle.las2(oracle.OracleOp(sp.identity(3, format="csc")), 0, 0, random_seed=7)
+    assert st == 0 and r["d"] == g["identity_3x3"]["d"]
+    assert abs(r["s"][0] - 1.0) < 1e-12
+
+
+def test_las2_format_independence():
+    # legacy.rs:2158-2165: COO, CSC and CSR inputs give identical results
+    coo = sp.coo_matrix(([3.0, 4.0], ([1, 2], [0, 1])), shape=(4, 4))
+    outs = [oracle.las2(oracle.OracleOp(f), 3, 0, random_seed=12345) for f in (coo, coo.tocsc(), coo.tocsr())]
+    for st, r in outs:
+        assert st == 0
+    for st, r in outs[1:]:
+        assert np.array_equal(r["s"], outs[0][1]["s"])
+        assert np.array_equal(r["vt"], outs[0][1]["vt"])
+
+
+@pytest.mark.parametrize("shape,density,k", [((60, 40), 0.2, 8), ((40, 90), 0.15, 6), ((300, 200), 0.05, 12)])
+def test_las2_matches_lapack(shape, density, k):
+    a = random_csr(*shape, density, seed=shape[1])
+    st, r = oracle.las2(oracle.OracleOp(a), k, 0, random_seed=42)
+    assert st == 0 and r["d"] >= 2
+    s_true = np.linalg.svd(a.toarray(), compute_uv=False)
+    d = r["d"]
+    assert np.all(np.diff(r["s"]) <= 0)
+    assert np.allclose(r["s"], s_true[:d], rtol=1e-10, atol=0)
+    assert triplet_residual(a, r["u"].T, r["s"], r["vt"]) < 1e-8
+    assert r["diagnostics"]["transposed"] == int(shape[1] >= 1.2 * shape[0])
+
+
+def test_las2_masked_equals_physical_subset():
+    # masked.rs:914-1000
+    a = random_csr(14, 10, 0.3, seed=42, low=0.1, high=10.0)
+    cols = [1, 3, 5, 7]
+    st1, r1 = oracle.las2(oracle.OracleOp(a, columns=cols), 0, 0, random_seed=5)
+    st2, r2 = oracle.las2(oracle.OracleOp(a[:, cols]), 0, 0, random_seed=5)
+    assert st1 == 0 and st2 == 0 and r1["d"] == r2["d"]
+    assert np.allclose(np.sort(r1["s"]), np.sort(r2["s"]), atol=1e-10)
+
+
+def test_las2_all_true_mask_equals_csr():
+    # lib.rs:111-154
+    a = random_csr(50, 30, 0.2, seed=12345)
+    st1, r1 = oracle.las2(oracle.OracleOp(a), 0, 0, random_seed=12345)
+    st2, r2 = oracle.las2(oracle.OracleOp(a, mask=np.ones(30, bool)), 0, 0, random_seed=12345)
+    assert st1 == 0 and st2 == 0 and r1["d"] == r2["d"]
+    assert np.allclose(r1["s"], r2["s"], rtol=1e-3)
+
+
+def test_las2_very_sparse_runs():
+    # lib.rs:156-164: 100 x 100 at 0.98% density, dims 50
+    a = random_csr(100, 100, 0.0098, seed=42)
+    st, r = oracle.las2(oracle.OracleOp(a), 50, 0, random_seed=42)
+    assert st == 0
+    s_true = np.linalg.svd(a.toarray(), compute_uv=False)
+    assert np.allclose(r["s"], s_true[: r["d"]], rtol=1e-8)
+
+
+def test_las2_error_paths():
+    st, r = oracle.las2(oracle.OracleOp(sp.csr_matrix(np.ones((1, 5)))), 0, 0, random_seed=1)
+    assert st == 5  # Las2Error: insufficient dimensions
+    st, r = oracle.las2(oracle.OracleOp(sp.csr_matrix((6, 4))), 0, 0, random_seed=1)
+    assert st == 2  # StartvError: A = 0
+
+
+def test_las2_f32_epsilon_mode():
+    a = random_csr(80, 60, 0.1, seed=3).astype(np.float32).astype(np.float64)
+    st, r = oracle.las2(oracle.OracleOp(a), 6, 0, random_seed=9, eps=oracle.F32_EPS)
+    assert st == 0
+    s_true = np.linalg.svd(a.toarray(), compute_uv=False)
+    assert np.allclose(r["s"], s_true[: r["d"]], rtol=1e-5)
+
+
+def test_start_vector_closed_form():
+    v = oracle.start_vector(42, 5)
+    phi = 0x9E3779B97F4A7C15
+    mask = (1 << 64) - 1
+    key = ((42 * phi) & mask) ^ 0x5356444C49423031
+    for i in range(5):
+        z = (key + (i + 1) * phi) & mask
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        z ^= z >> 31
+        assert v[i] == 2.0 * ((z >> 11) * 2.0 ** -53) - 1.0
+
+
+# ------------------------------------------------------------------ randomized
+
+def test_rsvd_oracle_exact_on_low_rank():
+    rng = np.random.default_rng(0)
+    a = sp.csr_matrix(rng.normal(size=(80, 6)) @ rng.normal(size=(6, 50)))
+    op = oracle.OracleOp(a)
+    om = rng.normal(size=(50, 10))
+    u, s, vt = oracle.randomized_svd(op, 6, 4, 0, "QR", omega=om)
+    s_true = np.linalg.svd(a.toarray(), compute_uv=False)[:6]
+    assert np.allclose(s, s_true, rtol=1e-10)
+    assert triplet_residual(a, u, s, vt) < 1e-9
+
+
+def test_rsvd_oracle_power_iterations_improve_fit():
+    # randomized/mod.rs:692-777
+    rng = np.random.default_rng(987)
+    dense = rng.uniform(-1, 1, (100, 10)) @ rng.uniform(-1, 1, (10, 50)) + rng.uniform(-0.01, 0.01, (100, 50))
+    op = oracle.OracleOp(sp.csr_matrix(dense))
+    om = rng.normal(size=(50, 15))
+    errs = []
+    for q in (0, 1, 3, 5):
+        u, s, vt = oracle.randomized_svd(op, 10, 5, q, "QR", omega=om)
+        errs.append(np.linalg.norm(dense - u @ np.diag(s) @ vt) / np.linalg.norm(dense))
+    assert min(errs[1:]) < 0.9 * errs[0] or errs[0] < 1e-3
+
+
+@pytest.mark.parametrize("norm", ["QR", "LU", "NONE"])
+def test_rsvd_oracle_normalizers_span_same_space(norm):
+    a = random_csr(300, 120, 0.05, seed=11)
+    op = oracle.OracleOp(a)
+    om = np.random.default_rng(5).normal(size=(120, 15))
+    u0, s0, vt0 = oracle.randomized_svd(op, 10, 5, 2, "QR", omega=om)
+    u1, s1, vt1 = oracle.randomized_svd(op, 10, 5, 2, norm, omega=om)
+    assert np.allclose(s0, s1, rtol=1e-8)
+
+
+def test_rsvd_oracle_mean_center_matches_dense():
+    # randomized/mod.rs:585-649 shape: rank-3 factors plus column means, so the
+    # centred matrix has rank <= 3 and l = 6 recovers it exactly
+    rng = np.random.default_rng(123)
+    dense = rng.uniform(-1, 1, (30, 3)) @ rng.uniform(-1, 1, (3, 10)) + np.arange(10) * 2.0
+    op = oracle.OracleOp(sp.csr_matrix(dense))
+    om = rng.normal(size=(10, 6))
+    u, s, vt = oracle.randomized_svd(op, 3, 3, 2, "QR", mean_center=True, omega=om)
+    centered = dense - dense.mean(axis=0)
+    s_true = np.linalg.svd(centered, compute_uv=False)[:3]
+    assert np.allclose(s, s_true, rtol=1e-6)
