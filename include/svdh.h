@@ -108,9 +108,10 @@ int32_t svdh_ctx_destroy(svdh_ctx *ctx);
 int32_t svdh_ctx_synchronize(svdh_ctx *ctx);
 const char *svdh_last_error(void);
 /* Kernel timing with HIP events on the context's stream. Kernel classes:
- * 0 = SpMV, 1 = SpMM, 2 = dense (Gram / apply / GEMM), 3 = BLAS-1,
- * 4 = basis GEMV (reorthogonalisation). total_ms and launches accumulate
- * until svdh_ctx_profile is called again with enable = 1. */
+ * 0 = SpMV, 1 = SpMM Y = A X, 2 = dense (Gram / apply / GEMM), 3 = BLAS-1,
+ * 4 = basis GEMV (reorthogonalisation), 5 = SpMM Z = A^T Y. total_ms,
+ * launches and the launches' algorithmic bytes accumulate until
+ * svdh_ctx_profile is called again with enable = 1. */
 int32_t svdh_ctx_profile(svdh_ctx *ctx, int32_t enable);
 int32_t svdh_ctx_kernel_time(svdh_ctx *ctx, int32_t kernel_class, double *total_ms,
                              int64_t *launches, double *algorithmic_bytes);

@@ -51,7 +51,7 @@ struct DevBuf {
 };
 
 // Kernel classes for event timing (svdh_ctx_kernel_time).
-enum KernelClass { KC_SPMV = 0, KC_SPMM = 1, KC_DENSE = 2, KC_BLAS1 = 3, KC_GEMV = 4, KC_COUNT = 5 };
+enum KernelClass { KC_SPMV = 0, KC_SPMM = 1, KC_DENSE = 2, KC_BLAS1 = 3, KC_GEMV = 4, KC_SPMM_T = 5, KC_COUNT = 6 };
 
 struct Ctx {
   int device = 0;
@@ -111,8 +111,13 @@ struct Schedule {
   int32_t total_blocks = 0;
   DevBuf rows;      // int32 sorted output rows
   DevBuf chunks;    // int32 x 5 per chunk: row, begin, end, first_slot, nchunks_of_row
-  DevBuf counters;  // int32 per long row
+  DevBuf counters;  // int32 x 3 per row with several chunks: row, first chunk, chunk count
   DevBuf partials;  // chunk partial results (grown on demand)
+  // SpMM segment schedule: per source block offsets/counts into `chunks`.
+  int nblk = 1;
+  int32_t blk_off[8] = {0};
+  int32_t blk_cnt[8] = {0};
+  int32_t nslots = 0;
 };
 
 // One direction of an operator as a CSR: rows of this matrix are output rows.
@@ -141,6 +146,7 @@ struct svdh_op {
   svdh::DevBuf rowptr, colidx, vals[3];
   svdh::DevBuf t_rowptr, t_colidx, t_vals[3];  // the transpose (CSC of A)
   std::vector<int32_t> h_rowptr, h_t_rowptr;
+  std::vector<int32_t> h_colidx, h_t_colidx;  // host indices, for SpMM segment schedules
   // Masked view storage.
   svdh::DevBuf o2m, m2o;
   std::vector<int32_t> h_m2o;
@@ -187,10 +193,11 @@ void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, dou
 // Y (m x lo) <- Y (m x li) * M (li x lo, FP64 row-major, host), in place or to Z.
 void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, int ydt,
                  const double *M_host, void *Z, int64_t ldz, int zdt);
-// nr2[c] = |Y1 M1[:, c] + Y2 M2[:, c]|^2 for c < k (FP64); Y1, Y2 are m x w
-// row-major panels with leading dimension ld, M1/M2 are w x k host matrices.
-void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t k, const void *Y1, const void *Y2, int64_t ld, int dt,
-                    const double *M1, const double *M2, double *nr2_host);
+// nr2[c] = |Y1 M1[:, c] + Y2 M2[:, c]|^2 for c < k (FP64); Y1 (m x w) and
+// Y2 (m x w2, w2 <= w) are row-major panels with leading dimension ld,
+// M1 (w x k) and M2 (w2 x k) are host matrices.
+void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const void *Y1, const void *Y2, int64_t ld,
+                    int dt, const double *M1, const double *M2, double *nr2_host);
 // Column L2 normalisation, columns with norm <= 1e-10 left as they are.
 void panel_normalize_columns(Ctx *c, int64_t m, int64_t l, void *Y, int64_t ldy, int ydt);
 // Standard normal panel (m x l row-major) from a counter-based generator.

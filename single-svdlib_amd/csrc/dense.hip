@@ -198,133 +198,195 @@ template <> __device__ __forceinline__ void std_<bf16_t>(bf16_t *p, int64_t i, d
   p[i] = (bf16_t)(u >> 16);
 }
 
-// Gram partials: each workgroup accumulates Y[rows]^T Y[rows] over a strided
-// set of 64-row slabs into part[blockIdx.x * L * L] (FP64). Threads own 4x4
-// tiles of the (padded) L x L result.
-constexpr int kGramSlab = 64;
+// Tall-skinny dense kernels work on 64-row tiles of a row-major panel held in
+// LDS; each of the 256 threads owns a 4 x 4 block of a 64 x 64 output tile.
+constexpr int kTile = 64;
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gram_part(int64_t m, int l, const T *Y, int64_t ldy, double *part) {
-  extern __shared__ double slab[];  // kGramSlab x lp
-  const int lp = (l + 3) & ~3;
-  const int nt = lp / 4;  // tiles per dimension
-  const int ntiles = nt * nt;
-  double acc[4][16];
-  int tiles[4];
-  int ntl = 0;
-  for (int t = threadIdx.x; t < ntiles && ntl < 4; t += kBlock) tiles[ntl++] = t;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 16; ++b) acc[a][b] = 0.0;
-  for (int64_t r0 = (int64_t)blockIdx.x * kGramSlab; r0 < m; r0 += (int64_t)gridDim.x * kGramSlab) {
-    const int rows = (int)min<int64_t>(kGramSlab, m - r0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < kGramSlab * lp; e += kBlock) {
-      const int rr = e / lp, cc = e % lp;
-      slab[e] = (rr < rows && cc < l) ? ldd<T>(Y, (r0 + rr) * ldy + cc) : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      if (a >= ntl) continue;
-      const int ti = tiles[a] / nt, tj = tiles[a] % nt;
-      for (int rr = 0; rr < rows; ++rr) {
-        const double *row = slab + rr * lp;
-        double xi[4], xj[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          xi[q] = row[ti * 4 + q];
-          xj[q] = row[tj * 4 + q];
-        }
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[a][p * 4 + q] += xi[p] * xj[q];
-      }
-    }
+template <typename T, typename C>
+__device__ __forceinline__ void load_tile(const T *Y, int64_t ldy, int64_t r0, int rows, int w, C *s, int sld) {
+  // s[k * sld + r] = Y[r0 + r, k] (transposed so 4 consecutive rows are one 16-byte read)
+  for (int e = threadIdx.x; e < kTile * w; e += kBlock) {
+    const int r = e / w, k = e % w;
+    s[k * sld + r] = r < rows ? (C)ldd<T>(Y, (r0 + r) * ldy + k) : C(0);
   }
-  double *out = part + (int64_t)blockIdx.x * l * l;
+}
+
+// Gram partials (FP64 accumulation): workgroup b accumulates Y^T Y over row
+// tiles b, b + grid, ... into part[b * l * l], one 64 x 64 output block
+// (c0, d0) at a time (d0 >= c0, mirrored). Tiles are row-major in LDS in the
+// storage type S (FP32 for FP32/BF16 panels: products of two FP32 values are
+// exact in FP64), so both operands of a thread are read along one row.
+template <typename T, typename S>
+__global__ __launch_bounds__(kBlock) void k_gram_part(int64_t m, int l, const T *Y, int64_t ldy, double *part) {
+  __shared__ S sa[kTile * kTile];
+  __shared__ S sb[kTile * kTile];
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  for (int c0 = 0; c0 < l; c0 += kTile)
+    for (int d0 = c0; d0 < l; d0 += kTile) {
+      const int wc = min(kTile, l - c0), wd = min(kTile, l - d0);
+      const bool same = c0 == d0;
+      const S *sbp = same ? sa : sb;
+      double acc[4][4] = {};
+      for (int64_t r0 = (int64_t)blockIdx.x * kTile; r0 < m; r0 += (int64_t)gridDim.x * kTile) {
+        const int rows = (int)min<int64_t>(kTile, m - r0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < kTile * kTile; e += kBlock) {
+          const int r = e / kTile, k = e % kTile;
+          const bool ok = r < rows;
+          sa[e] = (ok && k < wc) ? (S)ldd<T>(Y, (r0 + r) * ldy + c0 + k) : S(0);
+          if (!same) sb[e] = (ok && k < wd) ? (S)ldd<T>(Y, (r0 + r) * ldy + d0 + k) : S(0);
+        }
+        __syncthreads();
+        for (int r = 0; r < rows; ++r) {
+          double a4[4], b4[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    if (a < ntl) {
-      const int ti = tiles[a] / nt, tj = tiles[a] % nt;
+          for (int q = 0; q < 4; ++q) {
+            a4[q] = (double)sa[r * kTile + tr * 4 + q];
+            b4[q] = (double)sbp[r * kTile + tc * 4 + q];
+          }
+#pragma unroll
+          for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[p][q] += a4[p] * b4[q];
+        }
+      }
+      double *out = part + (int64_t)blockIdx.x * l * l;
 #pragma unroll
       for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int i = ti * 4 + p, j = tj * 4 + q;
-          if (i < l && j < l) out[i * l + j] = acc[a][p * 4 + q];
+          const int i = c0 + tr * 4 + p, j = d0 + tc * 4 + q;
+          if (i < l && j < l) {
+            out[(int64_t)i * l + j] = acc[p][q];
+            out[(int64_t)j * l + i] = acc[p][q];
+          }
+        }
+    }
+}
+
+// G[e] = sum over partials in order; 64 outputs per workgroup, 4 lanes per
+// output each summing every fourth partial, then the 4 lanes in order.
+__global__ __launch_bounds__(kBlock) void k_gram_reduce(int nb, int64_t ll, const double *part, double *G) {
+  __shared__ double red[4][64];
+  const int o = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t e = blockIdx.x * (int64_t)64 + o;
+  double s = 0.0;
+  if (e < ll)
+    for (int b = q; b < nb; b += 4) s += part[(int64_t)b * ll + e];
+  red[q][o] = s;
+  __syncthreads();
+  if (q == 0 && e < ll) G[e] = ((red[0][o] + red[1][o]) + red[2][o]) + red[3][o];
+}
+
+// Z[r, 0:lo] = Y[r, 0:li] M (li x lo, row-major) on 64-row tiles; the
+// tile is read into LDS before any output is written, so Z may alias Y.
+// C is the arithmetic type (FP32 for FP32 panels, FP64 otherwise). M is
+// staged in LDS too when MLDS is set (li * lo entries of C).
+template <typename TY, typename TZ, typename C, bool MLDS>
+__global__ __launch_bounds__(kBlock) void k_apply(int64_t m, int li, int lo, const TY *Y, int64_t ldy, const double *M,
+                                                 TZ *Z, int64_t ldz) {
+  extern __shared__ char smem[];
+  C *s = reinterpret_cast<C *>(smem);  // li x (kTile + 1)
+  C *sm = s + (size_t)li * (kTile + 1);  // li x lo (when MLDS)
+  const int sld = kTile + 1;
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  const int64_t r0 = (int64_t)blockIdx.x * kTile;
+  const int rows = (int)min<int64_t>(kTile, m - r0);
+  load_tile<TY, C>(Y, ldy, r0, rows, li, s, sld);
+  if (MLDS)
+    for (int e = threadIdx.x; e < li * lo; e += kBlock) sm[e] = (C)M[e];
+  __syncthreads();
+  for (int c0 = 0; c0 < lo; c0 += kTile) {
+    C acc[4][4] = {};
+    for (int k = 0; k < li; ++k) {
+      C a4[4], b4[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a4[q] = s[k * sld + tr * 4 + q];
+        const int c = c0 + tc * 4 + q;
+        b4[q] = c < lo ? (MLDS ? sm[k * lo + c] : (C)M[(int64_t)k * lo + c]) : C(0);
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[p][q] += a4[p] * b4[q];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const int r = tr * 4 + p;
+      if (r < rows)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int c = c0 + tc * 4 + q;
+          if (c < lo) std_<TZ>(Z, (r0 + r) * ldz + c, (double)acc[p][q]);
         }
     }
   }
 }
 
-__global__ void k_gram_reduce(int nb, int64_t ll, const double *part, double *G) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= ll) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(int64_t)b * ll + e];
-  G[e] = s;
-}
-
-// Z[i, 0:lo] = Y[i, 0:li] M (li x lo row-major FP64); one wavefront per row,
-// row held in registers so Z may alias Y.
-template <typename TY, typename TZ>
-__global__ __launch_bounds__(kBlock) void k_apply(int64_t m, int li, int lo, const TY *Y, int64_t ldy, const double *M,
-                                                 TZ *Z, int64_t ldz) {
-  const int64_t row = blockIdx.x * (int64_t)(kBlock / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= m) return;
-  double y0 = lane < li ? ldd<TY>(Y, row * ldy + lane) : 0.0;
-  double y1 = lane + 64 < li ? ldd<TY>(Y, row * ldy + lane + 64) : 0.0;
-  double y2 = lane + 128 < li ? ldd<TY>(Y, row * ldy + lane + 128) : 0.0;
-  double y3 = lane + 192 < li ? ldd<TY>(Y, row * ldy + lane + 192) : 0.0;
-  for (int c0 = 0; c0 < lo; c0 += 64) {
-    const int c = c0 + lane;
-    double acc = 0.0;
-    for (int j = 0; j < li; ++j) {
-      const double src = j < 64 ? y0 : j < 128 ? y1 : j < 192 ? y2 : y3;
-      const double yj = __shfl(src, j & 63, 64);
-      if (c < lo) acc += yj * M[(int64_t)j * lo + c];
+// Residual column norms: r = Y1 M1 + Y2 M2 on 64-row tiles (arithmetic in
+// C), squared and summed over the tile's rows in FP64; per-workgroup column
+// partials (workgroups stride over tiles). M1, M2 (w x k) are staged in LDS
+// when MLDS is set.
+template <typename T, typename C, bool MLDS>
+__global__ __launch_bounds__(kBlock) void k_resid(int64_t m, int w, int w2, int k, const T *Y1, const T *Y2,
+                                                 int64_t ld, const double *M1, const double *M2, double *part) {
+  // Y1 has w columns, Y2 has w2 <= w; M1 is w x k, M2 is w2 x k (zero rows past w2 are implied)
+  extern __shared__ char smem[];
+  C *s1 = reinterpret_cast<C *>(smem);
+  const int sld = kTile + 1;
+  C *s2 = s1 + (size_t)w * sld;
+  C *sm1 = s2 + (size_t)w * sld;
+  C *sm2 = sm1 + (size_t)w * k;
+  __shared__ double red[16][kTile + 4];
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  if (MLDS)
+    for (int e = threadIdx.x; e < w * k; e += kBlock) {
+      sm1[e] = (C)M1[e];
+      sm2[e] = e < w2 * k ? (C)M2[e] : C(0);
     }
-    if (c < lo) std_<TZ>(Z, row * ldz + c, acc);
-  }
-}
-
-// Residual column norms: one wavefront per row (rows strided over the grid),
-// lanes over output columns; per-workgroup column partials in FP64.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_resid(int64_t m, int w, int k, const T *Y1, const T *Y2, int64_t ld,
-                                                 const double *M1, const double *M2, double *part) {
-  __shared__ double red[kBlock / 64][128];
-  const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  double sq0 = 0.0, sq1 = 0.0;
-  const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
-  for (int64_t row = blockIdx.x * (int64_t)(kBlock / 64) + wv; row < m; row += nw) {
-    const double a0 = lane < w ? ldd<T>(Y1, row * ld + lane) : 0.0;
-    const double a1 = lane + 64 < w ? ldd<T>(Y1, row * ld + lane + 64) : 0.0;
-    const double b0 = lane < w ? ldd<T>(Y2, row * ld + lane) : 0.0;
-    const double b1 = lane + 64 < w ? ldd<T>(Y2, row * ld + lane + 64) : 0.0;
-    double r0 = 0.0, r1 = 0.0;
-    for (int j = 0; j < w; ++j) {
-      const double y1 = __shfl(j < 64 ? a0 : a1, j & 63, 64);
-      const double y2 = __shfl(j < 64 ? b0 : b1, j & 63, 64);
-      if (lane < k) r0 += y1 * M1[(int64_t)j * k + lane] + y2 * M2[(int64_t)j * k + lane];
-      if (lane + 64 < k) r1 += y1 * M1[(int64_t)j * k + lane + 64] + y2 * M2[(int64_t)j * k + lane + 64];
-    }
-    sq0 += r0 * r0;
-    sq1 += r1 * r1;
-  }
-  red[wv][lane] = sq0;
-  red[wv][lane + 64] = sq1;
-  __syncthreads();
-  if (threadIdx.x < 128 && threadIdx.x < k) {
-    double t = 0.0;
+  double colsq[2][4] = {};
+  for (int64_t r0 = (int64_t)blockIdx.x * kTile; r0 < m; r0 += (int64_t)gridDim.x * kTile) {
+    const int rows = (int)min<int64_t>(kTile, m - r0);
+    __syncthreads();
+    load_tile<T, C>(Y1, ld, r0, rows, w, s1, sld);
+    load_tile<T, C>(Y2, ld, r0, rows, w2, s2, sld);
+    __syncthreads();
+    for (int c0 = 0, pass = 0; c0 < k; c0 += kTile, ++pass) {
+      C acc[4][4] = {};
+      for (int j = 0; j < w; ++j) {
+        C a4[4], b4[4], m1[4], m2[4];
 #pragma unroll
-    for (int i = 0; i < kBlock / 64; ++i) t += red[i][threadIdx.x];
-    part[(int64_t)blockIdx.x * k + threadIdx.x] = t;
+        for (int q = 0; q < 4; ++q) {
+          a4[q] = s1[j * sld + tr * 4 + q];
+          b4[q] = j < w2 ? s2[j * sld + tr * 4 + q] : C(0);
+          const int c = c0 + tc * 4 + q;
+          m1[q] = c < k ? (MLDS ? sm1[j * k + c] : (C)M1[(int64_t)j * k + c]) : C(0);
+          m2[q] = (c < k && j < w2) ? (MLDS ? sm2[j * k + c] : (C)M2[(int64_t)j * k + c]) : C(0);
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[p][q] += a4[p] * m1[q] + b4[p] * m2[q];
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        if (tr * 4 + p < rows)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) colsq[pass & 1][q] += (double)acc[p][q] * (double)acc[p][q];
+    }
+  }
+  for (int pass = 0; pass * kTile < k && pass < 2; ++pass) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[tr][tc * 4 + q] = colsq[pass][q];
+    __syncthreads();
+    if (threadIdx.x < kTile) {
+      double t = 0.0;
+      for (int i = 0; i < 16; ++i) t += red[i][threadIdx.x];
+      const int c = pass * kTile + threadIdx.x;
+      if (c < k) part[(int64_t)blockIdx.x * k + c] = t;
+    }
   }
 }
 
@@ -464,83 +526,102 @@ void basis_combine(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, c
 
 void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, double *G_host) {
   if (l > 128) fail(SVDH_ERR_ARG, "panel width %lld exceeds the Gram kernel limit 128", (long long)l);
-  const int nb = grid_of(m, kGramSlab, 256);
+  const int nb = grid_of(m, kTile, 512);
   const int64_t ll = l * l;
   const size_t need = (size_t)nb * ll * sizeof(double) + ll * sizeof(double);
   c->red_part.ensure(std::max<size_t>(need, c->red_part.bytes));
   double *part = c->red_part.as<double>();
   double *G = part + (int64_t)nb * ll;
-  const int lp = (int)((l + 3) & ~3);
-  const size_t shm = (size_t)kGramSlab * lp * sizeof(double);
   {
     KTimer t(c, KC_DENSE, (double)m * l * dtype_size(ydt));
     if (ydt == SVDH_F64)
-      hipLaunchKernelGGL(k_gram_part<double>, nb, kBlock, shm, c->stream, m, (int)l, (const double *)Y, ldy, part);
+      hipLaunchKernelGGL((k_gram_part<double, double>), nb, kBlock, 0, c->stream, m, (int)l, (const double *)Y, ldy, part);
     else if (ydt == SVDH_F32)
-      hipLaunchKernelGGL(k_gram_part<float>, nb, kBlock, shm, c->stream, m, (int)l, (const float *)Y, ldy, part);
+      hipLaunchKernelGGL((k_gram_part<float, float>), nb, kBlock, 0, c->stream, m, (int)l, (const float *)Y, ldy, part);
     else
-      hipLaunchKernelGGL(k_gram_part<bf16_t>, nb, kBlock, shm, c->stream, m, (int)l, (const bf16_t *)Y, ldy, part);
+      hipLaunchKernelGGL((k_gram_part<bf16_t, float>), nb, kBlock, 0, c->stream, m, (int)l, (const bf16_t *)Y, ldy, part);
     HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_gram_reduce, grid_of(ll), kBlock, 0, c->stream, nb, ll, (const double *)part, G);
+    hipLaunchKernelGGL(k_gram_reduce, (int)((ll + 63) / 64), kBlock, 0, c->stream, nb, ll, (const double *)part, G);
     HIP_CHECK(hipGetLastError());
   }
   HIP_CHECK(hipMemcpyAsync(G_host, G, ll * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
-template <typename TY, typename TZ>
+template <typename TY, typename TZ, typename C>
 static void apply_t(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, const double *M, void *Z,
                     int64_t ldz) {
-  const int64_t blocks = (m + (kBlock / 64) - 1) / (kBlock / 64);
-  hipLaunchKernelGGL((k_apply<TY, TZ>), (int)blocks, kBlock, 0, c->stream, m, (int)li, (int)lo, (const TY *)Y, ldy, M,
-                     (TZ *)Z, ldz);
+  const int64_t blocks = (m + kTile - 1) / kTile;
+  const size_t shm = (size_t)li * (kTile + 1) * sizeof(C);
+  if (shm > 65536) fail(SVDH_ERR_ARG, "panel width %lld too wide for the apply kernel", (long long)li);
+  const size_t shm_m = shm + (size_t)li * lo * sizeof(C);
+  if (shm_m <= 65536)
+    hipLaunchKernelGGL((k_apply<TY, TZ, C, true>), (int)blocks, kBlock, shm_m, c->stream, m, (int)li, (int)lo,
+                       (const TY *)Y, ldy, M, (TZ *)Z, ldz);
+  else
+    hipLaunchKernelGGL((k_apply<TY, TZ, C, false>), (int)blocks, kBlock, shm, c->stream, m, (int)li, (int)lo,
+                       (const TY *)Y, ldy, M, (TZ *)Z, ldz);
   HIP_CHECK(hipGetLastError());
 }
 
 void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, int ydt,
                  const double *M_host, void *Z, int64_t ldz, int zdt) {
-  if (li > 256) fail(SVDH_ERR_ARG, "panel width %lld exceeds the apply kernel limit 256", (long long)li);
+  if (m == 0 || lo == 0) return;
   const size_t mb = (size_t)li * lo * sizeof(double);
   c->stage.ensure(mb);
   double *Md = c->stage.as<double>();
   HIP_CHECK(hipMemcpyAsync(Md, M_host, mb, hipMemcpyHostToDevice, c->stream));
   KTimer t(c, KC_DENSE, (double)m * (li * dtype_size(ydt) + lo * dtype_size(zdt)));
   if (ydt == SVDH_F64 && zdt == SVDH_F64)
-    apply_t<double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<double, double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
   else if (ydt == SVDH_F32 && zdt == SVDH_F32)
-    apply_t<float, float>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<float, float, float>(c, m, li, lo, Y, ldy, Md, Z, ldz);
   else if (ydt == SVDH_F32 && zdt == SVDH_F64)
-    apply_t<float, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<float, double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
   else if (ydt == SVDH_F64 && zdt == SVDH_F32)
-    apply_t<double, float>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<double, float, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
   else if (ydt == SVDH_F32 && zdt == SVDH_BF16)
-    apply_t<float, bf16_t>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<float, bf16_t, float>(c, m, li, lo, Y, ldy, Md, Z, ldz);
   else
     fail(SVDH_ERR_ARG, "unsupported apply dtypes %d -> %d", ydt, zdt);
   // The staged matrix must stay valid until the kernel has read it.
   HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
-void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t k, const void *Y1, const void *Y2, int64_t ld, int dt,
-                    const double *M1, const double *M2, double *nr2_host) {
-  if (w > 128 || k > 128) fail(SVDH_ERR_ARG, "residual panel wider than 128");
-  const int nb = grid_of(m, kBlock / 64 * 8, 1024);
+void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const void *Y1, const void *Y2, int64_t ld,
+                    int dt, const double *M1, const double *M2, double *nr2_host) {
+  if (w > 128 || k > 128 || w2 > w) fail(SVDH_ERR_ARG, "residual panel wider than 128");
+  const int nb = grid_of(m, kTile, 1024);
   const size_t mb = (size_t)w * k * sizeof(double);
   c->stage.ensure(2 * mb);
   double *M1d = c->stage.as<double>();
   double *M2d = M1d + w * k;
   HIP_CHECK(hipMemcpyAsync(M1d, M1, mb, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(M2d, M2, mb, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(M2d, M2, (size_t)w2 * k * sizeof(double), hipMemcpyHostToDevice, c->stream));
   c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * k * sizeof(double)));
   double *part = c->red_part.as<double>();
   {
     KTimer t(c, KC_DENSE, (double)m * 2 * w * dtype_size(dt));
-    if (dt == SVDH_F64)
-      hipLaunchKernelGGL(k_resid<double>, nb, kBlock, 0, c->stream, m, (int)w, (int)k, (const double *)Y1,
-                         (const double *)Y2, ld, M1d, M2d, part);
-    else
-      hipLaunchKernelGGL(k_resid<float>, nb, kBlock, 0, c->stream, m, (int)w, (int)k, (const float *)Y1,
-                         (const float *)Y2, ld, M1d, M2d, part);
+    if (dt == SVDH_F64) {
+      const size_t shm = 2 * (size_t)w * (kTile + 1) * sizeof(double);
+      if (shm > 65536) fail(SVDH_ERR_ARG, "FP64 residual panel too wide");
+      const size_t shm_m = shm + 2 * (size_t)w * k * sizeof(double);
+      if (shm_m <= 65536)
+        hipLaunchKernelGGL((k_resid<double, double, true>), nb, kBlock, shm_m, c->stream, m, (int)w, (int)w2, (int)k,
+                           (const double *)Y1, (const double *)Y2, ld, M1d, M2d, part);
+      else
+        hipLaunchKernelGGL((k_resid<double, double, false>), nb, kBlock, shm, c->stream, m, (int)w, (int)w2, (int)k,
+                           (const double *)Y1, (const double *)Y2, ld, M1d, M2d, part);
+    } else {
+      const size_t shm = 2 * (size_t)w * (kTile + 1) * sizeof(float);
+      const size_t shm_m = shm + 2 * (size_t)w * k * sizeof(float);
+      if (shm_m <= 65536)
+        hipLaunchKernelGGL((k_resid<float, float, true>), nb, kBlock, shm_m, c->stream, m, (int)w, (int)w2, (int)k,
+                           (const float *)Y1, (const float *)Y2, ld, M1d, M2d, part);
+      else
+        hipLaunchKernelGGL((k_resid<float, float, false>), nb, kBlock, shm, c->stream, m, (int)w, (int)w2, (int)k,
+                           (const float *)Y1, (const float *)Y2, ld, M1d, M2d, part);
+    }
     HIP_CHECK(hipGetLastError());
   }
   std::vector<double> h((size_t)nb * k);

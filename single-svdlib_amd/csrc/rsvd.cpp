@@ -1,28 +1,35 @@
 // Randomized SVD (Halko-style range finder with power iterations) on one
 // MI355X: src/randomized/mod.rs:24-237.
 //
-// Panels (Omega, Y, Z, Q) are row-major in HBM with a leading dimension
-// padded to 8 elements, so each SpMM row gather is a contiguous 16-byte
-// vector per lane. Per step:
-//   Y = A Omega, then n_power_iters times  Z = A^T Y, normalise Z,
-//   Y = A Z, normalise Y;  Q = qr(Y);  B^T = A^T Q;  SVD(B) from the Gram
-//   B B^T = (B^T)^T B^T (l x l, FP64) -> B = W S X^T with X = B^T W S^-1;
-//   U = Q W.
-// The QR normaliser is CholeskyQR2: FP64 Gram of the panel, host Cholesky,
-// Y R^-1 applied on the device, twice. If a Cholesky pivot is not positive
-// (rank deficient or too ill-conditioned panel) the step falls back to the
-// eigen-decomposition of the Gram (Y W L^-1/2), counted in qr_fallbacks.
-// The thin Q of nalgebra's Householder QR and this Q span the same space and
-// differ only in column signs/rotation, which the final SVD absorbs.
+// Panels (Omega, Y, Z) are row-major in HBM with a leading dimension padded
+// to 8 elements, so each SpMM row gather is a contiguous 16-byte vector per
+// lane.
 //
-// Adaptive mode (tol > 0): Z = A^T Q is B^T for the current Q, so its Gram
-// gives the Ritz values s_i and vectors w_i of the current subspace, and the
-// next power step's A Z gives every Ritz residual with one small product:
-//   A v_i - s_i u_i = (A Z) w_i / s_i - s_i Q w_i,
-// with u_i = Q w_i, v_i = Z w_i / s_i (and A^T u_i = s_i v_i by construction).
-// The residual panel is formed and normed in FP64 from the stored panels, so
-// it resolves residuals down to the panel precision, unlike the cancelling
-// Gram identity |r_i|^2 = w_i^T (A Z)^T (A Z) w_i / s_i^2 - s_i^2.
+// QR normaliser (the reference's default) as implicit CholeskyQR: Q = Y M is
+// never formed. M = R^-1 comes from the Cholesky factor of the FP64 Gram
+// G = Y^T Y, accumulated from the FP32 panel (products of two FP32 values are
+// exact in FP64), so Q is orthonormal to FP64 rounding without an FP32 pass
+// over Y. M is folded into the small n-side panel instead: Z = A^T Q =
+// (A^T Y) M. One power step is therefore two SpMMs, one Gram of the m x l
+// panel and one small apply on the n x l panel; the reference's extra QR of
+// Z does not change span(A Z) and is not needed. If the Cholesky factor does
+// not exist (rank-deficient or too ill-conditioned panel) M is taken from the
+// Gram's eigen-decomposition over its leading directions (qr_fallbacks).
+// The thin Q of nalgebra's Householder QR and this Q span the same space;
+// the final SVD absorbs the difference.
+//
+// After the last step Z = A^T Q = B^T. SVD(B) comes from the Gram
+// Z^T Z = B B^T = W S^2 W^T (FP64): U = Q W = Y (M W), V = Z W S^-1.
+//
+// Adaptive mode (tol > 0): the next power step's A Z gives the Ritz
+// residuals of the current subspace with one small product,
+//   A v_i - s_i u_i = (A Z) w_i / s_i - s_i Q w_i = (A Z) w_i / s_i - Y (M w_i s_i),
+// formed and normed in FP64 from the stored panels (A^T u_i = s_i v_i holds
+// by construction). Checks are scheduled from the observed contraction rate.
+//
+// LU and None normalisers (src/randomized/mod.rs:102-105, 126-127, 239-293)
+// keep the reference's explicit steps: column L2 normalisation or nothing
+// between the products, then an explicit CholeskyQR2 of Y before B.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -45,15 +52,28 @@ struct Panel {
     dt = dtype;
     ld = (width + 7) / 8 * 8;
     buf.alloc(std::max<size_t>((size_t)r * ld * dtype_size(dtype), 64));
+    HIP_CHECK(hipMemset(buf.p, 0, buf.bytes));
   }
   void *p() const { return buf.p; }
 };
+
+// Eigen-decomposition of a symmetric matrix sorted by descending eigenvalue.
+void desc_eig(int64_t w, const std::vector<double> &G, std::vector<double> &lam, std::vector<double> &W) {
+  std::vector<double> la(w), V(w * w);
+  sym_eig(w, G.data(), la.data(), V.data());
+  lam.assign(w, 0.0);
+  W.assign(w * w, 0.0);
+  for (int64_t r = 0; r < w; ++r) {
+    const int64_t k = w - 1 - r;
+    lam[r] = la[k];
+    for (int64_t i = 0; i < w; ++i) W[i * w + r] = V[i * w + k];
+  }
+}
 
 struct Rsvd {
   Ctx *c;
   svdh_op *op;
   int vdt;     // sparse values of the FP32/FP64 passes
-  int pdt;     // panel dtype
   bool bf16;   // BF16 ladder available
   int64_t fallbacks = 0;
   DevBuf bf_in;  // BF16 copy of an input panel
@@ -70,68 +90,40 @@ struct Rsvd {
     }
   }
 
-  // One Cholesky-QR pass from a given Gram; false if the Cholesky fails.
-  bool chol_pass(Panel &P, const std::vector<double> &G) {
+  // M (w x wo, row-major) with P M orthonormal, from the FP64 Gram of P.
+  void normaliser(const Panel &P, std::vector<double> &M, int64_t &wo) {
     const int64_t w = P.w;
-    std::vector<double> R(w * w), Ri(w * w);
-    if (!cholesky_upper(w, G.data(), R.data())) return false;
-    upper_inverse(w, R.data(), Ri.data());
-    panel_apply(c, P.rows, w, w, P.p(), P.ld, P.dt, Ri.data(), P.p(), P.ld, P.dt);
-    return true;
-  }
-
-  // Orthonormalise the columns of P (CholeskyQR2). G0 (optional) is P's Gram.
-  void orth(Panel &P, const std::vector<double> *G0 = nullptr) {
-    const int64_t w = P.w;
-    std::vector<double> G(w * w);
-    if (G0)
-      G = *G0;
-    else
-      gram(c, P.rows, w, P.p(), P.ld, P.dt, G.data());
+    std::vector<double> G(w * w), R(w * w);
+    gram(c, P.rows, w, P.p(), P.ld, P.dt, G.data());
     const int64_t target = std::min(P.rows, w);
-    if (target == w && chol_pass(P, G)) {
-      gram(c, P.rows, w, P.p(), P.ld, P.dt, G.data());
-      if (chol_pass(P, G)) return;
+    if (target == w && cholesky_upper(w, G.data(), R.data())) {
+      M.assign(w * w, 0.0);
+      upper_inverse(w, R.data(), M.data());
+      wo = w;
+      return;
     }
-    // Eigen fallback: keep the leading `target` directions.
     ++fallbacks;
-    std::vector<double> lam(w), V(w * w);
-    sym_eig(w, G.data(), lam.data(), V.data());
-    const double lmax = std::max(lam[w - 1], 0.0);
-    const double thr = lmax * (P.dt == SVDH_F64 ? 1e-24 : 1e-12);
-    std::vector<double> M(w * target, 0.0);
-    for (int64_t r = 0; r < target; ++r) {
-      const int64_t k = w - 1 - r;
-      if (lam[k] > thr && lam[k] > 0.0)
-        for (int64_t i = 0; i < w; ++i) M[i * target + r] = V[i * w + k] / std::sqrt(lam[k]);
-    }
-    panel_apply(c, P.rows, w, target, P.p(), P.ld, P.dt, M.data(), P.p(), P.ld, P.dt);
-    P.w = target;
-    gram(c, P.rows, P.w, P.p(), P.ld, P.dt, G.data());
-    G.resize(P.w * P.w);
-    chol_pass(P, G);
+    std::vector<double> lam, W;
+    desc_eig(w, G, lam, W);
+    const double thr = std::max(lam[0], 0.0) * (P.dt == SVDH_F64 ? 1e-24 : 1e-12);
+    M.assign(w * target, 0.0);
+    for (int64_t r = 0; r < target; ++r)
+      if (lam[r] > thr && lam[r] > 0.0)
+        for (int64_t i = 0; i < w; ++i) M[i * target + r] = W[i * w + r] / std::sqrt(lam[r]);
+    wo = target;
   }
 
-  void normalize(Panel &P, int normalizer) {
-    if (normalizer == SVDH_NORM_QR)
-      orth(P);
-    else if (normalizer == SVDH_NORM_LU)
-      panel_normalize_columns(c, P.rows, P.w, P.p(), P.ld, P.dt);
+  // Explicit orthonormalisation (CholeskyQR2 with the eigen fallback).
+  void orth(Panel &P) {
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<double> M;
+      int64_t wo;
+      normaliser(P, M, wo);
+      panel_apply(c, P.rows, P.w, wo, P.p(), P.ld, P.dt, M.data(), P.p(), P.ld, P.dt);
+      P.w = wo;
+    }
   }
 };
-
-// Eigen-decomposition of a Gram sorted by descending eigenvalue.
-void desc_eig(int64_t w, const std::vector<double> &G, std::vector<double> &lam, std::vector<double> &W) {
-  std::vector<double> la(w), V(w * w);
-  sym_eig(w, G.data(), la.data(), V.data());
-  lam.assign(w, 0.0);
-  W.assign(w * w, 0.0);
-  for (int64_t r = 0; r < w; ++r) {
-    const int64_t k = w - 1 - r;
-    lam[r] = la[k];
-    for (int64_t i = 0; i < w; ++i) W[i * w + r] = V[i * w + k];
-  }
-}
 
 }  // namespace
 
@@ -151,121 +143,147 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
   const bool adaptive = tol > 0.0;
   if (adaptive && normalizer != SVDH_NORM_QR) fail(SVDH_ERR_ARG, "adaptive power iterations need the QR normaliser");
   if (max_power_iters < n_power_iters) max_power_iters = n_power_iters;
+  const int pdt = precision == SVDH_PREC_F64 ? SVDH_F64 : SVDH_F32;
+  if (pdt == SVDH_F64 && l > 64) fail(SVDH_ERR_ARG, "FP64 panels are limited to rank + oversamples <= 64");
 
   Rsvd R;
   R.c = ctx;
   R.op = op;
-  R.vdt = precision == SVDH_PREC_F64 ? SVDH_F64 : SVDH_F32;
-  R.pdt = precision == SVDH_PREC_F64 ? SVDH_F64 : SVDH_F32;
+  R.vdt = pdt;
   R.bf16 = precision == SVDH_PREC_BF16;
-  const int pdt = R.pdt;
   const uint64_t s = has_seed ? seed : 0;
 
-  Panel Om, Y, Z;
-  Om.init(n, l, pdt);
-  Y.init(m, l, pdt);
-  Z.init(n, l, pdt);
-  HIP_CHECK(hipMemsetAsync(Om.p(), 0, Om.buf.bytes, ctx->stream));
-  HIP_CHECK(hipMemsetAsync(Y.p(), 0, Y.buf.bytes, ctx->stream));
-  HIP_CHECK(hipMemsetAsync(Z.p(), 0, Z.buf.bytes, ctx->stream));
-  if (omega) {
-    // column-major n x l FP64 -> row-major panel
-    std::vector<double> h((size_t)n * Om.ld, 0.0);
-    for (int64_t j = 0; j < l; ++j)
-      for (int64_t i = 0; i < n; ++i) h[i * Om.ld + j] = omega[i + j * n];
-    DevBuf tmp(h.size() * sizeof(double));
-    HIP_CHECK(hipMemcpyAsync(tmp.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    panel_convert(ctx, n, l, tmp.p, Om.ld, SVDH_F64, Om.p(), Om.ld, pdt);
-    HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  } else {
-    panel_gaussian(ctx, n, l, s, Om.p(), Om.ld, pdt);
+  Panel Y, Z;
+  {
+    Panel Om;
+    Om.init(n, l, pdt);
+    if (omega) {
+      // column-major n x l FP64 -> row-major panel
+      std::vector<double> h((size_t)n * Om.ld, 0.0);
+      for (int64_t j = 0; j < l; ++j)
+        for (int64_t i = 0; i < n; ++i) h[i * Om.ld + j] = omega[i + j * n];
+      DevBuf tmp(h.size() * sizeof(double));
+      HIP_CHECK(hipMemcpyAsync(tmp.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      panel_convert(ctx, n, l, tmp.p, Om.ld, SVDH_F64, Om.p(), Om.ld, pdt);
+      HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } else {
+      panel_gaussian(ctx, n, l, s, Om.p(), Om.ld, pdt);
+    }
+    Y.init(m, l, pdt);
+    Z.init(n, l, pdt);
+    // BF16 ladder: the power steps use BF16 values and BF16 copies of the
+    // input panels until the residual estimate (adaptive) drops below
+    // 1e-2, or for the first half of a fixed count; the final B^T = A^T Q and
+    // everything after are FP32.
+    R.mult(0, Om, Y, R.bf16);
   }
-
-  // BF16 ladder: early passes run on BF16 values/panels until the residual
-  // estimate (adaptive) falls below this, or for the first half of a fixed count.
   const double bf16_switch = 1e-2;
   bool low = R.bf16;
-  R.mult(0, Om, Y, low);
   int64_t iters_run = 0;
   double est = -1.0;
-  std::vector<double> GZ, lam, W;
+  std::vector<double> GZ, lam, W, Mq;  // Mq: Q = Y Mq
+  int64_t wq = Y.w;
 
-  if (!adaptive) {
-    for (int64_t i = 0; i < n_power_iters; ++i) {
-      if (R.bf16) low = i < (n_power_iters + 1) / 2;
-      R.mult(1, Y, Z, low);
-      R.normalize(Z, normalizer);
-      R.mult(0, Z, Y, low);
-      R.normalize(Y, normalizer);
-      ++iters_run;
-    }
-    R.orth(Y);  // randomized/mod.rs:143-144
-  } else {
-    R.orth(Y);
+  if (normalizer == SVDH_NORM_QR) {
+    R.normaliser(Y, Mq, wq);
     Panel Yn;
     Yn.init(m, l, pdt);
+    int64_t next_check = n_power_iters;
+    int64_t last_check_it = -1;
+    double last_est = 0.0;
     for (;;) {
-      R.mult(1, Y, Z, low);  // Z = A^T Q = B^T for the current Q
-      const int64_t w = Z.w;
-      GZ.assign(w * w, 0.0);
-      gram(ctx, n, w, Z.p(), Z.ld, Z.dt, GZ.data());
-      desc_eig(w, GZ, lam, W);
-      R.mult(0, Z, Yn, low);  // A Z = A v_i s_i for the Ritz vectors of Q
-      if (iters_run >= n_power_iters) {
-        // r_i = A v_i - s_i u_i = (A Z) w_i / s_i - Q w_i s_i, column norms in FP64.
-        const int64_t k = std::min(target_rank, std::min(w, n));
-        std::vector<double> M1(w * k, 0.0), M2(w * k, 0.0), nr2(k);
+      if (!adaptive && R.bf16) low = iters_run < (n_power_iters + 1) / 2;
+      const bool final_fixed = !adaptive && iters_run >= n_power_iters;
+      // Z = A^T Q = (A^T Y) Mq
+      R.mult(1, Y, Z, low && !final_fixed);
+      panel_apply(ctx, n, Y.w, wq, Z.p(), Z.ld, Z.dt, Mq.data(), Z.p(), Z.ld, Z.dt);
+      Z.w = wq;
+      const bool check = adaptive && (iters_run >= next_check || iters_run >= max_power_iters);
+      if (final_fixed || check) {
+        GZ.assign(wq * wq, 0.0);
+        gram(ctx, n, wq, Z.p(), Z.ld, Z.dt, GZ.data());
+        desc_eig(wq, GZ, lam, W);
+      }
+      if (final_fixed) break;
+      R.mult(0, Z, Yn, low);  // A Z
+      if (check) {
+        const int64_t k = std::min(target_rank, std::min(wq, n));
+        // r_i = (A Z) w_i / s_i - Y (Mq w_i s_i)
+        std::vector<double> M1(wq * k, 0.0), M2(Y.w * k, 0.0), nr2(k);
         bool ok = true;
         for (int64_t r = 0; r < k; ++r) {
           const double sr = std::sqrt(std::max(lam[r], 0.0));
           if (!(sr > 0.0)) ok = false;
-          for (int64_t i = 0; i < w; ++i) {
-            M1[i * k + r] = sr > 0.0 ? W[i * w + r] / sr : 0.0;
-            M2[i * k + r] = -W[i * w + r] * sr;
+          for (int64_t i = 0; i < wq; ++i) M1[i * k + r] = sr > 0.0 ? W[i * wq + r] / sr : 0.0;
+          for (int64_t i = 0; i < Y.w; ++i) {
+            double t = 0.0;
+            for (int64_t j = 0; j < wq; ++j) t += Mq[i * wq + j] * W[j * wq + r];
+            M2[i * k + r] = -t * sr;
           }
         }
-        panel_residual(ctx, m, w, k, Yn.p(), Y.p(), Y.ld, pdt, M1.data(), M2.data(), nr2.data());
+        panel_residual(ctx, m, Y.w, wq, k, Y.p(), Yn.p(), Y.ld, pdt, M2.data(), M1.data(), nr2.data());
         est = 0.0;
         for (int64_t r = 0; r < k; ++r) est = std::max(est, std::sqrt(nr2[r]) / std::sqrt(std::max(lam[r], 0.0)));
-        if (!ok) est = INFINITY;
-        if (est <= tol || iters_run >= max_power_iters) break;
+        if (!ok || !std::isfinite(est)) est = INFINITY;
+        if ((est <= tol && !low) || iters_run >= max_power_iters) break;
         if (R.bf16 && low && est <= bf16_switch) low = false;
+        // Next check: predicted from the contraction rate since the last one.
+        int64_t step = 1;
+        if (last_check_it >= 0 && est < last_est && est > tol && !low) {
+          const double rate = std::pow(est / last_est, 1.0 / (double)(iters_run - last_check_it));
+          if (rate > 0.0 && rate < 1.0) {
+            const double need = std::log(tol / est) / std::log(rate);
+            step = std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)std::floor(0.8 * need)));
+          }
+        }
+        last_check_it = iters_run;
+        last_est = est;
+        next_check = iters_run + step;
       }
-      // Next subspace: Q = orth(A Z).
+      // Next subspace: Y <- A Z and its normaliser.
       std::swap(Y.buf, Yn.buf);
       std::swap(Y.ld, Yn.ld);
-      Y.w = w;
-      Y.rows = m;
-      R.orth(Y);
+      Y.w = Z.w;
+      R.normaliser(Y, Mq, wq);
       ++iters_run;
     }
+  } else {
+    for (int64_t i = 0; i < n_power_iters; ++i) {
+      if (R.bf16) low = i < (n_power_iters + 1) / 2;
+      R.mult(1, Y, Z, low);
+      if (normalizer == SVDH_NORM_LU) panel_normalize_columns(ctx, Z.rows, Z.w, Z.p(), Z.ld, Z.dt);
+      R.mult(0, Z, Y, low);
+      if (normalizer == SVDH_NORM_LU) panel_normalize_columns(ctx, Y.rows, Y.w, Y.p(), Y.ld, Y.dt);
+      ++iters_run;
+    }
+    R.orth(Y);  // randomized/mod.rs:143-144
+    Mq.assign(Y.w * Y.w, 0.0);
+    for (int64_t i = 0; i < Y.w; ++i) Mq[i * Y.w + i] = 1.0;
+    wq = Y.w;
+    R.mult(1, Y, Z, false);
+    GZ.assign(wq * wq, 0.0);
+    gram(ctx, n, wq, Z.p(), Z.ld, Z.dt, GZ.data());
+    desc_eig(wq, GZ, lam, W);
   }
 
-  // B^T = A^T Q in full precision, SVD via its Gram (skipped when the
-  // adaptive loop already holds it for the current Q).
-  if (!adaptive || low) {
-    R.mult(1, Y, Z, false);
-    const int64_t w = Z.w;
-    GZ.assign(w * w, 0.0);
-    gram(ctx, n, w, Z.p(), Z.ld, Z.dt, GZ.data());
-    desc_eig(w, GZ, lam, W);
-  }
-  const int64_t w = Z.w;
-  const int64_t k = std::min(target_rank, std::min(w, n));
-  std::vector<double> sv(k), Wk(w * k), Ws(w * k);
+  // B^T = Z (n x wq) = A^T Q; SVD(B) = W S X^T with X = Z W S^-1; U = Q W = Y (Mq W).
+  const int64_t k = std::min(target_rank, std::min(wq, n));
+  std::vector<double> sv(k), MW(Y.w * k, 0.0), Ws(wq * k);
   for (int64_t r = 0; r < k; ++r) {
     sv[r] = std::sqrt(std::max(lam[r], 0.0));
     const double inv = sv[r] > 0.0 ? 1.0 / sv[r] : 0.0;
-    for (int64_t i = 0; i < w; ++i) {
-      Wk[i * k + r] = W[i * w + r];
-      Ws[i * k + r] = W[i * w + r] * inv;
+    for (int64_t i = 0; i < wq; ++i) Ws[i * k + r] = W[i * wq + r] * inv;
+    for (int64_t i = 0; i < Y.w; ++i) {
+      double t = 0.0;
+      for (int64_t j = 0; j < wq; ++j) t += Mq[i * wq + j] * W[j * wq + r];
+      MW[i * k + r] = t;
     }
   }
   Panel U, V;
   U.init(m, k, SVDH_F64);
   V.init(n, k, SVDH_F64);
-  panel_apply(ctx, m, w, k, Y.p(), Y.ld, Y.dt, Wk.data(), U.p(), U.ld, SVDH_F64);
-  panel_apply(ctx, n, w, k, Z.p(), Z.ld, Z.dt, Ws.data(), V.p(), V.ld, SVDH_F64);
+  panel_apply(ctx, m, Y.w, k, Y.p(), Y.ld, Y.dt, MW.data(), U.p(), U.ld, SVDH_F64);
+  panel_apply(ctx, n, wq, k, Z.p(), Z.ld, Z.dt, Ws.data(), V.p(), V.ld, SVDH_F64);
   std::vector<double> hu((size_t)m * U.ld), hv((size_t)n * V.ld);
   HIP_CHECK(hipMemcpyAsync(hu.data(), U.p(), hu.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIP_CHECK(hipMemcpyAsync(hv.data(), V.p(), hv.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

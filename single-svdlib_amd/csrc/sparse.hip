@@ -10,10 +10,11 @@
 // rows (A^T is the CSC built once at upload). Output rows are sorted by nnz
 // and cut into bins; a bin assigns 2^lg lanes to a row (SpMV) or a row group
 // of lanes across panel columns (SpMM). Rows longer than a threshold are cut
-// into fixed nnz chunks, one workgroup each; the last chunk of a row to finish
-// (atomic counter, self-resetting) adds the chunk partials in chunk order, so
-// results are deterministic and no output is written with atomics.
+// into fixed nnz chunks, one workgroup each, whose partials a second small
+// kernel adds in chunk order, so results are deterministic, no output is
+// written with atomics and no workgroup waits on another.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -25,8 +26,6 @@ namespace {
 
 constexpr int kSpmvLong = 1024;   // SpMV rows above this nnz are chunked
 constexpr int kSpmvChunk = 4096;  // SpMV chunk nnz (256 threads x 16)
-constexpr int kSpmmLong = 768;    // SpMM rows above this nnz are chunked
-constexpr int kSpmmChunk = 1024;  // SpMM chunk nnz
 
 typedef uint16_t bf16_t;
 
@@ -122,7 +121,6 @@ struct SpmvArgs {
   const int32_t *colmap;
   const int32_t *rows;
   const int32_t *chunks;
-  int32_t *counters;
   double *partials;
   const X *x;
   Y *y;
@@ -177,9 +175,8 @@ __device__ __forceinline__ void spmv_group(const SpmvArgs<V, X, Y> &a, const Bin
 template <bool MC, typename V, typename X, typename Y>
 __device__ __forceinline__ void spmv_long(const SpmvArgs<V, X, Y> &a, int ch) {
   __shared__ double red[kBlock / 64];
-  __shared__ int is_last;
   const int *cd = a.chunks + 6 * ch;
-  const int r = cd[0], beg = cd[1], end = cd[2], first = cd[3], nch = cd[4], lid = cd[5];
+  const int r = cd[0], beg = cd[1], end = cd[2], nch = cd[4];
   double acc = 0.0;
   int k = beg + threadIdx.x;
   for (; k + 3 * kBlock < end; k += 4 * kBlock) {
@@ -209,24 +206,23 @@ __device__ __forceinline__ void spmv_long(const SpmvArgs<V, X, Y> &a, int ch) {
     double t = 0.0;
 #pragma unroll
     for (int w = 0; w < kBlock / 64; ++w) t += red[w];
-    if (nch == 1) {
+    if (nch == 1)
       a.y[r] = static_cast<Y>(t);
-      is_last = 0;
-    } else {
+    else
       a.partials[ch] = t;
-      __threadfence();
-      const int prev = atomicAdd(&a.counters[lid], 1);
-      is_last = (prev == nch - 1);
-    }
   }
-  __syncthreads();
-  if (is_last && threadIdx.x == 0) {
-    __threadfence();
-    double t = 0.0;
-    for (int i = 0; i < nch; ++i) t += __builtin_nontemporal_load(&a.partials[first + i]);
-    a.y[r] = static_cast<Y>(t);
-    a.counters[lid] = 0;
-  }
+}
+
+// Second pass for SpMV rows cut into several chunks: partials added in chunk
+// order, one thread per such row (the kernel boundary orders the passes).
+template <typename Y>
+__global__ void k_spmv_fin(int nrows, const int32_t *multi, const double *partials, Y *y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  const int r = multi[3 * i], first = multi[3 * i + 1], nch = multi[3 * i + 2];
+  double t = 0.0;
+  for (int k = 0; k < nch; ++k) t += partials[first + k];
+  y[r] = static_cast<Y>(t);
 }
 
 template <bool MC, typename V, typename X, typename Y>
@@ -254,9 +250,26 @@ __global__ __launch_bounds__(kBlock) void k_spmv(SpmvArgs<V, X, Y> a) {
 
 // ------------------------------------------------------------------------
 // SpMM: Y (rows x l, row-major, ldy) = A X (X row-major, ldx)
-// A row group of R = 2^LGR lanes owns one output row; lane t covers panel
-// columns [c0 + t*NV, c0 + (t+1)*NV) of each column pass.
+//
+// Two kinds of work in one launch:
+//  - chunks of long rows (more than SpmmTune::long_nnz), one workgroup each: the
+//    row groups of the workgroup take slices of the chunk and are reduced in
+//    LDS in group order. When the operand panel X is larger than one XCD's
+//    L2, long rows are first cut at 8 source-block boundaries and workgroup
+//    w takes chunks of block w % 8 only: workgroups are dispatched
+//    round-robin over the 8 XCDs, so each XCD gathers from one eighth of X.
+//    A row with several chunks writes one partial per chunk; a second kernel
+//    (k_spmm_fin) adds them in chunk order, so results do not depend on
+//    timing and no workgroup waits on or signals another;
+//  - short rows, one row group of R = 2^LGR lanes each, rows sorted by nnz.
+// Lane t of a row group covers panel columns [c0 + t*NV, c0 + (t+1)*NV) of
+// each column pass.
 // ------------------------------------------------------------------------
+
+constexpr int kChunkInts = 8;  // r, b, e, slot, first_slot, nchunks, row_id (unused), pad
+#ifndef SVDH_SPMM_UNROLL
+#define SVDH_SPMM_UNROLL 4  // panel gathers in flight per row group
+#endif
 
 template <typename V, typename X, typename Y, typename Acc>
 struct SpmmArgs {
@@ -265,151 +278,174 @@ struct SpmmArgs {
   const V *vals;
   const int32_t *rowmap;
   const int32_t *colmap;
-  const int32_t *rows;
-  const int32_t *chunks;
-  int32_t *counters;
-  Acc *partials;  // nchunks x ldp
+  const int32_t *rows;    // short rows, sorted by nnz (descending)
+  const int32_t *chunks;  // kChunkInts per chunk, grouped by source block
+  Acc *partials;  // slot x ldp
   const X *x;
   Y *y;
   int64_t ldx, ldy, ldp;
   int32_t l;
-  int32_t row_count;
-  int32_t long_block_begin;
+  int32_t nshort;
+  int32_t nblk;
+  int32_t chunk_blocks;  // nblk * max chunks per block
+  int32_t blk_off[8];
+  int32_t blk_cnt[8];
 };
 
+// acc += sum over nnz k in [kb, ke) of A_k * X[col_k, cols of this lane]
+template <int R, bool MC, typename V, typename X, typename Y, typename Acc>
+__device__ __forceinline__ void spmm_accum(const SpmmArgs<V, X, Y, Acc> &a, int kb, int ke, int lane, int col,
+                                           bool colok, Acc (&acc)[PanelVec<X>::N]) {
+  constexpr int NV = PanelVec<X>::N;
+  for (int base = kb; base < ke; base += R) {
+    int c = 0;
+    Acc v = Acc(0);
+    const int kk = base + lane;
+    if (kk < ke) {
+      c = __builtin_nontemporal_load(&a.colidx[kk]);
+      v = ld_as<Acc>(a.vals, kk);
+      if (MC) {
+        c = a.colmap[c];
+        if (c < 0) {
+          c = 0;
+          v = Acc(0);
+        }
+      }
+    }
+    const int n = min(R, ke - base);
+    constexpr int U = SVDH_SPMM_UNROLL < R ? SVDH_SPMM_UNROLL : R;
+    int j = 0;
+    for (; j + U <= n; j += U) {
+      int cj[U];
+      Acc vj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        cj[u] = __shfl(c, j + u, R);
+        vj[u] = __shfl(v, j + u, R);
+      }
+      if (colok) {
+        Acc xv[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; ++u) ld_panel<Acc, X>(a.x + (int64_t)cj[u] * a.ldx + col, xv[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int i = 0; i < NV; ++i) acc[i] += vj[u] * xv[u][i];
+      }
+    }
+    for (; j < n; ++j) {
+      const int cj = __shfl(c, j, R);
+      const Acc vj = __shfl(v, j, R);
+      if (colok) {
+        Acc xv[NV];
+        ld_panel<Acc, X>(a.x + (int64_t)cj * a.ldx + col, xv);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += vj * xv[i];
+      }
+    }
+  }
+}
+
 template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
-__device__ __forceinline__ void spmm_rows(const SpmmArgs<V, X, Y, Acc> &a, int b) {
+__device__ __forceinline__ void spmm_short(const SpmmArgs<V, X, Y, Acc> &a, int b) {
   constexpr int R = 1 << LGR;
   constexpr int NV = PanelVec<X>::N;
   constexpr int RPB = kBlock >> LGR;
   const int gid = threadIdx.x >> LGR;
   const int lane = threadIdx.x & (R - 1);
   const int idx = b * RPB + gid;
-  const bool active = idx < a.row_count;
-  int r = 0, k0 = 0, e = 0;
-  if (active) {
-    r = a.rows[idx];
-    const int src = a.rowmap ? a.rowmap[r] : r;
-    k0 = a.rowptr[src];
-    e = a.rowptr[src + 1];
-  }
+  if (idx >= a.nshort) return;  // the whole row group leaves together
+  const int r = a.rows[idx];
+  const int src = a.rowmap ? a.rowmap[r] : r;
+  const int kb = a.rowptr[src], ke = a.rowptr[src + 1];
   for (int c0 = 0; c0 < a.l; c0 += R * NV) {
     const int col = c0 + lane * NV;
     const bool colok = col < a.l;
     Acc acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
-    for (int base = k0; base < e; base += R) {
-      int c = -1;
-      Acc v = Acc(0);
-      const int kk = base + lane;
-      if (kk < e) {
-        c = a.colidx[kk];
-        if (MC) c = a.colmap[c];
-        v = ld_as<Acc>(a.vals, kk);
-      }
-      const int cnt = min(R, e - base);
-      for (int j = 0; j < cnt; ++j) {
-        const int cj = __shfl(c, j, R);
-        const Acc vj = __shfl(v, j, R);
-        if ((!MC || cj >= 0) && colok) {
-          Acc xv[NV];
-          ld_panel<Acc, X>(a.x + (int64_t)cj * a.ldx + col, xv);
-#pragma unroll
-          for (int i = 0; i < NV; ++i) acc[i] += vj * xv[i];
-        }
-      }
-    }
-    if (active && colok) st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
+    spmm_accum<R, MC>(a, kb, ke, lane, col, colok, acc);
+    if (colok) st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
   }
 }
 
 template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
-__device__ __forceinline__ void spmm_long(const SpmmArgs<V, X, Y, Acc> &a, int ch) {
+__device__ __forceinline__ void spmm_chunk(const SpmmArgs<V, X, Y, Acc> &a, int ci) {
   constexpr int R = 1 << LGR;
   constexpr int NV = PanelVec<X>::N;
-  constexpr int NG = kBlock >> LGR;  // row groups per workgroup
+  constexpr int NG = kBlock >> LGR;
   __shared__ Acc red[kBlock * PanelVec<X>::N];
-  __shared__ int is_last;
-  const int *cd = a.chunks + 6 * ch;
-  const int r = cd[0], beg = cd[1], end = cd[2], first = cd[3], nch = cd[4], lid = cd[5];
+  const int *cd = a.chunks + (int64_t)ci * kChunkInts;
+  const int r = cd[0], cb = cd[1], ce = cd[2], slot = cd[3], nch = cd[5];
   const int gid = threadIdx.x >> LGR;
   const int lane = threadIdx.x & (R - 1);
-  const int len = end - beg;
-  const int per = (len + NG - 1) / NG;
-  const int gb = beg + gid * per;
-  const int ge = min(end, gb + per);
+  const int per = (ce - cb + NG - 1) / NG;
+  const int gb = min(ce, cb + gid * per);
+  const int ge = min(ce, gb + per);
   for (int c0 = 0; c0 < a.l; c0 += R * NV) {
     const int col = c0 + lane * NV;
     const bool colok = col < a.l;
     Acc acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
-    for (int base = gb; base < ge; base += R) {
-      int c = -1;
-      Acc v = Acc(0);
-      const int kk = base + lane;
-      if (kk < ge) {
-        c = a.colidx[kk];
-        if (MC) c = a.colmap[c];
-        v = ld_as<Acc>(a.vals, kk);
-      }
-      const int cnt = min(R, ge - base);
-      for (int j = 0; j < cnt; ++j) {
-        const int cj = __shfl(c, j, R);
-        const Acc vj = __shfl(v, j, R);
-        if ((!MC || cj >= 0) && colok) {
-          Acc xv[NV];
-          ld_panel<Acc, X>(a.x + (int64_t)cj * a.ldx + col, xv);
-#pragma unroll
-          for (int i = 0; i < NV; ++i) acc[i] += vj * xv[i];
-        }
-      }
-    }
-    // Reduce the NG group partials of this column pass in group order.
+    spmm_accum<R, MC>(a, gb, ge, lane, col, colok, acc);
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < NV; ++i) red[gid * (R * NV) + lane * NV + i] = acc[i];
     __syncthreads();
     for (int t = threadIdx.x; t < R * NV; t += kBlock) {
       const int cc = c0 + t;
       if (cc < a.l) {
-        Acc s = Acc(0);
-        for (int g = 0; g < NG; ++g) s += red[g * (R * NV) + t];
+        Acc sum = Acc(0);
+        for (int g = 0; g < NG; ++g) sum += red[g * (R * NV) + t];
         if (nch == 1)
-          a.y[(int64_t)r * a.ldy + cc] = static_cast<Y>(s);
+          a.y[(int64_t)r * a.ldy + cc] = static_cast<Y>(sum);
         else
-          a.partials[(int64_t)ch * a.ldp + cc] = s;
+          a.partials[(int64_t)slot * a.ldp + cc] = sum;
       }
     }
+  }
+}
+
+// Second pass for rows cut into several chunks: one workgroup per such row
+// adds the chunk partials in chunk order (4 interleaved chains per column,
+// combined in a fixed order). The kernel boundary makes the partials of the
+// first pass visible, so no fences or atomics are needed.
+template <typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm_fin(const int32_t *multi, const Acc *partials, int64_t ldp, Y *y,
+                                                    int64_t ldy, int l) {
+  __shared__ Acc red[kBlock];
+  const int *md = multi + 3 * (int64_t)blockIdx.x;
+  const int r = md[0], first = md[1], nch = md[2];
+  const int q = threadIdx.x >> 6, o = threadIdx.x & 63;
+  for (int c0 = 0; c0 < l; c0 += 64) {
+    const int cc = c0 + o;
+    Acc sum = Acc(0);
+    if (cc < l)
+      for (int i = q; i < nch; i += 4) sum += partials[(int64_t)(first + i) * ldp + cc];
     __syncthreads();
-  }
-  if (nch == 1) return;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int prev = atomicAdd(&a.counters[lid], 1);
-    is_last = (prev == nch - 1);
-  }
-  __syncthreads();
-  if (is_last) {
-    __threadfence();
-    for (int cc = threadIdx.x; cc < a.l; cc += kBlock) {
-      Acc s = Acc(0);
-      for (int i = 0; i < nch; ++i)
-        s += __builtin_nontemporal_load(&a.partials[(int64_t)(first + i) * a.ldp + cc]);
-      a.y[(int64_t)r * a.ldy + cc] = static_cast<Y>(s);
-    }
-    if (threadIdx.x == 0) a.counters[lid] = 0;
+    red[q * 64 + o] = sum;
+    __syncthreads();
+    if (q == 0 && cc < l) y[(int64_t)r * ldy + cc] = static_cast<Y>(((red[o] + red[64 + o]) + red[128 + o]) + red[192 + o]);
   }
 }
 
 template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
 __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs<V, X, Y, Acc> a) {
-  const int b = blockIdx.x;
-  if (b >= a.long_block_begin)
-    spmm_long<LGR, MC>(a, b - a.long_block_begin);
-  else
-    spmm_rows<LGR, MC>(a, b);
+  if ((int)blockIdx.x < a.chunk_blocks) {
+    const int xb = blockIdx.x % a.nblk;
+    const int kb = blockIdx.x / a.nblk;
+    int off = 0, cnt = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (i == xb) {
+        off = a.blk_off[i];
+        cnt = a.blk_cnt[i];
+      }
+    if (kb < cnt) spmm_chunk<LGR, MC>(a, off + kb);
+  } else {
+    spmm_short<LGR, MC>(a, blockIdx.x - a.chunk_blocks);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -511,7 +547,6 @@ namespace svdh {
 // Build (once) the schedule of one operator direction.
 //   kind 0 (SpMV): bins of 2^lg lanes per row, lg = ceil(log2(ceil(nnz/4)))
 //                  clamped to [0, 6]; rows above kSpmvLong nnz are chunked.
-//   kind 1 (SpMM): one bin of short rows, rows above kSpmmLong chunked.
 static Schedule &get_schedule(svdh_op *op, int direction, int kind) {
   Schedule &s = op->sched[direction][kind];
   if (s.built) return s;
@@ -526,8 +561,8 @@ static Schedule &get_schedule(svdh_op *op, int direction, int kind) {
   std::vector<int32_t> order(nout);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return len[a] < len[b]; });
-  const int long_t = kind == 0 ? kSpmvLong : kSpmmLong;
-  const int chunk = kind == 0 ? kSpmvChunk : kSpmmChunk;
+  const int long_t = kSpmvLong;
+  const int chunk = kSpmvChunk;
   int64_t nshort = 0;
   while (nshort < nout && len[order[nshort]] <= long_t) ++nshort;
   // Long-row chunks.
@@ -583,9 +618,157 @@ static Schedule &get_schedule(svdh_op *op, int direction, int kind) {
   if (nout) HIP_CHECK(hipMemcpy(s.rows.p, order.data(), nout * sizeof(int32_t), hipMemcpyHostToDevice));
   s.chunks.alloc(std::max<size_t>(ch.size(), 6) * sizeof(int32_t));
   if (!ch.empty()) HIP_CHECK(hipMemcpy(s.chunks.p, ch.data(), ch.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  s.counters.alloc(std::max<int32_t>(nlong, 1) * sizeof(int32_t));
-  HIP_CHECK(hipMemset(s.counters.p, 0, std::max<int32_t>(nlong, 1) * sizeof(int32_t)));
+  // rows with several chunks: r, first chunk, nchunks (for the second pass)
+  std::vector<int32_t> multi;
+  for (size_t k = 0; k < ch.size(); k += 6)
+    if (ch[k + 4] > 1 && (int32_t)(k / 6) == ch[k + 3]) {
+      multi.push_back(ch[k]);
+      multi.push_back(ch[k + 3]);
+      multi.push_back(ch[k + 4]);
+    }
+  s.nslots = (int32_t)(multi.size() / 3);
+  s.counters.alloc(std::max<size_t>(multi.size(), 3) * sizeof(int32_t));
+  if (!multi.empty())
+    HIP_CHECK(hipMemcpy(s.counters.p, multi.data(), multi.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   (void)c;
+  s.built = true;
+  return s;
+}
+
+// SpMM schedule parameters (defaults; SVDH_SPMM_LONG / _CHUNK / _SPLIT /
+// _NBLK override them for tuning).
+// Defaults measured on the 100,000 x 20,000 Zipf workload (width 60, FP32):
+// A^T Y 195 us with these, 356 us without source blocks, 660 us when chunk
+// partials were combined inside the kernel.
+struct SpmmTune {
+  int long_nnz = 256;      // rows above this nnz are cut into chunks
+  int chunk = 1024;        // nnz per chunk (one workgroup)
+  int split_min = 512;     // rows with at least this nnz are cut at source blocks
+  int nblk = 8;            // source blocks when the panel exceeds one XCD's L2
+};
+
+static SpmmTune spmm_tune() {
+  SpmmTune t;
+  auto get = [](const char *k, int d) {
+    const char *e = std::getenv(k);
+    return e ? std::atoi(e) : d;
+  };
+  t.long_nnz = std::max(1, get("SVDH_SPMM_LONG", t.long_nnz));
+  t.chunk = std::max(64, get("SVDH_SPMM_CHUNK", t.chunk));
+  t.split_min = std::max(1, get("SVDH_SPMM_SPLIT", t.split_min));
+  t.nblk = std::max(1, std::min(8, get("SVDH_SPMM_NBLK", t.nblk)));
+  return t;
+}
+
+// Build (once) the SpMM schedule of one operator direction (see k_spmm).
+// Source blocks: 8 when the operand panel has at least 65,536 rows and the
+// source indices of every long row are sorted, otherwise 1.
+static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
+  Schedule &s = op->sched[direction][1];
+  if (s.built) return s;
+  CsrView v = op->view(direction);
+  const svdh_op *own = op->base ? op->base : op;
+  const std::vector<int32_t> &rp = *v.h_rowptr;
+  const std::vector<int32_t> &ci = direction == 0 ? own->h_colidx : own->h_t_colidx;
+  const int64_t nout = v.nrows, nsrc = v.ncols;
+  auto src_of = [&](int64_t r) { return v.h_rowmap ? (int64_t)(*v.h_rowmap)[r] : r; };
+  const SpmmTune tune = spmm_tune();
+  std::vector<int32_t> shortr, longr;
+  for (int64_t r = 0; r < nout; ++r) {
+    const int64_t sr = src_of(r);
+    (rp[sr + 1] - rp[sr] > tune.long_nnz ? longr : shortr).push_back((int32_t)r);
+  }
+  std::stable_sort(shortr.begin(), shortr.end(), [&](int32_t a, int32_t b) {
+    return rp[src_of(a) + 1] - rp[src_of(a)] > rp[src_of(b) + 1] - rp[src_of(b)];
+  });
+  int nblk = (nsrc >= 65536 && !(direction == 0 && op->base)) ? tune.nblk : 1;
+  for (size_t i = 0; i < longr.size() && nblk > 1; ++i) {
+    const int64_t sr = src_of(longr[i]);
+    for (int32_t k = rp[sr] + 1; k < rp[sr + 1]; ++k)
+      if (ci[k - 1] > ci[k]) {
+        nblk = 1;
+        break;
+      }
+  }
+  struct Ch { int32_t r, b, e, slot, first, n, rid, blk; };
+  std::vector<std::vector<Ch>> per(nblk);
+  std::vector<Ch> row;
+  int32_t nslots = 0, nmulti = 0;
+  int64_t rr = 0;  // round-robin list of chunks of rows that are not cut at source blocks
+  for (int32_t r : longr) {
+    const int64_t sr = src_of(r);
+    const int32_t b0 = rp[sr], e0 = rp[sr + 1];
+    row.clear();
+    int32_t lo = b0;
+    const int nb = (e0 - b0 >= tune.split_min) ? nblk : 1;
+    for (int x = 0; x < nb; ++x) {
+      int32_t hi = e0;
+      if (x + 1 < nb) {
+        const int32_t bound = (int32_t)(nsrc * (x + 1) / nb);
+        hi = (int32_t)(std::lower_bound(ci.begin() + lo, ci.begin() + e0, bound) - ci.begin());
+      }
+      const int32_t len = hi - lo;
+      const int32_t pieces = (len + tune.chunk - 1) / tune.chunk;
+      for (int32_t p = 0; p < pieces; ++p) {
+        const int32_t pb = lo + (int32_t)((int64_t)len * p / pieces);
+        const int32_t pe = lo + (int32_t)((int64_t)len * (p + 1) / pieces);
+        row.push_back(Ch{r, pb, pe, 0, 0, 0, 0, nb > 1 ? x : (int)(rr++ % nblk)});
+      }
+      lo = hi;
+    }
+    const int32_t n = (int32_t)row.size();
+    const int32_t first = n > 1 ? nslots : 0;
+    const int32_t rid = n > 1 ? nmulti++ : 0;
+    for (int32_t i = 0; i < n; ++i) {
+      Ch g = row[i];
+      g.n = n;
+      g.first = first;
+      g.slot = n > 1 ? first + i : 0;
+      g.rid = rid;
+      per[g.blk].push_back(g);
+    }
+    if (n > 1) nslots += n;
+  }
+  std::vector<int32_t> multi;  // r, first_slot, nchunks of every row with several chunks
+  for (auto &L : per)
+    for (const Ch &g : L)
+      if (g.n > 1 && g.slot == g.first) {
+        multi.push_back(g.r);
+        multi.push_back(g.first);
+        multi.push_back(g.n);
+      }
+  std::vector<int32_t> flat;
+  int32_t off = 0, maxcnt = 0;
+  s.nblk = nblk;
+  for (int x = 0; x < 8; ++x) {
+    s.blk_off[x] = 0;
+    s.blk_cnt[x] = 0;
+  }
+  for (int x = 0; x < nblk; ++x) {
+    auto &L = per[x];
+    std::stable_sort(L.begin(), L.end(), [](const Ch &a, const Ch &b) { return (a.e - a.b) > (b.e - b.b); });
+    s.blk_off[x] = off;
+    s.blk_cnt[x] = (int32_t)L.size();
+    maxcnt = std::max(maxcnt, (int32_t)L.size());
+    for (const Ch &g : L) {
+      const int32_t w[kChunkInts] = {g.r, g.b, g.e, g.slot, g.first, g.n, g.rid, 0};
+      flat.insert(flat.end(), w, w + kChunkInts);
+    }
+    off += (int32_t)L.size();
+  }
+  s.nchunks = off;
+  s.total_blocks = maxcnt * nblk;  // chunk blocks; short-row blocks depend on l
+  s.nslots = nslots;
+  s.nlong_rows = nmulti;
+  s.bins[0].row_count = (int32_t)shortr.size();
+  s.rows.alloc(std::max<size_t>(shortr.size(), 1) * sizeof(int32_t));
+  if (!shortr.empty())
+    HIP_CHECK(hipMemcpy(s.rows.p, shortr.data(), shortr.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  s.chunks.alloc(std::max<size_t>(flat.size(), kChunkInts) * sizeof(int32_t));
+  if (!flat.empty()) HIP_CHECK(hipMemcpy(s.chunks.p, flat.data(), flat.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  s.counters.alloc(std::max<size_t>(multi.size(), 3) * sizeof(int32_t));
+  if (!multi.empty())
+    HIP_CHECK(hipMemcpy(s.counters.p, multi.data(), multi.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   s.built = true;
   return s;
 }
@@ -605,7 +788,6 @@ static void launch_spmv(svdh_op *op, int direction, const void *vals, const void
   a.colmap = v.colmap;
   a.rows = s.rows.as<int32_t>();
   a.chunks = s.chunks.as<int32_t>();
-  a.counters = s.counters.as<int32_t>();
   a.partials = s.partials.as<double>();
   a.x = static_cast<const X *>(x);
   a.y = static_cast<Y *>(y);
@@ -621,6 +803,11 @@ static void launch_spmv(svdh_op *op, int direction, const void *vals, const void
   else
     hipLaunchKernelGGL((k_spmv<false, V, X, Y>), s.total_blocks, kBlock, 0, c->stream, a);
   HIP_CHECK(hipGetLastError());
+  if (s.nslots > 0) {
+    hipLaunchKernelGGL((k_spmv_fin<Y>), (s.nslots + kBlock - 1) / kBlock, kBlock, 0, c->stream, s.nslots,
+                       s.counters.as<int32_t>(), (const double *)s.partials.p, static_cast<Y *>(y));
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 void spmv(svdh_op *op, int direction, int vdt, const void *x, void *y, int xdt) {
@@ -660,17 +847,18 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
                         void *Y_, int64_t ldy, int64_t l) {
   Ctx *c = op->ctx;
   CsrView v = op->view(direction);
-  Schedule &s = get_schedule(op, direction, 1);
+  Schedule &s = get_spmm_schedule(op, direction);
   if (v.nrows == 0 || l == 0) return;
   constexpr int NV = PanelVec<X>::N;
   if (ldx % NV || ldy % NV) fail(SVDH_ERR_ARG, "panel leading dimensions must be multiples of %d", NV);
-  // lanes per row: enough to cover l with NV columns each, at least 2, at most 64
+  // lanes per row group: enough to cover l with NV columns each, 2..64
   int lgr = 1;
   while ((1 << lgr) * NV < l && lgr < 6) ++lgr;
   const int rpb = kBlock >> lgr;
-  const int short_blocks = (s.bins[0].row_count + rpb - 1) / rpb;
+  const int nshort = s.bins[0].row_count;
+  const int blocks = s.total_blocks + (nshort + rpb - 1) / rpb;
   const int64_t ldp = (l + 7) / 8 * 8;
-  s.partials.ensure((size_t)std::max<int32_t>(s.nchunks, 1) * ldp * sizeof(Acc));
+  s.partials.ensure((size_t)std::max<int32_t>(s.nslots, 1) * ldp * sizeof(Acc));
   SpmmArgs<V, X, Y, Acc> a;
   a.rowptr = v.rowptr;
   a.colidx = v.colidx;
@@ -679,7 +867,6 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   a.colmap = v.colmap;
   a.rows = s.rows.as<int32_t>();
   a.chunks = s.chunks.as<int32_t>();
-  a.counters = s.counters.as<int32_t>();
   a.partials = s.partials.as<Acc>();
   a.x = static_cast<const X *>(X_);
   a.y = static_cast<Y *>(Y_);
@@ -687,17 +874,27 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   a.ldy = ldy;
   a.ldp = ldp;
   a.l = (int32_t)l;
-  a.row_count = s.bins[0].row_count;
-  a.long_block_begin = short_blocks;
-  const int blocks = short_blocks + s.nchunks;
+  a.nshort = nshort;
+  a.nblk = s.nblk;
+  a.chunk_blocks = s.total_blocks;
+  for (int x = 0; x < 8; ++x) {
+    a.blk_off[x] = s.blk_off[x];
+    a.blk_cnt[x] = x < s.nblk ? s.blk_cnt[x] : 0;
+  }
+  if (blocks == 0) return;
   const double bytes = (double)v.nnz * (sizeof(V) + 4) + (double)(v.nrows + 1) * 4 +
                        (double)v.ncols * l * sizeof(X) + (double)v.nrows * l * sizeof(Y);
-  KTimer t(c, KC_SPMM, bytes);
+  KTimer t(c, direction == 0 ? KC_SPMM : KC_SPMM_T, bytes);
   if (v.colmap)
     spmm_dispatch<true>(c, a, lgr, blocks);
   else
     spmm_dispatch<false>(c, a, lgr, blocks);
   HIP_CHECK(hipGetLastError());
+  if (s.nlong_rows > 0) {
+    hipLaunchKernelGGL((k_spmm_fin<Y, Acc>), s.nlong_rows, kBlock, 0, c->stream, s.counters.as<int32_t>(),
+                       (const Acc *)s.partials.p, ldp, static_cast<Y *>(Y_), ldy, (int)l);
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 void spmm(svdh_op *op, int direction, int vdt, const void *X, int64_t ldx, int xdt, void *Y,
@@ -782,6 +979,8 @@ svdh_op *csr_create(Ctx *c, int64_t nrows, int64_t ncols, int64_t nnz, const int
       b.alloc(std::max<size_t>(bytes, 16));
       if (bytes) HIP_CHECK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
     };
+    op->h_colidx = ci;
+    op->h_t_colidx = tci;
     up(op->rowptr, op->h_rowptr.data(), (nrows + 1) * sizeof(int32_t));
     up(op->colidx, ci.data(), nnz * sizeof(int32_t));
     up(op->vals[vdt], vals, nnz * vs);

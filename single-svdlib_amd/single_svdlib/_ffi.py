@@ -17,7 +17,7 @@ SVDH_F64, SVDH_F32, SVDH_BF16 = 0, 1, 2
 PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
 NORM_QR, NORM_LU, NORM_NONE = 0, 1, 2
 KEEP_F32, KEEP_BF16 = 0x1, 0x2
-KC_SPMV, KC_SPMM, KC_DENSE, KC_BLAS1, KC_GEMV = 0, 1, 2, 3, 4
+KC_SPMV, KC_SPMM, KC_DENSE, KC_BLAS1, KC_GEMV, KC_SPMM_T = 0, 1, 2, 3, 4, 5
 
 
 class Diagnostics_(C.Structure):
