@@ -3,8 +3,8 @@ workload at a top-50 relative residual <= 1e-5, with the SpMM roofline.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|bf16] [--no-cpu]
 
-A step is one whole solve (matrix already resident in HBM, results on the
-host): rank 50, oversample 10, QR normaliser, power iterations until the
+A step is one whole solve (matrix already resident in HBM, U and V^T left in
+HBM, S on the host; the host-output time is reported beside it): rank 50, oversample 10, QR normaliser, power iterations until the
 Rayleigh-Ritz residual estimate of the top 50 triplets is <= 1e-5. The final
 triplets are re-checked in FP64 on the CPU outside the timed region:
 max_i sqrt(|A v_i - s_i u_i|^2 + |A^T u_i - s_i v_i|^2) / s_i.
@@ -106,21 +106,31 @@ def main():
     log(f"uploaded in {time.perf_counter() - t0:.1f}s")
     prec = _ffi.PREC_BF16 if args.precision == "bf16" else _ffi.PREC_F32
 
-    def solve(tol, q, precision=prec, keep=False):
+    m, n = a.shape
+    k = min(RANK, m, n)
+    u_dev = _ffi.DeviceArray((m, k))
+    vt_dev = _ffi.DeviceArray((k, n))
+    s_host = np.zeros(k)
+
+    def solve(tol, q):
+        """One solve with U and V^T left in HBM (the measured step)."""
+        meta = _ffi.Result_()
+        _ffi.check(lib.svdh_rsvd_device(ctx, op._handle, RANK, OVERSAMPLE, q, _ffi.NORM_QR, 0, 42, 1, None, prec,
+                                        tol, args.max_iters, u_dev.ptr, k, _ffi.ptr(s_host), vt_dev.ptr, n,
+                                        C.byref(meta)))
+        return meta.power_iters_run, meta.residual_estimate
+
+    def solve_host(tol, q):
+        """The same solve returning U, S, V^T in host memory (PCIe-inclusive)."""
         res = _ffi.Result_()
-        _ffi.check(lib.svdh_rsvd(ctx, op._handle, RANK, OVERSAMPLE, q, _ffi.NORM_QR, 0, 42, 1, None, precision,
+        _ffi.check(lib.svdh_rsvd(ctx, op._handle, RANK, OVERSAMPLE, q, _ffi.NORM_QR, 0, 42, 1, None, prec,
                                  tol, args.max_iters, C.byref(res)))
-        if keep:
-            return res
-        out = (res.power_iters_run, res.residual_estimate)
-        lib.svdh_result_free(C.byref(res))
-        return out
+        return res
 
     for i in range(args.warmup):
         solve(TOL, 3)
         log(f"warmup {i} done")
     _ffi.check(lib.svdh_ctx_synchronize(ctx))
-    _ffi.check(lib.svdh_ctx_profile(ctx, 1))
     iters = []
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -129,15 +139,25 @@ def main():
     elapsed = time.perf_counter() - t0
     per_step = elapsed / args.steps
     log(f"timed {args.steps} solves: {per_step * 1e3:.1f} ms each, power iterations {iters}")
-    ms, n, by = C.c_double(), C.c_int64(), C.c_double()
+
+    # The same steps again with HIP events around every kernel launch on the
+    # library's stream: per-kernel averages for the roofline.
+    _ffi.check(lib.svdh_ctx_profile(ctx, 1))
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        solve(TOL, 3)
+    _ffi.check(lib.svdh_ctx_synchronize(ctx))
+    per_step_prof = (time.perf_counter() - t0) / args.steps
+    log(f"profiled {args.steps} solves: {per_step_prof * 1e3:.1f} ms each")
+    ms, nl, by = C.c_double(), C.c_int64(), C.c_double()
     kc = {}
-    for name, k in (("spmm_ax", _ffi.KC_SPMM), ("spmm_aty", _ffi.KC_SPMM_T), ("dense", _ffi.KC_DENSE),
-                    ("spmv", _ffi.KC_SPMV)):
-        _ffi.check(lib.svdh_ctx_kernel_time(ctx, k, C.byref(ms), C.byref(n), C.byref(by)))
-        kc[name] = {"ms_per_solve": ms.value / args.steps, "launches_per_solve": n.value / args.steps,
-                    "avg_launch_us": ms.value * 1e3 / max(n.value, 1),
+    for name, kk in (("spmm_ax", _ffi.KC_SPMM), ("spmm_aty", _ffi.KC_SPMM_T), ("dense", _ffi.KC_DENSE),
+                     ("spmv", _ffi.KC_SPMV)):
+        _ffi.check(lib.svdh_ctx_kernel_time(ctx, kk, C.byref(ms), C.byref(nl), C.byref(by)))
+        kc[name] = {"ms_per_solve": ms.value / args.steps, "launches_per_solve": nl.value / args.steps,
+                    "avg_launch_us": ms.value * 1e3 / max(nl.value, 1),
                     "gbs": by.value / (ms.value * 1e-3) / 1e9 if ms.value else 0.0,
-                    "_ms": ms.value, "_n": n.value, "_bytes": by.value}
+                    "_ms": ms.value, "_n": nl.value, "_bytes": by.value}
     # the dominant kernel: SpMM, both directions (A X and A^T Y) together
     sp_ms = kc["spmm_ax"]["_ms"] + kc["spmm_aty"]["_ms"]
     sp_n = kc["spmm_ax"]["_n"] + kc["spmm_aty"]["_n"]
@@ -151,10 +171,16 @@ def main():
     _ffi.check(lib.svdh_ctx_profile(ctx, 0))
 
     # correctness of the timed configuration (outside the timed region)
-    res = solve(TOL, 3, keep=True)
-    rec = sl.utils.result_to_svdrec(res, np.float32)
-    r_rel, r_norm = residual(a, rec.u, rec.s.astype(np.float64), rec.vt)
+    solve(TOL, 3)
+    r_rel, r_norm = residual(a, u_dev.numpy(), s_host.copy(), vt_dev.numpy())
     log(f"FP64 residual of the returned top-{RANK}: per-sigma {r_rel:.2e}, |A|-relative {r_norm:.2e}")
+
+    # PCIe-inclusive variant: results handed over in host memory
+    t0 = time.perf_counter()
+    for _ in range(3):
+        lib.svdh_result_free(C.byref(solve_host(TOL, 3)))
+    t_host = (time.perf_counter() - t0) / 3
+    log(f"host-output solve {t_host * 1e3:.1f} ms")
 
     # the configuration as specified (3 power iterations) for reference
     t0 = time.perf_counter()
@@ -162,7 +188,8 @@ def main():
     for _ in range(q3_runs):
         solve(0.0, 3)
     t_q3 = (time.perf_counter() - t0) / q3_runs
-    rec3 = sl.utils.result_to_svdrec(solve(0.0, 3, keep=True), np.float32)
+    res3 = solve_host(0.0, 3)
+    rec3 = sl.utils.result_to_svdrec(res3, np.float32)
     r3_rel, _ = residual(a, rec3.u, rec3.s.astype(np.float64), rec3.vt)
     log(f"q=3 solve {t_q3 * 1e3:.1f} ms, residual {r3_rel:.2e}")
 
@@ -201,6 +228,8 @@ def main():
         "cpu_baseline": cpu,
         "residual_fp64": {"per_sigma": r_rel, "norm_relative": r_norm},
         "kernels": kc,
+        "ms_per_step_profiled": per_step_prof * 1e3,
+        "host_output_solve_s": t_host,
         "q3_solve_s": t_q3,
         "q3_residual_fp64": r3_rel,
     }

@@ -197,6 +197,21 @@ int32_t svdh_rsvd(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, int64_t
                   uint64_t seed, int32_t has_seed, const double *omega, int32_t precision,
                   double tol, int64_t max_power_iters, svdh_result *out);
 
+/* svdh_rsvd with the singular vectors left in device memory, for callers
+ * that keep working on the GPU (and the throughput measurement, which
+ * excludes the PCIe hand-over). k = min(target_rank, nrows, ncols);
+ * u: device, m x k row-major with leading dimension ldu >= k;
+ * vt: device, k x n row-major with leading dimension ldvt >= n;
+ * s: host, k entries. meta receives d, the shapes, the diagnostics and the
+ * extra fields of svdh_result; its u, s and vt stay NULL. Same algorithm and
+ * arguments as svdh_rsvd (src/randomized/mod.rs:24-199). Returns after u and
+ * vt are written. */
+int32_t svdh_rsvd_device(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, int64_t n_oversamples,
+                         int64_t n_power_iters, int32_t normalizer, int32_t mean_center,
+                         uint64_t seed, int32_t has_seed, const double *omega, int32_t precision,
+                         double tol, int64_t max_power_iters, double *u, int64_t ldu, double *s,
+                         double *vt, int64_t ldvt, svdh_result *meta);
+
 void svdh_result_free(svdh_result *res);
 
 #ifdef __cplusplus

@@ -43,6 +43,19 @@ void DevBuf::release() {
 
 void Ctx::sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
 
+void *Ctx::upload(const void *host, size_t n) {
+  if (n > kRingSlot) fail(SVDH_ERR_ARG, "upload of %zu bytes exceeds the staging slot", n);
+  const int i = ring_next;
+  ring_next = (ring_next + 1) % kRing;
+  HIP_CHECK(hipEventSynchronize(ring_ev[i]));  // the slot's previous copy has left the host buffer
+  char *h = h_ring + (size_t)i * kRingSlot;
+  std::memcpy(h, host, n);
+  char *d = d_ring.as<char>() + (size_t)i * kRingSlot;
+  HIP_CHECK(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, stream));
+  HIP_CHECK(hipEventRecord(ring_ev[i], stream));
+  return d;
+}
+
 hipEvent_t Ctx::get_event() {
   if (!free_events.empty()) {
     hipEvent_t e = free_events.back();
@@ -90,7 +103,8 @@ int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t itera
                    double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out);
 int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_oversamples, int64_t n_power_iters,
                    int32_t normalizer, int32_t mean_center, uint64_t seed, int32_t has_seed, const double *omega,
-                   int32_t precision, double tol, int64_t max_power_iters, svdh_result *out);
+                   int32_t precision, double tol, int64_t max_power_iters, svdh_result *out,
+                   const RsvdDeviceOut *dev);
 
 template <typename F>
 static int32_t guarded(F &&f) {
@@ -151,7 +165,19 @@ int32_t svdh_ctx_create(int32_t device, svdh_ctx **out) {
       c->red_count.alloc(256);
       HIP_CHECK(hipMemset(c->red_count.p, 0, 256));
       c->stage.alloc(1 << 16);
+      HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_ring), Ctx::kRing * Ctx::kRingSlot,
+                              hipHostMallocDefault));
+      c->d_ring.alloc(Ctx::kRing * Ctx::kRingSlot);
+      for (int i = 0; i < Ctx::kRing; ++i) {
+        HIP_CHECK(hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(c->ring_ev[i], c->stream));
+      }
+      HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_small), Ctx::kSmall * sizeof(double),
+                              hipHostMallocDefault));
+      HIP_CHECK(hipEventCreateWithFlags(&c->small_ev, hipEventDisableTiming));
     } catch (...) {
+      if (c->h_ring) (void)hipHostFree(c->h_ring);
+      if (c->h_small) (void)hipHostFree(c->h_small);
       if (c->h_scal) (void)hipHostFree(c->h_scal);
       if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
@@ -171,6 +197,11 @@ int32_t svdh_ctx_destroy(svdh_ctx *ctx) {
       (void)hipEventDestroy(p.b);
     }
     for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+    for (int i = 0; i < Ctx::kRing; ++i)
+      if (ctx->ring_ev[i]) (void)hipEventDestroy(ctx->ring_ev[i]);
+    if (ctx->small_ev) (void)hipEventDestroy(ctx->small_ev);
+    if (ctx->h_ring) (void)hipHostFree(ctx->h_ring);
+    if (ctx->h_small) (void)hipHostFree(ctx->h_small);
     if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -395,10 +426,30 @@ int32_t svdh_rsvd(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, int64_t
     check_op(ctx, op);
     if (!out) fail(SVDH_ERR_ARG, "null result");
     rsvd_solve(ctx, const_cast<svdh_op *>(op), target_rank, n_oversamples, n_power_iters, normalizer, mean_center,
-               seed, has_seed, omega, precision, tol, max_power_iters, out);
+               seed, has_seed, omega, precision, tol, max_power_iters, out, nullptr);
   });
   if (st != SVDH_OK && out) svdh_result_free(out);
   return st;
+}
+
+int32_t svdh_rsvd_device(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, int64_t n_oversamples,
+                         int64_t n_power_iters, int32_t normalizer, int32_t mean_center, uint64_t seed,
+                         int32_t has_seed, const double *omega, int32_t precision, double tol,
+                         int64_t max_power_iters, double *u, int64_t ldu, double *s, double *vt, int64_t ldvt,
+                         svdh_result *meta) {
+  if (meta) std::memset(meta, 0, sizeof *meta);
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    if (!meta) fail(SVDH_ERR_ARG, "null result");
+    const int64_t k = std::max<int64_t>(0, std::min(target_rank, std::min(op->nrows, op->ncols)));
+    if (k > 0 && (!u || !s || !vt)) fail(SVDH_ERR_ARG, "null output buffer");
+    if (ldu < k || ldvt < op->ncols) fail(SVDH_ERR_ARG, "leading dimension too small (ldu %lld < %lld or ldvt %lld < %lld)",
+                                          (long long)ldu, (long long)k, (long long)ldvt, (long long)op->ncols);
+    RsvdDeviceOut dev{u, ldu, s, vt, ldvt};
+    rsvd_solve(ctx, const_cast<svdh_op *>(op), target_rank, n_oversamples, n_power_iters, normalizer, mean_center,
+               seed, has_seed, omega, precision, tol, max_power_iters, meta, &dev);
+  });
 }
 
 void svdh_result_free(svdh_result *res) {

@@ -64,6 +64,32 @@ struct Ctx {
   DevBuf red_count;
   // Small host-to-device matrices (staged, then read by one kernel).
   DevBuf stage;
+  // Device workspaces kept for the context's lifetime (panels of the
+  // randomized solver), grown on demand; slot meanings belong to the caller.
+  static constexpr int kWorkspaces = 10;
+  DevBuf ws[kWorkspaces];
+  void *workspace(int slot, size_t bytes) {
+    if (bytes > ws[slot].bytes) {
+      sync();  // queued kernels may still use the old buffer
+      ws[slot].alloc(bytes);
+    }
+    return ws[slot].p;
+  }
+  // Pinned ring for small host-to-device uploads: slot i is reused only after
+  // the event of its previous copy has completed.
+  static constexpr int kRing = 8;
+  static constexpr size_t kRingSlot = 2 * 128 * 128 * sizeof(double);
+  char *h_ring = nullptr;
+  DevBuf d_ring;
+  hipEvent_t ring_ev[kRing] = {};
+  int ring_next = 0;
+  // Pinned landing area for small device-to-host results (Gram, norms).
+  double *h_small = nullptr;
+  static constexpr size_t kSmall = 4 * 128 * 128;  // doubles
+  hipEvent_t small_ev = nullptr;
+  // Copy n bytes (<= kRingSlot) to the device, ordered on the stream; returns
+  // the device address, valid for kernels launched after this call.
+  void *upload(const void *host, size_t n);
   // Profiling.
   bool profile = false;
   struct Pending { hipEvent_t a, b; int cls; double bytes; };
@@ -169,6 +195,17 @@ void spmm(svdh_op *op, int direction, int vdt, const void *X, int64_t ldx, int x
           int64_t ldy, int ydt, int64_t l);
 void ensure_values(svdh_op *op, int dtype);
 
+// Device-resident outputs of the randomized solver: u (m x k, row-major,
+// leading dimension ldu) and vt (k x n, row-major, ldvt) in device memory,
+// s (k entries) in host memory.
+struct RsvdDeviceOut {
+  double *u;
+  int64_t ldu;
+  double *s;
+  double *vt;
+  int64_t ldvt;
+};
+
 // ---- vector / dense kernels (dense.hip) -----------------------------------
 // Scalar slots of Ctx::scal.
 enum Slot { S_A = 0, S_B = 1, S_C = 2, S_D = 3, S_E = 4, S_F = 5, S_NSLOT = 64 };
@@ -190,9 +227,15 @@ void basis_combine(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, c
                    int64_t d, double *C, int64_t ldc);
 // Gram G (l x l, FP64, row-major) = Y^T Y for row-major Y (m x l, ldy) of dtype ydt.
 void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, double *G_host);
-// Y (m x lo) <- Y (m x li) * M (li x lo, FP64 row-major, host), in place or to Z.
+// Asynchronous Gram: enqueues the kernels and the copy of G into the pinned
+// area c->h_small + off (doubles) and records c->small_ev; the caller waits
+// on the event before reading.
+void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, size_t off);
+// Z (m x lo) = Y (m x li) * M (li x lo, FP64 row-major, host), in place or to
+// another panel. Element (i, c) of Z is stored at Z[i * ldz + c * zcs]
+// (zcs = 1: row-major; ldz = 1, zcs = ld: the transposed, column-major store).
 void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, int ydt,
-                 const double *M_host, void *Z, int64_t ldz, int zdt);
+                 const double *M_host, void *Z, int64_t ldz, int zdt, int64_t zcs = 1);
 // nr2[c] = |Y1 M1[:, c] + Y2 M2[:, c]|^2 for c < k (FP64); Y1 (m x w) and
 // Y2 (m x w2, w2 <= w) are row-major panels with leading dimension ld,
 // M1 (w x k) and M2 (w2 x k) are host matrices.
@@ -207,6 +250,10 @@ void panel_convert(Ctx *c, int64_t m, int64_t l, const void *X, int64_t ldx, int
                    int64_t ldy, int ydt);
 // Transpose-copy between a host column-major panel and a device row-major panel.
 void panel_fill(Ctx *c, int64_t m, int64_t l, double v, void *Y, int64_t ldy, int ydt);
+// Rank-one correction Y (ry x w) -= scale * a s^T with s = X^T wt, X (rx x w).
+// wt and a are device FP64 vectors; nullptr stands for all ones.
+void panel_rank1_correct(Ctx *c, int64_t rx, int64_t w, const void *X, int64_t ldx, int xdt, const double *wt,
+                         int64_t ry, void *Y, int64_t ldy, int ydt, const double *a, double scale);
 
 // ---- host dense algebra (hostla.cpp) --------------------------------------
 double pythag(double a, double b, double eps);

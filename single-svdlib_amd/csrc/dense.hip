@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_gram_reduce(int nb, int64_t ll, cons
 // staged in LDS too when MLDS is set (li * lo entries of C).
 template <typename TY, typename TZ, typename C, bool MLDS>
 __global__ __launch_bounds__(kBlock) void k_apply(int64_t m, int li, int lo, const TY *Y, int64_t ldy, const double *M,
-                                                 TZ *Z, int64_t ldz) {
+                                                 TZ *Z, int64_t ldz, int64_t zcs) {
   extern __shared__ char smem[];
   C *s = reinterpret_cast<C *>(smem);  // li x (kTile + 1)
   C *sm = s + (size_t)li * (kTile + 1);  // li x lo (when MLDS)
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(int64_t m, int li, int lo, con
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int c = c0 + tc * 4 + q;
-          if (c < lo) std_<TZ>(Z, (r0 + r) * ldz + c, (double)acc[p][q]);
+          if (c < lo) std_<TZ>(Z, (r0 + r) * ldz + c * zcs, (double)acc[p][q]);
         }
     }
   }
@@ -524,8 +524,9 @@ void basis_combine(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, c
   HIP_CHECK(hipGetLastError());
 }
 
-void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, double *G_host) {
+void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, size_t off) {
   if (l > 128) fail(SVDH_ERR_ARG, "panel width %lld exceeds the Gram kernel limit 128", (long long)l);
+  if (off + (size_t)l * l > Ctx::kSmall) fail(SVDH_ERR_ARG, "Gram landing area too small");
   const int nb = grid_of(m, kTile, 512);
   const int64_t ll = l * l;
   const size_t need = (size_t)nb * ll * sizeof(double) + ll * sizeof(double);
@@ -544,48 +545,50 @@ void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, dou
     hipLaunchKernelGGL(k_gram_reduce, (int)((ll + 63) / 64), kBlock, 0, c->stream, nb, ll, (const double *)part, G);
     HIP_CHECK(hipGetLastError());
   }
-  HIP_CHECK(hipMemcpyAsync(G_host, G, ll * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->h_small + off, G, ll * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipEventRecord(c->small_ev, c->stream));
+}
+
+void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, double *G_host) {
+  gram_async(c, m, l, Y, ldy, ydt, 0);
+  HIP_CHECK(hipEventSynchronize(c->small_ev));
+  std::memcpy(G_host, c->h_small, (size_t)l * l * sizeof(double));
 }
 
 template <typename TY, typename TZ, typename C>
 static void apply_t(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, const double *M, void *Z,
-                    int64_t ldz) {
+                    int64_t ldz, int64_t zcs) {
   const int64_t blocks = (m + kTile - 1) / kTile;
   const size_t shm = (size_t)li * (kTile + 1) * sizeof(C);
   if (shm > 65536) fail(SVDH_ERR_ARG, "panel width %lld too wide for the apply kernel", (long long)li);
   const size_t shm_m = shm + (size_t)li * lo * sizeof(C);
   if (shm_m <= 65536)
     hipLaunchKernelGGL((k_apply<TY, TZ, C, true>), (int)blocks, kBlock, shm_m, c->stream, m, (int)li, (int)lo,
-                       (const TY *)Y, ldy, M, (TZ *)Z, ldz);
+                       (const TY *)Y, ldy, M, (TZ *)Z, ldz, zcs);
   else
     hipLaunchKernelGGL((k_apply<TY, TZ, C, false>), (int)blocks, kBlock, shm, c->stream, m, (int)li, (int)lo,
-                       (const TY *)Y, ldy, M, (TZ *)Z, ldz);
+                       (const TY *)Y, ldy, M, (TZ *)Z, ldz, zcs);
   HIP_CHECK(hipGetLastError());
 }
 
 void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, int ydt,
-                 const double *M_host, void *Z, int64_t ldz, int zdt) {
+                 const double *M_host, void *Z, int64_t ldz, int zdt, int64_t zcs) {
   if (m == 0 || lo == 0) return;
   const size_t mb = (size_t)li * lo * sizeof(double);
-  c->stage.ensure(mb);
-  double *Md = c->stage.as<double>();
-  HIP_CHECK(hipMemcpyAsync(Md, M_host, mb, hipMemcpyHostToDevice, c->stream));
+  const double *Md = static_cast<const double *>(c->upload(M_host, mb));
   KTimer t(c, KC_DENSE, (double)m * (li * dtype_size(ydt) + lo * dtype_size(zdt)));
   if (ydt == SVDH_F64 && zdt == SVDH_F64)
-    apply_t<double, double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<double, double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else if (ydt == SVDH_F32 && zdt == SVDH_F32)
-    apply_t<float, float, float>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<float, float, float>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else if (ydt == SVDH_F32 && zdt == SVDH_F64)
-    apply_t<float, double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<float, double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else if (ydt == SVDH_F64 && zdt == SVDH_F32)
-    apply_t<double, float, double>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<double, float, double>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else if (ydt == SVDH_F32 && zdt == SVDH_BF16)
-    apply_t<float, bf16_t, float>(c, m, li, lo, Y, ldy, Md, Z, ldz);
+    apply_t<float, bf16_t, float>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else
     fail(SVDH_ERR_ARG, "unsupported apply dtypes %d -> %d", ydt, zdt);
-  // The staged matrix must stay valid until the kernel has read it.
-  HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
 void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const void *Y1, const void *Y2, int64_t ld,
@@ -593,12 +596,9 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
   if (w > 128 || k > 128 || w2 > w) fail(SVDH_ERR_ARG, "residual panel wider than 128");
   const int nb = grid_of(m, kTile, 1024);
   const size_t mb = (size_t)w * k * sizeof(double);
-  c->stage.ensure(2 * mb);
-  double *M1d = c->stage.as<double>();
-  double *M2d = M1d + w * k;
-  HIP_CHECK(hipMemcpyAsync(M1d, M1, mb, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(M2d, M2, (size_t)w2 * k * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * k * sizeof(double)));
+  const double *M1d = static_cast<const double *>(c->upload(M1, mb));
+  const double *M2d = static_cast<const double *>(c->upload(M2, (size_t)w2 * k * sizeof(double)));
+  c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)(nb + 1) * k * sizeof(double)));
   double *part = c->red_part.as<double>();
   {
     KTimer t(c, KC_DENSE, (double)m * 2 * w * dtype_size(dt));
@@ -624,14 +624,13 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
     }
     HIP_CHECK(hipGetLastError());
   }
-  std::vector<double> h((size_t)nb * k);
-  HIP_CHECK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  // column sums over the workgroup partials, on the device (same kernel as the Gram reduction)
+  double *sums = part + (size_t)nb * k;
+  hipLaunchKernelGGL(k_gram_reduce, (int)((k + 63) / 64), kBlock, 0, c->stream, nb, k, (const double *)part, sums);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(c->h_small, sums, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  for (int64_t r = 0; r < k; ++r) {
-    double t = 0.0;
-    for (int b = 0; b < nb; ++b) t += h[(size_t)b * k + r];
-    nr2_host[r] = t;
-  }
+  std::memcpy(nr2_host, c->h_small, k * sizeof(double));
 }
 
 void panel_normalize_columns(Ctx *c, int64_t m, int64_t l, void *Y, int64_t ldy, int ydt) {
@@ -663,6 +662,66 @@ void panel_normalize_columns(Ctx *c, int64_t m, int64_t l, void *Y, int64_t ldy,
     hipLaunchKernelGGL(k_colscale<float>, grid_of(m * l), kBlock, 0, c->stream, m, (int)l, (float *)Y, ldy, sd);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// Weighted column sums s_c = sum_i wt_i X[i, c] (wt == nullptr: all ones),
+// per-workgroup FP64 partials; w <= 128, thread t owns column t % 128 and
+// rows of phase t / 128, so each wavefront reads 64 consecutive columns.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_wcolsum(int64_t rows, int w, const T *X, int64_t ld, const double *wt,
+                                                   double *part) {
+  const int col = threadIdx.x & 127, ph = threadIdx.x >> 7;
+  double acc = 0.0;
+  if (col < w)
+    for (int64_t i = (int64_t)blockIdx.x * 2 + ph; i < rows; i += (int64_t)gridDim.x * 2) {
+      const double x = ldd<T>(X, i * ld + col);
+      acc += wt ? wt[i] * x : x;
+    }
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < 128 && col < w) part[(int64_t)blockIdx.x * w + col] = red[threadIdx.x] + red[threadIdx.x + 128];
+}
+
+// Y[i, c] -= scale * a_i * s_c (a == nullptr: all ones).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rank1(int64_t rows, int w, T *Y, int64_t ld, const double *a,
+                                                 const double *sv, double scale) {
+  const int64_t total = rows * w;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = e / w;
+    const int col = (int)(e - i * w);
+    const double ai = a ? a[i] : 1.0;
+    std_<T>(Y, i * ld + col, ldd<T>(Y, i * ld + col) - scale * ai * sv[col]);
+  }
+}
+
+void panel_rank1_correct(Ctx *c, int64_t rx, int64_t w, const void *X, int64_t ldx, int xdt, const double *wt,
+                         int64_t ry, void *Y, int64_t ldy, int ydt, const double *a, double scale) {
+  if (w > 128) fail(SVDH_ERR_ARG, "panel wider than 128");
+  if (w == 0) return;
+  const int nb = grid_of(rx, 2, 512);
+  c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)(nb + 1) * w * sizeof(double)));
+  double *part = c->red_part.as<double>();
+  double *sums = part + (size_t)nb * w;
+  KTimer t(c, KC_DENSE, (double)rx * w * dtype_size(xdt) + 2.0 * ry * w * dtype_size(ydt));
+  if (xdt == SVDH_F64)
+    hipLaunchKernelGGL(k_wcolsum<double>, nb, kBlock, 0, c->stream, rx, (int)w, (const double *)X, ldx, wt, part);
+  else if (xdt == SVDH_F32)
+    hipLaunchKernelGGL(k_wcolsum<float>, nb, kBlock, 0, c->stream, rx, (int)w, (const float *)X, ldx, wt, part);
+  else
+    fail(SVDH_ERR_ARG, "unsupported panel dtype %d", xdt);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_gram_reduce, (int)((w + 63) / 64), kBlock, 0, c->stream, nb, w, (const double *)part, sums);
+  HIP_CHECK(hipGetLastError());
+  const int g = grid_of(ry * w);
+  if (ydt == SVDH_F64)
+    hipLaunchKernelGGL(k_rank1<double>, g, kBlock, 0, c->stream, ry, (int)w, (double *)Y, ldy, a, sums, scale);
+  else if (ydt == SVDH_F32)
+    hipLaunchKernelGGL(k_rank1<float>, g, kBlock, 0, c->stream, ry, (int)w, (float *)Y, ldy, a, sums, scale);
+  else
+    fail(SVDH_ERR_ARG, "unsupported panel dtype %d", ydt);
+  HIP_CHECK(hipGetLastError());
 }
 
 void panel_gaussian(Ctx *c, int64_t m, int64_t l, uint64_t seed, void *Y, int64_t ldy, int ydt) {

@@ -42,19 +42,25 @@ namespace svdh {
 
 namespace {
 
+// Workspace slots of the context used by the solver.
+enum { WS_OMEGA = 0, WS_Y = 1, WS_Z = 2, WS_YN = 3, WS_U = 4, WS_VT = 5, WS_BF16 = 6, WS_ONES = 7, WS_SUMS = 8 };
+
 struct Panel {
-  DevBuf buf;
+  void *ptr = nullptr;
   int64_t rows = 0, w = 0, ld = 0;
   int dt = SVDH_F32;
-  void init(int64_t r, int64_t width, int dtype) {
+  // Row-major rows x width panel in workspace slot `slot`, cleared on the
+  // stream (the padding columns past width stay zero through every kernel).
+  void init(Ctx *c, int slot, int64_t r, int64_t width, int dtype) {
     rows = r;
     w = width;
     dt = dtype;
     ld = (width + 7) / 8 * 8;
-    buf.alloc(std::max<size_t>((size_t)r * ld * dtype_size(dtype), 64));
-    HIP_CHECK(hipMemset(buf.p, 0, buf.bytes));
+    const size_t bytes = std::max<size_t>((size_t)r * ld * dtype_size(dtype), 64);
+    ptr = c->workspace(slot, bytes);
+    HIP_CHECK(hipMemsetAsync(ptr, 0, bytes, c->stream));
   }
-  void *p() const { return buf.p; }
+  void *p() const { return ptr; }
 };
 
 // Eigen-decomposition of a symmetric matrix sorted by descending eigenvalue.
@@ -76,25 +82,49 @@ struct Rsvd {
   int vdt;     // sparse values of the FP32/FP64 passes
   bool bf16;   // BF16 ladder available
   int64_t fallbacks = 0;
-  DevBuf bf_in;  // BF16 copy of an input panel
+  // mean_center: column sums of A on the device (mu = sums * inv_m), or null.
+  const double *sums = nullptr;
+  double inv_m = 0.0;
 
   // Y = op_dir X; uses BF16 values and a BF16 copy of X when low is set.
   void mult(int dir, const Panel &X, Panel &Y, bool low) {
     Y.w = X.w;
     if (low) {
-      bf_in.ensure((size_t)X.rows * X.ld * 2);
-      panel_convert(c, X.rows, X.w, X.p(), X.ld, X.dt, bf_in.p, X.ld, SVDH_BF16);
-      spmm(op, dir, SVDH_BF16, bf_in.p, X.ld, SVDH_BF16, Y.p(), Y.ld, Y.dt, X.w);
+      void *bf_in = c->workspace(WS_BF16, (size_t)X.rows * X.ld * 2);
+      panel_convert(c, X.rows, X.w, X.p(), X.ld, X.dt, bf_in, X.ld, SVDH_BF16);
+      spmm(op, dir, SVDH_BF16, bf_in, X.ld, SVDH_BF16, Y.p(), Y.ld, Y.dt, X.w);
     } else {
       spmm(op, dir, vdt, X.p(), X.ld, X.dt, Y.p(), Y.ld, Y.dt, X.w);
     }
+    // Centered products as exact rank-one corrections of the sparse ones
+    // (the reference's multiply_*_centered, randomized/mod.rs:440-469):
+    //   (A - 1 mu^T) X = A X - 1 (mu^T X),  (A - 1 mu^T)^T Y = A^T Y - mu (1^T Y).
+    if (sums) {
+      if (dir == 0)
+        panel_rank1_correct(c, X.rows, X.w, X.p(), X.ld, X.dt, sums, Y.rows, Y.p(), Y.ld, Y.dt, nullptr, inv_m);
+      else
+        panel_rank1_correct(c, X.rows, X.w, X.p(), X.ld, X.dt, nullptr, Y.rows, Y.p(), Y.ld, Y.dt, sums, inv_m);
+    }
+  }
+
+  // Waits for the Gram enqueued by gram_async and copies it out.
+  void gram_wait(int64_t w, std::vector<double> &G) {
+    HIP_CHECK(hipEventSynchronize(c->small_ev));
+    G.assign(c->h_small, c->h_small + w * w);
   }
 
   // M (w x wo, row-major) with P M orthonormal, from the FP64 Gram of P.
   void normaliser(const Panel &P, std::vector<double> &M, int64_t &wo) {
+    gram_async(c, P.rows, P.w, P.p(), P.ld, P.dt, 0);
+    finish_normaliser(P, M, wo);
+  }
+
+  // Second half of normaliser: the Gram of P was enqueued by gram_async, and
+  // the stream may have more work queued behind it.
+  void finish_normaliser(const Panel &P, std::vector<double> &M, int64_t &wo) {
     const int64_t w = P.w;
-    std::vector<double> G(w * w), R(w * w);
-    gram(c, P.rows, w, P.p(), P.ld, P.dt, G.data());
+    std::vector<double> G, R(w * w);
+    gram_wait(w, G);
     const int64_t target = std::min(P.rows, w);
     if (target == w && cholesky_upper(w, G.data(), R.data())) {
       M.assign(w * w, 0.0);
@@ -129,10 +159,10 @@ struct Rsvd {
 
 int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_oversamples, int64_t n_power_iters,
                    int32_t normalizer, int32_t mean_center, uint64_t seed, int32_t has_seed, const double *omega,
-                   int32_t precision, double tol, int64_t max_power_iters, svdh_result *out) {
+                   int32_t precision, double tol, int64_t max_power_iters, svdh_result *out,
+                   const RsvdDeviceOut *dev) {
   if (normalizer < 0 || normalizer > 2) fail(SVDH_ERR_ARG, "bad normalizer %d", normalizer);
   if (precision < 0 || precision > 2) fail(SVDH_ERR_ARG, "bad precision %d", precision);
-  if (mean_center) fail(SVDH_ERR_ARG, "mean_center is not supported by this build");
   if (target_rank < 0 || n_oversamples < 0 || n_power_iters < 0) fail(SVDH_ERR_ARG, "negative argument");
   const int64_t m = op->nrows, n = op->ncols;
   if (m == 0 || n == 0) fail(SVDH_ERR_SHAPE, "empty matrix");
@@ -152,25 +182,34 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
   R.vdt = pdt;
   R.bf16 = precision == SVDH_PREC_BF16;
   const uint64_t s = has_seed ? seed : 0;
+  if (mean_center) {
+    // column means = A^T 1 / m (randomized/mod.rs:46-53, utils.rs:27)
+    double *ones = static_cast<double *>(ctx->workspace(WS_ONES, (size_t)m * sizeof(double)));
+    double *sums = static_cast<double *>(ctx->workspace(WS_SUMS, (size_t)n * sizeof(double)));
+    panel_fill(ctx, m, 1, 1.0, ones, 1, SVDH_F64);
+    spmv(op, 1, pdt, ones, sums, SVDH_F64);
+    R.sums = sums;
+    R.inv_m = 1.0 / (double)m;
+  }
 
   Panel Y, Z;
   {
     Panel Om;
-    Om.init(n, l, pdt);
+    Om.init(ctx, WS_OMEGA, n, l, pdt);
     if (omega) {
       // column-major n x l FP64 -> row-major panel
       std::vector<double> h((size_t)n * Om.ld, 0.0);
       for (int64_t j = 0; j < l; ++j)
         for (int64_t i = 0; i < n; ++i) h[i * Om.ld + j] = omega[i + j * n];
-      DevBuf tmp(h.size() * sizeof(double));
+      DevBuf tmp(h.size() * sizeof(double));  // freed after the synchronisation below
       HIP_CHECK(hipMemcpyAsync(tmp.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
       panel_convert(ctx, n, l, tmp.p, Om.ld, SVDH_F64, Om.p(), Om.ld, pdt);
       HIP_CHECK(hipStreamSynchronize(ctx->stream));
     } else {
       panel_gaussian(ctx, n, l, s, Om.p(), Om.ld, pdt);
     }
-    Y.init(m, l, pdt);
-    Z.init(n, l, pdt);
+    Y.init(ctx, WS_Y, m, l, pdt);
+    Z.init(ctx, WS_Z, n, l, pdt);
     // BF16 ladder: the power steps use BF16 values and BF16 copies of the
     // input panels until the residual estimate (adaptive) drops below
     // 1e-2, or for the first half of a fixed count; the final B^T = A^T Q and
@@ -185,28 +224,34 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
   int64_t wq = Y.w;
 
   if (normalizer == SVDH_NORM_QR) {
-    R.normaliser(Y, Mq, wq);
+    // Pipelined: the Gram of Y is enqueued, then A^T Y (which does not depend
+    // on the normaliser), and only then does the host wait for the Gram and
+    // factor it while the SpMM runs.
+    if (!adaptive && R.bf16) low = 0 < (n_power_iters + 1) / 2;
+    gram_async(ctx, Y.rows, Y.w, Y.p(), Y.ld, Y.dt, 0);
+    R.mult(1, Y, Z, low && !(!adaptive && 0 >= n_power_iters));
+    R.finish_normaliser(Y, Mq, wq);
     Panel Yn;
-    Yn.init(m, l, pdt);
+    Yn.init(ctx, WS_YN, m, l, pdt);
     int64_t next_check = n_power_iters;
     int64_t last_check_it = -1;
     double last_est = 0.0;
     for (;;) {
-      if (!adaptive && R.bf16) low = iters_run < (n_power_iters + 1) / 2;
       const bool final_fixed = !adaptive && iters_run >= n_power_iters;
-      // Z = A^T Q = (A^T Y) Mq
-      R.mult(1, Y, Z, low && !final_fixed);
+      // Z holds A^T Y; Z = A^T Q = (A^T Y) Mq
       panel_apply(ctx, n, Y.w, wq, Z.p(), Z.ld, Z.dt, Mq.data(), Z.p(), Z.ld, Z.dt);
       Z.w = wq;
       const bool check = adaptive && (iters_run >= next_check || iters_run >= max_power_iters);
-      if (final_fixed || check) {
-        GZ.assign(wq * wq, 0.0);
-        gram(ctx, n, wq, Z.p(), Z.ld, Z.dt, GZ.data());
+      if (final_fixed || check) gram_async(ctx, n, wq, Z.p(), Z.ld, Z.dt, 0);
+      if (final_fixed) {
+        R.gram_wait(wq, GZ);
         desc_eig(wq, GZ, lam, W);
+        break;
       }
-      if (final_fixed) break;
       R.mult(0, Z, Yn, low);  // A Z
       if (check) {
+        R.gram_wait(wq, GZ);
+        desc_eig(wq, GZ, lam, W);
         const int64_t k = std::min(target_rank, std::min(wq, n));
         // r_i = (A Z) w_i / s_i - Y (Mq w_i s_i)
         std::vector<double> M1(wq * k, 0.0), M2(Y.w * k, 0.0), nr2(k);
@@ -240,12 +285,16 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         last_est = est;
         next_check = iters_run + step;
       }
-      // Next subspace: Y <- A Z and its normaliser.
-      std::swap(Y.buf, Yn.buf);
+      // Next subspace: Y <- A Z, its Gram, A^T Y, then its normaliser.
+      std::swap(Y.ptr, Yn.ptr);
       std::swap(Y.ld, Yn.ld);
       Y.w = Z.w;
-      R.normaliser(Y, Mq, wq);
       ++iters_run;
+      if (!adaptive && R.bf16) low = iters_run < (n_power_iters + 1) / 2;
+      const bool next_final = !adaptive && iters_run >= n_power_iters;
+      gram_async(ctx, Y.rows, Y.w, Y.p(), Y.ld, Y.dt, 0);
+      R.mult(1, Y, Z, low && !next_final);
+      R.finish_normaliser(Y, Mq, wq);
     }
   } else {
     for (int64_t i = 0; i < n_power_iters; ++i) {
@@ -279,30 +328,33 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       MW[i * k + r] = t;
     }
   }
-  Panel U, V;
-  U.init(m, k, SVDH_F64);
-  V.init(n, k, SVDH_F64);
-  panel_apply(ctx, m, Y.w, k, Y.p(), Y.ld, Y.dt, MW.data(), U.p(), U.ld, SVDH_F64);
-  panel_apply(ctx, n, wq, k, Z.p(), Z.ld, Z.dt, Ws.data(), V.p(), V.ld, SVDH_F64);
-  std::vector<double> hu((size_t)m * U.ld), hv((size_t)n * V.ld);
-  HIP_CHECK(hipMemcpyAsync(hu.data(), U.p(), hu.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_CHECK(hipMemcpyAsync(hv.data(), V.p(), hv.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  // U = Y (Mq W), row-major m x k; V^T = (Z W S^-1)^T, row-major k x n, written
+  // by the transposed store of the apply kernel.
+  double *ud = dev ? dev->u : static_cast<double *>(ctx->workspace(WS_U, std::max<size_t>(m * k, 1) * 8));
+  double *vtd = dev ? dev->vt : static_cast<double *>(ctx->workspace(WS_VT, std::max<size_t>(k * n, 1) * 8));
+  const int64_t ldu = dev ? dev->ldu : k, ldvt = dev ? dev->ldvt : n;
+  panel_apply(ctx, m, Y.w, k, Y.p(), Y.ld, Y.dt, MW.data(), ud, ldu, SVDH_F64);
+  panel_apply(ctx, n, wq, k, Z.p(), Z.ld, Z.dt, Ws.data(), vtd, 1, SVDH_F64, ldvt);
 
   out->d = k;
   out->u_rows = m;
   out->u_cols = k;
   out->vt_rows = k;
   out->vt_cols = n;
-  out->s = static_cast<double *>(std::calloc(std::max<int64_t>(k, 1), sizeof(double)));
-  out->u = static_cast<double *>(std::calloc(std::max<int64_t>(m * k, 1), sizeof(double)));
-  out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(k * n, 1), sizeof(double)));
-  if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
-  for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
-  for (int64_t i = 0; i < m; ++i)
-    for (int64_t r = 0; r < k; ++r) out->u[i * k + r] = hu[i * U.ld + r];
-  for (int64_t r = 0; r < k; ++r)
-    for (int64_t i = 0; i < n; ++i) out->vt[r * n + i] = hv[i * V.ld + r];
+  if (dev) {
+    for (int64_t r = 0; r < k; ++r) dev->s[r] = sv[r];
+  } else {
+    out->s = static_cast<double *>(std::calloc(std::max<int64_t>(k, 1), sizeof(double)));
+    out->u = static_cast<double *>(std::calloc(std::max<int64_t>(m * k, 1), sizeof(double)));
+    out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(k * n, 1), sizeof(double)));
+    if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
+    for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
+    if (k > 0) {
+      HIP_CHECK(hipMemcpyAsync(out->u, ud, (size_t)m * k * 8, hipMemcpyDeviceToHost, ctx->stream));
+      HIP_CHECK(hipMemcpyAsync(out->vt, vtd, (size_t)k * n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(ctx->stream));
 
   svdh_diagnostics &dg = out->diagnostics;  // randomized/mod.rs:214-237
   dg.non_zero = op->nnz;

@@ -8,6 +8,8 @@ import ctypes as C
 import os
 import threading
 
+import numpy as np
+
 from . import error
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -80,6 +82,8 @@ SIGNATURES = [
     ("svdh_las2_start_vector", _i32, [C.c_uint32, _i64, _f64p]),
     ("svdh_rsvd", _i32, [_vp, _vp, _i64, _i64, _i64, _i32, _i32, C.c_uint64, _i32, _f64p, _i32, C.c_double, _i64,
                          C.POINTER(Result_)]),
+    ("svdh_rsvd_device", _i32, [_vp, _vp, _i64, _i64, _i64, _i32, _i32, C.c_uint64, _i32, _f64p, _i32, C.c_double,
+                                _i64, _vp, _i64, _f64p, _vp, _i64, C.POINTER(Result_)]),
     ("svdh_result_free", None, [C.POINTER(Result_)]),
 ]
 
@@ -126,3 +130,62 @@ def context(device=0):
 
 def ptr(a, t=_f64p):
     return a.ctypes.data_as(t)
+
+
+_hip = None
+
+
+def hip():
+    """The HIP runtime libsvdh.so is linked against (libamdhip64.so.7 of the
+    ROCm install). A second HIP runtime in the process (PyTorch ships its own)
+    does not see the GPU once this one holds it, so device buffers for the
+    device-output entry points come from here."""
+    global _hip
+    if _hip is None:
+        lib()
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(_vp), C.c_size_t]
+        h.hipFree.argtypes = [_vp]
+        h.hipMemcpy.argtypes = [_vp, _vp, C.c_size_t, C.c_int]
+        h.hipMemset.argtypes = [_vp, C.c_int, C.c_size_t]
+        for f in (h.hipMalloc, h.hipFree, h.hipMemcpy, h.hipMemset):
+            f.restype = C.c_int
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """A C-contiguous FP64/FP32 array in device memory (hipMalloc)."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(d) for d in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = max(int(np.prod(self.shape)) * self.dtype.itemsize, 16)
+        p = _vp()
+        st = hip().hipMalloc(C.byref(p), self.nbytes)
+        if st != 0:
+            raise MemoryError(f"hipMalloc({self.nbytes}) failed with status {st}")
+        self.ptr = p
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype).reshape(self.shape)
+        if hip().hipMemcpy(self.ptr, a.ctypes.data_as(_vp), a.nbytes, 1) != 0:
+            raise RuntimeError("hipMemcpy host->device failed")
+        return self
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        if hip().hipMemcpy(out.ctypes.data_as(_vp), self.ptr, out.nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy device->host failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

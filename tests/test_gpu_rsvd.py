@@ -123,3 +123,59 @@ def test_rank_deficient_and_tiny(lib):
                                         False)
     s_true = np.linalg.svd(dense, compute_uv=False)[:3]
     assert np.allclose(rec.s, s_true, rtol=1e-8)
+
+
+def test_device_output_matches_host_output(lib):
+    # svdh_rsvd_device leaves U and V^T in HBM at caller-chosen leading
+    # dimensions; the solve is deterministic, so it equals svdh_rsvd bit for bit.
+    import ctypes as C
+    from single_svdlib import _ffi
+    a = random_csr(4000, 900, 0.01, seed=12)
+    op = lib.CsrMatrix.from_scipy(a)
+    k, ldu, ldvt = 20, 23, 905
+    ref = lib.randomized.randomized_svd(op, k, 10, 2, "QR", False, 9, True)
+    u = _ffi.DeviceArray((4000, ldu)).upload(np.full((4000, ldu), 7.0))
+    vt = _ffi.DeviceArray((k, ldvt)).upload(np.full((k, ldvt), 7.0))
+    s = np.zeros(k)
+    meta = _ffi.Result_()
+
+    def call(ld_u):
+        return _ffi.lib().svdh_rsvd_device(_ffi.context(0), op._handle, k, 10, 2, _ffi.NORM_QR, 0, 9, 1, None,
+                                           _ffi.PREC_F64, 0.0, 0, u.ptr, ld_u, _ffi.ptr(s), vt.ptr, ldvt,
+                                           C.byref(meta))
+    _ffi.check(call(ldu))
+    assert meta.d == k and not meta.u and not meta.vt
+    uh, vth = u.numpy(), vt.numpy()
+    assert np.array_equal(s, ref.s)
+    assert np.array_equal(uh[:, :k], ref.u) and np.all(uh[:, k:] == 7.0)
+    assert np.array_equal(vth[:, :900], ref.vt) and np.all(vth[:, 900:] == 7.0)
+    with pytest.raises(ValueError, match="leading dimension"):
+        _ffi.check(call(k - 1))
+
+
+@pytest.mark.parametrize("q", [0, 2])
+def test_mean_center_matches_oracle(lib, q):
+    # randomized/mod.rs:46-53 + 440-469: products with A - 1 mu^T, the column
+    # means folded in as rank-one corrections on the device.
+    a = random_csr(2500, 700, 0.02, seed=21)
+    a.data += 1.0  # non-zero means
+    om = np.random.default_rng(4).normal(size=(700, 25))
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a), 15, 10, q, "QR", True, 3, False, omega=om)
+    u, s, vt = oracle.randomized_svd(oracle.OracleOp(a), 15, 10, q, "QR", mean_center=True, omega=om)
+    assert np.allclose(rec.s, s, rtol=1e-10, atol=0), np.max(np.abs(rec.s - s) / s)
+    assert max_principal_angle(rec.u[:, :8], u[:, :8]) < 1e-8
+    assert max_principal_angle(rec.vt[:8].T, vt[:8].T) < 1e-8
+
+
+def test_mean_center_adaptive_reaches_centered_svd(lib):
+    # the adaptive solver on the centered operator converges to the dense
+    # SVD of A - 1 mu^T (numpy LAPACK), in FP32 and FP64
+    a = lowrank_plus_noise(1500, 400, 12, seed=8, noise=1e-2)
+    a.data += 0.5
+    dense = a.toarray()
+    s_true = np.linalg.svd(dense - dense.mean(axis=0), compute_uv=False)[:10]
+    for dt, tol in ((np.float64, 1e-9), (np.float32, 1e-5)):
+        rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a.astype(dt)), 10, 10, 0, "QR", True, 1, False,
+                                            tol=1e-7 if dt == np.float64 else 1e-5)
+        assert np.allclose(rec.s, s_true, rtol=tol * 10), np.max(np.abs(rec.s - s_true) / s_true)
+        assert triplet_residual(dense - dense.mean(axis=0), rec.u, rec.s, rec.vt) < 1e-4
