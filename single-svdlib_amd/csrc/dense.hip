@@ -211,71 +211,165 @@ __device__ __forceinline__ void load_tile(const T *Y, int64_t ldy, int64_t r0, i
   }
 }
 
-// Gram partials (FP64 accumulation): workgroup b accumulates Y^T Y over row
-// tiles b, b + grid, ... into part[b * l * l], one 64 x 64 output block
-// (c0, d0) at a time (d0 >= c0, mirrored). Tiles are row-major in LDS in the
-// storage type S (FP32 for FP32/BF16 panels: products of two FP32 values are
-// exact in FP64), so both operands of a thread are read along one row.
-template <typename T, typename S>
-__global__ __launch_bounds__(kBlock) void k_gram_part(int64_t m, int l, const T *Y, int64_t ldy, double *part) {
-  __shared__ S sa[kTile * kTile];
-  __shared__ S sb[kTile * kTile];
-  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-  for (int c0 = 0; c0 < l; c0 += kTile)
-    for (int d0 = c0; d0 < l; d0 += kTile) {
-      const int wc = min(kTile, l - c0), wd = min(kTile, l - d0);
-      const bool same = c0 == d0;
-      const S *sbp = same ? sa : sb;
-      double acc[4][4] = {};
-      for (int64_t r0 = (int64_t)blockIdx.x * kTile; r0 < m; r0 += (int64_t)gridDim.x * kTile) {
-        const int rows = (int)min<int64_t>(kTile, m - r0);
-        __syncthreads();
-        for (int e = threadIdx.x; e < kTile * kTile; e += kBlock) {
-          const int r = e / kTile, k = e % kTile;
-          const bool ok = r < rows;
-          sa[e] = (ok && k < wc) ? (S)ldd<T>(Y, (r0 + r) * ldy + c0 + k) : S(0);
-          if (!same) sb[e] = (ok && k < wd) ? (S)ldd<T>(Y, (r0 + r) * ldy + d0 + k) : S(0);
-        }
-        __syncthreads();
-        for (int r = 0; r < rows; ++r) {
-          double a4[4], b4[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            a4[q] = (double)sa[r * kTile + tr * 4 + q];
-            b4[q] = (double)sbp[r * kTile + tc * 4 + q];
-          }
-#pragma unroll
-          for (int p = 0; p < 4; ++p)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[p][q] += a4[p] * b4[q];
-        }
-      }
-      double *out = part + (int64_t)blockIdx.x * l * l;
-#pragma unroll
-      for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int i = c0 + tr * 4 + p, j = d0 + tc * 4 + q;
-          if (i < l && j < l) {
-            out[(int64_t)i * l + j] = acc[p][q];
-            out[(int64_t)j * l + i] = acc[p][q];
-          }
-        }
+// G[e] = sum over the nb partials part[b * ll + e]. 16 outputs per
+// workgroup, 16 lanes per output each summing every 16th partial (two
+// chains), then the 16 lane sums in order: a fixed order for every launch.
+__global__ __launch_bounds__(kBlock) void k_gram_reduce(int nb, int64_t ll, const double *part, double *G) {
+  __shared__ double red[16][17];
+  const int o = threadIdx.x & 15, q = threadIdx.x >> 4;
+  const int64_t e = blockIdx.x * (int64_t)16 + o;
+  double s0 = 0.0, s1 = 0.0;
+  if (e < ll) {
+    int b = q;
+    for (; b + 16 < nb; b += 32) {
+      s0 += part[(int64_t)b * ll + e];
+      s1 += part[(int64_t)(b + 16) * ll + e];
     }
+    if (b < nb) s0 += part[(int64_t)b * ll + e];
+  }
+  red[q][o] = s0 + s1;
+  __syncthreads();
+  if (q == 0 && e < ll) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) t += red[i][o];
+    G[e] = t;
+  }
 }
 
-// G[e] = sum over partials in order; 64 outputs per workgroup, 4 lanes per
-// output each summing every fourth partial, then the 4 lanes in order.
-__global__ __launch_bounds__(kBlock) void k_gram_reduce(int nb, int64_t ll, const double *part, double *G) {
-  __shared__ double red[4][64];
-  const int o = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int64_t e = blockIdx.x * (int64_t)64 + o;
-  double s = 0.0;
-  if (e < ll)
-    for (int b = q; b < nb; b += 4) s += part[(int64_t)b * ll + e];
-  red[q][o] = s;
-  __syncthreads();
-  if (q == 0 && e < ll) G[e] = ((red[0][o] + red[1][o]) + red[2][o]) + red[3][o];
+static int reduce_grid(int64_t ll) { return (int)((ll + 15) / 16); }
+
+// Gram partials on the FP64 matrix cores. v_mfma_f64_16x16x4 takes, per lane
+// l, A[l % 16][l / 16] and B[l / 16][l % 16] and returns D[l / 16 + 4 v][l % 16]
+// in element v (this lane layout was checked on gfx950 with one-hot inputs).
+// A 64-column group c0.. is split into 4 strided column blocks
+// b = {c0 + 4 t + b : t < 16}, so lane (k, t) = (l / 16, l % 16) reads the 4
+// consecutive columns c0 + 4 t .. c0 + 4 t + 3 of row 4 s + k with one vector
+// load and feeds all 4 blocks; the MFMA of blocks (b1, b2) adds
+// G[c0 + 4 i + b1][d0 + 4 j + b2] += sum_k Y[4 s + k][c0 + 4 i + b1] Y[4 s + k][d0 + 4 j + b2].
+// FP32 and BF16 inputs are exact in FP64, so this is the FP64 accumulation of
+// exact products. Each wave owns a contiguous range of 4-row steps; the waves
+// of a workgroup are summed in LDS in wave order and the workgroup partial is
+// written mirrored to part[blockIdx * l * l] (disjoint entries per launch
+// when l > 64 needs several (c0, d0) launches).
+typedef double d4_t __attribute__((ext_vector_type(4)));
+constexpr int kGramWaves = 8;
+
+template <typename T> struct Quad;
+template <> struct Quad<float> {
+  float4 v;
+  __device__ __forceinline__ void load(const float *p) { v = *reinterpret_cast<const float4 *>(p); }
+  __device__ __forceinline__ void zero() { v = make_float4(0.f, 0.f, 0.f, 0.f); }
+  __device__ __forceinline__ double get(int b) const { return (double)(b == 0 ? v.x : b == 1 ? v.y : b == 2 ? v.z : v.w); }
+};
+template <> struct Quad<double> {
+  double2 a, b2;
+  __device__ __forceinline__ void load(const double *p) {
+    a = reinterpret_cast<const double2 *>(p)[0];
+    b2 = reinterpret_cast<const double2 *>(p)[1];
+  }
+  __device__ __forceinline__ void zero() { a = make_double2(0.0, 0.0); b2 = a; }
+  __device__ __forceinline__ double get(int b) const { return b == 0 ? a.x : b == 1 ? a.y : b == 2 ? b2.x : b2.y; }
+};
+template <> struct Quad<bf16_t> {
+  uint2 v;
+  __device__ __forceinline__ void load(const bf16_t *p) { v = *reinterpret_cast<const uint2 *>(p); }
+  __device__ __forceinline__ void zero() { v = make_uint2(0u, 0u); }
+  __device__ __forceinline__ double get(int b) const {
+    const uint32_t w = b < 2 ? v.x : v.y;
+    return (double)__uint_as_float((b & 1) ? (w & 0xffff0000u) : (w << 16));
+  }
+};
+
+// Waves per workgroup: 8 for the diagonal launch (two waves per SIMD), 4 for
+// the off-diagonal one, whose 16 accumulators need the whole register file.
+template <bool SAME> constexpr int gram_waves() { return SAME ? kGramWaves : 4; }
+
+template <typename T, bool SAME>
+__global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m, int l, const T *Y, int64_t ldy,
+                                                                     int c0, int d0, double *part) {
+  constexpr int W = gram_waves<SAME>();
+  __shared__ double red[16 * 256];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int kr = lane >> 4, cc = lane & 15;
+  const int64_t nw = (int64_t)gridDim.x * W, gw = (int64_t)blockIdx.x * W + wv;
+  const int64_t steps = (m + 3) / 4;
+  const int64_t per = (steps + nw - 1) / nw;
+  const int64_t s0 = gw * per, s1 = min<int64_t>(steps, s0 + per);
+  const int colc = c0 + 4 * cc, cold = d0 + 4 * cc;
+  // columns at or past l read as zero; a quad starting below l stays inside
+  // the row's leading dimension (a multiple of 4 that is >= l)
+  const bool okc = colc < l, okd = cold < l;
+  const int limc = l - colc, limd = l - cold;
+  d4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+  constexpr int G = 4;  // row steps per group; the next group is loaded during this one's MFMAs
+  Quad<T> qc[G], qd[G], nc[G], nd[G];
+  auto load = [&](int64_t st0, Quad<T> (&xc)[G], Quad<T> (&xd)[G]) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int64_t r = (st0 + g) * 4 + kr;
+      const bool okr = st0 + g < s1 && r < m;
+      if (okr && okc) xc[g].load(Y + r * ldy + colc); else xc[g].zero();
+      if (!SAME) {
+        if (okr && okd) xd[g].load(Y + r * ldy + cold); else xd[g].zero();
+      }
+    }
+  };
+  auto mfmas = [&](const Quad<T> (&xc)[G], const Quad<T> (&xd)[G]) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      double a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = q < limc ? xc[g].get(q) : 0.0;
+        b[q] = SAME ? a[q] : (q < limd ? xd[g].get(q) : 0.0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = (SAME ? i : 0); j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  };
+  if (s0 < s1) load(s0, qc, qd);
+  for (int64_t st = s0; st < s1; st += 2 * G) {
+    const bool more = st + G < s1;
+    if (more) load(st + G, nc, nd);
+    mfmas(qc, qd);
+    if (!more) break;
+    if (st + 2 * G < s1) load(st + 2 * G, qc, qd);
+    mfmas(nc, nd);
+  }
+  // sum the waves in order through LDS: red[(i * 4 + j) * 256 + v * 64 + lane]
+  for (int w = 0; w < W; ++w) {
+    if (wv == w)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = (SAME ? i : 0); j < 4; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            double *q = red + (i * 4 + j) * 256 + v * 64 + lane;
+            *q = (w == 0 ? 0.0 : *q) + acc[i][j][v];
+          }
+    __syncthreads();
+  }
+  double *out = part + (int64_t)blockIdx.x * l * l;
+  for (int e = threadIdx.x; e < 16 * 256; e += 64 * W) {
+    const int blk = e >> 8, v = (e >> 6) & 3, ln = e & 63;
+    const int b1 = blk >> 2, b2 = blk & 3;
+    if (SAME && b2 < b1) continue;
+    const int gi = c0 + 4 * ((ln >> 4) + 4 * v) + b1, gj = d0 + 4 * (ln & 15) + b2;
+    if (gi < l && gj < l) {
+      const double x = red[e];
+      out[(int64_t)gi * l + gj] = x;
+      out[(int64_t)gj * l + gi] = x;
+    }
+  }
 }
 
 // Z[r, 0:lo] = Y[r, 0:li] M (li x lo, row-major) on 64-row tiles; the
@@ -524,10 +618,23 @@ void basis_combine(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, c
   HIP_CHECK(hipGetLastError());
 }
 
+template <typename T>
+static void launch_gram(Ctx *c, bool same, int nb, int64_t m, int l, const T *Y, int64_t ldy, int c0, int d0,
+                        double *part) {
+  if (same)
+    hipLaunchKernelGGL((k_gram_mfma<T, true>), nb, 64 * gram_waves<true>(), 0, c->stream, m, l, Y, ldy, c0, d0, part);
+  else
+    hipLaunchKernelGGL((k_gram_mfma<T, false>), nb, 64 * gram_waves<false>(), 0, c->stream, m, l, Y, ldy, c0, d0,
+                       part);
+  HIP_CHECK(hipGetLastError());
+}
+
 void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, size_t off) {
   if (l > 128) fail(SVDH_ERR_ARG, "panel width %lld exceeds the Gram kernel limit 128", (long long)l);
   if (off + (size_t)l * l > Ctx::kSmall) fail(SVDH_ERR_ARG, "Gram landing area too small");
-  const int nb = grid_of(m, kTile, 512);
+  if (ldy % 4 != 0) fail(SVDH_ERR_ARG, "Gram panel leading dimension %lld is not a multiple of 4", (long long)ldy);
+  // one workgroup per CU at most; each wave gets at least 8 row steps
+  const int nb = grid_of((m + 3) / 4, 8 * kGramWaves, 256);
   const int64_t ll = l * l;
   const size_t need = (size_t)nb * ll * sizeof(double) + ll * sizeof(double);
   c->red_part.ensure(std::max<size_t>(need, c->red_part.bytes));
@@ -535,14 +642,16 @@ void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int yd
   double *G = part + (int64_t)nb * ll;
   {
     KTimer t(c, KC_DENSE, (double)m * l * dtype_size(ydt));
-    if (ydt == SVDH_F64)
-      hipLaunchKernelGGL((k_gram_part<double, double>), nb, kBlock, 0, c->stream, m, (int)l, (const double *)Y, ldy, part);
-    else if (ydt == SVDH_F32)
-      hipLaunchKernelGGL((k_gram_part<float, float>), nb, kBlock, 0, c->stream, m, (int)l, (const float *)Y, ldy, part);
-    else
-      hipLaunchKernelGGL((k_gram_part<bf16_t, float>), nb, kBlock, 0, c->stream, m, (int)l, (const bf16_t *)Y, ldy, part);
-    HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_gram_reduce, (int)((ll + 63) / 64), kBlock, 0, c->stream, nb, ll, (const double *)part, G);
+    for (int c0 = 0; c0 < l; c0 += 64)
+      for (int d0 = c0; d0 < l; d0 += 64) {
+        if (ydt == SVDH_F64)
+          launch_gram<double>(c, c0 == d0, nb, m, (int)l, (const double *)Y, ldy, c0, d0, part);
+        else if (ydt == SVDH_F32)
+          launch_gram<float>(c, c0 == d0, nb, m, (int)l, (const float *)Y, ldy, c0, d0, part);
+        else
+          launch_gram<bf16_t>(c, c0 == d0, nb, m, (int)l, (const bf16_t *)Y, ldy, c0, d0, part);
+      }
+    hipLaunchKernelGGL(k_gram_reduce, reduce_grid(ll), kBlock, 0, c->stream, nb, ll, (const double *)part, G);
     HIP_CHECK(hipGetLastError());
   }
   HIP_CHECK(hipMemcpyAsync(c->h_small + off, G, ll * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -581,8 +690,10 @@ void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64
     apply_t<double, double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else if (ydt == SVDH_F32 && zdt == SVDH_F32)
     apply_t<float, float, float>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
-  else if (ydt == SVDH_F32 && zdt == SVDH_F64)
+  else if (ydt == SVDH_F32 && zdt == SVDH_F64 && li <= 64)
     apply_t<float, double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
+  else if (ydt == SVDH_F32 && zdt == SVDH_F64)  // FP64 tiles of > 64 columns exceed 64 KB of LDS
+    apply_t<float, double, float>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else if (ydt == SVDH_F64 && zdt == SVDH_F32)
     apply_t<double, float, double>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else if (ydt == SVDH_F32 && zdt == SVDH_BF16)
@@ -626,7 +737,7 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
   }
   // column sums over the workgroup partials, on the device (same kernel as the Gram reduction)
   double *sums = part + (size_t)nb * k;
-  hipLaunchKernelGGL(k_gram_reduce, (int)((k + 63) / 64), kBlock, 0, c->stream, nb, k, (const double *)part, sums);
+  hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, sums);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(c->h_small, sums, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -712,7 +823,7 @@ void panel_rank1_correct(Ctx *c, int64_t rx, int64_t w, const void *X, int64_t l
   else
     fail(SVDH_ERR_ARG, "unsupported panel dtype %d", xdt);
   HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_gram_reduce, (int)((w + 63) / 64), kBlock, 0, c->stream, nb, w, (const double *)part, sums);
+  hipLaunchKernelGGL(k_gram_reduce, reduce_grid(w), kBlock, 0, c->stream, nb, w, (const double *)part, sums);
   HIP_CHECK(hipGetLastError());
   const int g = grid_of(ry * w);
   if (ydt == SVDH_F64)

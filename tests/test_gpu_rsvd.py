@@ -179,3 +179,17 @@ def test_mean_center_adaptive_reaches_centered_svd(lib):
                                             tol=1e-7 if dt == np.float64 else 1e-5)
         assert np.allclose(rec.s, s_true, rtol=tol * 10), np.max(np.abs(rec.s - s_true) / s_true)
         assert triplet_residual(dense - dense.mean(axis=0), rec.u, rec.s, rec.vt) < 1e-4
+
+
+@pytest.mark.parametrize("rank,over", [(60, 20), (100, 28)])
+def test_wide_panels_match_oracle(lib, rank, over):
+    # l = rank + oversample above 64 exercises the multi-block Gram (the
+    # off-diagonal 64 x 64 column groups) and the wide apply/residual paths
+    a = random_csr(4000, 1500, 0.02, seed=31)
+    om = np.random.default_rng(6).normal(size=(1500, rank + over))
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a.astype(np.float32)), rank, over, 2, "QR", False, 5,
+                                        False, omega=om)
+    a64 = a.astype(np.float32).astype(np.float64)
+    u, s, vt = oracle.randomized_svd(oracle.OracleOp(a64), rank, over, 2, "QR", omega=om)
+    assert np.allclose(rec.s, s, rtol=2e-5, atol=0), np.max(np.abs(rec.s - s) / s)
+    assert max_principal_angle(rec.u[:, :5], u[:, :5]) < 1e-3
