@@ -109,7 +109,7 @@ int32_t svdh_ctx_synchronize(svdh_ctx *ctx);
 const char *svdh_last_error(void);
 /* Kernel timing with HIP events on the context's stream. Kernel classes:
  * 0 = SpMV, 1 = SpMM Y = A X, 2 = dense (Gram / apply / GEMM), 3 = BLAS-1,
- * 4 = basis GEMV (reorthogonalisation), 5 = SpMM Z = A^T Y. total_ms,
+ * 4 = basis GEMV (reorthogonalisation), 5 = SpMM Z = A^T Y, 6 = SDDMM. total_ms,
  * launches and the launches' algorithmic bytes accumulate until
  * svdh_ctx_profile is called again with enable = 1. */
 int32_t svdh_ctx_profile(svdh_ctx *ctx, int32_t enable);
@@ -211,6 +211,34 @@ int32_t svdh_rsvd_device(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, 
                          uint64_t seed, int32_t has_seed, const double *omega, int32_t precision,
                          double tol, int64_t max_power_iters, double *u, int64_t ldu, double *s,
                          double *vt, int64_t ldvt, svdh_result *meta);
+
+/* ---- adjoint (backward) ------------------------------------------------ */
+
+/* Losses of svdh_backward. No counterpart in the reference: the adjoint sits
+ * behind a backward() on SvdRec (utils.rs:43-50) in the Rust binding. */
+typedef enum {
+  SVDH_LOSS_SIGMA_SUM = 0,  /* L = sum_{i<k} s_i                 */
+  SVDH_LOSS_PROJECTION = 1  /* L = |U_k^T y|^2, y of length nrows */
+} svdh_loss;
+
+typedef struct {
+  double loss;                  /* L at the given triplets            */
+  int64_t cg_iterations;        /* projection loss: CG iterations run */
+  double cg_relative_residual;  /* max_i |r_i| / |P y| at exit        */
+} svdh_backward_info;
+
+/* Gradient of the loss with respect to the stored values of the operator's
+ * CSR (grad has nnz entries of grad_dtype, in CSR order of the underlying
+ * matrix; for a masked view the dropped columns get 0). res holds the top
+ * triplets of a solve of the same operator (svdh_rsvd or svdh_las2; either u
+ * layout is accepted); k <= min(res->d, 64). The projection loss solves k
+ * shifted systems (s_i^2 I - A A^T) w_i = (I - U_k U_k^T) y by conjugate
+ * gradients to the relative residual tol (<= 0: 1e-10 in FP64, 1e-5 in FP32)
+ * within max_iters (<= 0: 1000) iterations. precision: SVDH_PREC_F64 or
+ * SVDH_PREC_F32 (factor panels and SpMM arithmetic). */
+int32_t svdh_backward(svdh_ctx *ctx, const svdh_op *op, const svdh_result *res, int32_t loss, int64_t k,
+                      const double *y, int32_t precision, double tol, int64_t max_iters, void *grad,
+                      int32_t grad_dtype, svdh_backward_info *info);
 
 void svdh_result_free(svdh_result *res);
 

@@ -20,6 +20,8 @@ vector, a closed form or a library routine chosen so a plausible mistake fails):
               nalgebra's Householder QR / SVD sign conventions are parity
               unpinned (not in the container), so comparisons use subspace
               angles and singular values.
+  adjoint (oracle/adjoint.py, no reference counterpart) pinned: central
+              finite differences of the loss through numpy.linalg.svd.
 """
 from .pyoracle import (  # noqa: F401
     OracleOp,
@@ -33,3 +35,4 @@ from .pyoracle import (  # noqa: F401
     F32_EPS,
 )
 from .rsvd import randomized_svd  # noqa: F401
+from . import adjoint  # noqa: F401

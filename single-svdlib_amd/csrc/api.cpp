@@ -106,6 +106,10 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
                    int32_t precision, double tol, int64_t max_power_iters, svdh_result *out,
                    const RsvdDeviceOut *dev);
 
+int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32_t loss, int64_t k, const double *y,
+                       int32_t precision, double tol, int64_t max_iters, void *grad, int32_t grad_dtype,
+                       svdh_backward_info *info);
+
 template <typename F>
 static int32_t guarded(F &&f) {
   try {
@@ -449,6 +453,18 @@ int32_t svdh_rsvd_device(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, 
     RsvdDeviceOut dev{u, ldu, s, vt, ldvt};
     rsvd_solve(ctx, const_cast<svdh_op *>(op), target_rank, n_oversamples, n_power_iters, normalizer, mean_center,
                seed, has_seed, omega, precision, tol, max_power_iters, meta, &dev);
+  });
+}
+
+int32_t svdh_backward(svdh_ctx *ctx, const svdh_op *op, const svdh_result *res, int32_t loss, int64_t k,
+                      const double *y, int32_t precision, double tol, int64_t max_iters, void *grad,
+                      int32_t grad_dtype, svdh_backward_info *info) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    if (!res || !grad || !info) fail(SVDH_ERR_ARG, "null argument");
+    backward_solve(ctx, const_cast<svdh_op *>(op), res, loss, k, y, precision, tol, max_iters, grad, grad_dtype,
+                   info);
   });
 }
 

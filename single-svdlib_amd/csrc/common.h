@@ -51,7 +51,9 @@ struct DevBuf {
 };
 
 // Kernel classes for event timing (svdh_ctx_kernel_time).
-enum KernelClass { KC_SPMV = 0, KC_SPMM = 1, KC_DENSE = 2, KC_BLAS1 = 3, KC_GEMV = 4, KC_SPMM_T = 5, KC_COUNT = 6 };
+enum KernelClass {
+  KC_SPMV = 0, KC_SPMM = 1, KC_DENSE = 2, KC_BLAS1 = 3, KC_GEMV = 4, KC_SPMM_T = 5, KC_SDDMM = 6, KC_COUNT = 7
+};
 
 struct Ctx {
   int device = 0;
@@ -194,6 +196,12 @@ void spmv(svdh_op *op, int direction, int vdt, const void *x, void *y, int xdt);
 void spmm(svdh_op *op, int direction, int vdt, const void *X, int64_t ldx, int xdt, void *Y,
           int64_t ldy, int ydt, int64_t l);
 void ensure_values(svdh_op *op, int dtype);
+// grad[e] = sum_c s1_c L1[r_e, c] R1[col_e, c] (+ the same with L2, R2, s2 when
+// L2 is not null) over the pattern of op (CSR order of the underlying matrix;
+// columns a mask drops get 0). L panels have op->nrows rows, R panels
+// op->ncols rows; k <= 64; s1, s2 device vectors or null (all ones).
+void sddmm(svdh_op *op, int64_t k, const void *L1, const void *R1, const double *s1, const void *L2, const void *R2,
+           const double *s2, int64_t ldl, int64_t ldr, int dt, void *grad, int gdt);
 
 // Device-resident outputs of the randomized solver: u (m x k, row-major,
 // leading dimension ldu) and vt (k x n, row-major, ldvt) in device memory,
@@ -250,6 +258,21 @@ void panel_convert(Ctx *c, int64_t m, int64_t l, const void *X, int64_t ldx, int
                    int64_t ldy, int ydt);
 // Transpose-copy between a host column-major panel and a device row-major panel.
 void panel_fill(Ctx *c, int64_t m, int64_t l, double v, void *Y, int64_t ldy, int ydt);
+// Z (m x lo) = alpha * Y (m x li) M + beta * Z with M (li x lo, FP64 row-major)
+// already in device memory; Y and Z share the dtype dt (F64 or F32).
+void panel_gemm_dev(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, const double *M_dev,
+                    double alpha, double beta, void *Z, int64_t ldz, int dt);
+// C (k1 x k2, FP64 row-major, device) = X1^T X2 for row-major panels (k1, k2 <= 64).
+void gram_cross(Ctx *c, int64_t m, int64_t k1, const void *X1, int64_t ld1, int64_t k2, const void *X2, int64_t ld2,
+                int dt, double *C_dev);
+// out[c] = sum_i X[i, c] Y[i, c] for k <= 64 columns (device FP64 vector).
+void panel_coldots(Ctx *c, int64_t m, int64_t k, const void *X, const void *Y, int64_t ld, int dt, double *out_dev);
+// W += alpha_c P, R -= alpha_c MP per column; rr[c] = |R[:, c]|^2 afterwards (device).
+void panel_cg_update(Ctx *c, int64_t m, int64_t k, void *W, void *R, const void *P, const void *MP, int64_t ld,
+                     int dt, const double *alpha_dev, double *rr_dev);
+// Y[:, c] = a_c X[:, c] + b_c Y[:, c] (device coefficient vectors; null = 1).
+void panel_colaxpby(Ctx *c, int64_t m, int64_t k, const double *a_dev, const void *X, const double *b_dev, void *Y,
+                    int64_t ld, int dt);
 // Rank-one correction Y (ry x w) -= scale * a s^T with s = X^T wt, X (rx x w).
 // wt and a are device FP64 vectors; nullptr stands for all ones.
 void panel_rank1_correct(Ctx *c, int64_t rx, int64_t w, const void *X, int64_t ldx, int xdt, const double *wt,

@@ -287,7 +287,9 @@ template <bool SAME> constexpr int gram_waves() { return SAME ? kGramWaves : 4; 
 
 template <typename T, bool SAME>
 __global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m, int l, const T *Y, int64_t ldy,
-                                                                     int c0, int d0, double *part) {
+                                                                     int l2, const T *Y2, int64_t ld2, int c0,
+                                                                     int d0, double *part, int64_t pstride,
+                                                                     int out_ld, int mirror) {
   constexpr int W = gram_waves<SAME>();
   __shared__ double red[16 * 256];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -299,8 +301,8 @@ __global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m
   const int colc = c0 + 4 * cc, cold = d0 + 4 * cc;
   // columns at or past l read as zero; a quad starting below l stays inside
   // the row's leading dimension (a multiple of 4 that is >= l)
-  const bool okc = colc < l, okd = cold < l;
-  const int limc = l - colc, limd = l - cold;
+  const bool okc = colc < l, okd = cold < l2;
+  const int limc = l - colc, limd = l2 - cold;
   d4_t acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m
       const bool okr = st0 + g < s1 && r < m;
       if (okr && okc) xc[g].load(Y + r * ldy + colc); else xc[g].zero();
       if (!SAME) {
-        if (okr && okd) xd[g].load(Y + r * ldy + cold); else xd[g].zero();
+        if (okr && okd) xd[g].load(Y2 + r * ld2 + cold); else xd[g].zero();
       }
     }
   };
@@ -358,16 +360,16 @@ __global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m
           }
     __syncthreads();
   }
-  double *out = part + (int64_t)blockIdx.x * l * l;
+  double *out = part + (int64_t)blockIdx.x * pstride;
   for (int e = threadIdx.x; e < 16 * 256; e += 64 * W) {
     const int blk = e >> 8, v = (e >> 6) & 3, ln = e & 63;
     const int b1 = blk >> 2, b2 = blk & 3;
     if (SAME && b2 < b1) continue;
     const int gi = c0 + 4 * ((ln >> 4) + 4 * v) + b1, gj = d0 + 4 * (ln & 15) + b2;
-    if (gi < l && gj < l) {
+    if (gi < l && gj < l2) {
       const double x = red[e];
-      out[(int64_t)gi * l + gj] = x;
-      out[(int64_t)gj * l + gi] = x;
+      out[(int64_t)gi * out_ld + gj] = x;
+      if (mirror) out[(int64_t)gj * out_ld + gi] = x;
     }
   }
 }
@@ -378,7 +380,7 @@ __global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m
 // staged in LDS too when MLDS is set (li * lo entries of C).
 template <typename TY, typename TZ, typename C, bool MLDS>
 __global__ __launch_bounds__(kBlock) void k_apply(int64_t m, int li, int lo, const TY *Y, int64_t ldy, const double *M,
-                                                 TZ *Z, int64_t ldz, int64_t zcs) {
+                                                 TZ *Z, int64_t ldz, int64_t zcs, double alpha, double beta) {
   extern __shared__ char smem[];
   C *s = reinterpret_cast<C *>(smem);  // li x (kTile + 1)
   C *sm = s + (size_t)li * (kTile + 1);  // li x lo (when MLDS)
@@ -412,7 +414,11 @@ __global__ __launch_bounds__(kBlock) void k_apply(int64_t m, int li, int lo, con
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int c = c0 + tc * 4 + q;
-          if (c < lo) std_<TZ>(Z, (r0 + r) * ldz + c * zcs, (double)acc[p][q]);
+          if (c < lo) {
+            const int64_t at = (r0 + r) * ldz + c * zcs;
+            const double old = beta != 0.0 ? beta * ldd<TZ>(Z, at) : 0.0;
+            std_<TZ>(Z, at, old + alpha * (double)acc[p][q]);
+          }
         }
     }
   }
@@ -622,10 +628,11 @@ template <typename T>
 static void launch_gram(Ctx *c, bool same, int nb, int64_t m, int l, const T *Y, int64_t ldy, int c0, int d0,
                         double *part) {
   if (same)
-    hipLaunchKernelGGL((k_gram_mfma<T, true>), nb, 64 * gram_waves<true>(), 0, c->stream, m, l, Y, ldy, c0, d0, part);
+    hipLaunchKernelGGL((k_gram_mfma<T, true>), nb, 64 * gram_waves<true>(), 0, c->stream, m, l, Y, ldy, l, Y, ldy,
+                       c0, d0, part, (int64_t)l * l, l, 1);
   else
-    hipLaunchKernelGGL((k_gram_mfma<T, false>), nb, 64 * gram_waves<false>(), 0, c->stream, m, l, Y, ldy, c0, d0,
-                       part);
+    hipLaunchKernelGGL((k_gram_mfma<T, false>), nb, 64 * gram_waves<false>(), 0, c->stream, m, l, Y, ldy, l, Y, ldy,
+                       c0, d0, part, (int64_t)l * l, l, 1);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -666,17 +673,17 @@ void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, dou
 
 template <typename TY, typename TZ, typename C>
 static void apply_t(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, const double *M, void *Z,
-                    int64_t ldz, int64_t zcs) {
+                    int64_t ldz, int64_t zcs, double alpha = 1.0, double beta = 0.0) {
   const int64_t blocks = (m + kTile - 1) / kTile;
   const size_t shm = (size_t)li * (kTile + 1) * sizeof(C);
   if (shm > 65536) fail(SVDH_ERR_ARG, "panel width %lld too wide for the apply kernel", (long long)li);
   const size_t shm_m = shm + (size_t)li * lo * sizeof(C);
   if (shm_m <= 65536)
     hipLaunchKernelGGL((k_apply<TY, TZ, C, true>), (int)blocks, kBlock, shm_m, c->stream, m, (int)li, (int)lo,
-                       (const TY *)Y, ldy, M, (TZ *)Z, ldz, zcs);
+                       (const TY *)Y, ldy, M, (TZ *)Z, ldz, zcs, alpha, beta);
   else
     hipLaunchKernelGGL((k_apply<TY, TZ, C, false>), (int)blocks, kBlock, shm, c->stream, m, (int)li, (int)lo,
-                       (const TY *)Y, ldy, M, (TZ *)Z, ldz, zcs);
+                       (const TY *)Y, ldy, M, (TZ *)Z, ldz, zcs, alpha, beta);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -700,6 +707,40 @@ void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64
     apply_t<float, bf16_t, float>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else
     fail(SVDH_ERR_ARG, "unsupported apply dtypes %d -> %d", ydt, zdt);
+}
+
+void panel_gemm_dev(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, const double *M_dev,
+                    double alpha, double beta, void *Z, int64_t ldz, int dt) {
+  if (m == 0 || lo == 0) return;
+  KTimer t(c, KC_DENSE, (double)m * (li + 2 * lo) * dtype_size(dt));
+  if (dt == SVDH_F64)
+    apply_t<double, double, double>(c, m, li, lo, Y, ldy, M_dev, Z, ldz, 1, alpha, beta);
+  else if (dt == SVDH_F32)
+    apply_t<float, float, float>(c, m, li, lo, Y, ldy, M_dev, Z, ldz, 1, alpha, beta);
+  else
+    fail(SVDH_ERR_ARG, "unsupported panel dtype %d", dt);
+}
+
+void gram_cross(Ctx *c, int64_t m, int64_t k1, const void *X1, int64_t ld1, int64_t k2, const void *X2, int64_t ld2,
+                int dt, double *C_dev) {
+  if (k1 > 64 || k2 > 64) fail(SVDH_ERR_ARG, "cross Gram limited to 64 x 64");
+  if (ld1 % 4 || ld2 % 4) fail(SVDH_ERR_ARG, "cross Gram leading dimensions must be multiples of 4");
+  const int nb = grid_of((m + 3) / 4, 8 * 4, 256);
+  const int64_t kk = k1 * k2;
+  c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * kk * sizeof(double)));
+  double *part = c->red_part.as<double>();
+  KTimer t(c, KC_DENSE, (double)m * (k1 + k2) * dtype_size(dt));
+  if (dt == SVDH_F64)
+    hipLaunchKernelGGL((k_gram_mfma<double, false>), nb, 64 * gram_waves<false>(), 0, c->stream, m, (int)k1,
+                       (const double *)X1, ld1, (int)k2, (const double *)X2, ld2, 0, 0, part, kk, (int)k2, 0);
+  else if (dt == SVDH_F32)
+    hipLaunchKernelGGL((k_gram_mfma<float, false>), nb, 64 * gram_waves<false>(), 0, c->stream, m, (int)k1,
+                       (const float *)X1, ld1, (int)k2, (const float *)X2, ld2, 0, 0, part, kk, (int)k2, 0);
+  else
+    fail(SVDH_ERR_ARG, "unsupported panel dtype %d", dt);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_gram_reduce, reduce_grid(kk), kBlock, 0, c->stream, nb, kk, (const double *)part, C_dev);
+  HIP_CHECK(hipGetLastError());
 }
 
 void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const void *Y1, const void *Y2, int64_t ld,
@@ -807,6 +848,62 @@ __global__ __launch_bounds__(kBlock) void k_rank1(int64_t rows, int w, T *Y, int
   }
 }
 
+// Column dots s_c = sum_i X[i, c] Y[i, c] for k <= 64 columns: thread t owns
+// column t % 64 and rows of phase t / 64; per-workgroup FP64 partials.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_coldot(int64_t rows, int k, const T *X, const T *Y, int64_t ld,
+                                                  double *part) {
+  const int col = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  double acc = 0.0;
+  if (col < k)
+    for (int64_t i = (int64_t)blockIdx.x * 4 + ph; i < rows; i += (int64_t)gridDim.x * 4)
+      acc += ldd<T>(X, i * ld + col) * ldd<T>(Y, i * ld + col);
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64 && col < k)
+    part[(int64_t)blockIdx.x * k + col] =
+        (red[threadIdx.x] + red[threadIdx.x + 64]) + (red[threadIdx.x + 128] + red[threadIdx.x + 192]);
+}
+
+// Conjugate-gradient step on k columns at once (one shift per column):
+// W += alpha_c P, R -= alpha_c MP, and the new column norms of R (partials).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(int64_t rows, int k, T *W, T *R, const T *P, const T *MP,
+                                                     int64_t ld, const double *alpha, double *part) {
+  const int col = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  double acc = 0.0;
+  if (col < k) {
+    const double a = alpha[col];
+    for (int64_t i = (int64_t)blockIdx.x * 4 + ph; i < rows; i += (int64_t)gridDim.x * 4) {
+      const int64_t at = i * ld + col;
+      std_<T>(W, at, ldd<T>(W, at) + a * ldd<T>(P, at));
+      const double r = ldd<T>(R, at) - a * ldd<T>(MP, at);
+      std_<T>(R, at, r);
+      acc += r * r;
+    }
+  }
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64 && col < k)
+    part[(int64_t)blockIdx.x * k + col] =
+        (red[threadIdx.x] + red[threadIdx.x + 64]) + (red[threadIdx.x + 128] + red[threadIdx.x + 192]);
+}
+
+// Y[i, c] = a_c X[i, c] + b_c Y[i, c] (a or b null: 1).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_colaxpby(int64_t rows, int k, const double *a, const T *X,
+                                                    const double *b, T *Y, int64_t ld) {
+  const int64_t total = rows * k;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = e / k;
+    const int col = (int)(e - i * k);
+    const int64_t at = i * ld + col;
+    std_<T>(Y, at, (a ? a[col] : 1.0) * ldd<T>(X, at) + (b ? b[col] : 1.0) * ldd<T>(Y, at));
+  }
+}
+
 void panel_rank1_correct(Ctx *c, int64_t rx, int64_t w, const void *X, int64_t ldx, int xdt, const double *wt,
                          int64_t ry, void *Y, int64_t ldy, int ydt, const double *a, double scale) {
   if (w > 128) fail(SVDH_ERR_ARG, "panel wider than 128");
@@ -832,6 +929,54 @@ void panel_rank1_correct(Ctx *c, int64_t rx, int64_t w, const void *X, int64_t l
     hipLaunchKernelGGL(k_rank1<float>, g, kBlock, 0, c->stream, ry, (int)w, (float *)Y, ldy, a, sums, scale);
   else
     fail(SVDH_ERR_ARG, "unsupported panel dtype %d", ydt);
+  HIP_CHECK(hipGetLastError());
+}
+
+void panel_coldots(Ctx *c, int64_t m, int64_t k, const void *X, const void *Y, int64_t ld, int dt, double *out_dev) {
+  if (k > 64) fail(SVDH_ERR_ARG, "column dots limited to 64 columns");
+  const int nb = grid_of(m, 4, 512);
+  c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * k * sizeof(double)));
+  double *part = c->red_part.as<double>();
+  KTimer t(c, KC_DENSE, 2.0 * m * k * dtype_size(dt));
+  if (dt == SVDH_F64)
+    hipLaunchKernelGGL(k_coldot<double>, nb, kBlock, 0, c->stream, m, (int)k, (const double *)X, (const double *)Y,
+                       ld, part);
+  else
+    hipLaunchKernelGGL(k_coldot<float>, nb, kBlock, 0, c->stream, m, (int)k, (const float *)X, (const float *)Y, ld,
+                       part);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, out_dev);
+  HIP_CHECK(hipGetLastError());
+}
+
+void panel_cg_update(Ctx *c, int64_t m, int64_t k, void *W, void *R, const void *P, const void *MP, int64_t ld,
+                     int dt, const double *alpha_dev, double *rr_dev) {
+  if (k > 64) fail(SVDH_ERR_ARG, "CG panels limited to 64 columns");
+  const int nb = grid_of(m, 4, 512);
+  c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * k * sizeof(double)));
+  double *part = c->red_part.as<double>();
+  KTimer t(c, KC_DENSE, 6.0 * m * k * dtype_size(dt));
+  if (dt == SVDH_F64)
+    hipLaunchKernelGGL(k_cg_update<double>, nb, kBlock, 0, c->stream, m, (int)k, (double *)W, (double *)R,
+                       (const double *)P, (const double *)MP, ld, alpha_dev, part);
+  else
+    hipLaunchKernelGGL(k_cg_update<float>, nb, kBlock, 0, c->stream, m, (int)k, (float *)W, (float *)R,
+                       (const float *)P, (const float *)MP, ld, alpha_dev, part);
+  HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, rr_dev);
+  HIP_CHECK(hipGetLastError());
+}
+
+void panel_colaxpby(Ctx *c, int64_t m, int64_t k, const double *a_dev, const void *X, const double *b_dev, void *Y,
+                    int64_t ld, int dt) {
+  KTimer t(c, KC_DENSE, 3.0 * m * k * dtype_size(dt));
+  const int g = grid_of(m * k);
+  if (dt == SVDH_F64)
+    hipLaunchKernelGGL(k_colaxpby<double>, g, kBlock, 0, c->stream, m, (int)k, a_dev, (const double *)X, b_dev,
+                       (double *)Y, ld);
+  else
+    hipLaunchKernelGGL(k_colaxpby<float>, g, kBlock, 0, c->stream, m, (int)k, a_dev, (const float *)X, b_dev,
+                       (float *)Y, ld);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -915,6 +915,153 @@ void spmm(svdh_op *op, int direction, int vdt, const void *X, int64_t ldx, int x
     fail(SVDH_ERR_ARG, "unsupported SpMM dtypes values=%d in=%d out=%d", vdt, xdt, ydt);
 }
 
+// ------------------------------------------------------------------------
+// SDDMM over the CSR pattern (the adjoint of the SVD):
+//   grad[e] = sum_c s1_c L1[r_e, c] R1[col_e, c] + s2_c L2[r_e, c] R2[col_e, c]
+// for k <= 64 factor columns. One row group of R = 2^LGR lanes per row, each
+// lane owning NV consecutive factor columns; the left rows (scaled) stay in
+// registers, each nonzero gathers one right row, and the R lane partials are
+// combined with a butterfly so lane t keeps the value of the row's t-th
+// nonzero of the batch and the batch is written coalesced. A masked view
+// maps the column through colmap; dropped columns get 0.
+// ------------------------------------------------------------------------
+template <typename T, typename G>
+struct SddmmArgs {
+  const int32_t *rowptr;
+  const int32_t *colidx;
+  const int32_t *colmap;
+  const T *l1, *r1, *l2, *r2;
+  const double *s1, *s2;
+  int64_t ldl, ldr;
+  int32_t k;
+  int64_t nrows;
+  G *grad;
+};
+
+template <int LGR, bool MC, typename T, typename G>
+__global__ __launch_bounds__(kBlock) void k_sddmm(SddmmArgs<T, G> a) {
+  constexpr int R = 1 << LGR;
+  constexpr int NV = PanelVec<T>::N;
+  constexpr int RPB = kBlock >> LGR;
+  const int lane = threadIdx.x & (R - 1);
+  const int col = lane * NV;
+  const bool colok = col < a.k;
+  const bool two = a.l2 != nullptr;
+  for (int64_t r = (int64_t)blockIdx.x * RPB + (threadIdx.x >> LGR); r < a.nrows; r += (int64_t)gridDim.x * RPB) {
+    T lv1[NV], lv2[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      lv1[v] = T(0);
+      lv2[v] = T(0);
+    }
+    if (colok) {
+      ld_panel<T, T>(a.l1 + r * a.ldl + col, lv1);
+      if (two) ld_panel<T, T>(a.l2 + r * a.ldl + col, lv2);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const bool in = col + v < a.k;
+        lv1[v] = in ? (T)(a.s1 ? a.s1[col + v] * (double)lv1[v] : (double)lv1[v]) : T(0);
+        lv2[v] = in && two ? (T)(a.s2 ? a.s2[col + v] * (double)lv2[v] : (double)lv2[v]) : T(0);
+      }
+    }
+    const int kb = a.rowptr[r], ke = a.rowptr[r + 1];
+    for (int base = kb; base < ke; base += R) {
+      int c = -1;
+      if (base + lane < ke) {
+        c = __builtin_nontemporal_load(&a.colidx[base + lane]);
+        if (MC) c = a.colmap[c];
+      }
+      const int n = min(R, ke - base);
+      T mine = T(0);
+      for (int j = 0; j < n; ++j) {
+        const int cj = __shfl(c, j, R);
+        T part = T(0);
+        if (colok && cj >= 0) {
+          T rv[NV];
+          ld_panel<T, T>(a.r1 + (int64_t)cj * a.ldr + col, rv);
+#pragma unroll
+          for (int v = 0; v < NV; ++v) part += lv1[v] * rv[v];
+          if (two) {
+            ld_panel<T, T>(a.r2 + (int64_t)cj * a.ldr + col, rv);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) part += lv2[v] * rv[v];
+          }
+        }
+#pragma unroll
+        for (int off = R / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, R);
+        if (lane == j) mine = part;
+      }
+      if (lane < n) a.grad[base + lane] = static_cast<G>(mine);
+    }
+  }
+}
+
+template <int LGR, bool MC, typename T, typename G>
+static void sddmm_launch(Ctx *c, const SddmmArgs<T, G> &a) {
+  constexpr int RPB = kBlock >> LGR;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((a.nrows + RPB - 1) / RPB, 8192));
+  hipLaunchKernelGGL((k_sddmm<LGR, MC, T, G>), blocks, kBlock, 0, c->stream, a);
+}
+
+template <typename T, typename G>
+static void sddmm_t(svdh_op *op, int64_t k, const void *L1, const void *R1, const double *s1, const void *L2,
+                    const void *R2, const double *s2, int64_t ldl, int64_t ldr, void *grad) {
+  Ctx *c = op->ctx;
+  CsrView v = op->view(0);
+  constexpr int NV = PanelVec<T>::N;
+  if (ldl % NV || ldr % NV) fail(SVDH_ERR_ARG, "factor leading dimensions must be multiples of %d", NV);
+  SddmmArgs<T, G> a;
+  a.rowptr = v.rowptr;
+  a.colidx = v.colidx;
+  a.colmap = v.colmap;
+  a.l1 = static_cast<const T *>(L1);
+  a.r1 = static_cast<const T *>(R1);
+  a.l2 = static_cast<const T *>(L2);
+  a.r2 = static_cast<const T *>(R2);
+  a.s1 = s1;
+  a.s2 = s2;
+  a.ldl = ldl;
+  a.ldr = ldr;
+  a.k = (int32_t)k;
+  a.nrows = v.nrows;
+  a.grad = static_cast<G *>(grad);
+  const int pairs = L2 ? 2 : 1;
+  const double bytes = (double)v.nnz * (4 + sizeof(G)) + (double)(v.nrows + 1) * 4 +
+                       (double)pairs * k * sizeof(T) * (double)(v.nrows + op->ncols);
+  KTimer t(c, KC_SDDMM, bytes);
+  int lgr = 1;
+  while ((1 << lgr) * NV < k && lgr < 6) ++lgr;
+  const bool mc = v.colmap != nullptr;
+#define SVDH_SDDMM_CASE(L)                                   \
+  case L:                                                    \
+    if (mc) sddmm_launch<L, true>(c, a); else sddmm_launch<L, false>(c, a); \
+    break;
+  switch (lgr) {
+    SVDH_SDDMM_CASE(1)
+    SVDH_SDDMM_CASE(2)
+    SVDH_SDDMM_CASE(3)
+    SVDH_SDDMM_CASE(4)
+    SVDH_SDDMM_CASE(5)
+    default:
+      if (mc) sddmm_launch<6, true>(c, a); else sddmm_launch<6, false>(c, a);
+  }
+#undef SVDH_SDDMM_CASE
+  HIP_CHECK(hipGetLastError());
+}
+
+void sddmm(svdh_op *op, int64_t k, const void *L1, const void *R1, const double *s1, const void *L2, const void *R2,
+           const double *s2, int64_t ldl, int64_t ldr, int dt, void *grad, int gdt) {
+  if (k <= 0 || k > 64) fail(SVDH_ERR_ARG, "SDDMM factor width %lld outside 1..64", (long long)k);
+  if (dt == SVDH_F64 && gdt == SVDH_F64)
+    sddmm_t<double, double>(op, k, L1, R1, s1, L2, R2, s2, ldl, ldr, grad);
+  else if (dt == SVDH_F32 && gdt == SVDH_F32)
+    sddmm_t<float, float>(op, k, L1, R1, s1, L2, R2, s2, ldl, ldr, grad);
+  else if (dt == SVDH_F32 && gdt == SVDH_F64)
+    sddmm_t<float, double>(op, k, L1, R1, s1, L2, R2, s2, ldl, ldr, grad);
+  else
+    fail(SVDH_ERR_ARG, "unsupported SDDMM dtypes factors=%d grad=%d", dt, gdt);
+}
+
 void panel_to_bf16(Ctx *c, const float *src, uint16_t *dst, int64_t n) {
   convert_values(c, src, SVDH_F32, dst, SVDH_BF16, n);
 }

@@ -19,7 +19,7 @@ SVDH_F64, SVDH_F32, SVDH_BF16 = 0, 1, 2
 PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
 NORM_QR, NORM_LU, NORM_NONE = 0, 1, 2
 KEEP_F32, KEEP_BF16 = 0x1, 0x2
-KC_SPMV, KC_SPMM, KC_DENSE, KC_BLAS1, KC_GEMV, KC_SPMM_T = 0, 1, 2, 3, 4, 5
+KC_SPMV, KC_SPMM, KC_DENSE, KC_BLAS1, KC_GEMV, KC_SPMM_T, KC_SDDMM = 0, 1, 2, 3, 4, 5, 6
 
 
 class Diagnostics_(C.Structure):
@@ -55,6 +55,16 @@ class Result_(C.Structure):
     ]
 
 
+class BackwardInfo_(C.Structure):
+    _fields_ = [
+        ("loss", C.c_double),
+        ("cg_iterations", C.c_int64),
+        ("cg_relative_residual", C.c_double),
+    ]
+
+
+LOSS_SIGMA_SUM, LOSS_PROJECTION = 0, 1
+
 _vp = C.c_void_p
 _i64 = C.c_int64
 _i32 = C.c_int32
@@ -84,6 +94,8 @@ SIGNATURES = [
                          C.POINTER(Result_)]),
     ("svdh_rsvd_device", _i32, [_vp, _vp, _i64, _i64, _i64, _i32, _i32, C.c_uint64, _i32, _f64p, _i32, C.c_double,
                                 _i64, _vp, _i64, _f64p, _vp, _i64, C.POINTER(Result_)]),
+    ("svdh_backward", _i32, [_vp, _vp, C.POINTER(Result_), _i32, _i64, _f64p, _i32, C.c_double, _i64, _vp, _i32,
+                             C.POINTER(BackwardInfo_)]),
     ("svdh_result_free", None, [C.POINTER(Result_)]),
 ]
 

@@ -71,6 +71,13 @@ class SvdRec:
         """src/lanczos/mod.rs:1408-1413: u . diag(s) . vt (valid for the m x d layout)."""
         return self.u @ np.diag(self.s) @ self.vt
 
+    def backward(self, matrix, loss="sigma_sum", k=None, y=None, **kw):
+        """Gradient of a loss of these triplets with respect to the CSR values
+        of `matrix` (the operator they were computed from); see
+        single_svdlib.adjoint.backward."""
+        from .adjoint import backward
+        return backward(matrix, self, loss=loss, k=k, y=y, **kw)
+
 
 class SMat(Protocol):
     """The reference's plugin trait (src/utils.rs:22-33)."""

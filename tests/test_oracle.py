@@ -364,3 +364,57 @@ def test_rsvd_oracle_mean_center_matches_dense():
     centered = dense - dense.mean(axis=0)
     s_true = np.linalg.svd(centered, compute_uv=False)[:3]
     assert np.allclose(s, s_true, rtol=1e-6)
+
+
+def _adjoint_case(seed=3):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    a = sp.random(40, 25, density=0.3, random_state=seed, format="csr")
+    a.data = rng.normal(size=a.nnz)
+    y = rng.normal(size=40)
+    return a, y
+
+
+@pytest.mark.parametrize("loss", ["sigma_sum", "projection"])
+def test_adjoint_oracle_matches_finite_differences(loss):
+    # the oracle gradient is pinned by central differences of the loss itself
+    a, y = _adjoint_case()
+    k = 5
+    g = oracle.adjoint.gradient_on_pattern(a, loss, k, y)
+    rng = np.random.default_rng(1)
+    picks = rng.choice(a.nnz, 20, replace=False)
+    h = 1e-6
+    for e in picks:
+        ap, am = a.copy(), a.copy()
+        ap.data[e] += h
+        am.data[e] -= h
+        fd = (oracle.adjoint.svd_loss(ap.toarray(), loss, k, y) - oracle.adjoint.svd_loss(am.toarray(), loss, k, y)) / (2 * h)
+        assert abs(fd - g[e]) <= 1e-6 * max(1.0, np.max(np.abs(g))), (e, fd, g[e])
+
+
+def test_adjoint_oracle_planted_mistake_fails_fd():
+    # dropping the u_i (A^T w_i)^T half of the complement term is caught by FD
+    a, y = _adjoint_case()
+    k = 5
+    dense = a.toarray()
+    u, s, vt = np.linalg.svd(dense, full_matrices=True)
+    c = u[:, :k].T @ y
+    coef = u.T @ y
+    lam = np.zeros(40)
+    lam[:25] = s ** 2
+    g = np.zeros_like(dense)
+    for i in range(k):
+        w = u[:, k:] @ (coef[k:] / (s[i] ** 2 - lam[k:]))
+        g += 2.0 * c[i] * s[i] * np.outer(w, vt[i])  # planted: second term missing
+    rows = np.repeat(np.arange(40), np.diff(a.indptr))
+    gp = g[rows, a.indices]
+    h = 1e-6
+    worst = 0.0
+    for e in range(0, a.nnz, 7):
+        ap, am = a.copy(), a.copy()
+        ap.data[e] += h
+        am.data[e] -= h
+        fd = (oracle.adjoint.svd_loss(ap.toarray(), "projection", k, y)
+              - oracle.adjoint.svd_loss(am.toarray(), "projection", k, y)) / (2 * h)
+        worst = max(worst, abs(fd - gp[e]))
+    assert worst > 1e-3
