@@ -212,6 +212,19 @@ int32_t svdh_rsvd_device(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, 
                          double tol, int64_t max_power_iters, double *u, int64_t ldu, double *s,
                          double *vt, int64_t ldvt, svdh_result *meta);
 
+/* Batched randomized SVD: nops operators with the same number of rows
+ * (typically svdh_mask_create column masks of one matrix), each solved
+ * as svdh_rsvd would with tol = 0 (exactly n_power_iters power iterations),
+ * QR normaliser only. Every SpMM round over all operators is one kernel
+ * launch. All operators must give the same panel width
+ * min(target_rank, nrows, ncols_b) + n_oversamples <= 64. outs[b] receives
+ * operator b's result (free each with svdh_result_free). No counterpart in
+ * the reference, whose callers loop over randomized_svd
+ * (src/randomized/mod.rs:24-199) per MaskedCSRMatrix. */
+int32_t svdh_rsvd_batched(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops, int64_t target_rank,
+                          int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
+                          int32_t has_seed, int32_t precision, svdh_result *outs);
+
 /* ---- adjoint (backward) ------------------------------------------------ */
 
 /* Losses of svdh_backward. No counterpart in the reference: the adjoint sits

@@ -77,6 +77,8 @@ struct Ctx {
     }
     return ws[slot].p;
   }
+  // Panels of the batched randomized solver (4 per operator), grown on demand.
+  std::vector<DevBuf> batch_ws;
   // Pinned ring for small host-to-device uploads: slot i is reused only after
   // the event of its previous copy has completed.
   static constexpr int kRing = 8;
@@ -196,6 +198,10 @@ void spmv(svdh_op *op, int direction, int vdt, const void *x, void *y, int xdt);
 void spmm(svdh_op *op, int direction, int vdt, const void *X, int64_t ldx, int xdt, void *Y,
           int64_t ldy, int ydt, int64_t l);
 void ensure_values(svdh_op *op, int dtype);
+// spmm for nops operators in one launch (same direction, dtypes and width l;
+// panel b is X[b] / Y[b] with the shared leading dimensions).
+void spmm_batched(Ctx *c, int nops, svdh_op *const *ops, int direction, int vdt, const void *const *X, int64_t ldx,
+                  int xdt, void *const *Y, int64_t ldy, int ydt, int64_t l);
 // grad[e] = sum_c s1_c L1[r_e, c] R1[col_e, c] (+ the same with L2, R2, s2 when
 // L2 is not null) over the pattern of op (CSR order of the underlying matrix;
 // columns a mask drops get 0). L panels have op->nrows rows, R panels

@@ -76,6 +76,30 @@ void desc_eig(int64_t w, const std::vector<double> &G, std::vector<double> &lam,
   }
 }
 
+// M (w x wo, row-major) with P M orthonormal, from the FP64 Gram G of the
+// rows x w panel P: M = R^-1 of the Cholesky factor, or, when that does not
+// exist, the eigen-decomposition over the leading directions (returns false).
+bool normaliser_from_gram(int64_t w, int64_t rows, int dt, const std::vector<double> &G, std::vector<double> &M,
+                          int64_t &wo) {
+  std::vector<double> R(w * w);
+  const int64_t target = std::min(rows, w);
+  if (target == w && cholesky_upper(w, G.data(), R.data())) {
+    M.assign(w * w, 0.0);
+    upper_inverse(w, R.data(), M.data());
+    wo = w;
+    return true;
+  }
+  std::vector<double> lam, W;
+  desc_eig(w, G, lam, W);
+  const double thr = std::max(lam[0], 0.0) * (dt == SVDH_F64 ? 1e-24 : 1e-12);
+  M.assign(w * target, 0.0);
+  for (int64_t r = 0; r < target; ++r)
+    if (lam[r] > thr && lam[r] > 0.0)
+      for (int64_t i = 0; i < w; ++i) M[i * target + r] = W[i * w + r] / std::sqrt(lam[r]);
+  wo = target;
+  return false;
+}
+
 struct Rsvd {
   Ctx *c;
   svdh_op *op;
@@ -122,25 +146,9 @@ struct Rsvd {
   // Second half of normaliser: the Gram of P was enqueued by gram_async, and
   // the stream may have more work queued behind it.
   void finish_normaliser(const Panel &P, std::vector<double> &M, int64_t &wo) {
-    const int64_t w = P.w;
-    std::vector<double> G, R(w * w);
-    gram_wait(w, G);
-    const int64_t target = std::min(P.rows, w);
-    if (target == w && cholesky_upper(w, G.data(), R.data())) {
-      M.assign(w * w, 0.0);
-      upper_inverse(w, R.data(), M.data());
-      wo = w;
-      return;
-    }
-    ++fallbacks;
-    std::vector<double> lam, W;
-    desc_eig(w, G, lam, W);
-    const double thr = std::max(lam[0], 0.0) * (P.dt == SVDH_F64 ? 1e-24 : 1e-12);
-    M.assign(w * target, 0.0);
-    for (int64_t r = 0; r < target; ++r)
-      if (lam[r] > thr && lam[r] > 0.0)
-        for (int64_t i = 0; i < w; ++i) M[i * target + r] = W[i * w + r] / std::sqrt(lam[r]);
-    wo = target;
+    std::vector<double> G;
+    gram_wait(P.w, G);
+    if (!normaliser_from_gram(P.w, P.rows, P.dt, G, M, wo)) ++fallbacks;
   }
 
   // Explicit orthonormalisation (CholeskyQR2 with the eigen fallback).
@@ -372,6 +380,169 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
   out->power_iters_run = iters_run;
   out->residual_estimate = est;
   out->qr_fallbacks = R.fallbacks;
+  return SVDH_OK;
+}
+
+// Batched randomized SVD over several operators (column masks of one matrix),
+// QR normaliser, fixed power-iteration count. Every SpMM round (A_b X_b for
+// all b, or A_b^T Y_b for all b) is one launch; the Grams of a round are
+// enqueued together, then the next SpMM round, and only then does the host
+// factor them. Each operator follows exactly the steps of rsvd_solve with
+// tol = 0, so its result is the one a separate solve would return.
+int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int64_t target_rank,
+                           int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
+                           int32_t has_seed, int32_t precision, svdh_result *outs) {
+  if (nops <= 0) return SVDH_OK;
+  if (normalizer != SVDH_NORM_QR) fail(SVDH_ERR_ARG, "the batched solver supports the QR normaliser");
+  if (precision != SVDH_PREC_F64 && precision != SVDH_PREC_F32) fail(SVDH_ERR_ARG, "bad precision %d", precision);
+  if (target_rank < 0 || n_oversamples < 0 || n_power_iters < 0) fail(SVDH_ERR_ARG, "negative argument");
+  const int pdt = precision == SVDH_PREC_F64 ? SVDH_F64 : SVDH_F32;
+  const int64_t m = ops[0]->nrows;
+  int64_t l = -1;
+  for (int64_t b = 0; b < nops; ++b) {
+    if (ops[b]->nrows != m) fail(SVDH_ERR_ARG, "batched operators must have the same number of rows");
+    if (ops[b]->ncols == 0) fail(SVDH_ERR_SHAPE, "empty mask %lld", (long long)b);
+    const int64_t lb = std::min(target_rank, std::min(m, ops[b]->ncols)) + n_oversamples;
+    if (l >= 0 && lb != l) fail(SVDH_ERR_ARG, "batched operators need one panel width (rank + oversample)");
+    l = lb;
+  }
+  if (l <= 0 || l > 64) fail(SVDH_ERR_ARG, "batched panel width %lld outside 1..64", (long long)l);
+  const int64_t ld = (l + 7) / 8 * 8;
+  const size_t es = dtype_size(pdt);
+  const uint64_t sd = has_seed ? seed : 0;
+  Ctx *c = ctx;
+  // panels: Y_b, Yn_b (m x ld), Z_b and Omega_b (n_b x ld), in the context's batch workspace
+  std::vector<void *> Y(nops), Yn(nops), Z(nops), Om(nops);
+  c->batch_ws.resize(std::max<size_t>(c->batch_ws.size(), (size_t)nops * 4));
+  for (int64_t b = 0; b < nops; ++b) {
+    const int64_t nb = ops[b]->ncols;
+    void *p[4];
+    const size_t bytes[4] = {(size_t)m * ld * es, (size_t)m * ld * es, (size_t)nb * ld * es, (size_t)nb * ld * es};
+    for (int q = 0; q < 4; ++q) {
+      DevBuf &d = c->batch_ws[(size_t)b * 4 + q];
+      if (d.bytes < bytes[q]) {
+        c->sync();
+        d.alloc(bytes[q]);
+      }
+      p[q] = d.p;
+      HIP_CHECK(hipMemsetAsync(p[q], 0, bytes[q], c->stream));
+    }
+    Y[b] = p[0];
+    Yn[b] = p[1];
+    Z[b] = p[2];
+    Om[b] = p[3];
+    panel_gaussian(c, nb, l, sd, Om[b], ld, pdt);
+  }
+  std::vector<svdh_op *> opv(ops, ops + nops);
+  spmm_batched(c, (int)nops, opv.data(), 0, pdt, Om.data(), ld, pdt, Y.data(), ld, pdt, l);  // Y = A Omega
+
+  std::vector<std::vector<double>> Mq(nops);
+  std::vector<int64_t> wq(nops, l);
+  std::vector<int64_t> fallbacks(nops, 0);
+  const int64_t per_round = std::max<int64_t>(1, (int64_t)(Ctx::kSmall / (size_t)(l * l)));
+  // Grams of panels P_b (rows x l) for all b; `between` is enqueued after the
+  // first group of Grams and before the host waits for them.
+  auto grams = [&](const std::vector<void *> &P, const std::vector<int64_t> &rows, auto between,
+                   std::vector<std::vector<double>> &G) {
+    G.assign(nops, {});
+    bool ran = false;
+    for (int64_t b0 = 0; b0 < nops; b0 += per_round) {
+      const int64_t b1 = std::min(nops, b0 + per_round);
+      for (int64_t b = b0; b < b1; ++b) gram_async(c, rows[b], l, P[b], ld, pdt, (size_t)(b - b0) * l * l);
+      if (!ran) {
+        between();
+        ran = true;
+      }
+      HIP_CHECK(hipEventSynchronize(c->small_ev));
+      for (int64_t b = b0; b < b1; ++b)
+        G[b].assign(c->h_small + (b - b0) * l * l, c->h_small + (b - b0 + 1) * l * l);
+    }
+  };
+  std::vector<int64_t> mrows(nops, m), nrows(nops);
+  for (int64_t b = 0; b < nops; ++b) nrows[b] = ops[b]->ncols;
+  std::vector<std::vector<double>> G, GZ;
+  auto normalise_all = [&](const std::vector<std::vector<double>> &Gs) {
+    for (int64_t b = 0; b < nops; ++b)
+      if (!normaliser_from_gram(l, m, pdt, Gs[b], Mq[b], wq[b])) ++fallbacks[b];
+  };
+  // Gram(Y), then Z = A^T Y behind it, then the normalisers
+  grams(Y, mrows, [&] { spmm_batched(c, (int)nops, opv.data(), 1, pdt, Y.data(), ld, pdt, Z.data(), ld, pdt, l); },
+        G);
+  normalise_all(G);
+  int64_t iters_run = 0;
+  for (;;) {
+    const bool final_fixed = iters_run >= n_power_iters;
+    for (int64_t b = 0; b < nops; ++b)
+      panel_apply(c, nrows[b], l, wq[b], Z[b], ld, pdt, Mq[b].data(), Z[b], ld, pdt);  // Z = (A^T Y) Mq
+    if (final_fixed) break;
+    for (int64_t b = 0; b < nops; ++b)
+      if (wq[b] != l) fail(SVDH_ERR_ARG, "rank-deficient panel in the batched solver (operator %lld)", (long long)b);
+    spmm_batched(c, (int)nops, opv.data(), 0, pdt, Z.data(), ld, pdt, Yn.data(), ld, pdt, l);  // A Z
+    std::swap(Y, Yn);
+    ++iters_run;
+    grams(Y, mrows,
+          [&] { spmm_batched(c, (int)nops, opv.data(), 1, pdt, Y.data(), ld, pdt, Z.data(), ld, pdt, l); }, G);
+    normalise_all(G);
+  }
+  // B^T = Z_b; SVD(B) from its Gram, as in rsvd_solve
+  std::vector<int64_t> zw(wq);
+  grams(Z, nrows, [] {}, GZ);
+  for (int64_t b = 0; b < nops; ++b) {
+    svdh_op *op = ops[b];
+    const int64_t n = op->ncols, w = wq[b];
+    std::vector<double> Gb(w * w), lam, W;
+    for (int64_t i = 0; i < w; ++i)
+      for (int64_t j = 0; j < w; ++j) Gb[i * w + j] = GZ[b][i * l + j];
+    desc_eig(w, Gb, lam, W);
+    const int64_t k = std::min(target_rank, std::min(w, n));
+    std::vector<double> sv(k), MW(l * k, 0.0), Ws(w * k);
+    for (int64_t r = 0; r < k; ++r) {
+      sv[r] = std::sqrt(std::max(lam[r], 0.0));
+      const double inv = sv[r] > 0.0 ? 1.0 / sv[r] : 0.0;
+      for (int64_t i = 0; i < w; ++i) Ws[i * k + r] = W[i * w + r] * inv;
+      for (int64_t i = 0; i < l; ++i) {
+        double t = 0.0;
+        for (int64_t j = 0; j < w; ++j) t += Mq[b][i * w + j] * W[j * w + r];
+        MW[i * k + r] = t;
+      }
+    }
+    svdh_result *out = &outs[b];
+    double *ud = static_cast<double *>(ctx->workspace(WS_U, std::max<size_t>(m * k, 1) * 8));
+    double *vtd = static_cast<double *>(ctx->workspace(WS_VT, std::max<size_t>(k * n, 1) * 8));
+    panel_apply(c, m, l, k, Y[b], ld, pdt, MW.data(), ud, k, SVDH_F64);
+    panel_apply(c, n, w, k, Z[b], ld, pdt, Ws.data(), vtd, 1, SVDH_F64, n);
+    out->d = k;
+    out->u_rows = m;
+    out->u_cols = k;
+    out->vt_rows = k;
+    out->vt_cols = n;
+    out->s = static_cast<double *>(std::calloc(std::max<int64_t>(k, 1), sizeof(double)));
+    out->u = static_cast<double *>(std::calloc(std::max<int64_t>(m * k, 1), sizeof(double)));
+    out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(k * n, 1), sizeof(double)));
+    if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
+    for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
+    if (k > 0) {
+      HIP_CHECK(hipMemcpyAsync(out->u, ud, (size_t)m * k * 8, hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipMemcpyAsync(out->vt, vtd, (size_t)k * n * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));  // U/V workspaces are reused by the next operator
+    svdh_diagnostics &dg = out->diagnostics;  // randomized/mod.rs:214-237
+    dg.non_zero = op->nnz;
+    dg.dimensions = target_rank;
+    dg.iterations = n_power_iters;
+    dg.transposed = 0;
+    dg.lanczos_steps = 0;
+    dg.ritz_values_stabilized = k;
+    dg.significant_values = k;
+    dg.singular_values = k;
+    dg.end_interval[0] = -1e-30;
+    dg.end_interval[1] = 1e-30;
+    dg.kappa = 1e-6;
+    dg.random_seed = (uint32_t)sd;
+    out->power_iters_run = iters_run;
+    out->residual_estimate = -1.0;
+    out->qr_fallbacks = fallbacks[b];
+  }
   return SVDH_OK;
 }
 

@@ -288,6 +288,7 @@ struct SpmmArgs {
   int32_t nshort;
   int32_t nblk;
   int32_t chunk_blocks;  // nblk * max chunks per block
+  int32_t total_blocks;  // chunk blocks + short-row blocks
   int32_t blk_off[8];
   int32_t blk_cnt[8];
 };
@@ -412,10 +413,10 @@ __device__ __forceinline__ void spmm_chunk(const SpmmArgs<V, X, Y, Acc> &a, int 
 // combined in a fixed order). The kernel boundary makes the partials of the
 // first pass visible, so no fences or atomics are needed.
 template <typename Y, typename Acc>
-__global__ __launch_bounds__(kBlock) void k_spmm_fin(const int32_t *multi, const Acc *partials, int64_t ldp, Y *y,
-                                                    int64_t ldy, int l) {
+__device__ __forceinline__ void spmm_fin_row(const int32_t *multi, const Acc *partials, int64_t ldp, Y *y,
+                                             int64_t ldy, int l, int row) {
   __shared__ Acc red[kBlock];
-  const int *md = multi + 3 * (int64_t)blockIdx.x;
+  const int *md = multi + 3 * (int64_t)row;
   const int r = md[0], first = md[1], nch = md[2];
   const int q = threadIdx.x >> 6, o = threadIdx.x & 63;
   for (int c0 = 0; c0 < l; c0 += 64) {
@@ -430,11 +431,45 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin(const int32_t *multi, const
   }
 }
 
+template <typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm_fin(const int32_t *multi, const Acc *partials, int64_t ldp, Y *y,
+                                                    int64_t ldy, int l) {
+  spmm_fin_row<Y, Acc>(multi, partials, ldp, y, ldy, l, blockIdx.x);
+}
+
+// Batched forms: workgroup w belongs to operator b with boff[b] <= w < boff[b + 1].
+__device__ __forceinline__ int batch_of(const int32_t *boff, int nb, int w) {
+  int lo = 0, hi = nb;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (boff[mid] <= w) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <typename Y, typename Acc>
+struct FinArgs {
+  const int32_t *multi;
+  const Acc *partials;
+  int64_t ldp;
+  Y *y;
+  int64_t ldy;
+  int32_t l;
+};
+
+template <typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm_fin_batched(const FinArgs<Y, Acc> *args, const int32_t *boff,
+                                                            int nb) {
+  const int b = batch_of(boff, nb, blockIdx.x);
+  const FinArgs<Y, Acc> &f = args[b];
+  spmm_fin_row<Y, Acc>(f.multi, f.partials, f.ldp, f.y, f.ldy, f.l, (int)blockIdx.x - boff[b]);
+}
+
 template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
-__global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs<V, X, Y, Acc> a) {
-  if ((int)blockIdx.x < a.chunk_blocks) {
-    const int xb = blockIdx.x % a.nblk;
-    const int kb = blockIdx.x / a.nblk;
+__device__ __forceinline__ void spmm_block(const SpmmArgs<V, X, Y, Acc> &a, int w) {
+  if (w < a.chunk_blocks) {
+    const int xb = w % a.nblk;
+    const int kb = w / a.nblk;
     int off = 0, cnt = 0;
 #pragma unroll
     for (int i = 0; i < 8; ++i)
@@ -444,8 +479,25 @@ __global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs<V, X, Y, Acc> a) {
       }
     if (kb < cnt) spmm_chunk<LGR, MC>(a, off + kb);
   } else {
-    spmm_short<LGR, MC>(a, blockIdx.x - a.chunk_blocks);
+    spmm_short<LGR, MC>(a, w - a.chunk_blocks);
   }
+}
+
+template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm(SpmmArgs<V, X, Y, Acc> a) {
+  spmm_block<LGR, MC>(a, blockIdx.x);
+}
+
+// Several operators in one launch. Each operator's first workgroup is a
+// multiple of 8, so w mod 8 (its XCD under round-robin dispatch) is the same
+// as in a launch of its own.
+template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm_batched(const SpmmArgs<V, X, Y, Acc> *args, const int32_t *boff,
+                                                        int nb) {
+  const int b = batch_of(boff, nb, blockIdx.x);
+  const SpmmArgs<V, X, Y, Acc> &a = args[b];
+  const int w = (int)blockIdx.x - boff[b];
+  if (w < a.total_blocks) spmm_block<LGR, MC>(a, w);
 }
 
 // ------------------------------------------------------------------------
@@ -843,12 +895,21 @@ static void spmm_dispatch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, int lgr, int 
 }
 
 template <typename V, typename X, typename Y, typename Acc>
-static void launch_spmm(svdh_op *op, int direction, const void *vals, const void *X_, int64_t ldx,
-                        void *Y_, int64_t ldy, int64_t l) {
-  Ctx *c = op->ctx;
+struct SpmmPlan {
+  SpmmArgs<V, X, Y, Acc> a;
+  Schedule *s = nullptr;
+  int lgr = 1, blocks = 0;
+  bool mc = false;
+  double bytes = 0.0;
+};
+
+// Arguments and launch shape of one SpMM (false: nothing to compute).
+template <typename V, typename X, typename Y, typename Acc>
+static bool plan_spmm(svdh_op *op, int direction, const void *vals, const void *X_, int64_t ldx, void *Y_,
+                      int64_t ldy, int64_t l, SpmmPlan<V, X, Y, Acc> &p) {
   CsrView v = op->view(direction);
   Schedule &s = get_spmm_schedule(op, direction);
-  if (v.nrows == 0 || l == 0) return;
+  if (v.nrows == 0 || l == 0) return false;
   constexpr int NV = PanelVec<X>::N;
   if (ldx % NV || ldy % NV) fail(SVDH_ERR_ARG, "panel leading dimensions must be multiples of %d", NV);
   // lanes per row group: enough to cover l with NV columns each, 2..64
@@ -859,7 +920,7 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   const int blocks = s.total_blocks + (nshort + rpb - 1) / rpb;
   const int64_t ldp = (l + 7) / 8 * 8;
   s.partials.ensure((size_t)std::max<int32_t>(s.nslots, 1) * ldp * sizeof(Acc));
-  SpmmArgs<V, X, Y, Acc> a;
+  SpmmArgs<V, X, Y, Acc> &a = p.a;
   a.rowptr = v.rowptr;
   a.colidx = v.colidx;
   a.vals = static_cast<const V *>(vals);
@@ -877,23 +938,107 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   a.nshort = nshort;
   a.nblk = s.nblk;
   a.chunk_blocks = s.total_blocks;
+  a.total_blocks = blocks;
   for (int x = 0; x < 8; ++x) {
     a.blk_off[x] = s.blk_off[x];
     a.blk_cnt[x] = x < s.nblk ? s.blk_cnt[x] : 0;
   }
-  if (blocks == 0) return;
-  const double bytes = (double)v.nnz * (sizeof(V) + 4) + (double)(v.nrows + 1) * 4 +
-                       (double)v.ncols * l * sizeof(X) + (double)v.nrows * l * sizeof(Y);
-  KTimer t(c, direction == 0 ? KC_SPMM : KC_SPMM_T, bytes);
-  if (v.colmap)
-    spmm_dispatch<true>(c, a, lgr, blocks);
+  p.s = &s;
+  p.lgr = lgr;
+  p.blocks = blocks;
+  p.mc = v.colmap != nullptr;
+  p.bytes = (double)v.nnz * (sizeof(V) + 4) + (double)(v.nrows + 1) * 4 + (double)v.ncols * l * sizeof(X) +
+            (double)v.nrows * l * sizeof(Y);
+  return blocks > 0;
+}
+
+template <typename V, typename X, typename Y, typename Acc>
+static void launch_spmm(svdh_op *op, int direction, const void *vals, const void *X_, int64_t ldx,
+                        void *Y_, int64_t ldy, int64_t l) {
+  Ctx *c = op->ctx;
+  SpmmPlan<V, X, Y, Acc> p;
+  if (!plan_spmm(op, direction, vals, X_, ldx, Y_, ldy, l, p)) return;
+  KTimer t(c, direction == 0 ? KC_SPMM : KC_SPMM_T, p.bytes);
+  if (p.mc)
+    spmm_dispatch<true>(c, p.a, p.lgr, p.blocks);
   else
-    spmm_dispatch<false>(c, a, lgr, blocks);
+    spmm_dispatch<false>(c, p.a, p.lgr, p.blocks);
   HIP_CHECK(hipGetLastError());
-  if (s.nlong_rows > 0) {
-    hipLaunchKernelGGL((k_spmm_fin<Y, Acc>), s.nlong_rows, kBlock, 0, c->stream, s.counters.as<int32_t>(),
-                       (const Acc *)s.partials.p, ldp, static_cast<Y *>(Y_), ldy, (int)l);
+  const Schedule *sp = p.s;
+  if (sp->nlong_rows > 0) {
+    hipLaunchKernelGGL((k_spmm_fin<Y, Acc>), sp->nlong_rows, kBlock, 0, c->stream, sp->counters.as<int32_t>(),
+                       (const Acc *)sp->partials.p, p.a.ldp, static_cast<Y *>(Y_), ldy, (int)l);
     HIP_CHECK(hipGetLastError());
+  }
+}
+
+template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+static void spmm_batched_one(Ctx *c, const SpmmArgs<V, X, Y, Acc> *args, const int32_t *boff, int nb, int blocks) {
+  hipLaunchKernelGGL((k_spmm_batched<LGR, MC, V, X, Y, Acc>), blocks, kBlock, 0, c->stream, args, boff, nb);
+}
+
+// One launch for the SpMMs of several operators of the same direction and
+// panel width (plus one launch of the chunk sums). Operators with and without
+// a column remap go to separate launches.
+template <typename V, typename X, typename Y, typename Acc>
+static void launch_spmm_batched(Ctx *c, int nops, svdh_op *const *ops, int direction, int vdt, const void *const *Xs,
+                                int64_t ldx, void *const *Ys, int64_t ldy, int64_t l) {
+  std::vector<SpmmPlan<V, X, Y, Acc>> plans(nops);
+  std::vector<char> live(nops, 0);
+  for (int b = 0; b < nops; ++b)
+    live[b] = plan_spmm(ops[b], direction, ops[b]->values(direction, vdt), Xs[b], ldx, Ys[b], ldy, l, plans[b]);
+  for (int g = 0; g < 2; ++g) {
+    const bool mc = g == 1;
+    std::vector<SpmmArgs<V, X, Y, Acc>> args;
+    std::vector<int32_t> boff(1, 0);
+    std::vector<FinArgs<Y, Acc>> fargs;
+    std::vector<int32_t> foff(1, 0);
+    double bytes = 0.0;
+    int lgr = 0;
+    for (int b = 0; b < nops; ++b) {
+      if (!live[b] || plans[b].mc != mc) continue;
+      if (lgr && plans[b].lgr != lgr) fail(SVDH_ERR_ARG, "batched SpMM needs one lane layout");
+      lgr = plans[b].lgr;
+      args.push_back(plans[b].a);
+      boff.push_back(boff.back() + (plans[b].blocks + 7) / 8 * 8);
+      bytes += plans[b].bytes;
+      const Schedule &s = *plans[b].s;
+      if (s.nlong_rows > 0) {
+        fargs.push_back(FinArgs<Y, Acc>{s.counters.as<int32_t>(), (const Acc *)s.partials.p, plans[b].a.ldp,
+                                        plans[b].a.y, ldy, (int32_t)l});
+        foff.push_back(foff.back() + s.nlong_rows);
+      }
+    }
+    if (args.empty()) continue;
+    const int nb = (int)args.size();
+    const auto *dargs = static_cast<const SpmmArgs<V, X, Y, Acc> *>(c->upload(args.data(), args.size() * sizeof(args[0])));
+    const auto *dboff = static_cast<const int32_t *>(c->upload(boff.data(), boff.size() * sizeof(int32_t)));
+    KTimer t(c, direction == 0 ? KC_SPMM : KC_SPMM_T, bytes);
+    const int blocks = boff.back();
+#define SVDH_BATCH_CASE(L)                                                                   \
+  case L:                                                                                   \
+    if (mc) spmm_batched_one<L, true>(c, dargs, dboff, nb, blocks);                          \
+    else spmm_batched_one<L, false>(c, dargs, dboff, nb, blocks);                            \
+    break;
+    switch (lgr) {
+      SVDH_BATCH_CASE(1)
+      SVDH_BATCH_CASE(2)
+      SVDH_BATCH_CASE(3)
+      SVDH_BATCH_CASE(4)
+      SVDH_BATCH_CASE(5)
+      default:
+        if (mc) spmm_batched_one<6, true>(c, dargs, dboff, nb, blocks);
+        else spmm_batched_one<6, false>(c, dargs, dboff, nb, blocks);
+    }
+#undef SVDH_BATCH_CASE
+    HIP_CHECK(hipGetLastError());
+    if (!fargs.empty()) {
+      const auto *dfargs = static_cast<const FinArgs<Y, Acc> *>(c->upload(fargs.data(), fargs.size() * sizeof(fargs[0])));
+      const auto *dfoff = static_cast<const int32_t *>(c->upload(foff.data(), foff.size() * sizeof(int32_t)));
+      hipLaunchKernelGGL((k_spmm_fin_batched<Y, Acc>), foff.back(), kBlock, 0, c->stream, dfargs, dfoff,
+                         (int)fargs.size());
+      HIP_CHECK(hipGetLastError());
+    }
   }
 }
 
@@ -1060,6 +1205,17 @@ void sddmm(svdh_op *op, int64_t k, const void *L1, const void *R1, const double 
     sddmm_t<float, double>(op, k, L1, R1, s1, L2, R2, s2, ldl, ldr, grad);
   else
     fail(SVDH_ERR_ARG, "unsupported SDDMM dtypes factors=%d grad=%d", dt, gdt);
+}
+
+void spmm_batched(Ctx *c, int nops, svdh_op *const *ops, int direction, int vdt, const void *const *X, int64_t ldx,
+                  int xdt, void *const *Y, int64_t ldy, int ydt, int64_t l) {
+  for (int b = 0; b < nops; ++b) ensure_values(ops[b], vdt);
+  if (vdt == SVDH_F64 && xdt == SVDH_F64 && ydt == SVDH_F64)
+    launch_spmm_batched<double, double, double, double>(c, nops, ops, direction, vdt, X, ldx, Y, ldy, l);
+  else if (vdt == SVDH_F32 && xdt == SVDH_F32 && ydt == SVDH_F32)
+    launch_spmm_batched<float, float, float, float>(c, nops, ops, direction, vdt, X, ldx, Y, ldy, l);
+  else
+    fail(SVDH_ERR_ARG, "unsupported batched SpMM dtypes values=%d in=%d out=%d", vdt, xdt, ydt);
 }
 
 void panel_to_bf16(Ctx *c, const float *src, uint16_t *dst, int64_t n) {

@@ -50,6 +50,31 @@ def randomized_svd(m, target_rank, n_oversamples, n_power_iters, power_iteration
     return result_to_svdrec(res, op.dtype)
 
 
+def randomized_svd_batched(matrices, target_rank, n_oversamples, n_power_iters,
+                           power_iteration_normalizer=PowerIterationNormalizer.QR, seed=None, *, precision=None):
+    """Many randomized_svd solves over column masks of one matrix (a list of
+    MaskedCSRMatrix views of the same CsrMatrix), every SpMM round of all of
+    them in one launch. Returns one SvdRec per matrix, equal to what
+    randomized_svd(matrix, ..., mean_center=False, seed) returns for it."""
+    ops = [as_operator(m) for m in matrices]
+    if not ops:
+        return []
+    norm = power_iteration_normalizer
+    if isinstance(norm, str):
+        norm = PowerIterationNormalizer[norm.upper()]
+    dtype = ops[0].dtype
+    if precision is None:
+        prec = _ffi.PREC_F64 if dtype == np.float64 else _ffi.PREC_F32
+    else:
+        prec = {"f64": _ffi.PREC_F64, "f32": _ffi.PREC_F32}[precision]
+    handles = (C.c_void_p * len(ops))(*[o._handle for o in ops])
+    outs = (_ffi.Result_ * len(ops))()
+    _ffi.check(_ffi.lib().svdh_rsvd_batched(ops[0]._ctx, len(ops), handles, int(target_rank), int(n_oversamples),
+                                            int(n_power_iters), norm.value, int(seed or 0) & 0xFFFFFFFFFFFFFFFF,
+                                            int(seed is not None), prec, outs))
+    return [result_to_svdrec(outs[i], dtype) for i in range(len(ops))]
+
+
 def svd_flip(u=None, v=None, u_based_decision=True):
     """randomized/mod.rs:332-438, in place on numpy arrays (u m x k columns,
     v k x n rows)."""

@@ -1,0 +1,67 @@
+"""Batched masked randomized SVD (svdh_rsvd_batched): every SpMM round of all
+masks in one launch. Each mask's result must equal its own randomized_svd
+(same kernels, same reduction orders: bit for bit), and masked == physical
+subset (masked.rs:914-1000) within 1e-10."""
+import numpy as np
+import pytest
+
+from helpers import max_principal_angle, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import single_svdlib
+    from single_svdlib import _ffi
+    _ffi.context(0)
+    return single_svdlib
+
+
+def masks_of(n, sizes, seed):
+    rng = np.random.default_rng(seed)
+    return [np.sort(rng.choice(n, s, replace=False)) for s in sizes]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batched_equals_separate_solves(lib, dtype):
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(3000, 1200, 0.02, seed=17).astype(dtype)
+    base = lib.CsrMatrix.from_scipy(a)
+    cols = masks_of(1200, [40, 700, 120, 1200, 333, 90], seed=5)
+    views = [MaskedCSRMatrix.with_columns(base, c) for c in cols]
+    recs = lib.randomized.randomized_svd_batched(views, 10, 10, 2, "QR", seed=9)
+    assert len(recs) == len(views)
+    for v, r in zip(views, recs):
+        one = lib.randomized.randomized_svd(v, 10, 10, 2, "QR", False, 9, False)
+        assert r.u.shape == one.u.shape and r.vt.shape == one.vt.shape
+        np.testing.assert_array_equal(r.s, one.s)
+        np.testing.assert_array_equal(r.u, one.u)
+        np.testing.assert_array_equal(r.vt, one.vt)
+        assert r.diagnostics.iterations == 2 and r.extra["power_iters_run"] == 2
+
+
+def test_batched_masked_equals_physical(lib):
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(2500, 900, 0.03, seed=23)
+    base = lib.CsrMatrix.from_scipy(a)
+    cols = masks_of(900, [300, 650, 120], seed=8)
+    recs = lib.randomized.randomized_svd_batched([MaskedCSRMatrix.with_columns(base, c) for c in cols], 8, 8, 3,
+                                                 "QR", seed=4)
+    phys = lib.randomized.randomized_svd_batched([lib.CsrMatrix.from_scipy(a[:, c].tocsr()) for c in cols], 8, 8,
+                                                 3, "QR", seed=4)
+    for r, p in zip(recs, phys):
+        assert np.allclose(r.s, p.s, rtol=1e-10)
+        assert max_principal_angle(r.u[:, :4], p.u[:, :4]) < 1e-8
+
+
+def test_batched_rejects_mixed_widths_and_normalizers(lib):
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(500, 300, 0.05, seed=2)
+    base = lib.CsrMatrix.from_scipy(a)
+    small = MaskedCSRMatrix.with_columns(base, np.arange(5))     # rank clamps to 5 -> other width
+    big = MaskedCSRMatrix.with_columns(base, np.arange(100))
+    with pytest.raises(ValueError, match="panel width"):
+        lib.randomized.randomized_svd_batched([small, big], 10, 5, 1, "QR", seed=1)
+    with pytest.raises(ValueError, match="QR"):
+        lib.randomized.randomized_svd_batched([big], 10, 5, 1, "LU", seed=1)
