@@ -271,7 +271,11 @@ void panel_gemm_dev(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, in
 // C (k1 x k2, FP64 row-major, device) = X1^T X2 for row-major panels (k1, k2 <= 64).
 void gram_cross(Ctx *c, int64_t m, int64_t k1, const void *X1, int64_t ld1, int64_t k2, const void *X2, int64_t ld2,
                 int dt, double *C_dev);
-// out[c] = sum_i X[i, c] Y[i, c] for k <= 64 columns (device FP64 vector).
+// out (cols x rows, leading dimension ldo) = (X diag(scale))^T for a
+// row-major FP64 panel X (rows x cols); scale is a device vector or null.
+void panel_transpose_scale(Ctx *c, int64_t rows, int64_t cols, const double *X, int64_t ldx, const double *scale_dev,
+                           double *out, int64_t ldo);
+// out[c] = sum_i X[i, c] Y[i, c] for k columns (device FP64 vector).
 void panel_coldots(Ctx *c, int64_t m, int64_t k, const void *X, const void *Y, int64_t ld, int dt, double *out_dev);
 // W += alpha_c P, R -= alpha_c MP per column; rr[c] = |R[:, c]|^2 afterwards (device).
 void panel_cg_update(Ctx *c, int64_t m, int64_t k, void *W, void *R, const void *P, const void *MP, int64_t ld,

@@ -933,7 +933,13 @@ void panel_rank1_correct(Ctx *c, int64_t rx, int64_t w, const void *X, int64_t l
 }
 
 void panel_coldots(Ctx *c, int64_t m, int64_t k, const void *X, const void *Y, int64_t ld, int dt, double *out_dev) {
-  if (k > 64) fail(SVDH_ERR_ARG, "column dots limited to 64 columns");
+  if (k > 64) {  // 64 columns per pass
+    const size_t es = dtype_size(dt);
+    for (int64_t c0 = 0; c0 < k; c0 += 64)
+      panel_coldots(c, m, std::min<int64_t>(64, k - c0), static_cast<const char *>(X) + c0 * es,
+                    static_cast<const char *>(Y) + c0 * es, ld, dt, out_dev + c0);
+    return;
+  }
   const int nb = grid_of(m, 4, 512);
   c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * k * sizeof(double)));
   double *part = c->red_part.as<double>();
@@ -977,6 +983,36 @@ void panel_colaxpby(Ctx *c, int64_t m, int64_t k, const double *a_dev, const voi
   else
     hipLaunchKernelGGL(k_colaxpby<float>, g, kBlock, 0, c->stream, m, (int)k, a_dev, (const float *)X, b_dev,
                        (float *)Y, ld);
+  HIP_CHECK(hipGetLastError());
+}
+
+// out[c * ldo + i] = scale_c * X[i * ldx + c] for i < rows, c < cols: an
+// LDS-tiled (64 x 64) transpose with a per-column scale (null = 1).
+__global__ __launch_bounds__(kBlock) void k_transpose_scale(int64_t rows, int cols, const double *X, int64_t ldx,
+                                                           const double *scale, double *out, int64_t ldo) {
+  __shared__ double t[64][65];
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int rr = ty; rr < 64; rr += 4) {
+    const int64_t i = r0 + rr;
+    const int cc = c0 + tx;
+    t[rr][tx] = (i < rows && cc < cols) ? X[i * ldx + cc] : 0.0;
+  }
+  __syncthreads();
+  for (int cc = ty; cc < 64; cc += 4) {
+    const int col = c0 + cc;
+    const int64_t i = r0 + tx;
+    if (col < cols && i < rows) out[(int64_t)col * ldo + i] = (scale ? scale[col] : 1.0) * t[tx][cc];
+  }
+}
+
+void panel_transpose_scale(Ctx *c, int64_t rows, int64_t cols, const double *X, int64_t ldx, const double *scale_dev,
+                           double *out, int64_t ldo) {
+  if (rows == 0 || cols == 0) return;
+  KTimer t(c, KC_DENSE, 2.0 * rows * cols * 8.0);
+  dim3 g((unsigned)((rows + 63) / 64), (unsigned)((cols + 63) / 64));
+  hipLaunchKernelGGL(k_transpose_scale, g, kBlock, 0, c->stream, rows, (int)cols, X, ldx, scale_dev, out, ldo);
   HIP_CHECK(hipGetLastError());
 }
 

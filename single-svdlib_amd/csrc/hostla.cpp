@@ -135,6 +135,13 @@ int imtqlb(int64_t n, double *d, double *e, double *bnd, int64_t max_iter, doubl
 int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_iter, double eps) {
   if (n == 1) return SVDH_OK;
   const int64_t last = n - 1;
+  // The rotations mix columns i, i+1 of every row of z (row-major nm x n);
+  // they run on the transpose zt (n x nm), where those columns are two
+  // contiguous rows, and z is written back at the end. Same arithmetic per
+  // element, cache-friendly order.
+  std::vector<double> zt((size_t)n * nm);
+  for (int64_t row = 0; row < nm; ++row)
+    for (int64_t col = 0; col < n; ++col) zt[(size_t)col * nm + row] = z[row * n + col];
   for (int64_t i = 1; i < n; ++i) e[i - 1] = e[i];
   e[last] = 0.0;
   for (int64_t l = 0; l < n; ++l) {
@@ -170,12 +177,15 @@ int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_i
         p = s * r;
         d[i + 1] = g + p;
         g = c * r - b;
-        // Rotate columns i and i+1 of every row of z (row-major nm x n).
-        for (int64_t row = 0; row < nm; ++row) {
-          double *zr = z + row * n;
-          const double zi1 = zr[i + 1];
-          zr[i + 1] = s * zr[i] + c * zi1;
-          zr[i] = c * zr[i] - s * zi1;
+        // Rotate columns i and i+1 of z (rows i and i+1 of zt).
+        {
+          double *zi = zt.data() + (size_t)i * nm;
+          double *zj = zi + nm;
+          for (int64_t row = 0; row < nm; ++row) {
+            const double a = zi[row], b2 = zj[row];
+            zj[row] = s * a + c * b2;
+            zi[row] = c * a - s * b2;
+          }
         }
         if (i == l) break;
         --i;
@@ -201,9 +211,11 @@ int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_i
     if (k != i) {
       d[k] = d[i];
       d[i] = p;
-      for (int64_t row = 0; row < nm; ++row) std::swap(z[row * n + i], z[row * n + k]);
+      std::swap_ranges(zt.begin() + (size_t)i * nm, zt.begin() + (size_t)(i + 1) * nm, zt.begin() + (size_t)k * nm);
     }
   }
+  for (int64_t row = 0; row < nm; ++row)
+    for (int64_t col = 0; col < n; ++col) z[row * n + col] = zt[(size_t)col * nm + row];
   return SVDH_OK;
 }
 

@@ -417,23 +417,33 @@ struct Las2 {
     basis_combine(c, ncols, q(0), ldq, js, Wd.as<double>(), d, V.as<double>(), ldv);
     spmm(op, fwd, vdt, V.p, ldv, SVDH_F64, AV.p, ldv, SVDH_F64, d);
     spmm(op, bwd, vdt, AV.p, ldv, SVDH_F64, BV.p, ldv, SVDH_F64, d);
-    std::vector<double> hv(ncols * ldv), hav(nrows * ldv), hbv(ncols * ldv);
-    HIP_CHECK(hipMemcpyAsync(hv.data(), V.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipMemcpyAsync(hav.data(), AV.p, hav.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipMemcpyAsync(hbv.data(), BV.p, hbv.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // sigma_r = sqrt(max(v_r^T B v_r, 0)) (mod.rs:1212-1225), u_r = A v_r / max(sigma_r, eps')
+    // (mod.rs:1228-1243): dots, scaling and the transposes to the d x n row
+    // layout of SvdRec run on the device; only U^T, V^T and sigma come back.
+    DevBuf sc(2 * (size_t)std::max<int64_t>(d, 1) * sizeof(double));
+    double *dots = sc.as<double>(), *scale = dots + d;
+    panel_coldots(c, ncols, d, V.p, BV.p, ldv, SVDH_F64, dots);
+    std::vector<double> hd;
+    hd.resize(d);
+    HIP_CHECK(hipMemcpyAsync(hd.data(), dots, d * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    std::vector<double> hs(d);
+    for (int64_t r = 0; r < d; ++r) {
+      const double sval = std::sqrt(std::max(hd[r], 0.0));
+      out->s[r] = sval;
+      hs[r] = sval > aeps ? 1.0 / sval : 1.0 / std::max(sval, aeps);
+    }
+    HIP_CHECK(hipMemcpyAsync(scale, hs.data(), d * sizeof(double), hipMemcpyHostToDevice, c->stream));
     // The working (possibly transposed) layout: Ut is d x nrows, Vt is d x ncols.
     double *Ut = transposed ? out->vt : out->u;
     double *Vt = transposed ? out->u : out->vt;
-    for (int64_t r = 0; r < d; ++r) {
-      double t = 0.0;
-      for (int64_t i = 0; i < ncols; ++i) t += hv[i * ldv + r] * hbv[i * ldv + r];
-      const double sval = std::sqrt(std::max(t, 0.0));
-      out->s[r] = sval;
-      const double scale = sval > aeps ? 1.0 / sval : 1.0 / std::max(sval, aeps);
-      for (int64_t i = 0; i < nrows; ++i) Ut[r * nrows + i] = hav[i * ldv + r] * scale;
-      for (int64_t i = 0; i < ncols; ++i) Vt[r * ncols + i] = hv[i * ldv + r];
-    }
+    DevBuf T((size_t)d * std::max(nrows, ncols) * sizeof(double));
+    panel_transpose_scale(c, nrows, d, AV.as<double>(), ldv, scale, T.as<double>(), nrows);
+    HIP_CHECK(hipMemcpyAsync(Ut, T.p, (size_t)d * nrows * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    panel_transpose_scale(c, ncols, d, V.as<double>(), ldv, nullptr, T.as<double>(), ncols);
+    HIP_CHECK(hipMemcpyAsync(Vt, T.p, (size_t)d * ncols * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
   }
 };
 
