@@ -78,6 +78,258 @@ def cpu_baseline(a, iters_needed, threads):
                        f"{iters_needed} power iterations the GPU solve needed")}
 
 
+# ---------------------------------------------------------------------------
+# Other measured paths (python bench.py --workload ...). Each returns one JSON
+# object with its own metric, the roofline of its dominant kernel class and a
+# CPU baseline where one is affordable.
+# ---------------------------------------------------------------------------
+
+def _kernel_stats(lib, ctx, classes, steps):
+    ms, nl, by = C.c_double(), C.c_int64(), C.c_double()
+    out = {}
+    from single_svdlib import _ffi
+    for name, k in classes:
+        _ffi.check(lib.svdh_ctx_kernel_time(ctx, k, C.byref(ms), C.byref(nl), C.byref(by)))
+        out[name] = {"ms_per_step": ms.value / steps, "launches_per_step": nl.value / steps,
+                     "avg_launch_us": ms.value * 1e3 / max(nl.value, 1), "bytes": by.value, "_ms": ms.value,
+                     "_n": nl.value}
+    return out
+
+
+def _roofline(total_bytes, total_ms, launches):
+    avg_ms = total_ms / max(launches, 1)
+    per = total_bytes / max(launches, 1)
+    ach = per / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "algorithmic_bytes_per_launch": per, "avg_launch_ms": avg_ms}
+
+
+def _triplet_residual(a, u_cols, s, vt_rows):
+    a = sp.csr_matrix(a, dtype=np.float64)
+    v = np.asarray(vt_rows, dtype=np.float64).T
+    u = np.asarray(u_cols, dtype=np.float64)
+    r1 = a @ v - u * s
+    r2 = a.T @ u - v * s
+    per = np.sqrt((r1 * r1).sum(0) + (r2 * r2).sum(0))
+    return float(np.max(per / s))
+
+
+def bench_las2(args):
+    """lanczos::svd_dim_seed on the 2,000 x 1,500 (k = 10) or 500,000 x 30,000
+    (k = 100) workload, FP64 and FP32-SpMV/FP64-recurrence; the dominant
+    kernel is the SpMV (A x and A^T y)."""
+    import single_svdlib as sl
+    from single_svdlib import _ffi, synth
+    import oracle
+    cfg1 = args.workload == "las2-cfg1"
+    a = synth.config1() if cfg1 else synth.config3(np.float64)
+    k = 10 if cfg1 else 100
+    lib = _ffi.lib()
+    res = {}
+    for prec, dt in (("f64", np.float64), ("f32", np.float32)):
+        op = sl.CsrMatrix.from_scipy(a.astype(dt))
+        ctx = op._ctx
+        for _ in range(args.warmup):
+            sl.lanczos.svd_dim_seed(op, k, 42)
+        _ffi.check(lib.svdh_ctx_synchronize(ctx))
+        times = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            rec = sl.lanczos.svd_dim_seed(op, k, 42)
+            times.append(time.perf_counter() - t0)
+        _ffi.check(lib.svdh_ctx_profile(ctx, 1))
+        rec = sl.lanczos.svd_dim_seed(op, k, 42)
+        st = _kernel_stats(lib, ctx, (("spmv", _ffi.KC_SPMV), ("blas1", _ffi.KC_BLAS1), ("gemv", _ffi.KC_GEMV),
+                                      ("spmm", _ffi.KC_SPMM), ("spmm_t", _ffi.KC_SPMM_T), ("dense", _ffi.KC_DENSE)),
+                           1)
+        _ffi.check(lib.svdh_ctx_profile(ctx, 0))
+        d = rec.diagnostics
+        r = _triplet_residual(a, rec.u.T, rec.s.astype(np.float64), rec.vt)
+        res[prec] = {"s_per_solve": float(np.mean(times)), "lanczos_steps": d.lanczos_steps,
+                     "ritz_values_stabilized": d.ritz_values_stabilized, "significant_values": d.significant_values,
+                     "residual_fp64": r, "sigma_max": float(rec.s[0]), "sigma_min": float(rec.s[-1]),
+                     "roofline_spmv": _roofline(st["spmv"]["bytes"], st["spmv"]["_ms"], st["spmv"]["_n"]),
+                     "kernels_ms": {kname: v["ms_per_step"] for kname, v in st.items()}}
+        log(f"las2 {prec}: {res[prec]['s_per_solve'] * 1e3:.1f} ms, steps {d.lanczos_steps}, residual {r:.2e}")
+        del op
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        o = oracle.OracleOp(a.astype(np.float64), nthreads=threads)
+        if cfg1:
+            t0 = time.perf_counter()
+            oracle.las2(o, k, 0, (-1e-30, 1e-30), 1e-6, 42)
+            cpu = {"value": time.perf_counter() - t0, "unit": "s", "cores": threads, "kind": "port",
+                   "sample": "the FP64 C oracle's full LAS2 solve of the same matrix"}
+        else:
+            x = np.random.default_rng(0).normal(size=a.shape[1])
+            reps = 10
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                o.opa(o.opa(x, False), True)
+            per = (time.perf_counter() - t0) / reps
+            ops = res["f64"]["lanczos_steps"] + 2 * k  # one A^T A x per step, A v and A^T A v per Ritz vector
+            cpu = {"value": per * ops, "unit": "s", "cores": threads, "kind": "port",
+                   "sample": (f"FP64 oracle A^T(A x) timed over {reps} calls ({per * 1e3:.1f} ms each), projected "
+                              f"to {ops} products (Lanczos steps + 2 per Ritz vector); reorthogonalisation and "
+                              "eigensolves not included")}
+    f32 = res["f32"]
+    return {"metric": f"LAS2 seconds-to-solution, top-{k} singular triplets", "value": f32["s_per_solve"],
+            "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": f32["s_per_solve"] * 1e3, "higher_is_better": False, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f32 values, f64 recurrence", "data": "synthetic",
+            "config": {"workload": ("lanczos::svd_dim_seed(k=10, seed=42) on the 2,000 x 1,500 uniform CSR"
+                                    if cfg1 else "lanczos::svd_dim_seed(k=100, seed=42) on the 500,000 x 30,000 "
+                                    "Pareto-row CSR (nnz 19,698,610)"), "k": k, "nnz": int(a.nnz)},
+            "roofline": f32["roofline_spmv"], "cpu_baseline": cpu, "precisions": res}
+
+
+def bench_batched(args):
+    """64 randomized SVDs (rank 20, oversample 10, q = 3) over column masks of
+    the 200,000 x 25,000 matrix, every SpMM round in one launch, U and V^T left
+    in HBM, against the same 64 solves called one by one (also device outputs)."""
+    import single_svdlib as sl
+    from single_svdlib import _ffi, synth
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a, masks = synth.config4(np.float32)
+    t0 = time.perf_counter()
+    base = sl.CsrMatrix.from_scipy(a)
+    views = [MaskedCSRMatrix.with_columns(base, c) for c in masks]
+    log(f"cfg4 uploaded and 64 masks built in {time.perf_counter() - t0:.1f}s")
+    lib = _ffi.lib()
+    ctx = base._ctx
+    rank, over, q = 20, 10, 3
+    m = a.shape[0]
+    nb = len(views)
+    us = [_ffi.DeviceArray((m, rank)) for _ in views]
+    vts = [_ffi.DeviceArray((rank, len(c))) for c in masks]
+    handles = (C.c_void_p * nb)(*[v._handle for v in views])
+    up = (C.c_void_p * nb)(*[u.ptr for u in us])
+    vp = (C.c_void_p * nb)(*[v.ptr for v in vts])
+    ldu = (C.c_int64 * nb)(*[rank] * nb)
+    ldvt = (C.c_int64 * nb)(*[len(c) for c in masks])
+
+    def batched():
+        metas = (_ffi.Result_ * nb)()
+        _ffi.check(lib.svdh_rsvd_batched_device(ctx, nb, handles, rank, over, q, _ffi.NORM_QR, 7, 1, _ffi.PREC_F32,
+                                                up, ldu, vp, ldvt, metas))
+        s0 = [np.ctypeslib.as_array(metas[b].s, (rank,)).copy() for b in range(nb)]
+        for b in range(nb):
+            lib.svdh_result_free(C.byref(metas[b]))
+        return s0
+
+    for _ in range(args.warmup):
+        batched()
+    _ffi.check(lib.svdh_ctx_synchronize(ctx))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        svals = batched()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    _ffi.check(lib.svdh_ctx_profile(ctx, 1))
+    batched()
+    st = _kernel_stats(lib, ctx, (("spmm", _ffi.KC_SPMM), ("spmm_t", _ffi.KC_SPMM_T), ("dense", _ffi.KC_DENSE)), 1)
+    _ffi.check(lib.svdh_ctx_profile(ctx, 0))
+    s_host = np.zeros(rank)
+
+    def one(b):
+        meta = _ffi.Result_()
+        _ffi.check(lib.svdh_rsvd_device(ctx, views[b]._handle, rank, over, q, _ffi.NORM_QR, 0, 7, 1, None,
+                                        _ffi.PREC_F32, 0.0, q, us[b].ptr, rank, _ffi.ptr(s_host), vts[b].ptr,
+                                        len(masks[b]), C.byref(meta)))
+    for b in range(nb):
+        one(b)
+    t0 = time.perf_counter()
+    for b in range(nb):
+        one(b)
+    t_loop = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    recs = sl.randomized.randomized_svd_batched(views, rank, over, q, "QR", seed=7)
+    t_host = time.perf_counter() - t0
+    same = all(np.array_equal(r.s, sv.astype(r.s.dtype)) for r, sv in zip(recs, svals))
+    # algorithmic bytes of one batched round: the shared CSR once, the remap
+    # tables, every in-panel and out-panel (f32, l = 30)
+    n = a.shape[1]
+    l = rank + over
+    sizes = np.array([len(c) for c in masks])
+    fwd = a.nnz * 8 + (m + 1) * 4 + nb * n * 4 + int(sizes.sum()) * l * 4 + nb * m * l * 4
+    rounds = st["spmm"]["_n"] + st["spmm_t"]["_n"]
+    total_ms = st["spmm"]["_ms"] + st["spmm_t"]["_ms"]
+    roof = _roofline(fwd * rounds, total_ms, rounds)
+    check = []
+    for i in (0, 1):
+        sub = a[:, masks[i]].tocsr()
+        check.append(_triplet_residual(sub, recs[i].u, recs[i].s.astype(np.float64), recs[i].vt))
+    log(f"batched {t_batch * 1e3:.1f} ms vs one by one {t_loop * 1e3:.1f} ms; host outputs {t_host * 1e3:.1f} ms")
+    return {"metric": "batched masked randomized SVD seconds (64 masks)", "value": t_batch, "unit": "s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_batch * 1e3,
+            "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": ("randomized_svd (rank 20, oversample 10, 3 power iterations, QR) over 64 "
+                                    "column masks (sizes 500..8,000) of the 200,000 x 25,000 Poisson(60) CSR, "
+                                    "U and V^T left in HBM"), "masks": nb, "nnz": int(a.nnz)},
+            "roofline": roof, "cpu_baseline": None, "one_by_one_s": t_loop, "speedup_vs_one_by_one": t_loop / t_batch,
+            "host_output_s": t_host, "host_output_equals_device_output": same,
+            "residual_q3_first_masks": check,
+            "kernels_ms": {kname: v["ms_per_step"] for kname, v in st.items()}}
+
+
+def bench_adjoint(args):
+    """Gradients of sum of the top-32 singular values and of |U_32^T y|^2
+    with respect to the CSR values of the 100,000 x 20,000 matrix (FP32
+    factors and SpMM), compared on 1,000 random nonzeros with the FP64
+    gradient of FP64 factors."""
+    import single_svdlib as sl
+    from single_svdlib import _ffi, synth
+    a = synth.config2(np.float32)
+    lib = _ffi.lib()
+    k = 32
+    y = np.random.default_rng(20261207).normal(size=a.shape[0])
+    op = sl.CsrMatrix.from_scipy(a)
+    ctx = op._ctx
+    rec = sl.randomized.randomized_svd(op, k, 10, 3, "QR", False, 3, False, tol=1e-5, max_power_iters=400)
+    out = {}
+    for loss in ("sigma_sum", "projection"):
+        for _ in range(args.warmup):
+            rec.backward(op, loss=loss, k=k, y=y, grad_dtype=np.float32)
+        _ffi.check(lib.svdh_ctx_synchronize(ctx))
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g, info = rec.backward(op, loss=loss, k=k, y=y, grad_dtype=np.float32)
+        out[loss] = {"s": (time.perf_counter() - t0) / args.steps, "grad": g, "info": info}
+    _ffi.check(lib.svdh_ctx_profile(ctx, 1))
+    rec.backward(op, loss="sigma_sum", k=k, y=y, grad_dtype=np.float32)
+    st = _kernel_stats(lib, ctx, (("sddmm", _ffi.KC_SDDMM),), 1)
+    _ffi.check(lib.svdh_ctx_profile(ctx, 0))
+    m, n = a.shape
+    sddmm_bytes = a.nnz * (4 + 4) + (m + 1) * 4 + (m + n) * k * 4
+    roof = _roofline(sddmm_bytes * st["sddmm"]["_n"], st["sddmm"]["_ms"], st["sddmm"]["_n"])
+    # FP64 reference: FP64 factors and FP64 backward on the GPU
+    op64 = sl.CsrMatrix.from_scipy(a.astype(np.float64))
+    rec64 = sl.randomized.randomized_svd(op64, k, 10, 3, "QR", False, 3, False, tol=1e-9, max_power_iters=1000)
+    pick = np.random.default_rng(1).choice(a.nnz, 1000, replace=False)
+    errs = {}
+    for loss in ("sigma_sum", "projection"):
+        g64, _ = rec64.backward(op64, loss=loss, k=k, y=y, tol=1e-10)
+        ref = g64[pick]
+        errs[loss] = float(np.max(np.abs(out[loss]["grad"][pick] - ref)) / np.max(np.abs(ref)))
+    log(f"adjoint sigma_sum {out['sigma_sum']['s'] * 1e3:.1f} ms, projection {out['projection']['s'] * 1e3:.1f} ms "
+        f"({out['projection']['info']['cg_iterations']} CG iterations); errors vs FP64 {errs}")
+    return {"metric": "adjoint seconds (gradient of the top-32 singular values w.r.t. the CSR values)",
+            "value": out["sigma_sum"]["s"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": out["sigma_sum"]["s"] * 1e3, "higher_is_better": False, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": ("backward() of sum of the top-32 singular values and of |U_32^T y|^2 on the "
+                                    "100,000 x 20,000 Zipf CSR, factors from randomized_svd (rank 32 + 10, "
+                                    "tol 1e-5)"), "k": k, "nnz": int(a.nnz)},
+            "roofline": roof, "cpu_baseline": None,
+            "projection_loss_s": out["projection"]["s"],
+            "projection_cg_iterations": out["projection"]["info"]["cg_iterations"],
+            "max_rel_error_vs_fp64_on_1000_nonzeros": errs}
+
+
+OTHER_WORKLOADS = {"las2-cfg1": bench_las2, "las2-cfg3": bench_las2, "batched": bench_batched,
+                   "adjoint": bench_adjoint}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +341,16 @@ def main():
                     help="for rocprofv3 runs: the timed solves plus one 256 MiB device-to-device copy (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE), without the correctness and reference extras")
     ap.add_argument("--max-iters", type=int, default=400)
+    ap.add_argument("--workload", default="rsvd", choices=["rsvd", "las2-cfg1", "las2-cfg3", "batched", "adjoint"],
+                    help="rsvd (default, the headline metric) or one of the other measured paths")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a design "
               "in DESIGN.md, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "rsvd":
+        print(json.dumps(OTHER_WORKLOADS[args.workload](args)))
+        return
 
     import single_svdlib as sl
     from single_svdlib import _ffi, synth

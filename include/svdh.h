@@ -225,6 +225,17 @@ int32_t svdh_rsvd_batched(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops
                           int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
                           int32_t has_seed, int32_t precision, svdh_result *outs);
 
+/* svdh_rsvd_batched with the singular vectors left in device memory:
+ * operator b writes u[b] (m x k_b row-major, leading dimension ldu[b] >= k_b)
+ * and vt[b] (k_b x n_b row-major, ldvt[b] >= n_b), k_b = min(target_rank, m,
+ * n_b), FP64 device buffers from the same HIP runtime. metas[b] receives d,
+ * the shapes, the diagnostics and s (host, freed by svdh_result_free); its u
+ * and vt stay NULL. */
+int32_t svdh_rsvd_batched_device(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops, int64_t target_rank,
+                                 int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
+                                 int32_t has_seed, int32_t precision, double *const *u, const int64_t *ldu,
+                                 double *const *vt, const int64_t *ldvt, svdh_result *metas);
+
 /* ---- adjoint (backward) ------------------------------------------------ */
 
 /* Losses of svdh_backward. No counterpart in the reference: the adjoint sits

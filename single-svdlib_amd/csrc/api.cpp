@@ -108,7 +108,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
 
 int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int64_t target_rank,
                            int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
-                           int32_t has_seed, int32_t precision, svdh_result *outs);
+                           int32_t has_seed, int32_t precision, svdh_result *outs, const BatchedDeviceOut *dev);
 int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32_t loss, int64_t k, const double *y,
                        int32_t precision, double tol, int64_t max_iters, void *grad, int32_t grad_dtype,
                        svdh_backward_info *info);
@@ -472,10 +472,33 @@ int32_t svdh_rsvd_batched(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops
       v[b] = const_cast<svdh_op *>(ops[b]);
     }
     rsvd_batched_solve(ctx, nops, v.data(), target_rank, n_oversamples, n_power_iters, normalizer, seed, has_seed,
-                       precision, outs);
+                       precision, outs, nullptr);
   });
   if (st != SVDH_OK && outs)
     for (int64_t b = 0; b < nops; ++b) svdh_result_free(&outs[b]);
+  return st;
+}
+
+int32_t svdh_rsvd_batched_device(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops, int64_t target_rank,
+                                 int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
+                                 int32_t has_seed, int32_t precision, double *const *u, const int64_t *ldu,
+                                 double *const *vt, const int64_t *ldvt, svdh_result *metas) {
+  if (metas && nops > 0) std::memset(metas, 0, sizeof(svdh_result) * (size_t)nops);
+  int32_t st = guarded([&] {
+    check_ctx(ctx);
+    if (nops < 0 || (nops > 0 && (!ops || !metas || !u || !ldu || !vt || !ldvt))) fail(SVDH_ERR_ARG, "null argument");
+    std::vector<svdh_op *> v(nops);
+    for (int64_t b = 0; b < nops; ++b) {
+      check_op(ctx, ops[b]);
+      v[b] = const_cast<svdh_op *>(ops[b]);
+      if (!u[b] || !vt[b]) fail(SVDH_ERR_ARG, "null output buffer for operator %lld", (long long)b);
+    }
+    BatchedDeviceOut dev{u, ldu, vt, ldvt};
+    rsvd_batched_solve(ctx, nops, v.data(), target_rank, n_oversamples, n_power_iters, normalizer, seed, has_seed,
+                       precision, metas, &dev);
+  });
+  if (st != SVDH_OK && metas)
+    for (int64_t b = 0; b < nops; ++b) svdh_result_free(&metas[b]);
   return st;
 }
 

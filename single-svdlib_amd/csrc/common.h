@@ -180,9 +180,18 @@ struct svdh_op {
   // Masked view storage.
   svdh::DevBuf o2m, m2o;
   std::vector<int32_t> h_m2o;
+  // Compacted forward CSR of a masked view (built when the mask drops a
+  // quarter of the nonzeros or more): only the kept entries, column indices
+  // already in masked numbering, perm[e] = position in the base CSR.
+  bool compact = false;
+  int64_t c_nnz = 0;
+  svdh::DevBuf c_rowptr, c_colidx, c_perm, c_vals[3];
+  std::vector<int32_t> h_c_rowptr;
   // Schedules: [direction][kind], direction 0 = A x, 1 = A^T x; kind 0 = SpMV, 1 = SpMM.
   svdh::Schedule sched[2][2];
-  svdh::CsrView view(int direction) const;
+  // pattern = true: the base matrix's own pattern (with the column remap),
+  // never the compacted one (the adjoint writes one value per base nonzero).
+  svdh::CsrView view(int direction, bool pattern = false) const;
   const void *values(int direction, int dtype) const;
 };
 
@@ -218,6 +227,15 @@ struct RsvdDeviceOut {
   double *s;
   double *vt;
   int64_t ldvt;
+};
+
+// Device-resident outputs of the batched randomized solver, per operator b:
+// u[b] (m x k, ldu[b]) and vt[b] (k x n_b, ldvt[b]), FP64 device memory.
+struct BatchedDeviceOut {
+  double *const *u;
+  const int64_t *ldu;
+  double *const *vt;
+  const int64_t *ldvt;
 };
 
 // ---- vector / dense kernels (dense.hip) -----------------------------------

@@ -391,7 +391,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
 // tol = 0, so its result is the one a separate solve would return.
 int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int64_t target_rank,
                            int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
-                           int32_t has_seed, int32_t precision, svdh_result *outs) {
+                           int32_t has_seed, int32_t precision, svdh_result *outs, const BatchedDeviceOut *dev) {
   if (nops <= 0) return SVDH_OK;
   if (normalizer != SVDH_NORM_QR) fail(SVDH_ERR_ARG, "the batched solver supports the QR normaliser");
   if (precision != SVDH_PREC_F64 && precision != SVDH_PREC_F32) fail(SVDH_ERR_ARG, "bad precision %d", precision);
@@ -507,25 +507,33 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
       }
     }
     svdh_result *out = &outs[b];
-    double *ud = static_cast<double *>(ctx->workspace(WS_U, std::max<size_t>(m * k, 1) * 8));
-    double *vtd = static_cast<double *>(ctx->workspace(WS_VT, std::max<size_t>(k * n, 1) * 8));
-    panel_apply(c, m, l, k, Y[b], ld, pdt, MW.data(), ud, k, SVDH_F64);
-    panel_apply(c, n, w, k, Z[b], ld, pdt, Ws.data(), vtd, 1, SVDH_F64, n);
     out->d = k;
     out->u_rows = m;
     out->u_cols = k;
     out->vt_rows = k;
     out->vt_cols = n;
     out->s = static_cast<double *>(std::calloc(std::max<int64_t>(k, 1), sizeof(double)));
-    out->u = static_cast<double *>(std::calloc(std::max<int64_t>(m * k, 1), sizeof(double)));
-    out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(k * n, 1), sizeof(double)));
-    if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
+    if (!out->s) fail(SVDH_ERR_ALLOC, "result allocation failed");
     for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
-    if (k > 0) {
-      HIP_CHECK(hipMemcpyAsync(out->u, ud, (size_t)m * k * 8, hipMemcpyDeviceToHost, c->stream));
-      HIP_CHECK(hipMemcpyAsync(out->vt, vtd, (size_t)k * n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (dev) {  // U and V^T straight into the caller's device buffers
+      if (dev->ldu[b] < k || dev->ldvt[b] < n) fail(SVDH_ERR_ARG, "leading dimension too small for operator %lld",
+                                                    (long long)b);
+      panel_apply(c, m, l, k, Y[b], ld, pdt, MW.data(), dev->u[b], dev->ldu[b], SVDH_F64);
+      panel_apply(c, n, w, k, Z[b], ld, pdt, Ws.data(), dev->vt[b], 1, SVDH_F64, dev->ldvt[b]);
+    } else {
+      double *ud = static_cast<double *>(ctx->workspace(WS_U, std::max<size_t>(m * k, 1) * 8));
+      double *vtd = static_cast<double *>(ctx->workspace(WS_VT, std::max<size_t>(k * n, 1) * 8));
+      panel_apply(c, m, l, k, Y[b], ld, pdt, MW.data(), ud, k, SVDH_F64);
+      panel_apply(c, n, w, k, Z[b], ld, pdt, Ws.data(), vtd, 1, SVDH_F64, n);
+      out->u = static_cast<double *>(std::calloc(std::max<int64_t>(m * k, 1), sizeof(double)));
+      out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(k * n, 1), sizeof(double)));
+      if (!out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
+      if (k > 0) {
+        HIP_CHECK(hipMemcpyAsync(out->u, ud, (size_t)m * k * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipMemcpyAsync(out->vt, vtd, (size_t)k * n * 8, hipMemcpyDeviceToHost, c->stream));
+      }
+      HIP_CHECK(hipStreamSynchronize(c->stream));  // U/V workspaces are reused by the next operator
     }
-    HIP_CHECK(hipStreamSynchronize(c->stream));  // U/V workspaces are reused by the next operator
     svdh_diagnostics &dg = out->diagnostics;  // randomized/mod.rs:214-237
     dg.non_zero = op->nnz;
     dg.dimensions = target_rank;
@@ -543,6 +551,7 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
     out->residual_estimate = -1.0;
     out->qr_fallbacks = fallbacks[b];
   }
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   return SVDH_OK;
 }
 

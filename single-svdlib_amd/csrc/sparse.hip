@@ -550,8 +550,14 @@ static void convert_values(Ctx *c, const void *src, int sdt, void *dst, int ddt,
   HIP_CHECK(hipGetLastError());
 }
 
+static void gather_compact_values(svdh_op *op, int dtype);
+
 void ensure_values(svdh_op *op, int dtype) {
   svdh_op *owner = const_cast<svdh_op *>(op->base ? op->base : op);
+  if (op->base && op->compact && !op->c_vals[dtype].p) {
+    if (!owner->vals[dtype].p) ensure_values(owner, dtype);
+    gather_compact_values(op, dtype);
+  }
   if (owner->vals[dtype].p) return;
   if (owner->base_dtype == SVDH_BF16)
     fail(SVDH_ERR_ARG, "operator values are BF16; wider copies are not derivable");
@@ -569,9 +575,19 @@ void ensure_values(svdh_op *op, int dtype) {
 
 using namespace svdh;
 
-svdh::CsrView svdh_op::view(int direction) const {
+svdh::CsrView svdh_op::view(int direction, bool pattern) const {
   CsrView v;
   const svdh_op *own = base ? base : this;
+  if (direction == 0 && base && compact && !pattern) {
+    v.nrows = nrows;
+    v.ncols = ncols;
+    v.rowptr = c_rowptr.as<int32_t>();
+    v.colidx = c_colidx.as<int32_t>();
+    for (int d = 0; d < 3; ++d) v.vals[d] = c_vals[d].p;
+    v.h_rowptr = const_cast<std::vector<int32_t> *>(&h_c_rowptr);
+    v.nnz = c_nnz;
+    return v;
+  }
   if (direction == 0) {
     v.nrows = nrows;
     v.ncols = ncols;
@@ -1152,7 +1168,7 @@ template <typename T, typename G>
 static void sddmm_t(svdh_op *op, int64_t k, const void *L1, const void *R1, const double *s1, const void *L2,
                     const void *R2, const double *s2, int64_t ldl, int64_t ldr, void *grad) {
   Ctx *c = op->ctx;
-  CsrView v = op->view(0);
+  CsrView v = op->view(0, true);
   constexpr int NV = PanelVec<T>::N;
   if (ldl % NV || ldr % NV) fail(SVDH_ERR_ARG, "factor leading dimensions must be multiples of %d", NV);
   SddmmArgs<T, G> a;
@@ -1225,6 +1241,7 @@ void panel_to_bf16(Ctx *c, const float *src, uint16_t *dst, int64_t n) {
 }  // namespace svdh
 
 const void *svdh_op::values(int direction, int dtype) const {
+  if (direction == 0 && base && compact) return c_vals[dtype].p;
   const svdh_op *own = base ? base : this;
   return direction == 0 ? own->vals[dtype].p : own->t_vals[dtype].p;
 }
@@ -1299,6 +1316,98 @@ svdh_op *csr_create(Ctx *c, int64_t nrows, int64_t ncols, int64_t nnz, const int
   return op;
 }
 
+// Compaction of a masked view's forward CSR: one wave per row counts the kept
+// entries, the host scans the counts, and one wave per row writes them in
+// order (ballot + popcount give each kept entry its slot).
+__global__ __launch_bounds__(kBlock) void k_mask_count(int64_t m, const int32_t *rowptr, const int32_t *colidx,
+                                                      const int32_t *o2m, int32_t *cnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (r >= m) return;
+  int n = 0;
+  for (int k = rowptr[r] + lane; k < rowptr[r + 1]; k += 64) n += o2m[colidx[k]] >= 0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  if (lane == 0) cnt[r] = n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mask_fill(int64_t m, const int32_t *rowptr, const int32_t *colidx,
+                                                     const int32_t *o2m, const int32_t *c_rowptr, int32_t *c_colidx,
+                                                     int32_t *perm) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (r >= m) return;
+  int out = c_rowptr[r];
+  const int e = rowptr[r + 1];
+  for (int k0 = rowptr[r]; k0 < e; k0 += 64) {
+    const int k = k0 + lane;
+    const int mc = k < e ? o2m[colidx[k]] : -1;
+    const uint64_t keep = __ballot(mc >= 0);
+    if (mc >= 0) {
+      const int pos = out + __popcll(keep & ((1ull << lane) - 1ull));
+      c_colidx[pos] = mc;
+      perm[pos] = k;
+    }
+    out += __popcll(keep);
+  }
+}
+
+template <typename T>
+__global__ void k_gather(int64_t n, const int32_t *perm, const T *src, T *dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[perm[i]];
+}
+
+static void gather_compact_values(svdh_op *op, int dtype) {
+  const svdh_op *own = op->base;
+  Ctx *c = op->ctx;
+  const int64_t n = op->c_nnz;
+  op->c_vals[dtype].alloc(std::max<size_t>(n * dtype_size(dtype), 16));
+  if (n > 0) {
+    const int g = grid_for(n);
+    const int32_t *perm = op->c_perm.as<int32_t>();
+    if (dtype == SVDH_F64)
+      hipLaunchKernelGGL(k_gather<double>, g, kBlock, 0, c->stream, n, perm, own->vals[dtype].as<double>(),
+                         op->c_vals[dtype].as<double>());
+    else if (dtype == SVDH_F32)
+      hipLaunchKernelGGL(k_gather<float>, g, kBlock, 0, c->stream, n, perm, own->vals[dtype].as<float>(),
+                         op->c_vals[dtype].as<float>());
+    else
+      hipLaunchKernelGGL(k_gather<bf16_t>, g, kBlock, 0, c->stream, n, perm, own->vals[dtype].as<bf16_t>(),
+                         op->c_vals[dtype].as<bf16_t>());
+    HIP_CHECK(hipGetLastError());
+  }
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+static void build_compact(Ctx *c, svdh_op *op) {
+  const svdh_op *own = op->base;
+  const int64_t m = own->nrows;
+  DevBuf cnt(std::max<size_t>(m * 4, 16));
+  const int g = (int)std::max<int64_t>(1, (m * 64 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_mask_count, g, kBlock, 0, c->stream, m, own->rowptr.as<int32_t>(), own->colidx.as<int32_t>(),
+                     op->o2m.as<int32_t>(), cnt.as<int32_t>());
+  HIP_CHECK(hipGetLastError());
+  std::vector<int32_t> h(m);
+  if (m) HIP_CHECK(hipMemcpyAsync(h.data(), cnt.p, m * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  op->h_c_rowptr.assign(m + 1, 0);
+  for (int64_t r = 0; r < m; ++r) op->h_c_rowptr[r + 1] = op->h_c_rowptr[r] + h[r];
+  op->c_nnz = op->h_c_rowptr[m];
+  op->c_rowptr.alloc((m + 1) * 4);
+  HIP_CHECK(hipMemcpyAsync(op->c_rowptr.p, op->h_c_rowptr.data(), (m + 1) * 4, hipMemcpyHostToDevice, c->stream));
+  op->c_colidx.alloc(std::max<size_t>(op->c_nnz * 4, 16));
+  op->c_perm.alloc(std::max<size_t>(op->c_nnz * 4, 16));
+  hipLaunchKernelGGL(k_mask_fill, g, kBlock, 0, c->stream, m, own->rowptr.as<int32_t>(), own->colidx.as<int32_t>(),
+                     op->o2m.as<int32_t>(), op->c_rowptr.as<int32_t>(), op->c_colidx.as<int32_t>(),
+                     op->c_perm.as<int32_t>());
+  HIP_CHECK(hipGetLastError());
+  op->compact = true;
+  for (int d = 0; d < 3; ++d)
+    if (own->vals[d].p) gather_compact_values(op, d);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
 svdh_op *mask_create(Ctx *c, const svdh_op *base, const uint8_t *mask) {
   if (!base || !mask) fail(SVDH_ERR_ARG, "null base or mask");
   if (base->base) fail(SVDH_ERR_ARG, "a masked view cannot be masked again");
@@ -1324,6 +1433,9 @@ svdh_op *mask_create(Ctx *c, const svdh_op *base, const uint8_t *mask) {
     op->m2o.alloc(std::max<size_t>(op->h_m2o.size() * 4, 16));
     if (!op->h_m2o.empty())
       HIP_CHECK(hipMemcpy(op->m2o.p, op->h_m2o.data(), op->h_m2o.size() * 4, hipMemcpyHostToDevice));
+    // Forward products of a mask that drops a quarter of the nonzeros or more
+    // read a compacted copy instead of skipping entries of the full rows.
+    if (4 * nnz <= 3 * base->nnz) build_compact(c, op);
   } catch (...) {
     delete op;
     throw;

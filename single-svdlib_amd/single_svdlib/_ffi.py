@@ -96,6 +96,8 @@ SIGNATURES = [
                                 _i64, _vp, _i64, _f64p, _vp, _i64, C.POINTER(Result_)]),
     ("svdh_rsvd_batched", _i32, [_vp, _i64, C.POINTER(_vp), _i64, _i64, _i64, _i32, C.c_uint64, _i32, _i32,
                                  C.POINTER(Result_)]),
+    ("svdh_rsvd_batched_device", _i32, [_vp, _i64, C.POINTER(_vp), _i64, _i64, _i64, _i32, C.c_uint64, _i32, _i32,
+                                        C.POINTER(_vp), _i64p, C.POINTER(_vp), _i64p, C.POINTER(Result_)]),
     ("svdh_backward", _i32, [_vp, _vp, C.POINTER(Result_), _i32, _i64, _f64p, _i32, C.c_double, _i64, _vp, _i32,
                              C.POINTER(BackwardInfo_)]),
     ("svdh_result_free", None, [C.POINTER(Result_)]),

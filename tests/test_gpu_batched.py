@@ -65,3 +65,33 @@ def test_batched_rejects_mixed_widths_and_normalizers(lib):
         lib.randomized.randomized_svd_batched([small, big], 10, 5, 1, "QR", seed=1)
     with pytest.raises(ValueError, match="QR"):
         lib.randomized.randomized_svd_batched([big], 10, 5, 1, "LU", seed=1)
+
+
+def test_batched_device_outputs_equal_host_outputs(lib):
+    import ctypes as C
+    from single_svdlib import _ffi
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(2000, 800, 0.02, seed=29)
+    base = lib.CsrMatrix.from_scipy(a)
+    cols = masks_of(800, [150, 500, 60], seed=2)
+    views = [MaskedCSRMatrix.with_columns(base, c) for c in cols]
+    recs = lib.randomized.randomized_svd_batched(views, 6, 6, 2, "QR", seed=3)
+    k = 6
+    us = [_ffi.DeviceArray((2000, k + 2)) for _ in cols]
+    vts = [_ffi.DeviceArray((k, len(c) + 3)) for c in cols]
+    nb = len(cols)
+    metas = (_ffi.Result_ * nb)()
+    handles = (C.c_void_p * nb)(*[v._handle for v in views])
+    up = (C.c_void_p * nb)(*[u.ptr for u in us])
+    vp = (C.c_void_p * nb)(*[v.ptr for v in vts])
+    ldu = (C.c_int64 * nb)(*[k + 2] * nb)
+    ldvt = (C.c_int64 * nb)(*[len(c) + 3 for c in cols])
+    _ffi.check(_ffi.lib().svdh_rsvd_batched_device(_ffi.context(0), nb, handles, 6, 6, 2, _ffi.NORM_QR, 3, 1,
+                                                   _ffi.PREC_F64, up, ldu, vp, ldvt, metas))
+    for b in range(nb):
+        assert metas[b].d == k and not metas[b].u and not metas[b].vt
+        s = np.ctypeslib.as_array(metas[b].s, (k,)).copy()
+        _ffi.lib().svdh_result_free(C.byref(metas[b]))
+        np.testing.assert_array_equal(s, recs[b].s)
+        np.testing.assert_array_equal(us[b].numpy()[:, :k], recs[b].u)
+        np.testing.assert_array_equal(vts[b].numpy()[:, :len(cols[b])], recs[b].vt)

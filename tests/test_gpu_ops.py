@@ -117,30 +117,33 @@ def test_spmm_f32(svdlib, name):
     check_close(Y, want, bound(a64, X.astype(np.float64), EPS32) + EPS32 * np.abs(want))
 
 
-def test_masked_ops_equal_physical_subset(svdlib):
+@pytest.mark.parametrize("keep", [400, 1450])
+def test_masked_ops_equal_physical_subset(svdlib, keep):
+    # 400 of 1500 columns: the forward products read a compacted copy of the
+    # kept entries; 1450: they skip dropped entries through the column remap
     from single_svdlib.lanczos.masked import MaskedCSRMatrix
     a = skewed_csr(5000, 1500, seed=8, mean_row=16, max_row=2000)
-    cols = np.sort(np.random.default_rng(3).choice(1500, 400, replace=False))
+    cols = np.sort(np.random.default_rng(3).choice(1500, keep, replace=False))
     base = svdlib.CsrMatrix.from_scipy(a)
     m = MaskedCSRMatrix.with_columns(base, cols)
     sub = a[:, cols]
     ref = oracle.OracleOp(a, columns=cols)
-    assert m.nrows() == 5000 and m.ncols() == 400 and m.nnz() == sub.nnz == ref.nnz()
+    assert m.nrows() == 5000 and m.ncols() == keep and m.nnz() == sub.nnz == ref.nnz()
     rng = np.random.default_rng(4)
-    x = rng.normal(size=400)
+    x = rng.normal(size=keep)
     y = np.empty(5000)
     m.svd_opa(x, y, False)
     check_close(y, ref.opa(x), bound(sub, x, EPS64))
     xt = rng.normal(size=5000)
-    yt = np.empty(400)
+    yt = np.empty(keep)
     m.svd_opa(xt, yt, True)
     check_close(yt, ref.opa(xt, True), bound(sub.T, xt, EPS64))
-    X = rng.normal(size=(400, 30))
+    X = rng.normal(size=(keep, 30))
     Y = np.zeros((5000, 30))
     m.multiply_with_dense(X, Y, False)
     check_close(Y, ref.multiply_with_dense(X), bound(sub, X, EPS64))
     Yin = rng.normal(size=(5000, 30))
-    Z = np.zeros((400, 30))
+    Z = np.zeros((keep, 30))
     m.multiply_with_dense(Yin, Z, True)
     check_close(Z, ref.multiply_with_dense(Yin, True), bound(sub.T, Yin, EPS64))
     means = m.compute_column_means()
