@@ -148,6 +148,10 @@ struct Schedule {
   int32_t blk_off[8] = {0};
   int32_t blk_cnt[8] = {0};
   int32_t nslots = 0;
+  // Short rows cut at source blocks: kSegInts per segment, grouped by block.
+  DevBuf segs;
+  int32_t seg_off[8] = {0};
+  int32_t seg_cnt[8] = {0};
 };
 
 // One direction of an operator as a CSR: rows of this matrix are output rows.

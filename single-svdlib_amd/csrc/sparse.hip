@@ -267,6 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(SpmvArgs<V, X, Y> a) {
 // ------------------------------------------------------------------------
 
 constexpr int kChunkInts = 8;  // r, b, e, slot, first_slot, nchunks, row_id (unused), pad
+constexpr int kSegInts = 4;    // r, b, e, slot (-1: the row's only segment, written to y)
 #ifndef SVDH_SPMM_UNROLL
 #define SVDH_SPMM_UNROLL 4  // panel gathers in flight per row group
 #endif
@@ -288,7 +289,11 @@ struct SpmmArgs {
   int32_t nshort;
   int32_t nblk;
   int32_t chunk_blocks;  // nblk * max chunks per block
-  int32_t total_blocks;  // chunk blocks + short-row blocks
+  int32_t seg_blocks;    // nblk * max segment workgroups per block
+  int32_t total_blocks;  // chunk blocks + segment blocks + short-row blocks
+  const int32_t *segs;   // kSegInts per segment, grouped by source block
+  int32_t seg_off[8];
+  int32_t seg_cnt[8];
   int32_t blk_off[8];
   int32_t blk_cnt[8];
 };
@@ -367,6 +372,44 @@ __device__ __forceinline__ void spmm_short(const SpmmArgs<V, X, Y, Acc> &a, int 
     for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
     spmm_accum<R, MC>(a, kb, ke, lane, col, colok, acc);
     if (colok) st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
+  }
+}
+
+// Segments: the part of a short row that falls in one source block, one row
+// group each; workgroup w takes segments of block w % nblk only. A row with
+// several segments writes one partial per segment (summed by k_spmm_fin).
+template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+__device__ __forceinline__ void spmm_segment(const SpmmArgs<V, X, Y, Acc> &a, int w) {
+  constexpr int R = 1 << LGR;
+  constexpr int NV = PanelVec<X>::N;
+  constexpr int RPB = kBlock >> LGR;
+  const int xb = w % a.nblk, kb = w / a.nblk;
+  int off = 0, cnt = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (i == xb) {
+      off = a.seg_off[i];
+      cnt = a.seg_cnt[i];
+    }
+  const int gid = threadIdx.x >> LGR;
+  const int lane = threadIdx.x & (R - 1);
+  const int idx = kb * RPB + gid;
+  if (idx >= cnt) return;  // the whole row group leaves together
+  const int *sd = a.segs + (int64_t)(off + idx) * kSegInts;
+  const int r = sd[0], kb0 = sd[1], ke = sd[2], slot = sd[3];
+  for (int c0 = 0; c0 < a.l; c0 += R * NV) {
+    const int col = c0 + lane * NV;
+    const bool colok = col < a.l;
+    Acc acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
+    spmm_accum<R, MC>(a, kb0, ke, lane, col, colok, acc);
+    if (colok) {
+      if (slot < 0)
+        st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
+      else
+        st_panel<Acc, Acc, NV>(a.partials + (int64_t)slot * a.ldp + col, acc);
+    }
   }
 }
 
@@ -478,8 +521,10 @@ __device__ __forceinline__ void spmm_block(const SpmmArgs<V, X, Y, Acc> &a, int 
         cnt = a.blk_cnt[i];
       }
     if (kb < cnt) spmm_chunk<LGR, MC>(a, off + kb);
+  } else if (w < a.chunk_blocks + a.seg_blocks) {
+    spmm_segment<LGR, MC>(a, w - a.chunk_blocks);
   } else {
-    spmm_short<LGR, MC>(a, w - a.chunk_blocks);
+    spmm_short<LGR, MC>(a, w - a.chunk_blocks - a.seg_blocks);
   }
 }
 
@@ -741,23 +786,31 @@ static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
   const int64_t nout = v.nrows, nsrc = v.ncols;
   auto src_of = [&](int64_t r) { return v.h_rowmap ? (int64_t)(*v.h_rowmap)[r] : r; };
   const SpmmTune tune = spmm_tune();
-  std::vector<int32_t> shortr, longr;
-  for (int64_t r = 0; r < nout; ++r) {
-    const int64_t sr = src_of(r);
-    (rp[sr + 1] - rp[sr] > tune.long_nnz ? longr : shortr).push_back((int32_t)r);
-  }
-  std::stable_sort(shortr.begin(), shortr.end(), [&](int32_t a, int32_t b) {
-    return rp[src_of(a) + 1] - rp[src_of(a)] > rp[src_of(b) + 1] - rp[src_of(b)];
-  });
   int nblk = (nsrc >= 65536 && !(direction == 0 && op->base)) ? tune.nblk : 1;
-  for (size_t i = 0; i < longr.size() && nblk > 1; ++i) {
-    const int64_t sr = src_of(longr[i]);
+  // every row's source indices must be sorted for the block cuts
+  for (int64_t r = 0; r < nout && nblk > 1; ++r) {
+    const int64_t sr = src_of(r);
     for (int32_t k = rp[sr] + 1; k < rp[sr + 1]; ++k)
       if (ci[k - 1] > ci[k]) {
         nblk = 1;
         break;
       }
   }
+  // With source blocks, rows below split_min become per-block segments (one
+  // row group each) and longer rows chunks cut at the block bounds; without
+  // them, rows above long_nnz are chunked and the rest are short rows.
+  std::vector<int32_t> shortr, longr, segr;
+  for (int64_t r = 0; r < nout; ++r) {
+    const int64_t sr = src_of(r);
+    const int32_t len = rp[sr + 1] - rp[sr];
+    if (nblk > 1)
+      (len >= tune.split_min ? longr : segr).push_back((int32_t)r);
+    else
+      (len > tune.long_nnz ? longr : shortr).push_back((int32_t)r);
+  }
+  std::stable_sort(shortr.begin(), shortr.end(), [&](int32_t a, int32_t b) {
+    return rp[src_of(a) + 1] - rp[src_of(a)] > rp[src_of(b) + 1] - rp[src_of(b)];
+  });
   struct Ch { int32_t r, b, e, slot, first, n, rid, blk; };
   std::vector<std::vector<Ch>> per(nblk);
   std::vector<Ch> row;
@@ -805,6 +858,54 @@ static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
         multi.push_back(g.first);
         multi.push_back(g.n);
       }
+  // segments of the short rows, one per non-empty source block
+  std::vector<std::vector<int32_t>> segper(nblk);
+  for (int32_t r : segr) {
+    const int64_t sr = src_of(r);
+    const int32_t b0 = rp[sr], e0 = rp[sr + 1];
+    int32_t cuts[9];
+    cuts[0] = b0;
+    for (int x = 0; x < nblk; ++x) {
+      int32_t hi = e0;
+      if (x + 1 < nblk) {
+        const int32_t bound = (int32_t)(nsrc * (x + 1) / nblk);
+        hi = (int32_t)(std::lower_bound(ci.begin() + cuts[x], ci.begin() + e0, bound) - ci.begin());
+      }
+      cuts[x + 1] = hi;
+    }
+    int nseg = 0;
+    for (int x = 0; x < nblk; ++x) nseg += cuts[x + 1] > cuts[x];
+    if (nseg == 0) {  // empty row: one segment writes its zeros
+      segper[0].insert(segper[0].end(), {r, b0, e0, -1});
+      continue;
+    }
+    const int32_t first = nslots;
+    for (int x = 0; x < nblk; ++x)
+      if (cuts[x + 1] > cuts[x]) segper[x].insert(segper[x].end(), {r, cuts[x], cuts[x + 1], nseg > 1 ? nslots++ : -1});
+    if (nseg > 1) {
+      multi.push_back(r);
+      multi.push_back(first);
+      multi.push_back(nseg);
+      ++nmulti;
+    }
+  }
+  {
+    std::vector<int32_t> sflat;
+    int32_t soff = 0;
+    for (int x = 0; x < 8; ++x) {
+      s.seg_off[x] = 0;
+      s.seg_cnt[x] = 0;
+    }
+    for (int x = 0; x < nblk; ++x) {
+      s.seg_off[x] = soff;
+      s.seg_cnt[x] = (int32_t)(segper[x].size() / kSegInts);
+      soff += s.seg_cnt[x];
+      sflat.insert(sflat.end(), segper[x].begin(), segper[x].end());
+    }
+    s.segs.alloc(std::max<size_t>(sflat.size(), kSegInts) * sizeof(int32_t));
+    if (!sflat.empty())
+      HIP_CHECK(hipMemcpy(s.segs.p, sflat.data(), sflat.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
   std::vector<int32_t> flat;
   int32_t off = 0, maxcnt = 0;
   s.nblk = nblk;
@@ -933,10 +1034,19 @@ static bool plan_spmm(svdh_op *op, int direction, const void *vals, const void *
   while ((1 << lgr) * NV < l && lgr < 6) ++lgr;
   const int rpb = kBlock >> lgr;
   const int nshort = s.bins[0].row_count;
-  const int blocks = s.total_blocks + (nshort + rpb - 1) / rpb;
+  int seg_max = 0;
+  for (int x = 0; x < s.nblk; ++x) seg_max = std::max(seg_max, (s.seg_cnt[x] + rpb - 1) / rpb);
+  const int seg_blocks = seg_max * s.nblk;
+  const int blocks = s.total_blocks + seg_blocks + (nshort + rpb - 1) / rpb;
   const int64_t ldp = (l + 7) / 8 * 8;
   s.partials.ensure((size_t)std::max<int32_t>(s.nslots, 1) * ldp * sizeof(Acc));
   SpmmArgs<V, X, Y, Acc> &a = p.a;
+  a.segs = s.segs.as<int32_t>();
+  a.seg_blocks = seg_blocks;
+  for (int x = 0; x < 8; ++x) {
+    a.seg_off[x] = s.seg_off[x];
+    a.seg_cnt[x] = x < s.nblk ? s.seg_cnt[x] : 0;
+  }
   a.rowptr = v.rowptr;
   a.colidx = v.colidx;
   a.vals = static_cast<const V *>(vals);

@@ -161,3 +161,39 @@ def test_empty_matrix_and_shape_errors(svdlib):
         op.svd_opa(np.ones(4), y, False)
     with pytest.raises(ValueError):
         svdlib.CsrMatrix(np.array([0, 1]), np.array([7]), np.array([1.0]), (1, 3))
+
+
+def tall_zipf(m, n, seed, dtype=np.float64):
+    """A tall matrix (m >= 65,536 rows, so A^T products cut rows at source
+    blocks): Zipf-popular columns give long CSC rows (chunks), short ones
+    (segments) and empty columns."""
+    rng = np.random.default_rng(seed)
+    nnz = 6 * m
+    pop = 1.0 / np.arange(1, n + 1) ** 1.2
+    pop /= pop.sum()
+    cols = rng.choice(n, size=nnz, p=pop)
+    rows = rng.integers(0, m, size=nnz)
+    a = sp.csr_matrix((rng.normal(size=nnz), (rows, cols)), shape=(m, n + 5))  # 5 empty columns
+    a.sum_duplicates()
+    return a.astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmm_transposed_source_blocks(svdlib, dtype):
+    a = tall_zipf(80_000, 700, seed=5, dtype=dtype)
+    print(f"case tall {a.shape} nnz={a.nnz} dtype={np.dtype(dtype).name}", flush=True)
+    op = svdlib.CsrMatrix.from_scipy(a)
+    a64 = a.astype(np.float64)
+    rng = np.random.default_rng(1)
+    eps = EPS64 if dtype == np.float64 else EPS32
+    for l in (8, 30, 60):
+        Yin = rng.normal(size=(a.shape[0], l)).astype(dtype)
+        Z = np.zeros((a.shape[1], l), dtype=dtype)
+        op.multiply_with_dense(Yin, Z, True)
+        want = a64.T @ Yin.astype(np.float64)
+        check_close(Z, want, bound(a64.T, Yin.astype(np.float64), eps) + eps * np.abs(want))
+        assert np.all(Z[-5:] == 0)
+    y = rng.normal(size=a.shape[0])
+    z = np.empty(a.shape[1])
+    op.svd_opa(y.astype(np.float64), z, True)
+    check_close(z, a64.T @ y, bound(a64.T, y, EPS64 if dtype == np.float64 else EPS32))
