@@ -165,6 +165,10 @@ int32_t svdh_ctx_create(int32_t device, svdh_ctx **out) {
     try {
       c->device = device;
       HIP_CHECK(hipStreamCreate(&c->stream));
+      int ncu = 0;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        c->ncu = ncu;
+      (void)hipGetLastError();
       c->scal.alloc(S_NSLOT * sizeof(double));
       HIP_CHECK(hipMemset(c->scal.p, 0, S_NSLOT * sizeof(double)));
       HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_scal), S_NSLOT * sizeof(double), hipHostMallocDefault));

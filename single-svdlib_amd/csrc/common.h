@@ -56,6 +56,7 @@ enum KernelClass {
 };
 
 struct Ctx {
+  int ncu = 256;  // compute units of the device
   int device = 0;
   hipStream_t stream = nullptr;
   // Device scalar block written by reduction kernels, and its pinned mirror.
@@ -148,6 +149,11 @@ struct Schedule {
   int32_t blk_off[8] = {0};
   int32_t blk_cnt[8] = {0};
   int32_t nslots = 0;
+  // A x with staged hot columns: hot_h rows staged (0: not used, -1: not
+  // decided yet), hot_colidx = column indices with the staged ones encoded as
+  // -(1 + slot), hot_rows = the staged columns by slot.
+  int hot_h = -1;
+  DevBuf hot_colidx, hot_rows;
   // Short rows cut at source blocks: kSegInts per segment, grouped by block.
   DevBuf segs;
   int32_t seg_off[8] = {0};

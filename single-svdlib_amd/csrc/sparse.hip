@@ -508,6 +508,112 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin_batched(const FinArgs<Y, Ac
   spmm_fin_row<Y, Acc>(f.multi, f.partials, f.ldp, f.y, f.ldy, f.l, (int)blockIdx.x - boff[b]);
 }
 
+// ------------------------------------------------------------------------
+// A X with the panel rows of the most popular columns staged in LDS
+// (power-law column popularity: a few hundred columns hold half of the
+// nonzeros). hcol is the column index array with those columns encoded as
+// -(1 + slot); a persistent grid of 1024-thread workgroups (two per CU, eight
+// waves per SIMD) copies the h staged rows once and then strides over the
+// short rows. Gathers of staged rows read LDS, the others global memory.
+// ------------------------------------------------------------------------
+constexpr int kHotBlock = 1024;
+
+template <int R, typename V, typename X, typename Y, typename Acc>
+__device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, const int32_t *hcol, const X *xs,
+                                               int kb, int ke, int lane, int col, bool colok,
+                                               Acc (&acc)[PanelVec<X>::N]) {
+  constexpr int NV = PanelVec<X>::N;
+  for (int base = kb; base < ke; base += R) {
+    int c = 0;
+    Acc v = Acc(0);
+    const int kk = base + lane;
+    if (kk < ke) {
+      c = __builtin_nontemporal_load(&hcol[kk]);
+      v = ld_as<Acc>(a.vals, kk);
+    }
+    const int n = min(R, ke - base);
+    constexpr int U = SVDH_SPMM_UNROLL < R ? SVDH_SPMM_UNROLL : R;
+    int j = 0;
+    for (; j + U <= n; j += U) {
+      int cj[U];
+      Acc vj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        cj[u] = __shfl(c, j + u, R);
+        vj[u] = __shfl(v, j + u, R);
+      }
+      if (colok) {
+        Acc xv[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (cj[u] >= 0)
+            ld_panel<Acc, X>(a.x + (int64_t)cj[u] * a.ldx + col, xv[u]);
+          else
+            ld_panel<Acc, X>(xs + (int64_t)(-1 - cj[u]) * a.ldx + col, xv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int i = 0; i < NV; ++i) acc[i] += vj[u] * xv[u][i];
+      }
+    }
+    for (; j < n; ++j) {
+      const int cj = __shfl(c, j, R);
+      const Acc vj = __shfl(v, j, R);
+      if (colok) {
+        Acc xv[NV];
+        if (cj >= 0)
+          ld_panel<Acc, X>(a.x + (int64_t)cj * a.ldx + col, xv);
+        else
+          ld_panel<Acc, X>(xs + (int64_t)(-1 - cj) * a.ldx + col, xv);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += vj * xv[i];
+      }
+    }
+  }
+}
+
+template <int LGR, typename V, typename X, typename Y, typename Acc>
+__global__ __launch_bounds__(kHotBlock) void k_spmm_hot(SpmmArgs<V, X, Y, Acc> a, const int32_t *hcol,
+                                                        const int32_t *hot_rows, int h) {
+  extern __shared__ uint4 xs_raw[];
+  X *xs = reinterpret_cast<X *>(xs_raw);
+  constexpr int R = 1 << LGR;
+  constexpr int NV = PanelVec<X>::N;
+  constexpr int RPB = kHotBlock >> LGR;
+  const int vpr = (int)(a.ldx / NV);  // 16-byte vectors per panel row
+  for (int e = threadIdx.x; e < h * vpr; e += kHotBlock) {
+    const int r = e / vpr, q = e - r * vpr;
+    xs_raw[e] = reinterpret_cast<const uint4 *>(a.x + (int64_t)hot_rows[r] * a.ldx)[q];
+  }
+  __syncthreads();
+  const int gid = threadIdx.x >> LGR;
+  const int lane = threadIdx.x & (R - 1);
+  for (int64_t b0 = (int64_t)blockIdx.x * RPB; b0 < a.nshort; b0 += (int64_t)gridDim.x * RPB) {
+    const int64_t idx = b0 + gid;
+    if (idx >= a.nshort) break;
+    const int r = a.rows[idx];
+    const int kb = a.rowptr[r], ke = a.rowptr[r + 1];
+    for (int c0 = 0; c0 < a.l; c0 += R * NV) {
+      const int col = c0 + lane * NV;
+      const bool colok = col < a.l;
+      Acc acc[NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
+      spmm_accum_hot<R>(a, hcol, xs, kb, ke, lane, col, colok, acc);
+      if (colok) st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
+    }
+  }
+}
+
+__global__ void k_hot_encode(int64_t nnz, const int32_t *colidx, const int32_t *slot, int32_t *out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = colidx[e];
+    const int s = slot[c];
+    out[e] = s >= 0 ? -1 - s : c;
+  }
+}
+
 template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
 __device__ __forceinline__ void spmm_block(const SpmmArgs<V, X, Y, Acc> &a, int w) {
   if (w < a.chunk_blocks) {
@@ -1078,12 +1184,75 @@ static bool plan_spmm(svdh_op *op, int direction, const void *vals, const void *
   return blocks > 0;
 }
 
+// Staged hot columns for A x of a plain operator whose rows are all short:
+// h = as many panel rows as fit 64 KB of LDS, used when the h most popular
+// columns hold at least a fifth of the nonzeros. Returns h (0: not used).
+static int hot_columns(svdh_op *op, Schedule &s, int h) {
+  if (op->base || s.total_blocks > 0 || s.nblk > 1 || h < 16 || op->ncols < 4 * (int64_t)h) return 0;
+  const char *env = std::getenv("SVDH_SPMM_HOT");
+  if (env && std::atoi(env) == 0) return 0;
+  if (s.hot_h == h) return h;
+  if (s.hot_h == 0) return 0;  // decided against for this operator
+  Ctx *c = op->ctx;
+  const int64_t n = op->ncols, nnz = op->nnz;
+  std::vector<int64_t> cnt(n, 0);
+  for (int32_t cidx : op->h_colidx) ++cnt[cidx];
+  std::vector<int32_t> order(n);
+  for (int64_t j = 0; j < n; ++j) order[j] = (int32_t)j;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cnt[x] > cnt[y]; });
+  int64_t cover = 0;
+  for (int i = 0; i < h; ++i) cover += cnt[order[i]];
+  if (5 * cover < nnz) {
+    s.hot_h = 0;
+    return 0;
+  }
+  std::vector<int32_t> slot(n, -1);
+  for (int i = 0; i < h; ++i) slot[order[i]] = i;
+  DevBuf dslot(n * sizeof(int32_t));
+  HIP_CHECK(hipMemcpyAsync(dslot.p, slot.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  s.hot_rows.alloc((size_t)h * sizeof(int32_t));
+  HIP_CHECK(hipMemcpyAsync(s.hot_rows.p, order.data(), (size_t)h * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  s.hot_colidx.alloc(std::max<size_t>(nnz * sizeof(int32_t), 16));
+  hipLaunchKernelGGL(k_hot_encode, grid_for(nnz), kBlock, 0, c->stream, nnz, op->colidx.as<int32_t>(),
+                     dslot.as<int32_t>(), s.hot_colidx.as<int32_t>());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  s.hot_h = h;
+  return h;
+}
+
+template <int LGR, typename V, typename X, typename Y, typename Acc>
+static void spmm_hot_launch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, const Schedule &s, int h) {
+  constexpr int RPB = kHotBlock >> LGR;
+  const int64_t need = ((int64_t)a.nshort + RPB - 1) / RPB;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)c->ncu, need));
+  const size_t shm = (size_t)h * a.ldx * sizeof(X);
+  hipLaunchKernelGGL((k_spmm_hot<LGR, V, X, Y, Acc>), blocks, kHotBlock, shm, c->stream, a,
+                     s.hot_colidx.as<int32_t>(), s.hot_rows.as<int32_t>(), h);
+}
+
 template <typename V, typename X, typename Y, typename Acc>
 static void launch_spmm(svdh_op *op, int direction, const void *vals, const void *X_, int64_t ldx,
                         void *Y_, int64_t ldy, int64_t l) {
   Ctx *c = op->ctx;
   SpmmPlan<V, X, Y, Acc> p;
   if (!plan_spmm(op, direction, vals, X_, ldx, Y_, ldy, l, p)) return;
+  if (direction == 0 && !p.mc) {
+    const int h = hot_columns(op, *p.s, std::min<int>(1024, (int)(65536 / (ldx * (int64_t)sizeof(X)))));
+    if (h > 0) {
+      KTimer t(c, KC_SPMM, p.bytes);
+      switch (p.lgr) {
+        case 1: spmm_hot_launch<1>(c, p.a, *p.s, h); break;
+        case 2: spmm_hot_launch<2>(c, p.a, *p.s, h); break;
+        case 3: spmm_hot_launch<3>(c, p.a, *p.s, h); break;
+        case 4: spmm_hot_launch<4>(c, p.a, *p.s, h); break;
+        case 5: spmm_hot_launch<5>(c, p.a, *p.s, h); break;
+        default: spmm_hot_launch<6>(c, p.a, *p.s, h); break;
+      }
+      HIP_CHECK(hipGetLastError());
+      return;
+    }
+  }
   KTimer t(c, direction == 0 ? KC_SPMM : KC_SPMM_T, p.bytes);
   if (p.mc)
     spmm_dispatch<true>(c, p.a, p.lgr, p.blocks);

@@ -197,3 +197,28 @@ def test_spmm_transposed_source_blocks(svdlib, dtype):
     z = np.empty(a.shape[1])
     op.svd_opa(y.astype(np.float64), z, True)
     check_close(z, a64.T @ y, bound(a64.T, y, EPS64 if dtype == np.float64 else EPS32))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmm_hot_columns_staged_in_lds(svdlib, dtype):
+    # Zipf columns: A X stages the most popular panel rows in LDS; the result
+    # equals the oracle and, bit for bit, the path without staging
+    import os
+    a = skewed_csr(8000, 3000, seed=12, mean_row=30, max_row=200, zipf=1.2).astype(dtype)
+    op = svdlib.CsrMatrix.from_scipy(a)
+    a64 = a.astype(np.float64)
+    rng = np.random.default_rng(2)
+    eps = EPS64 if dtype == np.float64 else EPS32
+    for l in (8, 30, 60):
+        X = rng.normal(size=(3000, l)).astype(dtype)
+        Y = np.zeros((8000, l), dtype=dtype)
+        op.multiply_with_dense(X, Y, False)
+        want = a64 @ X.astype(np.float64)
+        check_close(Y, want, bound(a64, X.astype(np.float64), eps) + eps * np.abs(want))
+        os.environ["SVDH_SPMM_HOT"] = "0"
+        try:
+            Y2 = np.zeros_like(Y)
+            op.multiply_with_dense(X, Y2, False)
+        finally:
+            del os.environ["SVDH_SPMM_HOT"]
+        np.testing.assert_array_equal(Y, Y2)
