@@ -31,6 +31,7 @@
 // keep the reference's explicit steps: column L2 normalisation or nothing
 // between the products, then an explicit CholeskyQR2 of Y before B.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -43,7 +44,8 @@ namespace svdh {
 namespace {
 
 // Workspace slots of the context used by the solver.
-enum { WS_OMEGA = 0, WS_Y = 1, WS_Z = 2, WS_YN = 3, WS_U = 4, WS_VT = 5, WS_BF16 = 6, WS_ONES = 7, WS_SUMS = 8 };
+enum { WS_OMEGA = 0, WS_Y = 1, WS_Z = 2, WS_YN = 3, WS_U = 4, WS_VT = 5, WS_BF16 = 6, WS_ONES = 7, WS_SUMS = 8,
+       WS_YP = 9 };
 
 struct Panel {
   void *ptr = nullptr;
@@ -244,6 +246,30 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
     int64_t next_check = n_power_iters;
     int64_t last_check_it = -1;
     double last_est = 0.0;
+    // Adaptive mode shifts each power step, Y <- (A A^T - c I) Q with c half
+    // the smallest Ritz value of the block (a lower bound of sigma_l^2): the
+    // unwanted spectrum [0, sigma_{l+1}^2] maps into [-c, c] and the top-k
+    // directions contract against it by (s_k^2 - c) / c instead of
+    // s_k^2 / s_{l+1}^2. Fixed-count mode keeps the plain power iteration.
+    //
+    // On top of the shift, a Chebyshev three-term recurrence (momentum):
+    //   X_{t+1} = (A A^T - c I) Q_t - (c^2 / 4) Q_{t-1} R_t^-1,  Q_{t+1} R_{t+1} = X_{t+1},
+    // which is the unnormalised W_{t+1} = (B - c) W_t - (c^2/4) W_{t-1} with
+    // every step's QR factor divided out. Each unwanted component (|lambda - c|
+    // <= c) then decays at c / 2 per step and the top-k directions contract by
+    // 1 / (t + sqrt(t^2 - 1)), t = (s_k^2 - c) / c: the Chebyshev rate. In the
+    // implicit form Q_{t-1} R_t^-1 = Y_{t-1} (M_{t-1} M_t). The recurrence
+    // restarts whenever c changes, and is dropped for good if a residual check
+    // ever goes up.
+    double shift = 0.0, shift_used = 0.0;
+    double shift_frac = 0.5;
+    if (const char *e = std::getenv("SVDH_RSVD_SHIFT")) shift_frac = std::atof(e);
+    bool momentum = true;
+    if (const char *e = std::getenv("SVDH_RSVD_MOMENTUM")) momentum = std::atoi(e) != 0;
+    Panel Yp;  // Y_{t-1}
+    std::vector<double> Mp;  // M_{t-1}
+    bool have_prev = false;
+    if (adaptive && momentum) Yp.init(ctx, WS_YP, m, l, pdt);
     for (;;) {
       const bool final_fixed = !adaptive && iters_run >= n_power_iters;
       // Z holds A^T Y; Z = A^T Q = (A^T Y) Mq
@@ -289,13 +315,45 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
             step = std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)std::floor(0.8 * need)));
           }
         }
+        if (last_check_it >= 0 && est > last_est) momentum = false;
         last_check_it = iters_run;
         last_est = est;
         next_check = iters_run + step;
+        // a new shift restarts the recurrence, so small moves of the Ritz
+        // estimate keep the current one
+        const double cand = shift_frac * std::max(lam[wq - 1], 0.0);
+        if (!(std::fabs(cand - shift) <= 0.03 * cand)) shift = cand;
+      }
+      if (adaptive && shift > 0.0) {  // Yn = A Z - c Q = A Z - c Y Mq
+        const double *Md = static_cast<const double *>(ctx->upload(Mq.data(), Mq.size() * sizeof(double)));
+        panel_gemm_dev(ctx, m, Y.w, wq, Y.p(), Y.ld, Md, -shift, 1.0, Yn.p(), Yn.ld, pdt);
+        if (momentum && have_prev && shift == shift_used) {  // Yn -= (c^2/4) Y_{t-1} (M_{t-1} M_t)
+          const int64_t pw = Yp.w, mid = Y.w;
+          std::vector<double> PM(pw * wq, 0.0);
+          for (int64_t i = 0; i < pw; ++i)
+            for (int64_t j = 0; j < mid; ++j) {
+              const double a = Mp[i * mid + j];
+              if (a != 0.0)
+                for (int64_t q = 0; q < wq; ++q) PM[i * wq + q] += a * Mq[j * wq + q];
+            }
+          const double *PMd = static_cast<const double *>(ctx->upload(PM.data(), PM.size() * sizeof(double)));
+          panel_gemm_dev(ctx, m, pw, wq, Yp.p(), Yp.ld, PMd, -0.25 * shift * shift, 1.0, Yn.p(), Yn.ld, pdt);
+        }
+        shift_used = shift;
       }
       // Next subspace: Y <- A Z, its Gram, A^T Y, then its normaliser.
-      std::swap(Y.ptr, Yn.ptr);
-      std::swap(Y.ld, Yn.ld);
+      if (adaptive && momentum && shift > 0.0) {  // keep Y_t and M_t as the previous iterate
+        std::swap(Yp.ptr, Y.ptr);
+        std::swap(Yp.ld, Y.ld);
+        Yp.w = Y.w;
+        Mp = Mq;
+        have_prev = true;
+        std::swap(Y.ptr, Yn.ptr);
+        std::swap(Y.ld, Yn.ld);
+      } else {
+        std::swap(Y.ptr, Yn.ptr);
+        std::swap(Y.ld, Yn.ld);
+      }
       Y.w = Z.w;
       ++iters_run;
       if (!adaptive && R.bf16) low = iters_run < (n_power_iters + 1) / 2;
