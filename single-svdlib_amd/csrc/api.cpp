@@ -51,7 +51,7 @@ void *Ctx::upload(const void *host, size_t n) {
   char *h = h_ring + (size_t)i * kRingSlot;
   std::memcpy(h, host, n);
   char *d = d_ring.as<char>() + (size_t)i * kRingSlot;
-  HIP_CHECK(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, stream));
+  fetch_from_host(this, dh_ring + (size_t)i * kRingSlot, d, n);
   HIP_CHECK(hipEventRecord(ring_ev[i], stream));
   return d;
 }
@@ -176,15 +176,19 @@ int32_t svdh_ctx_create(int32_t device, svdh_ctx **out) {
       c->red_count.alloc(256);
       HIP_CHECK(hipMemset(c->red_count.p, 0, 256));
       c->stage.alloc(1 << 16);
+      // fine-grained (coherent) pinned memory: kernels read uploads from it
+      // and write small results into it, with no copy engine round trip
       HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_ring), Ctx::kRing * Ctx::kRingSlot,
-                              hipHostMallocDefault));
+                              hipHostMallocCoherent | hipHostMallocMapped));
+      HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->dh_ring), c->h_ring, 0));
       c->d_ring.alloc(Ctx::kRing * Ctx::kRingSlot);
       for (int i = 0; i < Ctx::kRing; ++i) {
         HIP_CHECK(hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming));
         HIP_CHECK(hipEventRecord(c->ring_ev[i], c->stream));
       }
       HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_small), Ctx::kSmall * sizeof(double),
-                              hipHostMallocDefault));
+                              hipHostMallocCoherent | hipHostMallocMapped));
+      HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_small), c->h_small, 0));
       HIP_CHECK(hipEventCreateWithFlags(&c->small_ev, hipEventDisableTiming));
     } catch (...) {
       if (c->h_ring) (void)hipHostFree(c->h_ring);

@@ -90,6 +90,8 @@ struct Ctx {
   int ring_next = 0;
   // Pinned landing area for small device-to-host results (Gram, norms).
   double *h_small = nullptr;
+  double *d_small = nullptr;  // device address of h_small (kernels write results there)
+  char *dh_ring = nullptr;    // device address of h_ring (a kernel fetches uploads from it)
   static constexpr size_t kSmall = 4 * 128 * 128;  // doubles
   hipEvent_t small_ev = nullptr;
   // Copy n bytes (<= kRingSlot) to the device, ordered on the stream; returns
@@ -269,6 +271,9 @@ void basis_combine(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, c
                    int64_t d, double *C, int64_t ldc);
 // Gram G (l x l, FP64, row-major) = Y^T Y for row-major Y (m x l, ldy) of dtype ydt.
 void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, double *G_host);
+// Copies n bytes from pinned host memory (device address src) to device
+// memory with a kernel: a small copy costs a dispatch, not a DMA round trip.
+void fetch_from_host(Ctx *c, const void *src, void *dst, size_t n);
 // Asynchronous Gram: enqueues the kernels and the copy of G into the pinned
 // area c->h_small + off (doubles) and records c->small_ev; the caller waits
 // on the event before reading.

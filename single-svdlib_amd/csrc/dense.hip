@@ -636,6 +636,20 @@ static void launch_gram(Ctx *c, bool same, int nb, int64_t m, int l, const T *Y,
   HIP_CHECK(hipGetLastError());
 }
 
+__global__ void k_fetch(const uint4 *src, uint4 *dst, int64_t n16) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void fetch_from_host(Ctx *c, const void *src, void *dst, size_t n) {
+  if (n == 0) return;
+  const int64_t n16 = (int64_t)((n + 15) / 16);  // whole 16-byte vectors (slots are 16-byte multiples)
+  const int g = (int)std::min<int64_t>(64, (n16 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_fetch, g, kBlock, 0, c->stream, static_cast<const uint4 *>(src), static_cast<uint4 *>(dst),
+                     n16);
+  HIP_CHECK(hipGetLastError());
+}
+
 void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, size_t off) {
   if (l > 128) fail(SVDH_ERR_ARG, "panel width %lld exceeds the Gram kernel limit 128", (long long)l);
   if (off + (size_t)l * l > Ctx::kSmall) fail(SVDH_ERR_ARG, "Gram landing area too small");
@@ -658,10 +672,12 @@ void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int yd
         else
           launch_gram<bf16_t>(c, c0 == d0, nb, m, (int)l, (const bf16_t *)Y, ldy, c0, d0, part);
       }
-    hipLaunchKernelGGL(k_gram_reduce, reduce_grid(ll), kBlock, 0, c->stream, nb, ll, (const double *)part, G);
+    // the sums go straight to the pinned landing area
+    hipLaunchKernelGGL(k_gram_reduce, reduce_grid(ll), kBlock, 0, c->stream, nb, ll, (const double *)part,
+                       c->d_small + off);
     HIP_CHECK(hipGetLastError());
   }
-  HIP_CHECK(hipMemcpyAsync(c->h_small + off, G, ll * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  (void)G;
   HIP_CHECK(hipEventRecord(c->small_ev, c->stream));
 }
 
@@ -778,9 +794,9 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
   }
   // column sums over the workgroup partials, on the device (same kernel as the Gram reduction)
   double *sums = part + (size_t)nb * k;
-  hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, sums);
+  (void)sums;
+  hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, c->d_small);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(c->h_small, sums, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   std::memcpy(nr2_host, c->h_small, k * sizeof(double));
 }
