@@ -301,6 +301,12 @@ void panel_fill(Ctx *c, int64_t m, int64_t l, double v, void *Y, int64_t ldy, in
 // already in device memory; Y and Z share the dtype dt (F64 or F32).
 void panel_gemm_dev(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, const double *M_dev,
                     double alpha, double beta, void *Z, int64_t ldz, int dt);
+// FP32 panels on the matrix cores: Z = beta Z + sum_p alpha_p Y_p M_p for one
+// or two terms (M_p li_p x lo FP64 row-major in device memory, lo, li_p <= 64).
+// Z may be Y[0] when beta = 0.
+void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
+                const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
+                double beta);
 // C (k1 x k2, FP64 row-major, device) = X1^T X2 for row-major panels (k1, k2 <= 64).
 void gram_cross(Ctx *c, int64_t m, int64_t k1, const void *X1, int64_t ld1, int64_t k2, const void *X2, int64_t ld2,
                 int dt, double *C_dev);

@@ -636,6 +636,148 @@ static void launch_gram(Ctx *c, bool same, int nb, int64_t m, int l, const T *Y,
   HIP_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------------------------------------
+// FP32 panel products on the matrix cores:
+//   Z[r, 0:lo] = beta Z[r, 0:lo] + sum_p Y_p[r, 0:li_p] (alpha_p M_p)
+// for up to two terms, lo, li_p <= 64. v_mfma_f32_16x16x4 takes A[l % 16][l / 16]
+// and B[l / 16][l % 16] per lane and returns D[4 (l / 16) + v][l % 16] (lane
+// layout checked on gfx950). A workgroup stages alpha_p M_p (FP32, zero
+// padded to 64 x 64) in LDS once, then strides over 32-row tiles: the tile of
+// every Y_p is read into LDS before any row of Z is written, so Z may be Y_0
+// when beta = 0. Wave w computes rows 16 (w % 2) .. + 16, columns
+// 32 (w / 2) .. + 32 of the tile.
+// ------------------------------------------------------------------------
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+struct PanelTerm {
+  const float *y;
+  int64_t ld;
+  int li;
+  const double *m;  // li x lo, row-major, FP64 device
+  double alpha;
+};
+
+// One wave per 16-row block and all 64 output columns. The contraction index
+// k is taken in groups of 16: in step s of group g, lane l supplies
+// Y[row l % 16][16 g + 4 (l / 16) + s] (component s of one 16-byte load) and
+// alpha M[16 g + 4 (l / 16) + s][16 q + l % 16] (kept in registers for the
+// whole kernel), so a wave reads whole panel rows and each lane's A values
+// come from one vector load. A wave reads all of its rows before writing any,
+// so Z may be Y_0 when beta = 0.
+template <int NP>
+__device__ __forceinline__ void panel_rows_load(const PanelTerm &t0, const PanelTerm &t1, int64_t rows, int64_t row,
+                                                int kq, f4_t (&ya)[NP][4]) {
+  // whole 16-byte loads at clamped addresses; out-of-range parts are zeroed
+  // by selects afterwards, so no load sits behind a branch
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const PanelTerm &t = p == 0 ? t0 : t1;
+    const int64_t rr = row < rows ? row : rows - 1;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int kc = 16 * g + 4 * kq;
+      const int kk = kc < t.li ? kc : 0;
+      const float4 w = *reinterpret_cast<const float4 *>(t.y + rr * t.ld + kk);
+      const bool ok = row < rows && kc < t.li;
+      ya[p][g] = f4_t{ok ? w.x : 0.f, ok && kc + 1 < t.li ? w.y : 0.f, ok && kc + 2 < t.li ? w.z : 0.f,
+                      ok && kc + 3 < t.li ? w.w : 0.f};
+    }
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
+                                                      int64_t ldz, float beta) {
+  __shared__ float ms[NP][64 * 64];
+  for (int p = 0; p < NP; ++p) {
+    const PanelTerm &t = p == 0 ? t0 : t1;
+    for (int e = threadIdx.x; e < 64 * 64; e += kBlock) {
+      const int k = e >> 6, col = e & 63;
+      ms[p][e] = (k < t.li && col < lo) ? (float)(t.alpha * t.m[(int64_t)k * lo + col]) : 0.f;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int kq = lane >> 4, rl = lane & 15;
+  float bm[NP][4][4][4];  // [term][group][step][column block]
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bm[p][g][st][q] = ms[p][(16 * g + 4 * kq + st) * 64 + 16 * q + rl];
+  const int64_t nblk = (rows + 15) / 16;
+  const int64_t wave = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  f4_t ya[NP][4], yn[NP][4];
+  if (wave < nblk) panel_rows_load<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
+  for (int64_t b = wave; b < nblk; b += nwaves) {
+    const int64_t r0 = b * 16;
+    const bool more = b + nwaves < nblk;
+    if (more) panel_rows_load<NP>(t0, t1, rows, (b + nwaves) * 16 + rl, kq, yn);  // next block in flight
+    float zold[4][4];
+    if (beta != 0.f)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t gr = r0 + 4 * kq + v;
+          const int c = 16 * q + rl;
+          zold[q][v] = z[(gr < rows ? gr : rows - 1) * ldz + (c < lo ? c : 0)];
+        }
+    f4_t acc[4] = {f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f},
+                   f4_t{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], bm[p][g][st][q], acc[q], 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gr = r0 + 4 * kq + v;
+        const int c = 16 * q + rl;
+        if (gr < rows && c < lo) z[gr * ldz + c] = (beta != 0.f ? beta * zold[q][v] : 0.f) + acc[q][v];
+      }
+    if (more)
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) ya[p][g] = yn[p][g];
+  }
+}
+
+void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
+                const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
+                double beta) {
+  if (rows == 0 || lo == 0) return;
+  if (lo > 64 || nterms < 1 || nterms > 2) fail(SVDH_ERR_ARG, "panel_mfma: lo <= 64 and one or two terms");
+  PanelTerm t[2] = {};
+  double bytes = (double)rows * lo * 4.0 * (beta != 0.0 ? 2.0 : 1.0);
+  for (int p = 0; p < nterms; ++p) {
+    if (li[p] > 64) fail(SVDH_ERR_ARG, "panel_mfma: li <= 64");
+    t[p] = PanelTerm{Y[p], ldy[p], (int)li[p], M_dev[p], alpha[p]};
+    bytes += (double)rows * li[p] * 4.0;
+  }
+  for (int p = 0; p < nterms; ++p)
+    if (ldy[p] % 4) fail(SVDH_ERR_ARG, "panel_mfma: leading dimensions must be multiples of 4");
+  KTimer tm(c, KC_DENSE, bytes);
+  const int64_t nblk = (rows + 15) / 16;
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + 3) / 4, 4 * (int64_t)c->ncu));
+  if (nterms == 1)
+    hipLaunchKernelGGL(k_panel_mfma<1>, g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz, (float)beta);
+  else
+    hipLaunchKernelGGL(k_panel_mfma<2>, g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz, (float)beta);
+  HIP_CHECK(hipGetLastError());
+}
+
 __global__ void k_fetch(const uint4 *src, uint4 *dst, int64_t n16) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];

@@ -272,8 +272,36 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
     if (adaptive && momentum) Yp.init(ctx, WS_YP, m, l, pdt);
     for (;;) {
       const bool final_fixed = !adaptive && iters_run >= n_power_iters;
-      // Z holds A^T Y; Z = A^T Q = (A^T Y) Mq
-      panel_apply(ctx, n, Y.w, wq, Z.p(), Z.ld, Z.dt, Mq.data(), Z.p(), Z.ld, Z.dt);
+      // FP32 panels of width <= 64 use the matrix-core panel products; the
+      // step's small matrices (Mq, and M_{t-1} M_t for the momentum term)
+      // travel to the device in one upload.
+      const bool mfma = pdt == SVDH_F32 && Y.w <= 64 && wq <= 64 && (!have_prev || Yp.w <= 64);
+      std::vector<double> PM;
+      if (adaptive && momentum && have_prev) {  // M_{t-1} M_t
+        const int64_t pw = Yp.w, mid = Y.w;
+        PM.assign(pw * wq, 0.0);
+        for (int64_t i = 0; i < pw; ++i)
+          for (int64_t j = 0; j < mid; ++j) {
+            const double a = Mp[i * mid + j];
+            if (a != 0.0)
+              for (int64_t q = 0; q < wq; ++q) PM[i * wq + q] += a * Mq[j * wq + q];
+          }
+      }
+      const double *dMq = nullptr, *dPM = nullptr;
+      if (mfma) {
+        std::vector<double> both(Mq);
+        both.insert(both.end(), PM.begin(), PM.end());
+        dMq = static_cast<const double *>(ctx->upload(both.data(), both.size() * sizeof(double)));
+        dPM = dMq + Mq.size();
+        const float *yin[1] = {static_cast<const float *>(Z.p())};
+        const int64_t ldin[1] = {Z.ld}, liin[1] = {Y.w};
+        const double *min_[1] = {dMq};
+        const double al[1] = {1.0};
+        panel_mfma(ctx, n, wq, 1, yin, ldin, liin, min_, al, static_cast<float *>(Z.p()), Z.ld, 0.0);
+      } else {
+        // Z holds A^T Y; Z = A^T Q = (A^T Y) Mq
+        panel_apply(ctx, n, Y.w, wq, Z.p(), Z.ld, Z.dt, Mq.data(), Z.p(), Z.ld, Z.dt);
+      }
       Z.w = wq;
       const bool check = adaptive && (iters_run >= next_check || iters_run >= max_power_iters);
       if (final_fixed || check) gram_async(ctx, n, wq, Z.p(), Z.ld, Z.dt, 0);
@@ -324,20 +352,22 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         const double cand = shift_frac * std::max(lam[wq - 1], 0.0);
         if (!(std::fabs(cand - shift) <= 0.03 * cand)) shift = cand;
       }
-      if (adaptive && shift > 0.0) {  // Yn = A Z - c Q = A Z - c Y Mq
-        const double *Md = static_cast<const double *>(ctx->upload(Mq.data(), Mq.size() * sizeof(double)));
-        panel_gemm_dev(ctx, m, Y.w, wq, Y.p(), Y.ld, Md, -shift, 1.0, Yn.p(), Yn.ld, pdt);
-        if (momentum && have_prev && shift == shift_used) {  // Yn -= (c^2/4) Y_{t-1} (M_{t-1} M_t)
-          const int64_t pw = Yp.w, mid = Y.w;
-          std::vector<double> PM(pw * wq, 0.0);
-          for (int64_t i = 0; i < pw; ++i)
-            for (int64_t j = 0; j < mid; ++j) {
-              const double a = Mp[i * mid + j];
-              if (a != 0.0)
-                for (int64_t q = 0; q < wq; ++q) PM[i * wq + q] += a * Mq[j * wq + q];
-            }
-          const double *PMd = static_cast<const double *>(ctx->upload(PM.data(), PM.size() * sizeof(double)));
-          panel_gemm_dev(ctx, m, pw, wq, Yp.p(), Yp.ld, PMd, -0.25 * shift * shift, 1.0, Yn.p(), Yn.ld, pdt);
+      if (adaptive && shift > 0.0) {
+        // Yn = A Z - c Q - (c^2/4) Q_{t-1} R_t^-1 = A Z - c Y Mq - (c^2/4) Y_{t-1} (M_{t-1} M_t)
+        const bool mom = momentum && have_prev && shift == shift_used && !PM.empty();
+        if (mfma) {
+          const float *yin[2] = {static_cast<const float *>(Y.p()), static_cast<const float *>(Yp.p())};
+          const int64_t ldin[2] = {Y.ld, Yp.ld}, liin[2] = {Y.w, Yp.w};
+          const double *min_[2] = {dMq, dPM};
+          const double al[2] = {-shift, -0.25 * shift * shift};
+          panel_mfma(ctx, m, wq, mom ? 2 : 1, yin, ldin, liin, min_, al, static_cast<float *>(Yn.p()), Yn.ld, 1.0);
+        } else {
+          const double *Md = static_cast<const double *>(ctx->upload(Mq.data(), Mq.size() * sizeof(double)));
+          panel_gemm_dev(ctx, m, Y.w, wq, Y.p(), Y.ld, Md, -shift, 1.0, Yn.p(), Yn.ld, pdt);
+          if (mom) {
+            const double *PMd = static_cast<const double *>(ctx->upload(PM.data(), PM.size() * sizeof(double)));
+            panel_gemm_dev(ctx, m, Yp.w, wq, Yp.p(), Yp.ld, PMd, -0.25 * shift * shift, 1.0, Yn.p(), Yn.ld, pdt);
+          }
         }
         shift_used = shift;
       }
@@ -530,8 +560,18 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
   int64_t iters_run = 0;
   for (;;) {
     const bool final_fixed = iters_run >= n_power_iters;
-    for (int64_t b = 0; b < nops; ++b)
-      panel_apply(c, nrows[b], l, wq[b], Z[b], ld, pdt, Mq[b].data(), Z[b], ld, pdt);  // Z = (A^T Y) Mq
+    for (int64_t b = 0; b < nops; ++b) {  // Z = (A^T Y) Mq, the same kernels as rsvd_solve
+      if (pdt == SVDH_F32 && l <= 64 && wq[b] <= 64) {
+        const double *dM = static_cast<const double *>(c->upload(Mq[b].data(), Mq[b].size() * sizeof(double)));
+        const float *yin[1] = {static_cast<const float *>(Z[b])};
+        const int64_t ldin[1] = {ld}, liin[1] = {l};
+        const double *min_[1] = {dM};
+        const double al[1] = {1.0};
+        panel_mfma(c, nrows[b], wq[b], 1, yin, ldin, liin, min_, al, static_cast<float *>(Z[b]), ld, 0.0);
+      } else {
+        panel_apply(c, nrows[b], l, wq[b], Z[b], ld, pdt, Mq[b].data(), Z[b], ld, pdt);
+      }
+    }
     if (final_fixed) break;
     for (int64_t b = 0; b < nops; ++b)
       if (wq[b] != l) fail(SVDH_ERR_ARG, "rank-deficient panel in the batched solver (operator %lld)", (long long)b);
