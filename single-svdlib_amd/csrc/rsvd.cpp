@@ -266,6 +266,8 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
     if (const char *e = std::getenv("SVDH_RSVD_SHIFT")) shift_frac = std::atof(e);
     bool momentum = true;
     if (const char *e = std::getenv("SVDH_RSVD_MOMENTUM")) momentum = std::atoi(e) != 0;
+    double restart_tol = 0.03;
+    if (const char *e = std::getenv("SVDH_RSVD_RESTART")) restart_tol = std::atof(e);
     Panel Yp;  // Y_{t-1}
     std::vector<double> Mp;  // M_{t-1}
     bool have_prev = false;
@@ -350,7 +352,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         // a new shift restarts the recurrence, so small moves of the Ritz
         // estimate keep the current one
         const double cand = shift_frac * std::max(lam[wq - 1], 0.0);
-        if (!(std::fabs(cand - shift) <= 0.03 * cand)) shift = cand;
+        if (!(std::fabs(cand - shift) <= restart_tol * cand)) shift = cand;
       }
       if (adaptive && shift > 0.0) {
         // Yn = A Z - c Q - (c^2/4) Q_{t-1} R_t^-1 = A Z - c Y Mq - (c^2/4) Y_{t-1} (M_{t-1} M_t)
