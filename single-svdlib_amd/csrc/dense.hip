@@ -685,9 +685,12 @@ __device__ __forceinline__ void panel_rows_load(const PanelTerm &t0, const Panel
   }
 }
 
-template <int NP>
+// RESID: the product is not stored; its column sums of squares (FP64) are
+// accumulated instead and each workgroup writes its lo partials to
+// part[blockIdx.x * lo ...] (the adaptive residual check).
+template <int NP, bool RESID>
 __global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
-                                                      int64_t ldz, float beta) {
+                                                      int64_t ldz, float beta, double *part) {
   __shared__ float ms[NP][64 * 64];
   for (int p = 0; p < NP; ++p) {
     const PanelTerm &t = p == 0 ? t0 : t1;
@@ -712,13 +715,14 @@ __global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, Pan
   const int64_t wave = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
   f4_t ya[NP][4], yn[NP][4];
+  double csq[4] = {0.0, 0.0, 0.0, 0.0};  // RESID: columns 16 q + rl
   if (wave < nblk) panel_rows_load<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
   for (int64_t b = wave; b < nblk; b += nwaves) {
     const int64_t r0 = b * 16;
     const bool more = b + nwaves < nblk;
     if (more) panel_rows_load<NP>(t0, t1, rows, (b + nwaves) * 16 + rl, kq, yn);  // next block in flight
     float zold[4][4];
-    if (beta != 0.f)
+    if (!RESID && beta != 0.f)
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -738,19 +742,40 @@ __global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, Pan
 #pragma unroll
           for (int q = 0; q < 4; ++q)
             acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], bm[p][g][st][q], acc[q], 0, 0, 0);
+    if (RESID) {
+      // rows past the end were loaded as zeros, so they add nothing
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t gr = r0 + 4 * kq + v;
-        const int c = 16 * q + rl;
-        if (gr < rows && c < lo) z[gr * ldz + c] = (beta != 0.f ? beta * zold[q][v] : 0.f) + acc[q][v];
-      }
+        for (int v = 0; v < 4; ++v) csq[q] += (double)acc[q][v] * (double)acc[q][v];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t gr = r0 + 4 * kq + v;
+          const int c = 16 * q + rl;
+          if (gr < rows && c < lo) z[gr * ldz + c] = (beta != 0.f ? beta * zold[q][v] : 0.f) + acc[q][v];
+        }
+    }
     if (more)
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int g = 0; g < 4; ++g) ya[p][g] = yn[p][g];
+  }
+  if (RESID) {
+    // fixed-order sum over the four lane groups of each wave, then the waves
+    __shared__ double red[kBlock / 64][4][64];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[threadIdx.x >> 6][kq][16 * q + rl] = csq[q];
+    __syncthreads();
+    if (threadIdx.x < 64 && (int)threadIdx.x < lo) {
+      double t = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w)
+        for (int g = 0; g < 4; ++g) t += red[w][g][threadIdx.x];
+      part[(int64_t)blockIdx.x * lo + threadIdx.x] = t;
+    }
   }
 }
 
@@ -772,9 +797,11 @@ void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const
   const int64_t nblk = (rows + 15) / 16;
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + 3) / 4, 4 * (int64_t)c->ncu));
   if (nterms == 1)
-    hipLaunchKernelGGL(k_panel_mfma<1>, g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz, (float)beta);
+    hipLaunchKernelGGL((k_panel_mfma<1, false>), g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz, (float)beta,
+                       nullptr);
   else
-    hipLaunchKernelGGL(k_panel_mfma<2>, g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz, (float)beta);
+    hipLaunchKernelGGL((k_panel_mfma<2, false>), g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz, (float)beta,
+                       nullptr);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -904,6 +931,34 @@ void gram_cross(Ctx *c, int64_t m, int64_t k1, const void *X1, int64_t ld1, int6
 void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const void *Y1, const void *Y2, int64_t ld,
                     int dt, const double *M1, const double *M2, double *nr2_host) {
   if (w > 128 || k > 128 || w2 > w) fail(SVDH_ERR_ARG, "residual panel wider than 128");
+  if (dt == SVDH_F32 && w <= 64 && k <= 64 && ld % 4 == 0 && m > 0) {
+    // FP32 panels on the matrix cores: column norms of Y1 M1 + Y2 M2 (FP32
+    // products, as the vector kernel below computes them), squares summed in FP64
+    std::vector<double> both(M1, M1 + w * k);
+    both.insert(both.end(), M2, M2 + w2 * k);
+    const double *Md = static_cast<const double *>(c->upload(both.data(), both.size() * sizeof(double)));
+    const int64_t nblk = (m + 15) / 16;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + 3) / 4, 4 * (int64_t)c->ncu));
+    c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * k * sizeof(double)));
+    double *part = c->red_part.as<double>();
+    const PanelTerm t0{static_cast<const float *>(Y1), ld, (int)w, Md, 1.0};
+    const PanelTerm t1{static_cast<const float *>(Y2), ld, (int)w2, Md + w * k, 1.0};
+    {
+      KTimer t(c, KC_DENSE, (double)m * (w + w2) * 4.0);
+      if (w2 > 0)
+        hipLaunchKernelGGL((k_panel_mfma<2, true>), nb, kBlock, 0, c->stream, m, (int)k, t0, t1, nullptr, (int64_t)0,
+                           0.f, part);
+      else
+        hipLaunchKernelGGL((k_panel_mfma<1, true>), nb, kBlock, 0, c->stream, m, (int)k, t0, t0, nullptr, (int64_t)0,
+                           0.f, part);
+      HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, c->d_small);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    std::memcpy(nr2_host, c->h_small, k * sizeof(double));
+    return;
+  }
   const int nb = grid_of(m, kTile, 1024);
   const size_t mb = (size_t)w * k * sizeof(double);
   const double *M1d = static_cast<const double *>(c->upload(M1, mb));

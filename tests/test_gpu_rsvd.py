@@ -91,6 +91,22 @@ def test_adaptive_reaches_tolerance(lib):
     assert triplet_residual(a, rec.u, rec.s, rec.vt) < 3e-5
 
 
+@pytest.mark.parametrize("rank,over,dtype", [(20, 10, np.float32), (50, 14, np.float32), (60, 20, np.float32),
+                                             (20, 10, np.float64)])
+def test_adaptive_residual_estimate_is_the_fp64_residual(lib, rank, over, dtype):
+    # the on-device residual check (matrix-core path for FP32 panels up to 64
+    # wide, vector path otherwise) must report the residual of the triplets it
+    # returns, as recomputed from them in FP64 on the CPU
+    a = skewed_csr(20000, 3000, seed=23, mean_row=30, max_row=2000).astype(dtype)
+    tol = 1e-5 if dtype == np.float32 else 1e-8
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a), rank, over, 0, "QR", False, 3, False, tol=tol,
+                                        max_power_iters=400)
+    est = rec.extra["residual_estimate"]
+    true = triplet_residual(a, rec.u, rec.s, rec.vt)
+    assert est <= tol
+    assert abs(est - true) <= 0.05 * true, (est, true)
+
+
 def test_bf16_ladder_matches_f32(lib):
     a = skewed_csr(20000, 3000, seed=22, mean_row=30, max_row=2000).astype(np.float32)
     op = lib.CsrMatrix.from_scipy(a, keep_bf16=True)
