@@ -190,6 +190,7 @@ int32_t svdh_ctx_create(int32_t device, svdh_ctx **out) {
                               hipHostMallocCoherent | hipHostMallocMapped));
       HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_small), c->h_small, 0));
       HIP_CHECK(hipEventCreateWithFlags(&c->small_ev, hipEventDisableTiming));
+      HIP_CHECK(hipEventCreateWithFlags(&c->small_ev2, hipEventDisableTiming));
     } catch (...) {
       if (c->h_ring) (void)hipHostFree(c->h_ring);
       if (c->h_small) (void)hipHostFree(c->h_small);
@@ -215,6 +216,7 @@ int32_t svdh_ctx_destroy(svdh_ctx *ctx) {
     for (int i = 0; i < Ctx::kRing; ++i)
       if (ctx->ring_ev[i]) (void)hipEventDestroy(ctx->ring_ev[i]);
     if (ctx->small_ev) (void)hipEventDestroy(ctx->small_ev);
+    if (ctx->small_ev2) (void)hipEventDestroy(ctx->small_ev2);
     if (ctx->h_ring) (void)hipHostFree(ctx->h_ring);
     if (ctx->h_small) (void)hipHostFree(ctx->h_small);
     if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);

@@ -69,7 +69,7 @@ struct Ctx {
   DevBuf stage;
   // Device workspaces kept for the context's lifetime (panels of the
   // randomized solver), grown on demand; slot meanings belong to the caller.
-  static constexpr int kWorkspaces = 10;
+  static constexpr int kWorkspaces = 12;
   DevBuf ws[kWorkspaces];
   void *workspace(int slot, size_t bytes) {
     if (bytes > ws[slot].bytes) {
@@ -94,6 +94,7 @@ struct Ctx {
   char *dh_ring = nullptr;    // device address of h_ring (a kernel fetches uploads from it)
   static constexpr size_t kSmall = 4 * 128 * 128;  // doubles
   hipEvent_t small_ev = nullptr;
+  hipEvent_t small_ev2 = nullptr;  // a second landing event (two Grams in flight)
   // Copy n bytes (<= kRingSlot) to the device, ordered on the stream; returns
   // the device address, valid for kernels launched after this call.
   void *upload(const void *host, size_t n);
@@ -275,9 +276,10 @@ void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, dou
 // memory with a kernel: a small copy costs a dispatch, not a DMA round trip.
 void fetch_from_host(Ctx *c, const void *src, void *dst, size_t n);
 // Asynchronous Gram: enqueues the kernels and the copy of G into the pinned
-// area c->h_small + off (doubles) and records c->small_ev; the caller waits
-// on the event before reading.
-void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, size_t off);
+// area c->h_small + off (doubles) and records ev (c->small_ev when null); the
+// caller waits on the event before reading.
+void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, size_t off,
+                hipEvent_t ev = nullptr);
 // Z (m x lo) = Y (m x li) * M (li x lo, FP64 row-major, host), in place or to
 // another panel. Element (i, c) of Z is stored at Z[i * ldz + c * zcs]
 // (zcs = 1: row-major; ldz = 1, zcs = ld: the transposed, column-major store).
@@ -301,12 +303,13 @@ void panel_fill(Ctx *c, int64_t m, int64_t l, double v, void *Y, int64_t ldy, in
 // already in device memory; Y and Z share the dtype dt (F64 or F32).
 void panel_gemm_dev(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, const double *M_dev,
                     double alpha, double beta, void *Z, int64_t ldz, int dt);
-// FP32 panels on the matrix cores: Z = beta Z + sum_p alpha_p Y_p M_p for one
-// or two terms (M_p li_p x lo FP64 row-major in device memory, lo, li_p <= 64).
-// Z may be Y[0] when beta = 0.
+// FP32 panels on the matrix cores: Z = beta Zin + sum_p alpha_p Y_p M_p for
+// one or two terms (M_p li_p x lo FP64 row-major in device memory, lo, li_p
+// <= 64); Zin is Z unless given (leading dimension ldzin). Z may be Y[0] when
+// beta = 0.
 void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
                 const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
-                double beta);
+                double beta, const float *Zin = nullptr, int64_t ldzin = 0);
 // C (k1 x k2, FP64 row-major, device) = X1^T X2 for row-major panels (k1, k2 <= 64).
 void gram_cross(Ctx *c, int64_t m, int64_t k1, const void *X1, int64_t ld1, int64_t k2, const void *X2, int64_t ld2,
                 int dt, double *C_dev);
@@ -332,7 +335,8 @@ double pythag(double a, double b, double eps);
 int imtqlb(int64_t n, double *d, double *e, double *bnd, int64_t max_iter, double eps);
 int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_iter, double eps);
 // Symmetric eigen-decomposition of an l x l row-major matrix: eigenvalues
-// ascending in w, eigenvectors as columns of V (row-major l x l).
+// ascending in w, eigenvectors as columns of V (row-major l x l); V null:
+// eigenvalues only (the same values).
 void sym_eig(int64_t l, const double *A, double *w, double *V);
 // Cholesky A = R^T R (upper R, row-major). Returns false on a non-positive pivot.
 bool cholesky_upper(int64_t l, const double *A, double *R);

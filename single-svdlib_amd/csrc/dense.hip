@@ -690,7 +690,8 @@ __device__ __forceinline__ void panel_rows_load(const PanelTerm &t0, const Panel
 // part[blockIdx.x * lo ...] (the adaptive residual check).
 template <int NP, bool RESID>
 __global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
-                                                      int64_t ldz, float beta, double *part) {
+                                                      int64_t ldz, const float *zin, int64_t ldzin, float beta,
+                                                      double *part) {
   __shared__ float ms[NP][64 * 64];
   for (int p = 0; p < NP; ++p) {
     const PanelTerm &t = p == 0 ? t0 : t1;
@@ -729,7 +730,7 @@ __global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, Pan
         for (int v = 0; v < 4; ++v) {
           const int64_t gr = r0 + 4 * kq + v;
           const int c = 16 * q + rl;
-          zold[q][v] = z[(gr < rows ? gr : rows - 1) * ldz + (c < lo ? c : 0)];
+          zold[q][v] = zin[(gr < rows ? gr : rows - 1) * ldzin + (c < lo ? c : 0)];
         }
     f4_t acc[4] = {f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f},
                    f4_t{0.f, 0.f, 0.f, 0.f}};
@@ -781,8 +782,12 @@ __global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, Pan
 
 void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
                 const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
-                double beta) {
+                double beta, const float *Zin, int64_t ldzin) {
   if (rows == 0 || lo == 0) return;
+  if (!Zin) {
+    Zin = Z;
+    ldzin = ldz;
+  }
   if (lo > 64 || nterms < 1 || nterms > 2) fail(SVDH_ERR_ARG, "panel_mfma: lo <= 64 and one or two terms");
   PanelTerm t[2] = {};
   double bytes = (double)rows * lo * 4.0 * (beta != 0.0 ? 2.0 : 1.0);
@@ -797,11 +802,11 @@ void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const
   const int64_t nblk = (rows + 15) / 16;
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + 3) / 4, 4 * (int64_t)c->ncu));
   if (nterms == 1)
-    hipLaunchKernelGGL((k_panel_mfma<1, false>), g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz, (float)beta,
-                       nullptr);
+    hipLaunchKernelGGL((k_panel_mfma<1, false>), g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz, Zin,
+                       ldzin, (float)beta, nullptr);
   else
-    hipLaunchKernelGGL((k_panel_mfma<2, false>), g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz, (float)beta,
-                       nullptr);
+    hipLaunchKernelGGL((k_panel_mfma<2, false>), g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz, Zin,
+                       ldzin, (float)beta, nullptr);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -819,7 +824,7 @@ void fetch_from_host(Ctx *c, const void *src, void *dst, size_t n) {
   HIP_CHECK(hipGetLastError());
 }
 
-void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, size_t off) {
+void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, size_t off, hipEvent_t ev) {
   if (l > 128) fail(SVDH_ERR_ARG, "panel width %lld exceeds the Gram kernel limit 128", (long long)l);
   if (off + (size_t)l * l > Ctx::kSmall) fail(SVDH_ERR_ARG, "Gram landing area too small");
   if (ldy % 4 != 0) fail(SVDH_ERR_ARG, "Gram panel leading dimension %lld is not a multiple of 4", (long long)ldy);
@@ -847,7 +852,7 @@ void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int yd
     HIP_CHECK(hipGetLastError());
   }
   (void)G;
-  HIP_CHECK(hipEventRecord(c->small_ev, c->stream));
+  HIP_CHECK(hipEventRecord(ev ? ev : c->small_ev, c->stream));
 }
 
 void gram(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, double *G_host) {
@@ -947,10 +952,10 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
       KTimer t(c, KC_DENSE, (double)m * (w + w2) * 4.0);
       if (w2 > 0)
         hipLaunchKernelGGL((k_panel_mfma<2, true>), nb, kBlock, 0, c->stream, m, (int)k, t0, t1, nullptr, (int64_t)0,
-                           0.f, part);
+                           nullptr, (int64_t)0, 0.f, part);
       else
         hipLaunchKernelGGL((k_panel_mfma<1, true>), nb, kBlock, 0, c->stream, m, (int)k, t0, t0, nullptr, (int64_t)0,
-                           0.f, part);
+                           nullptr, (int64_t)0, 0.f, part);
       HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, c->d_small);

@@ -219,67 +219,159 @@ int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_i
   return SVDH_OK;
 }
 
-// Symmetric eigen-decomposition of a small dense matrix: Householder
-// reduction to tridiagonal form with the reflections accumulated in V, then
-// implicit QL (imtql2 with f64 epsilon) applied to V's columns. The matrix
-// is scaled by its largest entry first so the absolute-epsilon deflation test
-// of imtql2 behaves the same for every scale.
-void sym_eig(int64_t l, const double *A, double *w, double *V) {
-  if (l == 0) return;
+// Symmetric eigen-decomposition of a small dense matrix (the Rayleigh-Ritz
+// and normaliser solves of the randomized SVD; LAS2 keeps imtql2 above).
+// Householder reduction to tridiagonal form on the trailing blocks only, the
+// reflections accumulated backwards (each step touches the block it changes),
+// then implicit QL with a relative deflation test. The eigenvectors are kept
+// transposed, so every QL rotation combines two contiguous rows. The matrix
+// is scaled by its largest entry first. V[i * n + r] is component i of the
+// eigenvector of w[r]; w ascending. With V null only the eigenvalues are
+// computed (the same values: the vectors do not feed back into the QL
+// recurrence).
+void sym_eig(int64_t n, const double *A, double *w, double *V) {
+  if (n == 0) return;
+  const bool vectors = V != nullptr;
   double scale = 0.0;
-  for (int64_t i = 0; i < l * l; ++i) scale = std::max(scale, std::fabs(A[i]));
+  for (int64_t i = 0; i < n * n; ++i) scale = std::max(scale, std::fabs(A[i]));
   if (!(scale > 0.0)) {
-    for (int64_t i = 0; i < l; ++i) w[i] = 0.0;
-    for (int64_t i = 0; i < l * l; ++i) V[i] = 0.0;
-    for (int64_t i = 0; i < l; ++i) V[i * l + i] = 1.0;
+    for (int64_t i = 0; i < n; ++i) w[i] = 0.0;
+    if (vectors) {
+      for (int64_t i = 0; i < n * n; ++i) V[i] = 0.0;
+      for (int64_t i = 0; i < n; ++i) V[i * n + i] = 1.0;
+    }
     return;
   }
-  std::vector<double> a(l * l);
-  for (int64_t i = 0; i < l * l; ++i) a[i] = A[i] / scale;
-  // V accumulates Q with A = Q T Q^T.
-  std::vector<double> q(l * l, 0.0);
-  for (int64_t i = 0; i < l; ++i) q[i * l + i] = 1.0;
-  std::vector<double> diag(l), off(l, 0.0), v(l), p(l), kv(l);
-  for (int64_t k = 0; k + 2 < l; ++k) {
-    // Householder vector for column k below the sub-diagonal.
-    double alpha = 0.0;
-    for (int64_t i = k + 1; i < l; ++i) alpha += a[i * l + k] * a[i * l + k];
-    alpha = std::sqrt(alpha);
-    if (alpha == 0.0) continue;
-    const double x0 = a[(k + 1) * l + k];
-    if (x0 > 0) alpha = -alpha;
-    for (int64_t i = 0; i < l; ++i) v[i] = 0.0;
+  std::vector<double> a(n * n);
+  const double inv = 1.0 / scale;
+  for (int64_t i = 0; i < n * n; ++i) a[i] = A[i] * inv;
+  std::vector<double> d(n), e(n, 0.0), vs((size_t)n * n, 0.0), betas(n, 0.0), p(n), kv(n);
+  // Reduction: step k zeroes column k below the sub-diagonal with
+  // H_k = I - beta v v^T, v supported on k+1..n-1 (stored in row k of vs).
+  for (int64_t k = 0; k + 2 < n; ++k) {
+    double *v = vs.data() + (size_t)k * n;
+    double xn = 0.0;
+    for (int64_t i = k + 1; i < n; ++i) xn += a[i * n + k] * a[i * n + k];
+    xn = std::sqrt(xn);
+    const double x0 = a[(k + 1) * n + k];
+    if (xn == 0.0) continue;
+    const double alpha = x0 > 0.0 ? -xn : xn;
     v[k + 1] = x0 - alpha;
-    for (int64_t i = k + 2; i < l; ++i) v[i] = a[i * l + k];
+    for (int64_t i = k + 2; i < n; ++i) v[i] = a[i * n + k];
     double vn = 0.0;
-    for (int64_t i = k + 1; i < l; ++i) vn += v[i] * v[i];
+    for (int64_t i = k + 1; i < n; ++i) vn += v[i] * v[i];
     if (vn == 0.0) continue;
     const double beta = 2.0 / vn;
-    // A <- H A H with H = I - beta v v^T: p = beta A v, K = p - (beta/2)(v^T p) v,
-    // A <- A - v K^T - K v^T.
-    for (int64_t i = 0; i < l; ++i) {
-      double s = 0.0;
-      for (int64_t j = k + 1; j < l; ++j) s += a[i * l + j] * v[j];
-      p[i] = beta * s;
-    }
+    betas[k] = beta;
+    // column/row k become (.., alpha, 0, ..)
+    a[(k + 1) * n + k] = alpha;
+    a[k * n + k + 1] = alpha;
+    for (int64_t i = k + 2; i < n; ++i) a[i * n + k] = a[k * n + i] = 0.0;
+    // trailing block T = a[k+1.., k+1..]: p = beta T v, K = p - (beta/2)(v.p) v,
+    // T <- T - v K^T - K v^T
     double vp = 0.0;
-    for (int64_t i = k + 1; i < l; ++i) vp += v[i] * p[i];
-    for (int64_t i = 0; i < l; ++i) kv[i] = p[i] - 0.5 * beta * vp * v[i];
-    for (int64_t i = 0; i < l; ++i)
-      for (int64_t j = 0; j < l; ++j) a[i * l + j] -= v[i] * kv[j] + kv[i] * v[j];
-    // Q <- Q H
-    for (int64_t i = 0; i < l; ++i) {
-      double s = 0.0;
-      for (int64_t j = k + 1; j < l; ++j) s += q[i * l + j] * v[j];
-      s *= beta;
-      for (int64_t j = k + 1; j < l; ++j) q[i * l + j] -= s * v[j];
+    for (int64_t i = k + 1; i < n; ++i) {
+      const double *ai = a.data() + i * n;
+      double t = 0.0;
+      for (int64_t j = k + 1; j < n; ++j) t += ai[j] * v[j];
+      p[i] = beta * t;
+      vp += v[i] * p[i];
+    }
+    const double h = 0.5 * beta * vp;
+    for (int64_t i = k + 1; i < n; ++i) kv[i] = p[i] - h * v[i];
+    for (int64_t i = k + 1; i < n; ++i) {
+      double *ai = a.data() + i * n;
+      const double vi = v[i], ki = kv[i];
+      for (int64_t j = k + 1; j < n; ++j) ai[j] -= vi * kv[j] + ki * v[j];
     }
   }
-  for (int64_t i = 0; i < l; ++i) diag[i] = a[i * l + i];
-  for (int64_t i = 1; i < l; ++i) off[i] = a[i * l + i - 1];
-  imtql2(l, l, diag.data(), off.data(), q.data(), 100, 2.220446049250313e-16);
-  for (int64_t i = 0; i < l; ++i) w[i] = diag[i] * scale;
-  for (int64_t i = 0; i < l * l; ++i) V[i] = q[i];
+  for (int64_t i = 0; i < n; ++i) d[i] = a[i * n + i];
+  for (int64_t i = 0; i + 1 < n; ++i) e[i] = a[(i + 1) * n + i];  // e[i] couples i and i+1
+  // Q = H_0 H_1 ... H_{n-3}, accumulated from the right end: before step k
+  // only the block k+1.. of Q differs from the identity.
+  std::vector<double> q(vectors ? (size_t)n * n : 0, 0.0), wv(n), zt(vectors ? (size_t)n * n : 0);
+  for (int64_t i = 0; vectors && i < n; ++i) q[i * n + i] = 1.0;
+  for (int64_t k = n - 3; vectors && k >= 0; --k) {
+    const double beta = betas[k];
+    if (beta == 0.0) continue;
+    const double *v = vs.data() + (size_t)k * n;
+    for (int64_t j = k + 1; j < n; ++j) wv[j] = 0.0;
+    for (int64_t i = k + 1; i < n; ++i) {
+      const double vi = v[i];
+      const double *qi = q.data() + i * n;
+      for (int64_t j = k + 1; j < n; ++j) wv[j] += vi * qi[j];
+    }
+    for (int64_t i = k + 1; i < n; ++i) {
+      const double f = beta * v[i];
+      double *qi = q.data() + i * n;
+      for (int64_t j = k + 1; j < n; ++j) qi[j] -= f * wv[j];
+    }
+  }
+  // zt row r = column r of Q
+  for (int64_t i = 0; vectors && i < n; ++i)
+    for (int64_t r = 0; r < n; ++r) zt[(size_t)r * n + i] = q[i * n + r];
+  // Implicit QL on (d, e).
+  const double eps = 2.220446049250313e-16;
+  auto hyp = [](double x, double y) {
+    const double r = std::sqrt(x * x + y * y);
+    return (r > 1e-150 && r < 1e150) ? r : std::hypot(x, y);
+  };
+  for (int64_t l = 0; l < n; ++l) {
+    for (int64_t iter = 0;; ++iter) {
+      int64_t m = l;
+      for (; m + 1 < n; ++m)
+        if (std::fabs(e[m]) <= eps * (std::fabs(d[m]) + std::fabs(d[m + 1]))) break;
+      if (m == l) break;
+      if (iter == 60) fail(SVDH_ERR_IMTQL2, "symmetric eigensolver: no convergence after %d iterations", 60);
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hyp(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+      double s = 1.0, c = 1.0, pp = 0.0;
+      int64_t i = m - 1;
+      bool early = false;
+      for (; i >= l; --i) {
+        const double f = s * e[i], b = c * e[i];
+        r = hyp(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= pp;
+          e[m] = 0.0;
+          early = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - pp;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        pp = s * r;
+        d[i + 1] = g + pp;
+        g = c * r - b;
+        if (vectors) {
+          double *__restrict zi = zt.data() + (size_t)i * n;
+          double *__restrict zj = zi + n;
+          for (int64_t t = 0; t < n; ++t) {
+            const double x = zi[t], y = zj[t];
+            zj[t] = s * x + c * y;
+            zi[t] = c * x - s * y;
+          }
+        }
+      }
+      if (early) continue;
+      d[l] -= pp;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  // ascending order
+  std::vector<int64_t> ord(n);
+  for (int64_t i = 0; i < n; ++i) ord[i] = i;
+  std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return d[x] < d[y]; });
+  for (int64_t r = 0; r < n; ++r) {
+    w[r] = d[ord[r]] * scale;
+    if (!vectors) continue;
+    const double *zr = zt.data() + (size_t)ord[r] * n;
+    for (int64_t i = 0; i < n; ++i) V[i * n + r] = zr[i];
+  }
 }
 
 bool cholesky_upper(int64_t l, const double *A, double *R) {

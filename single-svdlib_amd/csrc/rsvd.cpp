@@ -45,7 +45,11 @@ namespace {
 
 // Workspace slots of the context used by the solver.
 enum { WS_OMEGA = 0, WS_Y = 1, WS_Z = 2, WS_YN = 3, WS_U = 4, WS_VT = 5, WS_BF16 = 6, WS_ONES = 7, WS_SUMS = 8,
-       WS_YP = 9 };
+       WS_YP = 9, WS_YS = 10, WS_Z2 = 11 };
+
+// Landing offset (doubles) of the Gram of the speculative next panel, clear
+// of a Gram of width <= 128 at offset 0.
+constexpr size_t kGramOff2 = 128 * 128;
 
 struct Panel {
   void *ptr = nullptr;
@@ -64,6 +68,14 @@ struct Panel {
   }
   void *p() const { return ptr; }
 };
+
+// Eigenvalues of a symmetric matrix, descending.
+void desc_eigvals(int64_t w, const std::vector<double> &G, std::vector<double> &lam) {
+  std::vector<double> la(w);
+  sym_eig(w, G.data(), la.data(), nullptr);
+  lam.assign(w, 0.0);
+  for (int64_t r = 0; r < w; ++r) lam[r] = la[w - 1 - r];
+}
 
 // Eigen-decomposition of a symmetric matrix sorted by descending eigenvalue.
 void desc_eig(int64_t w, const std::vector<double> &G, std::vector<double> &lam, std::vector<double> &W) {
@@ -133,10 +145,11 @@ struct Rsvd {
     }
   }
 
-  // Waits for the Gram enqueued by gram_async and copies it out.
-  void gram_wait(int64_t w, std::vector<double> &G) {
-    HIP_CHECK(hipEventSynchronize(c->small_ev));
-    G.assign(c->h_small, c->h_small + w * w);
+  // Waits for the Gram enqueued by gram_async (landing offset off, event ev)
+  // and copies it out.
+  void gram_wait(int64_t w, std::vector<double> &G, size_t off = 0, hipEvent_t ev = nullptr) {
+    HIP_CHECK(hipEventSynchronize(ev ? ev : c->small_ev));
+    G.assign(c->h_small + off, c->h_small + off + w * w);
   }
 
   // M (w x wo, row-major) with P M orthonormal, from the FP64 Gram of P.
@@ -147,9 +160,10 @@ struct Rsvd {
 
   // Second half of normaliser: the Gram of P was enqueued by gram_async, and
   // the stream may have more work queued behind it.
-  void finish_normaliser(const Panel &P, std::vector<double> &M, int64_t &wo) {
+  void finish_normaliser(const Panel &P, std::vector<double> &M, int64_t &wo, size_t off = 0,
+                         hipEvent_t ev = nullptr) {
     std::vector<double> G;
-    gram_wait(P.w, G);
+    gram_wait(P.w, G, off, ev);
     if (!normaliser_from_gram(P.w, P.rows, P.dt, G, M, wo)) ++fallbacks;
   }
 
@@ -272,6 +286,19 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
     std::vector<double> Mp;  // M_{t-1}
     bool have_prev = false;
     if (adaptive && momentum) Yp.init(ctx, WS_YP, m, l, pdt);
+    // At a residual check the next iterate is built before the check is
+    // decided: the shift needs only the eigenvalues of the check's Gram, so
+    // Y_{t+1} goes to Ys (A Z in Yn stays for the residual) and its Gram and
+    // A^T Ys (into Z2) are enqueued; the device works through them while the
+    // host finds the eigenvectors and the residual. The steps are the ones of
+    // the unpipelined order: a converged check drops the speculative step,
+    // and if the check turns the momentum or the BF16 ladder off, the step is
+    // rebuilt from A Z without them.
+    Panel Ys, Z2;
+    if (adaptive) {
+      Ys.init(ctx, WS_YS, m, l, pdt);
+      Z2.init(ctx, WS_Z2, n, l, pdt);
+    }
     for (;;) {
       const bool final_fixed = !adaptive && iters_run >= n_power_iters;
       // FP32 panels of width <= 64 use the matrix-core panel products; the
@@ -314,7 +341,52 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       }
       R.mult(0, Z, Yn, low);  // A Z
       if (check) {
+        // The shift for the next iterate, from the Ritz values: a new shift
+        // restarts the recurrence, so small moves of the Ritz estimate keep
+        // the current one. (If this check converges, the shift is unused.)
         R.gram_wait(wq, GZ);
+        desc_eigvals(wq, GZ, lam);
+        const double cand = shift_frac * std::max(lam[wq - 1], 0.0);
+        if (!(std::fabs(cand - shift) <= restart_tol * cand)) shift = cand;
+      }
+      // Next iterate: Y_{t+1} = A Z - c Q - (c^2/4) Q_{t-1} R_t^-1
+      //                      = A Z - c Y Mq - (c^2/4) Y_{t-1} (M_{t-1} M_t),
+      // in place in Yn, or into Ys at a check.
+      Panel *next = check ? &Ys : &Yn;
+      auto build_next = [&](bool mom) {
+        next->w = Yn.w;
+        if (mfma) {
+          const float *yin[2] = {static_cast<const float *>(Y.p()), static_cast<const float *>(Yp.p())};
+          const int64_t ldin[2] = {Y.ld, Yp.ld}, liin[2] = {Y.w, Yp.w};
+          const double *min_[2] = {dMq, dPM};
+          const double al[2] = {-shift, -0.25 * shift * shift};
+          panel_mfma(ctx, m, wq, mom ? 2 : 1, yin, ldin, liin, min_, al, static_cast<float *>(next->p()), next->ld,
+                     1.0, static_cast<const float *>(Yn.p()), Yn.ld);
+        } else {
+          if (next != &Yn)
+            HIP_CHECK(hipMemcpyAsync(next->p(), Yn.p(), (size_t)m * Yn.ld * dtype_size(pdt), hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+          const double *Md = static_cast<const double *>(ctx->upload(Mq.data(), Mq.size() * sizeof(double)));
+          panel_gemm_dev(ctx, m, Y.w, wq, Y.p(), Y.ld, Md, -shift, 1.0, next->p(), next->ld, pdt);
+          if (mom) {
+            const double *PMd = static_cast<const double *>(ctx->upload(PM.data(), PM.size() * sizeof(double)));
+            panel_gemm_dev(ctx, m, Yp.w, wq, Yp.p(), Yp.ld, PMd, -0.25 * shift * shift, 1.0, next->p(), next->ld,
+                           pdt);
+          }
+        }
+      };
+      bool mom_used = false;
+      const bool shifted = adaptive && shift > 0.0;
+      if (shifted) {
+        mom_used = momentum && have_prev && shift == shift_used && !PM.empty();
+        build_next(mom_used);
+      } else if (check) {
+        next = &Yn;  // unshifted: A Z itself, read-only from here on
+      }
+      if (check) {
+        // the speculative next step: Gram of Y_{t+1} and A^T Y_{t+1}
+        gram_async(ctx, m, next->w, next->p(), next->ld, next->dt, kGramOff2, ctx->small_ev2);
+        R.mult(1, *next, Z2, low);
         desc_eig(wq, GZ, lam, W);
         const int64_t k = std::min(target_rank, std::min(wq, n));
         // r_i = (A Z) w_i / s_i - Y (Mq w_i s_i)
@@ -335,7 +407,11 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         for (int64_t r = 0; r < k; ++r) est = std::max(est, std::sqrt(nr2[r]) / std::sqrt(std::max(lam[r], 0.0)));
         if (!ok || !std::isfinite(est)) est = INFINITY;
         if ((est <= tol && !low) || iters_run >= max_power_iters) break;
-        if (R.bf16 && low && est <= bf16_switch) low = false;
+        bool redo = false;
+        if (R.bf16 && low && est <= bf16_switch) {
+          low = false;
+          redo = true;
+        }
         // Next check: predicted from the contraction rate since the last one.
         int64_t step = 1;
         if (last_check_it >= 0 && est < last_est && est > tol && !low) {
@@ -345,49 +421,39 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
             step = std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)std::floor(0.8 * need)));
           }
         }
-        if (last_check_it >= 0 && est > last_est) momentum = false;
+        if (last_check_it >= 0 && est > last_est) {
+          momentum = false;
+          if (mom_used) redo = true;
+        }
         last_check_it = iters_run;
         last_est = est;
         next_check = iters_run + step;
-        // a new shift restarts the recurrence, so small moves of the Ritz
-        // estimate keep the current one
-        const double cand = shift_frac * std::max(lam[wq - 1], 0.0);
-        if (!(std::fabs(cand - shift) <= restart_tol * cand)) shift = cand;
-      }
-      if (adaptive && shift > 0.0) {
-        // Yn = A Z - c Q - (c^2/4) Q_{t-1} R_t^-1 = A Z - c Y Mq - (c^2/4) Y_{t-1} (M_{t-1} M_t)
-        const bool mom = momentum && have_prev && shift == shift_used && !PM.empty();
-        if (mfma) {
-          const float *yin[2] = {static_cast<const float *>(Y.p()), static_cast<const float *>(Yp.p())};
-          const int64_t ldin[2] = {Y.ld, Yp.ld}, liin[2] = {Y.w, Yp.w};
-          const double *min_[2] = {dMq, dPM};
-          const double al[2] = {-shift, -0.25 * shift * shift};
-          panel_mfma(ctx, m, wq, mom ? 2 : 1, yin, ldin, liin, min_, al, static_cast<float *>(Yn.p()), Yn.ld, 1.0);
-        } else {
-          const double *Md = static_cast<const double *>(ctx->upload(Mq.data(), Mq.size() * sizeof(double)));
-          panel_gemm_dev(ctx, m, Y.w, wq, Y.p(), Y.ld, Md, -shift, 1.0, Yn.p(), Yn.ld, pdt);
-          if (mom) {
-            const double *PMd = static_cast<const double *>(ctx->upload(PM.data(), PM.size() * sizeof(double)));
-            panel_gemm_dev(ctx, m, Yp.w, wq, Yp.p(), Yp.ld, PMd, -0.25 * shift * shift, 1.0, Yn.p(), Yn.ld, pdt);
-          }
+        if (redo) {
+          if (shifted) build_next(false);
+          gram_async(ctx, m, next->w, next->p(), next->ld, next->dt, kGramOff2, ctx->small_ev2);
+          R.mult(1, *next, Z2, low);
         }
-        shift_used = shift;
       }
-      // Next subspace: Y <- A Z, its Gram, A^T Y, then its normaliser.
-      if (adaptive && momentum && shift > 0.0) {  // keep Y_t and M_t as the previous iterate
-        std::swap(Yp.ptr, Y.ptr);
-        std::swap(Yp.ld, Y.ld);
-        Yp.w = Y.w;
+      if (shifted) shift_used = shift;
+      // Adopt the next iterate: Y <- Y_{t+1}, keeping Y_t and M_t as the
+      // previous iterate for the momentum term.
+      if (adaptive && momentum && shift > 0.0) {
+        Panel old_prev = Yp;
+        Yp = Y;
         Mp = Mq;
         have_prev = true;
-        std::swap(Y.ptr, Yn.ptr);
-        std::swap(Y.ld, Yn.ld);
+        Y = *next;
+        *next = old_prev;
       } else {
-        std::swap(Y.ptr, Yn.ptr);
-        std::swap(Y.ld, Yn.ld);
+        std::swap(Y, *next);
       }
       Y.w = Z.w;
       ++iters_run;
+      if (check) {  // the speculative step's Gram and A^T Y are already queued
+        std::swap(Z, Z2);
+        R.finish_normaliser(Y, Mq, wq, kGramOff2, ctx->small_ev2);
+        continue;
+      }
       if (!adaptive && R.bf16) low = iters_run < (n_power_iters + 1) / 2;
       const bool next_final = !adaptive && iters_run >= n_power_iters;
       gram_async(ctx, Y.rows, Y.w, Y.p(), Y.ld, Y.dt, 0);
