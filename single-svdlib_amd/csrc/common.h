@@ -288,8 +288,11 @@ void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64
 // nr2[c] = |Y1 M1[:, c] + Y2 M2[:, c]|^2 for c < k (FP64); Y1 (m x w) and
 // Y2 (m x w2, w2 <= w) are row-major panels with leading dimension ld,
 // M1 (w x k) and M2 (w2 x k) are host matrices.
+// With nr2_host null the kernels are only enqueued; panel_residual_wait then
+// waits for them and copies the k norms out.
 void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const void *Y1, const void *Y2, int64_t ld,
                     int dt, const double *M1, const double *M2, double *nr2_host);
+void panel_residual_wait(Ctx *c, int64_t k, double *nr2_host);
 // Column L2 normalisation, columns with norm <= 1e-10 left as they are.
 void panel_normalize_columns(Ctx *c, int64_t m, int64_t l, void *Y, int64_t ldy, int ydt);
 // Standard normal panel (m x l row-major) from a counter-based generator.

@@ -960,8 +960,8 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
     }
     hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, c->d_small);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(c->stream));
-    std::memcpy(nr2_host, c->h_small, k * sizeof(double));
+    HIP_CHECK(hipEventRecord(c->small_ev, c->stream));
+    if (nr2_host) panel_residual_wait(c, k, nr2_host);
     return;
   }
   const int nb = grid_of(m, kTile, 1024);
@@ -999,7 +999,12 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
   (void)sums;
   hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, c->d_small);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipEventRecord(c->small_ev, c->stream));
+  if (nr2_host) panel_residual_wait(c, k, nr2_host);
+}
+
+void panel_residual_wait(Ctx *c, int64_t k, double *nr2_host) {
+  HIP_CHECK(hipEventSynchronize(c->small_ev));
   std::memcpy(nr2_host, c->h_small, k * sizeof(double));
 }
 

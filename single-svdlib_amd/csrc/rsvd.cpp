@@ -299,23 +299,33 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       Ys.init(ctx, WS_YS, m, l, pdt);
       Z2.init(ctx, WS_Z2, n, l, pdt);
     }
+    // A B for row-major A (rows x mid) and B (mid x cols)
+    auto small_mm = [](const std::vector<double> &A, int64_t rows, int64_t mid, const std::vector<double> &B,
+                       int64_t cols) {
+      std::vector<double> C(rows * cols, 0.0);
+      for (int64_t i = 0; i < rows; ++i)
+        for (int64_t j = 0; j < mid; ++j) {
+          const double a = A[i * mid + j];
+          if (a != 0.0)
+            for (int64_t q = 0; q < cols; ++q) C[i * cols + q] += a * B[j * cols + q];
+        }
+      return C;
+    };
+    std::vector<double> PM;  // M_{t-1} M_t when prepared during a check
+    bool pm_ready = false;
+    std::vector<double> Mq_spec;  // normaliser of the iterate built at a check
+    int64_t wq_spec = 0;
     for (;;) {
       const bool final_fixed = !adaptive && iters_run >= n_power_iters;
       // FP32 panels of width <= 64 use the matrix-core panel products; the
       // step's small matrices (Mq, and M_{t-1} M_t for the momentum term)
       // travel to the device in one upload.
       const bool mfma = pdt == SVDH_F32 && Y.w <= 64 && wq <= 64 && (!have_prev || Yp.w <= 64);
-      std::vector<double> PM;
-      if (adaptive && momentum && have_prev) {  // M_{t-1} M_t
-        const int64_t pw = Yp.w, mid = Y.w;
-        PM.assign(pw * wq, 0.0);
-        for (int64_t i = 0; i < pw; ++i)
-          for (int64_t j = 0; j < mid; ++j) {
-            const double a = Mp[i * mid + j];
-            if (a != 0.0)
-              for (int64_t q = 0; q < wq; ++q) PM[i * wq + q] += a * Mq[j * wq + q];
-          }
-      }
+      if (!(adaptive && momentum && have_prev))
+        PM.clear();
+      else if (!pm_ready)
+        PM = small_mm(Mp, Yp.w, Y.w, Mq, wq);  // M_{t-1} M_t
+      pm_ready = false;
       const double *dMq = nullptr, *dPM = nullptr;
       if (mfma) {
         std::vector<double> both(Mq);
@@ -402,11 +412,23 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
             M2[i * k + r] = -t * sr;
           }
         }
-        panel_residual(ctx, m, Y.w, wq, k, Y.p(), Yn.p(), Y.ld, pdt, M2.data(), M1.data(), nr2.data());
+        panel_residual(ctx, m, Y.w, wq, k, Y.p(), Yn.p(), Y.ld, pdt, M2.data(), M1.data(), nullptr);
+        // While the residual runs: the normaliser of the next iterate and its
+        // momentum product (dropped if the check converges).
+        std::vector<double> Mq_next;
+        int64_t wq_next = 0;
+        const int64_t fb = R.fallbacks;
+        R.finish_normaliser(*next, Mq_next, wq_next, kGramOff2, ctx->small_ev2);
+        std::vector<double> PM_next;
+        if (momentum && shift > 0.0) PM_next = small_mm(Mq, Y.w, wq, Mq_next, wq_next);
+        panel_residual_wait(ctx, k, nr2.data());
         est = 0.0;
         for (int64_t r = 0; r < k; ++r) est = std::max(est, std::sqrt(nr2[r]) / std::sqrt(std::max(lam[r], 0.0)));
         if (!ok || !std::isfinite(est)) est = INFINITY;
-        if ((est <= tol && !low) || iters_run >= max_power_iters) break;
+        if ((est <= tol && !low) || iters_run >= max_power_iters) {
+          R.fallbacks = fb;
+          break;
+        }
         bool redo = false;
         if (R.bf16 && low && est <= bf16_switch) {
           low = false;
@@ -432,7 +454,18 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
           if (shifted) build_next(false);
           gram_async(ctx, m, next->w, next->p(), next->ld, next->dt, kGramOff2, ctx->small_ev2);
           R.mult(1, *next, Z2, low);
+          R.fallbacks = fb;
+          R.finish_normaliser(*next, Mq_next, wq_next, kGramOff2, ctx->small_ev2);
+          PM_next.clear();
         }
+        // the prepared products become the next step's (PM only if the
+        // rotation below keeps Y_t as the previous iterate)
+        if (momentum && shift > 0.0 && !PM_next.empty()) {
+          PM = std::move(PM_next);
+          pm_ready = true;
+        }
+        Mq_spec = std::move(Mq_next);
+        wq_spec = wq_next;
       }
       if (shifted) shift_used = shift;
       // Adopt the next iterate: Y <- Y_{t+1}, keeping Y_t and M_t as the
@@ -449,9 +482,10 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       }
       Y.w = Z.w;
       ++iters_run;
-      if (check) {  // the speculative step's Gram and A^T Y are already queued
+      if (check) {  // the speculative step's A^T Y is queued and its normaliser known
         std::swap(Z, Z2);
-        R.finish_normaliser(Y, Mq, wq, kGramOff2, ctx->small_ev2);
+        Mq = std::move(Mq_spec);
+        wq = wq_spec;
         continue;
       }
       if (!adaptive && R.bf16) low = iters_run < (n_power_iters + 1) / 2;
