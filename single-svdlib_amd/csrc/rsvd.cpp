@@ -264,24 +264,31 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
     // the smallest Ritz value of the block (a lower bound of sigma_l^2): the
     // unwanted spectrum [0, sigma_{l+1}^2] maps into [-c, c] and the top-k
     // directions contract against it by (s_k^2 - c) / c instead of
-    // s_k^2 / s_{l+1}^2. Fixed-count mode keeps the plain power iteration.
+    // s_k^2 / s_{l+1}^2. c is refreshed every step from the eigenvalues of
+    // the Gram of Z = A^T Q (Q^T A A^T Q, whose eigenvalues are the Ritz
+    // values), enqueued before A Z and solved on the host while A Z runs.
+    // Fixed-count mode keeps the plain power iteration.
     //
     // On top of the shift, a Chebyshev three-term recurrence (momentum):
     //   X_{t+1} = (A A^T - c I) Q_t - (c^2 / 4) Q_{t-1} R_t^-1,  Q_{t+1} R_{t+1} = X_{t+1},
     // which is the unnormalised W_{t+1} = (B - c) W_t - (c^2/4) W_{t-1} with
-    // every step's QR factor divided out. Each unwanted component (|lambda - c|
-    // <= c) then decays at c / 2 per step and the top-k directions contract by
+    // every step's QR factor divided out (c^2/2 on the first momentum step:
+    // 2 (c/2)^2 T_2). Each unwanted component (|lambda - c| <= c) then decays
+    // at c / 2 per step and the top-k directions contract by
     // 1 / (t + sqrt(t^2 - 1)), t = (s_k^2 - c) / c: the Chebyshev rate. In the
     // implicit form Q_{t-1} R_t^-1 = Y_{t-1} (M_{t-1} M_t). The recurrence
-    // restarts whenever c changes, and is dropped for good if a residual check
-    // ever goes up.
+    // carries on through the small step-to-step changes of c (restarting
+    // only when c moves by more than restart_tol relative, by default never)
+    // and is dropped for good if a residual check ever goes up.
     double shift = 0.0, shift_used = 0.0;
     double shift_frac = 0.5;
     if (const char *e = std::getenv("SVDH_RSVD_SHIFT")) shift_frac = std::atof(e);
     bool momentum = true;
     if (const char *e = std::getenv("SVDH_RSVD_MOMENTUM")) momentum = std::atoi(e) != 0;
-    double restart_tol = 0.03;
+    double restart_tol = INFINITY;
     if (const char *e = std::getenv("SVDH_RSVD_RESTART")) restart_tol = std::atof(e);
+    const bool eig_step = adaptive && shift_frac > 0.0;
+    int64_t mom_run = 0;  // consecutive momentum steps
     Panel Yp;  // Y_{t-1}
     std::vector<double> Mp;  // M_{t-1}
     bool have_prev = false;
@@ -343,33 +350,31 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       }
       Z.w = wq;
       const bool check = adaptive && (iters_run >= next_check || iters_run >= max_power_iters);
-      if (final_fixed || check) gram_async(ctx, n, wq, Z.p(), Z.ld, Z.dt, 0);
+      if (final_fixed || check || eig_step) gram_async(ctx, n, wq, Z.p(), Z.ld, Z.dt, 0);
       if (final_fixed) {
         R.gram_wait(wq, GZ);
         desc_eig(wq, GZ, lam, W);
         break;
       }
       R.mult(0, Z, Yn, low);  // A Z
-      if (check) {
-        // The shift for the next iterate, from the Ritz values: a new shift
-        // restarts the recurrence, so small moves of the Ritz estimate keep
-        // the current one. (If this check converges, the shift is unused.)
+      if (check || eig_step) {
+        // The shift for the next iterate, from the Ritz values (if this is a
+        // check that converges, the shift is unused).
         R.gram_wait(wq, GZ);
         desc_eigvals(wq, GZ, lam);
-        const double cand = shift_frac * std::max(lam[wq - 1], 0.0);
-        if (!(std::fabs(cand - shift) <= restart_tol * cand)) shift = cand;
+        shift = shift_frac * std::max(lam[wq - 1], 0.0);
       }
       // Next iterate: Y_{t+1} = A Z - c Q - (c^2/4) Q_{t-1} R_t^-1
       //                      = A Z - c Y Mq - (c^2/4) Y_{t-1} (M_{t-1} M_t),
       // in place in Yn, or into Ys at a check.
       Panel *next = check ? &Ys : &Yn;
-      auto build_next = [&](bool mom) {
+      auto build_next = [&](bool mom, double coef) {
         next->w = Yn.w;
         if (mfma) {
           const float *yin[2] = {static_cast<const float *>(Y.p()), static_cast<const float *>(Yp.p())};
           const int64_t ldin[2] = {Y.ld, Yp.ld}, liin[2] = {Y.w, Yp.w};
           const double *min_[2] = {dMq, dPM};
-          const double al[2] = {-shift, -0.25 * shift * shift};
+          const double al[2] = {-shift, -coef * shift * shift};
           panel_mfma(ctx, m, wq, mom ? 2 : 1, yin, ldin, liin, min_, al, static_cast<float *>(next->p()), next->ld,
                      1.0, static_cast<const float *>(Yn.p()), Yn.ld);
         } else {
@@ -380,7 +385,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
           panel_gemm_dev(ctx, m, Y.w, wq, Y.p(), Y.ld, Md, -shift, 1.0, next->p(), next->ld, pdt);
           if (mom) {
             const double *PMd = static_cast<const double *>(ctx->upload(PM.data(), PM.size() * sizeof(double)));
-            panel_gemm_dev(ctx, m, Yp.w, wq, Yp.p(), Yp.ld, PMd, -0.25 * shift * shift, 1.0, next->p(), next->ld,
+            panel_gemm_dev(ctx, m, Yp.w, wq, Yp.p(), Yp.ld, PMd, -coef * shift * shift, 1.0, next->p(), next->ld,
                            pdt);
           }
         }
@@ -388,8 +393,9 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       bool mom_used = false;
       const bool shifted = adaptive && shift > 0.0;
       if (shifted) {
-        mom_used = momentum && have_prev && shift == shift_used && !PM.empty();
-        build_next(mom_used);
+        const bool restart = std::fabs(shift - shift_used) > restart_tol * shift;
+        mom_used = momentum && have_prev && !restart && !PM.empty();
+        build_next(mom_used, mom_run == 0 ? 0.5 : 0.25);
       } else if (check) {
         next = &Yn;  // unshifted: A Z itself, read-only from here on
       }
@@ -434,13 +440,14 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
           low = false;
           redo = true;
         }
-        // Next check: predicted from the contraction rate since the last one.
+        // Next check: half the step count predicted from the contraction rate
+        // since the last check (the rate improves as the shift settles).
         int64_t step = 1;
         if (last_check_it >= 0 && est < last_est && est > tol && !low) {
           const double rate = std::pow(est / last_est, 1.0 / (double)(iters_run - last_check_it));
           if (rate > 0.0 && rate < 1.0) {
             const double need = std::log(tol / est) / std::log(rate);
-            step = std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)std::floor(0.8 * need)));
+            step = std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)std::floor(0.5 * need)));
           }
         }
         if (last_check_it >= 0 && est > last_est) {
@@ -451,7 +458,12 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         last_est = est;
         next_check = iters_run + step;
         if (redo) {
-          if (shifted) build_next(false);
+          // the iterate itself depends on the momentum flag only; its A^T
+          // product also on the BF16 ladder
+          if (shifted && mom_used && !momentum) {
+            build_next(false, 0.0);
+            mom_used = false;
+          }
           gram_async(ctx, m, next->w, next->p(), next->ld, next->dt, kGramOff2, ctx->small_ev2);
           R.mult(1, *next, Z2, low);
           R.fallbacks = fb;
@@ -468,6 +480,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         wq_spec = wq_next;
       }
       if (shifted) shift_used = shift;
+      mom_run = mom_used ? mom_run + 1 : 0;
       // Adopt the next iterate: Y <- Y_{t+1}, keeping Y_t and M_t as the
       // previous iterate for the momentum term.
       if (adaptive && momentum && shift > 0.0) {
