@@ -152,11 +152,20 @@ struct Schedule {
   int32_t blk_off[8] = {0};
   int32_t blk_cnt[8] = {0};
   int32_t nslots = 0;
-  // A x with staged hot columns: hot_h rows staged (0: not used, -1: not
-  // decided yet), hot_colidx = column indices with the staged ones encoded as
-  // -(1 + slot), hot_rows = the staged columns by slot.
-  int hot_h = -1;
-  DevBuf hot_colidx, hot_rows;
+  // A x with staged hot columns. hot_state: -1 not decided yet, 0 not used,
+  // 1 used; hot_cover[i] = nonzeros in the i + 1 most popular columns,
+  // hot_order = columns by popularity. Each encoding stages the h most
+  // popular columns (h depends on the panel element size): colidx = column
+  // indices with the staged ones encoded as -(1 + slot), rows = the staged
+  // columns by slot.
+  struct HotEnc {
+    int h = 0;
+    DevBuf colidx, rows;
+  };
+  int hot_state = -1;
+  std::vector<int64_t> hot_cover;
+  std::vector<int32_t> hot_order;
+  HotEnc hot[2];
   // Short rows cut at source blocks: kSegInts per segment, grouped by block.
   DevBuf segs;
   int32_t seg_off[8] = {0};

@@ -1186,49 +1186,58 @@ static bool plan_spmm(svdh_op *op, int direction, const void *vals, const void *
 
 // Staged hot columns for A x of a plain operator whose rows are all short:
 // h = as many panel rows as fit 64 KB of LDS, used when the h most popular
-// columns hold at least a fifth of the nonzeros. Returns h (0: not used).
-static int hot_columns(svdh_op *op, Schedule &s, int h) {
-  if (op->base || s.total_blocks > 0 || s.nblk > 1 || h < 16 || op->ncols < 4 * (int64_t)h) return 0;
+// columns hold at least a fifth of the nonzeros. Returns the encoding for h,
+// or null when staging is not used. The popularity order is computed once per
+// operator; encodings for two values of h (FP32 and BF16 panels) are kept.
+static const Schedule::HotEnc *hot_columns(svdh_op *op, Schedule &s, int h) {
+  if (op->base || s.total_blocks > 0 || s.nblk > 1 || h < 16 || op->ncols < 4 * (int64_t)h) return nullptr;
   const char *env = std::getenv("SVDH_SPMM_HOT");
-  if (env && std::atoi(env) == 0) return 0;
-  if (s.hot_h == h) return h;
-  if (s.hot_h == 0) return 0;  // decided against for this operator
-  Ctx *c = op->ctx;
+  if (env && std::atoi(env) == 0) return nullptr;
+  if (s.hot_state == 0) return nullptr;
   const int64_t n = op->ncols, nnz = op->nnz;
-  std::vector<int64_t> cnt(n, 0);
-  for (int32_t cidx : op->h_colidx) ++cnt[cidx];
-  std::vector<int32_t> order(n);
-  for (int64_t j = 0; j < n; ++j) order[j] = (int32_t)j;
-  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cnt[x] > cnt[y]; });
-  int64_t cover = 0;
-  for (int i = 0; i < h; ++i) cover += cnt[order[i]];
-  if (5 * cover < nnz) {
-    s.hot_h = 0;
-    return 0;
+  if (s.hot_state < 0) {
+    std::vector<int64_t> cnt(n, 0);
+    for (int32_t cidx : op->h_colidx) ++cnt[cidx];
+    std::vector<int32_t> order(n);
+    for (int64_t j = 0; j < n; ++j) order[j] = (int32_t)j;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cnt[x] > cnt[y]; });
+    const int64_t keep = std::min<int64_t>(n, 1024);
+    s.hot_cover.assign(keep, 0);
+    int64_t cover = 0;
+    for (int64_t i = 0; i < keep; ++i) s.hot_cover[i] = cover += cnt[order[i]];
+    order.resize(keep);
+    s.hot_order = std::move(order);
+    s.hot_state = 1;
   }
+  if (h > (int)s.hot_cover.size() || 5 * s.hot_cover[h - 1] < nnz) return nullptr;
+  for (const auto &e : s.hot)
+    if (e.h == h) return &e;
+  Schedule::HotEnc &e = s.hot[0].h == 0 ? s.hot[0] : s.hot[1];
+  Ctx *c = op->ctx;
   std::vector<int32_t> slot(n, -1);
-  for (int i = 0; i < h; ++i) slot[order[i]] = i;
+  for (int i = 0; i < h; ++i) slot[s.hot_order[i]] = i;
   DevBuf dslot(n * sizeof(int32_t));
   HIP_CHECK(hipMemcpyAsync(dslot.p, slot.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  s.hot_rows.alloc((size_t)h * sizeof(int32_t));
-  HIP_CHECK(hipMemcpyAsync(s.hot_rows.p, order.data(), (size_t)h * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  s.hot_colidx.alloc(std::max<size_t>(nnz * sizeof(int32_t), 16));
+  e.rows.alloc((size_t)h * sizeof(int32_t));
+  HIP_CHECK(hipMemcpyAsync(e.rows.p, s.hot_order.data(), (size_t)h * sizeof(int32_t), hipMemcpyHostToDevice,
+                           c->stream));
+  e.colidx.alloc(std::max<size_t>(nnz * sizeof(int32_t), 16));
   hipLaunchKernelGGL(k_hot_encode, grid_for(nnz), kBlock, 0, c->stream, nnz, op->colidx.as<int32_t>(),
-                     dslot.as<int32_t>(), s.hot_colidx.as<int32_t>());
+                     dslot.as<int32_t>(), e.colidx.as<int32_t>());
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  s.hot_h = h;
-  return h;
+  e.h = h;
+  return &e;
 }
 
 template <int LGR, typename V, typename X, typename Y, typename Acc>
-static void spmm_hot_launch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, const Schedule &s, int h) {
+static void spmm_hot_launch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, const Schedule::HotEnc &e) {
   constexpr int RPB = kHotBlock >> LGR;
   const int64_t need = ((int64_t)a.nshort + RPB - 1) / RPB;
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)c->ncu, need));
-  const size_t shm = (size_t)h * a.ldx * sizeof(X);
+  const size_t shm = (size_t)e.h * a.ldx * sizeof(X);
   hipLaunchKernelGGL((k_spmm_hot<LGR, V, X, Y, Acc>), blocks, kHotBlock, shm, c->stream, a,
-                     s.hot_colidx.as<int32_t>(), s.hot_rows.as<int32_t>(), h);
+                     e.colidx.as<int32_t>(), e.rows.as<int32_t>(), e.h);
 }
 
 template <typename V, typename X, typename Y, typename Acc>
@@ -1238,16 +1247,17 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   SpmmPlan<V, X, Y, Acc> p;
   if (!plan_spmm(op, direction, vals, X_, ldx, Y_, ldy, l, p)) return;
   if (direction == 0 && !p.mc) {
-    const int h = hot_columns(op, *p.s, std::min<int>(1024, (int)(65536 / (ldx * (int64_t)sizeof(X)))));
-    if (h > 0) {
+    const Schedule::HotEnc *e =
+        hot_columns(op, *p.s, std::min<int>(1024, (int)(65536 / (ldx * (int64_t)sizeof(X)))));
+    if (e) {
       KTimer t(c, KC_SPMM, p.bytes);
       switch (p.lgr) {
-        case 1: spmm_hot_launch<1>(c, p.a, *p.s, h); break;
-        case 2: spmm_hot_launch<2>(c, p.a, *p.s, h); break;
-        case 3: spmm_hot_launch<3>(c, p.a, *p.s, h); break;
-        case 4: spmm_hot_launch<4>(c, p.a, *p.s, h); break;
-        case 5: spmm_hot_launch<5>(c, p.a, *p.s, h); break;
-        default: spmm_hot_launch<6>(c, p.a, *p.s, h); break;
+        case 1: spmm_hot_launch<1>(c, p.a, *e); break;
+        case 2: spmm_hot_launch<2>(c, p.a, *e); break;
+        case 3: spmm_hot_launch<3>(c, p.a, *e); break;
+        case 4: spmm_hot_launch<4>(c, p.a, *e); break;
+        case 5: spmm_hot_launch<5>(c, p.a, *e); break;
+        default: spmm_hot_launch<6>(c, p.a, *e); break;
       }
       HIP_CHECK(hipGetLastError());
       return;
