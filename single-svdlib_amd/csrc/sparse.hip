@@ -995,6 +995,21 @@ static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
       ++nmulti;
     }
   }
+  // longest segments first within each block, so the row groups of a wave
+  // (consecutive segments) have nearly equal trip counts
+  for (auto &v : segper) {
+    const size_t ns = v.size() / kSegInts;
+    std::vector<int32_t> idx(ns);
+    for (size_t i = 0; i < ns; ++i) idx[i] = (int32_t)i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
+      return v[x * kSegInts + 2] - v[x * kSegInts + 1] > v[y * kSegInts + 2] - v[y * kSegInts + 1];
+    });
+    std::vector<int32_t> sorted(v.size());
+    for (size_t i = 0; i < ns; ++i)
+      std::copy(v.begin() + (size_t)idx[i] * kSegInts, v.begin() + (size_t)(idx[i] + 1) * kSegInts,
+                sorted.begin() + i * kSegInts);
+    v.swap(sorted);
+  }
   {
     std::vector<int32_t> sflat;
     int32_t soff = 0;
