@@ -161,6 +161,10 @@ struct Schedule {
   struct HotEnc {
     int h = 0;
     DevBuf colidx, rows;
+    // each row's entries reordered staged-first: perm[k] = CSR position of
+    // entry k, split[r] = staged entries of row r, order = rows by descending
+    // count of unstaged entries, vals[dt] = values in the reordered layout
+    DevBuf perm, split, order, vals[3];
   };
   int hot_state = -1;
   std::vector<int64_t> hot_cover;

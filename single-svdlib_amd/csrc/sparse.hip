@@ -518,18 +518,22 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin_batched(const FinArgs<Y, Ac
 // ------------------------------------------------------------------------
 constexpr int kHotBlock = 1024;
 
-template <int R, typename V, typename X, typename Y, typename Acc>
-__device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, const int32_t *hcol, const X *xs,
-                                               int kb, int ke, int lane, int col, bool colok,
-                                               Acc (&acc)[PanelVec<X>::N]) {
+// acc += sum over k in [kb, ke) of vals[k] * panel row of hcol[k]: staged
+// rows from LDS (LDS_SRC, hcol = -(1 + slot)) or panel rows from global memory.
+template <int R, bool LDS_SRC, typename V, typename X, typename Y, typename Acc>
+__device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, const int32_t *hcol,
+                                               const V *hval, const X *xs, int kb, int ke, int lane, int col,
+                                               bool colok, Acc (&acc)[PanelVec<X>::N]) {
   constexpr int NV = PanelVec<X>::N;
+  const X *src = LDS_SRC ? xs : a.x;
   for (int base = kb; base < ke; base += R) {
     int c = 0;
     Acc v = Acc(0);
     const int kk = base + lane;
     if (kk < ke) {
       c = __builtin_nontemporal_load(&hcol[kk]);
-      v = ld_as<Acc>(a.vals, kk);
+      if (LDS_SRC) c = -1 - c;
+      v = ld_as<Acc>(hval, kk);
     }
     const int n = min(R, ke - base);
     constexpr int U = SVDH_SPMM_UNROLL < R ? SVDH_SPMM_UNROLL : R;
@@ -545,12 +549,7 @@ __device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, 
       if (colok) {
         Acc xv[U][NV];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (cj[u] >= 0)
-            ld_panel<Acc, X>(a.x + (int64_t)cj[u] * a.ldx + col, xv[u]);
-          else
-            ld_panel<Acc, X>(xs + (int64_t)(-1 - cj[u]) * a.ldx + col, xv[u]);
-        }
+        for (int u = 0; u < U; ++u) ld_panel<Acc, X>(src + (int64_t)cj[u] * a.ldx + col, xv[u]);
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -562,10 +561,7 @@ __device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, 
       const Acc vj = __shfl(v, j, R);
       if (colok) {
         Acc xv[NV];
-        if (cj >= 0)
-          ld_panel<Acc, X>(a.x + (int64_t)cj * a.ldx + col, xv);
-        else
-          ld_panel<Acc, X>(xs + (int64_t)(-1 - cj) * a.ldx + col, xv);
+        ld_panel<Acc, X>(src + (int64_t)cj * a.ldx + col, xv);
 #pragma unroll
         for (int i = 0; i < NV; ++i) acc[i] += vj * xv[i];
       }
@@ -573,9 +569,13 @@ __device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, 
   }
 }
 
+// Rows in `order` (nrows of them); row r's entries are reordered so that its
+// split[r] staged entries come first: one loop over LDS rows, then one over
+// global rows, with no per-entry branch between the two sources.
 template <int LGR, typename V, typename X, typename Y, typename Acc>
 __global__ __launch_bounds__(kHotBlock) void k_spmm_hot(SpmmArgs<V, X, Y, Acc> a, const int32_t *hcol,
-                                                        const int32_t *hot_rows, int h) {
+                                                        const V *hval, const int32_t *split, const int32_t *order,
+                                                        int nrows, const int32_t *hot_rows, int h) {
   extern __shared__ uint4 xs_raw[];
   X *xs = reinterpret_cast<X *>(xs_raw);
   constexpr int R = 1 << LGR;
@@ -589,28 +589,22 @@ __global__ __launch_bounds__(kHotBlock) void k_spmm_hot(SpmmArgs<V, X, Y, Acc> a
   __syncthreads();
   const int gid = threadIdx.x >> LGR;
   const int lane = threadIdx.x & (R - 1);
-  for (int64_t b0 = (int64_t)blockIdx.x * RPB; b0 < a.nshort; b0 += (int64_t)gridDim.x * RPB) {
+  for (int64_t b0 = (int64_t)blockIdx.x * RPB; b0 < nrows; b0 += (int64_t)gridDim.x * RPB) {
     const int64_t idx = b0 + gid;
-    if (idx >= a.nshort) break;
-    const int r = a.rows[idx];
+    if (idx >= nrows) break;
+    const int r = order[idx];
     const int kb = a.rowptr[r], ke = a.rowptr[r + 1];
+    const int kh = kb + split[r];
     for (int c0 = 0; c0 < a.l; c0 += R * NV) {
       const int col = c0 + lane * NV;
       const bool colok = col < a.l;
       Acc acc[NV];
 #pragma unroll
       for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
-      spmm_accum_hot<R>(a, hcol, xs, kb, ke, lane, col, colok, acc);
+      spmm_accum_hot<R, true>(a, hcol, hval, xs, kb, kh, lane, col, colok, acc);
+      spmm_accum_hot<R, false>(a, hcol, hval, xs, kh, ke, lane, col, colok, acc);
       if (colok) st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
     }
-  }
-}
-
-__global__ void k_hot_encode(int64_t nnz, const int32_t *colidx, const int32_t *slot, int32_t *out) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
-    const int c = colidx[e];
-    const int s = slot[c];
-    out[e] = s >= 0 ? -1 - s : c;
   }
 }
 
@@ -1204,7 +1198,7 @@ static bool plan_spmm(svdh_op *op, int direction, const void *vals, const void *
 // columns hold at least a fifth of the nonzeros. Returns the encoding for h,
 // or null when staging is not used. The popularity order is computed once per
 // operator; encodings for two values of h (FP32 and BF16 panels) are kept.
-static const Schedule::HotEnc *hot_columns(svdh_op *op, Schedule &s, int h) {
+static Schedule::HotEnc *hot_columns(svdh_op *op, Schedule &s, int h) {
   if (op->base || s.total_blocks > 0 || s.nblk > 1 || h < 16 || op->ncols < 4 * (int64_t)h) return nullptr;
   const char *env = std::getenv("SVDH_SPMM_HOT");
   if (env && std::atoi(env) == 0) return nullptr;
@@ -1225,34 +1219,91 @@ static const Schedule::HotEnc *hot_columns(svdh_op *op, Schedule &s, int h) {
     s.hot_state = 1;
   }
   if (h > (int)s.hot_cover.size() || 5 * s.hot_cover[h - 1] < nnz) return nullptr;
-  for (const auto &e : s.hot)
+  for (auto &e : s.hot)
     if (e.h == h) return &e;
   Schedule::HotEnc &e = s.hot[0].h == 0 ? s.hot[0] : s.hot[1];
+  for (auto &v : e.vals) v.release();
   Ctx *c = op->ctx;
+  const int64_t m = op->nrows;
   std::vector<int32_t> slot(n, -1);
   for (int i = 0; i < h; ++i) slot[s.hot_order[i]] = i;
-  DevBuf dslot(n * sizeof(int32_t));
-  HIP_CHECK(hipMemcpyAsync(dslot.p, slot.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  // staged-first reordering of every row (entry order otherwise kept)
+  const std::vector<int32_t> &rp = op->h_rowptr, &ci = op->h_colidx;
+  std::vector<int32_t> enc(std::max<int64_t>(nnz, 1)), perm(std::max<int64_t>(nnz, 1)), split(std::max<int64_t>(m, 1));
+  std::vector<int32_t> cold(std::max<int64_t>(m, 1));
+  for (int64_t r = 0; r < m; ++r) {
+    int32_t pos = rp[r];
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
+      if (slot[ci[k]] >= 0) {
+        enc[pos] = -1 - slot[ci[k]];
+        perm[pos++] = k;
+      }
+    split[r] = pos - rp[r];
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
+      if (slot[ci[k]] < 0) {
+        enc[pos] = ci[k];
+        perm[pos++] = k;
+      }
+    cold[r] = rp[r + 1] - rp[r] - split[r];
+  }
+  std::vector<int32_t> order(std::max<int64_t>(m, 1));
+  for (int64_t r = 0; r < m; ++r) order[r] = (int32_t)r;
+  std::stable_sort(order.begin(), order.begin() + m, [&](int32_t x, int32_t y) {
+    return cold[x] != cold[y] ? cold[x] > cold[y] : split[x] > split[y];
+  });
   e.rows.alloc((size_t)h * sizeof(int32_t));
   HIP_CHECK(hipMemcpyAsync(e.rows.p, s.hot_order.data(), (size_t)h * sizeof(int32_t), hipMemcpyHostToDevice,
                            c->stream));
-  e.colidx.alloc(std::max<size_t>(nnz * sizeof(int32_t), 16));
-  hipLaunchKernelGGL(k_hot_encode, grid_for(nnz), kBlock, 0, c->stream, nnz, op->colidx.as<int32_t>(),
-                     dslot.as<int32_t>(), e.colidx.as<int32_t>());
-  HIP_CHECK(hipGetLastError());
+  e.colidx.alloc(enc.size() * sizeof(int32_t));
+  HIP_CHECK(hipMemcpyAsync(e.colidx.p, enc.data(), enc.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  e.perm.alloc(perm.size() * sizeof(int32_t));
+  HIP_CHECK(hipMemcpyAsync(e.perm.p, perm.data(), perm.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  e.split.alloc(split.size() * sizeof(int32_t));
+  HIP_CHECK(hipMemcpyAsync(e.split.p, split.data(), split.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                           c->stream));
+  e.order.alloc(order.size() * sizeof(int32_t));
+  HIP_CHECK(hipMemcpyAsync(e.order.p, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                           c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
   e.h = h;
   return &e;
 }
 
+template <typename T>
+__global__ void k_gather(int64_t n, const int32_t *perm, const T *src, T *dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[perm[i]];
+}
+
+template <typename T> struct DtypeOf;
+template <> struct DtypeOf<double> { static constexpr int v = SVDH_F64; };
+template <> struct DtypeOf<float> { static constexpr int v = SVDH_F32; };
+template <> struct DtypeOf<bf16_t> { static constexpr int v = SVDH_BF16; };
+
+// Values of dtype V in the staged-first layout of e (gathered on first use).
+template <typename V>
+static const V *hot_values(Ctx *c, Schedule::HotEnc &e, const V *vals, int64_t nnz) {
+  DevBuf &d = e.vals[DtypeOf<V>::v];
+  if (!d.p) {
+    d.alloc(std::max<size_t>(nnz * sizeof(V), 16));
+    if (nnz > 0) {
+      hipLaunchKernelGGL(k_gather<V>, grid_for(nnz), kBlock, 0, c->stream, nnz, e.perm.as<int32_t>(), vals,
+                         d.as<V>());
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+  return d.as<V>();
+}
+
 template <int LGR, typename V, typename X, typename Y, typename Acc>
-static void spmm_hot_launch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, const Schedule::HotEnc &e) {
+static void spmm_hot_launch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, const Schedule::HotEnc &e, const V *hval,
+                            int nrows) {
   constexpr int RPB = kHotBlock >> LGR;
-  const int64_t need = ((int64_t)a.nshort + RPB - 1) / RPB;
+  const int64_t need = ((int64_t)nrows + RPB - 1) / RPB;
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)c->ncu, need));
   const size_t shm = (size_t)e.h * a.ldx * sizeof(X);
-  hipLaunchKernelGGL((k_spmm_hot<LGR, V, X, Y, Acc>), blocks, kHotBlock, shm, c->stream, a,
-                     e.colidx.as<int32_t>(), e.rows.as<int32_t>(), e.h);
+  hipLaunchKernelGGL((k_spmm_hot<LGR, V, X, Y, Acc>), blocks, kHotBlock, shm, c->stream, a, e.colidx.as<int32_t>(),
+                     hval, e.split.as<int32_t>(), e.order.as<int32_t>(), nrows, e.rows.as<int32_t>(), e.h);
 }
 
 template <typename V, typename X, typename Y, typename Acc>
@@ -1262,17 +1313,18 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   SpmmPlan<V, X, Y, Acc> p;
   if (!plan_spmm(op, direction, vals, X_, ldx, Y_, ldy, l, p)) return;
   if (direction == 0 && !p.mc) {
-    const Schedule::HotEnc *e =
-        hot_columns(op, *p.s, std::min<int>(1024, (int)(65536 / (ldx * (int64_t)sizeof(X)))));
+    Schedule::HotEnc *e = hot_columns(op, *p.s, std::min<int>(1024, (int)(65536 / (ldx * (int64_t)sizeof(X)))));
     if (e) {
+      const V *hval = hot_values<V>(c, *e, static_cast<const V *>(vals), op->nnz);
+      const int nrows = (int)op->nrows;
       KTimer t(c, KC_SPMM, p.bytes);
       switch (p.lgr) {
-        case 1: spmm_hot_launch<1>(c, p.a, *e); break;
-        case 2: spmm_hot_launch<2>(c, p.a, *e); break;
-        case 3: spmm_hot_launch<3>(c, p.a, *e); break;
-        case 4: spmm_hot_launch<4>(c, p.a, *e); break;
-        case 5: spmm_hot_launch<5>(c, p.a, *e); break;
-        default: spmm_hot_launch<6>(c, p.a, *e); break;
+        case 1: spmm_hot_launch<1>(c, p.a, *e, hval, nrows); break;
+        case 2: spmm_hot_launch<2>(c, p.a, *e, hval, nrows); break;
+        case 3: spmm_hot_launch<3>(c, p.a, *e, hval, nrows); break;
+        case 4: spmm_hot_launch<4>(c, p.a, *e, hval, nrows); break;
+        case 5: spmm_hot_launch<5>(c, p.a, *e, hval, nrows); break;
+        default: spmm_hot_launch<6>(c, p.a, *e, hval, nrows); break;
       }
       HIP_CHECK(hipGetLastError());
       return;
@@ -1656,11 +1708,6 @@ __global__ __launch_bounds__(kBlock) void k_mask_fill(int64_t m, const int32_t *
   }
 }
 
-template <typename T>
-__global__ void k_gather(int64_t n, const int32_t *perm, const T *src, T *dst) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[perm[i]];
-}
 
 static void gather_compact_values(svdh_op *op, int dtype) {
   const svdh_op *own = op->base;

@@ -201,8 +201,9 @@ def test_spmm_transposed_source_blocks(svdlib, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_spmm_hot_columns_staged_in_lds(svdlib, dtype):
-    # Zipf columns: A X stages the most popular panel rows in LDS; the result
-    # equals the oracle and, bit for bit, the path without staging
+    # Zipf columns: A X stages the most popular panel rows in LDS and sums
+    # each row's staged entries first; both it and the path without staging
+    # equal the exact product within the FP error bound
     import os
     a = skewed_csr(8000, 3000, seed=12, mean_row=30, max_row=200, zipf=1.2).astype(dtype)
     op = svdlib.CsrMatrix.from_scipy(a)
@@ -221,4 +222,5 @@ def test_spmm_hot_columns_staged_in_lds(svdlib, dtype):
             op.multiply_with_dense(X, Y2, False)
         finally:
             del os.environ["SVDH_SPMM_HOT"]
-        np.testing.assert_array_equal(Y, Y2)
+        check_close(Y2, want, bound(a64, X.astype(np.float64), eps) + eps * np.abs(want))
+        assert not np.array_equal(Y, np.zeros_like(Y))
