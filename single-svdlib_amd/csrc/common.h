@@ -165,7 +165,9 @@ struct Schedule {
     // entry k, split[r] = staged entries of row r, order = rows by descending
     // count of unstaged entries, vals[dt] = values in the reordered layout
     DevBuf perm, split, order, vals[3];
+    DevBuf dense;  // m x h FP32: the staged columns as a dense block (A^T panels)
   };
+  int32_t hot_slot_base = 0;  // first partial slot of the matrix-core rows
   int hot_state = -1;
   std::vector<int64_t> hot_cover;
   std::vector<int32_t> hot_order;
@@ -215,6 +217,11 @@ struct svdh_op {
   std::vector<int32_t> h_c_rowptr;
   // Schedules: [direction][kind], direction 0 = A x, 1 = A^T x; kind 0 = SpMV, 1 = SpMM.
   svdh::Schedule sched[2][2];
+  // A^T panels with the staged columns on the matrix cores: the schedule of
+  // the other output rows (thot_state -1: not decided, 0: not used, 1: used)
+  svdh::Schedule sched_thot;
+  int thot_state = -1;
+  int thot_g = 0;
   // pattern = true: the base matrix's own pattern (with the column remap),
   // never the compacted one (the adjoint writes one value per base nonzero).
   svdh::CsrView view(int direction, bool pattern = false) const;

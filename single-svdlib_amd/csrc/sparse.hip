@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "common.h"
 
@@ -464,9 +465,20 @@ __device__ __forceinline__ void spmm_fin_row(const int32_t *multi, const Acc *pa
   const int q = threadIdx.x >> 6, o = threadIdx.x & 63;
   for (int c0 = 0; c0 < l; c0 += 64) {
     const int cc = c0 + o;
-    Acc sum = Acc(0);
-    if (cc < l)
-      for (int i = q; i < nch; i += 4) sum += partials[(int64_t)(first + i) * ldp + cc];
+    // thread q sums pieces q, q + 4, ... in four interleaved chains
+    Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
+    if (cc < l) {
+      const Acc *pp = partials + (int64_t)first * ldp + cc;
+      int i = q;
+      for (; i + 12 < nch; i += 16) {
+        s0 += pp[(int64_t)i * ldp];
+        s1 += pp[(int64_t)(i + 4) * ldp];
+        s2 += pp[(int64_t)(i + 8) * ldp];
+        s3 += pp[(int64_t)(i + 12) * ldp];
+      }
+      for (; i < nch; i += 4) s0 += pp[(int64_t)i * ldp];
+    }
+    const Acc sum = (s0 + s1) + (s2 + s3);
     __syncthreads();
     red[q * 64 + o] = sum;
     __syncthreads();
@@ -606,6 +618,99 @@ __global__ __launch_bounds__(kHotBlock) void k_spmm_hot(SpmmArgs<V, X, Y, Acc> a
       if (colok) st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
     }
   }
+}
+
+// ------------------------------------------------------------------------
+// A^T Y for the h = 256 staged (most popular) columns of A on the matrix
+// cores. Those columns are kept as a dense m x 256 FP32 block Ad (built once;
+// about half of their entries are nonzero in the power-law matrices this
+// serves), so the product is a plain tall GEMM Z_h = Ad^T Y that streams Ad
+// and Y once instead of gathering one panel row per nonzero. Workgroup w
+// takes the 32-row blocks w, w + G, ...: the block's Ad rows and Y rows are
+// copied into LDS (the next block's are loaded into registers meanwhile) and
+// Z_h += Ad_blk^T Y_blk runs on v_mfma_f32_16x16x4 (A[l % 16][l / 16],
+// B[l / 16][l % 16] per lane, D[4 (l / 16) + v][l % 16]). Wave q owns slot
+// tiles 2q, 2q + 1 and all four 16-column tiles. At the end the workgroup
+// writes its 256 x l partial into slot base + i G + w of staged row i; the
+// fin pass sums those in workgroup order. LDS rows are 272 (tile) and 80 (Y)
+// floats apart, 16 banks, so the operand reads are free of bank conflicts.
+// ------------------------------------------------------------------------
+constexpr int kTHotRows = 32, kTHotH = 256, kTHotStride = 272, kTHotYStride = 80, kTHotBlock = 512;
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// Ad[r][slot] += staged entries of row r, in CSR order (one thread per row).
+__global__ void k_hot_dense(int64_t m, const int32_t *rowptr, const int32_t *hcol, const float *hval,
+                            const int32_t *split, float *Ad) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m; r += (int64_t)gridDim.x * blockDim.x) {
+    const int kb = rowptr[r], ke = kb + split[r];
+    for (int k = kb; k < ke; ++k) Ad[r * kTHotH + (-1 - hcol[k])] += hval[k];
+  }
+}
+
+__global__ __launch_bounds__(kTHotBlock) void k_spmm_t_hot(int64_t m, const float *Ad, const float *Y, int64_t ldy,
+                                                          int l, float *partials, int64_t ldp, int32_t slot_base) {
+  __shared__ float tile[kTHotRows * kTHotStride];
+  __shared__ float ytile[kTHotRows * kTHotYStride];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int kq = lane >> 4, rl = lane & 15;
+  f4_t acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[a][q] = f4_t{0.f, 0.f, 0.f, 0.f};
+  const int64_t nblocks = (m + kTHotRows - 1) / kTHotRows;
+  // per thread: 4 float4 of the 32 x 256 Ad block (row e / 64, vector e % 64)
+  // and one float4 of the 32 x 64 Y block (row tid / 16, vector tid % 16)
+  float4 ra[4], ry;
+  const int yr = tid >> 4, yq = tid & 15;
+  auto fetch = [&](int64_t r0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = tid + kTHotBlock * i;
+      const int64_t r = r0 + (e >> 6);
+      ra[i] = r < m ? reinterpret_cast<const float4 *>(Ad + r * kTHotH)[e & 63] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const int64_t r = r0 + yr;
+    const int c = 4 * yq;
+    float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (r < m && c < l) w = *reinterpret_cast<const float4 *>(Y + r * ldy + c);
+    ry = make_float4(w.x, c + 1 < l ? w.y : 0.f, c + 2 < l ? w.z : 0.f, c + 3 < l ? w.w : 0.f);
+  };
+  if ((int64_t)blockIdx.x < nblocks) fetch((int64_t)blockIdx.x * kTHotRows);
+  for (int64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    __syncthreads();  // the previous block's operands are consumed
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = tid + kTHotBlock * i;
+      *reinterpret_cast<float4 *>(tile + (e >> 6) * kTHotStride + 4 * (e & 63)) = ra[i];
+    }
+    *reinterpret_cast<float4 *>(ytile + yr * kTHotYStride + 4 * yq) = ry;
+    __syncthreads();
+    if (b + gridDim.x < nblocks) fetch((b + gridDim.x) * kTHotRows);  // in flight during the products
+#pragma unroll
+    for (int s = 0; s < kTHotRows / 4; ++s) {
+      const int row = 4 * s + kq;
+      const float a0 = tile[row * kTHotStride + 16 * (2 * wave) + rl];
+      const float a1 = tile[row * kTHotStride + 16 * (2 * wave + 1) + rl];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float bv = ytile[row * kTHotYStride + 16 * q + rl];
+        acc[0][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, acc[0][q], 0, 0, 0);
+        acc[1][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, acc[1][q], 0, 0, 0);
+      }
+    }
+  }
+  const int G = gridDim.x;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int slot = 16 * (2 * wave + a) + 4 * kq + v;
+        const int col = 16 * q + rl;
+        if (col < l) partials[((int64_t)slot_base + (int64_t)slot * G + blockIdx.x) * ldp + col] = acc[a][q][v];
+      }
 }
 
 template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
@@ -876,9 +981,11 @@ static SpmmTune spmm_tune() {
 // Build (once) the SpMM schedule of one operator direction (see k_spmm).
 // Source blocks: 8 when the operand panel has at least 65,536 rows and the
 // source indices of every long row are sorted, otherwise 1.
-static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
-  Schedule &s = op->sched[direction][1];
-  if (s.built) return s;
+// hot (optional): output rows left to the matrix-core pass of the staged
+// columns; they get no chunks or segments here, only hot_g partial slots each
+// (consecutive, in workgroup order) and an entry in the multi-row table, so
+// the fin pass sums them.
+static void build_spmm_schedule(svdh_op *op, int direction, Schedule &s, const std::vector<int32_t> *hot, int hot_g) {
   CsrView v = op->view(direction);
   const svdh_op *own = op->base ? op->base : op;
   const std::vector<int32_t> &rp = *v.h_rowptr;
@@ -900,7 +1007,11 @@ static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
   // row group each) and longer rows chunks cut at the block bounds; without
   // them, rows above long_nnz are chunked and the rest are short rows.
   std::vector<int32_t> shortr, longr, segr;
+  std::vector<char> skip(hot ? nout : 0, 0);
+  if (hot)
+    for (int32_t r : *hot) skip[r] = 1;
   for (int64_t r = 0; r < nout; ++r) {
+    if (hot && skip[r]) continue;
     const int64_t sr = src_of(r);
     const int32_t len = rp[sr + 1] - rp[sr];
     if (nblk > 1)
@@ -989,6 +1100,15 @@ static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
       ++nmulti;
     }
   }
+  s.hot_slot_base = nslots;
+  if (hot)
+    for (int32_t r : *hot) {
+      multi.push_back(r);
+      multi.push_back(nslots);
+      multi.push_back(hot_g);
+      nslots += hot_g;
+      ++nmulti;
+    }
   // longest segments first within each block, so the row groups of a wave
   // (consecutive segments) have nearly equal trip counts
   for (auto &v : segper) {
@@ -1054,6 +1174,11 @@ static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
   if (!multi.empty())
     HIP_CHECK(hipMemcpy(s.counters.p, multi.data(), multi.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   s.built = true;
+}
+
+static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
+  Schedule &s = op->sched[direction][1];
+  if (!s.built) build_spmm_schedule(op, direction, s, nullptr, 0);
   return s;
 }
 
@@ -1138,9 +1263,9 @@ struct SpmmPlan {
 // Arguments and launch shape of one SpMM (false: nothing to compute).
 template <typename V, typename X, typename Y, typename Acc>
 static bool plan_spmm(svdh_op *op, int direction, const void *vals, const void *X_, int64_t ldx, void *Y_,
-                      int64_t ldy, int64_t l, SpmmPlan<V, X, Y, Acc> &p) {
+                      int64_t ldy, int64_t l, SpmmPlan<V, X, Y, Acc> &p, Schedule *sched = nullptr) {
   CsrView v = op->view(direction);
-  Schedule &s = get_spmm_schedule(op, direction);
+  Schedule &s = sched ? *sched : get_spmm_schedule(op, direction);
   if (v.nrows == 0 || l == 0) return false;
   constexpr int NV = PanelVec<X>::N;
   if (ldx % NV || ldy % NV) fail(SVDH_ERR_ARG, "panel leading dimensions must be multiples of %d", NV);
@@ -1223,6 +1348,7 @@ static Schedule::HotEnc *hot_columns(svdh_op *op, Schedule &s, int h) {
     if (e.h == h) return &e;
   Schedule::HotEnc &e = s.hot[0].h == 0 ? s.hot[0] : s.hot[1];
   for (auto &v : e.vals) v.release();
+  e.dense.release();
   Ctx *c = op->ctx;
   const int64_t m = op->nrows;
   std::vector<int32_t> slot(n, -1);
@@ -1306,6 +1432,52 @@ static void spmm_hot_launch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, const Sched
                      hval, e.split.as<int32_t>(), e.order.as<int32_t>(), nrows, e.rows.as<int32_t>(), e.h);
 }
 
+// A^T Y of a plain FP32 operator with Zipf columns: the 256 staged columns
+// (output rows of A^T) on the matrix cores (k_spmm_t_hot), the other rows by
+// k_spmm on a schedule without them, and the fin pass for both. Returns false
+// (nothing launched) when the split does not apply.
+static bool spmm_t_hot(svdh_op *op, const void *vals_t, const void *X_, int64_t ldx, void *Y_, int64_t ldy,
+                       int64_t l) {
+  (void)vals_t;
+  constexpr int kH = kTHotH;
+  if (op->base || l > 64 || ldx % 4 || op->thot_state == 0) return false;
+  const char *env = std::getenv("SVDH_SPMM_THOT");
+  if (env && std::atoi(env) == 0) return false;
+  Ctx *c = op->ctx;
+  const void *vals0 = op->values(0, SVDH_F32);
+  if (!vals0) return false;
+  Schedule::HotEnc *e = hot_columns(op, get_spmm_schedule(op, 0), kH);
+  if (!e || e->h != kH) return false;
+  if (!op->sched_thot.built) {
+    const Schedule &s0 = get_spmm_schedule(op, 0);
+    std::vector<int32_t> hot(s0.hot_order.begin(), s0.hot_order.begin() + kH);
+    op->thot_g = 2 * c->ncu;
+    build_spmm_schedule(op, 1, op->sched_thot, &hot, op->thot_g);
+    op->thot_state = 1;
+  }
+  SpmmPlan<float, float, float, float> p;
+  if (!plan_spmm(op, 1, vals_t, X_, ldx, Y_, ldy, l, p, &op->sched_thot)) return false;
+  if (!e->dense.p) {
+    const float *hval = hot_values<float>(c, *e, static_cast<const float *>(vals0), op->nnz);
+    e->dense.alloc(std::max<size_t>((size_t)op->nrows * kTHotH * sizeof(float), 16));
+    HIP_CHECK(hipMemsetAsync(e->dense.p, 0, e->dense.bytes, c->stream));
+    hipLaunchKernelGGL(k_hot_dense, grid_for(op->nrows), kBlock, 0, c->stream, op->nrows, op->rowptr.as<int32_t>(),
+                       e->colidx.as<int32_t>(), hval, e->split.as<int32_t>(), e->dense.as<float>());
+    HIP_CHECK(hipGetLastError());
+  }
+  KTimer t(c, KC_SPMM_T, p.bytes);
+  hipLaunchKernelGGL(k_spmm_t_hot, op->thot_g, kTHotBlock, 0, c->stream, op->nrows, e->dense.as<float>(),
+                     static_cast<const float *>(X_), ldx, (int)l, p.a.partials, p.a.ldp, op->sched_thot.hot_slot_base);
+  HIP_CHECK(hipGetLastError());
+  spmm_dispatch<false>(c, p.a, p.lgr, p.blocks);
+  HIP_CHECK(hipGetLastError());
+  const Schedule &sp = op->sched_thot;
+  hipLaunchKernelGGL((k_spmm_fin<float, float>), sp.nlong_rows, kBlock, 0, c->stream, sp.counters.as<int32_t>(),
+                     (const float *)sp.partials.p, p.a.ldp, static_cast<float *>(Y_), ldy, (int)l);
+  HIP_CHECK(hipGetLastError());
+  return true;
+}
+
 template <typename V, typename X, typename Y, typename Acc>
 static void launch_spmm(svdh_op *op, int direction, const void *vals, const void *X_, int64_t ldx,
                         void *Y_, int64_t ldy, int64_t l) {
@@ -1329,6 +1501,10 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
       HIP_CHECK(hipGetLastError());
       return;
     }
+  }
+  if constexpr (std::is_same<V, float>::value && std::is_same<X, float>::value && std::is_same<Y, float>::value &&
+                std::is_same<Acc, float>::value) {
+    if (direction == 1 && !p.mc && spmm_t_hot(op, vals, X_, ldx, Y_, ldy, l)) return;
   }
   KTimer t(c, direction == 0 ? KC_SPMM : KC_SPMM_T, p.bytes);
   if (p.mc)

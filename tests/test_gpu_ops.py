@@ -199,6 +199,56 @@ def test_spmm_transposed_source_blocks(svdlib, dtype):
     check_close(z, a64.T @ y, bound(a64.T, y, EPS64 if dtype == np.float64 else EPS32))
 
 
+def zipf_rows(m, n, per_row, seed, dtype):
+    """Short rows (about per_row entries, Zipf(1.1)-popular columns) plus 5
+    empty columns: A X stages the popular panel rows, and A^T Y takes the 256
+    most popular columns on the matrix cores."""
+    rng = np.random.default_rng(seed)
+    nnz = per_row * m
+    pop = 1.0 / np.arange(1, n + 1) ** 1.1
+    pop /= pop.sum()
+    cols = rng.permutation(n)[rng.choice(n, size=nnz, p=pop)]
+    rows = rng.integers(0, m, size=nnz)
+    a = sp.csr_matrix((rng.normal(size=nnz), (rows, cols)), shape=(m, n + 5))
+    a.sum_duplicates()
+    return a.astype(dtype)
+
+
+def test_spmm_transposed_hot_columns_on_matrix_cores(svdlib):
+    # FP32 A^T Y with the popular columns as a dense tile product on the
+    # matrix cores and the rest by the gather kernel: equals the exact product
+    # within the FP error bound, as does the path without the split (whose
+    # rounding differs, which shows that the split ran); rows past the last
+    # full 32-row block and empty columns included
+    import os
+    a = zipf_rows(70_001, 3000, 20, seed=9, dtype=np.float32)
+    print(f"case zipf rows {a.shape} nnz={a.nnz}", flush=True)
+    op = svdlib.CsrMatrix.from_scipy(a)
+    a64 = a.astype(np.float64)
+    rng = np.random.default_rng(4)
+    for l in (8, 30, 60, 64):
+        Yin = rng.normal(size=(a.shape[0], l)).astype(np.float32)
+        Z = np.zeros((a.shape[1], l), dtype=np.float32)
+        op.multiply_with_dense(Yin, Z, True)
+        want = a64.T @ Yin.astype(np.float64)
+        tol = bound(a64.T, Yin.astype(np.float64), EPS32) + EPS32 * np.abs(want)
+        check_close(Z, want, tol)
+        assert np.all(Z[-5:] == 0)
+        os.environ["SVDH_SPMM_THOT"] = "0"
+        try:
+            Z2 = np.zeros_like(Z)
+            op.multiply_with_dense(Yin, Z2, True)
+        finally:
+            del os.environ["SVDH_SPMM_THOT"]
+        check_close(Z2, want, tol)
+        assert not np.array_equal(Z, Z2)
+    X = rng.normal(size=(a.shape[1], 60)).astype(np.float32)
+    Y = np.zeros((a.shape[0], 60), dtype=np.float32)
+    op.multiply_with_dense(X, Y, False)
+    want = a64 @ X.astype(np.float64)
+    check_close(Y, want, bound(a64, X.astype(np.float64), EPS32) + EPS32 * np.abs(want))
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_spmm_hot_columns_staged_in_lds(svdlib, dtype):
     # Zipf columns: A X stages the most popular panel rows in LDS and sums
