@@ -168,6 +168,8 @@ struct Schedule {
     DevBuf dense;  // m x h FP32: the staged columns as a dense block (A^T panels)
   };
   int32_t hot_slot_base = 0;  // first partial slot of the matrix-core rows
+  int32_t hot_multi = 0;      // matrix-core rows (the last entries of the multi-row table)
+  int32_t chunk_multi = 0;    // rows cut into chunks (the first entries of the table)
   int hot_state = -1;
   std::vector<int64_t> hot_cover;
   std::vector<int32_t> hot_order;

@@ -492,6 +492,60 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin(const int32_t *multi, const
   spmm_fin_row<Y, Acc>(multi, partials, ldp, y, ldy, l, blockIdx.x);
 }
 
+// The same sum for panels of width l <= 64, four rows per workgroup: one
+// thread per column adds the row's pieces in four interleaved chains (piece
+// i goes to chain i mod 4), combined in a fixed order.
+template <typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm_fin4(const int32_t *multi, int nrows, const Acc *partials,
+                                                     int64_t ldp, Y *y, int64_t ldy, int l) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), o = threadIdx.x & 63;
+  if (row >= nrows || o >= l) return;
+  const int *md = multi + 3 * (int64_t)row;
+  const int r = md[0], first = md[1], nch = md[2];
+  const Acc *pp = partials + (int64_t)first * ldp + o;
+  Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
+  int i = 0;
+  for (; i + 3 < nch; i += 4) {
+    s0 += pp[(int64_t)i * ldp];
+    s1 += pp[(int64_t)(i + 1) * ldp];
+    s2 += pp[(int64_t)(i + 2) * ldp];
+    s3 += pp[(int64_t)(i + 3) * ldp];
+  }
+  for (; i < nch; ++i) s0 += pp[(int64_t)i * ldp];
+  y[(int64_t)r * ldy + o] = static_cast<Y>((s0 + s1) + (s2 + s3));
+}
+
+// The same sum for rows with hundreds of pieces (l <= 64): 1024 threads per
+// row, thread q of a column summing pieces q, q + 16, ... in four chains,
+// the 16 thread sums then added in q order.
+template <typename Y, typename Acc>
+__global__ __launch_bounds__(1024) void k_spmm_fin_wide(const int32_t *multi, const Acc *partials, int64_t ldp, Y *y,
+                                                       int64_t ldy, int l) {
+  __shared__ Acc red[1024];
+  const int *md = multi + 3 * (int64_t)blockIdx.x;
+  const int r = md[0], first = md[1], nch = md[2];
+  const int q = threadIdx.x >> 6, o = threadIdx.x & 63;
+  Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
+  if (o < l) {
+    const Acc *pp = partials + (int64_t)first * ldp + o;
+    int i = q;
+    for (; i + 48 < nch; i += 64) {
+      s0 += pp[(int64_t)i * ldp];
+      s1 += pp[(int64_t)(i + 16) * ldp];
+      s2 += pp[(int64_t)(i + 32) * ldp];
+      s3 += pp[(int64_t)(i + 48) * ldp];
+    }
+    for (; i < nch; i += 16) s0 += pp[(int64_t)i * ldp];
+  }
+  red[threadIdx.x] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (q == 0 && o < l) {
+    Acc t = Acc(0);
+    for (int k = 0; k < 16; ++k) t += red[k * 64 + o];
+    y[(int64_t)r * ldy + o] = static_cast<Y>(t);
+  }
+}
+
 // Batched forms: workgroup w belongs to operator b with boff[b] <= w < boff[b + 1].
 __device__ __forceinline__ int batch_of(const int32_t *boff, int nb, int w) {
   int lo = 0, hi = nb;
@@ -1069,6 +1123,7 @@ static void build_spmm_schedule(svdh_op *op, int direction, Schedule &s, const s
         multi.push_back(g.first);
         multi.push_back(g.n);
       }
+  s.chunk_multi = (int32_t)(multi.size() / 3);  // rows cut into chunks come first in the table
   // segments of the short rows, one per non-empty source block
   std::vector<std::vector<int32_t>> segper(nblk);
   for (int32_t r : segr) {
@@ -1101,6 +1156,7 @@ static void build_spmm_schedule(svdh_op *op, int direction, Schedule &s, const s
     }
   }
   s.hot_slot_base = nslots;
+  s.hot_multi = hot ? (int32_t)hot->size() : 0;
   if (hot)
     for (int32_t r : *hot) {
       multi.push_back(r);
@@ -1432,6 +1488,22 @@ static void spmm_hot_launch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, const Sched
                      hval, e.split.as<int32_t>(), e.order.as<int32_t>(), nrows, e.rows.as<int32_t>(), e.h);
 }
 
+// Sums the pieces of multi-row entries [0, n) of schedule s into Y: rows cut
+// into chunks (the first s.chunk_multi entries) one workgroup each, rows with
+// a few segments four per workgroup (l <= 64).
+template <typename Y, typename Acc>
+static void launch_fin(Ctx *c, const Schedule &s, int n, int64_t ldp, Y *y, int64_t ldy, int l) {
+  const int nch = l <= 64 ? std::min<int>(n, s.chunk_multi) : n;
+  if (nch > 0)
+    hipLaunchKernelGGL((k_spmm_fin<Y, Acc>), nch, kBlock, 0, c->stream, s.counters.as<int32_t>(),
+                       (const Acc *)s.partials.p, ldp, y, ldy, l);
+  if (n > nch)
+    hipLaunchKernelGGL((k_spmm_fin4<Y, Acc>), (n - nch + 3) / 4, kBlock, 0, c->stream,
+                       s.counters.as<int32_t>() + 3 * (int64_t)nch, n - nch, (const Acc *)s.partials.p, ldp, y, ldy,
+                       l);
+  HIP_CHECK(hipGetLastError());
+}
+
 // A^T Y of a plain FP32 operator with Zipf columns: the 256 staged columns
 // (output rows of A^T) on the matrix cores (k_spmm_t_hot), the other rows by
 // k_spmm on a schedule without them, and the fin pass for both. Returns false
@@ -1452,6 +1524,7 @@ static bool spmm_t_hot(svdh_op *op, const void *vals_t, const void *X_, int64_t 
     const Schedule &s0 = get_spmm_schedule(op, 0);
     std::vector<int32_t> hot(s0.hot_order.begin(), s0.hot_order.begin() + kH);
     op->thot_g = 2 * c->ncu;
+    if (const char *eg = std::getenv("SVDH_SPMM_THOT_G")) op->thot_g = std::max(1, std::atoi(eg));
     build_spmm_schedule(op, 1, op->sched_thot, &hot, op->thot_g);
     op->thot_state = 1;
   }
@@ -1472,8 +1545,11 @@ static bool spmm_t_hot(svdh_op *op, const void *vals_t, const void *X_, int64_t 
   spmm_dispatch<false>(c, p.a, p.lgr, p.blocks);
   HIP_CHECK(hipGetLastError());
   const Schedule &sp = op->sched_thot;
-  hipLaunchKernelGGL((k_spmm_fin<float, float>), sp.nlong_rows, kBlock, 0, c->stream, sp.counters.as<int32_t>(),
-                     (const float *)sp.partials.p, p.a.ldp, static_cast<float *>(Y_), ldy, (int)l);
+  const int ncold = sp.nlong_rows - sp.hot_multi;  // the staged rows come last in the table
+  if (ncold > 0) launch_fin<float, float>(c, sp, ncold, p.a.ldp, static_cast<float *>(Y_), ldy, (int)l);
+  hipLaunchKernelGGL((k_spmm_fin_wide<float, float>), sp.hot_multi, 1024, 0, c->stream,
+                     sp.counters.as<int32_t>() + 3 * (int64_t)ncold, (const float *)sp.partials.p, p.a.ldp,
+                     static_cast<float *>(Y_), ldy, (int)l);
   HIP_CHECK(hipGetLastError());
   return true;
 }
@@ -1513,11 +1589,7 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
     spmm_dispatch<false>(c, p.a, p.lgr, p.blocks);
   HIP_CHECK(hipGetLastError());
   const Schedule *sp = p.s;
-  if (sp->nlong_rows > 0) {
-    hipLaunchKernelGGL((k_spmm_fin<Y, Acc>), sp->nlong_rows, kBlock, 0, c->stream, sp->counters.as<int32_t>(),
-                       (const Acc *)sp->partials.p, p.a.ldp, static_cast<Y *>(Y_), ldy, (int)l);
-    HIP_CHECK(hipGetLastError());
-  }
+  if (sp->nlong_rows > 0) launch_fin<Y, Acc>(c, *sp, sp->nlong_rows, p.a.ldp, static_cast<Y *>(Y_), ldy, (int)l);
 }
 
 template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
