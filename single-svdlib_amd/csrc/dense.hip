@@ -648,6 +648,10 @@ static void launch_gram(Ctx *c, bool same, int nb, int64_t m, int l, const T *Y,
 // 32 (w / 2) .. + 32 of the tile.
 // ------------------------------------------------------------------------
 typedef float f4_t __attribute__((ext_vector_type(4)));
+#ifndef SVDH_PANEL_BLOCK
+#define SVDH_PANEL_BLOCK 256
+#endif
+constexpr int kPanelBlock = SVDH_PANEL_BLOCK;  // threads per workgroup of the panel products
 
 struct PanelTerm {
   const float *y;
@@ -689,13 +693,13 @@ __device__ __forceinline__ void panel_rows_load(const PanelTerm &t0, const Panel
 // accumulated instead and each workgroup writes its lo partials to
 // part[blockIdx.x * lo ...] (the adaptive residual check).
 template <int NP, bool RESID>
-__global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
+__global__ __launch_bounds__(kPanelBlock) void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
                                                       int64_t ldz, const float *zin, int64_t ldzin, float beta,
                                                       double *part) {
   __shared__ float ms[NP][64 * 64];
   for (int p = 0; p < NP; ++p) {
     const PanelTerm &t = p == 0 ? t0 : t1;
-    for (int e = threadIdx.x; e < 64 * 64; e += kBlock) {
+    for (int e = threadIdx.x; e < 64 * 64; e += kPanelBlock) {
       const int k = e >> 6, col = e & 63;
       ms[p][e] = (k < t.li && col < lo) ? (float)(t.alpha * t.m[(int64_t)k * lo + col]) : 0.f;
     }
@@ -713,8 +717,8 @@ __global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, Pan
 #pragma unroll
         for (int q = 0; q < 4; ++q) bm[p][g][st][q] = ms[p][(16 * g + 4 * kq + st) * 64 + 16 * q + rl];
   const int64_t nblk = (rows + 15) / 16;
-  const int64_t wave = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  const int64_t wave = (int64_t)blockIdx.x * (kPanelBlock / 64) + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * (kPanelBlock / 64);
   f4_t ya[NP][4], yn[NP][4];
   double csq[4] = {0.0, 0.0, 0.0, 0.0};  // RESID: columns 16 q + rl
   if (wave < nblk) panel_rows_load<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
@@ -767,13 +771,13 @@ __global__ __launch_bounds__(kBlock) void k_panel_mfma(int64_t rows, int lo, Pan
   }
   if (RESID) {
     // fixed-order sum over the four lane groups of each wave, then the waves
-    __shared__ double red[kBlock / 64][4][64];
+    __shared__ double red[kPanelBlock / 64][4][64];
 #pragma unroll
     for (int q = 0; q < 4; ++q) red[threadIdx.x >> 6][kq][16 * q + rl] = csq[q];
     __syncthreads();
     if (threadIdx.x < 64 && (int)threadIdx.x < lo) {
       double t = 0.0;
-      for (int w = 0; w < kBlock / 64; ++w)
+      for (int w = 0; w < kPanelBlock / 64; ++w)
         for (int g = 0; g < 4; ++g) t += red[w][g][threadIdx.x];
       part[(int64_t)blockIdx.x * lo + threadIdx.x] = t;
     }
@@ -800,13 +804,15 @@ void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const
     if (ldy[p] % 4) fail(SVDH_ERR_ARG, "panel_mfma: leading dimensions must be multiples of 4");
   KTimer tm(c, KC_DENSE, bytes);
   const int64_t nblk = (rows + 15) / 16;
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + 3) / 4, 4 * (int64_t)c->ncu));
+  // one workgroup per CU: the staged factors are read once per workgroup
+  constexpr int W = kPanelBlock / 64;
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + W - 1) / W, (int64_t)c->ncu));
   if (nterms == 1)
-    hipLaunchKernelGGL((k_panel_mfma<1, false>), g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz, Zin,
-                       ldzin, (float)beta, nullptr);
+    hipLaunchKernelGGL((k_panel_mfma<1, false>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz,
+                       Zin, ldzin, (float)beta, nullptr);
   else
-    hipLaunchKernelGGL((k_panel_mfma<2, false>), g, kBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz, Zin,
-                       ldzin, (float)beta, nullptr);
+    hipLaunchKernelGGL((k_panel_mfma<2, false>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz,
+                       Zin, ldzin, (float)beta, nullptr);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -943,7 +949,8 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
     both.insert(both.end(), M2, M2 + w2 * k);
     const double *Md = static_cast<const double *>(c->upload(both.data(), both.size() * sizeof(double)));
     const int64_t nblk = (m + 15) / 16;
-    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + 3) / 4, 4 * (int64_t)c->ncu));
+    constexpr int W = kPanelBlock / 64;
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + W - 1) / W, (int64_t)c->ncu));
     c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * k * sizeof(double)));
     double *part = c->red_part.as<double>();
     const PanelTerm t0{static_cast<const float *>(Y1), ld, (int)w, Md, 1.0};
@@ -951,11 +958,11 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
     {
       KTimer t(c, KC_DENSE, (double)m * (w + w2) * 4.0);
       if (w2 > 0)
-        hipLaunchKernelGGL((k_panel_mfma<2, true>), nb, kBlock, 0, c->stream, m, (int)k, t0, t1, nullptr, (int64_t)0,
-                           nullptr, (int64_t)0, 0.f, part);
+        hipLaunchKernelGGL((k_panel_mfma<2, true>), nb, kPanelBlock, 0, c->stream, m, (int)k, t0, t1, nullptr,
+                           (int64_t)0, nullptr, (int64_t)0, 0.f, part);
       else
-        hipLaunchKernelGGL((k_panel_mfma<1, true>), nb, kBlock, 0, c->stream, m, (int)k, t0, t0, nullptr, (int64_t)0,
-                           nullptr, (int64_t)0, 0.f, part);
+        hipLaunchKernelGGL((k_panel_mfma<1, true>), nb, kPanelBlock, 0, c->stream, m, (int)k, t0, t0, nullptr,
+                           (int64_t)0, nullptr, (int64_t)0, 0.f, part);
       HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_gram_reduce, reduce_grid(k), kBlock, 0, c->stream, nb, k, (const double *)part, c->d_small);
