@@ -363,6 +363,8 @@ int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_i
 // ascending in w, eigenvectors as columns of V (row-major l x l); V null:
 // eigenvalues only (the same values).
 void sym_eig(int64_t l, const double *A, double *w, double *V);
+// Smallest eigenvalue of a symmetric l x l row-major matrix.
+double sym_eig_min(int64_t l, const double *A);
 // Cholesky A = R^T R (upper R, row-major). Returns false on a non-positive pivot.
 bool cholesky_upper(int64_t l, const double *A, double *R);
 // Inverse of an upper-triangular matrix.

@@ -229,23 +229,23 @@ int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_i
 // eigenvector of w[r]; w ascending. With V null only the eigenvalues are
 // computed (the same values: the vectors do not feed back into the QL
 // recurrence).
-void sym_eig(int64_t n, const double *A, double *w, double *V) {
-  if (n == 0) return;
-  const bool vectors = V != nullptr;
+// Householder reduction of A / scale (scale = largest |A_ij|, returned; 0 for
+// a zero matrix, then nothing else is set) to the tridiagonal (d, e), e[i]
+// coupling i and i + 1. Step k zeroes column k below the sub-diagonal with
+// H_k = I - betas[k] v v^T, v stored in row k of vs (support k+1..n-1).
+static double tridiagonalize(int64_t n, const double *A, std::vector<double> &d, std::vector<double> &e,
+                             std::vector<double> &vs, std::vector<double> &betas) {
   double scale = 0.0;
   for (int64_t i = 0; i < n * n; ++i) scale = std::max(scale, std::fabs(A[i]));
-  if (!(scale > 0.0)) {
-    for (int64_t i = 0; i < n; ++i) w[i] = 0.0;
-    if (vectors) {
-      for (int64_t i = 0; i < n * n; ++i) V[i] = 0.0;
-      for (int64_t i = 0; i < n; ++i) V[i * n + i] = 1.0;
-    }
-    return;
-  }
+  if (!(scale > 0.0)) return 0.0;
   std::vector<double> a(n * n);
   const double inv = 1.0 / scale;
   for (int64_t i = 0; i < n * n; ++i) a[i] = A[i] * inv;
-  std::vector<double> d(n), e(n, 0.0), vs((size_t)n * n, 0.0), betas(n, 0.0), p(n), kv(n);
+  d.assign(n, 0.0);
+  e.assign(n, 0.0);
+  vs.assign((size_t)n * n, 0.0);
+  betas.assign(n, 0.0);
+  std::vector<double> p(n), kv(n);
   // Reduction: step k zeroes column k below the sub-diagonal with
   // H_k = I - beta v v^T, v supported on k+1..n-1 (stored in row k of vs).
   for (int64_t k = 0; k + 2 < n; ++k) {
@@ -269,24 +269,44 @@ void sym_eig(int64_t n, const double *A, double *w, double *V) {
     for (int64_t i = k + 2; i < n; ++i) a[i * n + k] = a[k * n + i] = 0.0;
     // trailing block T = a[k+1.., k+1..]: p = beta T v, K = p - (beta/2)(v.p) v,
     // T <- T - v K^T - K v^T
-    double vp = 0.0;
+    // p = beta T v as a sum of scaled rows (T symmetric): contiguous axpys
+    double *__restrict pp = p.data();
+    const double *__restrict vv = v;
+    for (int64_t j = k + 1; j < n; ++j) pp[j] = 0.0;
     for (int64_t i = k + 1; i < n; ++i) {
-      const double *ai = a.data() + i * n;
-      double t = 0.0;
-      for (int64_t j = k + 1; j < n; ++j) t += ai[j] * v[j];
-      p[i] = beta * t;
-      vp += v[i] * p[i];
+      const double *__restrict ai = a.data() + i * n;
+      const double vi = beta * vv[i];
+      for (int64_t j = k + 1; j < n; ++j) pp[j] += vi * ai[j];
     }
+    double vp = 0.0;
+    for (int64_t i = k + 1; i < n; ++i) vp += vv[i] * pp[i];
     const double h = 0.5 * beta * vp;
-    for (int64_t i = k + 1; i < n; ++i) kv[i] = p[i] - h * v[i];
+    double *__restrict kk = kv.data();
+    for (int64_t i = k + 1; i < n; ++i) kk[i] = pp[i] - h * vv[i];
     for (int64_t i = k + 1; i < n; ++i) {
-      double *ai = a.data() + i * n;
-      const double vi = v[i], ki = kv[i];
-      for (int64_t j = k + 1; j < n; ++j) ai[j] -= vi * kv[j] + ki * v[j];
+      double *__restrict ai = a.data() + i * n;
+      const double vi = vv[i], ki = kk[i];
+      for (int64_t j = k + 1; j < n; ++j) ai[j] -= vi * kk[j] + ki * vv[j];
     }
   }
   for (int64_t i = 0; i < n; ++i) d[i] = a[i * n + i];
   for (int64_t i = 0; i + 1 < n; ++i) e[i] = a[(i + 1) * n + i];  // e[i] couples i and i+1
+  return scale;
+}
+
+void sym_eig(int64_t n, const double *A, double *w, double *V) {
+  if (n == 0) return;
+  const bool vectors = V != nullptr;
+  std::vector<double> d, e, vs, betas;
+  const double scale = tridiagonalize(n, A, d, e, vs, betas);
+  if (!(scale > 0.0)) {
+    for (int64_t i = 0; i < n; ++i) w[i] = 0.0;
+    if (vectors) {
+      for (int64_t i = 0; i < n * n; ++i) V[i] = 0.0;
+      for (int64_t i = 0; i < n; ++i) V[i * n + i] = 1.0;
+    }
+    return;
+  }
   // Q = H_0 H_1 ... H_{n-3}, accumulated from the right end: before step k
   // only the block k+1.. of Q differs from the identity.
   std::vector<double> q(vectors ? (size_t)n * n : 0, 0.0), wv(n), zt(vectors ? (size_t)n * n : 0);
@@ -374,31 +394,81 @@ void sym_eig(int64_t n, const double *A, double *w, double *V) {
   }
 }
 
+// Smallest eigenvalue of a symmetric matrix: the tridiagonal form, then
+// bisection on Sturm counts (the number of negative pivots of T - x I is the
+// number of eigenvalues below x) inside the Gershgorin interval, to a
+// relative width of 1e-12 of the spectral scale.
+double sym_eig_min(int64_t n, const double *A) {
+  if (n == 0) return 0.0;
+  std::vector<double> d, e, vs, betas;
+  const double scale = tridiagonalize(n, A, d, e, vs, betas);
+  if (!(scale > 0.0)) return 0.0;
+  double lo = INFINITY, hi = -INFINITY;
+  for (int64_t i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  const double tiny = 1e-300;
+  auto below = [&](double x) {  // eigenvalues of T below x
+    int64_t cnt = 0;
+    double q = d[0] - x;
+    for (int64_t i = 0;; ++i) {
+      if (q == 0.0) q = -tiny;
+      cnt += q < 0.0;
+      if (i + 1 == n) break;
+      q = d[i + 1] - x - e[i] * e[i] / q;
+    }
+    return cnt;
+  };
+  // 1e-12 of the spectral scale: far below what the caller needs, about 40 halvings
+  const double width = 1e-12 * std::max(std::fabs(lo), std::fabs(hi));
+  for (int it = 0; it < 200 && hi - lo > width; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (below(mid) >= 1)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi) * scale;
+}
+
+// Upper Cholesky factor R (A = R^T R) by rows: row j of R from the updated
+// row j of A, then the upper triangle of the trailing rows updated by an
+// outer product, so every inner loop runs along contiguous rows.
 bool cholesky_upper(int64_t l, const double *A, double *R) {
+  std::vector<double> a(A, A + l * l);
   for (int64_t i = 0; i < l * l; ++i) R[i] = 0.0;
   for (int64_t j = 0; j < l; ++j) {
-    double s = A[j * l + j];
-    for (int64_t k = 0; k < j; ++k) s -= R[k * l + j] * R[k * l + j];
+    const double s = a[j * l + j];
     if (!(s > 0.0)) return false;
-    const double rjj = std::sqrt(s);
-    R[j * l + j] = rjj;
+    const double rjj = std::sqrt(s), inv = 1.0 / rjj;
+    double *rj = R + j * l;
+    const double *aj = a.data() + j * l;
+    rj[j] = rjj;
+    for (int64_t i = j + 1; i < l; ++i) rj[i] = aj[i] * inv;
     for (int64_t i = j + 1; i < l; ++i) {
-      double t = A[j * l + i];
-      for (int64_t k = 0; k < j; ++k) t -= R[k * l + j] * R[k * l + i];
-      R[j * l + i] = t / rjj;
+      const double f = rj[i];
+      double *ai = a.data() + i * l;
+      for (int64_t c = i; c < l; ++c) ai[c] -= f * rj[c];
     }
   }
   return true;
 }
 
+// Inverse of an upper-triangular R by rows from the bottom:
+// X[i] = (e_i - sum_{k > i} R[i][k] X[k]) / R[i][i].
 void upper_inverse(int64_t l, const double *R, double *Ri) {
   for (int64_t i = 0; i < l * l; ++i) Ri[i] = 0.0;
-  for (int64_t j = l - 1; j >= 0; --j) {
-    Ri[j * l + j] = 1.0 / R[j * l + j];
-    for (int64_t i = j - 1; i >= 0; --i) {
-      double s = 0.0;
-      for (int64_t k = i + 1; k <= j; ++k) s += R[i * l + k] * Ri[k * l + j];
-      Ri[i * l + j] = -s / R[i * l + i];
+  for (int64_t i = l - 1; i >= 0; --i) {
+    const double inv = 1.0 / R[i * l + i];
+    double *xi = Ri + i * l;
+    xi[i] = inv;
+    for (int64_t k = i + 1; k < l; ++k) {
+      const double f = -R[i * l + k] * inv;
+      if (f == 0.0) continue;
+      const double *xk = Ri + k * l;
+      for (int64_t c = k; c < l; ++c) xi[c] += f * xk[c];
     }
   }
 }

@@ -69,12 +69,20 @@ struct Panel {
   void *p() const { return ptr; }
 };
 
-// Eigenvalues of a symmetric matrix, descending.
-void desc_eigvals(int64_t w, const std::vector<double> &G, std::vector<double> &lam) {
-  std::vector<double> la(w);
-  sym_eig(w, G.data(), la.data(), nullptr);
-  lam.assign(w, 0.0);
-  for (int64_t r = 0; r < w; ++r) lam[r] = la[w - 1 - r];
+// C (rows x cols) = A (rows x mid) * B[:, :cols] for row-major A and B (B has
+// leading dimension ldb): row axpys, contiguous inner loops.
+std::vector<double> mat_mul(const double *A, int64_t rows, int64_t mid, const double *B, int64_t ldb, int64_t cols) {
+  std::vector<double> C(rows * cols, 0.0);
+  for (int64_t i = 0; i < rows; ++i) {
+    double *ci = C.data() + i * cols;
+    for (int64_t j = 0; j < mid; ++j) {
+      const double a = A[i * mid + j];
+      if (a == 0.0) continue;
+      const double *bj = B + j * ldb;
+      for (int64_t q = 0; q < cols; ++q) ci[q] += a * bj[q];
+    }
+  }
+  return C;
 }
 
 // Eigen-decomposition of a symmetric matrix sorted by descending eigenvalue.
@@ -306,18 +314,6 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       Ys.init(ctx, WS_YS, m, l, pdt);
       Z2.init(ctx, WS_Z2, n, l, pdt);
     }
-    // A B for row-major A (rows x mid) and B (mid x cols)
-    auto small_mm = [](const std::vector<double> &A, int64_t rows, int64_t mid, const std::vector<double> &B,
-                       int64_t cols) {
-      std::vector<double> C(rows * cols, 0.0);
-      for (int64_t i = 0; i < rows; ++i)
-        for (int64_t j = 0; j < mid; ++j) {
-          const double a = A[i * mid + j];
-          if (a != 0.0)
-            for (int64_t q = 0; q < cols; ++q) C[i * cols + q] += a * B[j * cols + q];
-        }
-      return C;
-    };
     std::vector<double> PM;  // M_{t-1} M_t when prepared during a check
     bool pm_ready = false;
     std::vector<double> Mq_spec;  // normaliser of the iterate built at a check
@@ -331,7 +327,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       if (!(adaptive && momentum && have_prev))
         PM.clear();
       else if (!pm_ready)
-        PM = small_mm(Mp, Yp.w, Y.w, Mq, wq);  // M_{t-1} M_t
+        PM = mat_mul(Mp.data(), Yp.w, Y.w, Mq.data(), wq, wq);  // M_{t-1} M_t
       pm_ready = false;
       const double *dMq = nullptr, *dPM = nullptr;
       if (mfma) {
@@ -361,8 +357,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         // The shift for the next iterate, from the Ritz values (if this is a
         // check that converges, the shift is unused).
         R.gram_wait(wq, GZ);
-        desc_eigvals(wq, GZ, lam);
-        shift = shift_frac * std::max(lam[wq - 1], 0.0);
+        shift = shift_frac * std::max(sym_eig_min(wq, GZ.data()), 0.0);
       }
       // Next iterate: Y_{t+1} = A Z - c Q - (c^2/4) Q_{t-1} R_t^-1
       //                      = A Z - c Y Mq - (c^2/4) Y_{t-1} (M_{t-1} M_t),
@@ -406,17 +401,14 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         desc_eig(wq, GZ, lam, W);
         const int64_t k = std::min(target_rank, std::min(wq, n));
         // r_i = (A Z) w_i / s_i - Y (Mq w_i s_i)
-        std::vector<double> M1(wq * k, 0.0), M2(Y.w * k, 0.0), nr2(k);
+        std::vector<double> M1(wq * k, 0.0), nr2(k);
+        std::vector<double> M2 = mat_mul(Mq.data(), Y.w, wq, W.data(), wq, k);  // Mq W, then * -s
         bool ok = true;
         for (int64_t r = 0; r < k; ++r) {
           const double sr = std::sqrt(std::max(lam[r], 0.0));
           if (!(sr > 0.0)) ok = false;
           for (int64_t i = 0; i < wq; ++i) M1[i * k + r] = sr > 0.0 ? W[i * wq + r] / sr : 0.0;
-          for (int64_t i = 0; i < Y.w; ++i) {
-            double t = 0.0;
-            for (int64_t j = 0; j < wq; ++j) t += Mq[i * wq + j] * W[j * wq + r];
-            M2[i * k + r] = -t * sr;
-          }
+          for (int64_t i = 0; i < Y.w; ++i) M2[i * k + r] *= -sr;
         }
         panel_residual(ctx, m, Y.w, wq, k, Y.p(), Yn.p(), Y.ld, pdt, M2.data(), M1.data(), nullptr);
         // While the residual runs: the normaliser of the next iterate and its
@@ -426,7 +418,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         const int64_t fb = R.fallbacks;
         R.finish_normaliser(*next, Mq_next, wq_next, kGramOff2, ctx->small_ev2);
         std::vector<double> PM_next;
-        if (momentum && shift > 0.0) PM_next = small_mm(Mq, Y.w, wq, Mq_next, wq_next);
+        if (momentum && shift > 0.0) PM_next = mat_mul(Mq.data(), Y.w, wq, Mq_next.data(), wq_next, wq_next);
         panel_residual_wait(ctx, k, nr2.data());
         est = 0.0;
         for (int64_t r = 0; r < k; ++r) est = std::max(est, std::sqrt(nr2[r]) / std::sqrt(std::max(lam[r], 0.0)));
@@ -528,16 +520,12 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
 
   // B^T = Z (n x wq) = A^T Q; SVD(B) = W S X^T with X = Z W S^-1; U = Q W = Y (Mq W).
   const int64_t k = std::min(target_rank, std::min(wq, n));
-  std::vector<double> sv(k), MW(Y.w * k, 0.0), Ws(wq * k);
+  std::vector<double> sv(k), Ws(wq * k);
+  const std::vector<double> MW = mat_mul(Mq.data(), Y.w, wq, W.data(), wq, k);
   for (int64_t r = 0; r < k; ++r) {
     sv[r] = std::sqrt(std::max(lam[r], 0.0));
     const double inv = sv[r] > 0.0 ? 1.0 / sv[r] : 0.0;
     for (int64_t i = 0; i < wq; ++i) Ws[i * k + r] = W[i * wq + r] * inv;
-    for (int64_t i = 0; i < Y.w; ++i) {
-      double t = 0.0;
-      for (int64_t j = 0; j < wq; ++j) t += Mq[i * wq + j] * W[j * wq + r];
-      MW[i * k + r] = t;
-    }
   }
   // U = Y (Mq W), row-major m x k; V^T = (Z W S^-1)^T, row-major k x n, written
   // by the transposed store of the apply kernel.
@@ -709,15 +697,11 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
     desc_eig(w, Gb, lam, W);
     const int64_t k = std::min(target_rank, std::min(w, n));
     std::vector<double> sv(k), MW(l * k, 0.0), Ws(w * k);
+    MW = mat_mul(Mq[b].data(), l, w, W.data(), w, k);
     for (int64_t r = 0; r < k; ++r) {
       sv[r] = std::sqrt(std::max(lam[r], 0.0));
       const double inv = sv[r] > 0.0 ? 1.0 / sv[r] : 0.0;
       for (int64_t i = 0; i < w; ++i) Ws[i * k + r] = W[i * w + r] * inv;
-      for (int64_t i = 0; i < l; ++i) {
-        double t = 0.0;
-        for (int64_t j = 0; j < w; ++j) t += Mq[b][i * w + j] * W[j * w + r];
-        MW[i * k + r] = t;
-      }
     }
     svdh_result *out = &outs[b];
     out->d = k;
