@@ -329,8 +329,65 @@ def bench_adjoint(args):
             "max_rel_error_vs_fp64_on_1000_nonzeros": errs}
 
 
+def bench_ingest(args):
+    """COO input (svdh_coo_create): the 100,000 x 20,000 Zipf matrix as FP32
+    triplets in a scrambled order with 1 % of the entries repeated (split into
+    two halves), converted to the device CSR operator. The device conversion
+    (radix sort by (row, column), run sums, row pointers) is timed with HIP
+    events; the whole call also includes the uploads, the CSR download and
+    the operator construction. CPU baseline: the oracle's COO -> CSR (C, one
+    thread) on the same triplets."""
+    import oracle
+    import single_svdlib as sl
+    from single_svdlib import _ffi, synth
+    a = synth.config2(np.float32).tocoo()
+    rng = np.random.default_rng(20261208)
+    rep = rng.choice(a.nnz, a.nnz // 100, replace=False)
+    half = (a.data[rep] * np.float32(0.5)).astype(np.float32)
+    rows = np.concatenate([a.row, a.row[rep]]).astype(np.int64)
+    cols = np.concatenate([a.col, a.col[rep]]).astype(np.int64)
+    vals = np.concatenate([a.data, half]).astype(np.float32)
+    vals[rep] = (a.data[rep] - half).astype(np.float32)
+    perm = rng.permutation(rows.size)
+    rows, cols, vals = rows[perm], cols[perm], vals[perm]
+    m, n = a.shape
+    lib = _ffi.lib()
+    ctx = _ffi.context(0)
+    for _ in range(args.warmup):
+        sl.CooMatrix(rows, cols, vals, (m, n))
+    _ffi.check(lib.svdh_ctx_synchronize(ctx))
+    _ffi.check(lib.svdh_ctx_profile(ctx, 1))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        op = sl.CooMatrix(rows, cols, vals, (m, n))
+    _ffi.check(lib.svdh_ctx_synchronize(ctx))
+    total = (time.perf_counter() - t0) / args.steps
+    st = _kernel_stats(lib, ctx, (("ingest", _ffi.KC_INGEST),), args.steps)
+    _ffi.check(lib.svdh_ctx_profile(ctx, 0))
+    dev_s = st["ingest"]["ms_per_step"] * 1e-3
+    u = op.nnz()
+    algo = rows.size * (8 + 8 + 4) + (m + 1) * 4 + u * (4 + 4)
+    roof = _roofline(algo * args.steps, st["ingest"]["_ms"], args.steps)
+    t0 = time.perf_counter()
+    rp, oc, ov = oracle.coo_to_csr(m, n, rows, cols, vals)
+    cpu_s = time.perf_counter() - t0
+    ok = ov.size == u
+    log(f"COO ingest: device conversion {dev_s * 1e3:.2f} ms, whole call {total * 1e3:.1f} ms, "
+        f"{rows.size} triplets -> {u} entries (oracle {ov.size}); CPU oracle {cpu_s * 1e3:.0f} ms")
+    return {"metric": "COO ingestion seconds (device conversion to CSR)", "value": dev_s, "unit": "s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": dev_s * 1e3, "higher_is_better": False,
+            "scaling": "strong", "vs_baseline": None, "dtype": "int64 indices, f32 values", "data": "synthetic",
+            "config": {"workload": ("svdh_coo_create of the 100,000 x 20,000 Zipf matrix as scrambled FP32 triplets "
+                                    "with 1 % repeated pairs"), "triplets": int(rows.size), "entries": int(u),
+                       "entries_match_oracle": bool(ok)},
+            "roofline": roof,
+            "cpu_baseline": {"value": cpu_s, "unit": "s", "cores": 1, "kind": "port",
+                             "sample": "the oracle's COO -> CSR (two counting sorts, run sums) on the same triplets"},
+            "whole_call_s": total}
+
+
 OTHER_WORKLOADS = {"las2-cfg1": bench_las2, "las2-cfg3": bench_las2, "batched": bench_batched,
-                   "adjoint": bench_adjoint}
+                   "adjoint": bench_adjoint, "coo-ingest": bench_ingest}
 
 
 def main():
@@ -344,7 +401,8 @@ def main():
                     help="for rocprofv3 runs: the timed solves plus one 256 MiB device-to-device copy (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE), without the correctness and reference extras")
     ap.add_argument("--max-iters", type=int, default=400)
-    ap.add_argument("--workload", default="rsvd", choices=["rsvd", "las2-cfg1", "las2-cfg3", "batched", "adjoint"],
+    ap.add_argument("--workload", default="rsvd",
+                    choices=["rsvd", "las2-cfg1", "las2-cfg3", "batched", "adjoint", "coo-ingest"],
                     help="rsvd (default, the headline metric) or one of the other measured paths")
     args = ap.parse_args()
     if args.gpus != 1:

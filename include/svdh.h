@@ -109,7 +109,9 @@ int32_t svdh_ctx_synchronize(svdh_ctx *ctx);
 const char *svdh_last_error(void);
 /* Kernel timing with HIP events on the context's stream. Kernel classes:
  * 0 = SpMV, 1 = SpMM Y = A X, 2 = dense (Gram / apply / GEMM), 3 = BLAS-1,
- * 4 = basis GEMV (reorthogonalisation), 5 = SpMM Z = A^T Y, 6 = SDDMM. total_ms,
+ * 4 = basis GEMV (reorthogonalisation), 5 = SpMM Z = A^T Y, 6 = SDDMM,
+ * 7 = COO / CSC ingestion (the device conversion, between the uploads of the
+ * input arrays and the download of the CSR arrays). total_ms,
  * launches and the launches' algorithmic bytes accumulate until
  * svdh_ctx_profile is called again with enable = 1. */
 int32_t svdh_ctx_profile(svdh_ctx *ctx, int32_t enable);
@@ -126,6 +128,26 @@ int32_t svdh_ctx_kernel_time(svdh_ctx *ctx, int32_t kernel_class, double *total_
 int32_t svdh_csr_create(svdh_ctx *ctx, int64_t nrows, int64_t ncols, int64_t nnz,
                         const int64_t *rowptr, const int64_t *colidx, const void *vals,
                         int32_t val_dtype, uint32_t flags, svdh_op **out);
+
+/* A COO matrix (nalgebra_sparse::CooMatrix: row_indices, col_indices,
+ * values of val_dtype F64 or F32; the operand of SMat for CooMatrix,
+ * src/lanczos/mod.rs:1657-1740). Repeated (row, column) pairs are allowed:
+ * the reference's products add every triplet, so they are summed here, left
+ * to right in input order in the value type. The conversion runs on the
+ * device (stable radix sort by (row, column), run sums, row pointers), then
+ * the operator is built as by svdh_csr_create. Indices out of range or
+ * nnz >= 2^31 are SVDH_ERR_ARG. */
+int32_t svdh_coo_create(svdh_ctx *ctx, int64_t nrows, int64_t ncols, int64_t nnz, const int64_t *rows,
+                        const int64_t *cols, const void *vals, int32_t val_dtype, uint32_t flags,
+                        svdh_op **out);
+
+/* A CSC matrix (nalgebra_sparse::CscMatrix csc_data(): colptr[ncols+1],
+ * row indices[nnz], values of val_dtype F64 or F32; the operand of SMat for
+ * CscMatrix, src/lanczos/mod.rs:1415-1504). Converted on the device as
+ * svdh_coo_create does (repeated pairs summed). */
+int32_t svdh_csc_create(svdh_ctx *ctx, int64_t nrows, int64_t ncols, int64_t nnz, const int64_t *colptr,
+                        const int64_t *rowidx, const void *vals, int32_t val_dtype, uint32_t flags,
+                        svdh_op **out);
 
 /* MaskedCSRMatrix::new(matrix, column_mask) (src/lanczos/masked.rs:21-46).
  * mask has ncols(base) bytes, nonzero = column kept. The view borrows base:

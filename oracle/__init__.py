@@ -22,6 +22,10 @@ vector, a closed form or a library routine chosen so a plausible mistake fails):
               angles and singular values.
   adjoint (oracle/adjoint.py, no reference counterpart) pinned: central
               finite differences of the loss through numpy.linalg.svd.
+  coo_to_csr  pinned: scipy.sparse COO -> CSR (structure exact; sums of
+              duplicates exact on integer-valued inputs, 1 ulp otherwise),
+              the triplet-sum product of SMat for CooMatrix (mod.rs:1657-1740),
+              and a sequential left-to-right sum of each duplicate run.
 """
 from .pyoracle import (  # noqa: F401
     OracleOp,
@@ -31,6 +35,7 @@ from .pyoracle import (  # noqa: F401
     pythag,
     start_vector,
     determine_chunk_size,
+    coo_to_csr,
     F64_EPS,
     F32_EPS,
 )

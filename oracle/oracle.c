@@ -1041,3 +1041,55 @@ int orc_las2(const orc_op *op, int64_t dimensions, int64_t iterations, double en
   ws_free(&w);
   return st;
 }
+
+/* ---- COO -> CSR ---------------------------------------------------------
+ * Two stable counting sorts (by column, then by row) give the (row, column)
+ * order with equal pairs in input order; each run of equal pairs is then
+ * summed left to right. */
+static int64_t coo_order(int64_t m, int64_t n, int64_t nnz, const int64_t *rows, const int64_t *cols,
+                         int64_t *ord) {
+  for (int64_t k = 0; k < nnz; ++k)
+    if (rows[k] < 0 || rows[k] >= m || cols[k] < 0 || cols[k] >= n) return -1;
+  int64_t *tmp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nnz > 0 ? nnz : 1));
+  int64_t *cnt = (int64_t *)calloc((size_t)((m > n ? m : n) + 1), sizeof(int64_t));
+  for (int64_t k = 0; k < nnz; ++k) ++cnt[cols[k] + 1];
+  for (int64_t j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
+  for (int64_t k = 0; k < nnz; ++k) tmp[cnt[cols[k]]++] = k;
+  memset(cnt, 0, sizeof(int64_t) * (size_t)((m > n ? m : n) + 1));
+  for (int64_t k = 0; k < nnz; ++k) ++cnt[rows[k] + 1];
+  for (int64_t i = 0; i < m; ++i) cnt[i + 1] += cnt[i];
+  for (int64_t t = 0; t < nnz; ++t) {
+    const int64_t k = tmp[t];
+    ord[cnt[rows[k]]++] = k;
+  }
+  free(tmp);
+  free(cnt);
+  return 0;
+}
+
+#define ORC_COO_TO_CSR(NAME, T)                                                                      \
+  int64_t NAME(int64_t m, int64_t n, int64_t nnz, const int64_t *rows, const int64_t *cols,         \
+               const T *vals, int64_t *rowptr, int64_t *out_cols, T *out_vals) {                    \
+    int64_t *ord = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nnz > 0 ? nnz : 1));               \
+    if (coo_order(m, n, nnz, rows, cols, ord)) {                                                     \
+      free(ord);                                                                                     \
+      return -1;                                                                                     \
+    }                                                                                                \
+    int64_t u = 0, r = 0;                                                                            \
+    rowptr[0] = 0;                                                                                   \
+    for (int64_t t = 0; t < nnz;) {                                                                  \
+      const int64_t k = ord[t];                                                                      \
+      T s = vals[k];                                                                                 \
+      int64_t e = t + 1;                                                                             \
+      while (e < nnz && rows[ord[e]] == rows[k] && cols[ord[e]] == cols[k]) s += vals[ord[e++]];     \
+      while (r < rows[k]) rowptr[++r] = u;                                                           \
+      out_cols[u] = cols[k];                                                                         \
+      out_vals[u++] = s;                                                                             \
+      t = e;                                                                                         \
+    }                                                                                                \
+    while (r < m) rowptr[++r] = u;                                                                   \
+    free(ord);                                                                                       \
+    return u;                                                                                        \
+  }
+ORC_COO_TO_CSR(orc_coo_to_csr, double)
+ORC_COO_TO_CSR(orc_coo_to_csr_f32, float)

@@ -119,6 +119,17 @@ int orc_las2(const orc_op *op, int64_t dimensions, int64_t iterations, double en
 void orc_svdrec_free(orc_svdrec *rec);
 const char *orc_last_error(void);
 
+/* COO -> CSR (the input formats of SMat for CooMatrix, mod.rs:1657-1740,
+ * whose products add every triplet, duplicates included): entries ordered
+ * by (row, column), duplicates summed in input order in the value type
+ * (FP64 in orc_coo_to_csr, FP32 in orc_coo_to_csr_f32). Outputs sized by the
+ * caller: rowptr[m + 1], cols[nnz], vals[nnz]; returns the number of
+ * distinct entries, or -1 on an index out of range. */
+int64_t orc_coo_to_csr(int64_t m, int64_t n, int64_t nnz, const int64_t *rows, const int64_t *cols,
+                       const double *vals, int64_t *rowptr, int64_t *out_cols, double *out_vals);
+int64_t orc_coo_to_csr_f32(int64_t m, int64_t n, int64_t nnz, const int64_t *rows, const int64_t *cols,
+                           const float *vals, int64_t *rowptr, int64_t *out_cols, float *out_vals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,12 @@ _lib.orc_svdrec_free.argtypes = [C.POINTER(_Rec)]
 _lib.orc_last_error.restype = C.c_char_p
 _lib.orc_determine_chunk_size.argtypes = [C.c_int64, C.c_int]
 _lib.orc_determine_chunk_size.restype = C.c_int
+_i64p = C.POINTER(C.c_int64)
+_f32p = C.POINTER(C.c_float)
+_lib.orc_coo_to_csr.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i64p, _i64p, _f64p, _i64p, _i64p, _f64p]
+_lib.orc_coo_to_csr.restype = C.c_int64
+_lib.orc_coo_to_csr_f32.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i64p, _i64p, _f32p, _i64p, _i64p, _f32p]
+_lib.orc_coo_to_csr_f32.restype = C.c_int64
 
 
 def _p(a, t=_f64p):
@@ -220,3 +226,27 @@ def las2(op, dimensions=0, iterations=0, end_interval=(-1e-30, 1e-30), kappa=1e-
     out["diagnostics"]["end_interval"] = list(rec.diag.end_interval)
     _lib.orc_svdrec_free(C.byref(rec))
     return 0, out
+
+
+def coo_to_csr(m, n, rows, cols, vals):
+    """COO triplets -> (rowptr, cols, vals) of the CSR with entries in (row,
+    column) order and duplicates summed in input order, in the dtype of vals
+    (float64 or float32). Raises ValueError on an index out of range."""
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    vals = np.ascontiguousarray(vals)
+    nnz = rows.size
+    rp = np.zeros(m + 1, dtype=np.int64)
+    oc = np.zeros(max(nnz, 1), dtype=np.int64)
+    if vals.dtype == np.float32:
+        ov = np.zeros(max(nnz, 1), dtype=np.float32)
+        u = _lib.orc_coo_to_csr_f32(m, n, nnz, _p(rows, _i64p), _p(cols, _i64p), _p(vals, _f32p), _p(rp, _i64p),
+                                    _p(oc, _i64p), _p(ov, _f32p))
+    else:
+        vals = vals.astype(np.float64, copy=False)
+        ov = np.zeros(max(nnz, 1), dtype=np.float64)
+        u = _lib.orc_coo_to_csr(m, n, nnz, _p(rows, _i64p), _p(cols, _i64p), _p(vals), _p(rp, _i64p),
+                                _p(oc, _i64p), _p(ov))
+    if u < 0:
+        raise ValueError("COO index out of range")
+    return rp, oc[:u].copy(), ov[:u].copy()

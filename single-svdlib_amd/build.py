@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsvdh.so")
-SOURCES = ["api.cpp", "hostla.cpp", "las2.cpp", "rsvd.cpp", "adjoint.cpp", "sparse.hip", "dense.hip"]
+SOURCES = ["api.cpp", "hostla.cpp", "las2.cpp", "rsvd.cpp", "adjoint.cpp", "sparse.hip", "dense.hip", "ingest.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = [

@@ -99,6 +99,10 @@ KTimer::~KTimer() {
 svdh_op *csr_create(Ctx *c, int64_t nrows, int64_t ncols, int64_t nnz, const int64_t *rowptr, const int64_t *colidx,
                     const void *vals, int32_t vdt, uint32_t flags);
 svdh_op *mask_create(Ctx *c, const svdh_op *base, const uint8_t *mask);
+svdh_op *coo_create(Ctx *c, int64_t m, int64_t n, int64_t nnz, const int64_t *rows, const int64_t *cols,
+                    const void *vals, int32_t vdt, uint32_t flags);
+svdh_op *csc_create(Ctx *c, int64_t m, int64_t n, int64_t nnz, const int64_t *colptr, const int64_t *rows,
+                    const void *vals, int32_t vdt, uint32_t flags);
 int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
                    double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out);
 int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_oversamples, int64_t n_power_iters,
@@ -263,6 +267,24 @@ int32_t svdh_csr_create(svdh_ctx *ctx, int64_t nrows, int64_t ncols, int64_t nnz
     check_ctx(ctx);
     if (!out) fail(SVDH_ERR_ARG, "null output pointer");
     *out = csr_create(ctx, nrows, ncols, nnz, rowptr, colidx, vals, val_dtype, flags);
+  });
+}
+
+int32_t svdh_coo_create(svdh_ctx *ctx, int64_t nrows, int64_t ncols, int64_t nnz, const int64_t *rows,
+                        const int64_t *cols, const void *vals, int32_t val_dtype, uint32_t flags, svdh_op **out) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!out) fail(SVDH_ERR_ARG, "null output pointer");
+    *out = coo_create(ctx, nrows, ncols, nnz, rows, cols, vals, val_dtype, flags);
+  });
+}
+
+int32_t svdh_csc_create(svdh_ctx *ctx, int64_t nrows, int64_t ncols, int64_t nnz, const int64_t *colptr,
+                        const int64_t *rowidx, const void *vals, int32_t val_dtype, uint32_t flags, svdh_op **out) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!out) fail(SVDH_ERR_ARG, "null output pointer");
+    *out = csc_create(ctx, nrows, ncols, nnz, colptr, rowidx, vals, val_dtype, flags);
   });
 }
 

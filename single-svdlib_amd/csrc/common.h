@@ -52,7 +52,8 @@ struct DevBuf {
 
 // Kernel classes for event timing (svdh_ctx_kernel_time).
 enum KernelClass {
-  KC_SPMV = 0, KC_SPMM = 1, KC_DENSE = 2, KC_BLAS1 = 3, KC_GEMV = 4, KC_SPMM_T = 5, KC_SDDMM = 6, KC_COUNT = 7
+  KC_SPMV = 0, KC_SPMM = 1, KC_DENSE = 2, KC_BLAS1 = 3, KC_GEMV = 4, KC_SPMM_T = 5, KC_SDDMM = 6, KC_INGEST = 7,
+  KC_COUNT = 8
 };
 
 struct Ctx {

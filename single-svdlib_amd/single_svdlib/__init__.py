@@ -7,6 +7,15 @@ single-svdlib v1.0.9 (module layout of src/lib.rs:1-9).
     rec = randomized.randomized_svd(a, 50, 10, 3, randomized.PowerIterationNormalizer.QR, False, 42, False)
 """
 from . import adjoint, error, lanczos, randomized  # noqa: F401
-from .utils import CsrMatrix, Diagnostics, SMat, SvdFloat, SvdRec, determine_chunk_size  # noqa: F401
+from .utils import (  # noqa: F401
+    CooMatrix,
+    CscMatrix,
+    CsrMatrix,
+    Diagnostics,
+    SMat,
+    SvdFloat,
+    SvdRec,
+    determine_chunk_size,
+)
 
 laczos = lanczos  # README.md:39 spells the module `laczos`

@@ -19,7 +19,7 @@ SVDH_F64, SVDH_F32, SVDH_BF16 = 0, 1, 2
 PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
 NORM_QR, NORM_LU, NORM_NONE = 0, 1, 2
 KEEP_F32, KEEP_BF16 = 0x1, 0x2
-KC_SPMV, KC_SPMM, KC_DENSE, KC_BLAS1, KC_GEMV, KC_SPMM_T, KC_SDDMM = 0, 1, 2, 3, 4, 5, 6
+KC_SPMV, KC_SPMM, KC_DENSE, KC_BLAS1, KC_GEMV, KC_SPMM_T, KC_SDDMM, KC_INGEST = 0, 1, 2, 3, 4, 5, 6, 7
 
 
 class Diagnostics_(C.Structure):
@@ -81,6 +81,8 @@ SIGNATURES = [
     ("svdh_ctx_kernel_time", _i32, [_vp, _i32, _f64p, _i64p, _f64p]),
     ("svdh_csr_create", _i32, [_vp, _i64, _i64, _i64, _i64p, _i64p, _vp, _i32, C.c_uint32, C.POINTER(_vp)]),
     ("svdh_mask_create", _i32, [_vp, _vp, C.POINTER(C.c_uint8), C.POINTER(_vp)]),
+    ("svdh_coo_create", _i32, [_vp, _i64, _i64, _i64, _i64p, _i64p, _vp, _i32, C.c_uint32, C.POINTER(_vp)]),
+    ("svdh_csc_create", _i32, [_vp, _i64, _i64, _i64, _i64p, _i64p, _vp, _i32, C.c_uint32, C.POINTER(_vp)]),
     ("svdh_op_destroy", _i32, [_vp]),
     ("svdh_op_shape", _i32, [_vp, _i64p, _i64p, _i64p]),
     ("svdh_spmv", _i32, [_vp, _vp, _i32, _vp, _vp, _i32, _i32]),

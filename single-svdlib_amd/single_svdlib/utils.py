@@ -240,6 +240,87 @@ class CsrMatrix(_DeviceOp):
             _ffi.lib().svdh_op_destroy(h)
 
 
+def _value_code(data):
+    data = np.asarray(data)
+    if data.dtype == np.float64:
+        return _ffi.SVDH_F64, np.ascontiguousarray(data)
+    if data.dtype == np.float32:
+        return _ffi.SVDH_F32, np.ascontiguousarray(data)
+    raise ValueError(f"values must be float64 or float32, got {data.dtype}")
+
+
+class CooMatrix(CsrMatrix):
+    """A COO matrix (nalgebra_sparse::CooMatrix, the operand of SMat for
+    CooMatrix, src/lanczos/mod.rs:1657-1740) converted to the device CSR
+    operator on the GPU. Repeated (row, column) pairs are summed in input
+    order (the reference's products add every triplet).
+
+    CooMatrix(rows, cols, data, shape, device=0, keep_f32=False, keep_bf16=False)
+    """
+
+    def __init__(self, rows, cols, data, shape, device=0, keep_f32=False, keep_bf16=False):
+        self.device = device
+        self._ctx = _ffi.context(device)
+        code, dv = _value_code(data)
+        self.dtype = dv.dtype.type
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        c = np.ascontiguousarray(cols, dtype=np.int64)
+        if r.size != dv.size or c.size != dv.size:
+            raise ValueError("rows, cols and data must have one entry per triplet")
+        flags = (_ffi.KEEP_F32 if keep_f32 else 0) | (_ffi.KEEP_BF16 if keep_bf16 else 0)
+        h = C.c_void_p()
+        _ffi.check(_ffi.lib().svdh_coo_create(self._ctx, int(shape[0]), int(shape[1]), int(dv.size),
+                                              r.ctypes.data_as(_ffi._i64p), c.ctypes.data_as(_ffi._i64p),
+                                              dv.ctypes.data if dv.size else None, code, flags, C.byref(h)))
+        self._handle = h
+
+    @classmethod
+    def from_scipy(cls, matrix, dtype=None, device=0, **kw):
+        """From any scipy.sparse matrix, its COO triplets as stored (a coo_matrix
+        keeps repeated pairs)."""
+        import scipy.sparse as sp
+        a = matrix if getattr(matrix, "format", None) == "coo" else sp.coo_matrix(matrix)
+        data = a.data.astype(dtype) if dtype is not None else a.data
+        if data.dtype not in (np.float64, np.float32):
+            data = data.astype(np.float64)
+        return cls(a.row, a.col, data, a.shape, device=device, **kw)
+
+
+class CscMatrix(CsrMatrix):
+    """A CSC matrix (nalgebra_sparse::CscMatrix csc_data(), the operand of SMat
+    for CscMatrix, src/lanczos/mod.rs:1415-1504) converted to the device CSR
+    operator on the GPU.
+
+    CscMatrix(indptr, indices, data, shape, device=0, keep_f32=False, keep_bf16=False)
+    """
+
+    def __init__(self, indptr, indices, data, shape, device=0, keep_f32=False, keep_bf16=False):
+        self.device = device
+        self._ctx = _ffi.context(device)
+        code, dv = _value_code(data)
+        self.dtype = dv.dtype.type
+        cp = np.ascontiguousarray(indptr, dtype=np.int64)
+        ri = np.ascontiguousarray(indices, dtype=np.int64)
+        nrows, ncols = int(shape[0]), int(shape[1])
+        if cp.size != ncols + 1:
+            raise ValueError("indptr must have ncols + 1 entries")
+        flags = (_ffi.KEEP_F32 if keep_f32 else 0) | (_ffi.KEEP_BF16 if keep_bf16 else 0)
+        h = C.c_void_p()
+        _ffi.check(_ffi.lib().svdh_csc_create(self._ctx, nrows, ncols, int(dv.size), cp.ctypes.data_as(_ffi._i64p),
+                                              ri.ctypes.data_as(_ffi._i64p), dv.ctypes.data if dv.size else None,
+                                              code, flags, C.byref(h)))
+        self._handle = h
+
+    @classmethod
+    def from_scipy(cls, matrix, dtype=None, device=0, **kw):
+        import scipy.sparse as sp
+        a = sp.csc_matrix(matrix)
+        data = a.data.astype(dtype) if dtype is not None else a.data
+        if data.dtype not in (np.float64, np.float32):
+            data = data.astype(np.float64)
+        return cls(a.indptr, a.indices, data, a.shape, device=device, **kw)
+
+
 def as_operator(a):
     """Accept a device operator or any scipy.sparse matrix."""
     if isinstance(a, _DeviceOp):

@@ -418,3 +418,56 @@ def test_adjoint_oracle_planted_mistake_fails_fd():
               - oracle.adjoint.svd_loss(am.toarray(), "projection", k, y)) / (2 * h)
         worst = max(worst, abs(fd - gp[e]))
     assert worst > 1e-3
+
+
+def _coo_case(m, n, nnz, seed, integer=True, dtype=np.float64):
+    rng = np.random.default_rng(seed)
+    r = rng.integers(0, m, nnz)
+    c = rng.integers(0, n, nnz)
+    v = (rng.integers(-7, 8, nnz) if integer else rng.normal(size=nnz)).astype(dtype)
+    return r, c, v
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_coo_to_csr_oracle_matches_scipy(dtype):
+    # integer-valued entries: every order of summation gives the same sums, so
+    # structure and values must equal scipy's COO -> CSR exactly
+    for m, n, nnz, seed in ((50, 40, 600, 0), (1, 7, 30, 1), (300, 2, 900, 2), (20, 30, 0, 3)):
+        r, c, v = _coo_case(m, n, nnz, seed, dtype=dtype)
+        rp, oc, ov = oracle.coo_to_csr(m, n, r, c, v)
+        a = sp.coo_matrix((v, (r, c)), shape=(m, n)).tocsr()
+        a.sum_duplicates()
+        a.sort_indices()
+        assert ov.dtype == dtype
+        assert np.array_equal(rp, a.indptr) and np.array_equal(oc, a.indices) and np.array_equal(ov, a.data)
+        if nnz > 40:
+            assert oc.size < nnz  # the case holds duplicates, so dropping or keeping them both would fail
+
+
+def test_coo_to_csr_oracle_sums_duplicates_in_input_order():
+    # FP64 sums are not associative: 1e16 + 1 - 1e16 depends on the order. The
+    # oracle adds each run left to right in input order.
+    r = np.array([0, 1, 0, 0, 1])
+    c = np.array([2, 0, 2, 2, 0])
+    v = np.array([1e16, 3.0, 1.0, -1e16, 4.0])
+    rp, oc, ov = oracle.coo_to_csr(2, 3, r, c, v)
+    assert list(rp) == [0, 1, 2] and list(oc) == [2, 0]
+    assert ov[0] == (1e16 + 1.0) - 1e16 and ov[1] == 7.0
+
+
+def test_coo_to_csr_oracle_product_equals_triplet_sum():
+    # SMat for CooMatrix (mod.rs:1657-1740) adds every triplet into y
+    r, c, v = _coo_case(200, 150, 3000, 5, integer=False)
+    rp, oc, ov = oracle.coo_to_csr(200, 150, r, c, v)
+    x = np.random.default_rng(1).normal(size=150)
+    y = np.zeros(200)
+    np.add.at(y, r, v * x[c])
+    a = sp.csr_matrix((ov, oc, rp), shape=(200, 150))
+    assert np.allclose(a @ x, y, rtol=1e-12, atol=1e-12)
+
+
+def test_coo_to_csr_oracle_rejects_out_of_range():
+    with pytest.raises(ValueError):
+        oracle.coo_to_csr(3, 3, np.array([0, 3]), np.array([0, 0]), np.array([1.0, 2.0]))
+    with pytest.raises(ValueError):
+        oracle.coo_to_csr(3, 3, np.array([0, 1]), np.array([0, -1]), np.array([1.0, 2.0]))
