@@ -3,14 +3,13 @@
 // mod.rs:1415-1504 and 1657-1740, whose products add every stored entry,
 // repeated entries included).
 //
-// Entries get the key (row << 32 | column) and their input position; a
-// stable LSD radix sort over the key bits that can be nonzero (4-bit digits,
-// column bits first) orders them by (row, column) with repeated pairs in
-// input order; each run of equal keys is summed left to right in the value
+// Entries get the key (row << 32 | column); a stable LSD radix sort over the
+// key bits that can be nonzero (4-bit digits, column bits first) carries the
+// values along and orders the entries by (row, column) with repeated pairs
+// in input order; each run of equal keys is summed left to right in the value
 // type; row pointers come from the sorted rows. Everything here is integer
-// and byte traffic bound by HBM: per radix pass one read and one write of
-// the 12-byte (key, position) pairs, staged through LDS so global loads are
-// coalesced. The CSR arrays then take the ordinary operator construction
+// and byte traffic bound by HBM: per radix pass one read and one write of the
+// (key, value) pairs, staged through LDS so global loads are coalesced. The CSR arrays then take the ordinary operator construction
 // (indices narrowed to int32, the CSC copy).
 #include <algorithm>
 #include <cstring>
@@ -32,23 +31,22 @@ constexpr int kDigits = 16;
 
 int ing_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kIngBlock - 1) / kIngBlock, 65536)); }
 
-// key = row << 32 | column, pos = input position; bad = 1 on an index out of
-// range (checked again on the host before use).
+// key = row << 32 | column (the values travel with the keys through the
+// sort); bad = 1 on an index out of range (checked on the host before use).
 __global__ void k_coo_pack(int64_t nnz, const int64_t *rows, const int64_t *cols, int64_t m, int64_t n,
-                           uint64_t *key, uint32_t *pos, int32_t *bad) {
+                           uint64_t *key, int32_t *bad) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = rows[i], c = cols[i];
     const bool ok = r >= 0 && r < m && c >= 0 && c < n;
     if (!ok) bad[0] = 1;
     key[i] = ok ? ((uint64_t)r << 32 | (uint64_t)c) : 0;
-    pos[i] = (uint32_t)i;
   }
 }
 
 // CSC: the column of entry i is the last c with colptr[c] <= i (binary
 // search over the validated column pointers).
 __global__ void k_csc_pack(int64_t nnz, const int64_t *colptr, int64_t n, const int64_t *rows, int64_t m,
-                           uint64_t *key, uint32_t *pos, int32_t *bad) {
+                           uint64_t *key, int32_t *bad) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
     int64_t lo = 0, hi = n;  // colptr[lo] <= i < colptr[hi]
     while (hi - lo > 1) {
@@ -62,7 +60,6 @@ __global__ void k_csc_pack(int64_t nnz, const int64_t *colptr, int64_t n, const 
     const bool ok = r >= 0 && r < m;
     if (!ok) bad[0] = 1;
     key[i] = ok ? ((uint64_t)r << 32 | (uint64_t)lo) : 0;
-    pos[i] = (uint32_t)i;
   }
 }
 
@@ -81,16 +78,18 @@ __global__ __launch_bounds__(kIngBlock) void k_radix_count(int64_t n, const uint
   if (threadIdx.x < kDigits) hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter of one radix tile. Thread t owns items 8t .. 8t + 7 of the
+// Stable scatter of one radix tile (payload P carried along). Thread t owns
+// items 8t .. 8t + 7 of the
 // tile (input order = thread order, then item order), so an item's rank among
 // the tile's items of its digit is: (items of that digit in threads < t) +
 // (earlier items of that digit in thread t). off = exclusive scan of hist.
-__global__ __launch_bounds__(kIngBlock) void k_radix_scatter(int64_t n, const uint64_t *key, const uint32_t *pos,
-                                                             int shift, int64_t ntiles, const int32_t *off,
-                                                             uint64_t *key_out, uint32_t *pos_out) {
+template <typename P>
+__global__ __launch_bounds__(kIngBlock) void k_radix_scatter(int64_t n, const uint64_t *key, const P *val, int shift,
+                                                             int64_t ntiles, const int32_t *off, uint64_t *key_out,
+                                                             P *val_out) {
   constexpr int IPT = kTileItems / kIngBlock;  // 8
   __shared__ uint64_t sk[kTileItems];
-  __shared__ uint32_t sp[kTileItems];
+  __shared__ P sp[kTileItems];
   __shared__ int32_t cnt[kDigits][kIngBlock];
   __shared__ int32_t gsum[kDigits][kIngBlock / 16];
   const int t = threadIdx.x;
@@ -99,7 +98,7 @@ __global__ __launch_bounds__(kIngBlock) void k_radix_scatter(int64_t n, const ui
   for (int j = t; j < kTileItems; j += kIngBlock) {  // coalesced staging
     if (j < nt) {
       sk[j] = key[base + j];
-      sp[j] = pos[base + j];
+      sp[j] = val[base + j];
     }
   }
 #pragma unroll
@@ -137,7 +136,7 @@ __global__ __launch_bounds__(kIngBlock) void k_radix_scatter(int64_t n, const ui
       const int d = (int)((sk[j] >> shift) & (kDigits - 1));
       const int32_t dst = gsum[d][t >> 4] + cnt[d][t]++;
       key_out[dst] = sk[j];
-      pos_out[dst] = sp[j];
+      val_out[dst] = sp[j];
     }
   }
 }
@@ -211,16 +210,17 @@ __global__ void k_run_heads(int64_t n, const uint64_t *key, int32_t *head) {
     head[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
 }
 
-// One thread per run head: the run's values summed left to right (input
-// order), written at the head's rank with its row and column.
+// One thread per run head: the run's values (sorted along with the keys, so
+// in input order within the run) summed left to right, written at the
+// head's rank with its row and column.
 template <typename T>
-__global__ void k_run_sum(int64_t n, const uint64_t *key, const uint32_t *pos, const int32_t *head,
-                          const int32_t *rank, const T *vals, int32_t *out_row, int32_t *out_col, T *out_val) {
+__global__ void k_run_sum(int64_t n, const uint64_t *key, const int32_t *head, const int32_t *rank, const T *vals,
+                          int32_t *out_row, int32_t *out_col, T *out_val) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     if (!head[i]) continue;
     const uint64_t k = key[i];
-    T s = vals[pos[i]];
-    for (int64_t j = i + 1; j < n && key[j] == k; ++j) s += vals[pos[j]];
+    T s = vals[i];
+    for (int64_t j = i + 1; j < n && key[j] == k; ++j) s += vals[j];
     const int32_t p = rank[i];
     out_row[p] = (int32_t)(k >> 32);
     out_col[p] = (int32_t)(k & 0xffffffffu);
@@ -268,15 +268,15 @@ struct DevScan {
   }
 };
 
-// Sorted, summed CSR arrays of the (key, pos) pairs on the device.
+// Sorted, summed CSR arrays of the (key, value) pairs on the device.
 template <typename T>
-svdh_op *finish_csr(Ctx *c, int64_t m, int64_t n, int64_t nnz, DevBuf &key, DevBuf &pos, const T *dvals,
-                    int32_t vdt, uint32_t flags) {
+svdh_op *finish_csr(Ctx *c, int64_t m, int64_t n, int64_t nnz, DevBuf &key, DevBuf &dvals, int32_t vdt,
+                    uint32_t flags) {
   std::vector<int32_t> rp32(m + 1, 0), ci32, rows32;
   std::vector<T> v;
   int64_t u = 0;
   if (nnz > 0) {
-    DevBuf key2(nnz * sizeof(uint64_t)), pos2(nnz * sizeof(uint32_t));
+    DevBuf key2(nnz * sizeof(uint64_t)), val2(nnz * sizeof(T));
     const int64_t ntiles = (nnz + kTileItems - 1) / kTileItems;
     DevBuf hist(std::max<size_t>(kDigits * ntiles * sizeof(int32_t), 16));
     DevBuf off(std::max<size_t>(kDigits * ntiles * sizeof(int32_t), 16));
@@ -285,7 +285,7 @@ svdh_op *finish_csr(Ctx *c, int64_t m, int64_t n, int64_t nnz, DevBuf &key, DevB
     {
       KTimer tm(c, KC_INGEST, 0.0);
       uint64_t *k0 = key.as<uint64_t>(), *k1 = key2.as<uint64_t>();
-      uint32_t *p0 = pos.as<uint32_t>(), *p1 = pos2.as<uint32_t>();
+      T *p0 = dvals.as<T>(), *p1 = val2.as<T>();
       std::vector<int> shifts;
       for (int s = 0; s < bits_for(n); s += 4) shifts.push_back(s);
       for (int s = 0; s < bits_for(m); s += 4) shifts.push_back(32 + s);
@@ -293,7 +293,7 @@ svdh_op *finish_csr(Ctx *c, int64_t m, int64_t n, int64_t nnz, DevBuf &key, DevB
         hipLaunchKernelGGL(k_radix_count, (int)ntiles, kIngBlock, 0, c->stream, nnz, k0, sh, ntiles,
                            hist.as<int32_t>());
         scan.run(hist.as<int32_t>(), off.as<int32_t>(), kDigits * ntiles, false);
-        hipLaunchKernelGGL(k_radix_scatter, (int)ntiles, kIngBlock, 0, c->stream, nnz, k0, p0, sh, ntiles,
+        hipLaunchKernelGGL(k_radix_scatter<T>, (int)ntiles, kIngBlock, 0, c->stream, nnz, k0, p0, sh, ntiles,
                            off.as<int32_t>(), k1, p1);
         HIP_CHECK(hipGetLastError());
         std::swap(k0, k1);
@@ -305,8 +305,8 @@ svdh_op *finish_csr(Ctx *c, int64_t m, int64_t n, int64_t nnz, DevBuf &key, DevB
       orow.alloc(std::max<size_t>(u * sizeof(int32_t), 16));
       ocol.alloc(std::max<size_t>(u * sizeof(int32_t), 16));
       oval.alloc(std::max<size_t>(u * sizeof(T), 16));
-      hipLaunchKernelGGL(k_run_sum<T>, ing_grid(nnz), kIngBlock, 0, c->stream, nnz, k0, p0, head.as<int32_t>(),
-                         rank.as<int32_t>(), dvals, orow.as<int32_t>(), ocol.as<int32_t>(), oval.as<T>());
+      hipLaunchKernelGGL(k_run_sum<T>, ing_grid(nnz), kIngBlock, 0, c->stream, nnz, k0, head.as<int32_t>(),
+                         rank.as<int32_t>(), p0, orow.as<int32_t>(), ocol.as<int32_t>(), oval.as<T>());
       hipLaunchKernelGGL(k_row_pointers, ing_grid(u), kIngBlock, 0, c->stream, u, m, orow.as<int32_t>(),
                          drp.as<int32_t>());
       HIP_CHECK(hipGetLastError());
@@ -337,20 +337,20 @@ template <typename PackFn>
 svdh_op *ingest(Ctx *c, int64_t m, int64_t n, int64_t nnz, const void *vals, int32_t vdt, uint32_t flags,
                 PackFn pack) {
   const size_t vs = dtype_size(vdt);
-  DevBuf key(std::max<size_t>(nnz * sizeof(uint64_t), 16)), pos(std::max<size_t>(nnz * sizeof(uint32_t), 16));
+  DevBuf key(std::max<size_t>(nnz * sizeof(uint64_t), 16));
   DevBuf dvals(std::max<size_t>(nnz * vs, 16)), bad(16);
   if (nnz > 0) {
     HIP_CHECK(hipMemsetAsync(bad.p, 0, 16, c->stream));
     HIP_CHECK(hipMemcpyAsync(dvals.p, vals, nnz * vs, hipMemcpyHostToDevice, c->stream));
-    pack(key, pos, bad.as<int32_t>());
+    pack(key, bad.as<int32_t>());
     HIP_CHECK(hipGetLastError());
     int32_t hb = 0;
     HIP_CHECK(hipMemcpyAsync(&hb, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     if (hb) fail(SVDH_ERR_ARG, "an index is out of range");
   }
-  if (vdt == SVDH_F64) return finish_csr<double>(c, m, n, nnz, key, pos, dvals.as<double>(), vdt, flags);
-  return finish_csr<float>(c, m, n, nnz, key, pos, dvals.as<float>(), vdt, flags);
+  if (vdt == SVDH_F64) return finish_csr<double>(c, m, n, nnz, key, dvals, vdt, flags);
+  return finish_csr<float>(c, m, n, nnz, key, dvals, vdt, flags);
 }
 
 }  // namespace
@@ -359,13 +359,13 @@ svdh_op *coo_create(Ctx *c, int64_t m, int64_t n, int64_t nnz, const int64_t *ro
                     const void *vals, int32_t vdt, uint32_t flags) {
   check_shape(m, n, nnz, vdt);
   if (nnz > 0 && (!rows || !cols || !vals)) fail(SVDH_ERR_ARG, "null COO array");
-  return ingest(c, m, n, nnz, vals, vdt, flags, [&](DevBuf &key, DevBuf &pos, int32_t *bad) {
+  return ingest(c, m, n, nnz, vals, vdt, flags, [&](DevBuf &key, int32_t *bad) {
     DevBuf dr(nnz * sizeof(int64_t)), dc(nnz * sizeof(int64_t));
     HIP_CHECK(hipMemcpyAsync(dr.p, rows, nnz * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(dc.p, cols, nnz * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
     KTimer tm(c, KC_INGEST, 0.0);
     hipLaunchKernelGGL(k_coo_pack, ing_grid(nnz), kIngBlock, 0, c->stream, nnz, dr.as<int64_t>(), dc.as<int64_t>(),
-                       m, n, key.as<uint64_t>(), pos.as<uint32_t>(), bad);
+                       m, n, key.as<uint64_t>(), bad);
     HIP_CHECK(hipStreamSynchronize(c->stream));  // dr, dc are freed on return
   });
 }
@@ -377,13 +377,13 @@ svdh_op *csc_create(Ctx *c, int64_t m, int64_t n, int64_t nnz, const int64_t *co
   if (colptr[0] != 0 || colptr[n] != nnz) fail(SVDH_ERR_ARG, "column pointers must start at 0 and end at nnz");
   for (int64_t j = 0; j < n; ++j)
     if (colptr[j + 1] < colptr[j]) fail(SVDH_ERR_ARG, "column pointers decrease at column %lld", (long long)j);
-  return ingest(c, m, n, nnz, vals, vdt, flags, [&](DevBuf &key, DevBuf &pos, int32_t *bad) {
+  return ingest(c, m, n, nnz, vals, vdt, flags, [&](DevBuf &key, int32_t *bad) {
     DevBuf dp((n + 1) * sizeof(int64_t)), dr(nnz * sizeof(int64_t));
     HIP_CHECK(hipMemcpyAsync(dp.p, colptr, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
     HIP_CHECK(hipMemcpyAsync(dr.p, rows, nnz * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
     KTimer tm(c, KC_INGEST, 0.0);
     hipLaunchKernelGGL(k_csc_pack, ing_grid(nnz), kIngBlock, 0, c->stream, nnz, dp.as<int64_t>(), n,
-                       dr.as<int64_t>(), m, key.as<uint64_t>(), pos.as<uint32_t>(), bad);
+                       dr.as<int64_t>(), m, key.as<uint64_t>(), bad);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
