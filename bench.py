@@ -119,8 +119,9 @@ def _triplet_residual(a, u_cols, s, vt_rows):
 
 def bench_las2(args):
     """lanczos::svd_dim_seed on the 2,000 x 1,500 (k = 10) or 500,000 x 30,000
-    (k = 100) workload, FP64 and FP32-SpMV/FP64-recurrence; the dominant
-    kernel is the SpMV (A x and A^T y)."""
+    (k = 100) workload, FP64 and FP32-SpMV/FP64-recurrence, U and V^T left in
+    HBM (svdh_las2_device; the host hand-over is timed beside it); the
+    dominant kernel is the SpMV (A x and A^T y)."""
     import single_svdlib as sl
     from single_svdlib import _ffi, synth
     import oracle
@@ -129,17 +130,33 @@ def bench_las2(args):
     k = 10 if cfg1 else 100
     lib = _ffi.lib()
     res = {}
+    m, n = a.shape
+    u_dev = _ffi.DeviceArray((k, m))
+    vt_dev = _ffi.DeviceArray((k, n))
+    s_host = np.zeros(k)
     for prec, dt in (("f64", np.float64), ("f32", np.float32)):
         op = sl.CsrMatrix.from_scipy(a.astype(dt))
         ctx = op._ctx
+        pcode = _ffi.PREC_F64 if prec == "f64" else _ffi.PREC_F32
+
+        def solve_device():
+            meta = _ffi.Result_()
+            _ffi.check(lib.svdh_las2_device(ctx, op._handle, k, 0, -1e-30, 1e-30, 1e-6, 42, None, pcode, u_dev.ptr,
+                                            m, _ffi.ptr(s_host), vt_dev.ptr, n, C.byref(meta)))
+            return meta
+
         for _ in range(args.warmup):
-            sl.lanczos.svd_dim_seed(op, k, 42)
+            solve_device()
         _ffi.check(lib.svdh_ctx_synchronize(ctx))
         times = []
         for _ in range(args.steps):
             t0 = time.perf_counter()
-            rec = sl.lanczos.svd_dim_seed(op, k, 42)
+            solve_device()
             times.append(time.perf_counter() - t0)
+        # the same solve handing U, s, V^T over in host memory (PCIe-inclusive)
+        t0 = time.perf_counter()
+        rec = sl.lanczos.svd_dim_seed(op, k, 42)
+        t_host = time.perf_counter() - t0
         _ffi.check(lib.svdh_ctx_profile(ctx, 1))
         rec = sl.lanczos.svd_dim_seed(op, k, 42)
         st = _kernel_stats(lib, ctx, (("spmv", _ffi.KC_SPMV), ("blas1", _ffi.KC_BLAS1), ("gemv", _ffi.KC_GEMV),
@@ -148,12 +165,13 @@ def bench_las2(args):
         _ffi.check(lib.svdh_ctx_profile(ctx, 0))
         d = rec.diagnostics
         r = _triplet_residual(a, rec.u.T, rec.s.astype(np.float64), rec.vt)
-        res[prec] = {"s_per_solve": float(np.mean(times)), "lanczos_steps": d.lanczos_steps,
+        res[prec] = {"s_per_solve": float(np.mean(times)), "host_output_s": t_host, "lanczos_steps": d.lanczos_steps,
                      "ritz_values_stabilized": d.ritz_values_stabilized, "significant_values": d.significant_values,
                      "residual_fp64": r, "sigma_max": float(rec.s[0]), "sigma_min": float(rec.s[-1]),
                      "roofline_spmv": _roofline(st["spmv"]["bytes"], st["spmv"]["_ms"], st["spmv"]["_n"]),
                      "kernels_ms": {kname: v["ms_per_step"] for kname, v in st.items()}}
-        log(f"las2 {prec}: {res[prec]['s_per_solve'] * 1e3:.1f} ms, steps {d.lanczos_steps}, residual {r:.2e}")
+        log(f"las2 {prec}: {res[prec]['s_per_solve'] * 1e3:.1f} ms (host outputs {t_host * 1e3:.1f} ms), "
+            f"steps {d.lanczos_steps}, residual {r:.2e}")
         del op
     cpu = None
     if not args.no_cpu:

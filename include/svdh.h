@@ -198,6 +198,19 @@ int32_t svdh_las2(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t 
                   double endl, double endr, double kappa, uint32_t random_seed,
                   const double *start, int32_t precision, svdh_result *out);
 
+/* svdh_las2 with the singular vectors left in device memory (the measured
+ * path; the host hand-over of U, d x nrows FP64, is PCIe-bound for tall
+ * matrices). u: device, room for min(dimensions, nrows, ncols) rows of
+ * nrows (row-major, leading dimension ldu >= nrows: rows are the left
+ * vectors, the layout of SvdRec.u for LAS2, mod.rs:172); vt: device, same
+ * count of rows of ncols (ldvt >= ncols); s: host, room for as many values.
+ * meta receives d (the rows written), the shapes and the diagnostics; its
+ * u, s and vt stay NULL. Same algorithm and arguments as svdh_las2. */
+int32_t svdh_las2_device(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t iterations,
+                         double endl, double endr, double kappa, uint32_t random_seed,
+                         const double *start, int32_t precision, double *u, int64_t ldu, double *s,
+                         double *vt, int64_t ldvt, svdh_result *meta);
+
 /* Start vector the LAS2 driver draws for a seed (host only, no GPU):
  * out[i] = 2 U_i - 1 with U_i = (mix64(key + (i+1) phi) >> 11) 2^-53,
  * key = seed phi XOR 0x5356444C49423031, phi = 0x9E3779B97F4A7C15 and mix64

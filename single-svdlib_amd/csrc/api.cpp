@@ -104,7 +104,8 @@ svdh_op *coo_create(Ctx *c, int64_t m, int64_t n, int64_t nnz, const int64_t *ro
 svdh_op *csc_create(Ctx *c, int64_t m, int64_t n, int64_t nnz, const int64_t *colptr, const int64_t *rows,
                     const void *vals, int32_t vdt, uint32_t flags);
 int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
-                   double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out);
+                   double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out,
+                   const RsvdDeviceOut *dev);
 int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_oversamples, int64_t n_power_iters,
                    int32_t normalizer, int32_t mean_center, uint64_t seed, int32_t has_seed, const double *omega,
                    int32_t precision, double tol, int64_t max_power_iters, svdh_result *out,
@@ -450,10 +451,32 @@ int32_t svdh_las2(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t 
     if (!out) fail(SVDH_ERR_ARG, "null result");
     if (precision == SVDH_PREC_F32) ensure_values(const_cast<svdh_op *>(op), SVDH_F32);
     las2_solve(ctx, const_cast<svdh_op *>(op), dimensions, iterations, endl, endr, kappa, random_seed, start,
-               precision, out);
+               precision, out, nullptr);
   });
   if (st != SVDH_OK && out) svdh_result_free(out);
   return st;
+}
+
+int32_t svdh_las2_device(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t iterations, double endl,
+                         double endr, double kappa, uint32_t random_seed, const double *start, int32_t precision,
+                         double *u, int64_t ldu, double *s, double *vt, int64_t ldvt, svdh_result *meta) {
+  if (meta) std::memset(meta, 0, sizeof *meta);
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    if (!meta || !u || !s || !vt) fail(SVDH_ERR_ARG, "null output");
+    // the written rows are d <= dims; the leading dimensions must cover a row
+    const int64_t dims_max = std::min(op->nrows, op->ncols);
+    const int64_t dims = dimensions <= 0 || dimensions > dims_max ? dims_max : dimensions;
+    (void)dims;
+    if (ldu < op->nrows || ldvt < op->ncols)
+      fail(SVDH_ERR_ARG, "ldu >= nrows and ldvt >= ncols are required (%lld, %lld)", (long long)ldu,
+           (long long)ldvt);
+    if (precision == SVDH_PREC_F32) ensure_values(const_cast<svdh_op *>(op), SVDH_F32);
+    const RsvdDeviceOut dev{u, ldu, s, vt, ldvt};
+    las2_solve(ctx, const_cast<svdh_op *>(op), dimensions, iterations, endl, endr, kappa, random_seed, start,
+               precision, meta, &dev);
+  });
 }
 
 int32_t svdh_rsvd(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, int64_t n_oversamples,

@@ -369,7 +369,10 @@ struct Las2 {
   }
 
   // mod.rs:1056-1259 with the Ritz indexing and ordering of legacy.rs:1796-1835.
-  void ritvec(int64_t dims, double kappa, int64_t steps, int64_t neig, svdh_result *out, int64_t *nsig_out) {
+  // dev (nullable): U (d x m) and V^T (d x n) go to these device buffers
+  // instead of host memory, s to dev->s.
+  void ritvec(int64_t dims, double kappa, int64_t steps, int64_t neig, svdh_result *out, int64_t *nsig_out,
+              const RsvdDeviceOut *dev) {
     const int64_t js = steps + 1;
     const double sp = sparsity();
     const double aeps = sp > 0.99 ? eps * 100.0 : sp > 0.9 ? eps * 10.0 : eps;
@@ -393,10 +396,12 @@ struct Las2 {
     out->u_cols = transposed ? ncols : nrows;
     out->vt_rows = d;
     out->vt_cols = transposed ? nrows : ncols;
-    out->s = static_cast<double *>(std::calloc(std::max<int64_t>(d, 1), sizeof(double)));
-    out->u = static_cast<double *>(std::calloc(std::max<int64_t>(d * out->u_cols, 1), sizeof(double)));
-    out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(d * out->vt_cols, 1), sizeof(double)));
-    if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
+    if (!dev) {
+      out->s = static_cast<double *>(std::calloc(std::max<int64_t>(d, 1), sizeof(double)));
+      out->u = static_cast<double *>(std::calloc(std::max<int64_t>(d * out->u_cols, 1), sizeof(double)));
+      out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(d * out->vt_cols, 1), sizeof(double)));
+      if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
+    }
     if (d == 0) return;
     // Coefficients W (js x d): column r is the eigenvector of T for the r-th
     // largest significant Ritz value, in the basis q_0..q_{js-1}.
@@ -428,12 +433,24 @@ struct Las2 {
     HIP_CHECK(hipMemcpyAsync(hd.data(), dots, d * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     std::vector<double> hs(d);
+    double *s_out = dev ? dev->s : out->s;
     for (int64_t r = 0; r < d; ++r) {
       const double sval = std::sqrt(std::max(hd[r], 0.0));
-      out->s[r] = sval;
+      s_out[r] = sval;
       hs[r] = sval > aeps ? 1.0 / sval : 1.0 / std::max(sval, aeps);
     }
     HIP_CHECK(hipMemcpyAsync(scale, hs.data(), d * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (dev) {
+      // the working (possibly transposed) layout written straight into the
+      // caller's device buffers: Ut (d x nrows) and Vt (d x ncols)
+      double *Ut = transposed ? dev->vt : dev->u;
+      double *Vt = transposed ? dev->u : dev->vt;
+      const int64_t ldut = transposed ? dev->ldvt : dev->ldu, ldvt = transposed ? dev->ldu : dev->ldvt;
+      panel_transpose_scale(c, nrows, d, AV.as<double>(), ldv, scale, Ut, ldut);
+      panel_transpose_scale(c, ncols, d, V.as<double>(), ldv, nullptr, Vt, ldvt);
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      return;
+    }
     // The working (possibly transposed) layout: Ut is d x nrows, Vt is d x ncols.
     double *Ut = transposed ? out->vt : out->u;
     double *Vt = transposed ? out->u : out->vt;
@@ -450,7 +467,8 @@ struct Las2 {
 }  // namespace
 
 int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
-                   double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out) {
+                   double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out,
+                   const RsvdDeviceOut *dev) {
   if (precision != SVDH_PREC_F64 && precision != SVDH_PREC_F32)
     fail(SVDH_ERR_ARG, "LAS2 precision must be F64 or F32");
   if (op->nrows <= 0 || op->ncols <= 0) fail(SVDH_ERR_LAS2, "svd_las2: empty matrix");
@@ -497,7 +515,7 @@ int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t itera
   int64_t neig = 0, nsig = 0;
   const int64_t steps = L.lanso(dimensions, endl, endr, random_seed, &neig);
   const double kap = std::max(std::fabs(kappa), std::pow(L.eps, 0.75));
-  L.ritvec(dimensions, kap, steps, neig, out, &nsig);
+  L.ritvec(dimensions, kap, steps, neig, out, &nsig, dev);
   svdh_diagnostics &dg = out->diagnostics;
   dg.non_zero = op->nnz;
   dg.dimensions = dimensions;

@@ -129,3 +129,28 @@ def test_error_paths(lib):
         lib.lanczos.svd(lib.CsrMatrix.from_scipy(sp.csr_matrix(np.ones((1, 5)))))
     with pytest.raises(error.StartvError):
         lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(sp.csr_matrix((6, 4))), 0, 1)
+
+
+@pytest.mark.parametrize("shape", [(900, 400), (300, 800)])  # the second takes the transposed path
+def test_device_output_equals_host_output(lib, shape):
+    # svdh_las2_device writes U (d x m) and V^T (d x n) into device buffers;
+    # same solve, so the values and vectors equal the host hand-over bit for bit
+    import ctypes as C
+    from single_svdlib import _ffi
+    a = random_csr(shape[0], shape[1], 0.03, seed=41)
+    op = lib.CsrMatrix.from_scipy(a)
+    rec = lib.lanczos.svd_dim_seed(op, 6, 123)
+    m, n = shape
+    k = 6
+    u_dev = _ffi.DeviceArray((k, m))
+    vt_dev = _ffi.DeviceArray((k, n))
+    s = np.zeros(k)
+    meta = _ffi.Result_()
+    _ffi.check(_ffi.lib().svdh_las2_device(op._ctx, op._handle, 6, 0, -1e-30, 1e-30, 1e-6, 123, None,
+                                           _ffi.PREC_F64, u_dev.ptr, m, _ffi.ptr(s), vt_dev.ptr, n,
+                                           C.byref(meta)))
+    d = meta.d
+    assert d == rec.d and meta.diagnostics.lanczos_steps == rec.diagnostics.lanczos_steps
+    assert np.array_equal(s[:d], rec.s)
+    assert np.array_equal(u_dev.numpy()[:d], rec.u) and np.array_equal(vt_dev.numpy()[:d], rec.vt)
+    assert bool(meta.diagnostics.transposed) == (n >= 1.2 * m)
