@@ -148,6 +148,22 @@ struct Schedule {
   DevBuf chunks;    // int32 x 5 per chunk: row, begin, end, first_slot, nchunks_of_row
   DevBuf counters;  // int32 x 3 per row with several chunks: row, first chunk, chunk count
   DevBuf partials;  // chunk partial results (grown on demand)
+  // Streamed SpMV (views without a row map): int4 tasks (row begin, row end or
+  // chunk code, first nonzero, end nonzero) over the rows in storage order.
+  bool stream = false;
+  int32_t ntasks = 0;
+  DevBuf tasks;
+  // Source-tiled SpMV (plain operators with many nonzeros): the matrix cut at
+  // source blocks of width sb and stacked, block-major, as nb * nrows rows with
+  // 16-bit in-block column indices; perm[p] = CSR position of stacked entry p,
+  // vals[dt] = the values in that order (gathered on first use). Workgroup
+  // ranges (block, first task, end task) of int4 tasks over the stacked rows;
+  // part holds one FP64 partial per stacked row when nb > 1.
+  struct Tiled {
+    bool on = false;
+    int32_t nb = 0, sb = 0, ntasks = 0, nranges = 0;
+    DevBuf rowptr, col, perm, vals[3], tasks, ranges, part;
+  } tiled;
   // SpMM segment schedule: per source block offsets/counts into `chunks`.
   int nblk = 1;
   int32_t blk_off[8] = {0};

@@ -26,6 +26,28 @@ namespace svdh {
 namespace {
 
 constexpr int kSpmvLong = 1024;   // SpMV rows above this nnz are chunked
+#ifndef SVDH_SPMV_PER_LANE
+#define SVDH_SPMV_PER_LANE 4
+#endif
+// Binned SpMV (masked views): a row of nnz nonzeros gets the fewest lanes (a
+// power of two) that hold it at this many per lane.
+constexpr int kSpmvPerLane = SVDH_SPMV_PER_LANE;
+// Streamed SpMV: a wave reads kStreamStep consecutive nonzeros per step, one
+// per lane and load; rows longer than a step are one task each (a chunk of at
+// most kStreamChunk nonzeros per task).
+constexpr int kStreamU = 8;
+constexpr int kStreamStep = 64 * kStreamU;
+constexpr int kStreamChunk = 8 * kStreamStep;
+constexpr int kStreamRows = 64;  // rows per task at most
+// Source-tiled SpMV: workgroups of kTiledBlock threads stage kTiledSrc source
+// entries in LDS; tasks as in the streamed SpMV with kTiledU nonzeros per lane
+// per step (the LDS left after the source block holds the waves' products).
+constexpr int kTiledBlock = 1024;
+constexpr int kTiledSrc = 16384;
+constexpr int kTiledU = 4;
+constexpr int kTiledStep = 64 * kTiledU;
+constexpr int kTiledMaxRow = 64 * kTiledStep;  // longest stacked row (one task, no chunks)
+constexpr int64_t kTiledMinNnz = 1 << 20;
 constexpr int kSpmvChunk = 4096;  // SpMV chunk nnz (256 threads x 16)
 
 typedef uint16_t bf16_t;
@@ -130,9 +152,16 @@ struct SpmvArgs {
   BinDesc bins[kMaxBins];
 };
 
+// One row per group of 2^LG lanes. The bin of a row is chosen so that a lane
+// holds at most kSpmvPerLane of its nonzeros, and the loads of one lane's
+// nonzeros are issued together, predicated on the row end, so that a row
+// costs one round of independent gathers rather than a chain of dependent
+// ones. Each lane adds its nonzeros in index order, then the lanes are
+// reduced by butterfly.
 template <int LG, bool MC, typename V, typename X, typename Y>
 __device__ __forceinline__ void spmv_group(const SpmvArgs<V, X, Y> &a, const BinDesc &d, int lb) {
   constexpr int G = 1 << LG;
+  constexpr int U = kSpmvPerLane;
   constexpr int RPB = kBlock >> LG;
   const int gid = threadIdx.x >> LG;
   const int lane = threadIdx.x & (G - 1);
@@ -143,27 +172,28 @@ __device__ __forceinline__ void spmv_group(const SpmvArgs<V, X, Y> &a, const Bin
   if (active) {
     r = a.rows[d.row_begin + idx];
     const int src = a.rowmap ? a.rowmap[r] : r;
-    int k = a.rowptr[src] + lane;
     const int e = a.rowptr[src + 1];
-    for (; k + 3 * G < e; k += 4 * G) {
-      int c[4];
-      double v[4];
+    for (int k = a.rowptr[src] + lane; k < e; k += U * G) {
+      int c[U];
+      V v[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        c[u] = a.colidx[k + u * G];
-        v[u] = ld_as<double>(a.vals, k + u * G);
+      for (int u = 0; u < U; ++u) {
+        // unconditional loads (entry 0 stands in past the row end), so that
+        // all of them are in flight before the first is used
+        const bool ok = k + u * G < e;
+        const int ks = ok ? k + u * G : 0;
+        c[u] = a.colidx[ks];
+        v[u] = a.vals[ks];
+        if (!ok) c[u] = -1;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         int cc = c[u];
-        if (MC) cc = a.colmap[cc];
-        if (!MC || cc >= 0) acc += v[u] * ld_as<double>(a.x, cc);
+        if (MC && cc >= 0) cc = a.colmap[cc];
+        // unconditional gather (entry 0 stands in for a dropped one), then a select
+        const double xv = ld_as<double>(a.x, cc >= 0 ? cc : 0);
+        acc += cc >= 0 ? ld_as<double>(v, u) * xv : 0.0;
       }
-    }
-    for (; k < e; k += G) {
-      int cc = a.colidx[k];
-      if (MC) cc = a.colmap[cc];
-      if (!MC || cc >= 0) acc += ld_as<double>(a.vals, k) * ld_as<double>(a.x, cc);
     }
   }
   if (G > 1) {
@@ -223,6 +253,239 @@ __global__ void k_spmv_fin(int nrows, const int32_t *multi, const double *partia
   const int r = multi[3 * i], first = multi[3 * i + 1], nch = multi[3 * i + 2];
   double t = 0.0;
   for (int k = 0; k < nch; ++k) t += partials[first + k];
+  y[r] = static_cast<Y>(t);
+}
+
+// Streamed SpMV over the rows in storage order. Task t = (ra, rb, kb, ke):
+//  - rb > ra: rows ra .. rb - 1, whose nonzeros kb .. ke - 1 fit in one step.
+//    Lane l reads nonzeros kb + l + 64 j (each load instruction covers 64
+//    consecutive entries), the products go to the wave's LDS row, and the
+//    rows are summed from there: with nr rows, G = 64 / 2^ceil(log2 nr) lanes
+//    per row, lane s of a group adds entries s, s + G, ... of its row and the
+//    group is reduced by butterfly.
+//  - rb <= 0: one row ra (longer than a step), nonzeros kb .. ke - 1 in steps;
+//    each lane adds its entries in order, then the wave is reduced. rb = 0:
+//    the task is the whole row (written to y); rb < 0: a chunk, written to
+//    partials[-rb - 1] and added in chunk order by k_spmv_fin.
+// No workgroup-level wait: a wave's LDS row is its own.
+template <typename V, typename X, typename Y>
+struct StreamArgs {
+  const int32_t *rowptr;
+  const int32_t *colidx;
+  const V *vals;
+  const int32_t *colmap;
+  const int4 *tasks;
+  double *partials;
+  const X *x;
+  Y *y;
+  int32_t ntasks;
+};
+
+template <bool MC, typename V, typename X, typename Y>
+__global__ __launch_bounds__(kBlock) void k_spmv_stream(StreamArgs<V, X, Y> a) {
+  __shared__ double prod[kBlock / 64][kStreamStep];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t = blockIdx.x * (kBlock / 64) + w;
+  if (t >= a.ntasks) return;
+  const int4 tk = a.tasks[t];
+  const int ra = tk.x, rb = tk.y, kb = tk.z, ke = tk.w;
+  if (rb > ra) {
+    const int nr = rb - ra;
+    const int lg = nr > 1 ? 32 - __clz(nr - 1) : 0;  // ceil(log2 nr)
+    const int sh = 6 - lg;                            // G = 2^sh lanes per row
+    const int gid = lane >> sh, sub = lane & ((1 << sh) - 1);
+    // lanes past the last row load row ra's bounds and then take an empty range
+    const int q = gid < nr ? ra + gid : ra;
+    const int p0 = a.rowptr[q], p1 = a.rowptr[q + 1];
+    const int rs = gid < nr ? p0 - kb : 0, re = gid < nr ? p1 - kb : 0;
+    int c[kStreamU];
+    V v[kStreamU];
+#pragma unroll
+    for (int j = 0; j < kStreamU; ++j) {
+      const int k = kb + lane + 64 * j;
+      const bool ok = k < ke;
+      const int ks = ok ? k : 0;  // unconditional loads, all in flight together
+      c[j] = a.colidx[ks];
+      v[j] = a.vals[ks];
+      if (!ok) c[j] = -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kStreamU; ++j) {
+      int cc = c[j];
+      if (MC && cc >= 0) cc = a.colmap[cc];
+      const double xv = ld_as<double>(a.x, cc >= 0 ? cc : 0);  // unconditional gather, then a select
+      prod[w][lane + 64 * j] = cc >= 0 ? ld_as<double>(v, j) * xv : 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double acc = 0.0;
+    for (int k = rs + sub; k < re; k += 1 << sh) acc += prod[w][k];
+    for (int off = 32 >> lg; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (gid < nr && sub == 0) a.y[ra + gid] = static_cast<Y>(acc);
+    return;
+  }
+  double acc = 0.0;
+  for (int k0 = kb; k0 < ke; k0 += kStreamStep) {
+    int c[kStreamU];
+    V v[kStreamU];
+#pragma unroll
+    for (int j = 0; j < kStreamU; ++j) {
+      const int k = k0 + lane + 64 * j;
+      const bool ok = k < ke;
+      const int ks = ok ? k : 0;
+      c[j] = a.colidx[ks];
+      v[j] = a.vals[ks];
+      if (!ok) c[j] = -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kStreamU; ++j) {
+      int cc = c[j];
+      if (MC && cc >= 0) cc = a.colmap[cc];
+      const double xv = ld_as<double>(a.x, cc >= 0 ? cc : 0);
+      acc += cc >= 0 ? ld_as<double>(v, j) * xv : 0.0;
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    if (rb == 0)
+      a.y[ra] = static_cast<Y>(acc);
+    else
+      a.partials[-rb - 1] = acc;
+  }
+}
+
+// Source-tiled SpMV. Workgroup g takes range g = (block b, tasks t0 .. t1):
+// it copies source entries [b sb, b sb + sb) into LDS, then its waves take the
+// range's tasks in turn, each as in k_spmv_stream with the source read from
+// LDS through the 16-bit in-block column index. Stacked row q = b nout + r
+// is written to part[q] (nb > 1, added in block order by k_spmv_tiled_fin) or
+// to y[r] (nb = 1).
+template <typename V, typename X, typename Y>
+struct TiledArgs {
+  const int32_t *rowptr;  // stacked, nb * nout + 1
+  const uint16_t *col;
+  const V *vals;
+  const int4 *tasks;
+  const int4 *ranges;
+  const X *x;
+  double *part;
+  Y *y;
+  int64_t nsrc;
+  int32_t sb;
+  int32_t direct;
+};
+
+template <typename V, typename X, typename Y>
+__global__ __launch_bounds__(kTiledBlock) void k_spmv_tiled(TiledArgs<V, X, Y> a) {
+  __shared__ X xs[kTiledSrc];
+  __shared__ double prod[kTiledBlock / 64][kTiledStep];
+  const int4 rg = a.ranges[blockIdx.x];
+  const int64_t s0 = (int64_t)rg.x * a.sb;
+  const int slen = (int)min<int64_t>(a.sb, a.nsrc - s0);
+  // 8 loads per thread in flight, then the LDS writes (addresses past the
+  // block end read its last entry and are not written)
+  for (int i0 = threadIdx.x; i0 < slen; i0 += 8 * kTiledBlock) {
+    X t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = a.x[s0 + min(i0 + u * kTiledBlock, slen - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (i0 + u * kTiledBlock < slen) xs[i0 + u * kTiledBlock] = t[u];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int t = rg.y + w; t < rg.z; t += kTiledBlock / 64) {
+    const int4 tk = a.tasks[t];
+    const int qa = tk.x, qb = tk.y, kb = tk.z, ke = tk.w;
+    if (qb > qa) {
+      const int nr = qb - qa;
+      const int lg = nr > 1 ? 32 - __clz(nr - 1) : 0;
+      const int sh = 6 - lg;
+      const int gid = lane >> sh, sub = lane & ((1 << sh) - 1);
+      // lanes past the last row load row qa's bounds (unconditional loads)
+      // and then take an empty range
+      const int q = gid < nr ? qa + gid : qa;
+      const int p0 = a.rowptr[q], p1 = a.rowptr[q + 1];
+      const int rs = gid < nr ? p0 - kb : 0, re = gid < nr ? p1 - kb : 0;
+      int c[kTiledU];
+      V v[kTiledU];
+#pragma unroll
+      for (int j = 0; j < kTiledU; ++j) {
+        const int k = kb + lane + 64 * j;
+        const bool ok = k < ke;
+        const int ks = ok ? k : 0;  // unconditional loads, all in flight together
+        c[j] = a.col[ks];
+        v[j] = a.vals[ks];
+        if (!ok) c[j] = -1;
+      }
+      // the previous task's reads of this wave's products come first
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int j = 0; j < kTiledU; ++j)
+        // entries past the task end land in slots no row reads: no select
+        prod[w][lane + 64 * j] = ld_as<double>(v, j) * (double)xs[c[j] >= 0 ? c[j] : 0];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double acc = 0.0;
+      for (int k = rs + sub; k < re; k += 1 << sh) acc += prod[w][k];
+      for (int off = 32 >> lg; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      if (gid < nr && sub == 0) {
+        if (a.direct)
+          a.y[qa + gid] = static_cast<Y>(acc);
+        else
+          a.part[qa + gid] = acc;
+      }
+    } else {
+      double acc = 0.0;
+      for (int k0 = kb; k0 < ke; k0 += kTiledStep) {
+        int c[kTiledU];
+        V v[kTiledU];
+#pragma unroll
+        for (int j = 0; j < kTiledU; ++j) {
+          const int k = k0 + lane + 64 * j;
+          const bool ok = k < ke;
+          const int ks = ok ? k : 0;
+          c[j] = a.col[ks];
+          v[j] = a.vals[ks];
+          if (!ok) c[j] = -1;
+        }
+#pragma unroll
+        for (int j = 0; j < kTiledU; ++j) {
+          // unconditional LDS read at a clamped index, then a select
+          const double p = ld_as<double>(v, j) * (double)xs[c[j] >= 0 ? c[j] : 0];
+          acc += c[j] >= 0 ? p : 0.0;
+        }
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) {
+        if (a.direct)
+          a.y[qa] = static_cast<Y>(acc);
+        else
+          a.part[qa] = acc;
+      }
+    }
+  }
+}
+
+// y[r] = sum over source blocks b (in order) of part[b nout + r].
+template <typename Y>
+__global__ void k_spmv_tiled_fin(int64_t nout, int nb, const double *part, Y *y) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= nout) return;
+  double t = 0.0;
+  int b = 0;
+  for (; b + 8 <= nb; b += 8) {  // 8 loads in flight, added in block order
+    double p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] = part[(int64_t)(b + u) * nout + r];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += p[u];
+  }
+  for (; b < nb; ++b) t += part[(int64_t)b * nout + r];
   y[r] = static_cast<Y>(t);
 }
 
@@ -916,13 +1179,29 @@ svdh::CsrView svdh_op::view(int direction, bool pattern) const {
 
 namespace svdh {
 
+template <typename T>
+__global__ void k_gather(int64_t n, const int32_t *perm, const T *src, T *dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[perm[i]];
+}
+
+template <typename T> struct DtypeOf;
+template <> struct DtypeOf<double> { static constexpr int v = SVDH_F64; };
+template <> struct DtypeOf<float> { static constexpr int v = SVDH_F32; };
+template <> struct DtypeOf<bf16_t> { static constexpr int v = SVDH_BF16; };
+
 // Build (once) the schedule of one operator direction.
-//   kind 0 (SpMV): bins of 2^lg lanes per row, lg = ceil(log2(ceil(nnz/4)))
-//                  clamped to [0, 6]; rows above kSpmvLong nnz are chunked.
+//   kind 0 (SpMV): bins of 2^lg lanes per row,
+//                  lg = ceil(log2(ceil(nnz / kSpmvPerLane))) clamped to [0, 6];
+//                  rows above kSpmvLong nnz are chunked.
+static bool build_stream_schedule(svdh_op *op, const CsrView &v, Schedule &s);
+static bool build_tiled_schedule(svdh_op *op, int direction, const CsrView &v, Schedule &s);
+
 static Schedule &get_schedule(svdh_op *op, int direction, int kind) {
   Schedule &s = op->sched[direction][kind];
   if (s.built) return s;
   CsrView v = op->view(direction);
+  if (kind == 0 && (build_tiled_schedule(op, direction, v, s) || build_stream_schedule(op, v, s))) return s;
   const int64_t nout = v.nrows;
   const std::vector<int32_t> &rp = *v.h_rowptr;
   std::vector<int32_t> len(nout);
@@ -963,7 +1242,7 @@ static Schedule &get_schedule(svdh_op *op, int direction, int kind) {
   if (kind == 0) {
     int64_t i = 0;
     for (int lg = 0; lg <= 6 && i < nshort; ++lg) {
-      const int64_t lim = lg == 6 ? long_t : (4LL << lg);
+      const int64_t lim = lg == 6 ? long_t : ((int64_t)kSpmvPerLane << lg);
       int64_t j = i;
       while (j < nshort && len[order[j]] <= lim) ++j;
       if (j > i) {
@@ -1005,6 +1284,163 @@ static Schedule &get_schedule(svdh_op *op, int direction, int kind) {
   (void)c;
   s.built = true;
   return s;
+}
+
+// Source-tiled SpMV schedule (see k_spmv_tiled), for plain operators with at
+// least kTiledMinNnz nonzeros (SVDH_SPMV_TILED=0 turns it off). Source blocks
+// of equal width (a multiple of 64, at most kTiledSrc); each output row's
+// entries are split by block, keeping their order, and the pieces stacked
+// block-major. Tasks over the stacked rows of each block are cut as in the
+// streamed SpMV (runs of rows within one step, longer rows alone), and the
+// tasks of each block are dealt to workgroup ranges of about
+// nnz / (SVDH_SPMV_TILED_WG x CUs) nonzeros (default 2).
+static bool build_tiled_schedule(svdh_op *op, int direction, const CsrView &v, Schedule &s) {
+  if (op->base || v.rowmap || v.colmap || v.nnz < kTiledMinNnz) return false;
+  const char *env = std::getenv("SVDH_SPMV_TILED");
+  if (env && std::atoi(env) == 0) return false;
+  const std::vector<int32_t> &ci = direction == 0 ? op->h_colidx : op->h_t_colidx;
+  const std::vector<int32_t> &rp = *v.h_rowptr;
+  const int64_t nout = v.nrows, nsrc = v.ncols, nnz = v.nnz;
+  if ((int64_t)ci.size() != nnz || nsrc < 1 || nout < 1) return false;
+  const int64_t nb = (nsrc + kTiledSrc - 1) / kTiledSrc;
+  const int64_t sb = ((nsrc + nb - 1) / nb + 63) / 64 * 64;
+  if (nb * nout + 1 > INT32_MAX) return false;
+  // stacked row pointers
+  std::vector<int32_t> srp(nb * nout + 1, 0);
+  for (int64_t r = 0; r < nout; ++r)
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) ++srp[(ci[k] / sb) * nout + r + 1];
+  for (int64_t q = 0; q < nb * nout; ++q) {
+    if (srp[q + 1] > kTiledMaxRow) return false;
+    srp[q + 1] += srp[q];
+  }
+  std::vector<int32_t> next(srp.begin(), srp.end() - 1);
+  std::vector<uint16_t> col(nnz);
+  std::vector<int32_t> perm(nnz);
+  for (int64_t r = 0; r < nout; ++r)
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) {
+      const int64_t b = ci[k] / sb;
+      const int32_t p = next[b * nout + r]++;
+      col[p] = (uint16_t)(ci[k] - b * sb);
+      perm[p] = k;
+    }
+  // tasks per block, then workgroup ranges
+  std::vector<int4> tasks;
+  std::vector<int4> ranges;
+  const char *wenv = std::getenv("SVDH_SPMV_TILED_WG");
+  const double per_cu = wenv ? std::max(0.25, std::atof(wenv)) : 2.0;
+  const int64_t target = std::max<int64_t>(kTiledStep, (int64_t)(nnz / (per_cu * op->ctx->ncu)));
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t q0 = b * nout, q1 = q0 + nout;
+    const int32_t tfirst = (int32_t)tasks.size();
+    int64_t q = q0;
+    while (q < q1) {
+      const int32_t len = srp[q + 1] - srp[q];
+      if (len > kTiledStep) {
+        tasks.push_back(make_int4((int)q, 0, srp[q], srp[q + 1]));
+        ++q;
+        continue;
+      }
+      const int64_t qa = q;
+      const int32_t kb = srp[q];
+      while (q < q1 && q - qa < kStreamRows && srp[q + 1] - kb <= kTiledStep) ++q;
+      tasks.push_back(make_int4((int)qa, (int)q, kb, srp[q]));
+    }
+    // ranges of about `target` nonzeros, never crossing a block
+    int32_t t0 = tfirst;
+    int64_t acc = 0;
+    for (int32_t t = tfirst; t < (int32_t)tasks.size(); ++t) {
+      acc += tasks[t].w - tasks[t].z;
+      if (acc >= target) {
+        ranges.push_back(make_int4((int)b, t0, t + 1, 0));
+        t0 = t + 1;
+        acc = 0;
+      }
+    }
+    if (t0 < (int32_t)tasks.size()) ranges.push_back(make_int4((int)b, t0, (int)tasks.size(), 0));
+  }
+  Schedule::Tiled &T = s.tiled;
+  T.nb = (int32_t)nb;
+  T.sb = (int32_t)sb;
+  T.ntasks = (int32_t)tasks.size();
+  T.nranges = (int32_t)ranges.size();
+  auto up = [](DevBuf &d, const void *src, size_t bytes) {
+    d.alloc(std::max<size_t>(bytes, 16));
+    if (bytes) HIP_CHECK(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+  };
+  up(T.rowptr, srp.data(), srp.size() * sizeof(int32_t));
+  up(T.col, col.data(), col.size() * sizeof(uint16_t));
+  up(T.perm, perm.data(), perm.size() * sizeof(int32_t));
+  up(T.tasks, tasks.data(), tasks.size() * sizeof(int4));
+  up(T.ranges, ranges.data(), ranges.size() * sizeof(int4));
+  if (nb > 1) T.part.alloc((size_t)(nb * nout) * sizeof(double));
+  T.on = true;
+  s.built = true;
+  return true;
+}
+
+// Values of dtype V in the stacked order of the tiled schedule.
+template <typename V>
+static const V *tiled_values(Ctx *c, Schedule::Tiled &T, const V *vals, int64_t nnz) {
+  DevBuf &d = T.vals[DtypeOf<V>::v];
+  if (!d.p) {
+    d.alloc(std::max<size_t>(nnz * sizeof(V), 16));
+    hipLaunchKernelGGL(k_gather<V>, grid_for(nnz), kBlock, 0, c->stream, nnz, T.perm.as<int32_t>(), vals,
+                       d.as<V>());
+    HIP_CHECK(hipGetLastError());
+  }
+  return d.as<V>();
+}
+
+// Streamed SpMV tasks (rows in storage order; see k_spmv_stream): runs of
+// consecutive rows whose nonzeros fit in one step (at most kStreamRows rows),
+// and rows longer than a step as tasks of their own, cut into chunks of
+// kStreamChunk nonzeros. Not used for views with a row map (a masked
+// operator's transposed direction) or when SVDH_SPMV_STREAM=0.
+static bool build_stream_schedule(svdh_op *op, const CsrView &v, Schedule &s) {
+  if (v.h_rowmap) return false;
+  const char *env = std::getenv("SVDH_SPMV_STREAM");
+  if (env && std::atoi(env) == 0) return false;
+  const int64_t nout = v.nrows;
+  const std::vector<int32_t> &rp = *v.h_rowptr;
+  std::vector<int4> tasks;
+  std::vector<int32_t> multi;
+  int32_t nslots = 0;
+  int64_t r = 0;
+  while (r < nout) {
+    const int32_t len = rp[r + 1] - rp[r];
+    if (len > kStreamStep) {
+      const int32_t nch = (len + kStreamChunk - 1) / kStreamChunk;
+      if (nch == 1) {
+        tasks.push_back(make_int4((int)r, 0, rp[r], rp[r + 1]));
+      } else {
+        multi.push_back((int32_t)r);
+        multi.push_back(nslots);
+        multi.push_back(nch);
+        for (int32_t k = 0; k < nch; ++k)
+          tasks.push_back(make_int4((int)r, -1 - (nslots++), rp[r] + k * kStreamChunk,
+                                    std::min(rp[r + 1], rp[r] + (k + 1) * kStreamChunk)));
+      }
+      ++r;
+      continue;
+    }
+    const int64_t ra = r;
+    const int32_t kb = rp[r];
+    while (r < nout && r - ra < kStreamRows && rp[r + 1] - kb <= kStreamStep) ++r;
+    tasks.push_back(make_int4((int)ra, (int)r, kb, rp[r]));
+  }
+  s.stream = true;
+  s.ntasks = (int32_t)tasks.size();
+  s.tasks.alloc(std::max<size_t>(tasks.size(), 1) * sizeof(int4));
+  if (!tasks.empty())
+    HIP_CHECK(hipMemcpy(s.tasks.p, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
+  s.nchunks = nslots;
+  s.nslots = (int32_t)(multi.size() / 3);
+  s.counters.alloc(std::max<size_t>(multi.size(), 3) * sizeof(int32_t));
+  if (!multi.empty())
+    HIP_CHECK(hipMemcpy(s.counters.p, multi.data(), multi.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  (void)op;
+  s.built = true;
+  return true;
 }
 
 // SpMM schedule parameters (defaults; SVDH_SPMM_LONG / _CHUNK / _SPLIT /
@@ -1245,6 +1681,57 @@ static void launch_spmv(svdh_op *op, int direction, const void *vals, const void
   Schedule &s = get_schedule(op, direction, 0);
   if (v.nrows == 0) return;
   s.partials.ensure(std::max<int32_t>(s.nchunks, 1) * sizeof(double));
+  const double bytes = (double)v.nnz * (sizeof(V) + 4) + (double)(v.nrows + 1) * 4 +
+                       (double)v.ncols * sizeof(X) + (double)v.nrows * sizeof(Y);
+  if (s.tiled.on) {
+    Schedule::Tiled &T = s.tiled;
+    TiledArgs<V, X, Y> a;
+    a.rowptr = T.rowptr.as<int32_t>();
+    a.col = T.col.as<uint16_t>();
+    a.vals = tiled_values<V>(c, T, static_cast<const V *>(vals), v.nnz);
+    a.tasks = T.tasks.as<int4>();
+    a.ranges = T.ranges.as<int4>();
+    a.x = static_cast<const X *>(x);
+    a.part = T.part.as<double>();
+    a.y = static_cast<Y *>(y);
+    a.nsrc = v.ncols;
+    a.sb = T.sb;
+    a.direct = T.nb == 1;
+    KTimer t(c, KC_SPMV, bytes);
+    hipLaunchKernelGGL((k_spmv_tiled<V, X, Y>), T.nranges, kTiledBlock, 0, c->stream, a);
+    HIP_CHECK(hipGetLastError());
+    if (T.nb > 1) {
+      hipLaunchKernelGGL((k_spmv_tiled_fin<Y>), grid_for(v.nrows), kBlock, 0, c->stream, v.nrows, (int)T.nb,
+                         (const double *)T.part.p, static_cast<Y *>(y));
+      HIP_CHECK(hipGetLastError());
+    }
+    return;
+  }
+  if (s.stream) {
+    StreamArgs<V, X, Y> a;
+    a.rowptr = v.rowptr;
+    a.colidx = v.colidx;
+    a.vals = static_cast<const V *>(vals);
+    a.colmap = v.colmap;
+    a.tasks = s.tasks.as<int4>();
+    a.partials = s.partials.as<double>();
+    a.x = static_cast<const X *>(x);
+    a.y = static_cast<Y *>(y);
+    a.ntasks = s.ntasks;
+    const int blocks = (s.ntasks + kBlock / 64 - 1) / (kBlock / 64);
+    KTimer t(c, KC_SPMV, bytes);
+    if (v.colmap)
+      hipLaunchKernelGGL((k_spmv_stream<true, V, X, Y>), blocks, kBlock, 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_spmv_stream<false, V, X, Y>), blocks, kBlock, 0, c->stream, a);
+    HIP_CHECK(hipGetLastError());
+    if (s.nslots > 0) {
+      hipLaunchKernelGGL((k_spmv_fin<Y>), (s.nslots + kBlock - 1) / kBlock, kBlock, 0, c->stream, s.nslots,
+                         s.counters.as<int32_t>(), (const double *)s.partials.p, static_cast<Y *>(y));
+      HIP_CHECK(hipGetLastError());
+    }
+    return;
+  }
   SpmvArgs<V, X, Y> a;
   a.rowptr = v.rowptr;
   a.colidx = v.colidx;
@@ -1260,8 +1747,6 @@ static void launch_spmv(svdh_op *op, int direction, const void *vals, const void
   a.long_block_begin = s.long_block_begin;
   for (int i = 0; i < kMaxBins; ++i) a.bins[i] = i < s.nbins ? s.bins[i] : BinDesc{INT32_MAX, 0, 0, 0};
   if (s.nbins == 0) a.bins[0] = BinDesc{INT32_MAX, 0, 0, 0};
-  const double bytes = (double)v.nnz * (sizeof(V) + 4) + (double)(v.nrows + 1) * 4 +
-                       (double)v.ncols * sizeof(X) + (double)v.nrows * sizeof(Y);
   KTimer t(c, KC_SPMV, bytes);
   if (v.colmap)
     hipLaunchKernelGGL((k_spmv<true, V, X, Y>), s.total_blocks, kBlock, 0, c->stream, a);
@@ -1450,17 +1935,6 @@ static Schedule::HotEnc *hot_columns(svdh_op *op, Schedule &s, int h) {
   e.h = h;
   return &e;
 }
-
-template <typename T>
-__global__ void k_gather(int64_t n, const int32_t *perm, const T *src, T *dst) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[perm[i]];
-}
-
-template <typename T> struct DtypeOf;
-template <> struct DtypeOf<double> { static constexpr int v = SVDH_F64; };
-template <> struct DtypeOf<float> { static constexpr int v = SVDH_F32; };
-template <> struct DtypeOf<bf16_t> { static constexpr int v = SVDH_BF16; };
 
 // Values of dtype V in the staged-first layout of e (gathered on first use).
 template <typename V>

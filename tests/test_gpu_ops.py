@@ -83,6 +83,59 @@ def test_spmv_f32(svdlib, name):
     check_close(y, want, bound(a64, x.astype(np.float64), EPS64) + EPS32 * np.abs(want))
 
 
+def pareto_rows(m, n, seed, dtype, mean_row=6, long_rows=3, long_len=25_000):
+    """Pareto row lengths with uniform columns, a few rows far longer than the
+    rest, about 10 % empty rows and 3 empty columns: more than 2^20 nonzeros,
+    so the SpMV takes the source-tiled path."""
+    rng = np.random.default_rng(seed)
+    lens = np.minimum((mean_row / 2 / rng.uniform(size=m) ** 0.5).astype(np.int64), 4000)
+    lens[rng.random(m) < 0.1] = 0
+    lens[rng.choice(m, long_rows, replace=False)] = long_len
+    rows = np.repeat(np.arange(m), lens)
+    cols = rng.integers(0, n, size=rows.size)
+    a = sp.csr_matrix((rng.normal(size=rows.size), (rows, cols)), shape=(m, n + 3))
+    a.sum_duplicates()
+    return a.astype(dtype)
+
+
+@pytest.mark.parametrize("shape,mean_row", [((300_000, 40_000), 6), ((150_000, 9_000), 14)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmv_source_tiled(svdlib, shape, mean_row, dtype):
+    # the source vector staged in LDS one block at a time (three blocks for
+    # A x and one block, written directly, for 9,000 columns; 8 to 19 blocks
+    # for A^T x), block partials added in order: within the dot-product bound
+    # of the FP64 oracle, and the same bits on a second call
+    a = pareto_rows(*shape, seed=11, dtype=dtype, mean_row=mean_row)
+    assert a.nnz > 1 << 20
+    print(f"case tiled {a.shape} nnz={a.nnz} dtype={np.dtype(dtype).name}", flush=True)
+    op = svdlib.CsrMatrix.from_scipy(a)
+    a64 = a.astype(np.float64)
+    ref = oracle.OracleOp(a64)
+    rng = np.random.default_rng(2)
+    for transposed in (False, True):
+        # vectors travel in the operator's dtype; FP32 outputs add one rounding
+        nin = a.shape[0] if transposed else a.shape[1]
+        x = rng.normal(size=nin).astype(dtype)
+        y = np.empty(a.shape[1] if transposed else a.shape[0], dtype=dtype)
+        op.svd_opa(x, y, transposed)
+        want = ref.opa(x.astype(np.float64), transposed)
+        b = bound(a64.T if transposed else a64, x.astype(np.float64), EPS64)
+        if dtype == np.float32:
+            b = b + EPS32 * np.abs(want)
+        check_close(y, want, b)
+        y2 = np.empty_like(y)
+        op.svd_opa(x, y2, transposed)
+        assert np.array_equal(y, y2)
+        if dtype == np.float32:
+            # FP32 values with FP64 vectors (the LAS2 FP32-SpMV mode): FP64 bound
+            from single_svdlib import _ffi
+            x64 = rng.normal(size=nin)
+            y64 = np.empty(y.size)
+            _ffi.check(_ffi.lib().svdh_spmv(op._ctx, op._handle, int(transposed), x64.ctypes.data,
+                                            y64.ctypes.data, _ffi.SVDH_F64, 0))
+            check_close(y64, ref.opa(x64, transposed), bound(a64.T if transposed else a64, x64, EPS64))
+
+
 @pytest.mark.parametrize("name", ["uniform", "wide", "skewed", "empty_rows"])
 @pytest.mark.parametrize("l", [1, 7, 60])
 def test_spmm_f64(svdlib, name, l):
