@@ -558,7 +558,10 @@ int grid_of(int64_t n, int per = kBlock, int cap = 2048) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
 
-int fused_grid(int64_t n) { return grid_of(n, kBlock * 8, 512); }
+// one element per thread up to 1024 workgroups: the Lanczos vectors are short
+// (tens of thousands), and a few elements per thread already leave the kernel
+// waiting on one load chain per element
+int fused_grid(int64_t n) { return grid_of(n, kBlock, 1024); }
 
 }  // namespace
 

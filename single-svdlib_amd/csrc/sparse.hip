@@ -471,22 +471,30 @@ __global__ __launch_bounds__(kTiledBlock) void k_spmv_tiled(TiledArgs<V, X, Y> a
   }
 }
 
-// y[r] = sum over source blocks b (in order) of part[b nout + r].
-template <typename Y>
+// y[r] = sum over source blocks b of part[b nout + r]: P lanes per row, lane
+// t adds blocks [t nb / P, (t + 1) nb / P) in order (8 loads in flight), then
+// the P lane sums are added by butterfly (a fixed order).
+template <int P, typename Y>
 __global__ void k_spmv_tiled_fin(int64_t nout, int nb, const double *part, Y *y) {
-  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (r >= nout) return;
-  double t = 0.0;
-  int b = 0;
-  for (; b + 8 <= nb; b += 8) {  // 8 loads in flight, added in block order
-    double p[8];
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t r = g / P;
+  const int t = (int)(g % P);
+  double s = 0.0;
+  if (r < nout) {
+    const int b1 = (t + 1) * nb / P;
+    int b = t * nb / P;
+    for (; b + 8 <= b1; b += 8) {
+      double p[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) p[u] = part[(int64_t)(b + u) * nout + r];
+      for (int u = 0; u < 8; ++u) p[u] = part[(int64_t)(b + u) * nout + r];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) t += p[u];
+      for (int u = 0; u < 8; ++u) s += p[u];
+    }
+    for (; b < b1; ++b) s += part[(int64_t)b * nout + r];
   }
-  for (; b < nb; ++b) t += part[(int64_t)b * nout + r];
-  y[r] = static_cast<Y>(t);
+#pragma unroll
+  for (int off = 1; off < P; off <<= 1) s += __shfl_xor(s, off, 64);
+  if (r < nout && t == 0) y[r] = static_cast<Y>(s);
 }
 
 template <bool MC, typename V, typename X, typename Y>
@@ -603,14 +611,24 @@ __device__ __forceinline__ void spmm_accum(const SpmmArgs<V, X, Y, Acc> &a, int 
           for (int i = 0; i < NV; ++i) acc[i] += vj[u] * xv[u][i];
       }
     }
-    for (; j < n; ++j) {
-      const int cj = __shfl(c, j, R);
-      const Acc vj = __shfl(v, j, R);
-      if (colok) {
-        Acc xv[NV];
-        ld_panel<Acc, X>(a.x + (int64_t)cj * a.ldx + col, xv);
+    if (j < n) {
+      // the last n - j (< U) entries as one batch: lanes past the end hold
+      // column 0 and value 0, and their products are dropped by a select
+      int cj[U];
+      Acc vj[U];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) acc[i] += vj * xv[i];
+      for (int u = 0; u < U; ++u) {
+        cj[u] = __shfl(c, j + u, R);
+        vj[u] = __shfl(v, j + u, R);
+      }
+      if (colok) {
+        Acc xv[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; ++u) ld_panel<Acc, X>(a.x + (int64_t)cj[u] * a.ldx + col, xv[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int i = 0; i < NV; ++i) acc[i] += j + u < n ? vj[u] * xv[u][i] : Acc(0);
       }
     }
   }
@@ -885,14 +903,24 @@ __device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, 
           for (int i = 0; i < NV; ++i) acc[i] += vj[u] * xv[u][i];
       }
     }
-    for (; j < n; ++j) {
-      const int cj = __shfl(c, j, R);
-      const Acc vj = __shfl(v, j, R);
-      if (colok) {
-        Acc xv[NV];
-        ld_panel<Acc, X>(src + (int64_t)cj * a.ldx + col, xv);
+    if (j < n) {
+      // the last n - j (< U) entries as one batch; lanes past the end hold
+      // slot/column 0 and value 0, and their products are dropped by a select
+      int cj[U];
+      Acc vj[U];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) acc[i] += vj * xv[i];
+      for (int u = 0; u < U; ++u) {
+        cj[u] = __shfl(c, j + u, R);
+        vj[u] = __shfl(v, j + u, R);
+      }
+      if (colok) {
+        Acc xv[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; ++u) ld_panel<Acc, X>(src + (int64_t)cj[u] * a.ldx + col, xv[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int i = 0; i < NV; ++i) acc[i] += j + u < n ? vj[u] * xv[u][i] : Acc(0);
       }
     }
   }
@@ -1701,8 +1729,12 @@ static void launch_spmv(svdh_op *op, int direction, const void *vals, const void
     hipLaunchKernelGGL((k_spmv_tiled<V, X, Y>), T.nranges, kTiledBlock, 0, c->stream, a);
     HIP_CHECK(hipGetLastError());
     if (T.nb > 1) {
-      hipLaunchKernelGGL((k_spmv_tiled_fin<Y>), grid_for(v.nrows), kBlock, 0, c->stream, v.nrows, (int)T.nb,
-                         (const double *)T.part.p, static_cast<Y *>(y));
+      if (T.nb >= 8)
+        hipLaunchKernelGGL((k_spmv_tiled_fin<4, Y>), (int)((4 * v.nrows + kBlock - 1) / kBlock), kBlock, 0,
+                           c->stream, v.nrows, (int)T.nb, (const double *)T.part.p, static_cast<Y *>(y));
+      else
+        hipLaunchKernelGGL((k_spmv_tiled_fin<1, Y>), (int)((v.nrows + kBlock - 1) / kBlock), kBlock, 0,
+                           c->stream, v.nrows, (int)T.nb, (const double *)T.part.p, static_cast<Y *>(y));
       HIP_CHECK(hipGetLastError());
     }
     return;
