@@ -352,6 +352,12 @@ void panel_gemm_dev(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, in
 void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
                 const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
                 double beta, const float *Zin = nullptr, int64_t ldzin = 0);
+// panel_mfma plus the FP64 Gram Z^T Z of its output, accumulated in the same
+// kernel: the equivalent of panel_mfma followed by gram_async(Z, gram_off,
+// gram_ev) without reading Z again.
+void panel_mfma_gram(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
+                     const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
+                     double beta, const float *Zin, int64_t ldzin, size_t gram_off, hipEvent_t gram_ev = nullptr);
 // C (k1 x k2, FP64 row-major, device) = X1^T X2 for row-major panels (k1, k2 <= 64).
 void gram_cross(Ctx *c, int64_t m, int64_t k1, const void *X1, int64_t ld1, int64_t k2, const void *X2, int64_t ld2,
                 int dt, double *C_dev);

@@ -644,17 +644,19 @@ static void launch_gram(Ctx *c, bool same, int nb, int64_t m, int l, const T *Y,
 //   Z[r, 0:lo] = beta Z[r, 0:lo] + sum_p Y_p[r, 0:li_p] (alpha_p M_p)
 // for up to two terms, lo, li_p <= 64. v_mfma_f32_16x16x4 takes A[l % 16][l / 16]
 // and B[l / 16][l % 16] per lane and returns D[4 (l / 16) + v][l % 16] (lane
-// layout checked on gfx950). A workgroup stages alpha_p M_p (FP32, zero
-// padded to 64 x 64) in LDS once, then strides over 32-row tiles: the tile of
-// every Y_p is read into LDS before any row of Z is written, so Z may be Y_0
-// when beta = 0. Wave w computes rows 16 (w % 2) .. + 16, columns
-// 32 (w / 2) .. + 32 of the tile.
+// layout checked on gfx950). A workgroup (eight waves, one per CU) stages
+// alpha_p M_p (FP32, zero padded to 64 x 64) in LDS once; each wave then
+// strides over 16-row blocks, with the Y_p rows of a block in registers (the
+// next block's loaded during this one's products) and the M_p operands read
+// from LDS. A wave reads all rows of a block before writing any, so Z may be
+// Y_0 when beta = 0.
 // ------------------------------------------------------------------------
 typedef float f4_t __attribute__((ext_vector_type(4)));
 #ifndef SVDH_PANEL_BLOCK
-#define SVDH_PANEL_BLOCK 256
+#define SVDH_PANEL_BLOCK 512
 #endif
 constexpr int kPanelBlock = SVDH_PANEL_BLOCK;  // threads per workgroup of the panel products
+constexpr int kMsStride = 68;  // LDS row of the staged factors (64 + 4: rows 4 apart start 16 banks apart)
 
 struct PanelTerm {
   const float *y;
@@ -667,9 +669,9 @@ struct PanelTerm {
 // One wave per 16-row block and all 64 output columns. The contraction index
 // k is taken in groups of 16: in step s of group g, lane l supplies
 // Y[row l % 16][16 g + 4 (l / 16) + s] (component s of one 16-byte load) and
-// alpha M[16 g + 4 (l / 16) + s][16 q + l % 16] (kept in registers for the
-// whole kernel), so a wave reads whole panel rows and each lane's A values
-// come from one vector load. A wave reads all of its rows before writing any,
+// alpha M[16 g + 4 (l / 16) + s][16 q + l % 16] (the four q of one 16-byte LDS
+// read), so a wave reads whole panel rows and each lane's A values come from
+// one vector load. A wave reads all of its rows before writing any,
 // so Z may be Y_0 when beta = 0.
 template <int NP>
 __device__ __forceinline__ void panel_rows_load(const PanelTerm &t0, const PanelTerm &t1, int64_t rows, int64_t row,
@@ -695,37 +697,51 @@ __device__ __forceinline__ void panel_rows_load(const PanelTerm &t0, const Panel
 // RESID: the product is not stored; its column sums of squares (FP64) are
 // accumulated instead and each workgroup writes its lo partials to
 // part[blockIdx.x * lo ...] (the adaptive residual check).
-template <int NP, bool RESID>
-__global__ __launch_bounds__(kPanelBlock) void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
+// GRAM: the FP64 Gram Z^T Z of the stored (FP32) output is accumulated on the
+// FP64 matrix cores from the registers that hold it: for row v of each lane's
+// four, v_mfma_f64_16x16x4 with A[i][k] = Z[r0 + 4 k + v][16 bi + i] and
+// B[k][j] = Z[r0 + 4 k + v][16 bj + j] (lane (k, i) = (l / 16, l % 16) holds
+// exactly those) adds the block (bi, bj) of the 16 rows, for bi <= bj. The
+// waves are summed in LDS in wave order and each workgroup writes its
+// lo x lo partial (both triangles) to part[blockIdx.x * lo * lo ...].
+template <int NP, bool RESID, bool GRAM = false>
+__global__ __launch_bounds__(kPanelBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
                                                       int64_t ldz, const float *zin, int64_t ldzin, float beta,
                                                       double *part) {
-  __shared__ float ms[NP][64 * 64];
+  // alpha_p M_p in LDS as [k][rl][q] (element M[k][16 q + rl] at
+  // k * kMsStride + 4 rl + q): a lane's four column blocks for one k are one
+  // 16-byte read. The B operands are read from LDS at each step, which leaves
+  // registers for more than one wave per SIMD.
+  __shared__ float ms[NP][64 * kMsStride];
   for (int p = 0; p < NP; ++p) {
     const PanelTerm &t = p == 0 ? t0 : t1;
     for (int e = threadIdx.x; e < 64 * 64; e += kPanelBlock) {
       const int k = e >> 6, col = e & 63;
-      ms[p][e] = (k < t.li && col < lo) ? (float)(t.alpha * t.m[(int64_t)k * lo + col]) : 0.f;
+      ms[p][k * kMsStride + 4 * (col & 15) + (col >> 4)] =
+          (k < t.li && col < lo) ? (float)(t.alpha * t.m[(int64_t)k * lo + col]) : 0.f;
     }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int kq = lane >> 4, rl = lane & 15;
-  float bm[NP][4][4][4];  // [term][group][step][column block]
-#pragma unroll
-  for (int p = 0; p < NP; ++p)
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-#pragma unroll
-      for (int st = 0; st < 4; ++st)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) bm[p][g][st][q] = ms[p][(16 * g + 4 * kq + st) * 64 + 16 * q + rl];
   const int64_t nblk = (rows + 15) / 16;
   const int64_t wave = (int64_t)blockIdx.x * (kPanelBlock / 64) + (threadIdx.x >> 6);
   const int64_t nwaves = (int64_t)gridDim.x * (kPanelBlock / 64);
   f4_t ya[NP][4], yn[NP][4];
   double csq[4] = {0.0, 0.0, 0.0, 0.0};  // RESID: columns 16 q + rl
+  d4_t gacc[4][4];                       // GRAM: blocks (bi, bj), bi <= bj
+  if (GRAM)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = i; j < 4; ++j) gacc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
   if (wave < nblk) panel_rows_load<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
   for (int64_t b = wave; b < nblk; b += nwaves) {
+    // an offset of zero the compiler cannot see through: the B reads below
+    // stay in the loop instead of being hoisted into registers
+    int lds0;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(lds0));
     const int64_t r0 = b * 16;
     const bool more = b + nwaves < nblk;
     if (more) panel_rows_load<NP>(t0, t1, rows, (b + nwaves) * 16 + rl, kq, yn);  // next block in flight
@@ -746,10 +762,14 @@ __global__ __launch_bounds__(kPanelBlock) void k_panel_mfma(int64_t rows, int lo
 #pragma unroll
       for (int g = 0; g < 4; ++g)
 #pragma unroll
-        for (int st = 0; st < 4; ++st)
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], bm[p][g][st][q], acc[q], 0, 0, 0);
+        for (int st = 0; st < 4; ++st) {
+          const float4 b =
+              *reinterpret_cast<const float4 *>(&ms[p][(16 * g + 4 * kq + st) * kMsStride + 4 * rl + lds0]);
+          acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.x, acc[0], 0, 0, 0);
+          acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.y, acc[1], 0, 0, 0);
+          acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.z, acc[2], 0, 0, 0);
+          acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.w, acc[3], 0, 0, 0);
+        }
     if (RESID) {
       // rows past the end were loaded as zeros, so they add nothing
 #pragma unroll
@@ -757,20 +777,64 @@ __global__ __launch_bounds__(kPanelBlock) void k_panel_mfma(int64_t rows, int lo
 #pragma unroll
         for (int v = 0; v < 4; ++v) csq[q] += (double)acc[q][v] * (double)acc[q][v];
     } else {
+      float zv[4][4];  // the stored values; zero outside the panel (GRAM)
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int64_t gr = r0 + 4 * kq + v;
           const int c = 16 * q + rl;
-          if (gr < rows && c < lo) z[gr * ldz + c] = (beta != 0.f ? beta * zold[q][v] : 0.f) + acc[q][v];
+          const float val = (beta != 0.f ? beta * zold[q][v] : 0.f) + acc[q][v];
+          const bool ok = gr < rows && c < lo;
+          if (ok) z[gr * ldz + c] = val;
+          zv[q][v] = ok ? val : 0.f;
         }
+      if (GRAM)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = i; j < 4; ++j)
+              gacc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)zv[i][v], (double)zv[j][v], gacc[i][j], 0, 0,
+                                                                0);
     }
     if (more)
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int g = 0; g < 4; ++g) ya[p][g] = yn[p][g];
+  }
+  if (GRAM) {
+    // waves in order into LDS: gred[(blk * 4 + v) * 64 + lane], blk = 4 i + j
+    __shared__ double gred[16 * 4 * 64];
+    const int wv = threadIdx.x >> 6;
+    for (int w = 0; w < kPanelBlock / 64; ++w) {
+      if (wv == w)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = i; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              double *q = gred + ((i * 4 + j) * 4 + v) * 64 + lane;
+              *q = (w == 0 ? 0.0 : *q) + gacc[i][j][v];
+            }
+      __syncthreads();
+    }
+    double *out = part + (int64_t)blockIdx.x * lo * lo;
+    for (int e = threadIdx.x; e < 16 * 4 * 64; e += kPanelBlock) {
+      const int blk = e >> 8, v = (e >> 6) & 3, ln = e & 63;
+      const int bi = blk >> 2, bj = blk & 3;
+      if (bj < bi) continue;
+      // lane ln holds D[ln / 16 + 4 v][ln % 16] of block (bi, bj)
+      const int gi = 16 * bi + (ln >> 4) + 4 * v, gj = 16 * bj + (ln & 15);
+      if (gi < lo && gj < lo) {
+        const double x = gred[e];
+        out[(int64_t)gi * lo + gj] = x;
+        out[(int64_t)gj * lo + gi] = x;
+      }
+    }
   }
   if (RESID) {
     // fixed-order sum over the four lane groups of each wave, then the waves
@@ -787,15 +851,32 @@ __global__ __launch_bounds__(kPanelBlock) void k_panel_mfma(int64_t rows, int lo
   }
 }
 
-void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
-                const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
-                double beta, const float *Zin, int64_t ldzin) {
-  if (rows == 0 || lo == 0) return;
+// Workgroups per CU of the panel kernels (SVDH_PANEL_WG, default 1): the
+// staged factors are read once per workgroup; a workgroup of eight waves
+// gives two waves per SIMD (the register budget of the kernels), so one
+// wave's loads overlap the other's products.
+static int panel_wg_per_cu() {
+  static const int per_cu = [] {
+    const char *e = std::getenv("SVDH_PANEL_WG");
+    return e ? std::max(1, std::atoi(e)) : 1;
+  }();
+  return per_cu;
+}
+
+// gram_off >= 0: the Gram Z^T Z of the output (lo x lo, FP64) is accumulated by
+// the same kernel and summed into the pinned landing area c->h_small +
+// gram_off, after which gram_ev (c->small_ev when null) is recorded.
+static void panel_mfma_impl(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
+                            const int64_t *li, const double *const *M_dev, const double *alpha, float *Z,
+                            int64_t ldz, double beta, const float *Zin, int64_t ldzin, int64_t gram_off,
+                            hipEvent_t gram_ev) {
   if (!Zin) {
     Zin = Z;
     ldzin = ldz;
   }
   if (lo > 64 || nterms < 1 || nterms > 2) fail(SVDH_ERR_ARG, "panel_mfma: lo <= 64 and one or two terms");
+  const bool gram = gram_off >= 0;
+  if (gram && (size_t)gram_off + (size_t)lo * lo > Ctx::kSmall) fail(SVDH_ERR_ARG, "Gram landing area too small");
   PanelTerm t[2] = {};
   double bytes = (double)rows * lo * 4.0 * (beta != 0.0 ? 2.0 : 1.0);
   for (int p = 0; p < nterms; ++p) {
@@ -805,18 +886,56 @@ void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const
   }
   for (int p = 0; p < nterms; ++p)
     if (ldy[p] % 4) fail(SVDH_ERR_ARG, "panel_mfma: leading dimensions must be multiples of 4");
-  KTimer tm(c, KC_DENSE, bytes);
   const int64_t nblk = (rows + 15) / 16;
-  // one workgroup per CU: the staged factors are read once per workgroup
   constexpr int W = kPanelBlock / 64;
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + W - 1) / W, (int64_t)c->ncu));
-  if (nterms == 1)
-    hipLaunchKernelGGL((k_panel_mfma<1, false>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz,
-                       Zin, ldzin, (float)beta, nullptr);
-  else
-    hipLaunchKernelGGL((k_panel_mfma<2, false>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz,
-                       Zin, ldzin, (float)beta, nullptr);
-  HIP_CHECK(hipGetLastError());
+  const int g =
+      (int)std::max<int64_t>(1, std::min<int64_t>((nblk + W - 1) / W, (int64_t)panel_wg_per_cu() * c->ncu));
+  double *part = nullptr;
+  if (gram) {
+    c->red_part.ensure(std::max<size_t>((size_t)g * lo * lo * sizeof(double), c->red_part.bytes));
+    part = c->red_part.as<double>();
+  }
+  {
+    KTimer tm(c, KC_DENSE, bytes);
+    if (nterms == 1 && !gram)
+      hipLaunchKernelGGL((k_panel_mfma<1, false>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz,
+                         Zin, ldzin, (float)beta, nullptr);
+    else if (nterms == 1)
+      hipLaunchKernelGGL((k_panel_mfma<1, false, true>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z,
+                         ldz, Zin, ldzin, (float)beta, part);
+    else if (!gram)
+      hipLaunchKernelGGL((k_panel_mfma<2, false>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz,
+                         Zin, ldzin, (float)beta, nullptr);
+    else
+      hipLaunchKernelGGL((k_panel_mfma<2, false, true>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z,
+                         ldz, Zin, ldzin, (float)beta, part);
+    HIP_CHECK(hipGetLastError());
+    if (gram) {
+      const int64_t ll = lo * lo;
+      hipLaunchKernelGGL(k_gram_reduce, reduce_grid(ll), kBlock, 0, c->stream, g, ll, (const double *)part,
+                         c->d_small + gram_off);
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+  if (gram) HIP_CHECK(hipEventRecord(gram_ev ? gram_ev : c->small_ev, c->stream));
+}
+
+void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
+                const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
+                double beta, const float *Zin, int64_t ldzin) {
+  if (rows == 0 || lo == 0) return;
+  panel_mfma_impl(c, rows, lo, nterms, Y, ldy, li, M_dev, alpha, Z, ldz, beta, Zin, ldzin, -1, nullptr);
+}
+
+void panel_mfma_gram(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
+                     const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
+                     double beta, const float *Zin, int64_t ldzin, size_t gram_off, hipEvent_t gram_ev) {
+  if (rows == 0 || lo == 0) {
+    gram_async(c, rows, lo, Z, ldz, SVDH_F32, gram_off, gram_ev);  // the empty Gram, through the usual path
+    return;
+  }
+  panel_mfma_impl(c, rows, lo, nterms, Y, ldy, li, M_dev, alpha, Z, ldz, beta, Zin, ldzin, (int64_t)gram_off,
+                  gram_ev);
 }
 
 __global__ void k_fetch(const uint4 *src, uint4 *dst, int64_t n16) {
@@ -953,7 +1072,8 @@ void panel_residual(Ctx *c, int64_t m, int64_t w, int64_t w2, int64_t k, const v
     const double *Md = static_cast<const double *>(c->upload(both.data(), both.size() * sizeof(double)));
     const int64_t nblk = (m + 15) / 16;
     constexpr int W = kPanelBlock / 64;
-    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((nblk + W - 1) / W, (int64_t)c->ncu));
+    const int nb =
+        (int)std::max<int64_t>(1, std::min<int64_t>((nblk + W - 1) / W, (int64_t)panel_wg_per_cu() * c->ncu));
     c->red_part.ensure(std::max<size_t>(c->red_part.bytes, (size_t)nb * k * sizeof(double)));
     double *part = c->red_part.as<double>();
     const PanelTerm t0{static_cast<const float *>(Y1), ld, (int)w, Md, 1.0};

@@ -330,6 +330,9 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         PM = mat_mul(Mp.data(), Yp.w, Y.w, Mq.data(), wq, wq);  // M_{t-1} M_t
       pm_ready = false;
       const double *dMq = nullptr, *dPM = nullptr;
+      const bool check = adaptive && (iters_run >= next_check || iters_run >= max_power_iters);
+      const bool want_gz = final_fixed || check || eig_step;
+      bool gz_done = false;
       if (mfma) {
         std::vector<double> both(Mq);
         both.insert(both.end(), PM.begin(), PM.end());
@@ -339,14 +342,19 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         const int64_t ldin[1] = {Z.ld}, liin[1] = {Y.w};
         const double *min_[1] = {dMq};
         const double al[1] = {1.0};
-        panel_mfma(ctx, n, wq, 1, yin, ldin, liin, min_, al, static_cast<float *>(Z.p()), Z.ld, 0.0);
+        if (want_gz) {  // the Gram of Z from the same kernel
+          panel_mfma_gram(ctx, n, wq, 1, yin, ldin, liin, min_, al, static_cast<float *>(Z.p()), Z.ld, 0.0, nullptr,
+                          0, 0);
+          gz_done = true;
+        } else {
+          panel_mfma(ctx, n, wq, 1, yin, ldin, liin, min_, al, static_cast<float *>(Z.p()), Z.ld, 0.0);
+        }
       } else {
         // Z holds A^T Y; Z = A^T Q = (A^T Y) Mq
         panel_apply(ctx, n, Y.w, wq, Z.p(), Z.ld, Z.dt, Mq.data(), Z.p(), Z.ld, Z.dt);
       }
       Z.w = wq;
-      const bool check = adaptive && (iters_run >= next_check || iters_run >= max_power_iters);
-      if (final_fixed || check || eig_step) gram_async(ctx, n, wq, Z.p(), Z.ld, Z.dt, 0);
+      if (want_gz && !gz_done) gram_async(ctx, n, wq, Z.p(), Z.ld, Z.dt, 0);
       if (final_fixed) {
         R.gram_wait(wq, GZ);
         desc_eig(wq, GZ, lam, W);
@@ -363,15 +371,18 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       //                      = A Z - c Y Mq - (c^2/4) Y_{t-1} (M_{t-1} M_t),
       // in place in Yn, or into Ys at a check.
       Panel *next = check ? &Ys : &Yn;
-      auto build_next = [&](bool mom, double coef) {
+      // Returns whether the Gram of the iterate was accumulated on the way
+      // (matrix-core panels: landing offset goff, event gev).
+      auto build_next = [&](bool mom, double coef, size_t goff, hipEvent_t gev) -> bool {
         next->w = Yn.w;
         if (mfma) {
           const float *yin[2] = {static_cast<const float *>(Y.p()), static_cast<const float *>(Yp.p())};
           const int64_t ldin[2] = {Y.ld, Yp.ld}, liin[2] = {Y.w, Yp.w};
           const double *min_[2] = {dMq, dPM};
           const double al[2] = {-shift, -coef * shift * shift};
-          panel_mfma(ctx, m, wq, mom ? 2 : 1, yin, ldin, liin, min_, al, static_cast<float *>(next->p()), next->ld,
-                     1.0, static_cast<const float *>(Yn.p()), Yn.ld);
+          panel_mfma_gram(ctx, m, wq, mom ? 2 : 1, yin, ldin, liin, min_, al, static_cast<float *>(next->p()),
+                          next->ld, 1.0, static_cast<const float *>(Yn.p()), Yn.ld, goff, gev);
+          return true;
         } else {
           if (next != &Yn)
             HIP_CHECK(hipMemcpyAsync(next->p(), Yn.p(), (size_t)m * Yn.ld * dtype_size(pdt), hipMemcpyDeviceToDevice,
@@ -384,19 +395,22 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
                            pdt);
           }
         }
+        return false;
       };
       bool mom_used = false;
+      bool next_gram = false;  // the Gram of the next iterate is already enqueued
       const bool shifted = adaptive && shift > 0.0;
       if (shifted) {
         const bool restart = std::fabs(shift - shift_used) > restart_tol * shift;
         mom_used = momentum && have_prev && !restart && !PM.empty();
-        build_next(mom_used, mom_run == 0 ? 0.5 : 0.25);
+        next_gram = check ? build_next(mom_used, mom_run == 0 ? 0.5 : 0.25, kGramOff2, ctx->small_ev2)
+                          : build_next(mom_used, mom_run == 0 ? 0.5 : 0.25, 0, nullptr);
       } else if (check) {
         next = &Yn;  // unshifted: A Z itself, read-only from here on
       }
       if (check) {
         // the speculative next step: Gram of Y_{t+1} and A^T Y_{t+1}
-        gram_async(ctx, m, next->w, next->p(), next->ld, next->dt, kGramOff2, ctx->small_ev2);
+        if (!next_gram) gram_async(ctx, m, next->w, next->p(), next->ld, next->dt, kGramOff2, ctx->small_ev2);
         R.mult(1, *next, Z2, low);
         desc_eig(wq, GZ, lam, W);
         const int64_t k = std::min(target_rank, std::min(wq, n));
@@ -452,11 +466,12 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         if (redo) {
           // the iterate itself depends on the momentum flag only; its A^T
           // product also on the BF16 ladder
+          bool rebuilt = false;
           if (shifted && mom_used && !momentum) {
-            build_next(false, 0.0);
+            rebuilt = build_next(false, 0.0, kGramOff2, ctx->small_ev2);
             mom_used = false;
           }
-          gram_async(ctx, m, next->w, next->p(), next->ld, next->dt, kGramOff2, ctx->small_ev2);
+          if (!rebuilt) gram_async(ctx, m, next->w, next->p(), next->ld, next->dt, kGramOff2, ctx->small_ev2);
           R.mult(1, *next, Z2, low);
           R.fallbacks = fb;
           R.finish_normaliser(*next, Mq_next, wq_next, kGramOff2, ctx->small_ev2);
@@ -495,7 +510,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
       }
       if (!adaptive && R.bf16) low = iters_run < (n_power_iters + 1) / 2;
       const bool next_final = !adaptive && iters_run >= n_power_iters;
-      gram_async(ctx, Y.rows, Y.w, Y.p(), Y.ld, Y.dt, 0);
+      if (!next_gram) gram_async(ctx, Y.rows, Y.w, Y.p(), Y.ld, Y.dt, 0);
       R.mult(1, Y, Z, low && !next_final);
       R.finish_normaliser(Y, Mq, wq);
     }
