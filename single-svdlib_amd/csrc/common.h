@@ -319,6 +319,11 @@ void fetch_from_host(Ctx *c, const void *src, void *dst, size_t n);
 // caller waits on the event before reading.
 void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int ydt, size_t off,
                 hipEvent_t ev = nullptr);
+// Grams of count panels P[b] (rows[b] x l, l <= 64, shared ldy and dtype) in
+// one launch: Gram b lands at c->h_small + off + b * l * l; ev (c->small_ev
+// when null) is recorded after the last.
+void gram_batched_async(Ctx *c, int64_t count, const int64_t *rows, int64_t l, const void *const *P, int64_t ldy,
+                        int ydt, size_t off, hipEvent_t ev = nullptr);
 // Z (m x lo) = Y (m x li) * M (li x lo, FP64 row-major, host), in place or to
 // another panel. Element (i, c) of Z is stored at Z[i * ldz + c * zcs]
 // (zcs = 1: row-major; ldz = 1, zcs = ld: the transposed, column-major store).

@@ -214,10 +214,10 @@ __device__ __forceinline__ void load_tile(const T *Y, int64_t ldy, int64_t r0, i
 // G[e] = sum over the nb partials part[b * ll + e]. 16 outputs per
 // workgroup, 16 lanes per output each summing every 16th partial (two
 // chains), then the 16 lane sums in order: a fixed order for every launch.
-__global__ __launch_bounds__(kBlock) void k_gram_reduce(int nb, int64_t ll, const double *part, double *G) {
+__device__ __forceinline__ void gram_reduce_body(int nb, int64_t ll, const double *part, double *G, int64_t bx) {
   __shared__ double red[16][17];
   const int o = threadIdx.x & 15, q = threadIdx.x >> 4;
-  const int64_t e = blockIdx.x * (int64_t)16 + o;
+  const int64_t e = bx * 16 + o;
   double s0 = 0.0, s1 = 0.0;
   if (e < ll) {
     int b = q;
@@ -235,6 +235,10 @@ __global__ __launch_bounds__(kBlock) void k_gram_reduce(int nb, int64_t ll, cons
     for (int i = 0; i < 16; ++i) t += red[i][o];
     G[e] = t;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gram_reduce(int nb, int64_t ll, const double *part, double *G) {
+  gram_reduce_body(nb, ll, part, G, blockIdx.x);
 }
 
 static int reduce_grid(int64_t ll) { return (int)((ll + 15) / 16); }
@@ -281,36 +285,64 @@ template <> struct Quad<bf16_t> {
   }
 };
 
+// Two consecutive columns per lane (panels of width <= 32, two strided
+// column blocks of 16).
+template <typename T> struct Pair;
+template <> struct Pair<float> {
+  float2 v;
+  __device__ __forceinline__ void load(const float *p) { v = *reinterpret_cast<const float2 *>(p); }
+  __device__ __forceinline__ void zero() { v = make_float2(0.f, 0.f); }
+  __device__ __forceinline__ double get(int b) const { return (double)(b == 0 ? v.x : v.y); }
+};
+template <> struct Pair<double> {
+  double2 v;
+  __device__ __forceinline__ void load(const double *p) { v = *reinterpret_cast<const double2 *>(p); }
+  __device__ __forceinline__ void zero() { v = make_double2(0.0, 0.0); }
+  __device__ __forceinline__ double get(int b) const { return b == 0 ? v.x : v.y; }
+};
+template <> struct Pair<bf16_t> {
+  uint32_t v;
+  __device__ __forceinline__ void load(const bf16_t *p) { v = *reinterpret_cast<const uint32_t *>(p); }
+  __device__ __forceinline__ void zero() { v = 0u; }
+  __device__ __forceinline__ double get(int b) const {
+    return (double)__uint_as_float(b ? (v & 0xffff0000u) : (v << 16));
+  }
+};
+template <typename T, int NB> struct GramVec { using type = Quad<T>; };
+template <typename T> struct GramVec<T, 2> { using type = Pair<T>; };
+
 // Waves per workgroup: 8 for the diagonal launch (two waves per SIMD), 4 for
 // the off-diagonal one, whose 16 accumulators need the whole register file.
 template <bool SAME> constexpr int gram_waves() { return SAME ? kGramWaves : 4; }
 
-template <typename T, bool SAME>
-__global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m, int l, const T *Y, int64_t ldy,
-                                                                     int l2, const T *Y2, int64_t ld2, int c0,
-                                                                     int d0, double *part, int64_t pstride,
-                                                                     int out_ld, int mirror) {
+// Workgroup bid of nbk over the rows of one panel; writes that workgroup's
+// partial to out.
+template <typename T, bool SAME, int NB = 4>
+__device__ __forceinline__ void gram_mfma_body(int64_t bid, int64_t nbk, int64_t m, int l, const T *Y, int64_t ldy,
+                                               int l2, const T *Y2, int64_t ld2, int c0, int d0, double *out,
+                                               int out_ld, int mirror) {
   constexpr int W = gram_waves<SAME>();
   __shared__ double red[16 * 256];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int kr = lane >> 4, cc = lane & 15;
-  const int64_t nw = (int64_t)gridDim.x * W, gw = (int64_t)blockIdx.x * W + wv;
+  const int64_t nw = nbk * W, gw = bid * W + wv;
   const int64_t steps = (m + 3) / 4;
   const int64_t per = (steps + nw - 1) / nw;
   const int64_t s0 = gw * per, s1 = min<int64_t>(steps, s0 + per);
-  const int colc = c0 + 4 * cc, cold = d0 + 4 * cc;
-  // columns at or past l read as zero; a quad starting below l stays inside
-  // the row's leading dimension (a multiple of 4 that is >= l)
+  const int colc = c0 + NB * cc, cold = d0 + NB * cc;
+  // columns at or past l read as zero; a quad (pair) starting below l stays
+  // inside the row's leading dimension (a multiple of 4 that is >= l)
   const bool okc = colc < l, okd = cold < l2;
   const int limc = l - colc, limd = l2 - cold;
-  d4_t acc[4][4];
+  d4_t acc[NB][NB];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < NB; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < NB; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
   constexpr int G = 4;  // row steps per group; the next group is loaded during this one's MFMAs
-  Quad<T> qc[G], qd[G], nc[G], nd[G];
-  auto load = [&](int64_t st0, Quad<T> (&xc)[G], Quad<T> (&xd)[G]) {
+  using Vec = typename GramVec<T, NB>::type;
+  Vec qc[G], qd[G], nc[G], nd[G];
+  auto load = [&](int64_t st0, Vec (&xc)[G], Vec (&xd)[G]) {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int64_t r = (st0 + g) * 4 + kr;
@@ -321,19 +353,19 @@ __global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m
       }
     }
   };
-  auto mfmas = [&](const Quad<T> (&xc)[G], const Quad<T> (&xd)[G]) {
+  auto mfmas = [&](const Vec (&xc)[G], const Vec (&xd)[G]) {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-      double a[4], b[4];
+      double a[NB], b[NB];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NB; ++q) {
         a[q] = q < limc ? xc[g].get(q) : 0.0;
         b[q] = SAME ? a[q] : (q < limd ? xd[g].get(q) : 0.0);
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < NB; ++i)
 #pragma unroll
-        for (int j = (SAME ? i : 0); j < 4; ++j)
+        for (int j = (SAME ? i : 0); j < NB; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   };
@@ -346,32 +378,67 @@ __global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m
     if (st + 2 * G < s1) load(st + 2 * G, qc, qd);
     mfmas(nc, nd);
   }
-  // sum the waves in order through LDS: red[(i * 4 + j) * 256 + v * 64 + lane]
+  // sum the waves in order through LDS: red[(i * NB + j) * 256 + v * 64 + lane]
   for (int w = 0; w < W; ++w) {
     if (wv == w)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < NB; ++i)
 #pragma unroll
-        for (int j = (SAME ? i : 0); j < 4; ++j)
+        for (int j = (SAME ? i : 0); j < NB; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            double *q = red + (i * 4 + j) * 256 + v * 64 + lane;
+            double *q = red + (i * NB + j) * 256 + v * 64 + lane;
             *q = (w == 0 ? 0.0 : *q) + acc[i][j][v];
           }
     __syncthreads();
   }
-  double *out = part + (int64_t)blockIdx.x * pstride;
-  for (int e = threadIdx.x; e < 16 * 256; e += 64 * W) {
+  for (int e = threadIdx.x; e < NB * NB * 256; e += 64 * W) {
     const int blk = e >> 8, v = (e >> 6) & 3, ln = e & 63;
-    const int b1 = blk >> 2, b2 = blk & 3;
+    const int b1 = blk / NB, b2 = blk % NB;
     if (SAME && b2 < b1) continue;
-    const int gi = c0 + 4 * ((ln >> 4) + 4 * v) + b1, gj = d0 + 4 * (ln & 15) + b2;
+    const int gi = c0 + NB * ((ln >> 4) + 4 * v) + b1, gj = d0 + NB * (ln & 15) + b2;
     if (gi < l && gj < l2) {
       const double x = red[e];
       out[(int64_t)gi * out_ld + gj] = x;
       if (mirror) out[(int64_t)gj * out_ld + gi] = x;
     }
   }
+}
+
+template <typename T, bool SAME, int NB = 4>
+__global__ __launch_bounds__(64 * gram_waves<SAME>()) void k_gram_mfma(int64_t m, int l, const T *Y, int64_t ldy,
+                                                                     int l2, const T *Y2, int64_t ld2, int c0,
+                                                                     int d0, double *part, int64_t pstride,
+                                                                     int out_ld, int mirror) {
+  gram_mfma_body<T, SAME, NB>(blockIdx.x, gridDim.x, m, l, Y, ldy, l2, Y2, ld2, c0, d0,
+                              part + (int64_t)blockIdx.x * pstride, out_ld, mirror);
+}
+
+// Grams of many panels (width l <= 64) in one launch: panel b = blockIdx.y
+// takes workgroups 0 .. nblk - 1 of the x dimension, and its workgroup
+// partials go to part[(poff + workgroup) * l * l ...].
+struct GramPanel {
+  const void *y;
+  int64_t rows;
+  int64_t poff;
+  int64_t nblk;
+};
+
+template <typename T, int NB>
+__global__ __launch_bounds__(64 * gram_waves<true>()) void k_gram_mfma_batched(const GramPanel *pan, int l,
+                                                                             int64_t ldy, double *part) {
+  const GramPanel g = pan[blockIdx.y];
+  if ((int64_t)blockIdx.x >= g.nblk) return;
+  const T *Y = static_cast<const T *>(g.y);
+  gram_mfma_body<T, true, NB>(blockIdx.x, g.nblk, g.rows, l, Y, ldy, l, Y, ldy, 0, 0,
+                              part + (g.poff + (int64_t)blockIdx.x) * l * l, l, 1);
+}
+
+// k_gram_reduce for panel b = blockIdx.y: G + b * l * l = sum of its partials.
+__global__ __launch_bounds__(kBlock) void k_gram_reduce_batched(const GramPanel *pan, int64_t ll, const double *part,
+                                                                double *G) {
+  const GramPanel g = pan[blockIdx.y];
+  gram_reduce_body((int)g.nblk, ll, part + g.poff * ll, G + (int64_t)blockIdx.y * ll, blockIdx.x);
 }
 
 // Z[r, 0:lo] = Y[r, 0:li] M (li x lo, row-major) on 64-row tiles; the
@@ -627,10 +694,15 @@ void basis_combine(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, c
   HIP_CHECK(hipGetLastError());
 }
 
+// Panels of width <= 32 take the two-block variant (3 matrix-core products
+// per 4-row step instead of 10); wider ones 64-column groups of four blocks.
 template <typename T>
 static void launch_gram(Ctx *c, bool same, int nb, int64_t m, int l, const T *Y, int64_t ldy, int c0, int d0,
                         double *part) {
-  if (same)
+  if (same && l <= 32)
+    hipLaunchKernelGGL((k_gram_mfma<T, true, 2>), nb, 64 * gram_waves<true>(), 0, c->stream, m, l, Y, ldy, l, Y,
+                       ldy, c0, d0, part, (int64_t)l * l, l, 1);
+  else if (same)
     hipLaunchKernelGGL((k_gram_mfma<T, true>), nb, 64 * gram_waves<true>(), 0, c->stream, m, l, Y, ldy, l, Y, ldy,
                        c0, d0, part, (int64_t)l * l, l, 1);
   else
@@ -980,6 +1052,56 @@ void gram_async(Ctx *c, int64_t m, int64_t l, const void *Y, int64_t ldy, int yd
     HIP_CHECK(hipGetLastError());
   }
   (void)G;
+  HIP_CHECK(hipEventRecord(ev ? ev : c->small_ev, c->stream));
+}
+
+void gram_batched_async(Ctx *c, int64_t count, const int64_t *rows, int64_t l, const void *const *P, int64_t ldy,
+                        int ydt, size_t off, hipEvent_t ev) {
+  if (count <= 0) return;
+  if (l > 64) {
+    fail(SVDH_ERR_ARG, "batched Gram: panel width %lld exceeds 64", (long long)l);
+  }
+  if (off + (size_t)count * l * l > Ctx::kSmall) fail(SVDH_ERR_ARG, "Gram landing area too small");
+  if (ldy % 4 != 0) fail(SVDH_ERR_ARG, "Gram panel leading dimension %lld is not a multiple of 4", (long long)ldy);
+  if ((size_t)count * sizeof(GramPanel) > Ctx::kRingSlot) fail(SVDH_ERR_ARG, "batched Gram: too many panels");
+  // each panel is cut exactly as gram_async cuts it (same workgroups, so the
+  // same partials and sums: a batched Gram equals the single one bit for bit)
+  std::vector<GramPanel> pan(count);
+  int64_t total = 0, maxblk = 1;
+  double bytes = 0.0;
+  for (int64_t b = 0; b < count; ++b) {
+    const int64_t nblk = grid_of((rows[b] + 3) / 4, 8 * kGramWaves, 256);
+    pan[b] = GramPanel{P[b], rows[b], total, nblk};
+    total += nblk;
+    maxblk = std::max(maxblk, nblk);
+    bytes += (double)rows[b] * l * dtype_size(ydt);
+  }
+  const int64_t ll = l * l;
+  c->red_part.ensure(std::max<size_t>((size_t)total * ll * sizeof(double), c->red_part.bytes));
+  double *part = c->red_part.as<double>();
+  const GramPanel *dpan = static_cast<const GramPanel *>(c->upload(pan.data(), pan.size() * sizeof(GramPanel)));
+  {
+    KTimer t(c, KC_DENSE, bytes);
+    const dim3 grid((unsigned)maxblk, (unsigned)count);
+    const int th = 64 * gram_waves<true>();
+    const bool narrow = l <= 32;  // as launch_gram chooses
+    if (ydt == SVDH_F64 && narrow)
+      hipLaunchKernelGGL((k_gram_mfma_batched<double, 2>), grid, th, 0, c->stream, dpan, (int)l, ldy, part);
+    else if (ydt == SVDH_F64)
+      hipLaunchKernelGGL((k_gram_mfma_batched<double, 4>), grid, th, 0, c->stream, dpan, (int)l, ldy, part);
+    else if (ydt == SVDH_F32 && narrow)
+      hipLaunchKernelGGL((k_gram_mfma_batched<float, 2>), grid, th, 0, c->stream, dpan, (int)l, ldy, part);
+    else if (ydt == SVDH_F32)
+      hipLaunchKernelGGL((k_gram_mfma_batched<float, 4>), grid, th, 0, c->stream, dpan, (int)l, ldy, part);
+    else if (narrow)
+      hipLaunchKernelGGL((k_gram_mfma_batched<bf16_t, 2>), grid, th, 0, c->stream, dpan, (int)l, ldy, part);
+    else
+      hipLaunchKernelGGL((k_gram_mfma_batched<bf16_t, 4>), grid, th, 0, c->stream, dpan, (int)l, ldy, part);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_gram_reduce_batched, dim3((unsigned)reduce_grid(ll), (unsigned)count), kBlock, 0, c->stream,
+                       dpan, ll, (const double *)part, c->d_small + off);
+    HIP_CHECK(hipGetLastError());
+  }
   HIP_CHECK(hipEventRecord(ev ? ev : c->small_ev, c->stream));
 }
 

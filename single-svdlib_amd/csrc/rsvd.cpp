@@ -654,7 +654,12 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
     bool ran = false;
     for (int64_t b0 = 0; b0 < nops; b0 += per_round) {
       const int64_t b1 = std::min(nops, b0 + per_round);
-      for (int64_t b = b0; b < b1; ++b) gram_async(c, rows[b], l, P[b], ld, pdt, (size_t)(b - b0) * l * l);
+      if (l <= 64) {  // one launch for the group
+        std::vector<const void *> pb(P.begin() + b0, P.begin() + b1);
+        gram_batched_async(c, b1 - b0, rows.data() + b0, l, pb.data(), ld, pdt, 0);
+      } else {
+        for (int64_t b = b0; b < b1; ++b) gram_async(c, rows[b], l, P[b], ld, pdt, (size_t)(b - b0) * l * l);
+      }
       if (!ran) {
         between();
         ran = true;

@@ -1988,8 +1988,10 @@ static void spmm_hot_launch(Ctx *c, const SpmmArgs<V, X, Y, Acc> &a, const Sched
                             int nrows) {
   constexpr int RPB = kHotBlock >> LGR;
   const int64_t need = ((int64_t)nrows + RPB - 1) / RPB;
-  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(2 * (int64_t)c->ncu, need));
   const size_t shm = (size_t)e.h * a.ldx * sizeof(X);
+  // two workgroups per CU while two stagings fit the 160 KB of LDS, else one
+  const int per_cu = shm <= 80 * 1024 ? 2 : 1;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(per_cu * (int64_t)c->ncu, need));
   hipLaunchKernelGGL((k_spmm_hot<LGR, V, X, Y, Acc>), blocks, kHotBlock, shm, c->stream, a, e.colidx.as<int32_t>(),
                      hval, e.split.as<int32_t>(), e.order.as<int32_t>(), nrows, e.rows.as<int32_t>(), e.h);
 }
@@ -2067,7 +2069,11 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   SpmmPlan<V, X, Y, Acc> p;
   if (!plan_spmm(op, direction, vals, X_, ldx, Y_, ldy, l, p)) return;
   if (direction == 0 && !p.mc) {
-    Schedule::HotEnc *e = hot_columns(op, *p.s, std::min<int>(1024, (int)(65536 / (ldx * (int64_t)sizeof(X)))));
+    static const int64_t hot_lds = [] {  // LDS bytes for the staged panel rows (SVDH_SPMM_HOT_LDS)
+      const char *e = std::getenv("SVDH_SPMM_HOT_LDS");
+      return e ? std::max<int64_t>(4096, std::min<int64_t>(155648, std::atoll(e))) : 65536;
+    }();
+    Schedule::HotEnc *e = hot_columns(op, *p.s, std::min<int>(1024, (int)(hot_lds / (ldx * (int64_t)sizeof(X)))));
     if (e) {
       const V *hval = hot_values<V>(c, *e, static_cast<const V *>(vals), op->nnz);
       const int nrows = (int)op->nrows;
