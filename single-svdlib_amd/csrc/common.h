@@ -357,6 +357,12 @@ void panel_gemm_dev(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, in
 void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
                 const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
                 double beta, const float *Zin = nullptr, int64_t ldzin = 0);
+// Z_b (rows[b] x lo[b]) = Z_b (rows[b] x li) M_b in place for count FP32
+// panels (shared leading dimension ld), M_b li x lo[b] FP64 row-major on the
+// host: the factors travel in upload slots and each group that fits one slot
+// is one launch. Equal to panel_mfma on each panel, bit for bit.
+void panel_mfma_batched(Ctx *c, int64_t count, const int64_t *rows, int64_t li, const int64_t *lo, float *const *Z,
+                        int64_t ld, const std::vector<double> *M);
 // panel_mfma plus the FP64 Gram Z^T Z of its output, accumulated in the same
 // kernel: the equivalent of panel_mfma followed by gram_async(Z, gram_off,
 // gram_ev) without reading Z again.

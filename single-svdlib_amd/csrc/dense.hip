@@ -776,11 +776,12 @@ __device__ __forceinline__ void panel_rows_load(const PanelTerm &t0, const Panel
 // exactly those) adds the block (bi, bj) of the 16 rows, for bi <= bj. The
 // waves are summed in LDS in wave order and each workgroup writes its
 // lo x lo partial (both triangles) to part[blockIdx.x * lo * lo ...].
-template <int NP, bool RESID, bool GRAM = false>
-__global__ __launch_bounds__(kPanelBlock) __attribute__((amdgpu_waves_per_eu(2)))
-void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
-                                                      int64_t ldz, const float *zin, int64_t ldzin, float beta,
-                                                      double *part) {
+// Workgroup bid of nbk; part is this workgroup's partial (RESID: lo
+// entries, GRAM: lo x lo).
+template <int NP, bool RESID, bool GRAM>
+__device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_t rows, int lo, const PanelTerm &t0,
+                                                const PanelTerm &t1, float *z, int64_t ldz, const float *zin,
+                                                int64_t ldzin, float beta, double *part) {
   // alpha_p M_p in LDS as [k][rl][q] (element M[k][16 q + rl] at
   // k * kMsStride + 4 rl + q): a lane's four column blocks for one k are one
   // 16-byte read. The B operands are read from LDS at each step, which leaves
@@ -798,8 +799,8 @@ void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
   const int lane = threadIdx.x & 63;
   const int kq = lane >> 4, rl = lane & 15;
   const int64_t nblk = (rows + 15) / 16;
-  const int64_t wave = (int64_t)blockIdx.x * (kPanelBlock / 64) + (threadIdx.x >> 6);
-  const int64_t nwaves = (int64_t)gridDim.x * (kPanelBlock / 64);
+  const int64_t wave = bid * (kPanelBlock / 64) + (threadIdx.x >> 6);
+  const int64_t nwaves = nbk * (kPanelBlock / 64);
   f4_t ya[NP][4], yn[NP][4];
   double csq[4] = {0.0, 0.0, 0.0, 0.0};  // RESID: columns 16 q + rl
   d4_t gacc[4][4];                       // GRAM: blocks (bi, bj), bi <= bj
@@ -894,7 +895,7 @@ void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
             }
       __syncthreads();
     }
-    double *out = part + (int64_t)blockIdx.x * lo * lo;
+    double *out = part;
     for (int e = threadIdx.x; e < 16 * 4 * 64; e += kPanelBlock) {
       const int blk = e >> 8, v = (e >> 6) & 3, ln = e & 63;
       const int bi = blk >> 2, bj = blk & 3;
@@ -918,9 +919,38 @@ void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z,
       double t = 0.0;
       for (int w = 0; w < kPanelBlock / 64; ++w)
         for (int g = 0; g < 4; ++g) t += red[w][g][threadIdx.x];
-      part[(int64_t)blockIdx.x * lo + threadIdx.x] = t;
+      part[threadIdx.x] = t;
     }
   }
+}
+
+template <int NP, bool RESID, bool GRAM = false>
+__global__ __launch_bounds__(kPanelBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z, int64_t ldz, const float *zin,
+                  int64_t ldzin, float beta, double *part) {
+  double *pw = part ? part + (int64_t)blockIdx.x * (GRAM ? (int64_t)lo * lo : (int64_t)lo) : nullptr;
+  panel_mfma_body<NP, RESID, GRAM>(blockIdx.x, gridDim.x, rows, lo, t0, t1, z, ldz, zin, ldzin, beta, pw);
+}
+
+// Z_b = Y_b M_b for many panels in one launch (one term, beta 0; Z_b may be
+// Y_b): job b = blockIdx.y takes workgroups 0 .. nblk - 1 of the x dimension.
+// Each row block is computed by one wave as in k_panel_mfma, so the result
+// does not depend on the grid.
+struct PanelJob {
+  PanelTerm t;
+  float *z;
+  int64_t ldz;
+  int64_t rows;
+  int64_t nblk;
+  int lo;
+};
+
+__global__ __launch_bounds__(kPanelBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_panel_mfma_batched(const PanelJob *jobs) {
+  const PanelJob j = jobs[blockIdx.y];
+  if ((int64_t)blockIdx.x >= j.nblk) return;
+  panel_mfma_body<1, false, false>(blockIdx.x, j.nblk, j.rows, j.lo, j.t, j.t, j.z, j.ldz, j.z, j.ldz, 0.f,
+                                   nullptr);
 }
 
 // Workgroups per CU of the panel kernels (SVDH_PANEL_WG, default 1): the
@@ -1008,6 +1038,46 @@ void panel_mfma_gram(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *
   }
   panel_mfma_impl(c, rows, lo, nterms, Y, ldy, li, M_dev, alpha, Z, ldz, beta, Zin, ldzin, (int64_t)gram_off,
                   gram_ev);
+}
+
+void panel_mfma_batched(Ctx *c, int64_t count, const int64_t *rows, int64_t li, const int64_t *lo, float *const *Z,
+                        int64_t ld, const std::vector<double> *M) {
+  if (li > 64 || ld % 4) fail(SVDH_ERR_ARG, "panel_mfma_batched: li <= 64 and ld a multiple of 4");
+  constexpr int W = kPanelBlock / 64;
+  // groups whose factors fit one upload slot, one launch per group
+  int64_t b0 = 0;
+  while (b0 < count) {
+    int64_t b1 = b0;
+    size_t doubles = 0;
+    while (b1 < count && (doubles + M[b1].size()) * sizeof(double) <= Ctx::kRingSlot) doubles += M[b1++].size();
+    if (b1 == b0) fail(SVDH_ERR_ARG, "panel_mfma_batched: factor larger than the upload slot");
+    std::vector<double> mall;
+    mall.reserve(doubles);
+    for (int64_t b = b0; b < b1; ++b) mall.insert(mall.end(), M[b].begin(), M[b].end());
+    const double *dm = static_cast<const double *>(c->upload(mall.data(), mall.size() * sizeof(double)));
+    std::vector<PanelJob> jobs(b1 - b0);
+    int64_t maxblk = 1;
+    double bytes = 0.0;
+    size_t moff = 0;
+    for (int64_t b = b0; b < b1; ++b) {
+      if (lo[b] > 64) fail(SVDH_ERR_ARG, "panel_mfma_batched: lo <= 64");
+      const int64_t nblk16 = (rows[b] + 15) / 16;
+      const int64_t nblk =
+          std::max<int64_t>(1, std::min<int64_t>((nblk16 + W - 1) / W, (int64_t)panel_wg_per_cu() * c->ncu));
+      jobs[b - b0] = PanelJob{PanelTerm{Z[b], ld, (int)li, dm + moff, 1.0}, Z[b], ld, rows[b], nblk, (int)lo[b]};
+      moff += M[b].size();
+      maxblk = std::max(maxblk, nblk);
+      bytes += (double)rows[b] * (li + lo[b]) * 4.0;
+    }
+    const PanelJob *dj = static_cast<const PanelJob *>(c->upload(jobs.data(), jobs.size() * sizeof(PanelJob)));
+    {
+      KTimer tm(c, KC_DENSE, bytes);
+      hipLaunchKernelGGL(k_panel_mfma_batched, dim3((unsigned)maxblk, (unsigned)(b1 - b0)), kPanelBlock, 0,
+                         c->stream, dj);
+      HIP_CHECK(hipGetLastError());
+    }
+    b0 = b1;
+  }
 }
 
 __global__ void k_fetch(const uint4 *src, uint4 *dst, int64_t n16) {

@@ -683,7 +683,14 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
   int64_t iters_run = 0;
   for (;;) {
     const bool final_fixed = iters_run >= n_power_iters;
-    for (int64_t b = 0; b < nops; ++b) {  // Z = (A^T Y) Mq, the same kernels as rsvd_solve
+    bool all_mfma = pdt == SVDH_F32 && l <= 64;
+    for (int64_t b = 0; b < nops; ++b) all_mfma = all_mfma && wq[b] <= 64;
+    if (all_mfma) {  // Z = (A^T Y) Mq for every mask, the same kernels as rsvd_solve, one launch per group
+      std::vector<float *> zf(nops);
+      for (int64_t b = 0; b < nops; ++b) zf[b] = static_cast<float *>(Z[b]);
+      panel_mfma_batched(c, nops, nrows.data(), l, wq.data(), zf.data(), ld, Mq.data());
+    }
+    for (int64_t b = 0; b < nops && !all_mfma; ++b) {  // Z = (A^T Y) Mq, the same kernels as rsvd_solve
       if (pdt == SVDH_F32 && l <= 64 && wq[b] <= 64) {
         const double *dM = static_cast<const double *>(c->upload(Mq[b].data(), Mq[b].size() * sizeof(double)));
         const float *yin[1] = {static_cast<const float *>(Z[b])};
