@@ -190,7 +190,12 @@ struct Schedule {
   int hot_state = -1;
   std::vector<int64_t> hot_cover;
   std::vector<int32_t> hot_order;
-  HotEnc hot[2];
+  // Encodings for up to four values of h (panel widths and element sizes of
+  // the callers), replaced least recently used first.
+  static constexpr int kHotEnc = 4;
+  HotEnc hot[kHotEnc];
+  uint64_t hot_use[kHotEnc] = {0, 0, 0, 0};
+  uint64_t hot_clock = 0;
   // Short rows cut at source blocks: kSegInts per segment, grouped by block.
   DevBuf segs;
   int32_t seg_off[8] = {0};

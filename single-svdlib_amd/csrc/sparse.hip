@@ -1917,9 +1917,17 @@ static Schedule::HotEnc *hot_columns(svdh_op *op, Schedule &s, int h) {
     s.hot_state = 1;
   }
   if (h > (int)s.hot_cover.size() || 5 * s.hot_cover[h - 1] < nnz) return nullptr;
-  for (auto &e : s.hot)
-    if (e.h == h) return &e;
-  Schedule::HotEnc &e = s.hot[0].h == 0 ? s.hot[0] : s.hot[1];
+  for (int i = 0; i < Schedule::kHotEnc; ++i)
+    if (s.hot[i].h == h) {
+      s.hot_use[i] = ++s.hot_clock;
+      return &s.hot[i];
+    }
+  int victim = 0;  // an unused entry, else the least recently used
+  for (int i = 1; i < Schedule::kHotEnc; ++i)
+    if ((s.hot[i].h == 0) != (s.hot[victim].h == 0) ? s.hot[i].h == 0 : s.hot_use[i] < s.hot_use[victim])
+      victim = i;
+  s.hot_use[victim] = ++s.hot_clock;
+  Schedule::HotEnc &e = s.hot[victim];
   for (auto &v : e.vals) v.release();
   e.dense.release();
   Ctx *c = op->ctx;
