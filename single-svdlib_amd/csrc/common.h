@@ -63,9 +63,13 @@ struct Ctx {
   // Device scalar block written by reduction kernels, and its pinned mirror.
   DevBuf scal;
   double *h_scal = nullptr;
-  // Reduction partials and completion counters (self-resetting).
+  // Reduction partials.
   DevBuf red_part;
   DevBuf red_count;
+  // Workgroup partials of the fused BLAS-1 dot products, per scalar slot, and
+  // how many each slot's last writer left.
+  DevBuf slot_parts;
+  int slot_nb[64] = {0};
   // Small host-to-device matrices (staged, then read by one kernel).
   DevBuf stage;
   // Device workspaces kept for the context's lifetime (panels of the
@@ -298,11 +302,14 @@ struct BatchedDeviceOut {
 // ---- vector / dense kernels (dense.hip) -----------------------------------
 // Scalar slots of Ctx::scal.
 enum Slot { S_A = 0, S_B = 1, S_C = 2, S_D = 3, S_E = 4, S_F = 5, S_NSLOT = 64 };
-// y[i] += coef * x[i] for all listed updates, then dot(u, v) -> scal[out_slot].
-// coef is  sign * scal[slot]  when slot >= 0, else the immediate imm.
+// y[i] += coef * x[i] for all listed updates, then dot(u, v) -> slot out_slot.
+// coef is  sign * (value of slot)  when slot >= 0, else the immediate imm.
+// A slot's value stays on the device as workgroup partials; slots_finish(n)
+// writes the values of slots 0 .. n-1 to Ctx::scal (then copy them to the host).
 struct Axpy { double *y; const double *x; double imm; int slot; double sign; };
 void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *u,
                     const double *v, int out_slot, double *copy_dst, const double *copy_src);
+void slots_finish(Ctx *c, int n);
 void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double *z);
 void vec_copy(Ctx *c, int64_t n, const double *x, double *y);
 // Basis GEMV: cout[2k] = Q[:, c0:c0+k]^T [u v]  (k <= 4096)

@@ -42,27 +42,18 @@ __device__ __forceinline__ double block_sum(double v) {
   return t;
 }
 
-// Second level of a grid reduction: block partial into part[blockIdx.x]; the
-// last block to arrive adds the partials in block order into *out.
-__device__ __forceinline__ void grid_finish(double t, double *part, int *count, double *out) {
-  __shared__ int last;
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = t;
-    __threadfence();
-    last = atomicAdd(count, 1) == (int)gridDim.x - 1;
-  }
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    double s = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) s += __builtin_nontemporal_load(&part[i]);
-    // ordered: per-thread strided partial sums, then block_sum in fixed order
-    s = block_sum(s);
-    if (threadIdx.x == 0) {
-      *out = s;
-      *count = 0;
-    }
-  }
+// Dot products of the fused kernel leave one partial per workgroup in the
+// slot's row of Ctx::slot_parts (kSlotParts entries per slot); whoever needs
+// the value (the next fused kernel's coefficient, k_slot_finish for the host)
+// sums them with slot_sum, in the same order everywhere, so the device and the
+// host see the same bits. No fences or atomics between workgroups.
+constexpr int kSlotParts = 1024;  // >= fused_grid's cap
+
+// Called by whole waves; every lane returns the sum of p[0 .. nb).
+__device__ __forceinline__ double slot_sum(const double *p, int nb) {
+  double s = 0.0;
+  for (int b = threadIdx.x & 63; b < nb; b += 64) s += p[b];
+  return wave_sum_d(s);
 }
 
 struct AxpyD {
@@ -70,6 +61,7 @@ struct AxpyD {
   const double *x;
   double imm;
   int slot;
+  int nb;  // partials of the slot
   double sign;
 };
 
@@ -82,15 +74,17 @@ struct FusedArgs {
   int out_slot;
   double *copy_dst;
   const double *copy_src;
-  double *scal;
-  double *part;
-  int *count;
+  double *parts;  // Ctx::slot_parts
 };
 
 __global__ __launch_bounds__(kBlock) void k_fused_axpy_dot(FusedArgs a) {
   double c0 = 0.0, c1 = 0.0;
-  if (a.nups > 0) c0 = a.up[0].slot >= 0 ? a.up[0].sign * a.scal[a.up[0].slot] : a.up[0].imm;
-  if (a.nups > 1) c1 = a.up[1].slot >= 0 ? a.up[1].sign * a.scal[a.up[1].slot] : a.up[1].imm;
+  if (a.nups > 0)
+    c0 = a.up[0].slot >= 0 ? a.up[0].sign * slot_sum(a.parts + a.up[0].slot * kSlotParts, a.up[0].nb)
+                           : a.up[0].imm;
+  if (a.nups > 1)
+    c1 = a.up[1].slot >= 0 ? a.up[1].sign * slot_sum(a.parts + a.up[1].slot * kSlotParts, a.up[1].nb)
+                           : a.up[1].imm;
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < a.n; i += stride) {
@@ -101,7 +95,18 @@ __global__ __launch_bounds__(kBlock) void k_fused_axpy_dot(FusedArgs a) {
   }
   if (a.out_slot < 0) return;
   const double t = block_sum(acc);
-  grid_finish(t, a.part, a.count, &a.scal[a.out_slot]);
+  if (threadIdx.x == 0) a.parts[a.out_slot * kSlotParts + blockIdx.x] = t;
+}
+
+// scal[s] = the value of slot s (one 64-thread workgroup per slot, s < n).
+struct SlotCounts {
+  int nb[S_NSLOT];
+};
+__global__ void k_slot_finish(const double *parts, SlotCounts cnt, double *scal) {
+  const int s = blockIdx.x;
+  if (cnt.nb[s] <= 0) return;
+  const double v = slot_sum(parts + s * kSlotParts, cnt.nb[s]);
+  if (threadIdx.x == 0) scal[s] = v;
 }
 
 __global__ void k_scale2(int64_t n, double a, const double *x, double *y, double *z) {
@@ -638,11 +643,13 @@ void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *
   std::memset(&a, 0, sizeof a);
   a.n = n;
   a.nups = nups;
+  c->slot_parts.ensure((size_t)S_NSLOT * kSlotParts * sizeof(double));
   for (int i = 0; i < nups; ++i) {
     a.up[i].y = ups[i].y;
     a.up[i].x = ups[i].x;
     a.up[i].imm = ups[i].imm;
     a.up[i].slot = ups[i].slot;
+    a.up[i].nb = ups[i].slot >= 0 ? c->slot_nb[ups[i].slot] : 0;
     a.up[i].sign = ups[i].sign;
   }
   a.u = u;
@@ -650,12 +657,24 @@ void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *
   a.out_slot = out_slot;
   a.copy_dst = copy_dst;
   a.copy_src = copy_src;
-  a.scal = c->scal.as<double>();
-  a.part = c->red_part.as<double>();
-  a.count = c->red_count.as<int>();
+  a.parts = c->slot_parts.as<double>();
   const int g = fused_grid(n);
+  if (out_slot >= 0) {
+    if (out_slot >= S_NSLOT) fail(SVDH_ERR_ARG, "scalar slot %d out of range", out_slot);
+    c->slot_nb[out_slot] = g;
+  }
   KTimer t(c, KC_BLAS1, (double)n * 8.0 * (3 * nups + 2 + (copy_dst ? 2 : 0)));
   hipLaunchKernelGGL(k_fused_axpy_dot, g, kBlock, 0, c->stream, a);
+  HIP_CHECK(hipGetLastError());
+}
+
+void slots_finish(Ctx *c, int n) {
+  if (n <= 0) return;
+  if (n > S_NSLOT) fail(SVDH_ERR_ARG, "scalar slots %d out of range", n);
+  c->slot_parts.ensure((size_t)S_NSLOT * kSlotParts * sizeof(double));
+  SlotCounts cnt;
+  for (int s = 0; s < S_NSLOT; ++s) cnt.nb[s] = s < n ? c->slot_nb[s] : 0;
+  hipLaunchKernelGGL(k_slot_finish, n, 64, 0, c->stream, c->slot_parts.as<double>(), cnt, c->scal.as<double>());
   HIP_CHECK(hipGetLastError());
 }
 

@@ -79,6 +79,7 @@ struct Las2 {
 
   // Read n scalar slots (synchronises the stream).
   void read_slots(int n) {
+    slots_finish(c, n);
     HIP_CHECK(hipMemcpyAsync(c->h_scal, c->scal.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
