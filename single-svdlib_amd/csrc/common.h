@@ -310,7 +310,9 @@ struct Axpy { double *y; const double *x; double imm; int slot; double sign; };
 void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *u,
                     const double *v, int out_slot, double *copy_dst, const double *copy_src);
 void slots_finish(Ctx *c, int n);
-void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double *z);
+// y = a x, z *= a, and (when copy_dst is set) copy_dst = copy_src, in one pass.
+void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double *z, double *copy_dst = nullptr,
+                const double *copy_src = nullptr);
 void vec_copy(Ctx *c, int64_t n, const double *x, double *y);
 // Basis GEMV: cout[2k] = Q[:, c0:c0+k]^T [u v]  (k <= 4096)
 void basis_gemv_t(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *u,

@@ -109,10 +109,12 @@ __global__ void k_slot_finish(const double *parts, SlotCounts cnt, double *scal)
   if (threadIdx.x == 0) scal[s] = v;
 }
 
-__global__ void k_scale2(int64_t n, double a, const double *x, double *y, double *z) {
+__global__ void k_scale2(int64_t n, double a, const double *x, double *y, double *z, double *cdst,
+                         const double *csrc) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     y[i] = a * x[i];
     z[i] *= a;
+    if (cdst) cdst[i] = csrc[i];
   }
 }
 
@@ -678,9 +680,10 @@ void slots_finish(Ctx *c, int n) {
   HIP_CHECK(hipGetLastError());
 }
 
-void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double *z) {
-  KTimer t(c, KC_BLAS1, (double)n * 32.0);
-  hipLaunchKernelGGL(k_scale2, grid_of(n), kBlock, 0, c->stream, n, a, x, y, z);
+void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double *z, double *copy_dst,
+                const double *copy_src) {
+  KTimer t(c, KC_BLAS1, (double)n * (32.0 + (copy_dst ? 16.0 : 0.0)));
+  hipLaunchKernelGGL(k_scale2, grid_of(n), kBlock, 0, c->stream, n, a, x, y, z, copy_dst, copy_src);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -224,18 +224,24 @@ struct Las2 {
     for (; j < last; ++j) {
       std::swap(w[1], w[2]);
       std::swap(w[3], w[4]);
-      storq(j - 1, w[2]);
+      // q_{j-1} = w2 is stored by the scaling kernel below (w2 is not written
+      // before it), or right away on the restart and exit paths
+      ensure_slots(j - 1 + kMaxLL + 1);
+      bool stored = false;
       if (j - 1 < kMaxLL) storp(j - 1, w[4]);
       bet[j] = *rnm;
       if (zero(*rnm)) {
+        storq(j - 1, w[2]);
+        stored = true;
         startv(j, 0u, rnm);
         if (zero(*rnm)) *enough = true;
       }
       if (*enough) {
+        if (!stored) storq(j - 1, w[2]);
         std::swap(w[1], w[2]);
         break;
       }
-      vec_scale2(c, ncols, 1.0 / *rnm, w[0], w[1], w[3]);
+      vec_scale2(c, ncols, 1.0 / *rnm, w[0], w[1], w[3], stored ? nullptr : q(j - 1), w[2]);
       opb(w[3], w[0]);
       Axpy u0{w[0], w[2], -*rnm, -1, 1.0};
       fused_axpy_dot(c, ncols, &u0, 1, w[0], w[3], S_A, nullptr, nullptr);
