@@ -176,7 +176,10 @@ int32_t svdh_ctx_create(int32_t device, svdh_ctx **out) {
       (void)hipGetLastError();
       c->scal.alloc(S_NSLOT * sizeof(double));
       HIP_CHECK(hipMemset(c->scal.p, 0, S_NSLOT * sizeof(double)));
-      HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_scal), S_NSLOT * sizeof(double), hipHostMallocDefault));
+      // coherent and mapped: k_slot_finish writes the step's scalars straight into it
+      HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_scal), S_NSLOT * sizeof(double),
+                              hipHostMallocCoherent | hipHostMallocMapped));
+      HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->dh_scal), c->h_scal, 0));
       c->red_part.alloc(1 << 20);
       c->red_count.alloc(256);
       HIP_CHECK(hipMemset(c->red_count.p, 0, 256));

@@ -63,6 +63,7 @@ struct Ctx {
   // Device scalar block written by reduction kernels, and its pinned mirror.
   DevBuf scal;
   double *h_scal = nullptr;
+  double *dh_scal = nullptr;  // device address of h_scal
   // Reduction partials.
   DevBuf red_part;
   DevBuf red_count;
@@ -305,7 +306,8 @@ enum Slot { S_A = 0, S_B = 1, S_C = 2, S_D = 3, S_E = 4, S_F = 5, S_NSLOT = 64 }
 // y[i] += coef * x[i] for all listed updates, then dot(u, v) -> slot out_slot.
 // coef is  sign * (value of slot)  when slot >= 0, else the immediate imm.
 // A slot's value stays on the device as workgroup partials; slots_finish(n)
-// writes the values of slots 0 .. n-1 to Ctx::scal (then copy them to the host).
+// writes the values of slots 0 .. n-1 to Ctx::scal and to the host mirror
+// Ctx::h_scal (coherent memory: valid on the host once the stream has passed).
 struct Axpy { double *y; const double *x; double imm; int slot; double sign; };
 void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *u,
                     const double *v, int out_slot, double *copy_dst, const double *copy_src);

@@ -102,11 +102,14 @@ __global__ __launch_bounds__(kBlock) void k_fused_axpy_dot(FusedArgs a) {
 struct SlotCounts {
   int nb[S_NSLOT];
 };
-__global__ void k_slot_finish(const double *parts, SlotCounts cnt, double *scal) {
+__global__ void k_slot_finish(const double *parts, SlotCounts cnt, double *scal, double *host) {
   const int s = blockIdx.x;
   if (cnt.nb[s] <= 0) return;
   const double v = slot_sum(parts + s * kSlotParts, cnt.nb[s]);
-  if (threadIdx.x == 0) scal[s] = v;
+  if (threadIdx.x == 0) {
+    scal[s] = v;
+    host[s] = v;
+  }
 }
 
 __global__ void k_scale2(int64_t n, double a, const double *x, double *y, double *z, double *cdst,
@@ -676,7 +679,8 @@ void slots_finish(Ctx *c, int n) {
   c->slot_parts.ensure((size_t)S_NSLOT * kSlotParts * sizeof(double));
   SlotCounts cnt;
   for (int s = 0; s < S_NSLOT; ++s) cnt.nb[s] = s < n ? c->slot_nb[s] : 0;
-  hipLaunchKernelGGL(k_slot_finish, n, 64, 0, c->stream, c->slot_parts.as<double>(), cnt, c->scal.as<double>());
+  hipLaunchKernelGGL(k_slot_finish, n, 64, 0, c->stream, c->slot_parts.as<double>(), cnt, c->scal.as<double>(),
+                     c->dh_scal);
   HIP_CHECK(hipGetLastError());
 }
 

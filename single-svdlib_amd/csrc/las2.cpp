@@ -79,8 +79,7 @@ struct Las2 {
 
   // Read n scalar slots (synchronises the stream).
   void read_slots(int n) {
-    slots_finish(c, n);
-    HIP_CHECK(hipMemcpyAsync(c->h_scal, c->scal.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    slots_finish(c, n);  // writes the values into the host mirror too
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
   double dot_now(const double *u, const double *v) {
