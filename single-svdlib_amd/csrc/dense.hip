@@ -173,22 +173,56 @@ __global__ __launch_bounds__(kBlock) void k_gemv_n(int64_t n, const double *Q, i
 }
 
 // C (n x d row-major) = Q (n x k col-major) W (k x d row-major); 16 outputs per thread.
-__global__ __launch_bounds__(kBlock) void k_combine(int64_t n, const double *Q, int64_t ldq, int64_t k,
-                                                   const double *W, int64_t d, int64_t c0, double *C, int64_t ldc) {
-  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int64_t nd = min<int64_t>(16, d - c0);
-  double acc[16];
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// C (n x d, row-major, ldc) = Q (n x k, column-major, ldq) W (k x d, row-major)
+// on the FP64 matrix cores (v_mfma_f64_16x16x4: per lane l, A[l % 16][l / 16],
+// B[l / 16][l % 16], D[l / 16 + 4 v][l % 16]). One wave per 16-row block and
+// column groups of up to 8 tiles of 16; the k loop takes 4 steps of 4 per
+// batch, loads first. A[i][kk] = Q[r0 + i][k0 + kk] (16 consecutive rows of a
+// column per kk), B[kk][j] = W[k0 + kk][c0 + 16 t + j].
+__global__ __launch_bounds__(256) void k_combine_mfma(int64_t n, const double *Q, int64_t ldq, int64_t k,
+                                                      const double *W, int64_t d, double *C, int64_t ldc) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  if (r0 >= n) return;  // the whole wave
+  const int i = lane & 15, kk = lane >> 4;
+  const int64_t row = r0 + i;
+  const bool rok = row < n;
+  constexpr int T = 8, S = 4;
+  for (int64_t c0 = 0; c0 < d; c0 += 16 * T) {
+    d4_t acc[T];
 #pragma unroll
-  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
-  for (int64_t j = 0; j < k; ++j) {
-    const double q = Q[j * ldq + i];
-    const double *w = W + j * d + c0;
+    for (int t = 0; t < T; ++t) acc[t] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int64_t k0 = 0; k0 < k; k0 += 4 * S) {
+      double a[S], b[S][T];
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
-      if (t < nd) acc[t] += q * w[t];
+      for (int st = 0; st < S; ++st) {
+        const int64_t kq = k0 + 4 * st + kk;
+        const bool kok = kq < k;
+        const int64_t kqc = kok ? kq : 0;  // in-range address; the value is dropped below
+        const double av = Q[kqc * ldq + (rok ? row : r0)];
+        a[st] = kok && rok ? av : 0.0;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const int64_t col = c0 + 16 * t + i;
+          const double bv = W[kqc * d + (col < d ? col : 0)];
+          b[st][t] = kok && col < d ? bv : 0.0;
+        }
+      }
+#pragma unroll
+      for (int st = 0; st < S; ++st)
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b[st][t], acc[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t orow = r0 + kk + 4 * v, ocol = c0 + 16 * t + i;
+        if (orow < n && ocol < d) C[orow * ldc + ocol] = acc[t][v];
+      }
   }
-  for (int t = 0; t < nd; ++t) C[i * ldc + c0 + t] = acc[t];
 }
 
 template <typename T> __device__ __forceinline__ double ldd(const T *p, int64_t i);
@@ -266,7 +300,6 @@ static int reduce_grid(int64_t ll) { return (int)((ll + 15) / 16); }
 // of a workgroup are summed in LDS in wave order and the workgroup partial is
 // written mirrored to part[blockIdx * l * l] (disjoint entries per launch
 // when l > 64 needs several (c0, d0) launches).
-typedef double d4_t __attribute__((ext_vector_type(4)));
 constexpr int kGramWaves = 8;
 
 template <typename T> struct Quad;
@@ -713,10 +746,10 @@ void basis_gemv_n(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, co
 
 void basis_combine(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *W, int64_t d,
                    double *C, int64_t ldc) {
-  KTimer t(c, KC_DENSE, (double)n * k * 8.0 * ((d + 15) / 16) + (double)n * d * 8.0);
-  for (int64_t c0 = 0; c0 < d; c0 += 16)
-    hipLaunchKernelGGL(k_combine, (int)((n + kBlock - 1) / kBlock), kBlock, 0, c->stream, n, Q, ldq, k, W, d, c0, C,
-                       ldc);
+  if (n <= 0 || d <= 0) return;
+  KTimer t(c, KC_DENSE, (double)n * k * 8.0 * ((d + 127) / 128) + (double)n * d * 8.0);
+  const int64_t blocks16 = (n + 15) / 16;
+  hipLaunchKernelGGL(k_combine_mfma, (int)((blocks16 + 3) / 4), 256, 0, c->stream, n, Q, ldq, k, W, d, C, ldc);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -422,18 +422,18 @@ struct Las2 {
     const int64_t ldv = (d + 1) / 2 * 2;
     DevBuf Wd(W.size() * sizeof(double));
     HIP_CHECK(hipMemcpyAsync(Wd.p, W.data(), W.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    DevBuf V((size_t)ncols * ldv * sizeof(double)), AV((size_t)nrows * ldv * sizeof(double)),
-        BV((size_t)ncols * ldv * sizeof(double));
+    DevBuf V((size_t)ncols * ldv * sizeof(double)), AV((size_t)nrows * ldv * sizeof(double));
     HIP_CHECK(hipMemsetAsync(V.p, 0, V.bytes, c->stream));
     basis_combine(c, ncols, q(0), ldq, js, Wd.as<double>(), d, V.as<double>(), ldv);
     spmm(op, fwd, vdt, V.p, ldv, SVDH_F64, AV.p, ldv, SVDH_F64, d);
-    spmm(op, bwd, vdt, AV.p, ldv, SVDH_F64, BV.p, ldv, SVDH_F64, d);
-    // sigma_r = sqrt(max(v_r^T B v_r, 0)) (mod.rs:1212-1225), u_r = A v_r / max(sigma_r, eps')
+    // sigma_r = sqrt(max(v_r^T B v_r, 0)) with B = A^T A (mod.rs:1212-1225),
+    // taken as ||A v_r||^2 = v_r^T A^T A v_r from the A V needed for U anyway
+    // (one sparse product instead of two); u_r = A v_r / max(sigma_r, eps')
     // (mod.rs:1228-1243): dots, scaling and the transposes to the d x n row
     // layout of SvdRec run on the device; only U^T, V^T and sigma come back.
     DevBuf sc(2 * (size_t)std::max<int64_t>(d, 1) * sizeof(double));
     double *dots = sc.as<double>(), *scale = dots + d;
-    panel_coldots(c, ncols, d, V.p, BV.p, ldv, SVDH_F64, dots);
+    panel_coldots(c, nrows, d, AV.p, AV.p, ldv, SVDH_F64, dots);
     std::vector<double> hd;
     hd.resize(d);
     HIP_CHECK(hipMemcpyAsync(hd.data(), dots, d * sizeof(double), hipMemcpyDeviceToHost, c->stream));
