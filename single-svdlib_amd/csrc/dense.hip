@@ -225,6 +225,59 @@ __global__ __launch_bounds__(256) void k_combine_mfma(int64_t n, const double *Q
   }
 }
 
+// Z[r, c] (at r ldz + c zcs, FP64) = beta Z[r, c] + alpha (Y M)[r, c] for a
+// row-major panel Y (m x li, FP32 or FP64) and M (li x lo <= 16 T, FP64
+// row-major) on the FP64 matrix cores, laid out as k_combine_mfma with
+// A[i][kk] = Y[r0 + i][k0 + kk]. A wave reads all of its 16 rows before
+// writing any, so Z may be Y (zcs = 1, same rows).
+template <typename TY, int T>
+__global__ __launch_bounds__(256) void k_apply_mfma(int64_t m, int li, int lo, const TY *Y, int64_t ldy,
+                                                    const double *M, double *Z, int64_t ldz, int64_t zcs,
+                                                    double alpha, double beta) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  if (r0 >= m) return;  // the whole wave
+  const int i = lane & 15, kk = lane >> 4;
+  const int64_t row = r0 + i;
+  const bool rok = row < m;
+  constexpr int S = 4;
+  d4_t acc[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) acc[t] = d4_t{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < li; k0 += 4 * S) {
+    double a[S], b[S][T];
+#pragma unroll
+    for (int st = 0; st < S; ++st) {
+      const int kq = k0 + 4 * st + kk;
+      const bool kok = kq < li;
+      const int kqc = kok ? kq : 0;  // in-range addresses; the values are dropped below
+      const double av = (double)Y[(rok ? row : r0) * ldy + kqc];
+      a[st] = kok && rok ? av : 0.0;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int col = 16 * t + i;
+        const double bv = M[(int64_t)kqc * lo + (col < lo ? col : 0)];
+        b[st][t] = kok && col < lo ? bv : 0.0;
+      }
+    }
+#pragma unroll
+    for (int st = 0; st < S; ++st)
+#pragma unroll
+      for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st], b[st][t], acc[t], 0, 0, 0);
+  }
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t orow = r0 + kk + 4 * v;
+      const int ocol = 16 * t + i;
+      if (orow < m && ocol < lo) {
+        const int64_t at = orow * ldz + (int64_t)ocol * zcs;
+        Z[at] = (beta != 0.0 ? beta * Z[at] : 0.0) + alpha * acc[t][v];
+      }
+    }
+}
+
 template <typename T> __device__ __forceinline__ double ldd(const T *p, int64_t i);
 template <> __device__ __forceinline__ double ldd<double>(const double *p, int64_t i) { return p[i]; }
 template <> __device__ __forceinline__ double ldd<float>(const float *p, int64_t i) { return (double)p[i]; }
@@ -1256,13 +1309,37 @@ static void apply_t(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, in
   HIP_CHECK(hipGetLastError());
 }
 
+// FP64 outputs of up to 128 columns on the FP64 matrix cores.
+template <typename TY>
+static void apply_mfma(Ctx *c, int64_t m, int64_t li, int64_t lo, const TY *Y, int64_t ldy, const double *M,
+                       double *Z, int64_t ldz, int64_t zcs) {
+  const int g = (int)(((m + 15) / 16 + 3) / 4);
+  if (lo <= 16)
+    hipLaunchKernelGGL((k_apply_mfma<TY, 1>), g, 256, 0, c->stream, m, (int)li, (int)lo, Y, ldy, M, Z, ldz, zcs, 1.0,
+                       0.0);
+  else if (lo <= 32)
+    hipLaunchKernelGGL((k_apply_mfma<TY, 2>), g, 256, 0, c->stream, m, (int)li, (int)lo, Y, ldy, M, Z, ldz, zcs, 1.0,
+                       0.0);
+  else if (lo <= 64)
+    hipLaunchKernelGGL((k_apply_mfma<TY, 4>), g, 256, 0, c->stream, m, (int)li, (int)lo, Y, ldy, M, Z, ldz, zcs, 1.0,
+                       0.0);
+  else
+    hipLaunchKernelGGL((k_apply_mfma<TY, 8>), g, 256, 0, c->stream, m, (int)li, (int)lo, Y, ldy, M, Z, ldz, zcs, 1.0,
+                       0.0);
+  HIP_CHECK(hipGetLastError());
+}
+
 void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, int ydt,
                  const double *M_host, void *Z, int64_t ldz, int zdt, int64_t zcs) {
   if (m == 0 || lo == 0) return;
   const size_t mb = (size_t)li * lo * sizeof(double);
   const double *Md = static_cast<const double *>(c->upload(M_host, mb));
   KTimer t(c, KC_DENSE, (double)m * (li * dtype_size(ydt) + lo * dtype_size(zdt)));
-  if (ydt == SVDH_F64 && zdt == SVDH_F64)
+  if (zdt == SVDH_F64 && lo <= 128 && ydt == SVDH_F32)
+    apply_mfma<float>(c, m, li, lo, (const float *)Y, ldy, Md, (double *)Z, ldz, zcs);
+  else if (zdt == SVDH_F64 && lo <= 128 && ydt == SVDH_F64)
+    apply_mfma<double>(c, m, li, lo, (const double *)Y, ldy, Md, (double *)Z, ldz, zcs);
+  else if (ydt == SVDH_F64 && zdt == SVDH_F64)
     apply_t<double, double, double>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
   else if (ydt == SVDH_F32 && zdt == SVDH_F32)
     apply_t<float, float, float>(c, m, li, lo, Y, ldy, Md, Z, ldz, zcs);
