@@ -181,8 +181,6 @@ int32_t svdh_ctx_create(int32_t device, svdh_ctx **out) {
                               hipHostMallocCoherent | hipHostMallocMapped));
       HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->dh_scal), c->h_scal, 0));
       c->red_part.alloc(1 << 20);
-      c->red_count.alloc(256);
-      HIP_CHECK(hipMemset(c->red_count.p, 0, 256));
       c->stage.alloc(1 << 16);
       // fine-grained (coherent) pinned memory: kernels read uploads from it
       // and write small results into it, with no copy engine round trip

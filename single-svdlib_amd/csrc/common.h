@@ -66,7 +66,6 @@ struct Ctx {
   double *dh_scal = nullptr;  // device address of h_scal
   // Reduction partials.
   DevBuf red_part;
-  DevBuf red_count;
   // Workgroup partials of the fused BLAS-1 dot products, per scalar slot, and
   // how many each slot's last writer left.
   DevBuf slot_parts;
