@@ -1395,12 +1395,35 @@ static bool build_tiled_schedule(svdh_op *op, int direction, const CsrView &v, S
     d.alloc(std::max<size_t>(bytes, 16));
     if (bytes) HIP_CHECK(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
   };
-  up(T.rowptr, srp.data(), srp.size() * sizeof(int32_t));
-  up(T.col, col.data(), col.size() * sizeof(uint16_t));
-  up(T.perm, perm.data(), perm.size() * sizeof(int32_t));
-  up(T.tasks, tasks.data(), tasks.size() * sizeof(int4));
-  up(T.ranges, ranges.data(), ranges.size() * sizeof(int4));
-  if (nb > 1) T.part.alloc((size_t)(nb * nout) * sizeof(double));
+  try {  // the layout is a second copy of the matrix: without room for it, the streamed SpMV runs instead
+    up(T.rowptr, srp.data(), srp.size() * sizeof(int32_t));
+    up(T.col, col.data(), col.size() * sizeof(uint16_t));
+    up(T.perm, perm.data(), perm.size() * sizeof(int32_t));
+    up(T.tasks, tasks.data(), tasks.size() * sizeof(int4));
+    up(T.ranges, ranges.data(), ranges.size() * sizeof(int4));
+    if (nb > 1) T.part.alloc((size_t)(nb * nout) * sizeof(double));
+    // the values of the operator's own dtype in the stacked order (others on first use)
+    const int bdt = op->base_dtype;
+    if (v.vals[bdt]) {
+      DevBuf &d = T.vals[bdt];
+      const size_t es = bdt == SVDH_F64 ? 8 : bdt == SVDH_F32 ? 4 : 2;
+      d.alloc(std::max<size_t>((size_t)nnz * es, 16));
+      const int g = grid_for(nnz);
+      if (bdt == SVDH_F64)
+        hipLaunchKernelGGL(k_gather<double>, g, kBlock, 0, op->ctx->stream, nnz, T.perm.as<int32_t>(),
+                           static_cast<const double *>(v.vals[bdt]), d.as<double>());
+      else if (bdt == SVDH_F32)
+        hipLaunchKernelGGL(k_gather<float>, g, kBlock, 0, op->ctx->stream, nnz, T.perm.as<int32_t>(),
+                           static_cast<const float *>(v.vals[bdt]), d.as<float>());
+      else
+        hipLaunchKernelGGL(k_gather<bf16_t>, g, kBlock, 0, op->ctx->stream, nnz, T.perm.as<int32_t>(),
+                           static_cast<const bf16_t *>(v.vals[bdt]), d.as<bf16_t>());
+      HIP_CHECK(hipGetLastError());
+    }
+  } catch (const Error &) {
+    T = Schedule::Tiled();
+    return false;
+  }
   T.on = true;
   s.built = true;
   return true;
