@@ -85,6 +85,30 @@ def test_matches_oracle(lib, shape, density, k, seed):
     assert np.allclose(rec.s, s_true[: rec.d], rtol=1e-10)
 
 
+def test_source_tiled_operator_matches_oracle(lib):
+    # 1.2 M nonzeros, 150,000 x 20,000: both SpMV directions of A^T (A x) take
+    # the source-tiled layout (2 and 10 source blocks); Pareto row lengths with
+    # a few long rows
+    rng = np.random.default_rng(23)
+    m, n = 150_000, 20_000
+    lens = np.minimum((4 / rng.uniform(size=m) ** 0.5).astype(np.int64), 3000)
+    lens[rng.choice(m, 4, replace=False)] = 12_000
+    rows = np.repeat(np.arange(m), lens)
+    a = sp.csr_matrix((rng.normal(size=rows.size), (rows, rng.integers(0, n, size=rows.size))), shape=(m, n))
+    a.sum_duplicates()
+    assert a.nnz > 1 << 20
+    print(f"case tiled LAS2 shape={a.shape} nnz={a.nnz}", flush=True)
+    rec = lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(a), 6, 5)
+    st, r = oracle.las2(oracle.OracleOp(a, nthreads=4), 6, 0, random_seed=5)
+    assert st == 0
+    compare_with_oracle(a, rec, r, diag=False)
+    assert triplet_residual(a, rec.u.T, rec.s, rec.vt) < 1e-8
+    # FP32 values with the FP64 recurrence through the same fused operator
+    rec32 = lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(a.astype(np.float32)), 6, 5)
+    assert rec32.d >= 4
+    assert np.allclose(rec32.s[:4], r["s"][:4], rtol=1e-5)
+
+
 def test_cfg1_las2_f64(lib):
     a = cfg1_matrix()
     rec = lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(a), 10, 42)
