@@ -311,6 +311,21 @@ struct Axpy { double *y; const double *x; double imm; int slot; double sign; };
 void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *u,
                     const double *v, int out_slot, double *copy_dst, const double *copy_src);
 void slots_finish(Ctx *c, int n);
+// A chain of fused ops (each as fused_axpy_dot with at most one update). On
+// vectors of up to 8,192 entries the whole chain is one single-workgroup
+// launch that also publishes the produced slots to Ctx::scal and the host
+// mirror (returns true); longer vectors take one fused_axpy_dot launch per op
+// (returns false: slots_finish publishes them).
+struct FusedOp {
+  Axpy up;
+  int nups;
+  const double *u;
+  const double *v;
+  int out_slot;
+  double *copy_dst;
+  const double *copy_src;
+};
+bool fused_chain(Ctx *c, int64_t n, const FusedOp *ops, int nops);
 // y = a x, z *= a, and (when copy_dst is set) copy_dst = copy_src, in one pass.
 void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double *z, double *copy_dst = nullptr,
                 const double *copy_src = nullptr);

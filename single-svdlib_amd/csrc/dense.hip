@@ -48,6 +48,7 @@ __device__ __forceinline__ double block_sum(double v) {
 // sums them with slot_sum, in the same order everywhere, so the device and the
 // host see the same bits. No fences or atomics between workgroups.
 constexpr int kSlotParts = 1024;  // >= fused_grid's cap
+constexpr int64_t kChainMaxN = 8192;  // vectors a single workgroup chain takes
 
 // Called by whole waves; every lane returns the sum of p[0 .. nb).
 __device__ __forceinline__ double slot_sum(const double *p, int nb) {
@@ -96,6 +97,69 @@ __global__ __launch_bounds__(kBlock) void k_fused_axpy_dot(FusedArgs a) {
   if (a.out_slot < 0) return;
   const double t = block_sum(acc);
   if (threadIdx.x == 0) a.parts[a.out_slot * kSlotParts + blockIdx.x] = t;
+}
+
+// A chain of fused ops on short vectors in one workgroup: op o's dot is
+// reduced in the workgroup (waves in order) before op o + 1 reads it, every
+// thread touches the same entries in every op, so no other wait is needed.
+// Coefficients come from slots produced earlier in the chain (the workgroup's
+// own values) or before it (their partials, slot_sum). The produced slots are
+// published as their single partial, to scal and to the host mirror.
+constexpr int kChainMax = 8;
+constexpr int kChainBlock = 1024;
+struct ChainOp {
+  AxpyD up;
+  int nups;
+  int in_chain;  // up.slot was produced earlier in this chain
+  const double *u;
+  const double *v;
+  int out_slot;
+  double *copy_dst;
+  const double *copy_src;
+};
+struct ChainArgs {
+  int64_t n;
+  int nops;
+  ChainOp ops[kChainMax];
+  double *parts;
+  double *scal;
+  double *host;
+};
+
+__global__ __launch_bounds__(kChainBlock) void k_fused_chain(ChainArgs a) {
+  __shared__ double sv[S_NSLOT];
+  __shared__ double red[kChainBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int o = 0; o < a.nops; ++o) {
+    const ChainOp &op = a.ops[o];
+    double c0 = 0.0;
+    if (op.nups > 0) {
+      if (op.up.slot < 0)
+        c0 = op.up.imm;
+      else
+        c0 = op.up.sign * (op.in_chain ? sv[op.up.slot] : slot_sum(a.parts + op.up.slot * kSlotParts, op.up.nb));
+    }
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < a.n; i += kChainBlock) {
+      if (op.nups > 0) op.up.y[i] += c0 * op.up.x[i];
+      if (op.copy_dst) op.copy_dst[i] = op.copy_src[i];
+      if (op.out_slot >= 0) acc += op.u[i] * op.v[i];
+    }
+    if (op.out_slot >= 0) {
+      acc = wave_sum_d(acc);
+      if (lane == 0) red[wv] = acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kChainBlock / 64; ++w) t += red[w];
+        sv[op.out_slot] = t;
+        a.parts[op.out_slot * kSlotParts] = t;
+        a.scal[op.out_slot] = t;
+        a.host[op.out_slot] = t;
+      }
+    }
+    __syncthreads();  // the value (and red) before the next op
+  }
 }
 
 // scal[s] = the value of slot s (one 64-thread workgroup per slot, s < n).
@@ -757,6 +821,56 @@ void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *
   KTimer t(c, KC_BLAS1, (double)n * 8.0 * (3 * nups + 2 + (copy_dst ? 2 : 0)));
   hipLaunchKernelGGL(k_fused_axpy_dot, g, kBlock, 0, c->stream, a);
   HIP_CHECK(hipGetLastError());
+}
+
+bool fused_chain(Ctx *c, int64_t n, const FusedOp *ops, int nops) {
+  if (nops <= 0) return false;
+  if (nops > kChainMax || n > kChainMaxN) {
+    for (int o = 0; o < nops; ++o)
+      fused_axpy_dot(c, n, ops[o].nups ? &ops[o].up : nullptr, ops[o].nups, ops[o].u, ops[o].v, ops[o].out_slot,
+                     ops[o].copy_dst, ops[o].copy_src);
+    return false;
+  }
+  c->slot_parts.ensure((size_t)S_NSLOT * kSlotParts * sizeof(double));
+  ChainArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.n = n;
+  a.nops = nops;
+  bool produced[S_NSLOT] = {false};
+  double bytes = 0.0;
+  for (int o = 0; o < nops; ++o) {
+    const FusedOp &f = ops[o];
+    ChainOp &d = a.ops[o];
+    d.nups = f.nups;
+    if (f.nups) {
+      d.up.y = f.up.y;
+      d.up.x = f.up.x;
+      d.up.imm = f.up.imm;
+      d.up.slot = f.up.slot;
+      d.up.sign = f.up.sign;
+      d.in_chain = f.up.slot >= 0 && produced[f.up.slot];
+      d.up.nb = f.up.slot >= 0 ? c->slot_nb[f.up.slot] : 0;
+    }
+    d.u = f.u;
+    d.v = f.v;
+    d.out_slot = f.out_slot;
+    d.copy_dst = f.copy_dst;
+    d.copy_src = f.copy_src;
+    if (f.out_slot >= 0) {
+      if (f.out_slot >= S_NSLOT) fail(SVDH_ERR_ARG, "scalar slot %d out of range", f.out_slot);
+      produced[f.out_slot] = true;
+    }
+    bytes += (double)n * 8.0 * (3 * f.nups + 2 + (f.copy_dst ? 2 : 0));
+  }
+  for (int s = 0; s < S_NSLOT; ++s)
+    if (produced[s]) c->slot_nb[s] = 1;
+  a.parts = c->slot_parts.as<double>();
+  a.scal = c->scal.as<double>();
+  a.host = c->dh_scal;
+  KTimer t(c, KC_BLAS1, bytes);
+  hipLaunchKernelGGL(k_fused_chain, 1, kChainBlock, 0, c->stream, a);
+  HIP_CHECK(hipGetLastError());
+  return true;
 }
 
 void slots_finish(Ctx *c, int n) {

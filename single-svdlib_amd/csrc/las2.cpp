@@ -242,28 +242,30 @@ struct Las2 {
       }
       vec_scale2(c, ncols, 1.0 / *rnm, w[0], w[1], w[3], stored ? nullptr : q(j - 1), w[2]);
       opb(w[3], w[0]);
-      Axpy u0{w[0], w[2], -*rnm, -1, 1.0};
-      fused_axpy_dot(c, ncols, &u0, 1, w[0], w[3], S_A, nullptr, nullptr);
+      // The step's fused ops as one chain (one launch on short vectors).
+      std::vector<FusedOp> ch;
+      ch.push_back(FusedOp{Axpy{w[0], w[2], -*rnm, -1, 1.0}, 1, w[0], w[3], S_A, nullptr, nullptr});
       if (j <= kMaxLL) {
-        read_slots(1);
+        if (!fused_chain(c, ncols, ch.data(), (int)ch.size())) slots_finish(c, 1);
+        ch.clear();
+        HIP_CHECK(hipStreamSynchronize(c->stream));
         if (std::fabs(alf[j - 1]) > 4.0 * std::fabs(c->h_scal[S_A])) *ll = j;
       }
-      // Pending update: w0 -= alf_j w1, applied by the next fused kernel.
+      // Pending update: w0 -= alf_j w1, applied by the next fused op.
       Axpy pend{w[0], w[1], 0.0, S_A, -1.0};
       const int64_t lim = std::min(j - 1, *ll);
       for (int64_t i = 0; i < lim; ++i) {
         const int slot = 8 + (int)i;
-        fused_axpy_dot(c, ncols, &pend, 1, p(i), w[0], slot, nullptr, nullptr);
+        ch.push_back(FusedOp{pend, 1, p(i), w[0], slot, nullptr, nullptr});
         pend = Axpy{w[0], q(i), 0.0, slot, -1.0};
         eta[i] = eps1;
         oldeta[i] = eps1;
       }
-      fused_axpy_dot(c, ncols, &pend, 1, w[0], w[4], S_B, nullptr, nullptr);
-      Axpy u2{w[0], w[2], 0.0, S_B, -1.0};
-      fused_axpy_dot(c, ncols, &u2, 1, w[0], w[3], S_C, nullptr, nullptr);
-      Axpy u3{w[0], w[1], 0.0, S_C, -1.0};
-      fused_axpy_dot(c, ncols, &u3, 1, w[0], w[0], S_E, w[4], w[0]);
-      read_slots(S_E + 1);
+      ch.push_back(FusedOp{pend, 1, w[0], w[4], S_B, nullptr, nullptr});
+      ch.push_back(FusedOp{Axpy{w[0], w[2], 0.0, S_B, -1.0}, 1, w[0], w[3], S_C, nullptr, nullptr});
+      ch.push_back(FusedOp{Axpy{w[0], w[1], 0.0, S_C, -1.0}, 1, w[0], w[0], S_E, w[4], w[0]});
+      if (!fused_chain(c, ncols, ch.data(), (int)ch.size())) slots_finish(c, S_E + 1);
+      HIP_CHECK(hipStreamSynchronize(c->stream));
       const double a = c->h_scal[S_A], t1 = c->h_scal[S_B], t2 = c->h_scal[S_C];
       alf[j] = a;
       if (bet[j] > 0.0) bet[j] += t1;
