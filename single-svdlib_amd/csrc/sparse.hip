@@ -845,14 +845,38 @@ struct FinArgs {
   Y *y;
   int64_t ldy;
   int32_t l;
+  int32_t nchunk;  // entries [0, nchunk) summed as k_spmm_fin does, the rest as k_spmm_fin4
 };
 
+// One workgroup per multi-row entry of every operator, each summed in the
+// order of the single-operator pass (launch_fin): chunked rows as k_spmm_fin,
+// the others in k_spmm_fin4's four chains, so a batch rounds as separate
+// products do.
 template <typename Y, typename Acc>
 __global__ __launch_bounds__(kBlock) void k_spmm_fin_batched(const FinArgs<Y, Acc> *args, const int32_t *boff,
                                                             int nb) {
   const int b = batch_of(boff, nb, blockIdx.x);
   const FinArgs<Y, Acc> &f = args[b];
-  spmm_fin_row<Y, Acc>(f.multi, f.partials, f.ldp, f.y, f.ldy, f.l, (int)blockIdx.x - boff[b]);
+  const int row = (int)blockIdx.x - boff[b];
+  if (row < f.nchunk) {
+    spmm_fin_row<Y, Acc>(f.multi, f.partials, f.ldp, f.y, f.ldy, f.l, row);
+    return;
+  }
+  const int o = threadIdx.x;
+  if (o >= f.l) return;
+  const int *md = f.multi + 3 * (int64_t)row;
+  const int r = md[0], first = md[1], nch = md[2];
+  const Acc *pp = f.partials + (int64_t)first * f.ldp + o;
+  Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
+  int i = 0;
+  for (; i + 3 < nch; i += 4) {
+    s0 += pp[(int64_t)i * f.ldp];
+    s1 += pp[(int64_t)(i + 1) * f.ldp];
+    s2 += pp[(int64_t)(i + 2) * f.ldp];
+    s3 += pp[(int64_t)(i + 3) * f.ldp];
+  }
+  for (; i < nch; ++i) s0 += pp[(int64_t)i * f.ldp];
+  f.y[(int64_t)r * f.ldy + o] = static_cast<Y>((s0 + s1) + (s2 + s3));
 }
 
 // ------------------------------------------------------------------------
@@ -2167,8 +2191,9 @@ static void launch_spmm_batched(Ctx *c, int nops, svdh_op *const *ops, int direc
       bytes += plans[b].bytes;
       const Schedule &s = *plans[b].s;
       if (s.nlong_rows > 0) {
+        const int32_t nchunk = l <= 64 ? std::min<int32_t>(s.nlong_rows, s.chunk_multi) : s.nlong_rows;
         fargs.push_back(FinArgs<Y, Acc>{s.counters.as<int32_t>(), (const Acc *)s.partials.p, plans[b].a.ldp,
-                                        plans[b].a.y, ldy, (int32_t)l});
+                                        plans[b].a.y, ldy, (int32_t)l, nchunk});
         foff.push_back(foff.back() + s.nlong_rows);
       }
     }

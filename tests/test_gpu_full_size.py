@@ -1,0 +1,104 @@
+"""The benchmark workloads at their full sizes, checked through properties
+that hold at any size (the oracle's CPU solves take minutes there):
+
+- randomized SVD, 100,000 x 20,000 Zipf CSR (nnz 9,998,163), rank 50 +
+  oversample 10 to a residual of 1e-5: the residual the solver reports is
+  the FP64 residual of the triplets it returns, recomputed on the CPU; the
+  singular vectors are orthonormal; the values are positive and descending;
+- LAS2, 500,000 x 30,000 Pareto-row CSR (nnz 19,698,610), k = 100: the FP64
+  solve's triplets satisfy A v = s u and A^T u = s v to 1e-8, its vectors are
+  orthonormal; the FP32-value solve of the same matrix agrees with it;
+- batched randomized SVD over the 64 column masks of the 200,000 x 25,000
+  Poisson(60) CSR: two of the batch's results equal separate solves of the
+  same masked views: singular values bit for bit (at 200,000 rows the A^T Y
+  rows are cut at source blocks, so this covers the order of the piece sums),
+  singular vectors to within one FP32 rounding of the panels (about 1e-8 of
+  max |U|; a few hundred of the 4,000,000 entries of U differ).
+
+The matrices come from the deterministic generator (single_svdlib.synth);
+their SHA-256 checksums pin them.
+"""
+import numpy as np
+import pytest
+
+from helpers import triplet_residual
+
+pytestmark = pytest.mark.gpu
+
+# SHA-256 of (rowptr int64, colidx int64, FP32 values) as single_svdlib.synth.checksum computes it
+SHA = {
+    "100k_x_20k": "d144cf5c9fad6b4d5bdf934df9f043f4bd2c87680bd9ee5ad02dccf59501f966",
+    "500k_x_30k": "d236152c926adbe5031f16057669d7b2c85f40e14d44b5fbbcfef1c826e95cf1",
+    "200k_x_25k": "24a469a690b004d84333f005637f3a9ff47b982cfd817f9dc9a093d37195f7d1",
+}
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import single_svdlib
+    from single_svdlib import _ffi
+    _ffi.context(0)
+    return single_svdlib
+
+
+def orthonormality_error(rows):
+    """max |Q Q^T - I| for the rows of Q (FP64)."""
+    q = np.asarray(rows, dtype=np.float64)
+    return float(np.max(np.abs(q @ q.T - np.eye(q.shape[0]))))
+
+
+def test_rsvd_100k_x_20k_reaches_tolerance(lib):
+    from single_svdlib import synth
+    a = synth.config2(np.float32)
+    print(f"case rsvd 100,000 x 20,000 nnz={a.nnz}", flush=True)
+    assert a.shape == (100_000, 20_000) and a.nnz == 9_998_163 and synth.checksum(a) == SHA["100k_x_20k"]
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a), 50, 10, 3, "QR", False, 42, False, tol=1e-5,
+                                        max_power_iters=200)
+    est = rec.extra["residual_estimate"]
+    true = triplet_residual(a, rec.u, rec.s, rec.vt)
+    assert est <= 1e-5
+    assert abs(est - true) <= 0.05 * true, (est, true)
+    assert rec.u.shape == (100_000, 50) and rec.vt.shape == (50, 20_000)
+    assert orthonormality_error(rec.u.T) < 1e-5
+    assert orthonormality_error(rec.vt) < 1e-5
+    assert np.all(rec.s > 0) and np.all(np.diff(rec.s) <= 0)
+
+
+def test_las2_500k_x_30k_triplets(lib):
+    from single_svdlib import synth
+    a32 = synth.config3(np.float32)
+    a64 = a32.astype(np.float64)  # the same matrix in both solves
+    print(f"case las2 500,000 x 30,000 nnz={a32.nnz}", flush=True)
+    assert a32.shape == (500_000, 30_000) and a32.nnz == 19_698_610 and synth.checksum(a32) == SHA["500k_x_30k"]
+    r64 = lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(a64), 100, 42)
+    assert r64.d == 100 and r64.diagnostics.significant_values >= 100
+    assert triplet_residual(a64, r64.u.T, r64.s, r64.vt) < 1e-8
+    assert orthonormality_error(r64.u) < 1e-8
+    assert orthonormality_error(r64.vt) < 1e-8
+    assert np.all(r64.s > 0) and np.all(np.diff(r64.s) <= 0)
+    r32 = lib.lanczos.svd_dim_seed(lib.CsrMatrix.from_scipy(a32), 100, 42)
+    assert r32.s.dtype == np.float32 and r32.d == 100
+    assert triplet_residual(a64, r32.u.T, r32.s.astype(np.float64), r32.vt) < 5e-4
+    assert np.allclose(r32.s, r64.s, rtol=1e-4, atol=0)
+
+
+def test_batched_64_masks_equal_separate_solves(lib):
+    from single_svdlib import synth
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a, masks = synth.config4(np.float32)
+    print(f"case batched 64 masks of 200,000 x 25,000 nnz={a.nnz}", flush=True)
+    assert synth.checksum(a) == SHA["200k_x_25k"] and sum(len(c) for c in masks) == 272_846
+    base = lib.CsrMatrix.from_scipy(a)
+    views = [MaskedCSRMatrix.with_columns(base, c) for c in masks]
+    recs = lib.randomized.randomized_svd_batched(views, 20, 10, 3, "QR", seed=7)
+    assert len(recs) == 64
+    for i in (0, 63):
+        one = lib.randomized.randomized_svd(views[i], 20, 10, 3, "QR", False, 7, False)
+        du = float(np.max(np.abs(recs[i].u - one.u)))
+        dv = float(np.max(np.abs(recs[i].vt - one.vt)))
+        print(f"mask {i}: max |dU| {du:.3e}, max |dV^T| {dv:.3e}", flush=True)
+        np.testing.assert_array_equal(recs[i].s, one.s)
+        assert du <= 1e-6 * np.abs(one.u).max()
+        assert dv <= 1e-6 * np.abs(one.vt).max()
+    for r, c in zip(recs, masks):
+        assert r.vt.shape == (20, len(c)) and np.all(np.diff(r.s) <= 0)
