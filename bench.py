@@ -437,8 +437,9 @@ def main():
                     help="rsvd (default, the headline metric) or one of the other measured paths")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a design "
-              "in DESIGN.md, not a measured path", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the multi-GPU sharding of the "
+              "path (row blocks of A, one all-reduce per product) is designed, not built or measured",
+              file=sys.stderr)
         sys.exit(2)
     if args.workload != "rsvd":
         print(json.dumps(OTHER_WORKLOADS[args.workload](args)))
