@@ -1035,7 +1035,24 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = i; j < 4; ++j) gacc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-  if (wave < nblk) panel_rows_load<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
+  // the beta term's rows of a block (lane: rows 4 kq + v, column 16 q + rl),
+  // loaded one block ahead like the inputs
+  float zold[4][4], znext[4][4];
+  auto zin_load = [&](int64_t r0, float (&dst)[4][4]) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gr = r0 + 4 * kq + v;
+        const int c = 16 * q + rl;
+        dst[q][v] = zin[(gr < rows ? gr : rows - 1) * ldzin + (c < lo ? c : 0)];
+      }
+  };
+  const bool use_zin = !RESID && beta != 0.f;
+  if (wave < nblk) {
+    panel_rows_load<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
+    if (use_zin) zin_load(wave * 16, zold);
+  }
   for (int64_t b = wave; b < nblk; b += nwaves) {
     // an offset of zero the compiler cannot see through: the B reads below
     // stay in the loop instead of being hoisted into registers
@@ -1043,17 +1060,10 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
     asm volatile("v_mov_b32 %0, 0" : "=v"(lds0));
     const int64_t r0 = b * 16;
     const bool more = b + nwaves < nblk;
-    if (more) panel_rows_load<NP>(t0, t1, rows, (b + nwaves) * 16 + rl, kq, yn);  // next block in flight
-    float zold[4][4];
-    if (!RESID && beta != 0.f)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int64_t gr = r0 + 4 * kq + v;
-          const int c = 16 * q + rl;
-          zold[q][v] = zin[(gr < rows ? gr : rows - 1) * ldzin + (c < lo ? c : 0)];
-        }
+    if (more) {  // next block in flight
+      panel_rows_load<NP>(t0, t1, rows, (b + nwaves) * 16 + rl, kq, yn);
+      if (use_zin) zin_load((b + nwaves) * 16, znext);
+    }
     f4_t acc[4] = {f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f},
                    f4_t{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
@@ -1083,7 +1093,7 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
         for (int v = 0; v < 4; ++v) {
           const int64_t gr = r0 + 4 * kq + v;
           const int c = 16 * q + rl;
-          const float val = (beta != 0.f ? beta * zold[q][v] : 0.f) + acc[q][v];
+          const float val = (use_zin ? beta * zold[q][v] : 0.f) + acc[q][v];
           const bool ok = gr < rows && c < lo;
           if (ok) z[gr * ldz + c] = val;
           zv[q][v] = ok ? val : 0.f;
@@ -1098,11 +1108,17 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
               gacc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)zv[i][v], (double)zv[j][v], gacc[i][j], 0, 0,
                                                                 0);
     }
-    if (more)
+    if (more) {
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int g = 0; g < 4; ++g) ya[p][g] = yn[p][g];
+      if (use_zin)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) zold[q][v] = znext[q][v];
+    }
   }
   if (GRAM) {
     // waves in order into LDS: gred[(blk * 4 + v) * 64 + lane], blk = 4 i + j
