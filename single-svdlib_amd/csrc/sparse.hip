@@ -889,8 +889,30 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin_batched(const FinArgs<Y, Ac
 // ------------------------------------------------------------------------
 constexpr int kHotBlock = 1024;
 
+// Lane n of each 16-lane row of the wave, to every lane of that row (DPP
+// row_newbcast, a VALU move instead of an LDS permute); n must fold to a
+// constant once the caller's loops are unrolled.
+__device__ __forceinline__ int row_bcast16(int x, int n) {
+#define SVDH_RB(N) \
+  case N: return __builtin_amdgcn_update_dpp(0, x, 0x150 + N, 0xf, 0xf, false);
+  switch (n) {
+    SVDH_RB(0) SVDH_RB(1) SVDH_RB(2) SVDH_RB(3) SVDH_RB(4) SVDH_RB(5) SVDH_RB(6) SVDH_RB(7)
+    SVDH_RB(8) SVDH_RB(9) SVDH_RB(10) SVDH_RB(11) SVDH_RB(12) SVDH_RB(13) SVDH_RB(14)
+    default: return __builtin_amdgcn_update_dpp(0, x, 0x15f, 0xf, 0xf, false);
+  }
+#undef SVDH_RB
+}
+__device__ __forceinline__ float row_bcast16(float x, int n) { return __int_as_float(row_bcast16(__float_as_int(x), n)); }
+__device__ __forceinline__ double row_bcast16(double x, int n) {
+  const long long b = __double_as_longlong(x);
+  const int lo = row_bcast16((int)(b & 0xffffffffll), n), hi = row_bcast16((int)(b >> 32), n);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // acc += sum over k in [kb, ke) of vals[k] * panel row of hcol[k]: staged
 // rows from LDS (LDS_SRC, hcol = -(1 + slot)) or panel rows from global memory.
+// With 16 lanes per row the entries reach the lanes by row broadcasts, with
+// fewer by LDS permutes.
 template <int R, bool LDS_SRC, typename V, typename X, typename Y, typename Acc>
 __device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, const int32_t *hcol,
                                                const V *hval, const X *xs, int kb, int ke, int lane, int col,
@@ -909,22 +931,46 @@ __device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, 
     const int n = min(R, ke - base);
     constexpr int U = SVDH_SPMM_UNROLL < R ? SVDH_SPMM_UNROLL : R;
     int j = 0;
-    for (; j + U <= n; j += U) {
-      int cj[U];
-      Acc vj[U];
+    if constexpr (R == 16) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        cj[u] = __shfl(c, j + u, R);
-        vj[u] = __shfl(v, j + u, R);
+      for (int jb = 0; jb < R / U; ++jb) {
+        if (jb * U + U > n) break;
+        int cj[U];
+        Acc vj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          cj[u] = row_bcast16(c, jb * U + u);
+          vj[u] = row_bcast16(v, jb * U + u);
+        }
+        if (colok) {
+          Acc xv[U][NV];
+#pragma unroll
+          for (int u = 0; u < U; ++u) ld_panel<Acc, X>(src + (int64_t)cj[u] * a.ldx + col, xv[u]);
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) acc[i] += vj[u] * xv[u][i];
+        }
+        j += U;
       }
-      if (colok) {
-        Acc xv[U][NV];
+    } else {
+      for (; j + U <= n; j += U) {
+        int cj[U];
+        Acc vj[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) ld_panel<Acc, X>(src + (int64_t)cj[u] * a.ldx + col, xv[u]);
+        for (int u = 0; u < U; ++u) {
+          cj[u] = __shfl(c, j + u, R);
+          vj[u] = __shfl(v, j + u, R);
+        }
+        if (colok) {
+          Acc xv[U][NV];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+          for (int u = 0; u < U; ++u) ld_panel<Acc, X>(src + (int64_t)cj[u] * a.ldx + col, xv[u]);
 #pragma unroll
-          for (int i = 0; i < NV; ++i) acc[i] += vj[u] * xv[u][i];
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) acc[i] += vj[u] * xv[u][i];
+        }
       }
     }
     if (j < n) {
