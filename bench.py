@@ -245,8 +245,8 @@ def bench_batched(args):
 
     def batched():
         metas = (_ffi.Result_ * nb)()
-        _ffi.check(lib.svdh_rsvd_batched_device(ctx, nb, handles, rank, over, q, _ffi.NORM_QR, 7, 1, _ffi.PREC_F32,
-                                                up, ldu, vp, ldvt, metas))
+        _ffi.check(lib.svdh_rsvd_batched_device(ctx, nb, handles, rank, over, q, _ffi.NORM_QR, 7, 1, None,
+                                                _ffi.PREC_F32, up, ldu, vp, ldvt, metas))
         s0 = [np.ctypeslib.as_array(metas[b].s, (rank,)).copy() for b in range(nb)]
         for b in range(nb):
             lib.svdh_result_free(C.byref(metas[b]))

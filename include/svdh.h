@@ -107,6 +107,10 @@ int32_t svdh_ctx_create(int32_t device, svdh_ctx **out);
 int32_t svdh_ctx_destroy(svdh_ctx *ctx);
 int32_t svdh_ctx_synchronize(svdh_ctx *ctx);
 const char *svdh_last_error(void);
+/* SHA-256 (hex) of the sources the library was built from (no GPU needed):
+ * lets a binding refuse a library older than its sources. No counterpart in
+ * the reference. */
+const char *svdh_build_id(void);
 /* Kernel timing with HIP events on the context's stream. Kernel classes:
  * 0 = SpMV, 1 = SpMM Y = A X, 2 = dense (Gram / apply / GEMM), 3 = BLAS-1,
  * 4 = basis GEMV (reorthogonalisation), 5 = SpMM Z = A^T Y, 6 = SDDMM,
@@ -180,6 +184,24 @@ int32_t svdh_spmm(svdh_ctx *ctx, const svdh_op *op, int32_t transpose_self, cons
 int32_t svdh_spmm_qta(svdh_ctx *ctx, const svdh_op *op, const void *Q, int64_t ldq, int64_t l,
                       void *B, int64_t ldb, int32_t dtype);
 
+/* SMat::multiply_with_dense_centered (src/utils.rs:29, masked.rs:473-614):
+ * svdh_spmm of the implicitly centred matrix A - 1 mu^T, means = mu (ncols
+ * entries of dtype, in the operator's column numbering, as
+ * compute_column_means returns them).
+ * transpose_self == 0: Y (nrows x l) = A X - 1 (mu^T X);
+ * transpose_self != 0: Y (ncols x l) = A^T X - mu (1^T X).
+ * The forward form is the exact one; the reference's masked.rs:496-509
+ * subtracts (sum_j mu_j)(sum_i X_ic) instead. The corrections run on the
+ * device as rank-one updates of the sparse product. */
+int32_t svdh_spmm_centered(svdh_ctx *ctx, const svdh_op *op, int32_t transpose_self, const void *X, int64_t ldx,
+                           int64_t l, void *Y, int64_t ldy, const void *means, int32_t dtype);
+
+/* SMat::multiply_transposed_by_dense_centered (src/utils.rs:32,
+ * masked.rs:735-874): B (l x ncols, column-major, ldb) =
+ * Q^T (A - 1 mu^T) = Q^T A - (Q^T 1) mu^T. */
+int32_t svdh_spmm_qta_centered(svdh_ctx *ctx, const svdh_op *op, const void *Q, int64_t ldq, int64_t l, void *B,
+                               int64_t ldb, const void *means, int32_t dtype);
+
 /* SMat::compute_column_means (src/utils.rs:27, masked.rs:240-263). means has
  * ncols entries of dtype (SVDH_F64 or SVDH_F32). */
 int32_t svdh_column_means(svdh_ctx *ctx, const svdh_op *op, void *means, int32_t dtype);
@@ -252,13 +274,16 @@ int32_t svdh_rsvd_device(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, 
  * as svdh_rsvd would with tol = 0 (exactly n_power_iters power iterations),
  * QR normaliser only. Every SpMM round over all operators is one kernel
  * launch. All operators must give the same panel width
- * min(target_rank, nrows, ncols_b) + n_oversamples <= 64. outs[b] receives
- * operator b's result (free each with svdh_result_free). No counterpart in
- * the reference, whose callers loop over randomized_svd
- * (src/randomized/mod.rs:24-199) per MaskedCSRMatrix. */
+ * min(target_rank, nrows, ncols_b) + n_oversamples <= 64. omegas (nullable,
+ * and each omegas[b] nullable) injects operator b's ncols_b x (rank +
+ * oversample) column-major FP64 test matrix, as svdh_rsvd's omega does.
+ * outs[b] receives operator b's result (free each with svdh_result_free),
+ * bit for bit the result of svdh_rsvd on that operator with the same seed
+ * (or omega). No counterpart in the reference, whose callers loop over
+ * randomized_svd (src/randomized/mod.rs:24-199) per MaskedCSRMatrix. */
 int32_t svdh_rsvd_batched(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops, int64_t target_rank,
                           int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
-                          int32_t has_seed, int32_t precision, svdh_result *outs);
+                          int32_t has_seed, const double *const *omegas, int32_t precision, svdh_result *outs);
 
 /* svdh_rsvd_batched with the singular vectors left in device memory:
  * operator b writes u[b] (m x k_b row-major, leading dimension ldu[b] >= k_b)
@@ -268,7 +293,8 @@ int32_t svdh_rsvd_batched(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops
  * and vt stay NULL. */
 int32_t svdh_rsvd_batched_device(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops, int64_t target_rank,
                                  int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
-                                 int32_t has_seed, int32_t precision, double *const *u, const int64_t *ldu,
+                                 int32_t has_seed, const double *const *omegas, int32_t precision,
+                                 double *const *u, const int64_t *ldu,
                                  double *const *vt, const int64_t *ldvt, svdh_result *metas);
 
 /* ---- adjoint (backward) ------------------------------------------------ */

@@ -20,8 +20,16 @@ vector, a closed form or a library routine chosen so a plausible mistake fails):
               nalgebra's Householder QR / SVD sign conventions are parity
               unpinned (not in the container), so comparisons use subspace
               angles and singular values.
+  svd_flip    pinned: the sign invariant (largest-magnitude entry of each
+              decided vector positive), involution and error cases
+  gaussian_matrix  pinned: moments and a Kolmogorov-Smirnov test against
+              N(0, 1) and the column-major counter order (the reference's
+              rand_distr stream is parity unpinned: crate not present)
   adjoint (oracle/adjoint.py, no reference counterpart) pinned: central
-              finite differences of the loss through numpy.linalg.svd.
+              finite differences of the loss through numpy.linalg.svd; the
+              sparse FP64 path (top triplets + CPU CG) against the dense one,
+              and at full size against central differences (make_cfg5.py)
+  centred products (multiply_*_centered): dense (A - 1 mu^T) X in numpy
   coo_to_csr  pinned: scipy.sparse COO -> CSR (structure exact; sums of
               duplicates exact on integer-valued inputs, 1 ulp otherwise),
               the triplet-sum product of SMat for CooMatrix (mod.rs:1657-1740),
@@ -39,5 +47,5 @@ from .pyoracle import (  # noqa: F401
     F64_EPS,
     F32_EPS,
 )
-from .rsvd import randomized_svd  # noqa: F401
+from .rsvd import gaussian_matrix, randomized_svd, svd_flip  # noqa: F401
 from . import adjoint  # noqa: F401

@@ -164,6 +164,27 @@ class OracleOp:
         _lib.orc_multiply_transposed_by_dense(C.byref(self._st), _p(q), l, _p(out))
         return out
 
+    def multiply_with_dense_centered(self, dense, means, transpose_self=False):
+        """SMat::multiply_with_dense_centered (masked.rs:473-614) for the
+        implicit A - 1 mu^T, mu = means (ncols). Forward: A X - 1 (mu^T X),
+        the exact form; masked.rs:496-509 subtracts (sum_j mu_j)(sum_i X_ic)
+        (defect D7). Transposed: A^T Y - mu (1^T Y) (masked.rs:576-588)."""
+        d = np.asarray(dense, dtype=np.float64)
+        mu = np.asarray(means, dtype=np.float64)
+        out = self.multiply_with_dense(d, transpose_self)
+        if not transpose_self:
+            return out - np.ones((out.shape[0], 1)) @ (mu[None, :] @ d)
+        return out - mu[:, None] @ d.sum(axis=0)[None, :]
+
+    def multiply_transposed_by_dense_centered(self, q, means):
+        """SMat::multiply_transposed_by_dense_centered (masked.rs:735-874):
+        Q^T A - (Q^T 1) mu^T; the reference spreads the correction over its
+        row chunks by their row fractions (masked.rs:839-848), which sums to
+        the same term."""
+        q = np.asarray(q, dtype=np.float64)
+        mu = np.asarray(means, dtype=np.float64)
+        return self.multiply_transposed_by_dense(q) - q.sum(axis=0)[:, None] @ mu[None, :]
+
     def column_means(self):
         out = np.empty(self.ncols, dtype=np.float64)
         _lib.orc_compute_column_means(C.byref(self._st), _p(out))

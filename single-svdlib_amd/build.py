@@ -28,9 +28,17 @@ COMMON = [
 ]
 
 
-def _compile(src):
+def _source_hash():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("svdh_srchash", os.path.join(HERE, "single_svdlib", "srchash.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.source_hash()
+
+
+def _compile(src, defs=()):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *COMMON, *defs, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, r.returncode, r.stdout + r.stderr
 
@@ -39,8 +47,10 @@ def build(jobs=None, verbose=False):
     """Compile every source and link libsvdh.so. Raises on failure."""
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
+    # api.cpp carries the hash of the sources as svdh_build_id()
+    bid = [f'-DSVDH_BUILD_ID="{_source_hash()}"']
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(lambda s: _compile(s, bid if s == "api.cpp" else ()), SOURCES))
     errors = []
     for src, _obj, rc, out in results:
         # the host pass does not know the packed-fp32 feature flag; that warning is expected

@@ -113,7 +113,8 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
 
 int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int64_t target_rank,
                            int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
-                           int32_t has_seed, int32_t precision, svdh_result *outs, const BatchedDeviceOut *dev);
+                           int32_t has_seed, const double *const *omegas, int32_t precision, svdh_result *outs,
+                           const BatchedDeviceOut *dev);
 int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32_t loss, int64_t k, const double *y,
                        int32_t precision, double tol, int64_t max_iters, void *grad, int32_t grad_dtype,
                        svdh_backward_info *info);
@@ -153,6 +154,11 @@ using namespace svdh;
 extern "C" {
 
 const char *svdh_last_error(void) { return g_last_error.c_str(); }
+
+#ifndef SVDH_BUILD_ID
+#define SVDH_BUILD_ID "unknown"
+#endif
+const char *svdh_build_id(void) { return SVDH_BUILD_ID; }
 
 int32_t svdh_ctx_create(int32_t device, svdh_ctx **out) {
   return guarded([&] {
@@ -366,8 +372,12 @@ static void to_colmajor(const std::vector<unsigned char> &src, int64_t rows, int
     for (int64_t i = 0; i < rows; ++i) std::memcpy(d + (i + j * ld) * es, &src[(i * ldr + j) * es], es);
 }
 
+// means (nullable): the centred products of the implicit A - 1 mu^T, mu the
+// ncols host means of dtype dt, as exact rank-one corrections on the device:
+//   dir 0: (A - 1 mu^T) X   = A X   - 1 (mu^T X)
+//   dir 1: (A - 1 mu^T)^T X = A^T X - mu (1^T X)
 static void host_spmm(svdh_ctx *ctx, svdh_op *op, int dir, const void *X, int64_t ldx, int64_t l, void *Y, int64_t ldy,
-                      int dt, bool transpose_out) {
+                      int dt, bool transpose_out, const void *means = nullptr) {
   const int64_t nin = dir ? op->nrows : op->ncols;
   const int64_t nout = dir ? op->ncols : op->nrows;
   if (dt != SVDH_F64 && dt != SVDH_F32) fail(SVDH_ERR_ARG, "panel dtype must be F64 or F32");
@@ -386,6 +396,18 @@ static void host_spmm(svdh_ctx *ctx, svdh_op *op, int dir, const void *X, int64_
   DevBuf dx(std::max<size_t>(hx.size(), 16)), dy(std::max<size_t>(hy.size(), 16));
   if (!hx.empty()) HIP_CHECK(hipMemcpyAsync(dx.p, hx.data(), hx.size(), hipMemcpyHostToDevice, ctx->stream));
   if (l > 0) spmm(op, dir, vuse, dx.p, ldr, dt, dy.p, ldr, dt, l);
+  DevBuf dmu;
+  if (means && l > 0) {
+    std::vector<double> mu(op->ncols);
+    for (int64_t j = 0; j < op->ncols; ++j)
+      mu[j] = dt == SVDH_F64 ? static_cast<const double *>(means)[j] : (double)static_cast<const float *>(means)[j];
+    dmu.alloc(std::max<size_t>(mu.size() * sizeof(double), 16));
+    HIP_CHECK(hipMemcpyAsync(dmu.p, mu.data(), mu.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (dir == 0)
+      panel_rank1_correct(ctx, nin, l, dx.p, ldr, dt, dmu.as<double>(), nout, dy.p, ldr, dt, nullptr, 1.0);
+    else
+      panel_rank1_correct(ctx, nin, l, dx.p, ldr, dt, nullptr, nout, dy.p, ldr, dt, dmu.as<double>(), 1.0);
+  }
   if (!hy.empty()) HIP_CHECK(hipMemcpyAsync(hy.data(), dy.p, hy.size(), hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();
   if (!transpose_out) {
@@ -414,6 +436,27 @@ int32_t svdh_spmm_qta(svdh_ctx *ctx, const svdh_op *op, const void *Q, int64_t l
     check_op(ctx, op);
     // Q^T A = (A^T Q)^T: the transposed SpMM, written transposed.
     host_spmm(ctx, const_cast<svdh_op *>(op), 1, Q, ldq, l, B, ldb, dtype, true);
+  });
+}
+
+int32_t svdh_spmm_centered(svdh_ctx *ctx, const svdh_op *op, int32_t transpose_self, const void *X, int64_t ldx,
+                           int64_t l, void *Y, int64_t ldy, const void *means, int32_t dtype) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    if (!means) fail(SVDH_ERR_ARG, "null means");
+    host_spmm(ctx, const_cast<svdh_op *>(op), transpose_self ? 1 : 0, X, ldx, l, Y, ldy, dtype, false, means);
+  });
+}
+
+int32_t svdh_spmm_qta_centered(svdh_ctx *ctx, const svdh_op *op, const void *Q, int64_t ldq, int64_t l, void *B,
+                               int64_t ldb, const void *means, int32_t dtype) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    if (!means) fail(SVDH_ERR_ARG, "null means");
+    // Q^T (A - 1 mu^T) = ((A - 1 mu^T)^T Q)^T
+    host_spmm(ctx, const_cast<svdh_op *>(op), 1, Q, ldq, l, B, ldb, dtype, true, means);
   });
 }
 
@@ -517,7 +560,7 @@ int32_t svdh_rsvd_device(svdh_ctx *ctx, const svdh_op *op, int64_t target_rank, 
 
 int32_t svdh_rsvd_batched(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops, int64_t target_rank,
                           int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
-                          int32_t has_seed, int32_t precision, svdh_result *outs) {
+                          int32_t has_seed, const double *const *omegas, int32_t precision, svdh_result *outs) {
   if (outs && nops > 0) std::memset(outs, 0, sizeof(svdh_result) * (size_t)nops);
   int32_t st = guarded([&] {
     check_ctx(ctx);
@@ -528,7 +571,7 @@ int32_t svdh_rsvd_batched(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops
       v[b] = const_cast<svdh_op *>(ops[b]);
     }
     rsvd_batched_solve(ctx, nops, v.data(), target_rank, n_oversamples, n_power_iters, normalizer, seed, has_seed,
-                       precision, outs, nullptr);
+                       omegas, precision, outs, nullptr);
   });
   if (st != SVDH_OK && outs)
     for (int64_t b = 0; b < nops; ++b) svdh_result_free(&outs[b]);
@@ -537,7 +580,8 @@ int32_t svdh_rsvd_batched(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops
 
 int32_t svdh_rsvd_batched_device(svdh_ctx *ctx, int64_t nops, const svdh_op *const *ops, int64_t target_rank,
                                  int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
-                                 int32_t has_seed, int32_t precision, double *const *u, const int64_t *ldu,
+                                 int32_t has_seed, const double *const *omegas, int32_t precision,
+                                 double *const *u, const int64_t *ldu,
                                  double *const *vt, const int64_t *ldvt, svdh_result *metas) {
   if (metas && nops > 0) std::memset(metas, 0, sizeof(svdh_result) * (size_t)nops);
   int32_t st = guarded([&] {
@@ -551,7 +595,7 @@ int32_t svdh_rsvd_batched_device(svdh_ctx *ctx, int64_t nops, const svdh_op *con
     }
     BatchedDeviceOut dev{u, ldu, vt, ldvt};
     rsvd_batched_solve(ctx, nops, v.data(), target_rank, n_oversamples, n_power_iters, normalizer, seed, has_seed,
-                       precision, metas, &dev);
+                       omegas, precision, metas, &dev);
   });
   if (st != SVDH_OK && metas)
     for (int64_t b = 0; b < nops; ++b) svdh_result_free(&metas[b]);

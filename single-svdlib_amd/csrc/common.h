@@ -393,6 +393,12 @@ void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const
 // is one launch. Equal to panel_mfma on each panel, bit for bit.
 void panel_mfma_batched(Ctx *c, int64_t count, const int64_t *rows, int64_t li, const int64_t *lo, float *const *Z,
                         int64_t ld, const std::vector<double> *M);
+// panel_mfma_batched plus each panel's FP64 Gram Z_b^T Z_b accumulated in the
+// same kernel (as panel_mfma_gram does for one panel, bit for bit); waits for
+// them and returns G[b] (lo[b] x lo[b], row-major) on the host.
+void panel_mfma_gram_batched(Ctx *c, int64_t count, const int64_t *rows, int64_t li, const int64_t *lo,
+                             float *const *Z, int64_t ld, const std::vector<double> *M,
+                             std::vector<std::vector<double>> &G);
 // panel_mfma plus the FP64 Gram Z^T Z of its output, accumulated in the same
 // kernel: the equivalent of panel_mfma followed by gram_async(Z, gram_off,
 // gram_ev) without reading Z again.

@@ -1178,6 +1178,9 @@ void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z, in
 // Y_b): job b = blockIdx.y takes workgroups 0 .. nblk - 1 of the x dimension.
 // Each row block is computed by one wave as in k_panel_mfma, so the result
 // does not depend on the grid.
+// GRAM: the job's workgroup partials of Z^T Z go to part (nblk x lo x lo) and
+// k_gram_reduce_jobs sums them into gram, as k_panel_mfma<1, false, true> and
+// k_gram_reduce do for one panel.
 struct PanelJob {
   PanelTerm t;
   float *z;
@@ -1185,14 +1188,25 @@ struct PanelJob {
   int64_t rows;
   int64_t nblk;
   int lo;
+  double *part;
+  double *gram;
 };
 
+template <bool GRAM>
 __global__ __launch_bounds__(kPanelBlock) __attribute__((amdgpu_waves_per_eu(2)))
 void k_panel_mfma_batched(const PanelJob *jobs) {
   const PanelJob j = jobs[blockIdx.y];
   if ((int64_t)blockIdx.x >= j.nblk) return;
-  panel_mfma_body<1, false, false>(blockIdx.x, j.nblk, j.rows, j.lo, j.t, j.t, j.z, j.ldz, j.z, j.ldz, 0.f,
-                                   nullptr);
+  double *pw = GRAM ? j.part + (int64_t)blockIdx.x * j.lo * j.lo : nullptr;
+  panel_mfma_body<1, false, GRAM>(blockIdx.x, j.nblk, j.rows, j.lo, j.t, j.t, j.z, j.ldz, j.z, j.ldz, 0.f, pw);
+}
+
+// k_gram_reduce for job b = blockIdx.y (grid x: reduce_grid of the widest job).
+__global__ __launch_bounds__(kBlock) void k_gram_reduce_jobs(const PanelJob *jobs) {
+  const PanelJob j = jobs[blockIdx.y];
+  const int64_t ll = (int64_t)j.lo * j.lo;
+  if ((int64_t)blockIdx.x * 16 >= ll) return;  // the whole workgroup leaves together
+  gram_reduce_body((int)j.nblk, ll, j.part, j.gram, blockIdx.x);
 }
 
 // Workgroups per CU of the panel kernels (SVDH_PANEL_WG, default 1): the
@@ -1282,44 +1296,91 @@ void panel_mfma_gram(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *
                   gram_ev);
 }
 
-void panel_mfma_batched(Ctx *c, int64_t count, const int64_t *rows, int64_t li, const int64_t *lo, float *const *Z,
-                        int64_t ld, const std::vector<double> *M) {
+static void panel_mfma_batched_impl(Ctx *c, int64_t count, const int64_t *rows, int64_t li, const int64_t *lo,
+                                    float *const *Z, int64_t ld, const std::vector<double> *M,
+                                    std::vector<std::vector<double>> *G) {
   if (li > 64 || ld % 4) fail(SVDH_ERR_ARG, "panel_mfma_batched: li <= 64 and ld a multiple of 4");
   constexpr int W = kPanelBlock / 64;
-  // groups whose factors fit one upload slot, one launch per group
+  const bool gram = G != nullptr;
+  if (gram) G->assign(count, {});
+  // groups whose factors fit one upload slot (and, with the Grams, whose
+  // Grams fit the landing area), one launch per group
   int64_t b0 = 0;
   while (b0 < count) {
     int64_t b1 = b0;
-    size_t doubles = 0;
-    while (b1 < count && (doubles + M[b1].size()) * sizeof(double) <= Ctx::kRingSlot) doubles += M[b1++].size();
+    size_t doubles = 0, landing = 0;
+    while (b1 < count && (doubles + M[b1].size()) * sizeof(double) <= Ctx::kRingSlot &&
+           (!gram || landing + (size_t)(lo[b1] * lo[b1]) <= Ctx::kSmall)) {
+      doubles += M[b1].size();
+      landing += (size_t)(lo[b1] * lo[b1]);
+      ++b1;
+    }
     if (b1 == b0) fail(SVDH_ERR_ARG, "panel_mfma_batched: factor larger than the upload slot");
     std::vector<double> mall;
     mall.reserve(doubles);
     for (int64_t b = b0; b < b1; ++b) mall.insert(mall.end(), M[b].begin(), M[b].end());
     const double *dm = static_cast<const double *>(c->upload(mall.data(), mall.size() * sizeof(double)));
     std::vector<PanelJob> jobs(b1 - b0);
-    int64_t maxblk = 1;
+    int64_t maxblk = 1, maxll = 1;
     double bytes = 0.0;
-    size_t moff = 0;
+    size_t moff = 0, poff = 0, goff = 0;
+    std::vector<size_t> part_off(b1 - b0), gram_off(b1 - b0);
     for (int64_t b = b0; b < b1; ++b) {
       if (lo[b] > 64) fail(SVDH_ERR_ARG, "panel_mfma_batched: lo <= 64");
       const int64_t nblk16 = (rows[b] + 15) / 16;
       const int64_t nblk =
           std::max<int64_t>(1, std::min<int64_t>((nblk16 + W - 1) / W, (int64_t)panel_wg_per_cu() * c->ncu));
-      jobs[b - b0] = PanelJob{PanelTerm{Z[b], ld, (int)li, dm + moff, 1.0}, Z[b], ld, rows[b], nblk, (int)lo[b]};
+      jobs[b - b0] = PanelJob{PanelTerm{Z[b], ld, (int)li, dm + moff, 1.0}, Z[b], ld, rows[b], nblk, (int)lo[b],
+                              nullptr, nullptr};
+      part_off[b - b0] = poff;
+      gram_off[b - b0] = goff;
+      poff += (size_t)nblk * lo[b] * lo[b];
+      goff += (size_t)(lo[b] * lo[b]);
       moff += M[b].size();
       maxblk = std::max(maxblk, nblk);
+      maxll = std::max<int64_t>(maxll, lo[b] * lo[b]);
       bytes += (double)rows[b] * (li + lo[b]) * 4.0;
+    }
+    if (gram) {
+      c->red_part.ensure(std::max<size_t>(std::max<size_t>(poff, 1) * sizeof(double), c->red_part.bytes));
+      for (int64_t b = b0; b < b1; ++b) {
+        jobs[b - b0].part = c->red_part.as<double>() + part_off[b - b0];
+        jobs[b - b0].gram = c->d_small + gram_off[b - b0];
+      }
     }
     const PanelJob *dj = static_cast<const PanelJob *>(c->upload(jobs.data(), jobs.size() * sizeof(PanelJob)));
     {
       KTimer tm(c, KC_DENSE, bytes);
-      hipLaunchKernelGGL(k_panel_mfma_batched, dim3((unsigned)maxblk, (unsigned)(b1 - b0)), kPanelBlock, 0,
-                         c->stream, dj);
+      const dim3 grid((unsigned)maxblk, (unsigned)(b1 - b0));
+      if (gram) {
+        hipLaunchKernelGGL(k_panel_mfma_batched<true>, grid, kPanelBlock, 0, c->stream, dj);
+        HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_gram_reduce_jobs, dim3((unsigned)reduce_grid(maxll), (unsigned)(b1 - b0)), kBlock, 0,
+                           c->stream, dj);
+      } else {
+        hipLaunchKernelGGL(k_panel_mfma_batched<false>, grid, kPanelBlock, 0, c->stream, dj);
+      }
       HIP_CHECK(hipGetLastError());
+    }
+    if (gram) {
+      HIP_CHECK(hipEventRecord(c->small_ev, c->stream));
+      HIP_CHECK(hipEventSynchronize(c->small_ev));
+      for (int64_t b = b0; b < b1; ++b)
+        (*G)[b].assign(c->h_small + gram_off[b - b0], c->h_small + gram_off[b - b0] + lo[b] * lo[b]);
     }
     b0 = b1;
   }
+}
+
+void panel_mfma_batched(Ctx *c, int64_t count, const int64_t *rows, int64_t li, const int64_t *lo, float *const *Z,
+                        int64_t ld, const std::vector<double> *M) {
+  panel_mfma_batched_impl(c, count, rows, li, lo, Z, ld, M, nullptr);
+}
+
+void panel_mfma_gram_batched(Ctx *c, int64_t count, const int64_t *rows, int64_t li, const int64_t *lo,
+                             float *const *Z, int64_t ld, const std::vector<double> *M,
+                             std::vector<std::vector<double>> &G) {
+  panel_mfma_batched_impl(c, count, rows, li, lo, Z, ld, M, &G);
 }
 
 __global__ void k_fetch(const uint4 *src, uint4 *dst, int64_t n16) {

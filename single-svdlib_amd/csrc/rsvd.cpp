@@ -597,7 +597,8 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
 // tol = 0, so its result is the one a separate solve would return.
 int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int64_t target_rank,
                            int64_t n_oversamples, int64_t n_power_iters, int32_t normalizer, uint64_t seed,
-                           int32_t has_seed, int32_t precision, svdh_result *outs, const BatchedDeviceOut *dev) {
+                           int32_t has_seed, const double *const *omegas, int32_t precision, svdh_result *outs,
+                           const BatchedDeviceOut *dev) {
   if (nops <= 0) return SVDH_OK;
   if (normalizer != SVDH_NORM_QR) fail(SVDH_ERR_ARG, "the batched solver supports the QR normaliser");
   if (precision != SVDH_PREC_F64 && precision != SVDH_PREC_F32) fail(SVDH_ERR_ARG, "bad precision %d", precision);
@@ -637,7 +638,18 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
     Yn[b] = p[1];
     Z[b] = p[2];
     Om[b] = p[3];
-    panel_gaussian(c, nb, l, sd, Om[b], ld, pdt);
+    if (omegas && omegas[b]) {
+      // injected n_b x l column-major FP64 Omega, as rsvd_solve takes it
+      std::vector<double> h((size_t)nb * ld, 0.0);
+      for (int64_t j = 0; j < l; ++j)
+        for (int64_t i = 0; i < nb; ++i) h[i * ld + j] = omegas[b][i + j * nb];
+      DevBuf tmp(h.size() * sizeof(double));  // freed after the synchronisation below
+      HIP_CHECK(hipMemcpyAsync(tmp.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      panel_convert(c, nb, l, tmp.p, ld, SVDH_F64, Om[b], ld, pdt);
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    } else {
+      panel_gaussian(c, nb, l, sd, Om[b], ld, pdt);
+    }
   }
   std::vector<svdh_op *> opv(ops, ops + nops);
   spmm_batched(c, (int)nops, opv.data(), 0, pdt, Om.data(), ld, pdt, Y.data(), ld, pdt, l);  // Y = A Omega
@@ -681,6 +693,7 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
         G);
   normalise_all(G);
   int64_t iters_run = 0;
+  bool gz_done = false;  // GZ[b] (wq[b] x wq[b]) came with the last Z = (A^T Y) Mq
   for (;;) {
     const bool final_fixed = iters_run >= n_power_iters;
     bool all_mfma = pdt == SVDH_F32 && l <= 64;
@@ -688,7 +701,12 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
     if (all_mfma) {  // Z = (A^T Y) Mq for every mask, the same kernels as rsvd_solve, one launch per group
       std::vector<float *> zf(nops);
       for (int64_t b = 0; b < nops; ++b) zf[b] = static_cast<float *>(Z[b]);
-      panel_mfma_batched(c, nops, nrows.data(), l, wq.data(), zf.data(), ld, Mq.data());
+      if (final_fixed) {  // with the Gram of Z from the same kernel, as rsvd_solve's last step
+        panel_mfma_gram_batched(c, nops, nrows.data(), l, wq.data(), zf.data(), ld, Mq.data(), GZ);
+        gz_done = true;
+      } else {
+        panel_mfma_batched(c, nops, nrows.data(), l, wq.data(), zf.data(), ld, Mq.data());
+      }
     }
     for (int64_t b = 0; b < nops && !all_mfma; ++b) {  // Z = (A^T Y) Mq, the same kernels as rsvd_solve
       if (pdt == SVDH_F32 && l <= 64 && wq[b] <= 64) {
@@ -713,14 +731,14 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
     normalise_all(G);
   }
   // B^T = Z_b; SVD(B) from its Gram, as in rsvd_solve
-  std::vector<int64_t> zw(wq);
-  grams(Z, nrows, [] {}, GZ);
+  if (!gz_done) grams(Z, nrows, [] {}, GZ);
   for (int64_t b = 0; b < nops; ++b) {
     svdh_op *op = ops[b];
     const int64_t n = op->ncols, w = wq[b];
     std::vector<double> Gb(w * w), lam, W;
+    const int64_t gld = gz_done ? w : l;
     for (int64_t i = 0; i < w; ++i)
-      for (int64_t j = 0; j < w; ++j) Gb[i * w + j] = GZ[b][i * l + j];
+      for (int64_t j = 0; j < w; ++j) Gb[i * w + j] = GZ[b][i * gld + j];
     desc_eig(w, Gb, lam, W);
     const int64_t k = std::min(target_rank, std::min(w, n));
     std::vector<double> sv(k), MW(l * k, 0.0), Ws(w * k);

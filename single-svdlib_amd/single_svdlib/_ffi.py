@@ -10,7 +10,7 @@ import threading
 
 import numpy as np
 
-from . import error
+from . import error, srchash
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libsvdh.so")
@@ -77,6 +77,7 @@ SIGNATURES = [
     ("svdh_ctx_destroy", _i32, [_vp]),
     ("svdh_ctx_synchronize", _i32, [_vp]),
     ("svdh_last_error", C.c_char_p, []),
+    ("svdh_build_id", C.c_char_p, []),
     ("svdh_ctx_profile", _i32, [_vp, _i32]),
     ("svdh_ctx_kernel_time", _i32, [_vp, _i32, _f64p, _i64p, _f64p]),
     ("svdh_csr_create", _i32, [_vp, _i64, _i64, _i64, _i64p, _i64p, _vp, _i32, C.c_uint32, C.POINTER(_vp)]),
@@ -88,6 +89,8 @@ SIGNATURES = [
     ("svdh_spmv", _i32, [_vp, _vp, _i32, _vp, _vp, _i32, _i32]),
     ("svdh_spmm", _i32, [_vp, _vp, _i32, _vp, _i64, _i64, _vp, _i64, _i32]),
     ("svdh_spmm_qta", _i32, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _i32]),
+    ("svdh_spmm_centered", _i32, [_vp, _vp, _i32, _vp, _i64, _i64, _vp, _i64, _vp, _i32]),
+    ("svdh_spmm_qta_centered", _i32, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i32]),
     ("svdh_column_means", _i32, [_vp, _vp, _vp, _i32]),
     ("svdh_las2", _i32, [_vp, _vp, _i64, _i64, C.c_double, C.c_double, C.c_double, C.c_uint32, _f64p, _i32,
                          C.POINTER(Result_)]),
@@ -98,10 +101,11 @@ SIGNATURES = [
                          C.POINTER(Result_)]),
     ("svdh_rsvd_device", _i32, [_vp, _vp, _i64, _i64, _i64, _i32, _i32, C.c_uint64, _i32, _f64p, _i32, C.c_double,
                                 _i64, _vp, _i64, _f64p, _vp, _i64, C.POINTER(Result_)]),
-    ("svdh_rsvd_batched", _i32, [_vp, _i64, C.POINTER(_vp), _i64, _i64, _i64, _i32, C.c_uint64, _i32, _i32,
-                                 C.POINTER(Result_)]),
-    ("svdh_rsvd_batched_device", _i32, [_vp, _i64, C.POINTER(_vp), _i64, _i64, _i64, _i32, C.c_uint64, _i32, _i32,
-                                        C.POINTER(_vp), _i64p, C.POINTER(_vp), _i64p, C.POINTER(Result_)]),
+    ("svdh_rsvd_batched", _i32, [_vp, _i64, C.POINTER(_vp), _i64, _i64, _i64, _i32, C.c_uint64, _i32,
+                                 C.POINTER(_f64p), _i32, C.POINTER(Result_)]),
+    ("svdh_rsvd_batched_device", _i32, [_vp, _i64, C.POINTER(_vp), _i64, _i64, _i64, _i32, C.c_uint64, _i32,
+                                        C.POINTER(_f64p), _i32, C.POINTER(_vp), _i64p, C.POINTER(_vp), _i64p,
+                                        C.POINTER(Result_)]),
     ("svdh_backward", _i32, [_vp, _vp, C.POINTER(Result_), _i32, _i64, _f64p, _i32, C.c_double, _i64, _vp, _i32,
                              C.POINTER(BackwardInfo_)]),
     ("svdh_result_free", None, [C.POINTER(Result_)]),
@@ -125,6 +129,11 @@ def lib():
                     f = getattr(h, name)
                     f.restype = res
                     f.argtypes = args
+                built = h.svdh_build_id().decode()
+                want = srchash.source_hash()
+                if want is not None and built != want:
+                    raise ImportError(f"{LIB_PATH} was built from other sources (build id {built[:16]}, sources "
+                                      f"{want[:16]}): rebuild with `python single-svdlib_amd/build.py`")
                 _lib = h
     return _lib
 

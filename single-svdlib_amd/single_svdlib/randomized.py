@@ -51,11 +51,14 @@ def randomized_svd(m, target_rank, n_oversamples, n_power_iters, power_iteration
 
 
 def randomized_svd_batched(matrices, target_rank, n_oversamples, n_power_iters,
-                           power_iteration_normalizer=PowerIterationNormalizer.QR, seed=None, *, precision=None):
+                           power_iteration_normalizer=PowerIterationNormalizer.QR, seed=None, *, precision=None,
+                           omegas=None):
     """Many randomized_svd solves over column masks of one matrix (a list of
     MaskedCSRMatrix views of the same CsrMatrix), every SpMM round of all of
     them in one launch. Returns one SvdRec per matrix, equal to what
-    randomized_svd(matrix, ..., mean_center=False, seed) returns for it."""
+    randomized_svd(matrix, ..., mean_center=False, seed, omega=omegas[b])
+    returns for it. omegas (optional) holds one ncols_b x (rank + oversample)
+    test matrix per matrix (None entries draw from seed)."""
     ops = [as_operator(m) for m in matrices]
     if not ops:
         return []
@@ -69,9 +72,20 @@ def randomized_svd_batched(matrices, target_rank, n_oversamples, n_power_iters,
         prec = {"f64": _ffi.PREC_F64, "f32": _ffi.PREC_F32}[precision]
     handles = (C.c_void_p * len(ops))(*[o._handle for o in ops])
     outs = (_ffi.Result_ * len(ops))()
+    om_ptrs = None
+    keep = []
+    if omegas is not None:
+        if len(omegas) != len(ops):
+            raise ValueError("one omega per matrix")
+        om_ptrs = (_ffi._f64p * len(ops))()
+        for i, om in enumerate(omegas):
+            if om is not None:
+                a = np.asfortranarray(om, dtype=np.float64)
+                keep.append(a)
+                om_ptrs[i] = _ffi.ptr(a)
     _ffi.check(_ffi.lib().svdh_rsvd_batched(ops[0]._ctx, len(ops), handles, int(target_rank), int(n_oversamples),
                                             int(n_power_iters), norm.value, int(seed or 0) & 0xFFFFFFFFFFFFFFFF,
-                                            int(seed is not None), prec, outs))
+                                            int(seed is not None), om_ptrs, prec, outs))
     return [result_to_svdrec(outs[i], dtype) for i in range(len(ops))]
 
 

@@ -88,7 +88,9 @@ class SMat(Protocol):
     def svd_opa(self, x, y, transposed: bool) -> None: ...
     def compute_column_means(self): ...
     def multiply_with_dense(self, dense, result, transpose_self: bool) -> None: ...
+    def multiply_with_dense_centered(self, dense, result, transpose_self: bool, means) -> None: ...
     def multiply_transposed_by_dense(self, q, result) -> None: ...
+    def multiply_transposed_by_dense_centered(self, q, result, means) -> None: ...
 
 
 def result_to_svdrec(res, dtype, extra=None):
@@ -167,6 +169,48 @@ class _DeviceOp:
         _ffi.check(_ffi.lib().svdh_spmm(self._ctx, self._handle, int(bool(transpose_self)), x.ctypes.data,
                                         max(rows_in, 1), l, y.ctypes.data, max(rows_out, 1), code))
         result[...] = y
+
+    def multiply_with_dense_centered(self, dense, result, transpose_self, means):
+        """result = (A - 1 means^T) dense, or its transpose times dense
+        (utils.rs:29, masked.rs:473-614; the forward product in its exact
+        form A X - 1 (means^T X))."""
+        nr, nc, _ = self._shape()
+        rows_out, rows_in = (nc, nr) if transpose_self else (nr, nc)
+        if dense.shape[0] != rows_in:
+            raise ValueError("Dense matrix has incompatible row count")
+        if result.shape[0] != rows_out:
+            raise ValueError("Result matrix has incompatible row count")
+        if result.shape[1] != dense.shape[1]:
+            raise ValueError("Result matrix has incompatible column count")
+        if len(means) != nc:
+            raise ValueError(f"means must have A.ncols() = {nc} entries, got {len(means)}")
+        code, dt = self._vec_dtype()
+        x = np.asfortranarray(dense, dtype=dt)
+        mu = np.ascontiguousarray(means, dtype=dt)
+        l = x.shape[1]
+        y = np.zeros((rows_out, l), dtype=dt, order="F")
+        _ffi.check(_ffi.lib().svdh_spmm_centered(self._ctx, self._handle, int(bool(transpose_self)), x.ctypes.data,
+                                                 max(rows_in, 1), l, y.ctypes.data, max(rows_out, 1), mu.ctypes.data,
+                                                 code))
+        result[...] = y
+
+    def multiply_transposed_by_dense_centered(self, q, result, means):
+        """result (l x ncols) = q^T (A - 1 means^T) (utils.rs:32, masked.rs:735-874)."""
+        nr, nc, _ = self._shape()
+        if q.shape[0] != nr:
+            raise ValueError(f"Q matrix has incompatible row count: expected {nr}, got {q.shape[0]}")
+        if result.shape != (q.shape[1], nc):
+            raise ValueError("Result matrix has incompatible shape")
+        if len(means) != nc:
+            raise ValueError(f"means must have A.ncols() = {nc} entries, got {len(means)}")
+        code, dt = self._vec_dtype()
+        x = np.asfortranarray(q, dtype=dt)
+        mu = np.ascontiguousarray(means, dtype=dt)
+        l = x.shape[1]
+        b = np.zeros((l, nc), dtype=dt, order="F")
+        _ffi.check(_ffi.lib().svdh_spmm_qta_centered(self._ctx, self._handle, x.ctypes.data, max(nr, 1), l,
+                                                     b.ctypes.data, max(l, 1), mu.ctypes.data, code))
+        result[...] = b
 
     def multiply_transposed_by_dense(self, q, result):
         """result (l x ncols) = q^T A (masked.rs:616-733)."""

@@ -41,6 +41,49 @@ def test_batched_equals_separate_solves(lib, dtype):
         assert r.diagnostics.iterations == 2 and r.extra["power_iters_run"] == 2
 
 
+@pytest.mark.parametrize("dtype,sv_tol,ang_tol", [(np.float64, 1e-10, 1e-8), (np.float32, 1e-5, 1e-3)])
+def test_batched_matches_oracle_on_physical_subsets(lib, dtype, sv_tol, ang_tol):
+    # every mask against the FP64 oracle restatement of randomized_svd
+    # (randomized/mod.rs:24-199) on the physical column subset a[:, cols]
+    # (the masked == physical invariant, masked.rs:914-1000), with the same
+    # injected test matrix per mask
+    import oracle
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(3000, 1000, 0.02, seed=31).astype(dtype)
+    base = lib.CsrMatrix.from_scipy(a)
+    cols = masks_of(1000, [300, 800, 150, 1000, 520], seed=12)
+    rng = np.random.default_rng(3)
+    oms = [rng.normal(size=(len(c), 20)) for c in cols]
+    recs = lib.randomized.randomized_svd_batched([MaskedCSRMatrix.with_columns(base, c) for c in cols], 10, 10, 2,
+                                                 "QR", seed=1, omegas=oms)
+    a64 = a.astype(np.float64)
+    for c, om, r in zip(cols, oms, recs):
+        u, s, vt = oracle.randomized_svd(oracle.OracleOp(a64[:, c].tocsr()), 10, 10, 2, "QR", omega=om)
+        assert r.d == 10 and r.u.shape == (3000, 10) and r.vt.shape == (10, len(c))
+        err = float(np.max(np.abs(r.s - s) / s))
+        assert err < sv_tol, err
+        assert max_principal_angle(r.u[:, :5], u[:, :5]) < ang_tol
+        assert max_principal_angle(r.vt[:5].T, vt[:5].T) < ang_tol
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batched_injected_omegas_equal_separate_solves(lib, dtype):
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(2500, 900, 0.02, seed=37).astype(dtype)
+    base = lib.CsrMatrix.from_scipy(a)
+    cols = masks_of(900, [200, 900, 450], seed=4)
+    rng = np.random.default_rng(8)
+    oms = [rng.normal(size=(len(c), 16)) for c in cols]
+    oms[1] = None  # drawn from the seed, as the separate solve with that seed
+    views = [MaskedCSRMatrix.with_columns(base, c) for c in cols]
+    recs = lib.randomized.randomized_svd_batched(views, 8, 8, 2, "QR", seed=6, omegas=oms)
+    for v, om, r in zip(views, oms, recs):
+        one = lib.randomized.randomized_svd(v, 8, 8, 2, "QR", False, 6, False, omega=om)
+        np.testing.assert_array_equal(r.s, one.s)
+        np.testing.assert_array_equal(r.u, one.u)
+        np.testing.assert_array_equal(r.vt, one.vt)
+
+
 def test_batched_masked_equals_physical(lib):
     from single_svdlib.lanczos.masked import MaskedCSRMatrix
     a = random_csr(2500, 900, 0.03, seed=23)
@@ -87,7 +130,7 @@ def test_batched_device_outputs_equal_host_outputs(lib):
     ldu = (C.c_int64 * nb)(*[k + 2] * nb)
     ldvt = (C.c_int64 * nb)(*[len(c) + 3 for c in cols])
     _ffi.check(_ffi.lib().svdh_rsvd_batched_device(_ffi.context(0), nb, handles, 6, 6, 2, _ffi.NORM_QR, 3, 1,
-                                                   _ffi.PREC_F64, up, ldu, vp, ldvt, metas))
+                                                   None, _ffi.PREC_F64, up, ldu, vp, ldvt, metas))
     for b in range(nb):
         assert metas[b].d == k and not metas[b].u and not metas[b].vt
         s = np.ctypeslib.as_array(metas[b].s, (k,)).copy()

@@ -327,3 +327,48 @@ def test_spmm_hot_columns_staged_in_lds(svdlib, dtype):
             del os.environ["SVDH_SPMM_HOT"]
         check_close(Y2, want, bound(a64, X.astype(np.float64), eps) + eps * np.abs(want))
         assert not np.array_equal(Y, np.zeros_like(Y))
+
+
+@pytest.mark.parametrize("name", ["uniform", "wide", "skewed"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("masked", [False, True])
+def test_spmm_centered(svdlib, name, dtype, masked):
+    # SMat::multiply_with_dense_centered / multiply_transposed_by_dense_centered
+    # (utils.rs:29, 32; masked.rs:473-614, 735-874) against the oracle's
+    # centred products (the exact forward form, defect D7 corrected)
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = MATRICES[name]().astype(dtype)
+    print(f"case centred {name} {np.dtype(dtype).name} masked={masked}", flush=True)
+    n = a.shape[1]
+    cols = np.sort(np.random.default_rng(7).choice(n, max(1, n * 2 // 3), replace=False)) if masked else None
+    base = svdlib.CsrMatrix.from_scipy(a)
+    op = MaskedCSRMatrix.with_columns(base, cols) if masked else base
+    a64 = a.astype(np.float64)
+    ref = oracle.OracleOp(a64, columns=cols) if masked else oracle.OracleOp(a64)
+    sub = a64[:, cols] if masked else a64
+    mu = ref.column_means().astype(dtype)
+    mu64 = mu.astype(np.float64)
+    eps = EPS64 if dtype == np.float64 else EPS32
+    rng = np.random.default_rng(3)
+    nc = sub.shape[1]
+    X = rng.normal(size=(nc, 9)).astype(dtype)
+    Yin = rng.normal(size=(a.shape[0], 9)).astype(dtype)
+    X64, Y64 = X.astype(np.float64), Yin.astype(np.float64)
+
+    got = np.zeros((a.shape[0], 9), dtype=dtype)
+    op.multiply_with_dense_centered(X, got, False, mu)
+    want = ref.multiply_with_dense_centered(X64, mu64)
+    tol = bound(sub, X64, eps) + (nc + 2) * eps * (np.abs(mu64) @ np.abs(X64))[None, :] + 2 * eps * np.abs(want)
+    check_close(got, want, tol)
+
+    got = np.zeros((nc, 9), dtype=dtype)
+    op.multiply_with_dense_centered(Yin, got, True, mu)
+    want = ref.multiply_with_dense_centered(Y64, mu64, True)
+    tol = (bound(sub.T, Y64, eps) + (a.shape[0] + 2) * eps * np.abs(mu64)[:, None] * np.abs(Y64).sum(0)[None, :]
+           + 2 * eps * np.abs(want))
+    check_close(got, want, tol)
+
+    got = np.zeros((9, nc), dtype=dtype)
+    op.multiply_transposed_by_dense_centered(Yin, got, mu)
+    want = ref.multiply_transposed_by_dense_centered(Y64, mu64)
+    check_close(got, want, tol.T)

@@ -209,3 +209,33 @@ def test_wide_panels_match_oracle(lib, rank, over):
     u, s, vt = oracle.randomized_svd(oracle.OracleOp(a64), rank, over, 2, "QR", omega=om)
     assert np.allclose(rec.s, s, rtol=2e-5, atol=0), np.max(np.abs(rec.s - s) / s)
     assert max_principal_angle(rec.u[:, :5], u[:, :5]) < 1e-3
+
+
+def test_device_test_matrix_is_the_documented_generator(lib):
+    # the Omega drawn on the device for a seed (k_gauss) is the counter-based
+    # Box-Muller stream restated by oracle.gaussian_matrix: the seeded solve
+    # equals the solve with that matrix injected (FP64: the device's log and
+    # cospi may differ from numpy's in the last bit)
+    a = random_csr(2000, 700, 0.02, seed=44)
+    op = lib.CsrMatrix.from_scipy(a)
+    for seed in (0, 12345):
+        om = oracle.gaussian_matrix(700, 25, seed)
+        r1 = lib.randomized.randomized_svd(op, 15, 10, 1, "QR", False, seed, False)
+        r2 = lib.randomized.randomized_svd(op, 15, 10, 1, "QR", False, seed, False, omega=om)
+        assert np.allclose(r1.s, r2.s, rtol=1e-12, atol=0), np.max(np.abs(r1.s - r2.s) / r2.s)
+        assert max_principal_angle(r1.u[:, :10], r2.u[:, :10]) < 1e-9
+
+
+@pytest.mark.parametrize("norm,sv_tol", [("LU", 1e-10), ("NONE", 1e-8)])
+def test_normalizers_match_oracle(lib, norm, sv_tol):
+    # randomized/mod.rs:102-105, 126-127, 239-293 against the oracle's LU
+    # (column L2 normalisation) and None (no normalisation) loops with the
+    # same Omega; None's panel grows as s_i^(2q+1), so its final QR works on
+    # a panel of condition ~1e6 here and the bar is looser
+    a = random_csr(2000, 600, 0.02, seed=11)
+    om = np.random.default_rng(5).normal(size=(600, 15))
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a), 10, 5, 2, norm, False, 1, False, omega=om)
+    u, s, vt = oracle.randomized_svd(oracle.OracleOp(a), 10, 5, 2, norm, omega=om)
+    err = float(np.max(np.abs(rec.s - s) / s))
+    assert err < sv_tol, err
+    assert max_principal_angle(rec.u[:, :5], u[:, :5]) < 1e-6

@@ -471,3 +471,138 @@ def test_coo_to_csr_oracle_rejects_out_of_range():
         oracle.coo_to_csr(3, 3, np.array([0, 3]), np.array([0, 0]), np.array([1.0, 2.0]))
     with pytest.raises(ValueError):
         oracle.coo_to_csr(3, 3, np.array([0, 1]), np.array([0, -1]), np.array([1.0, 2.0]))
+
+
+# ------------------------------------------------------- centred products (f1)
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_centered_products_match_dense(masked):
+    a = random_csr(60, 40, 0.2, seed=9)
+    cols = np.sort(np.random.default_rng(2).choice(40, 25, replace=False)) if masked else np.arange(40)
+    op = oracle.OracleOp(a, columns=cols) if masked else oracle.OracleOp(a)
+    dense = a.toarray()[:, cols]
+    mu = op.column_means()
+    assert np.allclose(mu, dense.mean(axis=0), rtol=1e-13, atol=1e-14)
+    c = dense - mu[None, :]
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(len(cols), 7))
+    y = rng.normal(size=(60, 7))
+    assert np.allclose(op.multiply_with_dense_centered(x, mu), c @ x, rtol=1e-12, atol=1e-12)
+    assert np.allclose(op.multiply_with_dense_centered(y, mu, True), c.T @ y, rtol=1e-12, atol=1e-12)
+    assert np.allclose(op.multiply_transposed_by_dense_centered(y, mu), y.T @ c, rtol=1e-12, atol=1e-12)
+    # the reference's forward form (masked.rs:496-509, defect D7) subtracts
+    # (sum_j mu_j)(sum_i X_ic) from every row; the dense check rejects it
+    d7 = op.multiply_with_dense(x) - mu.sum() * x.sum(axis=0)[None, :]
+    assert not np.allclose(d7, c @ x, rtol=1e-6, atol=1e-6)
+
+
+# ------------------------------------------------------------ svd_flip (f3)
+
+def test_svd_flip_oracle_invariants():
+    rng = np.random.default_rng(6)
+    u = rng.normal(size=(30, 5))
+    v = rng.normal(size=(5, 12))
+    u[3, 2] = -50.0  # a negative dominant entry
+    u[:, 4] = 0.0    # an all-zero column keeps its sign
+    fu, fv, err = oracle.svd_flip(u, v, True)
+    assert err is None
+    for j in range(5):
+        col = fu[:, j]
+        assert col[np.argmax(np.abs(col))] >= 0
+        assert np.array_equal(np.abs(col), np.abs(u[:, j]))
+    assert np.array_equal(fu[:, 2], -u[:, 2]) and np.array_equal(fv[2], -v[2])
+    assert np.array_equal(fu[:, 4], u[:, 4])
+    # flipping twice changes nothing more; the product u v is unchanged
+    gu, gv, _ = oracle.svd_flip(fu, fv, True)
+    assert np.array_equal(gu, fu) and np.array_equal(gv, fv)
+    assert np.allclose(fu @ fv, u @ v, rtol=0, atol=1e-12)
+    # v-based decision
+    hu, hv, _ = oracle.svd_flip(u, v, False)
+    for i in range(5):
+        assert hv[i, np.argmax(np.abs(hv[i]))] >= 0
+    assert np.allclose(hu @ hv, u @ v, rtol=0, atol=1e-12)
+    # ties: the first entry of largest magnitude decides (strict >)
+    t = np.array([[2.0], [-2.0]])
+    assert oracle.svd_flip(t, None, True)[0][0, 0] == 2.0
+    t = np.array([[-2.0], [2.0]])
+    assert oracle.svd_flip(t, None, True)[0][0, 0] == 2.0
+    assert oracle.svd_flip(None, None, True)[2] == "Both u and v cannot be None"
+    assert oracle.svd_flip(None, v, True)[2] == "u cannot be None when u_based_decision is true"
+    assert oracle.svd_flip(u, None, False)[2] == "v cannot be None when u_based_decision is false"
+
+
+@pytest.mark.parametrize("decide_u", [True, False])
+def test_host_svd_flip_matches_oracle(decide_u):
+    from single_svdlib.error import Las2Error
+    from single_svdlib.randomized import svd_flip
+    rng = np.random.default_rng(8)
+    for k_u, k_v in ((6, 6), (6, 4), (4, 6)):
+        u = rng.normal(size=(40, k_u))
+        v = rng.normal(size=(k_v, 25))
+        u[5, 1] = -9.0
+        v[2, 3] = -9.0
+        ru, rv, err = oracle.svd_flip(u, v, decide_u)
+        hu, hv = u.copy(), v.copy()
+        svd_flip(hu, hv, decide_u)
+        assert err is None
+        assert np.array_equal(hu, ru) and np.array_equal(hv, rv)
+    with pytest.raises(Las2Error, match="Both u and v cannot be None"):
+        svd_flip(None, None)
+    with pytest.raises(Las2Error, match="u cannot be None"):
+        svd_flip(None, np.ones((2, 2)), True)
+    with pytest.raises(Las2Error, match="v cannot be None"):
+        svd_flip(np.ones((2, 2)), None, False)
+
+
+# ------------------------------------------------- Gaussian test matrix (a19)
+
+def test_gaussian_matrix_distribution_and_layout():
+    from scipy import stats
+    g = oracle.gaussian_matrix(20000, 6, 42)
+    assert g.shape == (20000, 6)
+    x = g.ravel()
+    assert abs(x.mean()) < 4 / np.sqrt(x.size)
+    assert abs(x.var() - 1.0) < 4 * np.sqrt(2.0 / x.size)
+    assert stats.kstest(x, "norm").pvalue > 1e-3
+    # columns are independent streams: no correlation above the noise level
+    c = np.corrcoef(g.T)
+    assert np.max(np.abs(c - np.eye(6))) < 5 / np.sqrt(20000)
+    # column-major counter order: the first rows x 1 block is the
+    # rows x 1 matrix of the same seed, and another seed differs
+    assert np.array_equal(oracle.gaussian_matrix(20000, 1, 42)[:, 0], g[:, 0])
+    assert not np.allclose(oracle.gaussian_matrix(20000, 1, 43)[:, 0], g[:, 0])
+
+
+# ------------------------------------------------ sparse adjoint path (a26)
+
+@pytest.mark.parametrize("loss", ["sigma_sum", "projection"])
+@pytest.mark.parametrize("extra", [0, 4])
+def test_adjoint_sparse_path_matches_dense(loss, extra):
+    a, y = _adjoint_case(seed=11)
+    k = 5
+    gd = oracle.adjoint.gradient_on_pattern(a, loss, k, y)
+    tr = oracle.adjoint.top_triplets(a, k + extra)
+    gs, info = oracle.adjoint.gradient_sparse(a, loss, k, y, triplets=tr, tol=1e-13)
+    assert np.max(np.abs(gs - gd)) < 1e-10 * np.max(np.abs(gd))
+    if loss == "projection":
+        assert info["cg_relative_residual"] <= 1e-13
+
+
+def test_adjoint_sparse_path_planted_mistake_fails():
+    # dropping the explicit part of the complement (the u_k .. u_{kk-1}
+    # directions) must show against the dense gradient
+    a, y = _adjoint_case(seed=11)
+    k = 5
+    gd = oracle.adjoint.gradient_on_pattern(a, "projection", k, y)
+    u, s, vt = oracle.adjoint.top_triplets(a, k + 4)
+    w, _, _ = oracle.adjoint.complement_solutions(a, y, u, s, k, tol=1e-13)
+    w_bad, _, _ = oracle.adjoint.complement_solutions(a, y, u[:, :k], s[:k], k, tol=1e-13)
+    assert np.max(np.abs(w - w_bad)) < 1e-9 * np.max(np.abs(w))  # both solve the same systems
+    # but a complement projected onto range(I - U_kk U_kk^T) only misses terms
+    w_miss = w - u[:, k:] @ (u[:, k:].T @ w)
+    rows = np.repeat(np.arange(a.shape[0]), np.diff(a.indptr))
+    c = u[:, :k].T @ y
+    atw = a.T @ w_miss
+    g_bad = (np.einsum("ei,ei->e", (w_miss * (2 * c * s[:k]))[rows], vt[:k, a.indices].T)
+             + np.einsum("ei,ei->e", (u[:, :k] * 2 * c)[rows], atw[a.indices]))
+    assert np.max(np.abs(g_bad - gd)) > 1e-3 * np.max(np.abs(gd))
