@@ -171,6 +171,23 @@ int32_t svdh_op_shape(const svdh_op *op, int64_t *nrows, int64_t *ncols, int64_t
 int32_t svdh_spmv(svdh_ctx *ctx, const svdh_op *op, int32_t transposed, const void *x, void *y,
                   int32_t vec_dtype, int32_t on_device);
 
+/* svdh_spmv with device vectors (on_device = 1) that allocates nothing. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The workgroup partials go to a static device
+ * scratch array of 16 MiB (2^21 FP64 partials) that each GPU the library runs
+ * on holds until the process ends; a product that needs more partials (the
+ * source-tiled layout needs source blocks x output rows) returns
+ * SVDH_ERR_ARG. The operator must have been prepared for this direction and
+ * vector dtype by one earlier svdh_spmv call (which builds its schedule and
+ * value layouts); otherwise SVDH_ERR_ARG. The product is enqueued on the
+ * context's stream and the call returns without waiting
+ * (svdh_ctx_synchronize waits). Every call on a GPU shares that GPU's scratch
+ * (each call writes every partial it reads), so calls that overlap on one
+ * GPU, from contexts on different streams, race. */
+int32_t svdh_spmv_noalloc(svdh_ctx *ctx, const svdh_op *op, int32_t transposed, const void *x, void *y,
+                          int32_t vec_dtype);
+
 /* SMat::multiply_with_dense (src/utils.rs:28, masked.rs:265-471).
  * transpose_self == 0: Y (nrows x l) = A X (ncols x l);
  * transpose_self != 0: Y (ncols x l) = A^T X (nrows x l).

@@ -83,7 +83,7 @@ void Ctx::drain_profile() {
 }
 
 KTimer::KTimer(Ctx *ctx, int k, double algorithmic_bytes) : c(ctx), cls(k), bytes(algorithmic_bytes) {
-  if (!c->profile) return;
+  if (!c || !c->profile) return;
   a = c->get_event();
   HIP_CHECK(hipEventRecord(a, c->stream));
 }
@@ -198,6 +198,7 @@ int32_t svdh_ctx_create(int32_t device, svdh_ctx **out) {
         HIP_CHECK(hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming));
         HIP_CHECK(hipEventRecord(c->ring_ev[i], c->stream));
       }
+      scratch_init(c);  // the device is current here
       HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->h_small), Ctx::kSmall * sizeof(double),
                               hipHostMallocCoherent | hipHostMallocMapped));
       HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_small), c->h_small, 0));
@@ -351,6 +352,22 @@ int32_t svdh_spmv(svdh_ctx *ctx, const svdh_op *op_, int32_t transposed, const v
     spmv(op, dir, vdt, dx.p, dy.p, vec_dtype);
     HIP_CHECK(hipMemcpyAsync(y, dy.p, nout * es, hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
+  });
+}
+
+int32_t svdh_spmv_noalloc(svdh_ctx *ctx, const svdh_op *op_, int32_t transposed, const void *x, void *y,
+                          int32_t vec_dtype) {
+  return guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op_);
+    svdh_op *op = const_cast<svdh_op *>(op_);
+    if (vec_dtype != SVDH_F64 && vec_dtype != SVDH_F32) fail(SVDH_ERR_ARG, "vector dtype must be F64 or F32");
+    if (!x || !y) fail(SVDH_ERR_ARG, "null vector");
+    // the value dtype svdh_spmv picks for these vectors
+    const svdh_op *own = op->base ? op->base : op;
+    const int vdt = own->base_dtype;
+    if (vdt == SVDH_BF16) fail(SVDH_ERR_ARG, "SpMV on BF16 values is not provided");
+    spmv_noalloc(op, transposed ? 1 : 0, vdt, x, y, vec_dtype);
   });
 }
 

@@ -111,6 +111,8 @@ struct Ctx {
   double prof_ms[KC_COUNT] = {0};
   int64_t prof_n[KC_COUNT] = {0};
   double prof_bytes[KC_COUNT] = {0};
+  // Device address of this GPU's copy of the allocation-free SpMV scratch.
+  double *scratch = nullptr;
 
   void sync();
   hipEvent_t get_event();
@@ -118,7 +120,7 @@ struct Ctx {
 };
 
 // RAII helper: records events around the kernels launched in its scope when
-// profiling is on.
+// profiling is on (a null context records nothing).
 struct KTimer {
   Ctx *c;
   int cls;
@@ -262,6 +264,13 @@ namespace svdh {
 // y = op_dir x for vectors; dtypes: values vdt, vectors xdt (F64 or F32).
 // Accumulation is FP64 when the vectors are FP64, FP32 otherwise.
 void spmv(svdh_op *op, int direction, int vdt, const void *x, void *y, int xdt);
+// spmv with the workgroup partials in the static device scratch: allocates
+// nothing; fails unless the operator's schedule and value layouts for this
+// direction and value dtype exist already (an earlier spmv builds them).
+void spmv_noalloc(svdh_op *op, int direction, int vdt, const void *x, void *y, int xdt);
+// Looks up the device address of the scratch for c's device (the device
+// must be current); once per device, at context creation.
+void scratch_init(Ctx *c);
 // Y = op_dir X for row-major panels of width l; X has leading dimension ldx
 // (elements) and dtype xdt (F64, F32 or BF16); Y has ldy and dtype ydt (F64 or
 // F32). Accumulation is FP64 for FP64 panels, FP32 otherwise.

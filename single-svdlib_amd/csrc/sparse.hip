@@ -19,6 +19,8 @@
 #include <numeric>
 #include <type_traits>
 
+#include <mutex>
+
 #include "common.h"
 
 namespace svdh {
@@ -1795,39 +1797,48 @@ static Schedule &get_spmm_schedule(svdh_op *op, int direction) {
   return s;
 }
 
+// scr (allocation-free calls, spmv_noalloc): the workgroup partials go to
+// this device buffer instead of the schedule's own, the schedule and value
+// layouts must exist already (checked by the caller), and no timer events
+// are taken.
 template <typename V, typename X, typename Y>
-static void launch_spmv(svdh_op *op, int direction, const void *vals, const void *x, void *y) {
+static void launch_spmv(svdh_op *op, int direction, const void *vals, const void *x, void *y,
+                        double *scr = nullptr) {
   Ctx *c = op->ctx;
   CsrView v = op->view(direction);
   Schedule &s = get_schedule(op, direction, 0);
   if (v.nrows == 0) return;
-  s.partials.ensure(std::max<int32_t>(s.nchunks, 1) * sizeof(double));
+  if (!scr) s.partials.ensure(std::max<int32_t>(s.nchunks, 1) * sizeof(double));
+  double *partials = scr ? scr : s.partials.as<double>();
+  Ctx *tc = scr ? nullptr : c;  // timer context
   const double bytes = (double)v.nnz * (sizeof(V) + 4) + (double)(v.nrows + 1) * 4 +
                        (double)v.ncols * sizeof(X) + (double)v.nrows * sizeof(Y);
   if (s.tiled.on) {
     Schedule::Tiled &T = s.tiled;
+    double *part = scr ? scr : T.part.as<double>();
     TiledArgs<V, X, Y> a;
     a.rowptr = T.rowptr.as<int32_t>();
     a.col = T.col.as<uint16_t>();
-    a.vals = tiled_values<V>(c, T, static_cast<const V *>(vals), v.nnz);
+    a.vals = scr ? static_cast<const V *>(T.vals[DtypeOf<V>::v].p)
+                 : tiled_values<V>(c, T, static_cast<const V *>(vals), v.nnz);
     a.tasks = T.tasks.as<int4>();
     a.ranges = T.ranges.as<int4>();
     a.x = static_cast<const X *>(x);
-    a.part = T.part.as<double>();
+    a.part = part;
     a.y = static_cast<Y *>(y);
     a.nsrc = v.ncols;
     a.sb = T.sb;
     a.direct = T.nb == 1;
-    KTimer t(c, KC_SPMV, bytes);
+    KTimer t(tc, KC_SPMV, bytes);
     hipLaunchKernelGGL((k_spmv_tiled<V, X, Y>), T.nranges, kTiledBlock, 0, c->stream, a);
     HIP_CHECK(hipGetLastError());
     if (T.nb > 1) {
       if (T.nb >= 8)
         hipLaunchKernelGGL((k_spmv_tiled_fin<4, Y>), (int)((4 * v.nrows + kBlock - 1) / kBlock), kBlock, 0,
-                           c->stream, v.nrows, (int)T.nb, (const double *)T.part.p, static_cast<Y *>(y));
+                           c->stream, v.nrows, (int)T.nb, (const double *)part, static_cast<Y *>(y));
       else
         hipLaunchKernelGGL((k_spmv_tiled_fin<1, Y>), (int)((v.nrows + kBlock - 1) / kBlock), kBlock, 0,
-                           c->stream, v.nrows, (int)T.nb, (const double *)T.part.p, static_cast<Y *>(y));
+                           c->stream, v.nrows, (int)T.nb, (const double *)part, static_cast<Y *>(y));
       HIP_CHECK(hipGetLastError());
     }
     return;
@@ -1839,12 +1850,12 @@ static void launch_spmv(svdh_op *op, int direction, const void *vals, const void
     a.vals = static_cast<const V *>(vals);
     a.colmap = v.colmap;
     a.tasks = s.tasks.as<int4>();
-    a.partials = s.partials.as<double>();
+    a.partials = partials;
     a.x = static_cast<const X *>(x);
     a.y = static_cast<Y *>(y);
     a.ntasks = s.ntasks;
     const int blocks = (s.ntasks + kBlock / 64 - 1) / (kBlock / 64);
-    KTimer t(c, KC_SPMV, bytes);
+    KTimer t(tc, KC_SPMV, bytes);
     if (v.colmap)
       hipLaunchKernelGGL((k_spmv_stream<true, V, X, Y>), blocks, kBlock, 0, c->stream, a);
     else
@@ -1852,7 +1863,7 @@ static void launch_spmv(svdh_op *op, int direction, const void *vals, const void
     HIP_CHECK(hipGetLastError());
     if (s.nslots > 0) {
       hipLaunchKernelGGL((k_spmv_fin<Y>), (s.nslots + kBlock - 1) / kBlock, kBlock, 0, c->stream, s.nslots,
-                         s.counters.as<int32_t>(), (const double *)s.partials.p, static_cast<Y *>(y));
+                         s.counters.as<int32_t>(), (const double *)partials, static_cast<Y *>(y));
       HIP_CHECK(hipGetLastError());
     }
     return;
@@ -1865,14 +1876,14 @@ static void launch_spmv(svdh_op *op, int direction, const void *vals, const void
   a.colmap = v.colmap;
   a.rows = s.rows.as<int32_t>();
   a.chunks = s.chunks.as<int32_t>();
-  a.partials = s.partials.as<double>();
+  a.partials = partials;
   a.x = static_cast<const X *>(x);
   a.y = static_cast<Y *>(y);
   a.nbins = s.nbins;
   a.long_block_begin = s.long_block_begin;
   for (int i = 0; i < kMaxBins; ++i) a.bins[i] = i < s.nbins ? s.bins[i] : BinDesc{INT32_MAX, 0, 0, 0};
   if (s.nbins == 0) a.bins[0] = BinDesc{INT32_MAX, 0, 0, 0};
-  KTimer t(c, KC_SPMV, bytes);
+  KTimer t(tc, KC_SPMV, bytes);
   if (v.colmap)
     hipLaunchKernelGGL((k_spmv<true, V, X, Y>), s.total_blocks, kBlock, 0, c->stream, a);
   else
@@ -1880,9 +1891,65 @@ static void launch_spmv(svdh_op *op, int direction, const void *vals, const void
   HIP_CHECK(hipGetLastError());
   if (s.nslots > 0) {
     hipLaunchKernelGGL((k_spmv_fin<Y>), (s.nslots + kBlock - 1) / kBlock, kBlock, 0, c->stream, s.nslots,
-                       s.counters.as<int32_t>(), (const double *)s.partials.p, static_cast<Y *>(y));
+                       s.counters.as<int32_t>(), (const double *)partials, static_cast<Y *>(y));
     HIP_CHECK(hipGetLastError());
   }
+}
+
+// ------------------------------------------------------------------------
+// Allocation-free SpMV (svdh_spmv_noalloc). Its workgroup partials live in a
+// file-scope device array of fixed size; each GPU holds its own copy, whose
+// device address is looked up once per device (scratch_init, at context
+// creation with the device current) and passed to the kernels. Every partial
+// a launch reads is written earlier in the same launch sequence (each
+// stacked row / chunk task writes its slot), so nothing needs clearing
+// between calls; calls that overlap on one GPU race on the array.
+// ------------------------------------------------------------------------
+constexpr size_t kScratchDoubles = (size_t)2 << 20;  // 16 MiB of FP64 partials
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+void scratch_init(Ctx *c) {
+  constexpr int kMaxDev = 64;
+  static std::mutex mu;
+  static void *addr[kMaxDev] = {};
+  if (c->device < 0 || c->device >= kMaxDev) fail(SVDH_ERR_ARG, "device %d out of range", c->device);
+  std::lock_guard<std::mutex> g(mu);
+  if (!addr[c->device]) {
+    void *p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(SVDH_ERR_HIP, "no device address for the SpMV scratch");
+    addr[c->device] = p;
+  }
+  c->scratch = static_cast<double *>(addr[c->device]);
+}
+
+void spmv_noalloc(svdh_op *op, int direction, int vdt, const void *x, void *y, int xdt) {
+  Ctx *c = op->ctx;
+  if (!c->scratch) fail(SVDH_ERR_HIP, "the SpMV scratch is not set up");
+  const CsrView v = op->view(direction);
+  if (v.nrows == 0) return;
+  const Schedule &s = op->sched[direction][0];
+  const void *vals = op->values(direction, vdt);
+  const char *how = "call svdh_spmv once with the same operator, direction and vector dtype first";
+  if (!s.built || !vals) fail(SVDH_ERR_ARG, "operator not prepared for this product: %s", how);
+  size_t need = (size_t)std::max<int32_t>(s.nchunks, s.nslots);
+  if (s.tiled.on) {
+    if (!s.tiled.vals[vdt].p) fail(SVDH_ERR_ARG, "operator not prepared for this product: %s", how);
+    need = s.tiled.nb > 1 ? (size_t)s.tiled.nb * (size_t)v.nrows : 0;
+  }
+  if (need > kScratchDoubles)
+    fail(SVDH_ERR_ARG, "this product needs %zu FP64 partials; the allocation-free SpMV holds at most %zu (%zu MiB)",
+         need, kScratchDoubles, kScratchDoubles * sizeof(double) >> 20);
+  if (vdt == SVDH_F64 && xdt == SVDH_F64)
+    launch_spmv<double, double, double>(op, direction, vals, x, y, c->scratch);
+  else if (vdt == SVDH_F32 && xdt == SVDH_F64)
+    launch_spmv<float, double, double>(op, direction, vals, x, y, c->scratch);
+  else if (vdt == SVDH_F32 && xdt == SVDH_F32)
+    launch_spmv<float, float, float>(op, direction, vals, x, y, c->scratch);
+  else if (vdt == SVDH_F64 && xdt == SVDH_F32)
+    launch_spmv<double, float, float>(op, direction, vals, x, y, c->scratch);
+  else
+    fail(SVDH_ERR_ARG, "unsupported SpMV dtypes values=%d vectors=%d", vdt, xdt);
 }
 
 void spmv(svdh_op *op, int direction, int vdt, const void *x, void *y, int xdt) {

@@ -87,6 +87,7 @@ SIGNATURES = [
     ("svdh_op_destroy", _i32, [_vp]),
     ("svdh_op_shape", _i32, [_vp, _i64p, _i64p, _i64p]),
     ("svdh_spmv", _i32, [_vp, _vp, _i32, _vp, _vp, _i32, _i32]),
+    ("svdh_spmv_noalloc", _i32, [_vp, _vp, _i32, _vp, _vp, _i32]),
     ("svdh_spmm", _i32, [_vp, _vp, _i32, _vp, _i64, _i64, _vp, _i64, _i32]),
     ("svdh_spmm_qta", _i32, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _i32]),
     ("svdh_spmm_centered", _i32, [_vp, _vp, _i32, _vp, _i64, _i64, _vp, _i64, _vp, _i32]),
