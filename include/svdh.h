@@ -336,8 +336,10 @@ typedef struct {
  * layout is accepted); k <= min(res->d, 64). The projection loss solves k
  * shifted systems (s_i^2 I - A A^T) w_i = (I - U_k U_k^T) y by conjugate
  * gradients to the relative residual tol (<= 0: 1e-10 in FP64, 1e-5 in FP32)
- * within max_iters (<= 0: 1000) iterations. precision: SVDH_PREC_F64 or
- * SVDH_PREC_F32 (factor panels and SpMM arithmetic). */
+ * within max_iters (<= 0: 1000) iterations. precision: SVDH_PREC_F64,
+ * SVDH_PREC_F32 (factor panels and SpMM arithmetic) or SVDH_PREC_BF16 (FP32
+ * panels and conjugate gradients; the SDDMM gathers BF16 copies of its factor
+ * panels and accumulates in FP32). */
 int32_t svdh_backward(svdh_ctx *ctx, const svdh_op *op, const svdh_result *res, int32_t loss, int64_t k,
                       const double *y, int32_t precision, double tol, int64_t max_iters, void *grad,
                       int32_t grad_dtype, svdh_backward_info *info);

@@ -206,6 +206,87 @@ void orc_multiply_transposed_by_dense(const orc_op *op, const double *q, int64_t
   free(partial);
 }
 
+/* The same two products with T = f32 (the reference instantiated with f32:
+ * products and accumulators in f32, masked.rs:300-360, 361-470, 616-733).
+ * vals32 holds the operator's values rounded to f32 (CSR order). */
+void orc_multiply_with_dense_f32(const orc_op *op, const float *vals32, const float *dense, int64_t l,
+                                 float *result, int transpose_self) {
+  const int nt = nthreads_of(op);
+  const int64_t nr = op->nrows, nc = orc_ncols(op);
+  if (!transpose_self) {
+#pragma omp parallel num_threads(nt)
+    {
+      float *row = (float *)malloc((size_t)l * sizeof(float));
+#pragma omp for schedule(static)
+      for (int64_t i = 0; i < nr; ++i) {
+        for (int64_t c = 0; c < l; ++c) row[c] = 0.0f;
+        for (int64_t j = op->rowptr[i]; j < op->rowptr[i + 1]; ++j) {
+          int64_t mc = mcol(op, j);
+          if (mc < 0) continue;
+          float v = vals32[j];
+          for (int64_t c = 0; c < l; ++c) row[c] += v * dense[mc + c * nc];
+        }
+        for (int64_t c = 0; c < l; ++c) result[i + c * nr] = row[c];
+      }
+      free(row);
+    }
+  } else {
+    const int64_t chunk = orc_determine_chunk_size(nr, nt);
+    const int64_t nchunks = (nr + chunk - 1) / chunk;
+    float *partial = (float *)calloc((size_t)(nchunks * nc * l), sizeof(float));
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+    for (int64_t ch = 0; ch < nchunks; ++ch) {
+      float *loc = partial + ch * nc * l;
+      int64_t end = (ch + 1) * chunk < nr ? (ch + 1) * chunk : nr;
+      for (int64_t i = ch * chunk; i < end; ++i)
+        for (int64_t j = op->rowptr[i]; j < op->rowptr[i + 1]; ++j) {
+          int64_t mc = mcol(op, j);
+          if (mc < 0) continue;
+          float v = vals32[j];
+          for (int64_t c = 0; c < l; ++c) loc[mc * l + c] += v * dense[i + c * nr];
+        }
+    }
+    for (int64_t k = 0; k < nc * l; ++k) result[k] = 0.0f;
+    for (int64_t ch = 0; ch < nchunks; ++ch) {
+      const float *loc = partial + ch * nc * l;
+      for (int64_t r = 0; r < nc; ++r)
+        for (int64_t c = 0; c < l; ++c) {
+          float v = loc[r * l + c];
+          if (v != 0.0f) result[r + c * nc] += v;
+        }
+    }
+    free(partial);
+  }
+}
+
+void orc_multiply_transposed_by_dense_f32(const orc_op *op, const float *vals32, const float *q, int64_t l,
+                                          float *result) {
+  const int nt = nthreads_of(op);
+  const int64_t nr = op->nrows, nc = orc_ncols(op);
+  const int64_t chunk = orc_determine_chunk_size(nr, nt);
+  const int64_t nchunks = (nr + chunk - 1) / chunk;
+  float *partial = (float *)calloc((size_t)(nchunks * nc * l), sizeof(float));
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1)
+  for (int64_t ch = 0; ch < nchunks; ++ch) {
+    float *loc = partial + ch * nc * l;
+    int64_t end = (ch + 1) * chunk < nr ? (ch + 1) * chunk : nr;
+    for (int64_t i = ch * chunk; i < end; ++i)
+      for (int64_t j = op->rowptr[i]; j < op->rowptr[i + 1]; ++j) {
+        int64_t mc = mcol(op, j);
+        if (mc < 0) continue;
+        float v = vals32[j];
+        for (int64_t c = 0; c < l; ++c) loc[c + mc * l] += q[i + c * nr] * v;
+      }
+  }
+  for (int64_t k = 0; k < nc * l; ++k) result[k] = 0.0f;
+  for (int64_t ch = 0; ch < nchunks; ++ch) {
+    const float *loc = partial + ch * nc * l;
+    for (int64_t k = 0; k < nc * l; ++k)
+      if (loc[k] != 0.0f) result[k] += loc[k];
+  }
+  free(partial);
+}
+
 /* masked.rs:240-263, mod.rs:1605-1627 */
 void orc_compute_column_means(const orc_op *op, double *means) {
   const int64_t nr = op->nrows, nc = orc_ncols(op);

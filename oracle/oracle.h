@@ -97,6 +97,11 @@ void orc_multiply_with_dense(const orc_op *op, const double *dense, int64_t l,
 /* result (l x ncols, column-major) = q^T A, q is nrows x l column-major. */
 void orc_multiply_transposed_by_dense(const orc_op *op, const double *q, int64_t l,
                                       double *result);
+/* f32 twins (T = f32): vals32 = the values rounded to f32, panels f32. */
+void orc_multiply_with_dense_f32(const orc_op *op, const float *vals32, const float *dense, int64_t l,
+                                 float *result, int transpose_self);
+void orc_multiply_transposed_by_dense_f32(const orc_op *op, const float *vals32, const float *q, int64_t l,
+                                          float *result);
 void orc_compute_column_means(const orc_op *op, double *means);
 
 /* Scalar helpers exposed for the known-answer tests. */

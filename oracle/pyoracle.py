@@ -66,6 +66,9 @@ class _Rec(C.Structure):
 _lib.orc_opa.argtypes = [C.POINTER(_Op), _f64p, _f64p, C.c_int]
 _lib.orc_multiply_with_dense.argtypes = [C.POINTER(_Op), _f64p, C.c_int64, _f64p, C.c_int]
 _lib.orc_multiply_transposed_by_dense.argtypes = [C.POINTER(_Op), _f64p, C.c_int64, _f64p]
+_f32p = C.POINTER(C.c_float)
+_lib.orc_multiply_with_dense_f32.argtypes = [C.POINTER(_Op), _f32p, _f32p, C.c_int64, _f32p, C.c_int]
+_lib.orc_multiply_transposed_by_dense_f32.argtypes = [C.POINTER(_Op), _f32p, _f32p, C.c_int64, _f32p]
 _lib.orc_compute_column_means.argtypes = [C.POINTER(_Op), _f64p]
 _lib.orc_nnz.argtypes = [C.POINTER(_Op)]
 _lib.orc_nnz.restype = C.c_int64
@@ -162,6 +165,31 @@ class OracleOp:
         l = q.shape[1]
         out = np.zeros((l, self.ncols), dtype=np.float64, order="F")
         _lib.orc_multiply_transposed_by_dense(C.byref(self._st), _p(q), l, _p(out))
+        return out
+
+    def _vals32(self):
+        if getattr(self, "_v32", None) is None:
+            self._v32 = np.ascontiguousarray(self.vals, dtype=np.float32)
+        return self._v32
+
+    def multiply_with_dense_f32(self, dense, transpose_self=False):
+        """multiply_with_dense with T = f32: values rounded to f32, f32
+        products and accumulators (the reference instantiated with f32)."""
+        d = np.asfortranarray(dense, dtype=np.float32)
+        l = d.shape[1]
+        rows = self.ncols if transpose_self else self.nrows
+        out = np.zeros((rows, l), dtype=np.float32, order="F")
+        v = self._vals32()
+        _lib.orc_multiply_with_dense_f32(C.byref(self._st), _p(v, _f32p), _p(d, _f32p), l, _p(out, _f32p),
+                                         int(bool(transpose_self)))
+        return out
+
+    def multiply_transposed_by_dense_f32(self, q):
+        q = np.asfortranarray(q, dtype=np.float32)
+        l = q.shape[1]
+        out = np.zeros((l, self.ncols), dtype=np.float32, order="F")
+        v = self._vals32()
+        _lib.orc_multiply_transposed_by_dense_f32(C.byref(self._st), _p(v, _f32p), _p(q, _f32p), l, _p(out, _f32p))
         return out
 
     def multiply_with_dense_centered(self, dense, means, transpose_self=False):

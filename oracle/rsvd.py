@@ -20,8 +20,11 @@ instead of sum_j mu_j X_jc (defect D7), corrected here.
 import numpy as np
 
 
-def _mult(op, dense, transpose, means):
-    out = op.multiply_with_dense(dense, transpose_self=transpose)
+def _mult(op, dense, transpose, means, f32=False):
+    if f32:
+        out = op.multiply_with_dense_f32(dense, transpose_self=transpose)
+    else:
+        out = op.multiply_with_dense(dense, transpose_self=transpose)
     if means is not None:
         if not transpose:
             out = out - np.ones((out.shape[0], 1)) @ (means[None, :] @ dense)
@@ -45,30 +48,36 @@ def _normalize_columns(a):
 
 
 def randomized_svd(op, target_rank, n_oversamples, n_power_iters, normalizer="QR", mean_center=False,
-                   omega=None):
-    """Returns (u m x d, s d, vt d x n). omega (n x l) must be supplied."""
+                   omega=None, dtype=np.float64):
+    """Returns (u m x d, s d, vt d x n). omega (n x l) must be supplied.
+    dtype np.float32 runs the reference's T = f32 instantiation: f32 sparse
+    products (oracle C, f32 accumulators), f32 QR and SVD (LAPACK single
+    precision for nalgebra's), no mean centring."""
     m, n = op.nrows, op.ncols
     rank = min(target_rank, m, n)
     l = rank + n_oversamples
+    f32 = np.dtype(dtype) == np.float32
+    if f32 and mean_center:
+        raise ValueError("the f32 oracle path has no mean centring")
     means = op.column_means() if mean_center else None
     if omega is None:
         raise ValueError("the oracle needs the test matrix Omega (n x l)")
-    omega = np.asarray(omega, dtype=np.float64)
+    omega = np.asarray(omega, dtype=dtype)
     assert omega.shape == (n, l)
-    y = _mult(op, omega, False, means)
+    y = _mult(op, omega, False, means, f32)
     for _ in range(n_power_iters):
-        z = _mult(op, y, True, means)
+        z = _mult(op, y, True, means, f32)
         if normalizer == "QR":
             z = _qr_q(z)
         elif normalizer == "LU":
             z = _normalize_columns(z)
-        y = _mult(op, z, False, means)
+        y = _mult(op, z, False, means, f32)
         if normalizer == "QR":
             y = _qr_q(y)
         elif normalizer == "LU":
             y = _normalize_columns(y)
     q = _qr_q(y)
-    b = op.multiply_transposed_by_dense(q)
+    b = op.multiply_transposed_by_dense_f32(q) if f32 else op.multiply_transposed_by_dense(q)
     if means is not None:
         b = b - q.sum(axis=0)[:, None] @ means[None, :]
     u_b, s, vt = np.linalg.svd(b, full_matrices=False)

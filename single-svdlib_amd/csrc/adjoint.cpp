@@ -18,7 +18,8 @@
 //
 // Factors live in HBM as row-major panels (leading dimension k rounded up to
 // 8); the CG scalars are per column and travel to the host once per
-// half-step.
+// half-step. BF16 precision stores the SDDMM's factor panels in BF16
+// (FP32 accumulation); everything before it runs in FP32.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -63,7 +64,8 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
   const int64_t m = op->nrows, n = op->ncols, d = res->d;
   if (loss != SVDH_LOSS_SIGMA_SUM && loss != SVDH_LOSS_PROJECTION) fail(SVDH_ERR_ARG, "bad loss kind %d", loss);
   if (k < 1 || k > d || k > 64) fail(SVDH_ERR_ARG, "k = %lld outside 1..min(d = %lld, 64)", (long long)k, (long long)d);
-  if (precision != SVDH_PREC_F64 && precision != SVDH_PREC_F32) fail(SVDH_ERR_ARG, "bad precision %d", precision);
+  if (precision != SVDH_PREC_F64 && precision != SVDH_PREC_F32 && precision != SVDH_PREC_BF16)
+    fail(SVDH_ERR_ARG, "bad precision %d", precision);
   if (grad_dtype != SVDH_F64 && grad_dtype != SVDH_F32) fail(SVDH_ERR_ARG, "bad gradient dtype %d", grad_dtype);
   if (loss == SVDH_LOSS_PROJECTION && !y) fail(SVDH_ERR_ARG, "the projection loss needs y");
   if (!res->u || !res->s || !res->vt) fail(SVDH_ERR_ARG, "result without factors");
@@ -80,8 +82,22 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
     fail(SVDH_ERR_SHAPE, "vt is %lld x %lld, expected %lld x %lld", (long long)res->vt_rows, (long long)res->vt_cols,
          (long long)d, (long long)n);
   Ctx *c = ctx;
+  // panels and SpMM arithmetic: FP64, or FP32 for both FP32 and BF16 (the
+  // CG needs an exact operator: BF16 inputs would stall it at their rounding)
   const int dt = precision == SVDH_PREC_F64 ? SVDH_F64 : SVDH_F32;
+  // BF16: the SDDMM gathers BF16 copies of its factor panels (half the
+  // gathered bytes) and accumulates in FP32
+  const bool bf = precision == SVDH_PREC_BF16;
   const int64_t ld = (k + 7) / 8 * 8;
+  std::vector<DevBuf> bf_panels;
+  auto sddmm_factor = [&](const void *P, int64_t rows) -> const void * {
+    if (!bf) return P;
+    bf_panels.emplace_back(std::max<size_t>((size_t)rows * ld * 2, 64));
+    void *q = bf_panels.back().p;
+    panel_convert(c, rows, ld, P, ld, SVDH_F32, q, ld, SVDH_BF16);
+    return q;
+  };
+  const int sdt = bf ? SVDH_BF16 : dt;
   auto uget = [&](int64_t i, int64_t j) {
     return u_rows_are_vectors ? res->u[j * m + i] : res->u[i * d + j];
   };
@@ -96,7 +112,8 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
     double L = 0.0;
     for (int64_t i = 0; i < k; ++i) L += res->s[i];
     info->loss = L;
-    sddmm(op, k, U.p, V.p, nullptr, nullptr, nullptr, nullptr, ld, ld, dt, G.p, grad_dtype);
+    sddmm(op, k, sddmm_factor(U.p, m), sddmm_factor(V.p, n), nullptr, nullptr, nullptr, nullptr, ld, ld, sdt, G.p,
+          grad_dtype);
   } else {
     // c = U^T y, b = P y on the host in FP64 (O(m k))
     std::vector<double> cy(k, 0.0), b(y, y + m);
@@ -188,7 +205,8 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
     }
     const double *s1_d = static_cast<const double *>(c->upload(s1.data(), 64 * sizeof(double)));
     const double *s2_d = static_cast<const double *>(c->upload(s2.data(), 64 * sizeof(double)));
-    sddmm(op, k, W.p, V.p, s1_d, U.p, Z.p, s2_d, ld, ld, dt, G.p, grad_dtype);
+    sddmm(op, k, sddmm_factor(W.p, m), sddmm_factor(V.p, n), s1_d, sddmm_factor(U.p, m), sddmm_factor(Z.p, n),
+          s2_d, ld, ld, sdt, G.p, grad_dtype);
   }
   HIP_CHECK(hipMemcpyAsync(grad, G.p, (size_t)nnz * dtype_size(grad_dtype), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));

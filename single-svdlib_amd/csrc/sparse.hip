@@ -2384,8 +2384,14 @@ struct SddmmArgs {
   G *grad;
 };
 
+// T is the storage type of the factor panels (FP64, FP32 or BF16); the
+// products are accumulated in FP64 for FP64 factors, FP32 otherwise.
+template <typename T> struct SddmmAcc { using type = float; };
+template <> struct SddmmAcc<double> { using type = double; };
+
 template <int LGR, bool MC, typename T, typename G>
 __global__ __launch_bounds__(kBlock) void k_sddmm(SddmmArgs<T, G> a) {
+  using A = typename SddmmAcc<T>::type;
   constexpr int R = 1 << LGR;
   constexpr int NV = PanelVec<T>::N;
   constexpr int RPB = kBlock >> LGR;
@@ -2394,20 +2400,20 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(SddmmArgs<T, G> a) {
   const bool colok = col < a.k;
   const bool two = a.l2 != nullptr;
   for (int64_t r = (int64_t)blockIdx.x * RPB + (threadIdx.x >> LGR); r < a.nrows; r += (int64_t)gridDim.x * RPB) {
-    T lv1[NV], lv2[NV];
+    A lv1[NV], lv2[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      lv1[v] = T(0);
-      lv2[v] = T(0);
+      lv1[v] = A(0);
+      lv2[v] = A(0);
     }
     if (colok) {
-      ld_panel<T, T>(a.l1 + r * a.ldl + col, lv1);
-      if (two) ld_panel<T, T>(a.l2 + r * a.ldl + col, lv2);
+      ld_panel<A, T>(a.l1 + r * a.ldl + col, lv1);
+      if (two) ld_panel<A, T>(a.l2 + r * a.ldl + col, lv2);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const bool in = col + v < a.k;
-        lv1[v] = in ? (T)(a.s1 ? a.s1[col + v] * (double)lv1[v] : (double)lv1[v]) : T(0);
-        lv2[v] = in && two ? (T)(a.s2 ? a.s2[col + v] * (double)lv2[v] : (double)lv2[v]) : T(0);
+        lv1[v] = in ? (A)(a.s1 ? a.s1[col + v] * (double)lv1[v] : (double)lv1[v]) : A(0);
+        lv2[v] = in && two ? (A)(a.s2 ? a.s2[col + v] * (double)lv2[v] : (double)lv2[v]) : A(0);
       }
     }
     const int kb = a.rowptr[r], ke = a.rowptr[r + 1];
@@ -2418,17 +2424,17 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(SddmmArgs<T, G> a) {
         if (MC) c = a.colmap[c];
       }
       const int n = min(R, ke - base);
-      T mine = T(0);
+      A mine = A(0);
       for (int j = 0; j < n; ++j) {
         const int cj = __shfl(c, j, R);
-        T part = T(0);
+        A part = A(0);
         if (colok && cj >= 0) {
-          T rv[NV];
-          ld_panel<T, T>(a.r1 + (int64_t)cj * a.ldr + col, rv);
+          A rv[NV];
+          ld_panel<A, T>(a.r1 + (int64_t)cj * a.ldr + col, rv);
 #pragma unroll
           for (int v = 0; v < NV; ++v) part += lv1[v] * rv[v];
           if (two) {
-            ld_panel<T, T>(a.r2 + (int64_t)cj * a.ldr + col, rv);
+            ld_panel<A, T>(a.r2 + (int64_t)cj * a.ldr + col, rv);
 #pragma unroll
             for (int v = 0; v < NV; ++v) part += lv2[v] * rv[v];
           }
@@ -2504,6 +2510,10 @@ void sddmm(svdh_op *op, int64_t k, const void *L1, const void *R1, const double 
     sddmm_t<float, float>(op, k, L1, R1, s1, L2, R2, s2, ldl, ldr, grad);
   else if (dt == SVDH_F32 && gdt == SVDH_F64)
     sddmm_t<float, double>(op, k, L1, R1, s1, L2, R2, s2, ldl, ldr, grad);
+  else if (dt == SVDH_BF16 && gdt == SVDH_F32)
+    sddmm_t<bf16_t, float>(op, k, L1, R1, s1, L2, R2, s2, ldl, ldr, grad);
+  else if (dt == SVDH_BF16 && gdt == SVDH_F64)
+    sddmm_t<bf16_t, double>(op, k, L1, R1, s1, L2, R2, s2, ldl, ldr, grad);
   else
     fail(SVDH_ERR_ARG, "unsupported SDDMM dtypes factors=%d grad=%d", dt, gdt);
 }

@@ -27,7 +27,7 @@ def backward(matrix, rec, loss="sigma_sum", k=None, y=None, precision=None, tol=
     if precision is None:
         prec = _ffi.PREC_F64 if op.dtype == np.float64 else _ffi.PREC_F32
     else:
-        prec = {"f64": _ffi.PREC_F64, "f32": _ffi.PREC_F32}[precision]
+        prec = {"f64": _ffi.PREC_F64, "f32": _ffi.PREC_F32, "bf16": _ffi.PREC_BF16}[precision]
     u = np.ascontiguousarray(rec.u, dtype=np.float64)
     s = np.ascontiguousarray(rec.s, dtype=np.float64)
     vt = np.ascontiguousarray(rec.vt, dtype=np.float64)

@@ -20,7 +20,9 @@ Everything here is the oracle (oracle/adjoint.py) in FP64 on the CPU:
 The script asserts that the differences agree with the gradients, which pins
 the sparse oracle at full size. Run time: about ten minutes on 8 cores.
 
-Usage: python tests/golden/make_cfg5.py
+Usage: python tests/golden/make_cfg5.py [--cache DIR]
+(--cache keeps each finite-difference point in DIR as it completes, so an
+interrupted run resumes where it stopped.)
 """
 import json
 import os
@@ -55,8 +57,16 @@ def matrix():
     return a32.astype(np.float64)
 
 
+CACHE = None
+
+
 def fd_point(e):
     import oracle
+    if CACHE:
+        path = os.path.join(CACHE, f"fd_{e}.json")
+        if os.path.exists(path):
+            with open(path) as f:
+                return tuple(json.load(f))
     a = matrix()
     y = np.random.default_rng(Y_SEED).standard_normal(a.shape[0])
     out = {}
@@ -66,7 +76,11 @@ def fd_point(e):
         u, s, _ = oracle.adjoint.top_triplets(ap, KK, tol=1e-14, polish=3)
         c = u[:, :K].T @ y
         out[sign] = (float(s[:K].sum()), float(c @ c))
-    return ((out[1.0][0] - out[-1.0][0]) / (2 * H), (out[1.0][1] - out[-1.0][1]) / (2 * H))
+    res = ((out[1.0][0] - out[-1.0][0]) / (2 * H), (out[1.0][1] - out[-1.0][1]) / (2 * H))
+    if CACHE:
+        with open(os.path.join(CACHE, f"fd_{e}.json"), "w") as f:
+            json.dump(list(res), f)
+    return res
 
 
 def main():
@@ -124,4 +138,7 @@ def main():
 
 
 if __name__ == "__main__":
+    if "--cache" in sys.argv:
+        CACHE = sys.argv[sys.argv.index("--cache") + 1]
+        os.makedirs(CACHE, exist_ok=True)
     main()
