@@ -3,11 +3,16 @@ workload at a top-50 relative residual <= 1e-5, with the SpMM roofline.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32|bf16] [--no-cpu]
 
-A step is one whole solve (matrix already resident in HBM, U and V^T left in
-HBM, S on the host; the host-output time is reported beside it): rank 50, oversample 10, QR normaliser, power iterations until the
-Rayleigh-Ritz residual estimate of the top 50 triplets is <= 1e-5. The final
-triplets are re-checked in FP64 on the CPU outside the timed region:
+A step is one whole solve from the call (matrix already resident in HBM) to
+U, S and V^T in host memory: rank 50, oversample 10, QR normaliser, power
+iterations until the Rayleigh-Ritz residual estimate of the top 50 triplets
+is <= 1e-5. value is the median of the timed steps after the warm-up; the
+device-resident time (U and V^T left in HBM), the upload and the cold first
+solve are reported beside it. The returned triplets are re-checked in FP64 on
+the CPU outside the timed region:
 max_i sqrt(|A v_i - s_i u_i|^2 + |A^T u_i - s_i v_i|^2) / s_i.
+The other configurations (LAS2, batched masks, adjoint) follow in the same
+line as extra keys unless --no-extras.
 
 Prints one JSON line (rank 0).
 """
@@ -57,28 +62,43 @@ def traffic_from_profiles(name="spmm_cfg2_pmc.md"):
     return None
 
 
+def cpu_info():
+    """CPU model and logical CPU count of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
 def cpu_baseline(a, iters_needed, threads):
-    """The FP64 oracle restatement (C, OpenMP, the reference's Rayon chunking)
-    timed on a bounded sample: solves with q = 0 and q = 2 power iterations,
-    projected to iters_needed power iterations: the count the reference's own
-    (plain power iteration) algorithm needs for the same residual, measured
-    on the GPU with the shift and momentum switched off."""
+    """The C restatement of the reference's Rayon path (oracle/, OpenMP: row-
+    parallel A X, per-chunk partials of A^T Y with determine_chunk_size and a
+    serial reduction) in the reference's f32 instantiation, run as one whole
+    randomized_svd (rank 50, oversample 10, QR normaliser, f32 products, QR
+    and SVD) with iters_needed plain power iterations: the count the
+    reference's own algorithm needs for a top-50 residual of 1e-5 on this
+    matrix (measured on the GPU with the shift and momentum switched off).
+    The returned triplets' residual is recomputed in FP64."""
     import oracle
     op = oracle.OracleOp(a, nthreads=threads)
     om = np.random.default_rng(0).normal(size=(a.shape[1], RANK + OVERSAMPLE))
     t0 = time.perf_counter()
-    oracle.randomized_svd(op, RANK, OVERSAMPLE, 0, "QR", omega=om)
-    t_q0 = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    oracle.randomized_svd(op, RANK, OVERSAMPLE, 2, "QR", omega=om)
-    t_q2 = time.perf_counter() - t0
-    per_iter = max(t_q2 - t_q0, 0.0) / 2.0
-    projected = t_q0 + iters_needed * per_iter
-    return {"value": projected, "unit": "s", "cores": threads, "kind": "port",
-            "sample": (f"FP64 oracle randomized_svd on the same matrix: q=0 solve {t_q0:.2f} s and q=2 solve "
-                       f"{t_q2:.2f} s measured ({per_iter:.2f} s per power iteration), projected to the "
-                       f"{iters_needed} plain power iterations the reference's algorithm needs for a top-50 "
-                       f"residual of 1e-5 on this matrix")}
+    u, s, vt = oracle.randomized_svd(op, RANK, OVERSAMPLE, iters_needed, "QR", omega=om, dtype=np.float32)
+    wall = time.perf_counter() - t0
+    r_rel, r_norm = residual(a, u, s.astype(np.float64), vt)
+    model, ncpu = cpu_info()
+    return {"value": wall, "unit": "s", "cores": threads, "kind": "port",
+            "sample": (f"the whole f32 randomized_svd (C restatement of the reference's Rayon decomposition, "
+                       f"{threads} OpenMP threads) with {iters_needed} plain power iterations on the same matrix; "
+                       f"FP64 residual of its top-50 triplets {r_rel:.2e} (|A|-relative {r_norm:.2e})"),
+            "cpu_model": model, "logical_cpus": ncpu, "threads": threads, "power_iterations": iters_needed,
+            "residual_fp64": r_rel}
 
 
 # ---------------------------------------------------------------------------
@@ -161,8 +181,8 @@ def bench_las2(args):
             for _ in range(2):
                 if _ffi.hip().hipMemcpy(dst.ptr, src.ptr, nb, 3) != 0:
                     raise RuntimeError("device-to-device copy failed")
-            log(f"las2 {prec}: {np.mean(times) * 1e3:.1f} ms; calibration copy done (2 x 268435456 bytes)")
-            return {"metric": "LAS2 seconds-to-solution", "value": float(np.mean(times)), "unit": "s",
+            log(f"las2 {prec}: {np.median(times) * 1e3:.1f} ms; calibration copy done (2 x 268435456 bytes)")
+            return {"metric": "LAS2 seconds-to-solution", "value": float(np.median(times)), "unit": "s",
                     "profile_run": True}
         # the same solve handing U, s, V^T over in host memory (PCIe-inclusive)
         t0 = time.perf_counter()
@@ -176,7 +196,7 @@ def bench_las2(args):
         _ffi.check(lib.svdh_ctx_profile(ctx, 0))
         d = rec.diagnostics
         r = _triplet_residual(a, rec.u.T, rec.s.astype(np.float64), rec.vt)
-        res[prec] = {"s_per_solve": float(np.mean(times)), "host_output_s": t_host, "lanczos_steps": d.lanczos_steps,
+        res[prec] = {"s_per_solve": float(np.median(times)), "host_output_s": t_host, "lanczos_steps": d.lanczos_steps,
                      "ritz_values_stabilized": d.ritz_values_stabilized, "significant_values": d.significant_values,
                      "residual_fp64": r, "sigma_max": float(rec.s[0]), "sigma_min": float(rec.s[-1]),
                      "roofline_spmv": _roofline(st["spmv"]["bytes"], st["spmv"]["_ms"], st["spmv"]["_n"]),
@@ -307,57 +327,65 @@ def bench_batched(args):
 
 
 def bench_adjoint(args):
-    """Gradients of sum of the top-32 singular values and of |U_32^T y|^2
-    with respect to the CSR values of the 100,000 x 20,000 matrix (FP32
-    factors and SpMM), compared on 1,000 random nonzeros with the FP64
-    gradient of FP64 factors."""
+    """backward() of sum of the top-32 singular values and of |U_32^T y|^2
+    with respect to the CSR values of the 100,000 x 20,000 matrix. The
+    factors come from the GPU randomized SVD with the BF16 sketch ladder
+    (rank 50, oversample 10, tol 1e-5); the backward runs with BF16 factor
+    panels in the SDDMM (FP32 accumulation) and in FP32. Accuracy on the
+    1,000 nonzeros of the FP64 CPU fixture (tests/golden/cfg5_grad.json) when
+    it is present."""
     import single_svdlib as sl
     from single_svdlib import _ffi, synth
     a = synth.config2(np.float32)
     lib = _ffi.lib()
     k = 32
-    y = np.random.default_rng(20261207).normal(size=a.shape[0])
-    op = sl.CsrMatrix.from_scipy(a)
+    y = np.random.default_rng(20261205).standard_normal(a.shape[0])  # the fixture's y
+    op = sl.CsrMatrix.from_scipy(a, keep_bf16=True)
     ctx = op._ctx
-    rec = sl.randomized.randomized_svd(op, k, 10, 3, "QR", False, 3, False, tol=1e-5, max_power_iters=400)
+    t0 = time.perf_counter()
+    rec = sl.randomized.randomized_svd(op, 50, 10, 3, "QR", False, 42, False, tol=1e-5, max_power_iters=400,
+                                       precision="bf16")
+    t_fwd = time.perf_counter() - t0
+    fix = None
+    path = os.path.join(ROOT, "tests", "golden", "cfg5_grad.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            fix = json.load(f)
     out = {}
-    for loss in ("sigma_sum", "projection"):
-        for _ in range(args.warmup):
-            rec.backward(op, loss=loss, k=k, y=y, grad_dtype=np.float32)
-        _ffi.check(lib.svdh_ctx_synchronize(ctx))
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            g, info = rec.backward(op, loss=loss, k=k, y=y, grad_dtype=np.float32)
-        out[loss] = {"s": (time.perf_counter() - t0) / args.steps, "grad": g, "info": info}
+    for prec in ("bf16", "f32"):
+        for loss in ("sigma_sum", "projection"):
+            for _ in range(args.warmup):
+                rec.backward(op, loss=loss, k=k, y=y, precision=prec, grad_dtype=np.float32)
+            _ffi.check(lib.svdh_ctx_synchronize(ctx))
+            ts = []
+            for _ in range(args.steps):
+                t0 = time.perf_counter()
+                g, info = rec.backward(op, loss=loss, k=k, y=y, precision=prec, grad_dtype=np.float32)
+                ts.append(time.perf_counter() - t0)
+            r = {"s": float(np.median(ts)), "cg_iterations": info["cg_iterations"]}
+            if fix is not None:
+                idx = np.asarray(fix["idx"])
+                ref = np.asarray(fix["grad_" + loss])
+                r["max_err_vs_fp64_fixture"] = float(np.max(np.abs(g[idx] - ref)) / np.max(np.abs(ref)))
+                pos = np.asarray(fix["fd_positions"])
+                fd = np.asarray(fix["fd_" + loss])
+                r["max_err_vs_central_differences"] = float(np.max(np.abs(g[idx[pos]] - fd)) / np.max(np.abs(ref)))
+            out[f"{loss}_{prec}"] = r
     _ffi.check(lib.svdh_ctx_profile(ctx, 1))
-    rec.backward(op, loss="sigma_sum", k=k, y=y, grad_dtype=np.float32)
+    rec.backward(op, loss="sigma_sum", k=k, y=y, precision="bf16", grad_dtype=np.float32)
     st = _kernel_stats(lib, ctx, (("sddmm", _ffi.KC_SDDMM),), 1)
     _ffi.check(lib.svdh_ctx_profile(ctx, 0))
-    m, n = a.shape
-    sddmm_bytes = a.nnz * (4 + 4) + (m + 1) * 4 + (m + n) * k * 4
-    roof = _roofline(sddmm_bytes * st["sddmm"]["_n"], st["sddmm"]["_ms"], st["sddmm"]["_n"])
-    # FP64 reference: FP64 factors and FP64 backward on the GPU
-    op64 = sl.CsrMatrix.from_scipy(a.astype(np.float64))
-    rec64 = sl.randomized.randomized_svd(op64, k, 10, 3, "QR", False, 3, False, tol=1e-9, max_power_iters=1000)
-    pick = np.random.default_rng(1).choice(a.nnz, 1000, replace=False)
-    errs = {}
-    for loss in ("sigma_sum", "projection"):
-        g64, _ = rec64.backward(op64, loss=loss, k=k, y=y, tol=1e-10)
-        ref = g64[pick]
-        errs[loss] = float(np.max(np.abs(out[loss]["grad"][pick] - ref)) / np.max(np.abs(ref)))
-    log(f"adjoint sigma_sum {out['sigma_sum']['s'] * 1e3:.1f} ms, projection {out['projection']['s'] * 1e3:.1f} ms "
-        f"({out['projection']['info']['cg_iterations']} CG iterations); errors vs FP64 {errs}")
+    roof = _roofline(st["sddmm"]["bytes"], st["sddmm"]["_ms"], st["sddmm"]["_n"])
+    log(f"adjoint: forward (BF16 ladder) {t_fwd * 1e3:.1f} ms; " +
+        ", ".join(f"{key} {v['s'] * 1e3:.2f} ms" for key, v in out.items()))
     return {"metric": "adjoint seconds (gradient of the top-32 singular values w.r.t. the CSR values)",
-            "value": out["sigma_sum"]["s"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": out["sigma_sum"]["s"] * 1e3, "higher_is_better": False, "scaling": "strong",
-            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "value": out["sigma_sum_bf16"]["s"], "unit": "s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": out["sigma_sum_bf16"]["s"] * 1e3, "higher_is_better": False,
+            "scaling": "strong", "vs_baseline": None, "dtype": "bf16 factors, f32 accumulation", "data": "synthetic",
             "config": {"workload": ("backward() of sum of the top-32 singular values and of |U_32^T y|^2 on the "
-                                    "100,000 x 20,000 Zipf CSR, factors from randomized_svd (rank 32 + 10, "
-                                    "tol 1e-5)"), "k": k, "nnz": int(a.nnz)},
-            "roofline": roof, "cpu_baseline": None,
-            "projection_loss_s": out["projection"]["s"],
-            "projection_cg_iterations": out["projection"]["info"]["cg_iterations"],
-            "max_rel_error_vs_fp64_on_1000_nonzeros": errs}
+                                    "100,000 x 20,000 Zipf CSR, factors from randomized_svd with the BF16 sketch "
+                                    "(rank 50 + 10, tol 1e-5)"), "k": k, "nnz": int(a.nnz)},
+            "roofline": roof, "cpu_baseline": None, "forward_s": t_fwd, "losses": out}
 
 
 def bench_ingest(args):
@@ -428,6 +456,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="f32", choices=["f32", "bf16"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="only the headline solve (no other configurations)")
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3 runs: the timed solves plus one 256 MiB device-to-device copy (the calibration "
                          "kernel for FETCH_SIZE / WRITE_SIZE), without the correctness and reference extras")
@@ -450,13 +479,15 @@ def main():
 
     t0 = time.perf_counter()
     a = synth.config2(np.float32)
-    log(f"generated cfg2 {a.shape} nnz={a.nnz} in {time.perf_counter() - t0:.1f}s")
+    t_gen = time.perf_counter() - t0
+    log(f"generated cfg2 {a.shape} nnz={a.nnz} in {t_gen:.1f}s")
     t0 = time.perf_counter()
     op = sl.CsrMatrix.from_scipy(a, keep_bf16=args.precision == "bf16")
     lib = _ffi.lib()
     ctx = op._ctx
     _ffi.check(lib.svdh_ctx_synchronize(ctx))
-    log(f"uploaded in {time.perf_counter() - t0:.1f}s")
+    t_upload = time.perf_counter() - t0
+    log(f"uploaded in {t_upload:.2f}s")
     prec = _ffi.PREC_BF16 if args.precision == "bf16" else _ffi.PREC_F32
 
     m, n = a.shape
@@ -466,7 +497,7 @@ def main():
     s_host = np.zeros(k)
 
     def solve(tol, q):
-        """One solve with U and V^T left in HBM (the measured step)."""
+        """One solve with U and V^T left in HBM."""
         meta = _ffi.Result_()
         _ffi.check(lib.svdh_rsvd_device(ctx, op._handle, RANK, OVERSAMPLE, q, _ffi.NORM_QR, 0, 42, 1, None, prec,
                                         tol, args.max_iters, u_dev.ptr, k, _ffi.ptr(s_host), vt_dev.ptr, n,
@@ -474,27 +505,53 @@ def main():
         return meta.power_iters_run, meta.residual_estimate
 
     def solve_host(tol, q):
-        """The same solve returning U, S, V^T in host memory (PCIe-inclusive)."""
+        """One solve returning U, S, V^T in host memory (the measured step):
+        randomized_svd's SvdRec, through svdh_rsvd."""
         res = _ffi.Result_()
         _ffi.check(lib.svdh_rsvd(ctx, op._handle, RANK, OVERSAMPLE, q, _ffi.NORM_QR, 0, 42, 1, None, prec,
                                  tol, args.max_iters, C.byref(res)))
         return res
 
-    for i in range(args.warmup):
-        solve(TOL, 3)
-        log(f"warmup {i} done")
-    _ffi.check(lib.svdh_ctx_synchronize(ctx))
-    iters = []
+    # the first solve after the upload builds the operator's schedules and
+    # popular-column layouts: timed on its own
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        iters.append(solve(TOL, 3))
+    first = solve_host(TOL, 3)
+    t_cold = time.perf_counter() - t0
+    lib.svdh_result_free(C.byref(first))
+    log(f"cold first solve {t_cold * 1e3:.1f} ms")
+    for i in range(args.warmup):
+        lib.svdh_result_free(C.byref(solve_host(TOL, 3)))
     _ffi.check(lib.svdh_ctx_synchronize(ctx))
-    elapsed = time.perf_counter() - t0
-    per_step = elapsed / args.steps
-    log(f"timed {args.steps} solves: {per_step * 1e3:.1f} ms each, power iterations {iters}")
+    # timed: K solves to results on the host, each timed on its own; value is
+    # their median
+    times, iters = [], []
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        res = solve_host(TOL, 3)
+        times.append(time.perf_counter() - t0)
+        iters.append((res.power_iters_run, res.residual_estimate))
+        if i < args.steps - 1:
+            lib.svdh_result_free(C.byref(res))
+    _ffi.check(lib.svdh_ctx_synchronize(ctx))
+    wall = (time.perf_counter() - t_all) / args.steps
+    per_step = float(np.median(times))
+    rec_host = sl.utils.result_to_svdrec(res, np.float64)
+    log(f"timed {args.steps} host-output solves: median {per_step * 1e3:.2f} ms (min {min(times) * 1e3:.2f}, "
+        f"max {max(times) * 1e3:.2f}), power iterations {sorted(set(i[0] for i in iters))}")
 
-    # The same steps again with HIP events around every kernel launch on the
-    # library's stream: per-kernel averages for the roofline.
+    # the same solve with U and V^T left in HBM
+    dev_times = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        solve(TOL, 3)
+        dev_times.append(time.perf_counter() - t0)
+    t_dev = float(np.median(dev_times))
+    log(f"device-resident solves: median {t_dev * 1e3:.2f} ms")
+
+    # The same device-resident solves again with HIP events around every
+    # kernel launch on the library's stream: per-kernel averages for the
+    # roofline.
     _ffi.check(lib.svdh_ctx_profile(ctx, 1))
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -518,6 +575,12 @@ def main():
     spmm_avg_ms = sp_ms / max(sp_n, 1)
     spmm_bytes = sp_bytes / max(sp_n, 1)
     achieved = spmm_bytes / (spmm_avg_ms * 1e-3) / 1e9 if sp_n else 0.0
+    per_dir = {}
+    for name in ("spmm_ax", "spmm_aty"):
+        v = kc[name]
+        per_dir[name] = {"achieved": v["gbs"], "frac": v["gbs"] / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_launch": v["_bytes"] / max(v["_n"], 1),
+                         "avg_launch_ms": v["_ms"] / max(v["_n"], 1)}
     for v in kc.values():
         for key in ("_ms", "_n", "_bytes"):
             v.pop(key)
@@ -536,17 +599,10 @@ def main():
                           "kernels": kc}))
         return
 
-    # correctness of the timed configuration (outside the timed region)
-    solve(TOL, 3)
-    r_rel, r_norm = residual(a, u_dev.numpy(), s_host.copy(), vt_dev.numpy())
+    # correctness of the timed configuration (outside the timed region): the
+    # last timed solve's host triplets in FP64
+    r_rel, r_norm = residual(a, rec_host.u, rec_host.s, rec_host.vt)
     log(f"FP64 residual of the returned top-{RANK}: per-sigma {r_rel:.2e}, |A|-relative {r_norm:.2e}")
-
-    # PCIe-inclusive variant: results handed over in host memory
-    t0 = time.perf_counter()
-    for _ in range(3):
-        lib.svdh_result_free(C.byref(solve_host(TOL, 3)))
-    t_host = (time.perf_counter() - t0) / 3
-    log(f"host-output solve {t_host * 1e3:.1f} ms")
 
     # The reference's algorithm (plain power iterations, no shift or momentum)
     # to the same tolerance on the GPU: its iteration count sizes the CPU
@@ -554,30 +610,42 @@ def main():
     os.environ["SVDH_RSVD_SHIFT"] = "0"
     os.environ["SVDH_RSVD_MOMENTUM"] = "0"
     try:
-        t0 = time.perf_counter()
-        plain = solve(TOL, 3)
-        t_plain = time.perf_counter() - t0
+        plain_t = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r = solve_host(TOL, 3)
+            plain_t.append(time.perf_counter() - t0)
+            plain = (r.power_iters_run, r.residual_estimate)
+            lib.svdh_result_free(C.byref(r))
+        t_plain = float(np.median(plain_t))
     finally:
         del os.environ["SVDH_RSVD_SHIFT"]
         del os.environ["SVDH_RSVD_MOMENTUM"]
     log(f"plain power iterations: {plain[0]} iterations, {t_plain * 1e3:.1f} ms")
 
     # the configuration as specified (3 power iterations) for reference
-    t0 = time.perf_counter()
-    q3_runs = 3
-    for _ in range(q3_runs):
-        solve(0.0, 3)
-    t_q3 = (time.perf_counter() - t0) / q3_runs
-    res3 = solve_host(0.0, 3)
+    q3_t = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        res3 = solve_host(0.0, 3)
+        q3_t.append(time.perf_counter() - t0)
+        if len(q3_t) < 5:
+            lib.svdh_result_free(C.byref(res3))
+    t_q3 = float(np.median(q3_t))
     rec3 = sl.utils.result_to_svdrec(res3, np.float32)
     r3_rel, _ = residual(a, rec3.u, rec3.s.astype(np.float64), rec3.vt)
     log(f"q=3 solve {t_q3 * 1e3:.1f} ms, residual {r3_rel:.2e}")
+
+    extras = {}
+    if not args.no_extras:
+        del op
+        extras = other_paths(args)
 
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(a, int(plain[0]), threads)
-        log(f"cpu baseline {cpu['value']:.2f}s")
+        log(f"cpu baseline {cpu['value']:.2f}s ({cpu['sample']})")
 
     traffic = traffic_from_profiles()
     out = {
@@ -596,25 +664,76 @@ def main():
         "config": {
             "workload": ("randomized_svd on the 100,000 x 20,000 CSR (row nnz ~ Poisson(100), Zipf(1.1) columns, "
                          "log1p(Poisson(2)+1) values, nnz 9,998,163), rank 50, oversample 10, QR normaliser, "
-                         "power iterations until the top-50 residual <= 1e-5"),
+                         "power iterations until the top-50 residual <= 1e-5, U, S, V^T returned in host memory"),
             "rank": RANK, "oversample": OVERSAMPLE, "tol": TOL, "nnz": int(a.nnz),
-            "power_iters": [i[0] for i in iters],
+            "power_iters": sorted(set(int(i[0]) for i in iters)),
         },
         "roofline": {
             "bound": "hbm", "kernel": "spmm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "algorithmic_bytes_per_launch": spmm_bytes, "avg_launch_ms": spmm_avg_ms,
+            "algorithmic_bytes_per_launch": spmm_bytes, "avg_launch_ms": spmm_avg_ms, "per_direction": per_dir,
         },
         "cpu_baseline": cpu,
+        "timing": {"statistic": f"median of {args.steps} solves after {args.warmup} warm-up",
+                   "mean_s": wall, "min_s": min(times), "max_s": max(times)},
+        "device_resident_solve_s": t_dev,
+        "upload_s": t_upload,
+        "generate_s": t_gen,
+        "cold_first_solve_s": t_cold,
         "residual_fp64": {"per_sigma": r_rel, "norm_relative": r_norm},
         "kernels": kc,
         "ms_per_step_profiled": per_step_prof * 1e3,
-        "host_output_solve_s": t_host,
-        "plain_power_iterations": {"iterations": plain[0], "s": t_plain},
+        "plain_power_iterations": {"iterations": int(plain[0]), "s": t_plain},
         "q3_solve_s": t_q3,
         "q3_residual_fp64": r3_rel,
     }
+    out.update(extras)
     print(json.dumps(out))
+
+
+def other_paths(args):
+    """The other configurations, timed in the same run (medians of 5 solves
+    after 1 warm-up): LAS2 at 2,000 x 1,500 (k = 10) and 500,000 x 30,000
+    (k = 100, FP32-SpMV/FP64-recurrence, with the SpMV roofline), the 64-mask
+    batched randomized SVD, and the adjoint on the 100,000 x 20,000 matrix."""
+    import types
+    sub = types.SimpleNamespace(steps=5, warmup=1, no_cpu=True, profile_run=False)
+    out = {}
+
+    def run(name, fn, wl):
+        t0 = time.perf_counter()
+        try:
+            r = fn(types.SimpleNamespace(**vars(sub), workload=wl))
+        except Exception as e:  # an extra must not lose the headline line
+            log(f"{name} failed: {e!r}")
+            out[name + "_error"] = repr(e)
+            return None
+        log(f"{name} done in {time.perf_counter() - t0:.1f}s")
+        return r
+
+    r = run("las2_cfg1", bench_las2, "las2-cfg1")
+    if r:
+        out["las2_cfg1_s"] = r["precisions"]["f64"]["s_per_solve"]
+        out["las2_cfg1_host_output_s"] = r["precisions"]["f64"]["host_output_s"]
+    r = run("las2_cfg3", bench_las2, "las2-cfg3")
+    if r:
+        out["las2_cfg3_s"] = r["value"]
+        out["las2_cfg3_f64_s"] = r["precisions"]["f64"]["s_per_solve"]
+        out["las2_cfg3_host_output_s"] = r["precisions"]["f32"]["host_output_s"]
+        out["las2_cfg3_spmv_roofline"] = r["roofline"]
+        out["las2_cfg3_lanczos_steps"] = r["precisions"]["f32"]["lanczos_steps"]
+    r = run("batched_cfg4", bench_batched, "batched")
+    if r:
+        out["batched_cfg4_s"] = r["value"]
+        out["batched_cfg4_one_by_one_s"] = r["one_by_one_s"]
+        out["batched_cfg4_roofline"] = r["roofline"]
+    r = run("adjoint_cfg5", bench_adjoint, "adjoint")
+    if r:
+        out["adjoint_cfg5_s"] = r["value"]
+        out["adjoint_cfg5"] = {key: r[key] for key in r if key not in ("metric", "n_gpus", "steps", "warmup",
+                                                                       "higher_is_better", "scaling", "vs_baseline",
+                                                                       "data", "cpu_baseline")}
+    return out
 
 
 if __name__ == "__main__":

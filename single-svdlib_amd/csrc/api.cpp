@@ -5,7 +5,10 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "common.h"
@@ -13,6 +16,67 @@
 namespace svdh {
 
 static thread_local std::string g_last_error;
+
+// Host result arrays (SvdRec u, s, vt) come from pinned memory that
+// svdh_result_free returns to a process-wide pool instead of releasing: the
+// next result of a similar size reuses it, so the hand-over of U and V^T is
+// a DMA copy into memory that is already mapped (no page faults). Free
+// buffers beyond kPoolKeep bytes are unpinned.
+namespace {
+std::mutex g_pool_mu;
+std::unordered_map<void *, size_t> g_pool_live;  // pinned buffers handed out: bytes
+std::multimap<size_t, void *> g_pool_free;        // pinned buffers to reuse, by size
+size_t g_pool_free_bytes = 0;
+constexpr size_t kPoolKeep = size_t(4) << 30;
+}  // namespace
+
+double *result_alloc(int64_t count) {
+  const size_t bytes = std::max<size_t>((size_t)std::max<int64_t>(count, 1) * sizeof(double), 64);
+  {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    auto it = g_pool_free.lower_bound(bytes);
+    if (it != g_pool_free.end() && it->first <= 2 * bytes) {
+      void *p = it->second;
+      g_pool_free_bytes -= it->first;
+      g_pool_live[p] = it->first;
+      g_pool_free.erase(it);
+      return static_cast<double *>(p);
+    }
+  }
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) {
+    (void)hipGetLastError();
+    p = std::malloc(bytes);  // unpinned: still correct, the copy is slower
+    if (!p) fail(SVDH_ERR_ALLOC, "result allocation of %zu bytes failed", bytes);
+    return static_cast<double *>(p);
+  }
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  g_pool_live[p] = bytes;
+  return static_cast<double *>(p);
+}
+
+void result_release(void *p) {
+  if (!p) return;
+  std::vector<void *> drop;
+  {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    auto it = g_pool_live.find(p);
+    if (it == g_pool_live.end()) {
+      std::free(p);
+      return;
+    }
+    g_pool_free.emplace(it->second, p);
+    g_pool_free_bytes += it->second;
+    g_pool_live.erase(it);
+    while (g_pool_free_bytes > kPoolKeep && !g_pool_free.empty()) {
+      auto big = std::prev(g_pool_free.end());
+      g_pool_free_bytes -= big->first;
+      drop.push_back(big->second);
+      g_pool_free.erase(big);
+    }
+  }
+  for (void *q : drop) (void)hipHostFree(q);
+}
 
 void fail(int32_t code, const char *fmt, ...) {
   char buf[1024];
@@ -633,9 +697,9 @@ int32_t svdh_backward(svdh_ctx *ctx, const svdh_op *op, const svdh_result *res, 
 
 void svdh_result_free(svdh_result *res) {
   if (!res) return;
-  std::free(res->u);
-  std::free(res->s);
-  std::free(res->vt);
+  result_release(res->u);
+  result_release(res->s);
+  result_release(res->vt);
   res->u = res->s = res->vt = nullptr;
 }
 

@@ -451,4 +451,9 @@ void upper_inverse(int64_t l, const double *R, double *Rinv);
 
 size_t dtype_size(int dt);
 
+// Host arrays of a result (SvdRec u, s, vt): pinned buffers from a pool that
+// result_release (svdh_result_free) refills; contents are not cleared.
+double *result_alloc(int64_t count);
+void result_release(void *p);
+
 }  // namespace svdh

@@ -405,9 +405,9 @@ struct Las2 {
     out->vt_rows = d;
     out->vt_cols = transposed ? nrows : ncols;
     if (!dev) {
-      out->s = static_cast<double *>(std::calloc(std::max<int64_t>(d, 1), sizeof(double)));
-      out->u = static_cast<double *>(std::calloc(std::max<int64_t>(d * out->u_cols, 1), sizeof(double)));
-      out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(d * out->vt_cols, 1), sizeof(double)));
+      out->s = result_alloc(d);
+      out->u = result_alloc(d * out->u_cols);
+      out->vt = result_alloc(d * out->vt_cols);
       if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
     }
     if (d == 0) return;

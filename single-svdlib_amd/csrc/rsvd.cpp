@@ -558,9 +558,9 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
   if (dev) {
     for (int64_t r = 0; r < k; ++r) dev->s[r] = sv[r];
   } else {
-    out->s = static_cast<double *>(std::calloc(std::max<int64_t>(k, 1), sizeof(double)));
-    out->u = static_cast<double *>(std::calloc(std::max<int64_t>(m * k, 1), sizeof(double)));
-    out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(k * n, 1), sizeof(double)));
+    out->s = result_alloc(k);
+    out->u = result_alloc(m * k);
+    out->vt = result_alloc(k * n);
     if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
     for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
     if (k > 0) {
@@ -754,7 +754,7 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
     out->u_cols = k;
     out->vt_rows = k;
     out->vt_cols = n;
-    out->s = static_cast<double *>(std::calloc(std::max<int64_t>(k, 1), sizeof(double)));
+    out->s = result_alloc(k);
     if (!out->s) fail(SVDH_ERR_ALLOC, "result allocation failed");
     for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
     if (dev) {  // U and V^T straight into the caller's device buffers
@@ -767,8 +767,8 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
       double *vtd = static_cast<double *>(ctx->workspace(WS_VT, std::max<size_t>(k * n, 1) * 8));
       panel_apply(c, m, l, k, Y[b], ld, pdt, MW.data(), ud, k, SVDH_F64);
       panel_apply(c, n, w, k, Z[b], ld, pdt, Ws.data(), vtd, 1, SVDH_F64, n);
-      out->u = static_cast<double *>(std::calloc(std::max<int64_t>(m * k, 1), sizeof(double)));
-      out->vt = static_cast<double *>(std::calloc(std::max<int64_t>(k * n, 1), sizeof(double)));
+      out->u = result_alloc(m * k);
+      out->vt = result_alloc(k * n);
       if (!out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
       if (k > 0) {
         HIP_CHECK(hipMemcpyAsync(out->u, ud, (size_t)m * k * 8, hipMemcpyDeviceToHost, c->stream));
