@@ -28,7 +28,7 @@ import json
 import os
 import sys
 import time
-from multiprocessing import Pool
+from multiprocessing import get_context
 
 import numpy as np
 
@@ -58,6 +58,11 @@ def matrix():
 
 
 CACHE = None
+
+
+def _set_cache(path):
+    global CACHE
+    CACHE = path
 
 
 def fd_point(e):
@@ -99,7 +104,9 @@ def main():
     g2, info = oracle.adjoint.gradient_sparse(a, "projection", K, y, triplets=tr, tol=1e-12)
     print(f"gradients in {time.time() - t0:.1f} s; CG {info}", flush=True)
     c = u[:, :K].T @ y
-    with Pool(4) as pool:
+    # spawned workers: forking after the threaded BLAS/OpenMP calls above
+    # leaves the children waiting on locks the parent's threads held
+    with get_context("spawn").Pool(4, initializer=_set_cache, initargs=(CACHE,)) as pool:
         fds = pool.map(fd_point, [int(idx[p]) for p in fd_pos])
     fd1 = np.array([f[0] for f in fds])
     fd2 = np.array([f[1] for f in fds])

@@ -239,3 +239,31 @@ def test_normalizers_match_oracle(lib, norm, sv_tol):
     err = float(np.max(np.abs(rec.s - s) / s))
     assert err < sv_tol, err
     assert max_principal_angle(rec.u[:, :5], u[:, :5]) < 1e-6
+
+
+def graded_matrix(m, n, r, span, seed, floor=1e-9):
+    """A (m x n, stored as CSR) = U diag(s) V^T + noise with random dense
+    orthonormal U, V and s log-spaced from 1 down to 1/span over r values."""
+    rng = np.random.default_rng(seed)
+    u, _ = np.linalg.qr(rng.normal(size=(m, r)))
+    v, _ = np.linalg.qr(rng.normal(size=(n, r)))
+    s = np.logspace(0, -np.log10(span), r)
+    return sp.csr_matrix((u * s) @ v.T + floor * rng.normal(size=(m, n)))
+
+
+@pytest.mark.parametrize("span", [1e4, 1e6])
+def test_graded_spectrum_matches_oracle(lib, span):
+    # the top-l singular values span 1e4 / 1e6: the randomized solve (implicit
+    # CholeskyQR, SVD of B from the Gram of Z = B^T) against the oracle's
+    # Householder QR and direct SVD of B (randomized/mod.rs:96-173) with the
+    # same Omega, at the FP64 bar for every returned value
+    a = graded_matrix(2000, 300, 30, span, seed=3)
+    om = np.random.default_rng(1).normal(size=(300, 30))
+    rec = lib.randomized.randomized_svd(lib.CsrMatrix.from_scipy(a), 20, 10, 2, "QR", False, 1, False, omega=om)
+    u, s, vt = oracle.randomized_svd(oracle.OracleOp(a), 20, 10, 2, "QR", omega=om)
+    err = float(np.max(np.abs(rec.s - s) / s))
+    assert s[-1] / s[0] < 10.0 / span ** (2 / 3)  # the returned values do span the range
+    assert err < 1e-10, err
+    assert max_principal_angle(rec.u, u) < 1e-8
+    assert max_principal_angle(rec.vt.T, vt.T) < 1e-8
+    assert rec.extra["qr_fallbacks"] == 0

@@ -606,3 +606,23 @@ def test_adjoint_sparse_path_planted_mistake_fails():
     g_bad = (np.einsum("ei,ei->e", (w_miss * (2 * c * s[:k]))[rows], vt[:k, a.indices].T)
              + np.einsum("ei,ei->e", (u[:, :k] * 2 * c)[rows], atw[a.indices]))
     assert np.max(np.abs(g_bad - gd)) > 1e-3 * np.max(np.abs(gd))
+
+
+def test_cfg5_fixture_is_consistent():
+    # the committed FP64 gradients agree with their own central differences
+    # (each from fresh FP64 solves of the perturbed matrix), and the sampled
+    # nonzeros are where the fixture says they are in the regenerated matrix
+    from single_svdlib import synth
+    fix = golden("cfg5_grad.json")
+    a = synth.config2(np.float32)
+    assert synth.checksum(a) == fix["sha256"]
+    idx = np.asarray(fix["idx"])
+    rows = np.repeat(np.arange(a.shape[0]), np.diff(a.indptr))
+    assert np.array_equal(rows[idx], fix["rows"]) and np.array_equal(a.indices[idx], fix["cols"])
+    pos = np.asarray(fix["fd_positions"])
+    for loss in ("sigma_sum", "projection"):
+        g = np.asarray(fix["grad_" + loss])
+        fd = np.asarray(fix["fd_" + loss])
+        assert np.max(np.abs(g[pos] - fd)) < 1e-5 * np.max(np.abs(g))
+    # the sigma-sum loss is the sum of the top 32 values listed
+    assert abs(sum(fix["sigma"][:32]) - fix["loss_sigma_sum"]) < 1e-9 * fix["loss_sigma_sum"]
