@@ -445,8 +445,78 @@ def bench_ingest(args):
             "whole_call_s": total}
 
 
+def bench_gather_ceiling(args):
+    """The measured ceiling of the SpMM's gathers on the 100,000 x 20,000
+    workload (libsvdh_probe.so: one panel row per stream entry, 16 bytes per
+    lane, no arithmetic and no output). Streams: the column indices of A in
+    CSR order (A X gathers rows of the 20,000-row panel) and the row indices
+    of A in CSC order (A^T Y gathers rows of the 100,000-row panel), each with
+    and without the 256 most popular columns (the SpMM serves those from LDS
+    or the matrix cores). FP32 rows of 60 values (240 bytes, 256 apart) and
+    BF16 rows (120 bytes, 128 apart); sweeps of loads in flight per row group
+    and waves per CU; the best rate of each stream is its ceiling."""
+    from single_svdlib import synth
+    from single_svdlib._ffi import PKG_DIR
+    lib = C.CDLL(os.path.join(PKG_DIR, "libsvdh_probe.so"))
+    lib.svdh_probe_gather.restype = C.c_int32
+    lib.svdh_probe_gather.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                      C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.c_int32,
+                                      C.POINTER(C.c_double)]
+    a = synth.config2(np.float32)
+    m, n = a.shape
+    counts = np.bincount(a.indices, minlength=n)
+    hot = np.argsort(-counts, kind="stable")[:256]
+    is_hot = np.zeros(n, dtype=bool)
+    is_hot[hot] = True
+    csc = a.tocsc()
+    streams = {
+        "ax_all": (a.indices.astype(np.int32), n),
+        "ax_unstaged": (a.indices[~is_hot[a.indices]].astype(np.int32), n),
+        "aty_all": (csc.indices.astype(np.int32), m),
+        "aty_unstaged": (np.concatenate([csc.indices[csc.indptr[c]:csc.indptr[c + 1]] for c in range(n)
+                                         if not is_hot[c]]).astype(np.int32), m),
+    }
+    # the same unstaged streams grouped by eight equal source blocks, each
+    # block gathered by the workgroups of one XCD (the SpMM segment schedule)
+    for name in ("ax_unstaged", "aty_unstaged"):
+        st, nsrc = streams[name]
+        bw = (nsrc + 7) // 8
+        blk = st // bw
+        order = np.argsort(blk, kind="stable")
+        off = np.zeros(9, dtype=np.int64)
+        np.cumsum(np.bincount(blk, minlength=8), out=off[1:])
+        streams[name + "_xcd8"] = (st[order].astype(np.int32), nsrc, off)
+    out = {}
+    for name, spec in streams.items():
+        st, nsrc = spec[0], spec[1]
+        off = spec[2] if len(spec) > 2 else None
+        st = np.ascontiguousarray(st)
+        offp = off.ctypes.data_as(C.POINTER(C.c_int64)) if off is not None else None
+        for dt, row, ld in (("f32", 240, 256), ("bf16", 120, 128)):
+            best = None
+            for u in (2, 4, 8):
+                for wpc in (8, 16, 32):
+                    ms = C.c_double()
+                    rc = lib.svdh_probe_gather(0, st.ctypes.data_as(C.POINTER(C.c_int32)), st.size, nsrc, ld, row, u,
+                                               256 * wpc, 5, offp, 8 if off is not None else 1, C.byref(ms))
+                    if rc != 0:
+                        raise RuntimeError(f"probe failed with {rc}")
+                    tbs = st.size * row / (ms.value * 1e-3) / 1e12
+                    if best is None or tbs > best["tb_s"]:
+                        best = {"tb_s": tbs, "ms": ms.value, "u": u, "waves_per_cu": wpc}
+            best["entries"] = int(st.size)
+            best["bytes"] = int(st.size) * row
+            out[f"{name}_{dt}"] = best
+            log(f"gather ceiling {name} {dt}: {best['tb_s']:.1f} TB/s ({best['ms'] * 1e3:.1f} us, U={best['u']}, "
+                f"{best['waves_per_cu']} waves/CU, {st.size} entries)")
+    return {"metric": "SpMM gather ceiling (TB/s of panel rows gathered, 100,000 x 20,000 Zipf CSR)",
+            "value": out["ax_unstaged_f32"]["tb_s"], "unit": "TB/s", "n_gpus": 1, "higher_is_better": True,
+            "data": "synthetic", "config": {"workload": "panel-row gathers of the SpMM column streams, no arithmetic"},
+            "streams": out}
+
+
 OTHER_WORKLOADS = {"las2-cfg1": bench_las2, "las2-cfg3": bench_las2, "batched": bench_batched,
-                   "adjoint": bench_adjoint, "coo-ingest": bench_ingest}
+                   "adjoint": bench_adjoint, "coo-ingest": bench_ingest, "gather-ceiling": bench_gather_ceiling}
 
 
 def main():
@@ -462,7 +532,7 @@ def main():
                          "kernel for FETCH_SIZE / WRITE_SIZE), without the correctness and reference extras")
     ap.add_argument("--max-iters", type=int, default=400)
     ap.add_argument("--workload", default="rsvd",
-                    choices=["rsvd", "las2-cfg1", "las2-cfg3", "batched", "adjoint", "coo-ingest"],
+                    choices=["rsvd", "las2-cfg1", "las2-cfg3", "batched", "adjoint", "coo-ingest", "gather-ceiling"],
                     help="rsvd (default, the headline metric) or one of the other measured paths")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -68,7 +68,22 @@ def build(jobs=None, verbose=False):
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
     build_synth()
+    build_probe()
     return LIB
+
+
+PROBE_LIB = os.path.join(HERE, "libsvdh_probe.so")
+
+
+def build_probe():
+    """The SpMM gather-ceiling probe (measurement only, used by bench.py)."""
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [HIPCC, *COMMON, "-shared", "-x", "hip", os.path.join(CSRC, "probe_gather.hip"), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed (probe):\n" + r.stdout + r.stderr)
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
 
 
 SYNTH_LIB = os.path.join(HERE, "libsynth.so")

@@ -117,3 +117,38 @@ def test_batched_64_masks_equal_separate_solves(lib):
         print(f"mask {i} vs oracle: sigma rel err {err:.2e}, top-5 U angle {ang:.2e}", flush=True)
         assert err < 1e-5
         assert ang < 1e-3
+
+
+def test_rsvd_100k_x_20k_bf16_ladder_and_oracle(lib):
+    # the mixed-precision leg of the same workload: BF16 values and BF16
+    # input panels with FP32 accumulation until the residual estimate is
+    # 1e-2, FP32 after. Checked against the FP64 truth of the committed cfg5
+    # fixture (ARPACK + Rayleigh-Ritz on the CPU: the top 40 values), against
+    # the FP32 solve, and through its FP64 residual; plus the FP32 fixed-count
+    # solve against the FP64 oracle with the same injected Omega.
+    import oracle
+    from helpers import golden
+    from single_svdlib import synth
+    a = synth.config2(np.float32)
+    print(f"case rsvd bf16 100,000 x 20,000 nnz={a.nnz}", flush=True)
+    assert synth.checksum(a) == SHA["100k_x_20k"]
+    truth = np.asarray(golden("cfg5_grad.json")["sigma"])
+    op = lib.CsrMatrix.from_scipy(a, keep_bf16=True)
+    r16 = lib.randomized.randomized_svd(op, 50, 10, 3, "QR", False, 42, False, tol=1e-5, max_power_iters=200,
+                                        precision="bf16")
+    r32 = lib.randomized.randomized_svd(op, 50, 10, 3, "QR", False, 42, False, tol=1e-5, max_power_iters=200)
+    res16 = triplet_residual(a, r16.u, r16.s, r16.vt)
+    e16 = float(np.max(np.abs(r16.s[:40] - truth) / truth))
+    e32 = float(np.max(np.abs(r32.s[:40] - truth) / truth))
+    print(f"bf16: {r16.extra['power_iters_run']} iterations, residual {res16:.2e}, sigma vs FP64 truth {e16:.2e}; "
+          f"f32: {r32.extra['power_iters_run']} iterations, sigma vs truth {e32:.2e}", flush=True)
+    assert r16.extra["residual_estimate"] <= 1e-5 and res16 <= 1.05e-5
+    assert e16 < 1e-6 and e32 < 1e-6
+    assert np.allclose(r16.s, r32.s, rtol=1e-6, atol=0)
+    # fixed count (q = 3, the workload as specified) with an injected Omega
+    om = np.random.default_rng(5).normal(size=(20_000, 60))
+    q3 = lib.randomized.randomized_svd(op, 50, 10, 3, "QR", False, 42, False, omega=om)
+    u, s, vt = oracle.randomized_svd(oracle.OracleOp(a.astype(np.float64), nthreads=16), 50, 10, 3, "QR", omega=om)
+    e = float(np.max(np.abs(q3.s - s) / s))
+    print(f"q=3 f32 vs FP64 oracle (same Omega): sigma rel err {e:.2e}", flush=True)
+    assert e < 1e-5
