@@ -238,6 +238,27 @@ def bench_las2(args):
             "roofline": f32["roofline_spmv"], "cpu_baseline": cpu, "precisions": res}
 
 
+def _tuned_configs(nops):
+    """The autotuner's choices for the batched SpMM rounds of nops operators
+    (lanes-per-short-row shift, gathers in flight, workgroups per CU with 0 =
+    the static launch, items per claim), read back from its tuning file."""
+    path = os.environ.get("SVDH_TUNE_FILE") or os.path.join(os.path.expanduser("~"), ".cache", "svdh", "tune.txt")
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                key, cfg, ms = line.rstrip("\n").split("\t")
+                if f" ops={nops} " in key:
+                    d = key.split(" dir=")[1].split()[0]
+                    dl, uu, wpc, claim = (int(x) for x in cfg.split())
+                    out[f"dir{d}"] = {"lanes_shift": dl, "gathers_in_flight": uu, "workgroups_per_cu": wpc,
+                                      "claim": claim, "launch": "static" if wpc == 0 else "persistent",
+                                      "tuning_ms": float(ms)}
+    except (OSError, ValueError):
+        pass
+    return out
+
+
 def bench_batched(args):
     """64 randomized SVDs (rank 20, oversample 10, q = 3) over column masks of
     the 200,000 x 25,000 matrix, every SpMM round in one launch, U and V^T left
@@ -272,13 +293,34 @@ def bench_batched(args):
             lib.svdh_result_free(C.byref(metas[b]))
         return s0
 
+    # the first call tunes the persistent SpMM launch (unless the tuning file
+    # already holds this workload): timed on its own
+    t0 = time.perf_counter()
+    batched()
+    t_tune = time.perf_counter() - t0
     for _ in range(args.warmup):
         batched()
     _ffi.check(lib.svdh_ctx_synchronize(ctx))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        svals = batched()
-    t_batch = (time.perf_counter() - t0) / args.steps
+
+    def timed():
+        ts = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            sv = batched()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), sv
+
+    t_batch, svals = timed()
+    variants = {}
+    for name, env in (("persistent_default_config", {"SVDH_BATCH_TUNE": "0"}),
+                      ("static_launch", {"SVDH_BATCH_PERSISTENT": "0"})):
+        os.environ.update(env)
+        try:
+            batched()
+            variants[name], _ = timed()
+        finally:
+            for key in env:
+                del os.environ[key]
     _ffi.check(lib.svdh_ctx_profile(ctx, 1))
     batched()
     st = _kernel_stats(lib, ctx, (("spmm", _ffi.KC_SPMM), ("spmm_t", _ffi.KC_SPMM_T), ("dense", _ffi.KC_DENSE)), 1)
@@ -313,7 +355,10 @@ def bench_batched(args):
     for i in (0, 1):
         sub = a[:, masks[i]].tocsr()
         check.append(_triplet_residual(sub, recs[i].u, recs[i].s.astype(np.float64), recs[i].vt))
-    log(f"batched {t_batch * 1e3:.1f} ms vs one by one {t_loop * 1e3:.1f} ms; host outputs {t_host * 1e3:.1f} ms")
+    log(f"batched {t_batch * 1e3:.1f} ms (tuned persistent; default config "
+        f"{variants['persistent_default_config'] * 1e3:.1f} ms, static launch {variants['static_launch'] * 1e3:.1f} ms, "
+        f"first call with tuning {t_tune * 1e3:.0f} ms) vs one by one {t_loop * 1e3:.1f} ms; host outputs "
+        f"{t_host * 1e3:.1f} ms")
     return {"metric": "batched masked randomized SVD seconds (64 masks)", "value": t_batch, "unit": "s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_batch * 1e3,
             "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -321,6 +366,7 @@ def bench_batched(args):
                                     "column masks (sizes 500..8,000) of the 200,000 x 25,000 Poisson(60) CSR, "
                                     "U and V^T left in HBM"), "masks": nb, "nnz": int(a.nnz)},
             "roofline": roof, "cpu_baseline": None, "one_by_one_s": t_loop, "speedup_vs_one_by_one": t_loop / t_batch,
+            "untuned_s": variants, "first_call_with_tuning_s": t_tune, "tuned_configs": _tuned_configs(nb),
             "host_output_s": t_host, "host_output_equals_device_output": same,
             "residual_q3_first_masks": check,
             "kernels_ms": {kname: v["ms_per_step"] for kname, v in st.items()}}
@@ -443,6 +489,52 @@ def bench_ingest(args):
             "cpu_baseline": {"value": cpu_s, "unit": "s", "cores": 1, "kind": "port",
                              "sample": "the oracle's COO -> CSR (two counting sorts, run sums) on the same triplets"},
             "whole_call_s": total}
+
+
+def _probe_lib():
+    from single_svdlib._ffi import PKG_DIR
+    lib = C.CDLL(os.path.join(PKG_DIR, "libsvdh_probe.so"))
+    lib.svdh_probe_gather.restype = C.c_int32
+    lib.svdh_probe_gather.argtypes = [C.c_int32, C.POINTER(C.c_int32), C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                      C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.c_int32,
+                                      C.POINTER(C.c_double)]
+    return lib
+
+
+def gather_ceilings(a, h=256):
+    """Gather ceilings (ms) of the two SpMM directions of the headline
+    workload, FP32 rows of 60 values: A X's unstaged column stream (the h most
+    popular columns come from LDS) and A^T Y's unstaged row stream grouped by
+    8 source blocks (as its segment schedule runs), best of U in (4, 8) at 16
+    waves per CU (the configurations the full sweep of --workload
+    gather-ceiling selects)."""
+    lib = _probe_lib()
+    m, n = a.shape
+    counts = np.bincount(a.indices, minlength=n)
+    is_hot = np.zeros(n, dtype=bool)
+    is_hot[np.argsort(-counts, kind="stable")[:h]] = True
+    ax = np.ascontiguousarray(a.indices[~is_hot[a.indices]].astype(np.int32))
+    csc = a.tocsc()
+    keep = np.repeat(~is_hot, np.diff(csc.indptr))
+    aty = csc.indices[keep].astype(np.int32)
+    bw = (m + 7) // 8
+    blk = aty // bw
+    aty = np.ascontiguousarray(aty[np.argsort(blk, kind="stable")])
+    off = np.zeros(9, dtype=np.int64)
+    np.cumsum(np.bincount(blk, minlength=8), out=off[1:])
+    out = {}
+    for name, st, nsrc, o in (("spmm_ax", ax, n, None), ("spmm_aty", aty, m, off)):
+        best = None
+        for u in (4, 8):
+            ms = C.c_double()
+            rc = lib.svdh_probe_gather(0, st.ctypes.data_as(C.POINTER(C.c_int32)), st.size, nsrc, 256, 240, u, 256 * 16,
+                                       5, o.ctypes.data_as(C.POINTER(C.c_int64)) if o is not None else None,
+                                       8 if o is not None else 1, C.byref(ms))
+            if rc != 0:
+                raise RuntimeError(f"probe failed with {rc}")
+            best = ms.value if best is None else min(best, ms.value)
+        out[name] = {"ceiling_ms": best, "entries": int(st.size), "bytes": int(st.size) * 240}
+    return out
 
 
 def bench_gather_ceiling(args):
@@ -655,6 +747,18 @@ def main():
         for key in ("_ms", "_n", "_bytes"):
             v.pop(key)
     _ffi.check(lib.svdh_ctx_profile(ctx, 0))
+    if not args.profile_run:
+        # the measured ceiling of each direction's gathers (libsvdh_probe.so)
+        try:
+            ceil = gather_ceilings(a)
+            for name, c in ceil.items():
+                per_dir[name]["gather_ceiling_ms"] = c["ceiling_ms"]
+                per_dir[name]["gather_ceiling_tb_s"] = c["bytes"] / (c["ceiling_ms"] * 1e-3) / 1e12
+                per_dir[name]["frac_of_gather_ceiling"] = c["ceiling_ms"] / per_dir[name]["avg_launch_ms"]
+            log("gather ceilings: " + ", ".join(f"{k} {v['gather_ceiling_ms'] * 1e3:.1f} us "
+                                                f"({v['frac_of_gather_ceiling']:.2f} of it)" for k, v in per_dir.items()))
+        except Exception as e:
+            log(f"gather ceiling probe failed: {e!r}")
 
     if args.profile_run:
         # plain copy of known size: 256 MiB read + 256 MiB written

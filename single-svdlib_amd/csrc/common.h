@@ -93,6 +93,8 @@ struct Ctx {
   DevBuf d_ring;
   hipEvent_t ring_ev[kRing] = {};
   int ring_next = 0;
+  // Queue counters of the persistent batched SpMM (8 per launch).
+  DevBuf pq_counters;
   // Pinned landing area for small device-to-host results (Gram, norms).
   double *h_small = nullptr;
   double *d_small = nullptr;  // device address of h_small (kernels write results there)
@@ -250,6 +252,9 @@ struct svdh_op {
   // A^T panels with the staged columns on the matrix cores: the schedule of
   // the other output rows (thot_state -1: not decided, 0: not used, 1: used)
   svdh::Schedule sched_thot;
+  // Row-length histogram per direction (log2 buckets of the output rows'
+  // stored entries), built on first use: part of the batched SpMM tuning key.
+  std::vector<int64_t> row_hist[2];
   int thot_state = -1;
   int thot_g = 0;
   // pattern = true: the base matrix's own pattern (with the column remap),

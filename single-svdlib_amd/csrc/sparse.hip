@@ -19,7 +19,13 @@
 #include <numeric>
 #include <type_traits>
 
+#include <sys/stat.h>
+
+#include <fstream>
+#include <map>
 #include <mutex>
+#include <sstream>
+#include <string>
 
 #include "common.h"
 
@@ -572,8 +578,15 @@ struct SpmmArgs {
   int32_t blk_cnt[8];
 };
 
+// One fused multiply-add per entry, written out: every entry is added the
+// same way whether it falls in a full batch or in the last one, so the
+// batch size (gathers in flight) and the lanes per row leave the sums
+// unchanged.
+__device__ __forceinline__ float fma_acc(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_acc(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // acc += sum over nnz k in [kb, ke) of A_k * X[col_k, cols of this lane]
-template <int R, bool MC, typename V, typename X, typename Y, typename Acc>
+template <int R, bool MC, int UU = SVDH_SPMM_UNROLL, typename V, typename X, typename Y, typename Acc>
 __device__ __forceinline__ void spmm_accum(const SpmmArgs<V, X, Y, Acc> &a, int kb, int ke, int lane, int col,
                                            bool colok, Acc (&acc)[PanelVec<X>::N]) {
   constexpr int NV = PanelVec<X>::N;
@@ -593,7 +606,7 @@ __device__ __forceinline__ void spmm_accum(const SpmmArgs<V, X, Y, Acc> &a, int 
       }
     }
     const int n = min(R, ke - base);
-    constexpr int U = SVDH_SPMM_UNROLL < R ? SVDH_SPMM_UNROLL : R;
+    constexpr int U = UU < R ? UU : R;
     int j = 0;
     for (; j + U <= n; j += U) {
       int cj[U];
@@ -610,12 +623,12 @@ __device__ __forceinline__ void spmm_accum(const SpmmArgs<V, X, Y, Acc> &a, int 
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
-          for (int i = 0; i < NV; ++i) acc[i] += vj[u] * xv[u][i];
+          for (int i = 0; i < NV; ++i) acc[i] = fma_acc(vj[u], xv[u][i], acc[i]);
       }
     }
     if (j < n) {
       // the last n - j (< U) entries as one batch: lanes past the end hold
-      // column 0 and value 0, and their products are dropped by a select
+      // column 0 and value 0, so their fused products add exactly nothing
       int cj[U];
       Acc vj[U];
 #pragma unroll
@@ -630,13 +643,13 @@ __device__ __forceinline__ void spmm_accum(const SpmmArgs<V, X, Y, Acc> &a, int 
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
-          for (int i = 0; i < NV; ++i) acc[i] += j + u < n ? vj[u] * xv[u][i] : Acc(0);
+          for (int i = 0; i < NV; ++i) acc[i] = fma_acc(vj[u], xv[u][i], acc[i]);
       }
     }
   }
 }
 
-template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+template <int LGR, bool MC, int UU = SVDH_SPMM_UNROLL, typename V, typename X, typename Y, typename Acc>
 __device__ __forceinline__ void spmm_short(const SpmmArgs<V, X, Y, Acc> &a, int b) {
   constexpr int R = 1 << LGR;
   constexpr int NV = PanelVec<X>::N;
@@ -654,7 +667,7 @@ __device__ __forceinline__ void spmm_short(const SpmmArgs<V, X, Y, Acc> &a, int 
     Acc acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
-    spmm_accum<R, MC>(a, kb, ke, lane, col, colok, acc);
+    spmm_accum<R, MC, UU>(a, kb, ke, lane, col, colok, acc);
     if (colok) st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
   }
 }
@@ -662,7 +675,7 @@ __device__ __forceinline__ void spmm_short(const SpmmArgs<V, X, Y, Acc> &a, int 
 // Segments: the part of a short row that falls in one source block, one row
 // group each; workgroup w takes segments of block w % nblk only. A row with
 // several segments writes one partial per segment (summed by k_spmm_fin).
-template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+template <int LGR, bool MC, int UU = SVDH_SPMM_UNROLL, typename V, typename X, typename Y, typename Acc>
 __device__ __forceinline__ void spmm_segment(const SpmmArgs<V, X, Y, Acc> &a, int w) {
   constexpr int R = 1 << LGR;
   constexpr int NV = PanelVec<X>::N;
@@ -687,7 +700,7 @@ __device__ __forceinline__ void spmm_segment(const SpmmArgs<V, X, Y, Acc> &a, in
     Acc acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
-    spmm_accum<R, MC>(a, kb0, ke, lane, col, colok, acc);
+    spmm_accum<R, MC, UU>(a, kb0, ke, lane, col, colok, acc);
     if (colok) {
       if (slot < 0)
         st_panel<Y, Acc, NV>(a.y + (int64_t)r * a.ldy + col, acc);
@@ -697,7 +710,7 @@ __device__ __forceinline__ void spmm_segment(const SpmmArgs<V, X, Y, Acc> &a, in
   }
 }
 
-template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
+template <int LGR, bool MC, int UU = SVDH_SPMM_UNROLL, typename V, typename X, typename Y, typename Acc>
 __device__ __forceinline__ void spmm_chunk(const SpmmArgs<V, X, Y, Acc> &a, int ci) {
   constexpr int R = 1 << LGR;
   constexpr int NV = PanelVec<X>::N;
@@ -716,7 +729,7 @@ __device__ __forceinline__ void spmm_chunk(const SpmmArgs<V, X, Y, Acc> &a, int 
     Acc acc[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) acc[i] = Acc(0);
-    spmm_accum<R, MC>(a, gb, ge, lane, col, colok, acc);
+    spmm_accum<R, MC, UU>(a, gb, ge, lane, col, colok, acc);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < NV; ++i) red[gid * (R * NV) + lane * NV + i] = acc[i];
@@ -2272,9 +2285,368 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   if (sp->nlong_rows > 0) launch_fin<Y, Acc>(c, *sp, sp->nlong_rows, p.a.ldp, static_cast<Y *>(Y_), ldy, (int)l);
 }
 
+// ------------------------------------------------------------------------
+// Persistent batched SpMM: one launch of a fixed grid of workgroups that take
+// their work from eight queues, one per XCD (workgroup w serves queue w mod 8,
+// the XCD round-robin dispatch puts it on), until the queues run dry. A work
+// item is a row group of one operator's schedule: a chunk of a long row, a
+// group of segments (rows cut at source blocks) or a group of short rows.
+// Chunks and segments of source block x go to queue x, so each XCD still
+// gathers from one block of the panel; the other items are dealt round robin.
+// The queues list the operators by decreasing nonzero count (large jobs
+// first). Each item computes its rows exactly as the per-operator launch does
+// (one lane sums a row's entries for its columns in storage order), so the
+// grid, the queue order, the gathers in flight (UU) and the lanes per short
+// row (LGRS) change the time, not the result; chunks keep the lane layout
+// LGRC of the panel width, which fixes their split of a row.
+// ------------------------------------------------------------------------
+struct PqRange {
+  int32_t b, kind, first, stride;  // kind 0: chunk index, 1: segment workgroup, 2: short-row group
+  int32_t count, start, pad0, pad1;  // items first + k * stride, k < count; queue positions start ..
+};
+
+// Two ways of taking items: claim = 0, workgroup j of the queue's W takes
+// items j, j + W, j + 2W, ... (no synchronisation; the queue order spreads
+// large and small items over the workgroups); claim > 0, a workgroup claims
+// `claim` consecutive items at a time from the queue's counter, the next
+// claim issued before the current items are processed.
+// The queue's ranges are copied into LDS first (kPqMaxRanges per queue at
+// most), and a workgroup walks them forward, since the items it takes come
+// in increasing order.
+constexpr int kPqMaxRanges = 384;
+
+template <int LGRC, int LGRS, int UU, bool MC, typename V, typename X, typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm_pq(const SpmmArgs<V, X, Y, Acc> *args, const PqRange *ranges,
+                                                   const int32_t *qoff, const int32_t *qlen, int32_t *counters,
+                                                   int claim) {
+  __shared__ int s_it;
+  __shared__ int4 s_rg[kPqMaxRanges];  // (b, kind, first, stride) of each range
+  __shared__ int s_end[kPqMaxRanges];  // start + count
+  const int x = blockIdx.x & 7;
+  const int r0 = qoff[x], nr = qoff[x + 1] - r0;
+  const int total = qlen[x];
+  for (int i = threadIdx.x; i < nr; i += kBlock) {
+    const PqRange g = ranges[r0 + i];
+    s_rg[i] = make_int4(g.b, g.kind, g.first, g.stride);
+    s_end[i] = g.start + g.count;
+  }
+  __syncthreads();
+  int cur = 0, cur_start = 0;  // the range holding the current item, and its first position
+  const int nwg = (int)(gridDim.x >> 3), jw = (int)(blockIdx.x >> 3);
+  int32_t *ctr = counters + 64 * x;  // one counter per 256-byte line
+  int it0, step, span;
+  if (claim > 0) {
+    if (threadIdx.x == 0) s_it = atomicAdd(ctr, claim);
+    __syncthreads();
+    it0 = s_it;
+    __syncthreads();
+    span = claim;
+    step = 0;
+  } else {
+    it0 = jw;
+    span = 1;
+    step = nwg;
+  }
+  for (;;) {
+    if (it0 >= total) break;
+    int nxt = 0;
+    if (claim > 0 && threadIdx.x == 0) nxt = atomicAdd(ctr, claim);  // in flight during the items
+    const int e = min(total, it0 + span);
+    for (int it = it0; it < e; ++it) {
+      while (s_end[cur] <= it) cur_start = s_end[cur++];  // items only increase
+      const int4 rg = s_rg[cur];
+      const int idx = rg.z + (it - cur_start) * rg.w;
+      const SpmmArgs<V, X, Y, Acc> &a = args[rg.x];
+      if (rg.y == 0)
+        spmm_chunk<LGRC, MC, UU>(a, idx);
+      else if (rg.y == 1)
+        spmm_segment<LGRS, MC, UU>(a, idx);
+      else
+        spmm_short<LGRS, MC, UU>(a, idx);
+    }
+    if (claim > 0) {
+      __syncthreads();  // the previous value of s_it has been read by every thread
+      if (threadIdx.x == 0) s_it = nxt;
+      __syncthreads();
+      it0 = s_it;
+    } else {
+      it0 += step;
+    }
+  }
+}
+
+__global__ void k_zero_counters(int32_t *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 template <int LGR, bool MC, typename V, typename X, typename Y, typename Acc>
 static void spmm_batched_one(Ctx *c, const SpmmArgs<V, X, Y, Acc> *args, const int32_t *boff, int nb, int blocks) {
   hipLaunchKernelGGL((k_spmm_batched<LGR, MC, V, X, Y, Acc>), blocks, kBlock, 0, c->stream, args, boff, nb);
+}
+
+template <typename V, typename X, typename Y, typename Acc>
+static void launch_spmm_static(Ctx *c, int direction, bool mc, int lgr, const std::vector<SpmmArgs<V, X, Y, Acc>> &args,
+                               const std::vector<int32_t> &boff, double bytes);
+
+// ---- autotuner of the persistent batched SpMM ----------------------------
+// Candidates: lanes per short row (the panel width's layout, or half of it:
+// twice the rows per wavefront in two column passes), gathers in flight per
+// row group (4 or 8), workgroups per CU of the persistent grid (4, 8, 16) and
+// how workgroups take items (static interleave, or claims of 16 items).
+// The first batched SpMM of a workload runs every candidate once (each
+// computes the same result), keeps the fastest, and appends the choice to a
+// tuning file keyed by the direction, value type, panel width, operator count,
+// rows, nonzeros and row-length histogram (SVDH_TUNE_FILE, default
+// ~/.cache/svdh/tune.txt), so later processes start tuned. SVDH_BATCH_TUNE=0
+// uses the default candidate without tuning.
+struct PqConfig {
+  int dl;     // lanes per short row = 2^(LGRC - dl)
+  int uu;     // gathers in flight per row group
+  int wpc;    // workgroups (256 threads) per CU
+  int claim;  // 0: static interleave; > 0: items claimed per counter update
+};
+// wpc = 0 stands for the static launch (one workgroup per schedule block of
+// every operator, dispatched by the hardware), which competes with the
+// persistent candidates.
+constexpr PqConfig kPqDefault{0, 4, 8, 0};
+constexpr PqConfig kPqCandidates[] = {
+    {0, 4, 0, 0}, {0, 4, 8, 0}, {0, 8, 8, 0}, {1, 4, 8, 0}, {1, 8, 8, 0}, {0, 4, 4, 0}, {0, 8, 4, 0},
+    {1, 4, 4, 0}, {1, 8, 4, 0}, {0, 4, 16, 0}, {0, 8, 16, 0}, {1, 4, 16, 0}, {1, 8, 16, 0}, {0, 4, 8, 16},
+    {0, 8, 8, 16}, {1, 4, 8, 16}, {1, 8, 8, 16}, {0, 4, 16, 16}, {1, 8, 16, 16}};
+
+struct PqTuner {
+  std::mutex mu;
+  bool loaded = false;
+  std::map<std::string, PqConfig> best;
+  std::string path() const {
+    if (const char *e = std::getenv("SVDH_TUNE_FILE")) return e;
+    const char *h = std::getenv("HOME");
+    return std::string(h ? h : ".") + "/.cache/svdh/tune.txt";
+  }
+  void load() {
+    if (loaded) return;
+    loaded = true;
+    std::ifstream f(path());
+    std::string line;
+    while (std::getline(f, line)) {
+      const size_t t = line.find('\t');
+      if (t == std::string::npos) continue;
+      std::istringstream is(line.substr(t + 1));
+      PqConfig c{};
+      if (is >> c.dl >> c.uu >> c.wpc >> c.claim) best[line.substr(0, t)] = c;
+    }
+  }
+  void store(const std::string &key, const PqConfig &c, double ms) {
+    best[key] = c;
+    const std::string p = path();
+    const size_t slash = p.rfind('/');
+    if (slash != std::string::npos) {  // create the directory (and its parent) if missing
+      const std::string dir = p.substr(0, slash);
+      const size_t s2 = dir.rfind('/');
+      if (s2 != std::string::npos && s2 > 0) (void)::mkdir(dir.substr(0, s2).c_str(), 0755);
+      (void)::mkdir(dir.c_str(), 0755);
+    }
+    std::ofstream f(p, std::ios::app);
+    if (f) f << key << '\t' << c.dl << ' ' << c.uu << ' ' << c.wpc << ' ' << c.claim << '\t' << ms << '\n';
+  }
+};
+
+static PqTuner &pq_tuner() {
+  static PqTuner t;
+  return t;
+}
+
+// Row-length histogram of an operator direction (log2 buckets), cached.
+static const std::vector<int64_t> &row_histogram(svdh_op *op, int direction) {
+  std::vector<int64_t> &h = op->row_hist[direction];
+  if (!h.empty()) return h;
+  h.assign(16, 0);
+  const CsrView v = op->view(direction);
+  const std::vector<int32_t> &rp = *v.h_rowptr;
+  for (int64_t r = 0; r < v.nrows; ++r) {
+    const int64_t src = v.h_rowmap ? (*v.h_rowmap)[r] : r;
+    const int32_t len = rp[src + 1] - rp[src];
+    int bkt = 0;
+    while (bkt < 15 && (1 << bkt) <= len) ++bkt;
+    ++h[bkt];
+  }
+  return h;
+}
+
+// Work ranges of the eight queues for the short-row layout LGRS: per
+// operator (largest first) its chunks and segments by source block, its
+// short-row groups round robin.
+template <typename V, typename X, typename Y, typename Acc>
+static void pq_ranges(const std::vector<const Schedule *> &sched, const std::vector<int64_t> &weight, int lgrs,
+                      std::vector<PqRange> &ranges, int32_t qoff[9], int32_t qlen[8]) {
+  const int rpbs = kBlock >> lgrs;
+  std::vector<int> order(sched.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return weight[a] > weight[b]; });
+  std::vector<PqRange> q[8];
+  auto add = [&](int x, int b, int kind, int first, int stride, int count) {
+    if (count > 0) q[x].push_back(PqRange{b, kind, first, stride, count, 0, 0, 0});
+  };
+  for (int b : order) {
+    const Schedule &s = *sched[b];
+    const int nblk = std::max(1, s.nblk);
+    for (int x = 0; x < nblk; ++x) {  // chunks of source block x
+      if (nblk > 1) {
+        add(x % 8, b, 0, s.blk_off[x], 1, s.blk_cnt[x]);
+      } else {
+        for (int y = 0; y < 8; ++y)
+          add(y, b, 0, s.blk_off[x] + y, 8, s.blk_cnt[x] > y ? (s.blk_cnt[x] - y + 7) / 8 : 0);
+      }
+    }
+    for (int x = 0; x < nblk; ++x) {  // segment groups kb of block x: workgroup index kb * nblk + x
+      const int groups = (s.seg_cnt[x] + rpbs - 1) / rpbs;
+      if (nblk > 1)
+        add(x % 8, b, 1, x, nblk, groups);
+      else
+        for (int y = 0; y < 8; ++y) add(y, b, 1, y, 8, groups > y ? (groups - y + 7) / 8 : 0);
+    }
+    const int ngroups = (s.bins[0].row_count + rpbs - 1) / rpbs;  // short rows
+    for (int y = 0; y < 8; ++y) add(y, b, 2, y, 8, ngroups > y ? (ngroups - y + 7) / 8 : 0);
+  }
+  ranges.clear();
+  qoff[0] = 0;
+  for (int x = 0; x < 8; ++x) {
+    int32_t pos = 0;
+    for (PqRange r : q[x]) {
+      r.start = pos;
+      pos += r.count;
+      ranges.push_back(r);
+    }
+    qlen[x] = pos;
+    qoff[x + 1] = (int32_t)ranges.size();
+  }
+}
+
+template <int LGRC, int LGRS, int UU, bool MC, typename V, typename X, typename Y, typename Acc>
+static void pq_launch_one(Ctx *c, int grid, int claim, const SpmmArgs<V, X, Y, Acc> *args, const PqRange *rg,
+                          const int32_t *qoff, const int32_t *qlen) {
+  if (claim > 0)
+    hipLaunchKernelGGL(k_zero_counters, 1, 512, 0, c->stream, c->pq_counters.as<int32_t>(), 512);
+  hipLaunchKernelGGL((k_spmm_pq<LGRC, LGRS, UU, MC, V, X, Y, Acc>), grid, kBlock, 0, c->stream, args, rg, qoff, qlen,
+                     c->pq_counters.as<int32_t>(), claim);
+}
+
+template <int LGRC, bool MC, typename V, typename X, typename Y, typename Acc>
+static void pq_launch_lgr(Ctx *c, const PqConfig &cf, int grid, const SpmmArgs<V, X, Y, Acc> *args, const PqRange *rg,
+                          const int32_t *qoff, const int32_t *qlen) {
+  if (cf.dl == 0 && cf.uu == 4) pq_launch_one<LGRC, LGRC, 4, MC>(c, grid, cf.claim, args, rg, qoff, qlen);
+  else if (cf.dl == 0) pq_launch_one<LGRC, LGRC, 8, MC>(c, grid, cf.claim, args, rg, qoff, qlen);
+  else if (cf.uu == 4) pq_launch_one<LGRC, LGRC - 1, 4, MC>(c, grid, cf.claim, args, rg, qoff, qlen);
+  else pq_launch_one<LGRC, LGRC - 1, 8, MC>(c, grid, cf.claim, args, rg, qoff, qlen);
+}
+
+// The persistent launch of one group of plans (all with the chunk layout lgr,
+// which must be 3, 4 or 5); returns false when the layout is not covered.
+template <typename V, typename X, typename Y, typename Acc, typename StaticRun>
+static bool launch_spmm_pq(Ctx *c, int direction, bool mc, int lgr, int64_t l,
+                           const std::vector<SpmmArgs<V, X, Y, Acc>> &args, const std::vector<const Schedule *> &sched,
+                           const std::vector<svdh_op *> &gops, double bytes, StaticRun static_run) {
+  if (lgr < 3 || lgr > 5) return false;
+  if (const char *e = std::getenv("SVDH_BATCH_PERSISTENT"))
+    if (std::atoi(e) == 0) return false;
+  c->pq_counters.ensure(512 * sizeof(int32_t));
+  std::vector<int64_t> weight(gops.size());
+  int64_t rows = 0, nnz = 0;
+  std::vector<int64_t> hist(16, 0);
+  for (size_t b = 0; b < gops.size(); ++b) {
+    const CsrView v = gops[b]->view(direction);
+    weight[b] = v.nnz;
+    rows += v.nrows;
+    nnz += v.nnz;
+    const std::vector<int64_t> &h = row_histogram(gops[b], direction);
+    for (int i = 0; i < 16; ++i) hist[i] += h[i];
+  }
+  std::ostringstream key;
+  key << "spmm_batched dir=" << direction << " mc=" << (int)mc << " val=" << sizeof(V) << " l=" << l
+      << " ops=" << gops.size() << " rows=" << rows << " nnz=" << nnz << " hist=";
+  for (int i = 0; i < 16; ++i) key << (i ? "," : "") << (rows ? (hist[i] * 1000 + rows / 2) / rows : 0);
+  const std::string k = key.str();
+  const char *te = std::getenv("SVDH_BATCH_TUNE");
+  const bool tune = !(te && std::atoi(te) == 0);
+  PqTuner &tn = pq_tuner();
+  PqConfig cfg = kPqDefault;
+  bool known = false;
+  {
+    std::lock_guard<std::mutex> g(tn.mu);
+    if (tune) {
+      tn.load();
+      auto it = tn.best.find(k);
+      if (it != tn.best.end()) {
+        cfg = it->second;
+        known = true;
+      }
+    }
+  }
+  // Each run uploads its own arguments: the upload ring's slots are reused
+  // in turn, and a slot's contents are valid only for the kernels launched
+  // before the next upload into it.
+  auto run = [&](const PqConfig &cf) {
+    if (cf.wpc == 0) {
+      static_run();
+      return;
+    }
+    const auto *dargs =
+        static_cast<const SpmmArgs<V, X, Y, Acc> *>(c->upload(args.data(), args.size() * sizeof(args[0])));
+    std::vector<PqRange> ranges;
+    int32_t qo[9], ql[8];
+    pq_ranges<V, X, Y, Acc>(sched, weight, lgr - cf.dl, ranges, qo, ql);
+    std::vector<int32_t> tab(ranges.size() * (sizeof(PqRange) / 4) + 17);
+    std::memcpy(tab.data(), ranges.data(), ranges.size() * sizeof(PqRange));
+    int32_t *tq = tab.data() + ranges.size() * (sizeof(PqRange) / 4);
+    std::memcpy(tq, qo, sizeof qo);
+    std::memcpy(tq + 9, ql, sizeof ql);
+    if (tab.size() * 4 > Ctx::kRingSlot) fail(SVDH_ERR_ARG, "batched SpMM: too many work ranges");
+    for (int x = 0; x < 8; ++x)
+      if (qo[x + 1] - qo[x] > kPqMaxRanges) fail(SVDH_ERR_ARG, "batched SpMM: too many work ranges in one queue");
+    const int32_t *dt = static_cast<const int32_t *>(c->upload(tab.data(), tab.size() * 4));
+    const PqRange *drg = reinterpret_cast<const PqRange *>(dt);
+    const int32_t *dqo = dt + ranges.size() * (sizeof(PqRange) / 4);
+    const int grid = std::max(8, cf.wpc * c->ncu / 8 * 8);
+    switch (lgr) {
+      case 3: if (mc) pq_launch_lgr<3, true>(c, cf, grid, dargs, drg, dqo, dqo + 9);
+              else pq_launch_lgr<3, false>(c, cf, grid, dargs, drg, dqo, dqo + 9); break;
+      case 4: if (mc) pq_launch_lgr<4, true>(c, cf, grid, dargs, drg, dqo, dqo + 9);
+              else pq_launch_lgr<4, false>(c, cf, grid, dargs, drg, dqo, dqo + 9); break;
+      default: if (mc) pq_launch_lgr<5, true>(c, cf, grid, dargs, drg, dqo, dqo + 9);
+               else pq_launch_lgr<5, false>(c, cf, grid, dargs, drg, dqo, dqo + 9); break;
+    }
+    HIP_CHECK(hipGetLastError());
+  };
+  if (tune && !known) {
+    // every candidate once, timed with events; all compute the same result
+    constexpr int nc = (int)(sizeof(kPqCandidates) / sizeof(kPqCandidates[0]));
+    hipEvent_t ev[nc + 1];
+    for (int i = 0; i <= nc; ++i) HIP_CHECK(hipEventCreate(&ev[i]));
+    HIP_CHECK(hipEventRecord(ev[0], c->stream));
+    for (int i = 0; i < nc; ++i) {
+      run(kPqCandidates[i]);
+      HIP_CHECK(hipEventRecord(ev[i + 1], c->stream));
+    }
+    HIP_CHECK(hipEventSynchronize(ev[nc]));
+    int bi = 0;
+    float bt = 0.f;
+    for (int i = 0; i < nc; ++i) {
+      float t = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+      if (i == 0 || t < bt) {
+        bt = t;
+        bi = i;
+      }
+    }
+    for (int i = 0; i <= nc; ++i) (void)hipEventDestroy(ev[i]);
+    std::lock_guard<std::mutex> g(tn.mu);
+    tn.store(k, kPqCandidates[bi], bt);
+    return true;
+  }
+  if (cfg.wpc == 0) return false;  // the static launch (timed by the caller)
+  KTimer t(c, direction == 0 ? KC_SPMM : KC_SPMM_T, bytes);
+  run(cfg);
+  return true;
 }
 
 // One launch for the SpMMs of several operators of the same direction and
@@ -2293,6 +2665,8 @@ static void launch_spmm_batched(Ctx *c, int nops, svdh_op *const *ops, int direc
     std::vector<int32_t> boff(1, 0);
     std::vector<FinArgs<Y, Acc>> fargs;
     std::vector<int32_t> foff(1, 0);
+    std::vector<const Schedule *> sched;
+    std::vector<svdh_op *> gops;
     double bytes = 0.0;
     int lgr = 0;
     for (int b = 0; b < nops; ++b) {
@@ -2300,6 +2674,8 @@ static void launch_spmm_batched(Ctx *c, int nops, svdh_op *const *ops, int direc
       if (lgr && plans[b].lgr != lgr) fail(SVDH_ERR_ARG, "batched SpMM needs one lane layout");
       lgr = plans[b].lgr;
       args.push_back(plans[b].a);
+      sched.push_back(plans[b].s);
+      gops.push_back(ops[b]);
       boff.push_back(boff.back() + (plans[b].blocks + 7) / 8 * 8);
       bytes += plans[b].bytes;
       const Schedule &s = *plans[b].s;
@@ -2311,6 +2687,25 @@ static void launch_spmm_batched(Ctx *c, int nops, svdh_op *const *ops, int direc
       }
     }
     if (args.empty()) continue;
+    auto static_run = [&] { launch_spmm_static<V, X, Y, Acc>(c, direction, mc, lgr, args, boff, bytes); };
+    if (!launch_spmm_pq<V, X, Y, Acc>(c, direction, mc, lgr, l, args, sched, gops, bytes, static_run)) static_run();
+    if (!fargs.empty()) {
+      const auto *dfargs = static_cast<const FinArgs<Y, Acc> *>(c->upload(fargs.data(), fargs.size() * sizeof(fargs[0])));
+      const auto *dfoff = static_cast<const int32_t *>(c->upload(foff.data(), foff.size() * sizeof(int32_t)));
+      hipLaunchKernelGGL((k_spmm_fin_batched<Y, Acc>), foff.back(), kBlock, 0, c->stream, dfargs, dfoff,
+                         (int)fargs.size());
+      HIP_CHECK(hipGetLastError());
+    }
+  }
+}
+
+// The static launch of one group: one workgroup per schedule block of every
+// operator (each operator's blocks start at a multiple of 8, so block w of an
+// operator lands on XCD w mod 8 as in a launch of its own).
+template <typename V, typename X, typename Y, typename Acc>
+static void launch_spmm_static(Ctx *c, int direction, bool mc, int lgr, const std::vector<SpmmArgs<V, X, Y, Acc>> &args,
+                               const std::vector<int32_t> &boff, double bytes) {
+  {
     const int nb = (int)args.size();
     const auto *dargs = static_cast<const SpmmArgs<V, X, Y, Acc> *>(c->upload(args.data(), args.size() * sizeof(args[0])));
     const auto *dboff = static_cast<const int32_t *>(c->upload(boff.data(), boff.size() * sizeof(int32_t)));
@@ -2333,13 +2728,6 @@ static void launch_spmm_batched(Ctx *c, int nops, svdh_op *const *ops, int direc
     }
 #undef SVDH_BATCH_CASE
     HIP_CHECK(hipGetLastError());
-    if (!fargs.empty()) {
-      const auto *dfargs = static_cast<const FinArgs<Y, Acc> *>(c->upload(fargs.data(), fargs.size() * sizeof(fargs[0])));
-      const auto *dfoff = static_cast<const int32_t *>(c->upload(foff.data(), foff.size() * sizeof(int32_t)));
-      hipLaunchKernelGGL((k_spmm_fin_batched<Y, Acc>), foff.back(), kBlock, 0, c->stream, dfargs, dfoff,
-                         (int)fargs.size());
-      HIP_CHECK(hipGetLastError());
-    }
   }
 }
 

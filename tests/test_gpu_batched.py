@@ -138,3 +138,31 @@ def test_batched_device_outputs_equal_host_outputs(lib):
         np.testing.assert_array_equal(s, recs[b].s)
         np.testing.assert_array_equal(us[b].numpy()[:, :k], recs[b].u)
         np.testing.assert_array_equal(vts[b].numpy()[:, :len(cols[b])], recs[b].vt)
+
+
+def test_batched_persistent_autotuned_and_static_launches_agree(lib, tmp_path, monkeypatch):
+    # the persistent work-queue launch, its autotuner (every candidate timed
+    # on the first call of a workload, the choice stored in the tuning file)
+    # and the static one-workgroup-per-block launch compute every row the same
+    # way: all equal the separate solves bit for bit
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(4000, 1500, 0.02, seed=41).astype(np.float32)
+    base = lib.CsrMatrix.from_scipy(a)
+    cols = masks_of(1500, [300, 1400, 800, 50, 1100], seed=13)
+    views = [MaskedCSRMatrix.with_columns(base, c) for c in cols]
+    tune = tmp_path / "tune.txt"
+    monkeypatch.setenv("SVDH_TUNE_FILE", str(tune))
+    first = lib.randomized.randomized_svd_batched(views, 10, 10, 2, "QR", seed=5)
+    text = tune.read_text()
+    assert "spmm_batched dir=0" in text and "spmm_batched dir=1" in text, text
+    again = lib.randomized.randomized_svd_batched(views, 10, 10, 2, "QR", seed=5)
+    monkeypatch.setenv("SVDH_BATCH_TUNE", "0")
+    default = lib.randomized.randomized_svd_batched(views, 10, 10, 2, "QR", seed=5)
+    monkeypatch.setenv("SVDH_BATCH_PERSISTENT", "0")
+    static = lib.randomized.randomized_svd_batched(views, 10, 10, 2, "QR", seed=5)
+    for i, v in enumerate(views):
+        one = lib.randomized.randomized_svd(v, 10, 10, 2, "QR", False, 5, False)
+        for r in (first[i], again[i], default[i], static[i]):
+            np.testing.assert_array_equal(r.s, one.s)
+            np.testing.assert_array_equal(r.u, one.u)
+            np.testing.assert_array_equal(r.vt, one.vt)
