@@ -237,6 +237,18 @@ int32_t svdh_las2(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t 
                   double endl, double endr, double kappa, uint32_t random_seed,
                   const double *start, int32_t precision, svdh_result *out);
 
+/* legacy::svdLAS2 (src/legacy.rs:569-663; svd / svd_dim / svd_dim_seed,
+ * legacy.rs:498-525, are its callers' defaults): FP64 only, the legacy
+ * control flow of lanso / ritvec / imtqlb / imtql2 (legacy.rs:961-2014):
+ * f64::EPSILON tolerances without the sparsity adaptation, an unreduced
+ * block ends where |beta| < eps, at most 30 QL iterations per eigenvalue
+ * (then SVDH_ERR_IMTQLB / SVDH_ERR_IMTQL2), significant Ritz values
+ * bnd <= kappa |ritz|. out->u is d x nrows (SvdRec.ut, legacy.rs:454-460).
+ * Arguments as svdh_las2. */
+int32_t svdh_las2_legacy(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t iterations,
+                         double endl, double endr, double kappa, uint32_t random_seed,
+                         const double *start, svdh_result *out);
+
 /* svdh_las2 with the singular vectors left in device memory (the measured
  * path; the host hand-over of U, d x nrows FP64, is PCIe-bound for tall
  * matrices). u: device, room for min(dimensions, nrows, ncols) rows of

@@ -13,6 +13,10 @@ vector, a closed form or a library routine chosen so a plausible mistake fails):
   imtqlb, imtql2  pinned: scipy.linalg.eigh_tridiagonal (LAPACK)
   las2        pinned: the reference's KATs (legacy.rs:137-283, 2184-2235),
               numpy.linalg.svd on random sparse matrices, and invariants
+  las2_legacy, imtqlb_legacy, imtql2_legacy  pinned: the same KATs, which
+              the reference runs on legacy::svdLAS2 itself; numpy.linalg.svd;
+              scipy.linalg.eigh_tridiagonal; the 30-iteration error forced by
+              a tridiagonal that needs more
   start_vector pinned by its own closed form (the reference's ChaCha12
               stream is parity unpinned: no Rust toolchain here)
   randomized_svd (oracle/rsvd.py) pinned: numpy.linalg.svd truth for
@@ -38,8 +42,11 @@ vector, a closed form or a library routine chosen so a plausible mistake fails):
 from .pyoracle import (  # noqa: F401
     OracleOp,
     las2,
+    las2_legacy,
     imtqlb,
     imtql2,
+    imtqlb_legacy,
+    imtql2_legacy,
     pythag,
     start_vector,
     determine_chunk_size,

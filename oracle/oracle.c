@@ -609,6 +609,7 @@ void orc_start_vector(uint32_t seed, int64_t n, double *out) {
 typedef struct {
   const orc_op *A;
   double eps;
+  int legacy; /* the f64 legacy API's control flow (legacy.rs:569-663, 1924-2014) */
   int64_t nrows, ncols; /* working (possibly transposed) shape */
   int transposed;
   double *w0, *w1, *w2, *w3, *w4, *w5;
@@ -829,6 +830,168 @@ static int lanczos_step(las2_ws *w, int64_t first, int64_t last, int64_t *ll, in
   return ORC_OK;
 }
 
+/* legacy.rs:961-1067: EISPACK imtqlb with the absolute convergence test
+ * compare(test, test + |e[m]|) (|b - a| < eps, legacy.rs:809-812), at most
+ * 30 QL iterations per eigenvalue (then ImtqlbError), underflow when
+ * compare(r, 0). */
+int orc_imtqlb_legacy(int64_t n, double *d, double *e, double *bnd, double eps) {
+  if (n == 1) return ORC_OK;
+  bnd[0] = 1.0;
+  const int64_t last = n - 1;
+  for (int64_t i = 1; i <= last; ++i) {
+    bnd[i] = 0.0;
+    e[i - 1] = e[i];
+  }
+  e[last] = 0.0;
+  for (int64_t l = 0; l <= last; ++l) {
+    for (int iteration = 0; iteration <= 30;) {
+      int64_t m = l;
+      while (m < last) {
+        double test = fabs(d[m]) + fabs(d[m + 1]);
+        if (cmp_eps(test, test + fabs(e[m]), eps)) break;
+        ++m;
+      }
+      double p = d[l], f = bnd[l];
+      if (m == l) {
+        int64_t i = l;
+        while (i >= 1 && p < d[i - 1]) {
+          d[i] = d[i - 1];
+          bnd[i] = bnd[i - 1];
+          --i;
+        }
+        d[i] = p;
+        bnd[i] = f;
+        break;
+      }
+      if (iteration == 30) {
+        set_err("imtqlb no convergence to an eigenvalue after 30 iterations");
+        return ORC_ERR_IMTQLB;
+      }
+      ++iteration;
+      double g = (d[l + 1] - p) / (2.0 * e[l]);
+      double r = orc_pythag(g, 1.0, eps);
+      g = d[m] - p + e[l] / (g + fsign(r, g));
+      double sn = 1.0, cs = 1.0;
+      p = 0.0;
+      int64_t i = m - 1;
+      int underflow = 0;
+      for (;;) {
+        f = sn * e[i];
+        double b = cs * e[i];
+        r = orc_pythag(f, g, eps);
+        e[i + 1] = r;
+        if (cmp_eps(r, 0.0, eps)) {
+          underflow = 1;
+          break;
+        }
+        sn = f / r;
+        cs = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * sn + 2.0 * cs * b;
+        p = sn * r;
+        d[i + 1] = g + p;
+        g = cs * r - b;
+        f = bnd[i + 1];
+        bnd[i + 1] = sn * bnd[i] + cs * f;
+        bnd[i] = cs * bnd[i] - sn * f;
+        if (i == l) break;
+        --i;
+      }
+      if (underflow) {
+        d[i + 1] -= p;
+      } else {
+        d[l] -= p;
+        e[l] = g;
+      }
+      e[m] = 0.0;
+    }
+  }
+  return ORC_OK;
+}
+
+/* legacy.rs:1575-1692: imtql2 with the same convergence and underflow tests,
+ * 30 iterations per eigenvalue (then Imtql2Error), eigenvectors in the
+ * columns of z (row-major nm x n: rotations mix columns i and i + 1 of every
+ * row), then a selection sort by ascending eigenvalue with column swaps. */
+int orc_imtql2_legacy(int64_t nm, int64_t n, double *d, double *e, double *z, double eps) {
+  if (n == 1) return ORC_OK;
+  const int64_t last = n - 1, nnm = n * nm;
+  for (int64_t i = 1; i < n; ++i) e[i - 1] = e[i];
+  e[last] = 0.0;
+  for (int64_t l = 0; l < n; ++l) {
+    for (int iteration = 0; iteration <= 30;) {
+      int64_t m = l;
+      while (m < last) {
+        double test = fabs(d[m]) + fabs(d[m + 1]);
+        if (cmp_eps(test, test + fabs(e[m]), eps)) break;
+        ++m;
+      }
+      if (m == l) break;
+      if (iteration == 30) {
+        set_err("imtql2 no convergence to an eigenvalue after 30 iterations");
+        return ORC_ERR_IMTQL2;
+      }
+      ++iteration;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = orc_pythag(g, 1.0, eps);
+      g = d[m] - d[l] + e[l] / (g + fsign(r, g));
+      double sn = 1.0, cs = 1.0, p = 0.0;
+      int64_t i = m - 1;
+      int underflow = 0;
+      for (;;) {
+        double f = sn * e[i];
+        double b = cs * e[i];
+        r = orc_pythag(f, g, eps);
+        e[i + 1] = r;
+        if (cmp_eps(r, 0.0, eps)) {
+          underflow = 1;
+          break;
+        }
+        sn = f / r;
+        cs = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * sn + 2.0 * cs * b;
+        p = sn * r;
+        d[i + 1] = g + p;
+        g = cs * r - b;
+        for (int64_t k = 0; k < nnm; k += n) {
+          f = z[k + i + 1];
+          z[k + i + 1] = sn * z[k + i] + cs * f;
+          z[k + i] = cs * z[k + i] - sn * f;
+        }
+        if (i == l) break;
+        --i;
+      }
+      if (underflow) {
+        d[i + 1] -= p;
+      } else {
+        d[l] -= p;
+        e[l] = g;
+      }
+      e[m] = 0.0;
+    }
+  }
+  for (int64_t l = 1; l < n; ++l) {
+    int64_t i = l - 1, k = i;
+    double p = d[i];
+    for (int64_t j = l; j < n; ++j)
+      if (d[j] < p) {
+        k = j;
+        p = d[j];
+      }
+    if (k != i) {
+      d[k] = d[i];
+      d[i] = p;
+      for (int64_t j = 0; j < nnm; j += n) {
+        double t = z[j + i];
+        z[j + i] = z[j + k];
+        z[j + k] = t;
+      }
+    }
+  }
+  return ORC_OK;
+}
+
 /* mod.rs:297-307 */
 static void insert_sort(int64_t n, double *a1, double *a2) {
   for (int64_t i = 1; i < n; ++i)
@@ -892,7 +1055,9 @@ static int lanso(las2_ws *w, int64_t dim, int64_t iterations, double endl, doubl
   const int64_t max_imtqlb =
       sparsity > 0.999 ? 500 : sparsity > 0.99 ? 300 : sparsity > 0.9 ? 100 : 50;
   const double eps = w->eps;
-  const double adaptive_eps = sparsity > 0.99 ? eps * 100.0 : sparsity > 0.9 ? eps * 10.0 : eps;
+  /* legacy.rs:1924-2014: plain eps, splits at compare(bet, 0), no extra steps */
+  const double adaptive_eps =
+      w->legacy ? eps : sparsity > 0.99 ? eps * 100.0 : sparsity > 0.9 ? eps * 10.0 : eps;
   double rnm, tol;
   int st = stpone(w, seed, &rnm, &tol);
   if (st) return st;
@@ -916,14 +1081,15 @@ static int lanso(las2_ws *w, int64_t dim, int64_t iterations, double endl, doubl
       if (l > j) break;
       int64_t i = l;
       while (i <= j) {
-        if (fabs(w->bet[i + 1]) <= adaptive_eps) break;
+        if (w->legacy ? cmp_eps(w->bet[i + 1], 0.0, eps) : fabs(w->bet[i + 1]) <= adaptive_eps) break;
         ++i;
       }
       if (i > j) i = j;
       int64_t sz = i - l;
       dcopy_rev(sz + 1, l, w->alf, w->ritz);
       dcopy_rev(sz, l + 1, w->bet, w->w5);
-      st = orc_imtqlb(sz + 1, w->ritz + l, w->w5 + l, w->bnd + l, max_imtqlb, eps);
+      st = w->legacy ? orc_imtqlb_legacy(sz + 1, w->ritz + l, w->w5 + l, w->bnd + l, eps)
+                     : orc_imtqlb(sz + 1, w->ritz + l, w->w5 + l, w->bnd + l, max_imtqlb, eps);
       if (st) return st;
       for (int64_t m = l; m <= i; ++m) w->bnd[m] = rnm * fabs(w->bnd[m]);
       l = i + 1;
@@ -939,7 +1105,7 @@ static int lanso(las2_ws *w, int64_t dim, int64_t iterations, double endl, doubl
         last = first + 9;
         intro = first;
       } else {
-        int64_t extra = sparsity > 0.99 ? 5 : 0;
+        int64_t extra = !w->legacy && sparsity > 0.99 ? 5 : 0;
         int64_t q = 1 + ((j - intro) * (dim - *neig)) / *neig;
         last = first + (q > 3 ? q : 3) + extra;
       }
@@ -960,7 +1126,8 @@ static int ritvec(las2_ws *w, int64_t dimensions, double kappa, int64_t steps, i
   const int64_t js = steps + 1, jsq = js * js;
   const double eps = w->eps;
   const double sparsity = sparsity_of(w->A);
-  const double adaptive_eps = sparsity > 0.99 ? eps * 100.0 : sparsity > 0.9 ? eps * 10.0 : eps;
+  const double adaptive_eps =
+      w->legacy ? eps : sparsity > 0.99 ? eps * 100.0 : sparsity > 0.9 ? eps * 10.0 : eps;
   const int64_t max_imtql2 =
       sparsity > 0.999 ? 500 : sparsity > 0.99 ? 300 : sparsity > 0.9 ? 200 : 50;
   double *s = (double *)calloc((size_t)jsq, sizeof(double));
@@ -973,7 +1140,8 @@ static int ritvec(las2_ws *w, int64_t dimensions, double kappa, int64_t steps, i
   for (int64_t i = 0; i < jsq; i += js + 1) s[i] = 1.0;
   dcopy_rev(js, 0, w->alf, ev);
   dcopy_rev(steps, 1, w->bet, w->w5);
-  int st = orc_imtql2(js, js, ev, w->w5, s, max_imtql2, eps);
+  int st = w->legacy ? orc_imtql2_legacy(js, js, ev, w->w5, s, eps)
+                     : orc_imtql2(js, js, ev, w->w5, s, max_imtql2, eps);
   if (st) {
     free(s);
     free(ev);
@@ -981,11 +1149,12 @@ static int ritvec(las2_ws *w, int64_t dimensions, double kappa, int64_t steps, i
   }
   double max_ev = 0.0;
   for (int64_t i = 0; i < js; ++i) max_ev = fmax(max_ev, fabs(ev[i]));
-  const double akappa = sparsity > 0.99 ? kappa * 10.0 : kappa;
+  const double akappa = !w->legacy && sparsity > 0.99 ? kappa * 10.0 : kappa;
   int64_t *sig = (int64_t *)malloc((size_t)js * sizeof(int64_t));
   int64_t nsig = 0;
   for (int64_t k = 0; k < js; ++k) {
-    double rel = akappa * fmax(fabs(w->ritz[k]), max_ev * adaptive_eps);
+    /* legacy.rs:1798: bnd[k] <= kappa |ritz[k]| */
+    double rel = w->legacy ? kappa * fabs(w->ritz[k]) : akappa * fmax(fabs(w->ritz[k]), max_ev * adaptive_eps);
     if (w->bnd[k] <= rel && k + 1 > js - neig) sig[nsig++] = k;
   }
   const int64_t d = dimensions < nsig ? dimensions : nsig;
@@ -1000,18 +1169,18 @@ static int ritvec(las2_ws *w, int64_t dimensions, double kappa, int64_t steps, i
     double *v = vt + r * nc;
     for (int64_t i = 0; i < js; ++i) {
       double coef = s[(js - 1 - i) * js + k]; /* D2 */
-      if (fabs(coef) > adaptive_eps) daxpy(nc, coef, retrq(w, i), v);
+      if (w->legacy ? coef != 0.0 : fabs(coef) > adaptive_eps) daxpy(nc, coef, retrq(w, i), v);
     }
   }
   for (int64_t r = 0; r < d; ++r) {
     double *v = vt + r * nc;
     opb(w, v, tmp);
     double t = ddot(nc, v, tmp);
-    double sval = sqrt(fmax(t, 0.0));
+    double sval = w->legacy ? sqrt(t) : sqrt(fmax(t, 0.0)); /* legacy.rs:1849 */
     sv[r] = sval;
     double *u = ut + r * nr;
     orc_opa(w->A, v, u, w->transposed);
-    double scale = sval > adaptive_eps ? 1.0 / sval : 1.0 / fmax(sval, adaptive_eps);
+    double scale = w->legacy ? 1.0 / sval : sval > adaptive_eps ? 1.0 / sval : 1.0 / fmax(sval, adaptive_eps);
     dscal(nr, scale, u);
   }
   free(tmp);
@@ -1052,9 +1221,25 @@ void orc_svdrec_free(orc_svdrec *rec) {
 }
 
 /* mod.rs:104-189 */
+static int las2_run(const orc_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
+                    double kappa, uint32_t random_seed, const double *start, double eps, int legacy,
+                    orc_svdrec *out);
+
 int orc_las2(const orc_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
              double kappa, uint32_t random_seed, const double *start, double eps,
              orc_svdrec *out) {
+  return las2_run(op, dimensions, iterations, endl, endr, kappa, random_seed, start, eps, 0, out);
+}
+
+/* legacy::svdLAS2 (legacy.rs:569-663): f64, the legacy control flow. */
+int orc_las2_legacy(const orc_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
+                    double kappa, uint32_t random_seed, const double *start, orc_svdrec *out) {
+  return las2_run(op, dimensions, iterations, endl, endr, kappa, random_seed, start, DBL_EPSILON, 1, out);
+}
+
+static int las2_run(const orc_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
+                    double kappa, uint32_t random_seed, const double *start, double eps, int legacy,
+                    orc_svdrec *out) {
   memset(out, 0, sizeof *out);
   g_blas_threads = nthreads_of(op);
   if (random_seed == 0) random_seed = (uint32_t)(mix64((uint64_t)&out ^ 0x1234567ULL) >> 32);
@@ -1066,13 +1251,14 @@ int orc_las2(const orc_op *op, int64_t dimensions, int64_t iterations, double en
   else if (iterations < dimensions)
     iterations = dimensions;
   if (dimensions < 2) {
-    set_err("svd_las2: insufficient dimensions: %lld", (long long)dimensions);
+    set_err("%s: insufficient dimensions: %lld", legacy ? "svdLAS2" : "svd_las2", (long long)dimensions);
     return ORC_ERR_LAS2;
   }
   las2_ws w;
   memset(&w, 0, sizeof w);
   w.A = op;
   w.eps = eps;
+  w.legacy = legacy;
   w.transposed = (double)n >= (double)m * 1.2;
   w.nrows = w.transposed ? n : m;
   w.ncols = w.transposed ? m : n;

@@ -121,6 +121,14 @@ void orc_start_vector(uint32_t seed, int64_t n, double *out);
 int orc_las2(const orc_op *op, int64_t dimensions, int64_t iterations, double endl,
              double endr, double kappa, uint32_t random_seed, const double *start,
              double eps, orc_svdrec *out);
+/* legacy::svdLAS2 (src/legacy.rs:569-663): f64, plain f64::EPSILON
+ * tolerances, splits at compare(bet, 0), imtqlb / imtql2 with the
+ * absolute convergence test and at most 30 iterations (then an error),
+ * significant values bnd <= kappa |ritz|. */
+int orc_las2_legacy(const orc_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
+                    double kappa, uint32_t random_seed, const double *start, orc_svdrec *out);
+int orc_imtqlb_legacy(int64_t n, double *d, double *e, double *bnd, double eps);
+int orc_imtql2_legacy(int64_t nm, int64_t n, double *d, double *e, double *z, double eps);
 void orc_svdrec_free(orc_svdrec *rec);
 const char *orc_last_error(void);
 

@@ -79,6 +79,10 @@ _lib.orc_imtql2.argtypes = [C.c_int64, C.c_int64, _f64p, _f64p, _f64p, C.c_int64
 _lib.orc_start_vector.argtypes = [C.c_uint32, C.c_int64, _f64p]
 _lib.orc_las2.argtypes = [C.POINTER(_Op), C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
                           C.c_uint32, _f64p, C.c_double, C.POINTER(_Rec)]
+_lib.orc_las2_legacy.argtypes = [C.POINTER(_Op), C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                 C.c_uint32, _f64p, C.POINTER(_Rec)]
+_lib.orc_imtqlb_legacy.argtypes = [C.c_int64, _f64p, _f64p, _f64p, C.c_double]
+_lib.orc_imtql2_legacy.argtypes = [C.c_int64, C.c_int64, _f64p, _f64p, _f64p, C.c_double]
 _lib.orc_svdrec_free.argtypes = [C.POINTER(_Rec)]
 _lib.orc_last_error.restype = C.c_char_p
 _lib.orc_determine_chunk_size.argtypes = [C.c_int64, C.c_int]
@@ -247,6 +251,31 @@ def imtql2(d, e, max_iter=100, eps=F64_EPS):
     return d, z
 
 
+def imtqlb_legacy(d, e):
+    """legacy.rs:961-1067: imtqlb with the absolute convergence test and at
+    most 30 iterations per eigenvalue (RuntimeError after)."""
+    d = np.array(d, dtype=np.float64)
+    e = np.array(e, dtype=np.float64)
+    bnd = np.zeros_like(d)
+    st = _lib.orc_imtqlb_legacy(d.size, _p(d), _p(e), _p(bnd), F64_EPS)
+    if st:
+        raise RuntimeError(_lib.orc_last_error().decode())
+    return d, bnd
+
+
+def imtql2_legacy(d, e):
+    """legacy.rs:1575-1692: imtql2 with the absolute convergence test and at
+    most 30 iterations per eigenvalue (RuntimeError after)."""
+    d = np.array(d, dtype=np.float64)
+    e = np.array(e, dtype=np.float64)
+    n = d.size
+    z = np.eye(n, dtype=np.float64)
+    st = _lib.orc_imtql2_legacy(n, n, _p(d), _p(e), _p(z), F64_EPS)
+    if st:
+        raise RuntimeError(_lib.orc_last_error().decode())
+    return d, z
+
+
 def start_vector(seed, n):
     out = np.empty(n, dtype=np.float64)
     _lib.orc_start_vector(seed, n, _p(out))
@@ -262,6 +291,23 @@ def las2(op, dimensions=0, iterations=0, end_interval=(-1e-30, 1e-30), kappa=1e-
         sv = np.ascontiguousarray(start, dtype=np.float64)
     st = _lib.orc_las2(C.byref(op._st), dimensions, iterations, end_interval[0], end_interval[1], kappa,
                        random_seed, _p(sv) if sv is not None else None, eps, C.byref(rec))
+    return _unpack(st, rec)
+
+
+def las2_legacy(op, dimensions=0, iterations=0, end_interval=(-1e-30, 1e-30), kappa=1e-6, random_seed=0,
+                start=None):
+    """legacy::svdLAS2 restatement (legacy.rs:569-663): FP64 only, the legacy
+    control flow. Returns (status, dict) as las2 does."""
+    rec = _Rec()
+    sv = None
+    if start is not None:
+        sv = np.ascontiguousarray(start, dtype=np.float64)
+    st = _lib.orc_las2_legacy(C.byref(op._st), dimensions, iterations, end_interval[0], end_interval[1], kappa,
+                              random_seed, _p(sv) if sv is not None else None, C.byref(rec))
+    return _unpack(st, rec)
+
+
+def _unpack(st, rec):
     if st:
         return st, {"error": _lib.orc_last_error().decode()}
     d, m, n = rec.d, rec.m, rec.n

@@ -169,7 +169,7 @@ svdh_op *csc_create(Ctx *c, int64_t m, int64_t n, int64_t nnz, const int64_t *co
                     const void *vals, int32_t vdt, uint32_t flags);
 int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
                    double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out,
-                   const RsvdDeviceOut *dev);
+                   const RsvdDeviceOut *dev, bool legacy = false);
 int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_oversamples, int64_t n_power_iters,
                    int32_t normalizer, int32_t mean_center, uint64_t seed, int32_t has_seed, const double *omega,
                    int32_t precision, double tol, int64_t max_power_iters, svdh_result *out,
@@ -577,6 +577,20 @@ int32_t svdh_las2(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t 
     if (precision == SVDH_PREC_F32) ensure_values(const_cast<svdh_op *>(op), SVDH_F32);
     las2_solve(ctx, const_cast<svdh_op *>(op), dimensions, iterations, endl, endr, kappa, random_seed, start,
                precision, out, nullptr);
+  });
+  if (st != SVDH_OK && out) svdh_result_free(out);
+  return st;
+}
+
+int32_t svdh_las2_legacy(svdh_ctx *ctx, const svdh_op *op, int64_t dimensions, int64_t iterations, double endl,
+                         double endr, double kappa, uint32_t random_seed, const double *start, svdh_result *out) {
+  if (out) std::memset(out, 0, sizeof *out);
+  int32_t st = guarded([&] {
+    check_ctx(ctx);
+    check_op(ctx, op);
+    if (!out) fail(SVDH_ERR_ARG, "null result");
+    las2_solve(ctx, const_cast<svdh_op *>(op), dimensions, iterations, endl, endr, kappa, random_seed, start,
+               SVDH_PREC_F64, out, nullptr, true);
   });
   if (st != SVDH_OK && out) svdh_result_free(out);
   return st;

@@ -442,6 +442,7 @@ void panel_rank1_correct(Ctx *c, int64_t rx, int64_t w, const void *X, int64_t l
 // ---- host dense algebra (hostla.cpp) --------------------------------------
 double pythag(double a, double b, double eps);
 int imtqlb(int64_t n, double *d, double *e, double *bnd, int64_t max_iter, double eps);
+int imtqlb_legacy(int64_t n, double *d, double *e, double *bnd, double eps);
 int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_iter, double eps);
 // Symmetric eigen-decomposition of an l x l row-major matrix: eigenvalues
 // ascending in w, eigenvectors as columns of V (row-major l x l); V null:

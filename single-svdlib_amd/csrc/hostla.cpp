@@ -132,6 +132,84 @@ int imtqlb(int64_t n, double *d, double *e, double *bnd, int64_t max_iter, doubl
   return SVDH_OK;
 }
 
+// legacy::imtqlb (src/legacy.rs:961-1067): the EISPACK form with the
+// absolute tests |b - a| < eps for convergence (test vs test + |e[m]|) and
+// underflow (r vs 0), at most 30 QL iterations per eigenvalue and then
+// ImtqlbError.
+int imtqlb_legacy(int64_t n, double *d, double *e, double *bnd, double eps) {
+  if (n == 1) return SVDH_OK;
+  const int64_t last = n - 1;
+  bnd[0] = 1.0;
+  for (int64_t i = 1; i <= last; ++i) {
+    bnd[i] = 0.0;
+    e[i - 1] = e[i];
+  }
+  e[last] = 0.0;
+  for (int64_t l = 0; l <= last; ++l) {
+    for (int it = 0;; ++it) {
+      int64_t m = l;
+      for (; m < last; ++m) {
+        const double test = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (near(test, test + std::fabs(e[m]), eps)) break;
+      }
+      double p = d[l];
+      double f = bnd[l];
+      if (m == l) {
+        int64_t i = l;
+        while (i >= 1 && p < d[i - 1]) {
+          d[i] = d[i - 1];
+          bnd[i] = bnd[i - 1];
+          --i;
+        }
+        d[i] = p;
+        bnd[i] = f;
+        break;
+      }
+      if (it == 30) fail(SVDH_ERR_IMTQLB, "imtqlb no convergence to an eigenvalue after 30 iterations");
+      double g = (d[l + 1] - p) / (2.0 * e[l]);
+      double r = pythag(g, 1.0, eps);
+      g = d[m] - p + e[l] / (g + sign_of(r, g));
+      double s = 1.0, c = 1.0;
+      p = 0.0;
+      int64_t i = m - 1;
+      bool underflow = false;
+      for (;;) {
+        f = s * e[i];
+        const double b = c * e[i];
+        r = pythag(f, g, eps);
+        e[i + 1] = r;
+        if (near(r, 0.0, eps)) {
+          underflow = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        f = bnd[i + 1];
+        bnd[i + 1] = s * bnd[i] + c * f;
+        bnd[i] = c * bnd[i] - s * f;
+        if (i == l) break;
+        --i;
+      }
+      if (underflow) {
+        d[i + 1] -= p;
+      } else {
+        d[l] -= p;
+        e[l] = g;
+      }
+      e[m] = 0.0;
+    }
+  }
+  return SVDH_OK;
+}
+
+// With max_iter = 30 this is also legacy::imtql2 (src/legacy.rs:1575-1692):
+// the convergence and underflow tests and the ascending selection sort are
+// the same.
 int imtql2(int64_t nm, int64_t n, double *d, double *e, double *z, int64_t max_iter, double eps) {
   if (n == 1) return SVDH_OK;
   const int64_t last = n - 1;

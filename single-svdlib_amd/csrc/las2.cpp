@@ -32,6 +32,7 @@ struct Las2 {
   svdh_op *op;
   int vdt;          // sparse value dtype
   double eps;       // epsilon of the reference's T
+  bool legacy = false;  // legacy::svdLAS2 control flow (src/legacy.rs:569-2014)
   bool transposed;
   int fwd, bwd;     // operator directions for "A x" and "A^T y" in working terms
   int64_t nrows, ncols;  // working shape
@@ -319,11 +320,12 @@ struct Las2 {
     for (int64_t i = 0; i < n; ++i) y[off + n - 1 - i] = x[off + i];
   }
 
-  // mod.rs:1263-1406
+  // mod.rs:1263-1406; legacy.rs:1924-2014 in legacy mode (plain eps, a split
+  // where compare(bet, 0), legacy imtqlb, no extra steps)
   int64_t lanso(int64_t dim, double endl, double endr, uint32_t seed, int64_t *neig) {
     const double sp = sparsity();
     const int64_t max_imtqlb = sp > 0.999 ? 500 : sp > 0.99 ? 300 : sp > 0.9 ? 100 : 50;
-    const double aeps = sp > 0.99 ? eps * 100.0 : sp > 0.9 ? eps * 10.0 : eps;
+    const double aeps = legacy ? eps : sp > 0.99 ? eps * 100.0 : sp > 0.9 ? eps * 10.0 : eps;
     double rnm, tol;
     stpone(seed, &rnm, &tol);
     const double eps1 = aeps * std::sqrt((double)ncols);
@@ -341,12 +343,15 @@ struct Las2 {
       int64_t l = 0;
       for (int64_t it = 0; it < j && l <= j; ++it) {
         int64_t i = l;
-        while (i <= j && !(std::fabs(bet[i + 1]) <= aeps)) ++i;
+        while (i <= j && !(legacy ? zero(bet[i + 1]) : std::fabs(bet[i + 1]) <= aeps)) ++i;
         i = std::min(i, j);
         const int64_t sz = i - l;
         copy_reversed(sz + 1, l, alf.data(), ritz.data());
         copy_reversed(sz, l + 1, bet.data(), w5.data());
-        imtqlb(sz + 1, ritz.data() + l, w5.data() + l, bnd.data() + l, max_imtqlb, eps);
+        if (legacy)
+          imtqlb_legacy(sz + 1, ritz.data() + l, w5.data() + l, bnd.data() + l, eps);
+        else
+          imtqlb(sz + 1, ritz.data() + l, w5.data() + l, bnd.data() + l, max_imtqlb, eps);
         for (int64_t m = l; m <= i; ++m) bnd[m] = rnm * std::fabs(bnd[m]);
         l = i + 1;
       }
@@ -363,7 +368,7 @@ struct Las2 {
           last = first + 9;
           intro = first;
         } else {
-          const int64_t extra = sp > 0.99 ? 5 : 0;
+          const int64_t extra = !legacy && sp > 0.99 ? 5 : 0;
           last = first + std::max<int64_t>(3, 1 + ((j - intro) * (dim - *neig)) / *neig) + extra;
         }
         last = std::min(last, iterations);
@@ -376,15 +381,18 @@ struct Las2 {
     return j;
   }
 
-  // mod.rs:1056-1259 with the Ritz indexing and ordering of legacy.rs:1796-1835.
+  // mod.rs:1056-1259 with the Ritz indexing and ordering of legacy.rs:1796-1835;
+  // in legacy mode legacy.rs:1765-1870 (imtql2 capped at 30 iterations, the
+  // significance test bnd <= kappa |ritz|, every non-zero coefficient, u = A v
+  // / sigma).
   // dev (nullable): U (d x m) and V^T (d x n) go to these device buffers
   // instead of host memory, s to dev->s.
   void ritvec(int64_t dims, double kappa, int64_t steps, int64_t neig, svdh_result *out, int64_t *nsig_out,
               const RsvdDeviceOut *dev) {
     const int64_t js = steps + 1;
     const double sp = sparsity();
-    const double aeps = sp > 0.99 ? eps * 100.0 : sp > 0.9 ? eps * 10.0 : eps;
-    const int64_t max_imtql2 = sp > 0.999 ? 500 : sp > 0.99 ? 300 : sp > 0.9 ? 200 : 50;
+    const double aeps = legacy ? eps : sp > 0.99 ? eps * 100.0 : sp > 0.9 ? eps * 10.0 : eps;
+    const int64_t max_imtql2 = legacy ? 30 : sp > 0.999 ? 500 : sp > 0.99 ? 300 : sp > 0.9 ? 200 : 50;
     std::vector<double> s(js * js, 0.0), ev(js), e(std::max<int64_t>(js, 1) + 1, 0.0);
     for (int64_t i = 0; i < js; ++i) s[i * js + i] = 1.0;
     copy_reversed(js, 0, alf.data(), ev.data());
@@ -392,10 +400,12 @@ struct Las2 {
     imtql2(js, js, ev.data(), e.data(), s.data(), max_imtql2, eps);
     double max_ev = 0.0;
     for (double v : ev) max_ev = std::max(max_ev, std::fabs(v));
-    const double akappa = sp > 0.99 ? kappa * 10.0 : kappa;
+    const double akappa = !legacy && sp > 0.99 ? kappa * 10.0 : kappa;
     std::vector<int64_t> sig;
-    for (int64_t k = 0; k < js; ++k)
-      if (bnd[k] <= akappa * std::max(std::fabs(ritz[k]), max_ev * aeps) && k + 1 > js - neig) sig.push_back(k);
+    for (int64_t k = 0; k < js; ++k) {
+      const double rel = legacy ? kappa * std::fabs(ritz[k]) : akappa * std::max(std::fabs(ritz[k]), max_ev * aeps);
+      if (bnd[k] <= rel && k + 1 > js - neig) sig.push_back(k);
+    }
     const int64_t nsig = (int64_t)sig.size();
     const int64_t d = std::min(dims, nsig);
     *nsig_out = nsig;
@@ -418,7 +428,7 @@ struct Las2 {
       const int64_t k = sig[nsig - 1 - r];
       for (int64_t i = 0; i < js; ++i) {
         const double coef = s[(js - 1 - i) * js + k];
-        if (std::fabs(coef) > aeps) W[i * d + r] = coef;
+        if (legacy ? coef != 0.0 : std::fabs(coef) > aeps) W[i * d + r] = coef;
       }
     }
     const int64_t ldv = (d + 1) / 2 * 2;
@@ -445,7 +455,7 @@ struct Las2 {
     for (int64_t r = 0; r < d; ++r) {
       const double sval = std::sqrt(std::max(hd[r], 0.0));
       s_out[r] = sval;
-      hs[r] = sval > aeps ? 1.0 / sval : 1.0 / std::max(sval, aeps);
+      hs[r] = legacy || sval > aeps ? 1.0 / sval : 1.0 / std::max(sval, aeps);
     }
     HIP_CHECK(hipMemcpyAsync(scale, hs.data(), d * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (dev) {
@@ -476,10 +486,12 @@ struct Las2 {
 
 int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t iterations, double endl, double endr,
                    double kappa, uint32_t random_seed, const double *start, int32_t precision, svdh_result *out,
-                   const RsvdDeviceOut *dev) {
+                   const RsvdDeviceOut *dev, bool legacy) {
   if (precision != SVDH_PREC_F64 && precision != SVDH_PREC_F32)
     fail(SVDH_ERR_ARG, "LAS2 precision must be F64 or F32");
-  if (op->nrows <= 0 || op->ncols <= 0) fail(SVDH_ERR_LAS2, "svd_las2: empty matrix");
+  if (legacy && precision != SVDH_PREC_F64) fail(SVDH_ERR_ARG, "svdLAS2 is FP64 only");
+  const char *name = legacy ? "svdLAS2" : "svd_las2";
+  if (op->nrows <= 0 || op->ncols <= 0) fail(SVDH_ERR_LAS2, "%s: empty matrix", name);
   if (random_seed == 0) random_seed = std::random_device{}();
   const int64_t mn = std::min(op->nrows, op->ncols);
   if (dimensions == 0 || dimensions > mn) dimensions = mn;
@@ -487,13 +499,14 @@ int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t itera
     iterations = mn;
   else if (iterations < dimensions)
     iterations = dimensions;
-  if (dimensions < 2) fail(SVDH_ERR_LAS2, "svd_las2: insufficient dimensions: %lld", (long long)dimensions);
+  if (dimensions < 2) fail(SVDH_ERR_LAS2, "%s: insufficient dimensions: %lld", name, (long long)dimensions);
 
   Las2 L;
   L.c = ctx;
   L.op = op;
   L.vdt = precision == SVDH_PREC_F64 ? SVDH_F64 : SVDH_F32;
   L.eps = precision == SVDH_PREC_F64 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+  L.legacy = legacy;
   L.transposed = (double)op->ncols >= (double)op->nrows * 1.2;
   L.fwd = L.transposed ? 1 : 0;
   L.bwd = 1 - L.fwd;

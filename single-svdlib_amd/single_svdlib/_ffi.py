@@ -95,6 +95,8 @@ SIGNATURES = [
     ("svdh_column_means", _i32, [_vp, _vp, _vp, _i32]),
     ("svdh_las2", _i32, [_vp, _vp, _i64, _i64, C.c_double, C.c_double, C.c_double, C.c_uint32, _f64p, _i32,
                          C.POINTER(Result_)]),
+    ("svdh_las2_legacy", _i32, [_vp, _vp, _i64, _i64, C.c_double, C.c_double, C.c_double, C.c_uint32, _f64p,
+                                C.POINTER(Result_)]),
     ("svdh_las2_device", _i32, [_vp, _vp, _i64, _i64, C.c_double, C.c_double, C.c_double, C.c_uint32, _f64p, _i32,
                                 _vp, _i64, _f64p, _vp, _i64, C.POINTER(Result_)]),
     ("svdh_las2_start_vector", _i32, [C.c_uint32, _i64, _f64p]),

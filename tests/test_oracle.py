@@ -304,6 +304,107 @@ def test_las2_f32_epsilon_mode():
     assert np.allclose(r["s"], s_true[: r["d"]], rtol=1e-5)
 
 
+# ------------------------------------------------- legacy::svdLAS2 (f64)
+
+@pytest.mark.parametrize("n", [2, 3, 7, 20, 41])
+def test_imtqlb_legacy_eigenvalues_and_bounds(n):
+    d, off = _tridiag(n, 200 + n)
+    w, z = sla.eigh_tridiagonal(d, off)
+    ev, bnd = oracle.imtqlb_legacy(d, np.concatenate([[0.0], off]))
+    assert np.max(np.abs(ev - w)) <= 1e-13 * np.abs(w).max()
+    assert np.max(np.abs(np.abs(bnd) - np.abs(z[0, :]))) <= 1e-10
+
+
+@pytest.mark.parametrize("n", [2, 5, 16, 61])
+def test_imtql2_legacy_eigenpairs(n):
+    d, off = _tridiag(n, 300 + n)
+    w, z = sla.eigh_tridiagonal(d, off)
+    ev, zz = oracle.imtql2_legacy(d, np.concatenate([[0.0], off]))
+    assert np.max(np.abs(ev - w)) <= 1e-13 * np.abs(w).max()
+    t = np.diag(d) + np.diag(off, 1) + np.diag(off, -1)
+    assert np.max(np.abs(t @ zz - zz * ev)) <= 1e-12 * np.abs(w).max()
+    assert np.max(np.abs(zz.T @ zz - np.eye(n))) <= 1e-12
+
+
+@pytest.mark.parametrize("legacy", [False, True])
+def test_imtql2_keeps_the_absolute_underflow_test(legacy):
+    # legacy.rs:1634 and mod.rs:970 end a QL sweep early when compare(r, 0),
+    # i.e. |r| < 2.2e-16 absolute, where EISPACK tests r == 0. On this
+    # tridiagonal one rotation radius is 5.1e-17: the eigenvalues stay exact
+    # but some eigenvectors lose accuracy (residual 5.5e-2). The restatement
+    # keeps the reference's test; an exact-zero test would leave 6e-15.
+    n = 60
+    d, off = _tridiag(n, 360)
+    t = np.diag(d) + np.diag(off, 1) + np.diag(off, -1)
+    e = np.concatenate([[0.0], off])
+    ev, zz = oracle.imtql2_legacy(d, e) if legacy else oracle.imtql2(d, e)
+    assert np.max(np.abs(ev - sla.eigh_tridiagonal(d, off, eigvals_only=True))) <= 1e-13 * np.abs(ev).max()
+    res = np.abs(t @ zz - zz * ev).max(axis=0)
+    assert 0.05 < res.max() < 0.06
+    assert np.sum(res > 1e-10) == 6
+
+
+def test_imtql_legacy_stop_after_30_iterations():
+    # legacy.rs:1015-1019, 1610-1614: a tridiagonal whose convergence test can
+    # never pass (a NaN entry) ends in the error after 30 QL iterations
+    with pytest.raises(RuntimeError, match="after 30 iterations"):
+        oracle.imtqlb_legacy([np.nan, 1.0, 2.0], [0.0, 1.0, 1.0])
+    with pytest.raises(RuntimeError, match="after 30 iterations"):
+        oracle.imtql2_legacy([np.nan, 1.0, 2.0], [0.0, 1.0, 1.0])
+
+
+def test_las2_legacy_mode_known_answers():
+    # the reference's tests run these on legacy::svd / svd_dim_seed themselves
+    # (legacy.rs:2169-2235; the 3x3 dump is svdlibrs' output for seed 3141)
+    g = golden("legacy_kats.json")
+    st, r = oracle.las2_legacy(oracle.OracleOp(sp.csc_matrix(np.array(g["matrix_3x3"]))), 0, 0, random_seed=3141)
+    assert st == 0 and r["d"] == 3
+    dump = g["svdlibrs_3x3_seed3141"]
+    assert np.allclose(r["s"], dump["s"], rtol=0, atol=1e-12)
+    for i in range(3):
+        for got, want in ((r["u"][i], np.array(dump["ut"][i])), (r["vt"][i], np.array(dump["vt"][i]))):
+            assert np.allclose(np.sign(got @ want) * got, want, atol=1e-11)
+    for key in ("non_zero", "dimensions", "iterations", "lanczos_steps", "ritz_values_stabilized",
+                "significant_values", "singular_values", "random_seed"):
+        assert r["diagnostics"][key] == dump["diagnostics"][key], key
+    st, r = oracle.las2_legacy(oracle.OracleOp(sp.csr_matrix(np.array(g["basic_2x2"]["matrix"]))), 0, 0,
+                               random_seed=1)
+    assert st == 0 and r["d"] == 2 and np.allclose(r["s"], g["basic_2x2"]["s"], atol=1e-12)
+    st, r = oracle.las2_legacy(oracle.OracleOp(sp.identity(3, format="csc")), 0, 0, random_seed=7)
+    assert st == 0 and r["d"] == g["identity_3x3"]["d"] and abs(r["s"][0] - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("shape,density,k", [((60, 40), 0.2, 8), ((40, 90), 0.15, 6), ((300, 200), 0.05, 12),
+                                             ((100, 100), 0.0098, 50)])
+def test_las2_legacy_mode_matches_lapack_and_current(shape, density, k):
+    # LAPACK truth, and lib.rs:60-110: legacy and lanczos agree on d, the
+    # singular values and the recomposition
+    a = random_csr(*shape, density, seed=shape[1])
+    st, r = oracle.las2_legacy(oracle.OracleOp(a), k, 0, random_seed=42)
+    st2, r2 = oracle.las2(oracle.OracleOp(a), k, 0, random_seed=42)
+    assert st == 0 and st2 == 0 and r["d"] >= 2 and r["d"] == r2["d"]
+    s_true = np.linalg.svd(a.toarray(), compute_uv=False)
+    assert np.allclose(r["s"], s_true[: r["d"]], rtol=1e-10, atol=0)
+    assert np.allclose(r["s"], r2["s"], rtol=1e-10, atol=0)
+    assert triplet_residual(a, r["u"].T, r["s"], r["vt"]) < 1e-8
+    rec = r["u"].T @ np.diag(r["s"]) @ r["vt"]
+    rec2 = r2["u"].T @ np.diag(r2["s"]) @ r2["vt"]
+    assert np.max(np.abs(rec - rec2)) < 1e-8 * max(1.0, np.abs(rec).max())
+
+
+def test_las2_legacy_mode_errors():
+    # a NaN in A reaches the tridiagonal: legacy imtqlb stops after 30
+    # iterations (ImtqlbError, status 1) where the current imtqlb forces
+    # convergence and imtql2 fails later (status 4)
+    a = random_csr(50, 40, 0.2, seed=1).tolil()
+    a[3, 4] = np.nan
+    a = a.tocsr()
+    st, r = oracle.las2_legacy(oracle.OracleOp(a), 5, 0, random_seed=3)
+    assert st == 1 and "30 iterations" in r["error"]
+    st, r = oracle.las2_legacy(oracle.OracleOp(sp.csr_matrix(np.ones((1, 5)))), 0, 0, random_seed=1)
+    assert st == 5 and "svdLAS2: insufficient dimensions" in r["error"]
+
+
 def test_start_vector_closed_form():
     v = oracle.start_vector(42, 5)
     phi = 0x9E3779B97F4A7C15
