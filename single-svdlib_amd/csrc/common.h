@@ -74,7 +74,7 @@ struct Ctx {
   DevBuf stage;
   // Device workspaces kept for the context's lifetime (panels of the
   // randomized solver), grown on demand; slot meanings belong to the caller.
-  static constexpr int kWorkspaces = 12;
+  static constexpr int kWorkspaces = 13;  // 0-11: randomized solver panels, 12: basis GEMV partials
   DevBuf ws[kWorkspaces];
   void *workspace(int slot, size_t bytes) {
     if (bytes > ws[slot].bytes) {
@@ -325,6 +325,16 @@ struct Axpy { double *y; const double *x; double imm; int slot; double sign; };
 void fused_axpy_dot(Ctx *c, int64_t n, const Axpy *ups, int nups, const double *u,
                     const double *v, int out_slot, double *copy_dst, const double *copy_src);
 void slots_finish(Ctx *c, int n);
+// Publishes slots first .. first + n - 1; with sqrt_from >= 0 (a slot of that
+// range) also sqrt(value of sqrt_from) as the single partial of sqrt_to.
+void slots_finish_range(Ctx *c, int first, int n, int sqrt_from, int sqrt_to);
+// Slots slot_a and slot_b get sum |a_i| and sum |b_i| over i < k (device
+// vectors), each summed in index order.
+void abs_sums_to_slots(Ctx *c, int64_t k, const double *a, const double *b, int slot_a, int slot_b);
+// y = s x, zout = s zin (and copy_dst = copy_src) with s = 1 / (value of
+// rnm_slot, a slot holding one published value) or, with rnm_slot < 0, s = a.
+void scale_step(Ctx *c, int64_t n, int rnm_slot, double a, const double *x, double *y, const double *zin,
+                double *zout, double *copy_dst = nullptr, const double *copy_src = nullptr);
 // A chain of fused ops (each as fused_axpy_dot with at most one update). On
 // vectors of up to 8,192 entries the whole chain is one single-workgroup
 // launch that also publishes the produced slots to Ctx::scal and the host
@@ -338,6 +348,7 @@ struct FusedOp {
   int out_slot;
   double *copy_dst;
   const double *copy_src;
+  int sqrt_slot = -1;  // chain only: also publish sqrt(dot) to this slot
 };
 bool fused_chain(Ctx *c, int64_t n, const FusedOp *ops, int nops);
 // y = a x, z *= a, and (when copy_dst is set) copy_dst = copy_src, in one pass.

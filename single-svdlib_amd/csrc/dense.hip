@@ -116,6 +116,7 @@ struct ChainOp {
   int out_slot;
   double *copy_dst;
   const double *copy_src;
+  int sqrt_slot;  // >= 0: sqrt of the dot is published to this slot as well
 };
 struct ChainArgs {
   int64_t n;
@@ -156,23 +157,41 @@ __global__ __launch_bounds__(kChainBlock) void k_fused_chain(ChainArgs a) {
         a.parts[op.out_slot * kSlotParts] = t;
         a.scal[op.out_slot] = t;
         a.host[op.out_slot] = t;
+        if (op.sqrt_slot >= 0) {
+          const double r = sqrt(t);
+          sv[op.sqrt_slot] = r;
+          a.parts[op.sqrt_slot * kSlotParts] = r;
+          a.scal[op.sqrt_slot] = r;
+          a.host[op.sqrt_slot] = r;
+        }
       }
     }
     __syncthreads();  // the value (and red) before the next op
   }
 }
 
-// scal[s] = the value of slot s (one 64-thread workgroup per slot, s < n).
+// scal[s] = the value of slot s (one 64-thread workgroup per slot, s = first
+// + block). The workgroup of slot sqrt_from also publishes sqrt(value) as the
+// single partial of slot sqrt_to (a slot no other workgroup of the launch
+// reads or writes).
 struct SlotCounts {
   int nb[S_NSLOT];
+  int first;
+  int sqrt_from, sqrt_to;
 };
-__global__ void k_slot_finish(const double *parts, SlotCounts cnt, double *scal, double *host) {
-  const int s = blockIdx.x;
+__global__ void k_slot_finish(double *parts, SlotCounts cnt, double *scal, double *host) {
+  const int s = cnt.first + blockIdx.x;
   if (cnt.nb[s] <= 0) return;
   const double v = slot_sum(parts + s * kSlotParts, cnt.nb[s]);
   if (threadIdx.x == 0) {
     scal[s] = v;
     host[s] = v;
+    if (s == cnt.sqrt_from) {
+      const double r = sqrt(v);
+      parts[cnt.sqrt_to * kSlotParts] = r;
+      scal[cnt.sqrt_to] = r;
+      host[cnt.sqrt_to] = r;
+    }
   }
 }
 
@@ -185,33 +204,114 @@ __global__ void k_scale2(int64_t n, double a, const double *x, double *y, double
   }
 }
 
+// slot values sum |a_i| and sum |b_i| (i < k), each added in index order by
+// one thread (the order of a sequential host loop).
+__global__ void k_abs_sums(int64_t k, const double *a, const double *b, double *pa, double *pb) {
+  if (threadIdx.x != 0) return;
+  double sa = 0.0, sb = 0.0;
+  for (int64_t i = 0; i < k; ++i) {
+    sa += fabs(a[i]);
+    sb += fabs(b[i]);
+  }
+  *pa = sa;
+  *pb = sb;
+}
+
+// The Lanczos step's scaling, out of place: s = 1 / rnm with rnm the single
+// partial of a slot (rpart) or s = a from the host; y = s x, zout = s zin,
+// and cdst = csrc when cdst is set.
+__global__ void k_scale_step(int64_t n, const double *rpart, double a, const double *x, double *y,
+                             const double *zin, double *zout, double *cdst, const double *csrc) {
+  const double s = rpart ? 1.0 / rpart[0] : a;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    y[i] = s * x[i];
+    zout[i] = s * zin[i];
+    if (cdst) cdst[i] = csrc[i];
+  }
+}
+
 __global__ void k_copy(int64_t n, const double *x, double *y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = x[i];
 }
 
-// cu[j] = Q[:, j]^T u, cv[j] = Q[:, j]^T v; one workgroup per column.
-__global__ __launch_bounds__(kBlock) void k_gemv_t(int64_t n, const double *Q, int64_t ldq, const double *u,
-                                                  const double *v, double *cu, double *cv) {
-  const double *q = Q + (int64_t)blockIdx.x * ldq;
-  double su = 0.0, sv = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kBlock) {
-    const double qi = q[i];
-    su += qi * u[i];
-    sv += qi * v[i];
+// Partial products of the reorthogonalisation's coefficients: workgroup
+// (tile, group) takes kGemvRows rows of u, v and the basis and kGemvCols
+// columns, and leaves part_u[col * ntiles + tile] = Q[rows, col]^T u[rows]
+// (part_v likewise); k_gemv_t_sum adds a column's tile partials in tile order.
+constexpr int kGemvRows = 4096;
+constexpr int kGemvCols = 8;
+__global__ __launch_bounds__(kBlock) void k_gemv_t(int64_t n, const double *Q, int64_t ldq, int64_t k,
+                                                  const double *u, const double *v, double *part_u,
+                                                  double *part_v) {
+  __shared__ double red[2 * kGemvCols][kBlock / 64];
+  const int64_t r0 = (int64_t)blockIdx.x * kGemvRows;
+  const int64_t r1 = min<int64_t>(n, r0 + kGemvRows);
+  const int64_t c0 = (int64_t)blockIdx.y * kGemvCols;
+  const int nc = (int)min<int64_t>(kGemvCols, k - c0);
+  double su[kGemvCols], sv[kGemvCols];
+  int64_t off[kGemvCols];  // columns past k read the last one; their sums are dropped
+#pragma unroll
+  for (int c = 0; c < kGemvCols; ++c) {
+    su[c] = sv[c] = 0.0;
+    off[c] = (c0 + min(c, nc - 1)) * ldq;
   }
-  su = block_sum(su);
-  sv = block_sum(sv);
-  if (threadIdx.x == 0) {
-    cu[blockIdx.x] = su;
-    cv[blockIdx.x] = sv;
+  // no branch around the loads: all of a row's loads are in flight together
+#pragma unroll 2
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+    const double ui = u[i], vi = v[i];
+    double qv[kGemvCols];
+#pragma unroll
+    for (int c = 0; c < kGemvCols; ++c) qv[c] = Q[off[c] + i];
+#pragma unroll
+    for (int c = 0; c < kGemvCols; ++c) {
+      su[c] += qv[c] * ui;
+      sv[c] += qv[c] * vi;
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < kGemvCols; ++c) {
+    const double a = wave_sum_d(su[c]), b = wave_sum_d(sv[c]);
+    if (lane == 0) {
+      red[c][wv] = a;
+      red[kGemvCols + c][wv] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * kGemvCols) {
+    const int c = threadIdx.x % kGemvCols;
+    if (c < nc) {
+      double t = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) t += red[threadIdx.x][w];
+      double *dst = threadIdx.x < kGemvCols ? part_u : part_v;
+      dst[(c0 + c) * gridDim.x + blockIdx.x] = t;
+    }
   }
 }
 
-// y1 -= Q c1, y2 -= Q c2 for an n x k column-major basis.
+__global__ void k_gemv_t_sum(int64_t k, int ntiles, const double *part_u, const double *part_v, double *cu,
+                             double *cv) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  double a = 0.0, b = 0.0;
+  for (int t = 0; t < ntiles; ++t) {
+    a += part_u[c * ntiles + t];
+    b += part_v[c * ntiles + t];
+  }
+  cu[c] = a;
+  cv[c] = b;
+}
+
+// y1 -= Q c1, y2 -= Q c2 for an n x k column-major basis: a workgroup takes
+// 64 rows, its wave w the columns j = w mod 4 (each load one 512-byte row
+// segment), and the four waves' sums are added in wave order.
 __global__ __launch_bounds__(kBlock) void k_gemv_n(int64_t n, const double *Q, int64_t ldq, int64_t k,
                                                   const double *c1, double *y1, const double *c2, double *y2) {
   __shared__ double s1[1024], s2[1024];
-  const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  __shared__ double red[2][kBlock / 64][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t i = blockIdx.x * (int64_t)64 + lane;
+  const bool ok = i < n;
   double a1 = 0.0, a2 = 0.0;
   for (int64_t j0 = 0; j0 < k; j0 += 1024) {
     const int64_t kk = min<int64_t>(1024, k - j0);
@@ -221,18 +321,27 @@ __global__ __launch_bounds__(kBlock) void k_gemv_n(int64_t n, const double *Q, i
       s2[t] = c2[j0 + t];
     }
     __syncthreads();
-    if (i < n) {
-      const double *q = Q + j0 * ldq + i;
-      for (int64_t j = 0; j < kk; ++j) {
+    if (ok) {
+      const double *q = Q + j0 * ldq + (ok ? i : 0);
+#pragma unroll 4
+      for (int64_t j = wv; j < kk; j += kBlock / 64) {
         const double qv = q[j * ldq];
         a1 += qv * s1[j];
         a2 += qv * s2[j];
       }
     }
   }
-  if (i < n) {
-    y1[i] -= a1;
-    y2[i] -= a2;
+  red[0][wv][lane] = a1;
+  red[1][wv][lane] = a2;
+  __syncthreads();
+  if (wv == 0 && ok) {
+    double t1 = 0.0, t2 = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      t1 += red[0][w][lane];
+      t2 += red[1][w][lane];
+    }
+    y1[i] -= t1;
+    y2[i] -= t2;
   }
 }
 
@@ -856,9 +965,14 @@ bool fused_chain(Ctx *c, int64_t n, const FusedOp *ops, int nops) {
     d.out_slot = f.out_slot;
     d.copy_dst = f.copy_dst;
     d.copy_src = f.copy_src;
+    d.sqrt_slot = f.out_slot >= 0 ? f.sqrt_slot : -1;
     if (f.out_slot >= 0) {
       if (f.out_slot >= S_NSLOT) fail(SVDH_ERR_ARG, "scalar slot %d out of range", f.out_slot);
       produced[f.out_slot] = true;
+    }
+    if (d.sqrt_slot >= 0) {
+      if (d.sqrt_slot >= S_NSLOT) fail(SVDH_ERR_ARG, "scalar slot %d out of range", d.sqrt_slot);
+      produced[d.sqrt_slot] = true;
     }
     bytes += (double)n * 8.0 * (3 * f.nups + 2 + (f.copy_dst ? 2 : 0));
   }
@@ -873,14 +987,49 @@ bool fused_chain(Ctx *c, int64_t n, const FusedOp *ops, int nops) {
   return true;
 }
 
-void slots_finish(Ctx *c, int n) {
+void slots_finish(Ctx *c, int n) { slots_finish_range(c, 0, n, -1, -1); }
+
+void slots_finish_range(Ctx *c, int first, int n, int sqrt_from, int sqrt_to) {
   if (n <= 0) return;
-  if (n > S_NSLOT) fail(SVDH_ERR_ARG, "scalar slots %d out of range", n);
+  if (first < 0 || first + n > S_NSLOT) fail(SVDH_ERR_ARG, "scalar slots %d..%d out of range", first, first + n);
+  if (sqrt_from >= 0 && (sqrt_from < first || sqrt_from >= first + n || sqrt_to < 0 || sqrt_to >= S_NSLOT ||
+                         (sqrt_to >= first && sqrt_to < first + n)))
+    fail(SVDH_ERR_ARG, "square-root slot %d -> %d outside the finished range", sqrt_from, sqrt_to);
   c->slot_parts.ensure((size_t)S_NSLOT * kSlotParts * sizeof(double));
   SlotCounts cnt;
-  for (int s = 0; s < S_NSLOT; ++s) cnt.nb[s] = s < n ? c->slot_nb[s] : 0;
+  for (int s = 0; s < S_NSLOT; ++s) cnt.nb[s] = s >= first && s < first + n ? c->slot_nb[s] : 0;
+  cnt.first = first;
+  cnt.sqrt_from = sqrt_from;
+  cnt.sqrt_to = sqrt_to;
+  if (sqrt_from >= 0) c->slot_nb[sqrt_to] = 1;
   hipLaunchKernelGGL(k_slot_finish, n, 64, 0, c->stream, c->slot_parts.as<double>(), cnt, c->scal.as<double>(),
                      c->dh_scal);
+  HIP_CHECK(hipGetLastError());
+}
+
+void abs_sums_to_slots(Ctx *c, int64_t k, const double *a, const double *b, int slot_a, int slot_b) {
+  if (slot_a < 0 || slot_a >= S_NSLOT || slot_b < 0 || slot_b >= S_NSLOT || slot_a == slot_b)
+    fail(SVDH_ERR_ARG, "scalar slots %d, %d out of range", slot_a, slot_b);
+  c->slot_parts.ensure((size_t)S_NSLOT * kSlotParts * sizeof(double));
+  double *parts = c->slot_parts.as<double>();
+  hipLaunchKernelGGL(k_abs_sums, 1, 64, 0, c->stream, k, a, b, parts + (size_t)slot_a * kSlotParts,
+                     parts + (size_t)slot_b * kSlotParts);
+  HIP_CHECK(hipGetLastError());
+  c->slot_nb[slot_a] = 1;
+  c->slot_nb[slot_b] = 1;
+}
+
+void scale_step(Ctx *c, int64_t n, int rnm_slot, double a, const double *x, double *y, const double *zin,
+                double *zout, double *copy_dst, const double *copy_src) {
+  const double *rpart = nullptr;
+  if (rnm_slot >= 0) {
+    if (rnm_slot >= S_NSLOT || c->slot_nb[rnm_slot] != 1)
+      fail(SVDH_ERR_ARG, "slot %d does not hold a single published value", rnm_slot);
+    rpart = c->slot_parts.as<double>() + (size_t)rnm_slot * kSlotParts;
+  }
+  KTimer t(c, KC_BLAS1, (double)n * (32.0 + (copy_dst ? 16.0 : 0.0)));
+  hipLaunchKernelGGL(k_scale_step, grid_of(n), kBlock, 0, c->stream, n, rpart, a, x, y, zin, zout, copy_dst,
+                     copy_src);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -898,8 +1047,16 @@ void vec_copy(Ctx *c, int64_t n, const double *x, double *y) {
 void basis_gemv_t(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *u, const double *v,
                   double *cu, double *cv) {
   if (k <= 0) return;
+  const int ntiles = (int)((n + kGemvRows - 1) / kGemvRows);
+  const int ngroups = (int)((k + kGemvCols - 1) / kGemvCols);
+  // workspace 12, sized for at least 1,024 columns so that it rarely grows
+  double *part = static_cast<double *>(
+      c->workspace(12, (size_t)2 * std::max<int64_t>(k, 1024) * ntiles * sizeof(double)));
   KTimer t(c, KC_GEMV, (double)n * k * 8.0 + 16.0 * n);
-  hipLaunchKernelGGL(k_gemv_t, (int)k, kBlock, 0, c->stream, n, Q, ldq, u, v, cu, cv);
+  hipLaunchKernelGGL(k_gemv_t, dim3(ntiles, ngroups), kBlock, 0, c->stream, n, Q, ldq, k, u, v, part,
+                     part + k * ntiles);
+  hipLaunchKernelGGL(k_gemv_t_sum, (int)((k + 255) / 256), 256, 0, c->stream, k, ntiles, part, part + k * ntiles, cu,
+                     cv);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -907,7 +1064,7 @@ void basis_gemv_n(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, co
                   const double *c2, double *y2) {
   if (k <= 0) return;
   KTimer t(c, KC_GEMV, (double)n * k * 8.0 + 32.0 * n);
-  hipLaunchKernelGGL(k_gemv_n, (int)((n + kBlock - 1) / kBlock), kBlock, 0, c->stream, n, Q, ldq, k, c1, y1, c2, y2);
+  hipLaunchKernelGGL(k_gemv_n, (int)((n + 63) / 64), kBlock, 0, c->stream, n, Q, ldq, k, c1, y1, c2, y2);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -14,6 +14,7 @@
 // (one 32-byte read) for the host-side recurrences (ortbnd, error bounds,
 // tridiagonal QL), which are O(j) / O(j^2) scalar work.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -37,10 +38,25 @@ struct Las2 {
   int fwd, bwd;     // operator directions for "A x" and "A^T y" in working terms
   int64_t nrows, ncols;  // working shape
   int64_t iterations;
-  // device vectors
-  DevBuf wbuf[5], tempb, cbuf;
+  // device vectors: the five working vectors w[] and free buffers that a
+  // step writes its outputs into (vbufs owns them all)
+  std::vector<DevBuf> vbufs;
+  std::vector<double *> vfree;
+  DevBuf tempb, cbuf;
   double *w[5];
   double *temp;
+  bool pipeline = true;  // enqueue step j + 1 before reading step j's scalars
+  hipEvent_t ev = nullptr;  // recorded after a step that has one enqueued behind it
+  ~Las2() {
+    if (ev) (void)hipEventDestroy(ev);
+  }
+  double *take() {
+    if (vfree.empty()) fail(SVDH_ERR_ARG, "LAS2 vector pool exhausted");
+    double *p = vfree.back();
+    vfree.pop_back();
+    return p;
+  }
+  void give(double *p) { vfree.push_back(p); }
   // basis
   DevBuf basis;
   int64_t ldq = 0, cap = 0;
@@ -177,18 +193,19 @@ struct Las2 {
 
   // mod.rs:760-813: up to two classical Gram-Schmidt passes of q (w1) and
   // r (w0) against q_ll .. q_{step-1}, done as two block GEMVs per pass.
-  void purge(int64_t ll, int64_t step, double *rnm, double tol) {
-    if (step < ll + 2) return;
+  // Returns whether the reorthogonalisation ran (it then changed w1, w0 and
+  // rnm, or left rnm <= tol).
+  bool purge(int64_t ll, int64_t step, double *rnm, double tol) {
+    if (step < ll + 2) return false;
     const double reps = std::sqrt(eps);
     const double eps1 = eps * std::sqrt((double)ncols);
     int64_t k = 0;
     for (int64_t i = 1; i < step - (ll + 1); ++i)
       if (std::fabs(eta[i]) > std::fabs(eta[k])) k = i;
     k += ll;
-    if (!(std::fabs(eta[k]) > reps)) return;
+    if (!(std::fabs(eta[k]) > reps)) return false;
     const double reps1 = eps1 / reps;
     const int64_t cnt = step - ll;
-    std::vector<double> hc(2 * cnt);
     bool flag = true;
     for (int it = 0; it < 2 && flag; ++it) {
       if (!(*rnm > tol)) continue;
@@ -200,14 +217,11 @@ struct Las2 {
       fused_axpy_dot(c, ncols, nullptr, 0, w[0], w[1], S_A, w[3], w[1]);
       Axpy up{w[0], w[1], 0.0, S_A, -1.0};
       fused_axpy_dot(c, ncols, &up, 1, w[0], w[0], S_B, w[4], w[0]);
-      HIP_CHECK(hipMemcpyAsync(hc.data(), cq, 2 * cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-      read_slots(2);
-      double tq = 0.0, tr = 0.0;
-      for (int64_t i = 0; i < cnt; ++i) {
-        tq += std::fabs(hc[i]);
-        tr += std::fabs(hc[cnt + i]);
-      }
-      tr += std::fabs(c->h_scal[S_A]);
+      // sum |coefficients| on the device, in the order of the host loop
+      abs_sums_to_slots(c, cnt, cq, cr, S_C, S_D);
+      read_slots(4);
+      const double tq = c->h_scal[S_C];
+      const double tr = c->h_scal[S_D] + std::fabs(c->h_scal[S_A]);
       *rnm = std::sqrt(c->h_scal[S_B]);
       if (tq <= reps1 && tr <= *rnm * reps1) flag = false;
     }
@@ -215,69 +229,180 @@ struct Las2 {
       eta[i] = eps1;
       oldeta[i] = eps1;
     }
+    return true;
   }
 
-  // mod.rs:674-758
+  // One Lanczos step's device work (mod.rs:690-745) after the buffer
+  // rotation W = (r, q_{j-1}, q_{j-2}, old r copy, p_{j-1}) and any restart:
+  // q_j = r / rnm, p = w3 / rnm, the store of q_{j-1} (unless done), A^T A p,
+  // the fused updates and dots, with the step's scalars in the slots at base
+  // (alternating between steps, so that a step enqueued ahead leaves the
+  // previous step's values alone) and rnm = sqrt(r.r) published beside them.
+  // Every output goes to a buffer taken from the pool, never to a vector the
+  // step reads: a step enqueued ahead is dropped by returning its outputs.
+  // rnm comes from the host (rnm_slot < 0) or, for a step enqueued ahead,
+  // from the previous step's published slot. Steps j <= kMaxLL wait for alpha
+  // and may move ll (mod.rs:719-731).
+  struct Step {
+    int64_t j = -1;
+    int base = 0;
+    double *w[5];
+    double *outs[4];   // buffers taken by the step
+    double *freed[4];  // buffers of W that the new state no longer holds
+  };
+  static int slot_base(int64_t j) { return (j & 1) ? 32 : 0; }
+  static constexpr int S_R = 6;  // rnm of the step, next to its dot slots
+
+  void enqueue_body(int64_t j, double *const W[5], double rnm, int rnm_slot, bool stored, int64_t *ll,
+                    Step *st) {
+    const int base = slot_base(j);
+    double *F1 = take(), *F2 = take(), *F3 = take(), *F4 = take();
+    scale_step(c, ncols, rnm_slot, 1.0 / rnm, W[0], F1, W[3], F2, stored ? nullptr : q(j - 1), W[2]);
+    opb(F2, F3);
+    std::vector<FusedOp> ch;
+    const Axpy first = rnm_slot >= 0 ? Axpy{F3, W[2], 0.0, rnm_slot, -1.0} : Axpy{F3, W[2], -rnm, -1, 1.0};
+    ch.push_back(FusedOp{first, 1, F3, F2, base + S_A, nullptr, nullptr});
+    if (j <= kMaxLL) {
+      if (!fused_chain(c, ncols, ch.data(), (int)ch.size())) slots_finish_range(c, base + S_A, 1, -1, -1);
+      ch.clear();
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      if (std::fabs(alf[j - 1]) > 4.0 * std::fabs(c->h_scal[base + S_A])) *ll = j;
+    }
+    // Pending update: r -= alpha_j q_j, applied by the next fused op.
+    Axpy pend{F3, F1, 0.0, base + S_A, -1.0};
+    const int64_t lim = std::min(j - 1, *ll);
+    for (int64_t i = 0; i < lim; ++i) {
+      const int slot = base + 8 + (int)i;
+      ch.push_back(FusedOp{pend, 1, p(i), F3, slot, nullptr, nullptr});
+      pend = Axpy{F3, q(i), 0.0, slot, -1.0};
+    }
+    ch.push_back(FusedOp{pend, 1, F3, W[4], base + S_B, nullptr, nullptr});
+    ch.push_back(FusedOp{Axpy{F3, W[2], 0.0, base + S_B, -1.0}, 1, F3, F2, base + S_C, nullptr, nullptr});
+    FusedOp last{Axpy{F3, F1, 0.0, base + S_C, -1.0}, 1, F3, F3, base + S_E, F4, F3};
+    last.sqrt_slot = base + S_R;
+    ch.push_back(last);
+    if (!fused_chain(c, ncols, ch.data(), (int)ch.size()))
+      slots_finish_range(c, base, S_E + 1, base + S_E, base + S_R);
+    st->j = j;
+    st->base = base;
+    double *nw[5] = {F3, F1, W[2], F2, F4};
+    double *fr[4] = {W[0], W[1], W[3], W[4]};
+    double *ou[4] = {F1, F2, F3, F4};
+    std::copy(nw, nw + 5, st->w);
+    std::copy(fr, fr + 4, st->freed);
+    std::copy(ou, ou + 4, st->outs);
+  }
+  void commit(const Step &st) {
+    std::copy(st.w, st.w + 5, w);
+    for (double *f : st.freed) give(f);
+  }
+  void drop(const Step &st) {
+    for (double *o : st.outs) give(o);
+  }
+
+  // Whether step j's reorthogonalisation (mod.rs:777-779) is likely, judged
+  // before step j's scalars arrive: ortbnd's recurrence (mod.rs:815-838) with
+  // alpha_j and rnm_j taken as alpha_{j-1} and beta_j = rnm_{j-1}, against half
+  // the threshold. Only the choice of enqueuing step j + 1 ahead depends on it.
+  bool purge_likely(int64_t j, int64_t ll, double eps1) const {
+    if (j < ll + 2) return false;
+    const double rg = bet[j];
+    if (zero(rg)) return true;
+    const double ag = alf[j - 1];
+    const int64_t cnt = j - (ll + 1);
+    std::vector<double> ne(j + 1, eps1);
+    if (j > 1) {
+      ne[0] = (bet[1] * eta[1] + (alf[0] - ag) * eta[0] - rg * oldeta[0]) / rg + eps1;
+      for (int64_t i = 1; i + 2 <= j; ++i)
+        ne[i] = (bet[i + 1] * eta[i + 1] + (alf[i] - ag) * eta[i] + bet[i] * eta[i - 1] - rg * oldeta[i]) / rg +
+                eps1;
+    }
+    int64_t k = 0;
+    for (int64_t i = 1; i < cnt; ++i)
+      if (std::fabs(ne[i]) > std::fabs(ne[k])) k = i;
+    return std::fabs(ne[k + ll]) > 0.5 * std::sqrt(eps);
+  }
+
+  // mod.rs:674-758. With the pipeline on, step j + 1 is enqueued on the
+  // device (its rnm taken from step j's published slot) before the host
+  // waits for step j's scalars; it is kept when step j's host side leaves
+  // the device state and rnm as they were (no reorthogonalisation, rnm above
+  // tol), else dropped and enqueued again from the host's values. Either way
+  // every step runs the same kernels on the same values as without the
+  // pipeline.
   int64_t lanczos_step(int64_t first, int64_t last, int64_t *ll, bool *enough, double *rnm, double *tol) {
     const double eps1 = eps * std::sqrt((double)ncols);
     int64_t j = first;
+    Step ahead;
+    bool have_ahead = false;
     for (; j < last; ++j) {
-      std::swap(w[1], w[2]);
-      std::swap(w[3], w[4]);
-      // q_{j-1} = w2 is stored by the scaling kernel below (w2 is not written
-      // before it), or right away on the restart and exit paths
-      ensure_slots(j - 1 + kMaxLL + 1);
-      bool stored = false;
-      if (j - 1 < kMaxLL) storp(j - 1, w[4]);
+      Step cur;
       bet[j] = *rnm;
-      if (zero(*rnm)) {
-        storq(j - 1, w[2]);
-        stored = true;
-        startv(j, 0u, rnm);
-        if (zero(*rnm)) *enough = true;
+      if (have_ahead) {
+        cur = ahead;
+        have_ahead = false;
+        commit(cur);
+      } else {
+        double *W[5] = {w[0], w[2], w[1], w[4], w[3]};
+        ensure_slots(j - 1 + kMaxLL + 1);
+        bool stored = false;
+        if (j - 1 < kMaxLL) storp(j - 1, W[4]);
+        if (zero(*rnm)) {
+          storq(j - 1, W[2]);
+          stored = true;
+          std::copy(W, W + 5, w);
+          startv(j, 0u, rnm);
+          if (zero(*rnm)) *enough = true;
+          std::copy(w, w + 5, W);
+        }
+        if (*enough) {
+          if (!stored) storq(j - 1, W[2]);
+          w[0] = W[0];
+          w[1] = W[2];
+          w[2] = W[1];
+          w[3] = W[3];
+          w[4] = W[4];
+          break;
+        }
+        enqueue_body(j, W, *rnm, -1, stored, ll, &cur);
+        commit(cur);
       }
-      if (*enough) {
-        if (!stored) storq(j - 1, w[2]);
-        std::swap(w[1], w[2]);
-        break;
-      }
-      vec_scale2(c, ncols, 1.0 / *rnm, w[0], w[1], w[3], stored ? nullptr : q(j - 1), w[2]);
-      opb(w[3], w[0]);
-      // The step's fused ops as one chain (one launch on short vectors).
-      std::vector<FusedOp> ch;
-      ch.push_back(FusedOp{Axpy{w[0], w[2], -*rnm, -1, 1.0}, 1, w[0], w[3], S_A, nullptr, nullptr});
-      if (j <= kMaxLL) {
-        if (!fused_chain(c, ncols, ch.data(), (int)ch.size())) slots_finish(c, 1);
-        ch.clear();
-        HIP_CHECK(hipStreamSynchronize(c->stream));
-        if (std::fabs(alf[j - 1]) > 4.0 * std::fabs(c->h_scal[S_A])) *ll = j;
-      }
-      // Pending update: w0 -= alf_j w1, applied by the next fused op.
-      Axpy pend{w[0], w[1], 0.0, S_A, -1.0};
       const int64_t lim = std::min(j - 1, *ll);
       for (int64_t i = 0; i < lim; ++i) {
-        const int slot = 8 + (int)i;
-        ch.push_back(FusedOp{pend, 1, p(i), w[0], slot, nullptr, nullptr});
-        pend = Axpy{w[0], q(i), 0.0, slot, -1.0};
         eta[i] = eps1;
         oldeta[i] = eps1;
       }
-      ch.push_back(FusedOp{pend, 1, w[0], w[4], S_B, nullptr, nullptr});
-      ch.push_back(FusedOp{Axpy{w[0], w[2], 0.0, S_B, -1.0}, 1, w[0], w[3], S_C, nullptr, nullptr});
-      ch.push_back(FusedOp{Axpy{w[0], w[1], 0.0, S_C, -1.0}, 1, w[0], w[0], S_E, w[4], w[0]});
-      if (!fused_chain(c, ncols, ch.data(), (int)ch.size())) slots_finish(c, S_E + 1);
-      HIP_CHECK(hipStreamSynchronize(c->stream));
-      const double a = c->h_scal[S_A], t1 = c->h_scal[S_B], t2 = c->h_scal[S_C];
+      if (pipeline && j + 1 < last && j + 1 > kMaxLL && !purge_likely(j, *ll, eps1)) {
+        ensure_slots(j + kMaxLL + 1);
+        HIP_CHECK(hipEventRecord(ev, c->stream));
+        double *W[5] = {w[0], w[2], w[1], w[4], w[3]};
+        enqueue_body(j + 1, W, 0.0, cur.base + S_R, false, ll, &ahead);
+        have_ahead = true;
+      }
+      // with a step behind it, wait for this one's event (a stream
+      // synchronisation would wait for both); else for the stream, which
+      // measured faster than an event wait
+      if (have_ahead)
+        HIP_CHECK(hipEventSynchronize(ev));
+      else
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+      const double *h = c->h_scal + cur.base;
+      const double a = h[S_A], t1 = h[S_B], t2 = h[S_C];
       alf[j] = a;
       if (bet[j] > 0.0) bet[j] += t1;
       alf[j] += t2;
-      *rnm = std::sqrt(c->h_scal[S_E]);
+      *rnm = h[S_R];
       const double anorm = bet[j] + std::fabs(alf[j]) + *rnm;
       *tol = std::sqrt(eps) * anorm;
       ortbnd(j, *rnm, eps1);
-      purge(*ll, j, rnm, *tol);
+      const bool purged = purge(*ll, j, rnm, *tol);
       if (*rnm <= *tol) *rnm = 0.0;
+      if (have_ahead && (purged || *rnm == 0.0)) {
+        drop(ahead);
+        have_ahead = false;
+      }
     }
+    if (have_ahead) drop(ahead);
     return j;
   }
 
@@ -514,11 +639,19 @@ int32_t las2_solve(svdh_ctx *ctx, svdh_op *op, int64_t dimensions, int64_t itera
   L.ncols = L.transposed ? op->nrows : op->ncols;
   L.iterations = iterations;
   const int64_t n = L.ncols;
-  for (int i = 0; i < 5; ++i) {
-    L.wbuf[i].alloc(n * sizeof(double));
-    L.w[i] = L.wbuf[i].as<double>();
-    HIP_CHECK(hipMemsetAsync(L.w[i], 0, n * sizeof(double), ctx->stream));
+  // five working vectors and the outputs of a step and of one enqueued ahead
+  L.vbufs.resize(13);
+  for (int i = 0; i < 13; ++i) {
+    L.vbufs[i].alloc(n * sizeof(double));
+    if (i < 5) {
+      L.w[i] = L.vbufs[i].as<double>();
+      HIP_CHECK(hipMemsetAsync(L.w[i], 0, n * sizeof(double), ctx->stream));
+    } else {
+      L.vfree.push_back(L.vbufs[i].as<double>());
+    }
   }
+  if (const char *pe = std::getenv("SVDH_LAS2_PIPELINE")) L.pipeline = std::atoi(pe) != 0;
+  HIP_CHECK(hipEventCreateWithFlags(&L.ev, hipEventDisableTiming));
   L.tempb.alloc(L.nrows * sizeof(double));
   L.temp = L.tempb.as<double>();
   L.ldq = (n + 31) / 32 * 32;

@@ -178,3 +178,38 @@ def test_device_output_equals_host_output(lib, shape):
     assert np.array_equal(s[:d], rec.s)
     assert np.array_equal(u_dev.numpy()[:d], rec.u) and np.array_equal(vt_dev.numpy()[:d], rec.vt)
     assert bool(meta.diagnostics.transposed) == (n >= 1.2 * m)
+
+
+@pytest.mark.parametrize("case", ["cfg1", "wide", "long_vectors", "rank_deficient", "masked"])
+def test_pipelined_steps_equal_synchronous_steps(lib, case, monkeypatch):
+    # With the pipeline, step j + 1 is enqueued before the host reads step j's
+    # scalars and dropped when the host side changes the state
+    # (reorthogonalisation, rnm <= tol, restarts). Every step must run the same
+    # kernels on the same values as the synchronous driver: equal bits.
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    rng = np.random.default_rng(3)
+    if case == "cfg1":
+        a, k = cfg1_matrix(), 10
+    elif case == "wide":
+        a, k = random_csr(300, 900, 0.03, seed=5), 40
+    elif case == "long_vectors":
+        a, k = random_csr(30_000, 12_000, 0.0005, seed=9), 12   # vectors above the one-workgroup chain
+    elif case == "rank_deficient":
+        b = sp.csr_matrix(rng.normal(size=(400, 12))) @ sp.csr_matrix(rng.normal(size=(12, 300)))
+        a, k = sp.csr_matrix(b), 30                             # invariant subspace: restarts
+    else:
+        a, k = random_csr(2000, 900, 0.02, seed=13), 15
+    op = lib.CsrMatrix.from_scipy(a)
+    if case == "masked":
+        op = MaskedCSRMatrix.with_columns(op, np.sort(rng.choice(900, 500, replace=False)))
+    print(f"case pipeline {case} shape={a.shape} k={k}", flush=True)
+    monkeypatch.setenv("SVDH_LAS2_PIPELINE", "1")
+    r1 = lib.lanczos.svd_dim_seed(op, k, 21)
+    monkeypatch.setenv("SVDH_LAS2_PIPELINE", "0")
+    r0 = lib.lanczos.svd_dim_seed(op, k, 21)
+    assert r1.d == r0.d and r1.d >= 2
+    for key in DIAG_KEYS:
+        assert getattr(r1.diagnostics, key) == getattr(r0.diagnostics, key), key
+    np.testing.assert_array_equal(r1.s, r0.s)
+    np.testing.assert_array_equal(r1.u, r0.u)
+    np.testing.assert_array_equal(r1.vt, r0.vt)
