@@ -576,12 +576,53 @@ struct SpmmArgs {
   int32_t seg_cnt[8];
   int32_t blk_off[8];
   int32_t blk_cnt[8];
+  int32_t off32;  // every panel row's byte offset fits 31 bits (the 16-lane DPP path)
 };
 
 // One fused multiply-add per entry, written out: every entry is added the
 // same way whether it falls in a full batch or in the last one, so the
 // batch size (gathers in flight) and the lanes per row leave the sums
 // unchanged.
+// Lane n of each 16-lane row of the wave, to every lane of that row (DPP
+// row_newbcast, a VALU move instead of an LDS permute); n must fold to a
+// constant once the caller's loops are unrolled.
+__device__ __forceinline__ int row_bcast16(int x, int n) {
+#define SVDH_RB(N) \
+  case N: return __builtin_amdgcn_update_dpp(0, x, 0x150 + N, 0xf, 0xf, false);
+  switch (n) {
+    SVDH_RB(0) SVDH_RB(1) SVDH_RB(2) SVDH_RB(3) SVDH_RB(4) SVDH_RB(5) SVDH_RB(6) SVDH_RB(7)
+    SVDH_RB(8) SVDH_RB(9) SVDH_RB(10) SVDH_RB(11) SVDH_RB(12) SVDH_RB(13) SVDH_RB(14)
+    default: return __builtin_amdgcn_update_dpp(0, x, 0x15f, 0xf, 0xf, false);
+  }
+#undef SVDH_RB
+}
+__device__ __forceinline__ float row_bcast16(float x, int n) { return __int_as_float(row_bcast16(__float_as_int(x), n)); }
+// The same broadcast as a plain DPP move (every lane is written, so no prior
+// value is kept: the compiler can fold it into the instruction that uses it).
+__device__ __forceinline__ int row_bcast16m(int x, int n) {
+#define SVDH_RB(N) \
+  case N: return __builtin_amdgcn_mov_dpp(x, 0x150 + N, 0xf, 0xf, false);
+  switch (n) {
+    SVDH_RB(0) SVDH_RB(1) SVDH_RB(2) SVDH_RB(3) SVDH_RB(4) SVDH_RB(5) SVDH_RB(6) SVDH_RB(7)
+    SVDH_RB(8) SVDH_RB(9) SVDH_RB(10) SVDH_RB(11) SVDH_RB(12) SVDH_RB(13) SVDH_RB(14)
+    default: return __builtin_amdgcn_mov_dpp(x, 0x15f, 0xf, 0xf, false);
+  }
+#undef SVDH_RB
+}
+__device__ __forceinline__ float row_bcast16m(float x, int n) {
+  return __int_as_float(row_bcast16m(__float_as_int(x), n));
+}
+__device__ __forceinline__ double row_bcast16m(double x, int n) {
+  const long long b = __double_as_longlong(x);
+  const int lo = row_bcast16m((int)(b & 0xffffffffll), n), hi = row_bcast16m((int)(b >> 32), n);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double row_bcast16(double x, int n) {
+  const long long b = __double_as_longlong(x);
+  const int lo = row_bcast16((int)(b & 0xffffffffll), n), hi = row_bcast16((int)(b >> 32), n);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 __device__ __forceinline__ float fma_acc(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_acc(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
@@ -590,6 +631,60 @@ template <int R, bool MC, int UU = SVDH_SPMM_UNROLL, typename V, typename X, typ
 __device__ __forceinline__ void spmm_accum(const SpmmArgs<V, X, Y, Acc> &a, int kb, int ke, int lane, int col,
                                            bool colok, Acc (&acc)[PanelVec<X>::N]) {
   constexpr int NV = PanelVec<X>::N;
+  if constexpr (R == 16) {
+    if (a.off32) {
+      // 16 lanes per row: the lane that loads an entry computes the byte
+      // offset of its panel row, and DPP row broadcasts (VALU moves) hand
+      // offset and value to the row's lanes; the same products in the same
+      // order as the general path below, so the results are the same bits.
+      const uint32_t rowb = (uint32_t)(a.ldx * (int64_t)sizeof(X));
+      const uint32_t colb = (uint32_t)(col * (int)sizeof(X));
+      const char *xb = reinterpret_cast<const char *>(a.x);
+      constexpr int U = UU < R ? UU : R;
+      for (int base = kb; base < ke; base += R) {
+        int c = 0;
+        Acc v = Acc(0);
+        const int kk = base + lane;
+        if (kk < ke) {
+          c = __builtin_nontemporal_load(&a.colidx[kk]);
+          v = ld_as<Acc>(a.vals, kk);
+          if (MC) {
+            c = a.colmap[c];
+            if (c < 0) {
+              c = 0;
+              v = Acc(0);
+            }
+          }
+        }
+        const int n = min(R, ke - base);
+        const int off = (int)((uint32_t)c * rowb);
+#pragma unroll
+        for (int jb = 0; jb < R / U; ++jb) {
+          if (jb * U >= n) break;
+          uint32_t oj[U];
+          Acc vj[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            oj[u] = (uint32_t)row_bcast16m(off, jb * U + u);
+            vj[u] = row_bcast16m(v, jb * U + u);
+          }
+          if (colok) {
+            Acc xv[U][NV];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+              ld_panel<Acc, X>(reinterpret_cast<const X *>(xb + (size_t)(oj[u] + colb)), xv[u]);
+            // lanes past the end hold offset 0 and value 0: their fused
+            // products add exactly nothing
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+              for (int i = 0; i < NV; ++i) acc[i] = fma_acc(vj[u], xv[u][i], acc[i]);
+          }
+        }
+      }
+      return;
+    }
+  }
   for (int base = kb; base < ke; base += R) {
     int c = 0;
     Acc v = Acc(0);
@@ -904,26 +999,6 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin_batched(const FinArgs<Y, Ac
 // ------------------------------------------------------------------------
 constexpr int kHotBlock = 1024;
 
-// Lane n of each 16-lane row of the wave, to every lane of that row (DPP
-// row_newbcast, a VALU move instead of an LDS permute); n must fold to a
-// constant once the caller's loops are unrolled.
-__device__ __forceinline__ int row_bcast16(int x, int n) {
-#define SVDH_RB(N) \
-  case N: return __builtin_amdgcn_update_dpp(0, x, 0x150 + N, 0xf, 0xf, false);
-  switch (n) {
-    SVDH_RB(0) SVDH_RB(1) SVDH_RB(2) SVDH_RB(3) SVDH_RB(4) SVDH_RB(5) SVDH_RB(6) SVDH_RB(7)
-    SVDH_RB(8) SVDH_RB(9) SVDH_RB(10) SVDH_RB(11) SVDH_RB(12) SVDH_RB(13) SVDH_RB(14)
-    default: return __builtin_amdgcn_update_dpp(0, x, 0x15f, 0xf, 0xf, false);
-  }
-#undef SVDH_RB
-}
-__device__ __forceinline__ float row_bcast16(float x, int n) { return __int_as_float(row_bcast16(__float_as_int(x), n)); }
-__device__ __forceinline__ double row_bcast16(double x, int n) {
-  const long long b = __double_as_longlong(x);
-  const int lo = row_bcast16((int)(b & 0xffffffffll), n), hi = row_bcast16((int)(b >> 32), n);
-  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
 // acc += sum over k in [kb, ke) of vals[k] * panel row of hcol[k]: staged
 // rows from LDS (LDS_SRC, hcol = -(1 + slot)) or panel rows from global memory.
 // With 16 lanes per row the entries reach the lanes by row broadcasts, with
@@ -934,6 +1009,13 @@ __device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, 
                                                bool colok, Acc (&acc)[PanelVec<X>::N]) {
   constexpr int NV = PanelVec<X>::N;
   const X *src = LDS_SRC ? xs : a.x;
+  // 16 lanes per row: byte offsets of the entries' panel rows are computed by
+  // the lane that loaded the entry (32-bit: the launch checks that the panel
+  // fits) and reach the row's lanes by DPP broadcasts, so an entry costs two
+  // moves, one add, one load and the FMAs.
+  const uint32_t rowb = (uint32_t)(a.ldx * (int64_t)sizeof(X));
+  const uint32_t colb = (uint32_t)(col * (int)sizeof(X));
+  const char *srcb = reinterpret_cast<const char *>(src);
   for (int base = kb; base < ke; base += R) {
     int c = 0;
     Acc v = Acc(0);
@@ -947,27 +1029,40 @@ __device__ __forceinline__ void spmm_accum_hot(const SpmmArgs<V, X, Y, Acc> &a, 
     constexpr int U = SVDH_SPMM_UNROLL < R ? SVDH_SPMM_UNROLL : R;
     int j = 0;
     if constexpr (R == 16) {
+      const int off = (int)((uint32_t)c * rowb);
 #pragma unroll
       for (int jb = 0; jb < R / U; ++jb) {
-        if (jb * U + U > n) break;
-        int cj[U];
+        if (jb * U >= n) break;
+        uint32_t oj[U];
         Acc vj[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          cj[u] = row_bcast16(c, jb * U + u);
-          vj[u] = row_bcast16(v, jb * U + u);
+          oj[u] = (uint32_t)row_bcast16m(off, jb * U + u);
+          vj[u] = row_bcast16m(v, jb * U + u);
         }
         if (colok) {
           Acc xv[U][NV];
 #pragma unroll
-          for (int u = 0; u < U; ++u) ld_panel<Acc, X>(src + (int64_t)cj[u] * a.ldx + col, xv[u]);
-#pragma unroll
           for (int u = 0; u < U; ++u)
+            ld_panel<Acc, X>(reinterpret_cast<const X *>(srcb + (size_t)(oj[u] + colb)), xv[u]);
+          if (jb * U + U <= n) {
 #pragma unroll
-            for (int i = 0; i < NV; ++i) acc[i] += vj[u] * xv[u][i];
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+              for (int i = 0; i < NV; ++i) acc[i] = fma_acc(vj[u], xv[u][i], acc[i]);
+          } else {
+            // the last entries of the row: products past the end dropped by a select
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+              for (int i = 0; i < NV; ++i) {
+                const Acc t = fma_acc(vj[u], xv[u][i], acc[i]);
+                acc[i] = jb * U + u < n ? t : acc[i];
+              }
+          }
         }
-        j += U;
       }
+      j = n;
     } else {
       for (; j + U <= n; j += U) {
         int cj[U];
@@ -2055,6 +2150,7 @@ static bool plan_spmm(svdh_op *op, int direction, const void *vals, const void *
     a.blk_off[x] = s.blk_off[x];
     a.blk_cnt[x] = x < s.nblk ? s.blk_cnt[x] : 0;
   }
+  a.off32 = (uint64_t)std::max<int64_t>(v.ncols, 1) * (uint64_t)ldx * sizeof(X) < (1ull << 31) ? 1 : 0;
   p.s = &s;
   p.lgr = lgr;
   p.blocks = blocks;
@@ -2249,7 +2345,8 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   Ctx *c = op->ctx;
   SpmmPlan<V, X, Y, Acc> p;
   if (!plan_spmm(op, direction, vals, X_, ldx, Y_, ldy, l, p)) return;
-  if (direction == 0 && !p.mc) {
+  // (the staged path addresses panel rows by 32-bit byte offsets)
+  if (direction == 0 && !p.mc && (uint64_t)op->ncols * (uint64_t)ldx * sizeof(X) < (1ull << 31)) {
     static const int64_t hot_lds = [] {  // LDS bytes for the staged panel rows (SVDH_SPMM_HOT_LDS)
       const char *e = std::getenv("SVDH_SPMM_HOT_LDS");
       return e ? std::max<int64_t>(4096, std::min<int64_t>(155648, std::atoll(e))) : 65536;
