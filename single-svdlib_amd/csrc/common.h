@@ -329,7 +329,7 @@ void slots_finish(Ctx *c, int n);
 // range) also sqrt(value of sqrt_from) as the single partial of sqrt_to.
 void slots_finish_range(Ctx *c, int first, int n, int sqrt_from, int sqrt_to);
 // Slots slot_a and slot_b get sum |a_i| and sum |b_i| over i < k (device
-// vectors), each summed in index order.
+// vectors), each summed in one fixed order.
 void abs_sums_to_slots(Ctx *c, int64_t k, const double *a, const double *b, int slot_a, int slot_b);
 // y = s x, zout = s zin (and copy_dst = copy_src) with s = 1 / (value of
 // rnm_slot, a slot holding one published value) or, with rnm_slot < 0, s = a.

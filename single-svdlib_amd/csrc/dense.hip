@@ -204,17 +204,21 @@ __global__ void k_scale2(int64_t n, double a, const double *x, double *y, double
   }
 }
 
-// slot values sum |a_i| and sum |b_i| (i < k), each added in index order by
-// one thread (the order of a sequential host loop).
-__global__ void k_abs_sums(int64_t k, const double *a, const double *b, double *pa, double *pb) {
-  if (threadIdx.x != 0) return;
+// slot values sum |a_i| and sum |b_i| (i < k): one workgroup, thread t adds
+// the entries t, t + 256, ... and the threads' sums are added in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_abs_sums(int64_t k, const double *a, const double *b, double *pa,
+                                                    double *pb) {
   double sa = 0.0, sb = 0.0;
-  for (int64_t i = 0; i < k; ++i) {
+  for (int64_t i = threadIdx.x; i < k; i += kBlock) {
     sa += fabs(a[i]);
     sb += fabs(b[i]);
   }
-  *pa = sa;
-  *pb = sb;
+  sa = block_sum(sa);
+  sb = block_sum(sb);
+  if (threadIdx.x == 0) {
+    *pa = sa;
+    *pb = sb;
+  }
 }
 
 // The Lanczos step's scaling, out of place: s = 1 / rnm with rnm the single
@@ -1012,7 +1016,7 @@ void abs_sums_to_slots(Ctx *c, int64_t k, const double *a, const double *b, int 
     fail(SVDH_ERR_ARG, "scalar slots %d, %d out of range", slot_a, slot_b);
   c->slot_parts.ensure((size_t)S_NSLOT * kSlotParts * sizeof(double));
   double *parts = c->slot_parts.as<double>();
-  hipLaunchKernelGGL(k_abs_sums, 1, 64, 0, c->stream, k, a, b, parts + (size_t)slot_a * kSlotParts,
+  hipLaunchKernelGGL(k_abs_sums, 1, kBlock, 0, c->stream, k, a, b, parts + (size_t)slot_a * kSlotParts,
                      parts + (size_t)slot_b * kSlotParts);
   HIP_CHECK(hipGetLastError());
   c->slot_nb[slot_a] = 1;

@@ -217,7 +217,7 @@ struct Las2 {
       fused_axpy_dot(c, ncols, nullptr, 0, w[0], w[1], S_A, w[3], w[1]);
       Axpy up{w[0], w[1], 0.0, S_A, -1.0};
       fused_axpy_dot(c, ncols, &up, 1, w[0], w[0], S_B, w[4], w[0]);
-      // sum |coefficients| on the device, in the order of the host loop
+      // sum |coefficients| on the device
       abs_sums_to_slots(c, cnt, cq, cr, S_C, S_D);
       read_slots(4);
       const double tq = c->h_scal[S_C];
