@@ -99,6 +99,14 @@ typedef struct {
   int64_t power_iters_run;
   double residual_estimate;
   int64_t qr_fallbacks;
+  /* Solves in SVDH_PREC_F32 or SVDH_PREC_BF16 hand their host arrays over
+   * in single precision, as SvdRec<f32> holds them (utils.rs:43-50): u32,
+   * s32 and vt32 (same shapes) are set and u, s, vt stay NULL; FP64 solves
+   * set u, s, vt and leave these NULL. The values are the FP64 results
+   * rounded to nearest. Freed by svdh_result_free as well. */
+  float *u32;
+  float *s32;
+  float *vt32;
 } svdh_result;
 
 /* ---- context ---------------------------------------------------------- */
@@ -345,7 +353,7 @@ typedef struct {
  * CSR (grad has nnz entries of grad_dtype, in CSR order of the underlying
  * matrix; for a masked view the dropped columns get 0). res holds the top
  * triplets of a solve of the same operator (svdh_rsvd or svdh_las2; either u
- * layout is accepted); k <= min(res->d, 64). The projection loss solves k
+ * layout, FP64 or single-precision arrays); k <= min(res->d, 64). The projection loss solves k
  * shifted systems (s_i^2 I - A A^T) w_i = (I - U_k U_k^T) y by conjugate
  * gradients to the relative residual tol (<= 0: 1e-10 in FP64, 1e-5 in FP32)
  * within max_iters (<= 0: 1000) iterations. precision: SVDH_PREC_F64,

@@ -68,7 +68,12 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
     fail(SVDH_ERR_ARG, "bad precision %d", precision);
   if (grad_dtype != SVDH_F64 && grad_dtype != SVDH_F32) fail(SVDH_ERR_ARG, "bad gradient dtype %d", grad_dtype);
   if (loss == SVDH_LOSS_PROJECTION && !y) fail(SVDH_ERR_ARG, "the projection loss needs y");
-  if (!res->u || !res->s || !res->vt) fail(SVDH_ERR_ARG, "result without factors");
+  // factors in FP64 (u, s, vt) or single precision (u32, s32, vt32)
+  const bool r64 = res->u && res->s && res->vt, r32 = res->u32 && res->s32 && res->vt32;
+  if (!r64 && !r32) fail(SVDH_ERR_ARG, "result without factors");
+  auto s_at = [&](int64_t i) { return r64 ? res->s[i] : (double)res->s32[i]; };
+  auto u_at = [&](int64_t i) { return r64 ? res->u[i] : (double)res->u32[i]; };
+  auto vt_at = [&](int64_t i) { return r64 ? res->vt[i] : (double)res->vt32[i]; };
   // u is m x d (randomized layout) or d x m (LAS2 layout, rows are left vectors)
   bool u_rows_are_vectors;
   if (res->u_rows == m && res->u_cols == d)
@@ -99,18 +104,18 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
   };
   const int sdt = bf ? SVDH_BF16 : dt;
   auto uget = [&](int64_t i, int64_t j) {
-    return u_rows_are_vectors ? res->u[j * m + i] : res->u[i * d + j];
+    return u_rows_are_vectors ? u_at(j * m + i) : u_at(i * d + j);
   };
   DevBuf U, V, G;
   upload_panel(c, U, m, k, ld, dt, uget);
-  upload_panel(c, V, n, k, ld, dt, [&](int64_t i, int64_t j) { return res->vt[j * n + i]; });
+  upload_panel(c, V, n, k, ld, dt, [&](int64_t i, int64_t j) { return vt_at(j * n + i); });
   const int64_t nnz = op->base ? op->base->nnz : op->nnz;
   G.alloc(std::max<size_t>((size_t)nnz * dtype_size(grad_dtype), 16));
   std::memset(info, 0, sizeof *info);
 
   if (loss == SVDH_LOSS_SIGMA_SUM) {
     double L = 0.0;
-    for (int64_t i = 0; i < k; ++i) L += res->s[i];
+    for (int64_t i = 0; i < k; ++i) L += s_at(i);
     info->loss = L;
     sddmm(op, k, sddmm_factor(U.p, m), sddmm_factor(V.p, n), nullptr, nullptr, nullptr, nullptr, ld, ld, sdt, G.p,
           grad_dtype);
@@ -145,7 +150,7 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
     double *shifts = scal.as<double>(), *neg1 = shifts + 64, *dots = shifts + 128, *rr = shifts + 192,
            *Cx = shifts + 256;
     std::vector<double> h(64, 0.0);
-    for (int64_t j = 0; j < k; ++j) h[j] = res->s[j] * res->s[j];
+    for (int64_t j = 0; j < k; ++j) h[j] = s_at(j) * s_at(j);
     HIP_CHECK(hipMemcpyAsync(shifts, h.data(), 64 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     std::vector<double> hm1(64, -1.0);
     HIP_CHECK(hipMemcpyAsync(neg1, hm1.data(), 64 * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -200,7 +205,7 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
     spmm(op, 1, dt, W.p, ld, dt, Z.p, ld, dt, k);  // Z = A^T W
     std::vector<double> s1(64, 0.0), s2(64, 0.0);
     for (int64_t j = 0; j < k; ++j) {
-      s1[j] = 2.0 * cy[j] * res->s[j];
+      s1[j] = 2.0 * cy[j] * s_at(j);
       s2[j] = 2.0 * cy[j];
     }
     const double *s1_d = static_cast<const double *>(c->upload(s1.data(), 64 * sizeof(double)));

@@ -30,8 +30,11 @@ size_t g_pool_free_bytes = 0;
 constexpr size_t kPoolKeep = size_t(4) << 30;
 }  // namespace
 
-double *result_alloc(int64_t count) {
-  const size_t bytes = std::max<size_t>((size_t)std::max<int64_t>(count, 1) * sizeof(double), 64);
+double *result_alloc(int64_t count) { return static_cast<double *>(result_alloc_bytes((size_t)count * sizeof(double))); }
+float *result_alloc_f32(int64_t count) { return static_cast<float *>(result_alloc_bytes((size_t)count * sizeof(float))); }
+
+void *result_alloc_bytes(size_t want) {
+  const size_t bytes = std::max<size_t>(std::max<size_t>(want, 8), 64);
   {
     std::lock_guard<std::mutex> g(g_pool_mu);
     auto it = g_pool_free.lower_bound(bytes);
@@ -40,7 +43,7 @@ double *result_alloc(int64_t count) {
       g_pool_free_bytes -= it->first;
       g_pool_live[p] = it->first;
       g_pool_free.erase(it);
-      return static_cast<double *>(p);
+      return p;
     }
   }
   void *p = nullptr;
@@ -48,11 +51,11 @@ double *result_alloc(int64_t count) {
     (void)hipGetLastError();
     p = std::malloc(bytes);  // unpinned: still correct, the copy is slower
     if (!p) fail(SVDH_ERR_ALLOC, "result allocation of %zu bytes failed", bytes);
-    return static_cast<double *>(p);
+    return p;
   }
   std::lock_guard<std::mutex> g(g_pool_mu);
   g_pool_live[p] = bytes;
-  return static_cast<double *>(p);
+  return p;
 }
 
 void result_release(void *p) {
@@ -714,7 +717,11 @@ void svdh_result_free(svdh_result *res) {
   result_release(res->u);
   result_release(res->s);
   result_release(res->vt);
+  result_release(res->u32);
+  result_release(res->s32);
+  result_release(res->vt32);
   res->u = res->s = res->vt = nullptr;
+  res->u32 = res->s32 = res->vt32 = nullptr;
 }
 
 }  // extern "C"

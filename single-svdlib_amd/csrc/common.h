@@ -74,7 +74,8 @@ struct Ctx {
   DevBuf stage;
   // Device workspaces kept for the context's lifetime (panels of the
   // randomized solver), grown on demand; slot meanings belong to the caller.
-  static constexpr int kWorkspaces = 13;  // 0-11: randomized solver panels, 12: basis GEMV partials
+  static constexpr int kWorkspaces = 14;  // 0-11: randomized solver panels, 12: basis GEMV partials,
+                                          // 13: single-precision copies of host results
   DevBuf ws[kWorkspaces];
   void *workspace(int slot, size_t bytes) {
     if (bytes > ws[slot].bytes) {
@@ -355,6 +356,8 @@ bool fused_chain(Ctx *c, int64_t n, const FusedOp *ops, int nops);
 void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double *z, double *copy_dst = nullptr,
                 const double *copy_src = nullptr);
 void vec_copy(Ctx *c, int64_t n, const double *x, double *y);
+// y = x rounded to single precision (round to nearest), on the stream.
+void convert_f64_f32(Ctx *c, int64_t n, const double *x, float *y);
 // Basis GEMV: cout[2k] = Q[:, c0:c0+k]^T [u v]  (k <= 4096)
 void basis_gemv_t(Ctx *c, int64_t n, const double *Q, int64_t ldq, int64_t k, const double *u,
                   const double *v, double *cu, double *cv);
@@ -471,6 +474,8 @@ size_t dtype_size(int dt);
 // Host arrays of a result (SvdRec u, s, vt): pinned buffers from a pool that
 // result_release (svdh_result_free) refills; contents are not cleared.
 double *result_alloc(int64_t count);
+float *result_alloc_f32(int64_t count);
+void *result_alloc_bytes(size_t bytes);
 void result_release(void *p);
 
 }  // namespace svdh

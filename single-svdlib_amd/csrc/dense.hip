@@ -234,6 +234,11 @@ __global__ void k_scale_step(int64_t n, const double *rpart, double a, const dou
   }
 }
 
+__global__ void k_f64_to_f32(int64_t n, const double *x, float *y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
 __global__ void k_copy(int64_t n, const double *x, double *y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = x[i];
 }
@@ -1041,6 +1046,12 @@ void vec_scale2(Ctx *c, int64_t n, double a, const double *x, double *y, double 
                 const double *copy_src) {
   KTimer t(c, KC_BLAS1, (double)n * (32.0 + (copy_dst ? 16.0 : 0.0)));
   hipLaunchKernelGGL(k_scale2, grid_of(n), kBlock, 0, c->stream, n, a, x, y, z, copy_dst, copy_src);
+  HIP_CHECK(hipGetLastError());
+}
+
+void convert_f64_f32(Ctx *c, int64_t n, const double *x, float *y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_f64_to_f32, grid_of(n), kBlock, 0, c->stream, n, x, y);
   HIP_CHECK(hipGetLastError());
 }
 

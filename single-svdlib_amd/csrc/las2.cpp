@@ -539,11 +539,16 @@ struct Las2 {
     out->u_cols = transposed ? ncols : nrows;
     out->vt_rows = d;
     out->vt_cols = transposed ? nrows : ncols;
-    if (!dev) {
+    // FP32-value solves hand over single-precision host arrays (SvdRec<f32>)
+    const bool single = vdt != SVDH_F64;
+    if (!dev && !single) {
       out->s = result_alloc(d);
       out->u = result_alloc(d * out->u_cols);
       out->vt = result_alloc(d * out->vt_cols);
-      if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
+    } else if (!dev) {
+      out->s32 = result_alloc_f32(d);
+      out->u32 = result_alloc_f32(d * out->u_cols);
+      out->vt32 = result_alloc_f32(d * out->vt_cols);
     }
     if (d == 0) return;
     // Coefficients W (js x d): column r is the eigenvector of T for the r-th
@@ -576,10 +581,14 @@ struct Las2 {
     HIP_CHECK(hipMemcpyAsync(hd.data(), dots, d * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     std::vector<double> hs(d);
-    double *s_out = dev ? dev->s : out->s;
     for (int64_t r = 0; r < d; ++r) {
       const double sval = std::sqrt(std::max(hd[r], 0.0));
-      s_out[r] = sval;
+      if (dev)
+        dev->s[r] = sval;
+      else if (single)
+        out->s32[r] = (float)sval;
+      else
+        out->s[r] = sval;
       hs[r] = legacy || sval > aeps ? 1.0 / sval : 1.0 / std::max(sval, aeps);
     }
     HIP_CHECK(hipMemcpyAsync(scale, hs.data(), d * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -595,15 +604,25 @@ struct Las2 {
       return;
     }
     // The working (possibly transposed) layout: Ut is d x nrows, Vt is d x ncols.
-    double *Ut = transposed ? out->vt : out->u;
-    double *Vt = transposed ? out->u : out->vt;
-    DevBuf T((size_t)d * std::max(nrows, ncols) * sizeof(double));
+    void *Ut = transposed ? (single ? (void *)out->vt32 : (void *)out->vt) : (single ? (void *)out->u32 : (void *)out->u);
+    void *Vt = transposed ? (single ? (void *)out->u32 : (void *)out->u) : (single ? (void *)out->vt32 : (void *)out->vt);
+    const size_t esz = single ? sizeof(float) : sizeof(double);
+    const int64_t big = std::max(nrows, ncols);
+    DevBuf T((size_t)d * big * sizeof(double) + (single ? (size_t)d * big * sizeof(float) : 0));
+    float *Tf = single ? reinterpret_cast<float *>(T.as<double>() + d * big) : nullptr;
+    auto hand_over = [&](int64_t len, void *dst) {
+      if (single) {
+        convert_f64_f32(c, d * len, T.as<double>(), Tf);
+        HIP_CHECK(hipMemcpyAsync(dst, Tf, (size_t)d * len * esz, hipMemcpyDeviceToHost, c->stream));
+      } else {
+        HIP_CHECK(hipMemcpyAsync(dst, T.p, (size_t)d * len * esz, hipMemcpyDeviceToHost, c->stream));
+      }
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    };
     panel_transpose_scale(c, nrows, d, AV.as<double>(), ldv, scale, T.as<double>(), nrows);
-    HIP_CHECK(hipMemcpyAsync(Ut, T.p, (size_t)d * nrows * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    hand_over(nrows, Ut);
     panel_transpose_scale(c, ncols, d, V.as<double>(), ldv, nullptr, T.as<double>(), ncols);
-    HIP_CHECK(hipMemcpyAsync(Vt, T.p, (size_t)d * ncols * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    hand_over(ncols, Vt);
   }
 };
 

@@ -189,6 +189,33 @@ struct Rsvd {
 
 }  // namespace
 
+// The host hand-over of a result: s (k values), U (nu values) and V^T (nv
+// values, FP64 on the device) into pooled pinned arrays, in FP64 for FP64
+// solves and rounded to FP32 on the device (half the bytes over PCIe) for
+// FP32 and BF16 solves, as SvdRec<f32> holds them.
+static void host_results(Ctx *c, int32_t precision, const double *sv, int64_t k, const double *ud, int64_t nu,
+                         const double *vtd, int64_t nv, svdh_result *out) {
+  if (precision == SVDH_PREC_F64) {
+    out->s = result_alloc(k);
+    out->u = result_alloc(nu);
+    out->vt = result_alloc(nv);
+    for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
+    if (nu > 0) HIP_CHECK(hipMemcpyAsync(out->u, ud, (size_t)nu * 8, hipMemcpyDeviceToHost, c->stream));
+    if (nv > 0) HIP_CHECK(hipMemcpyAsync(out->vt, vtd, (size_t)nv * 8, hipMemcpyDeviceToHost, c->stream));
+    return;
+  }
+  out->s32 = result_alloc_f32(k);
+  out->u32 = result_alloc_f32(nu);
+  out->vt32 = result_alloc_f32(nv);
+  for (int64_t r = 0; r < k; ++r) out->s32[r] = (float)sv[r];
+  if (nu + nv == 0) return;
+  float *f = static_cast<float *>(c->workspace(13, (size_t)(nu + nv) * sizeof(float)));
+  convert_f64_f32(c, nu, ud, f);
+  convert_f64_f32(c, nv, vtd, f + nu);
+  if (nu > 0) HIP_CHECK(hipMemcpyAsync(out->u32, f, (size_t)nu * 4, hipMemcpyDeviceToHost, c->stream));
+  if (nv > 0) HIP_CHECK(hipMemcpyAsync(out->vt32, f + nu, (size_t)nv * 4, hipMemcpyDeviceToHost, c->stream));
+}
+
 int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_oversamples, int64_t n_power_iters,
                    int32_t normalizer, int32_t mean_center, uint64_t seed, int32_t has_seed, const double *omega,
                    int32_t precision, double tol, int64_t max_power_iters, svdh_result *out,
@@ -558,15 +585,7 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
   if (dev) {
     for (int64_t r = 0; r < k; ++r) dev->s[r] = sv[r];
   } else {
-    out->s = result_alloc(k);
-    out->u = result_alloc(m * k);
-    out->vt = result_alloc(k * n);
-    if (!out->s || !out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
-    for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
-    if (k > 0) {
-      HIP_CHECK(hipMemcpyAsync(out->u, ud, (size_t)m * k * 8, hipMemcpyDeviceToHost, ctx->stream));
-      HIP_CHECK(hipMemcpyAsync(out->vt, vtd, (size_t)k * n * 8, hipMemcpyDeviceToHost, ctx->stream));
-    }
+    host_results(ctx, precision, sv.data(), k, ud, m * k, vtd, k * n, out);
   }
   HIP_CHECK(hipStreamSynchronize(ctx->stream));
 
@@ -754,10 +773,10 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
     out->u_cols = k;
     out->vt_rows = k;
     out->vt_cols = n;
-    out->s = result_alloc(k);
-    if (!out->s) fail(SVDH_ERR_ALLOC, "result allocation failed");
-    for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
-    if (dev) {  // U and V^T straight into the caller's device buffers
+    if (dev) {  // U and V^T straight into the caller's device buffers; s on the host in FP64
+      out->s = result_alloc(k);
+      if (!out->s) fail(SVDH_ERR_ALLOC, "result allocation failed");
+      for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
       if (dev->ldu[b] < k || dev->ldvt[b] < n) fail(SVDH_ERR_ARG, "leading dimension too small for operator %lld",
                                                     (long long)b);
       panel_apply(c, m, l, k, Y[b], ld, pdt, MW.data(), dev->u[b], dev->ldu[b], SVDH_F64);
@@ -767,13 +786,7 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
       double *vtd = static_cast<double *>(ctx->workspace(WS_VT, std::max<size_t>(k * n, 1) * 8));
       panel_apply(c, m, l, k, Y[b], ld, pdt, MW.data(), ud, k, SVDH_F64);
       panel_apply(c, n, w, k, Z[b], ld, pdt, Ws.data(), vtd, 1, SVDH_F64, n);
-      out->u = result_alloc(m * k);
-      out->vt = result_alloc(k * n);
-      if (!out->u || !out->vt) fail(SVDH_ERR_ALLOC, "result allocation failed");
-      if (k > 0) {
-        HIP_CHECK(hipMemcpyAsync(out->u, ud, (size_t)m * k * 8, hipMemcpyDeviceToHost, c->stream));
-        HIP_CHECK(hipMemcpyAsync(out->vt, vtd, (size_t)k * n * 8, hipMemcpyDeviceToHost, c->stream));
-      }
+      host_results(c, precision, sv.data(), k, ud, m * k, vtd, k * n, out);
       HIP_CHECK(hipStreamSynchronize(c->stream));  // U/V workspaces are reused by the next operator
     }
     svdh_diagnostics &dg = out->diagnostics;  // randomized/mod.rs:214-237

@@ -52,6 +52,9 @@ class Result_(C.Structure):
         ("power_iters_run", C.c_int64),
         ("residual_estimate", C.c_double),
         ("qr_fallbacks", C.c_int64),
+        ("u32", C.POINTER(C.c_float)),
+        ("s32", C.POINTER(C.c_float)),
+        ("vt32", C.POINTER(C.c_float)),
     ]
 
 

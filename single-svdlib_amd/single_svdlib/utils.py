@@ -94,10 +94,15 @@ class SMat(Protocol):
 
 
 def result_to_svdrec(res, dtype, extra=None):
+    """svdh_result -> SvdRec with arrays of dtype; the host arrays are FP64
+    (u, s, vt) or, for FP32 / BF16 solves, single precision (u32, s32, vt32)."""
     d = res.d
-    u = np.ctypeslib.as_array(res.u, (max(res.u_rows * res.u_cols, 1),))[: res.u_rows * res.u_cols]
-    vt = np.ctypeslib.as_array(res.vt, (max(res.vt_rows * res.vt_cols, 1),))[: res.vt_rows * res.vt_cols]
-    s = np.ctypeslib.as_array(res.s, (max(d, 1),))[:d]
+    single = bool(res.u32) or bool(res.s32)
+    pu, ps, pvt = (res.u32, res.s32, res.vt32) if single else (res.u, res.s, res.vt)
+    nu, nvt = res.u_rows * res.u_cols, res.vt_rows * res.vt_cols
+    u = np.ctypeslib.as_array(pu, (max(nu, 1),))[:nu] if pu else np.zeros(nu)
+    vt = np.ctypeslib.as_array(pvt, (max(nvt, 1),))[:nvt] if pvt else np.zeros(nvt)
+    s = np.ctypeslib.as_array(ps, (max(d, 1),))[:d] if ps else np.zeros(d)
     dg = res.diagnostics
     diag = Diagnostics(
         non_zero=dg.non_zero, dimensions=dg.dimensions, iterations=dg.iterations, transposed=bool(dg.transposed),
