@@ -213,3 +213,30 @@ def test_pipelined_steps_equal_synchronous_steps(lib, case, monkeypatch):
     np.testing.assert_array_equal(r1.s, r0.s)
     np.testing.assert_array_equal(r1.u, r0.u)
     np.testing.assert_array_equal(r1.vt, r0.vt)
+
+
+def test_f32_solve_hands_over_single_precision_arrays(lib):
+    # svd_las2::<f32> returns SvdRec<f32>: u32 / s32 / vt32, the FP64 device
+    # results of the same solve rounded to nearest, bit for bit
+    import ctypes as C
+    from single_svdlib import _ffi
+    a = random_csr(700, 400, 0.03, seed=43).astype(np.float32)
+    op = lib.CsrMatrix.from_scipy(a)
+    res = _ffi.Result_()
+    _ffi.check(_ffi.lib().svdh_las2(op._ctx, op._handle, 8, 0, -1e-30, 1e-30, 1e-6, 77, None, _ffi.PREC_F32,
+                                    C.byref(res)))
+    d = res.d
+    assert d >= 2 and not res.u and not res.s and res.u32 and res.s32 and res.vt32
+    u32 = np.ctypeslib.as_array(res.u32, (d * 700,)).reshape(d, 700).copy()
+    s32 = np.ctypeslib.as_array(res.s32, (d,)).copy()
+    vt32 = np.ctypeslib.as_array(res.vt32, (d * 400,)).reshape(d, 400).copy()
+    _ffi.lib().svdh_result_free(C.byref(res))
+    u_dev, vt_dev = _ffi.DeviceArray((8, 700)), _ffi.DeviceArray((8, 400))
+    s = np.zeros(8)
+    meta = _ffi.Result_()
+    _ffi.check(_ffi.lib().svdh_las2_device(op._ctx, op._handle, 8, 0, -1e-30, 1e-30, 1e-6, 77, None, _ffi.PREC_F32,
+                                           u_dev.ptr, 700, _ffi.ptr(s), vt_dev.ptr, 400, C.byref(meta)))
+    assert meta.d == d
+    np.testing.assert_array_equal(s32, s[:d].astype(np.float32))
+    np.testing.assert_array_equal(u32, u_dev.numpy()[:d].astype(np.float32))
+    np.testing.assert_array_equal(vt32, vt_dev.numpy()[:d].astype(np.float32))

@@ -267,3 +267,36 @@ def test_graded_spectrum_matches_oracle(lib, span):
     assert max_principal_angle(rec.u, u) < 1e-8
     assert max_principal_angle(rec.vt.T, vt.T) < 1e-8
     assert rec.extra["qr_fallbacks"] == 0
+
+
+def test_single_precision_host_arrays(lib):
+    # FP32 solves hand over u32 / s32 / vt32 (u, s, vt NULL): the FP64 device
+    # results of the same solve rounded to nearest, bit for bit
+    import ctypes as C
+    from single_svdlib import _ffi
+    a = random_csr(3000, 700, 0.01, seed=14).astype(np.float32)
+    op = lib.CsrMatrix.from_scipy(a)
+    k = 12
+    res = _ffi.Result_()
+    _ffi.check(_ffi.lib().svdh_rsvd(_ffi.context(0), op._handle, k, 8, 2, _ffi.NORM_QR, 0, 5, 1, None,
+                                    _ffi.PREC_F32, 0.0, 0, C.byref(res)))
+    assert not res.u and not res.s and not res.vt and res.u32 and res.s32 and res.vt32
+    u32 = np.ctypeslib.as_array(res.u32, (3000 * k,)).reshape(3000, k).copy()
+    s32 = np.ctypeslib.as_array(res.s32, (k,)).copy()
+    vt32 = np.ctypeslib.as_array(res.vt32, (k * 700,)).reshape(k, 700).copy()
+    _ffi.lib().svdh_result_free(C.byref(res))
+    assert not res.u32 and not res.s32 and not res.vt32
+    u = _ffi.DeviceArray((3000, k))
+    vt = _ffi.DeviceArray((k, 700))
+    s = np.zeros(k)
+    meta = _ffi.Result_()
+    _ffi.check(_ffi.lib().svdh_rsvd_device(_ffi.context(0), op._handle, k, 8, 2, _ffi.NORM_QR, 0, 5, 1, None,
+                                           _ffi.PREC_F32, 0.0, 0, u.ptr, k, _ffi.ptr(s), vt.ptr, 700, C.byref(meta)))
+    assert u32.dtype == np.float32
+    np.testing.assert_array_equal(u32, u.numpy().astype(np.float32))
+    np.testing.assert_array_equal(vt32, vt.numpy().astype(np.float32))
+    np.testing.assert_array_equal(s32, s.astype(np.float32))
+    # the Python mirror reads them as the reference's SvdRec<f32>
+    rec = lib.randomized.randomized_svd(op, k, 8, 2, "QR", False, 5, True)
+    assert rec.u.dtype == np.float32
+    np.testing.assert_array_equal(rec.u, u32)
