@@ -304,13 +304,14 @@ struct Las2 {
   // before step j's scalars arrive: ortbnd's recurrence (mod.rs:815-838) with
   // alpha_j and rnm_j taken as alpha_{j-1} and beta_j = rnm_{j-1}, against half
   // the threshold. Only the choice of enqueuing step j + 1 ahead depends on it.
-  bool purge_likely(int64_t j, int64_t ll, double eps1) const {
+  std::vector<double> ne;  // purge_likely's predicted eta
+  bool purge_likely(int64_t j, int64_t ll, double eps1) {
     if (j < ll + 2) return false;
     const double rg = bet[j];
     if (zero(rg)) return true;
     const double ag = alf[j - 1];
     const int64_t cnt = j - (ll + 1);
-    std::vector<double> ne(j + 1, eps1);
+    ne.assign(j + 1, eps1);
     if (j > 1) {
       ne[0] = (bet[1] * eta[1] + (alf[0] - ag) * eta[0] - rg * oldeta[0]) / rg + eps1;
       for (int64_t i = 1; i + 2 <= j; ++i)
