@@ -85,7 +85,14 @@ struct Ctx {
     return ws[slot].p;
   }
   // Panels of the batched randomized solver (4 per operator), grown on demand.
+  // batch_pad[i]: rows [0, rows) of batch_ws[i] laid out with leading
+  // dimension ld hold zeros in columns [l, ld) (no kernel writes past a
+  // panel's width); l = -1 when nothing is known.
+  struct PadState {
+    int64_t l = -1, ld = 0, rows = 0;
+  };
   std::vector<DevBuf> batch_ws;
+  std::vector<PadState> batch_pad;
   // Pinned ring for small host-to-device uploads: slot i is reused only after
   // the event of its previous copy has completed.
   static constexpr int kRing = 8;
@@ -385,6 +392,12 @@ void gram_batched_async(Ctx *c, int64_t count, const int64_t *rows, int64_t l, c
 // Z (m x lo) = Y (m x li) * M (li x lo, FP64 row-major, host), in place or to
 // another panel. Element (i, c) of Z is stored at Z[i * ldz + c * zcs]
 // (zcs = 1: row-major; ldz = 1, zcs = ld: the transposed, column-major store).
+// Z_b = Y_b M_b (FP64 outputs, lo <= 128 columns, Z_b element (r, c) at
+// r ldz[b] + c zcs[b]) for many panels in as few launches as the upload
+// slots allow; each result equals panel_apply's for the same panel.
+void panel_apply_batched(Ctx *c, int64_t count, const int64_t *m, const int64_t *li, int64_t lo,
+                         const void *const *Y, int64_t ldy, int ydt, const std::vector<double> *M, double *const *Z,
+                         const int64_t *ldz, const int64_t *zcs);
 void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, int ydt,
                  const double *M_host, void *Z, int64_t ldz, int zdt, int64_t zcs = 1);
 // nr2[c] = |Y1 M1[:, c] + Y2 M2[:, c]|^2 for c < k (FP64); Y1 (m x w) and
@@ -399,6 +412,10 @@ void panel_residual_wait(Ctx *c, int64_t k, double *nr2_host);
 void panel_normalize_columns(Ctx *c, int64_t m, int64_t l, void *Y, int64_t ldy, int ydt);
 // Standard normal panel (m x l row-major) from a counter-based generator.
 void panel_gaussian(Ctx *c, int64_t m, int64_t l, uint64_t seed, void *Y, int64_t ldy, int ydt);
+// panel_gaussian for count panels (rows m[b], pointer Y[b]) in one launch;
+// panel b holds the values panel_gaussian(c, m[b], l, seed, Y[b], ...) gives.
+void panel_gaussian_batched(Ctx *c, int64_t count, const int64_t *m, int64_t l, uint64_t seed, void *const *Y,
+                            int64_t ldy, int ydt);
 // Convert a panel between dtypes (same shape).
 void panel_convert(Ctx *c, int64_t m, int64_t l, const void *X, int64_t ldx, int xdt, void *Y,
                    int64_t ldy, int ydt);

@@ -413,11 +413,11 @@ __global__ __launch_bounds__(256) void k_combine_mfma(int64_t n, const double *Q
 // A[i][kk] = Y[r0 + i][k0 + kk]. A wave reads all of its 16 rows before
 // writing any, so Z may be Y (zcs = 1, same rows).
 template <typename TY, int T>
-__global__ __launch_bounds__(256) void k_apply_mfma(int64_t m, int li, int lo, const TY *Y, int64_t ldy,
-                                                    const double *M, double *Z, int64_t ldz, int64_t zcs,
-                                                    double alpha, double beta) {
+__device__ __forceinline__ void apply_mfma_body(int64_t bx, int64_t m, int li, int lo, const TY *Y, int64_t ldy,
+                                                const double *M, double *Z, int64_t ldz, int64_t zcs, double alpha,
+                                                double beta) {
   const int lane = threadIdx.x & 63;
-  const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  const int64_t r0 = (bx * 4 + (threadIdx.x >> 6)) * 16;
   if (r0 >= m) return;  // the whole wave
   const int i = lane & 15, kk = lane >> 4;
   const int64_t row = r0 + i;
@@ -458,6 +458,30 @@ __global__ __launch_bounds__(256) void k_apply_mfma(int64_t m, int li, int lo, c
         Z[at] = (beta != 0.0 ? beta * Z[at] : 0.0) + alpha * acc[t][v];
       }
     }
+}
+
+template <typename TY, int T>
+__global__ __launch_bounds__(256) void k_apply_mfma(int64_t m, int li, int lo, const TY *Y, int64_t ldy,
+                                                    const double *M, double *Z, int64_t ldz, int64_t zcs,
+                                                    double alpha, double beta) {
+  apply_mfma_body<TY, T>(blockIdx.x, m, li, lo, Y, ldy, M, Z, ldz, zcs, alpha, beta);
+}
+
+// Many independent Z_j = Y_j M_j in one launch: job j = blockIdx.y takes row
+// blocks 0 .. of the x dimension, each computed exactly as k_apply_mfma does.
+struct ApplyJob {
+  const void *Y;
+  int64_t ldy, m;
+  int32_t li, lo;
+  const double *M;
+  double *Z;
+  int64_t ldz, zcs;
+};
+template <typename TY, int T>
+__global__ __launch_bounds__(256) void k_apply_mfma_jobs(const ApplyJob *jobs) {
+  const ApplyJob &j = jobs[blockIdx.y];
+  apply_mfma_body<TY, T>(blockIdx.x, j.m, j.li, j.lo, static_cast<const TY *>(j.Y), j.ldy, j.M, j.Z, j.ldz, j.zcs,
+                         1.0, 0.0);
 }
 
 template <typename T> __device__ __forceinline__ double ldd(const T *p, int64_t i);
@@ -867,7 +891,7 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 // of an m x l panel uses counter j*m + i (column-major, the fill order of
 // DMatrix::from_fn at src/randomized/mod.rs:310-312).
 template <typename T>
-__global__ void k_gauss(int64_t m, int l, uint64_t seed, T *Y, int64_t ldy) {
+__device__ __forceinline__ void gauss_body(int64_t m, int l, uint64_t seed, T *Y, int64_t ldy) {
   const uint64_t phi = 0x9E3779B97F4A7C15ULL;
   const uint64_t key = (seed * phi) ^ 0x4F4D454741303031ULL;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m * l; e += (int64_t)gridDim.x * blockDim.x) {
@@ -881,6 +905,17 @@ __global__ void k_gauss(int64_t m, int l, uint64_t seed, T *Y, int64_t ldy) {
     const double z = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
     std_<T>(Y, i * ldy + j, z);
   }
+}
+
+template <typename T>
+__global__ void k_gauss(int64_t m, int l, uint64_t seed, T *Y, int64_t ldy) {
+  gauss_body<T>(m, l, seed, Y, ldy);
+}
+
+// panel b = blockIdx.y of a batch: rows m[b] at Y[b]
+template <typename T>
+__global__ void k_gauss_jobs(const int64_t *m, int l, uint64_t seed, T *const *Y, int64_t ldy) {
+  gauss_body<T>(m[blockIdx.y], l, seed, Y[blockIdx.y], ldy);
 }
 
 template <typename S, typename D>
@@ -1718,6 +1753,63 @@ void panel_apply(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64
     fail(SVDH_ERR_ARG, "unsupported apply dtypes %d -> %d", ydt, zdt);
 }
 
+void panel_apply_batched(Ctx *c, int64_t count, const int64_t *m, const int64_t *li, int64_t lo,
+                         const void *const *Y, int64_t ldy, int ydt, const std::vector<double> *M, double *const *Z,
+                         const int64_t *ldz, const int64_t *zcs) {
+  if (lo <= 0 || count <= 0) return;
+  if (lo > 128 || (ydt != SVDH_F32 && ydt != SVDH_F64))
+    fail(SVDH_ERR_ARG, "panel_apply_batched: FP32 or FP64 panels, at most 128 output columns");
+  // groups whose factors fit one upload slot, one launch per group
+  int64_t b0 = 0;
+  while (b0 < count) {
+    int64_t b1 = b0;
+    size_t doubles = 0;
+    while (b1 < count && (doubles + M[b1].size()) * sizeof(double) <= Ctx::kRingSlot &&
+           (b1 - b0 + 1) * sizeof(ApplyJob) <= Ctx::kRingSlot) {
+      doubles += M[b1].size();
+      ++b1;
+    }
+    if (b1 == b0) fail(SVDH_ERR_ARG, "panel_apply_batched: factor larger than the upload slot");
+    std::vector<double> mall;
+    mall.reserve(doubles);
+    for (int64_t b = b0; b < b1; ++b) mall.insert(mall.end(), M[b].begin(), M[b].end());
+    const double *dm = static_cast<const double *>(c->upload(mall.data(), std::max<size_t>(mall.size(), 1) * 8));
+    std::vector<ApplyJob> jobs(b1 - b0);
+    int64_t maxg = 1;
+    size_t moff = 0;
+    double bytes = 0.0;
+    for (int64_t b = b0; b < b1; ++b) {
+      if ((int64_t)M[b].size() != li[b] * lo) fail(SVDH_ERR_ARG, "panel_apply_batched: factor %lld is not li x lo",
+                                                   (long long)b);
+      jobs[b - b0] = ApplyJob{Y[b], ldy, m[b], (int32_t)li[b], (int32_t)lo, dm + moff, Z[b], ldz[b], zcs[b]};
+      moff += M[b].size();
+      maxg = std::max<int64_t>(maxg, ((m[b] + 15) / 16 + 3) / 4);
+      bytes += (double)m[b] * (li[b] * dtype_size(ydt) + lo * 8.0);
+    }
+    const ApplyJob *dj = static_cast<const ApplyJob *>(c->upload(jobs.data(), jobs.size() * sizeof(ApplyJob)));
+    KTimer t(c, KC_DENSE, bytes);
+    const dim3 grid((unsigned)maxg, (unsigned)(b1 - b0));
+    const bool f32 = ydt == SVDH_F32;
+#define SVDH_AJ(TT)                                                                         \
+  if (f32)                                                                                  \
+    hipLaunchKernelGGL((k_apply_mfma_jobs<float, TT>), grid, 256, 0, c->stream, dj);        \
+  else                                                                                      \
+    hipLaunchKernelGGL((k_apply_mfma_jobs<double, TT>), grid, 256, 0, c->stream, dj);
+    if (lo <= 16) {
+      SVDH_AJ(1)
+    } else if (lo <= 32) {
+      SVDH_AJ(2)
+    } else if (lo <= 64) {
+      SVDH_AJ(4)
+    } else {
+      SVDH_AJ(8)
+    }
+#undef SVDH_AJ
+    HIP_CHECK(hipGetLastError());
+    b0 = b1;
+  }
+}
+
 void panel_gemm_dev(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, int64_t ldy, const double *M_dev,
                     double alpha, double beta, void *Z, int64_t ldz, int dt) {
   if (m == 0 || lo == 0) return;
@@ -2068,6 +2160,34 @@ void panel_gaussian(Ctx *c, int64_t m, int64_t l, uint64_t seed, void *Y, int64_
   else
     hipLaunchKernelGGL(k_gauss<bf16_t>, grid_of(m * l), kBlock, 0, c->stream, m, (int)l, seed, (bf16_t *)Y, ldy);
   HIP_CHECK(hipGetLastError());
+}
+
+void panel_gaussian_batched(Ctx *c, int64_t count, const int64_t *m, int64_t l, uint64_t seed, void *const *Y,
+                            int64_t ldy, int ydt) {
+  // one upload slot: count row counts, then count pointers
+  const int64_t per = (int64_t)(Ctx::kRingSlot / 16);
+  for (int64_t b0 = 0; b0 < count; b0 += per) {
+    const int64_t nb = std::min(per, count - b0);
+    std::vector<int64_t> buf(2 * nb);
+    int64_t mmax = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+      buf[b] = m[b0 + b];
+      buf[nb + b] = (int64_t)(uintptr_t)Y[b0 + b];
+      mmax = std::max(mmax, m[b0 + b]);
+    }
+    const int64_t *dm = static_cast<const int64_t *>(c->upload(buf.data(), buf.size() * sizeof(int64_t)));
+    const dim3 grid((unsigned)grid_of(mmax * l), (unsigned)nb);
+    if (ydt == SVDH_F64)
+      hipLaunchKernelGGL(k_gauss_jobs<double>, grid, kBlock, 0, c->stream, dm, (int)l, seed,
+                         reinterpret_cast<double *const *>(dm + nb), ldy);
+    else if (ydt == SVDH_F32)
+      hipLaunchKernelGGL(k_gauss_jobs<float>, grid, kBlock, 0, c->stream, dm, (int)l, seed,
+                         reinterpret_cast<float *const *>(dm + nb), ldy);
+    else
+      hipLaunchKernelGGL(k_gauss_jobs<bf16_t>, grid, kBlock, 0, c->stream, dm, (int)l, seed,
+                         reinterpret_cast<bf16_t *const *>(dm + nb), ldy);
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 void panel_convert(Ctx *c, int64_t m, int64_t l, const void *X, int64_t ldx, int xdt, void *Y, int64_t ldy,

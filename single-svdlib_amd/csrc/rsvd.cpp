@@ -31,10 +31,14 @@
 // keep the reference's explicit steps: column L2 normalisation or nothing
 // between the products, then an explicit CholeskyQR2 of Y before B.
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -83,6 +87,35 @@ std::vector<double> mat_mul(const double *A, int64_t rows, int64_t mid, const do
     }
   }
   return C;
+}
+
+// fn(0) .. fn(n - 1) on up to 8 host threads (the caller's among them); the
+// first exception thrown is rethrown here after every thread has finished.
+template <typename F>
+static void parallel_for(int64_t n, F fn) {
+  const int64_t threads = std::min<int64_t>({n / 4, (int64_t)std::max(1u, std::thread::hardware_concurrency()), 8});
+  if (threads <= 1) {
+    for (int64_t b = 0; b < n; ++b) fn(b);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  std::exception_ptr err;
+  std::mutex mu;
+  auto work = [&] {
+    for (int64_t b; (b = next.fetch_add(1)) < n;) {
+      try {
+        fn(b);
+      } catch (...) {
+        std::lock_guard<std::mutex> g(mu);
+        if (!err) err = std::current_exception();
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int64_t t = 1; t < threads; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  if (err) std::rethrow_exception(err);
 }
 
 // Eigen-decomposition of a symmetric matrix sorted by descending eigenvalue.
@@ -640,18 +673,32 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
   // panels: Y_b, Yn_b (m x ld), Z_b and Omega_b (n_b x ld), in the context's batch workspace
   std::vector<void *> Y(nops), Yn(nops), Z(nops), Om(nops);
   c->batch_ws.resize(std::max<size_t>(c->batch_ws.size(), (size_t)nops * 4));
+  c->batch_pad.resize(c->batch_ws.size());
+  std::vector<int64_t> g_rows;  // operators whose Omega is drawn on the device
+  std::vector<void *> g_ptr;
   for (int64_t b = 0; b < nops; ++b) {
     const int64_t nb = ops[b]->ncols;
     void *p[4];
-    const size_t bytes[4] = {(size_t)m * ld * es, (size_t)m * ld * es, (size_t)nb * ld * es, (size_t)nb * ld * es};
+    const int64_t rows[4] = {m, m, nb, nb};
     for (int q = 0; q < 4; ++q) {
-      DevBuf &d = c->batch_ws[(size_t)b * 4 + q];
-      if (d.bytes < bytes[q]) {
+      const size_t i = (size_t)b * 4 + q;
+      DevBuf &d = c->batch_ws[i];
+      Ctx::PadState &ps = c->batch_pad[i];
+      const size_t bytes = (size_t)rows[q] * ld * es;
+      if (d.bytes < bytes) {
         c->sync();
-        d.alloc(bytes[q]);
+        d.alloc(bytes);
+        ps = Ctx::PadState{};
       }
       p[q] = d.p;
-      HIP_CHECK(hipMemsetAsync(p[q], 0, bytes[q], c->stream));
+      // the padding columns [l, ld) must read as zero; the panel columns
+      // below l are written before they are read
+      if (!(ps.l >= 0 && ps.ld == ld && ps.l <= l && ps.rows >= rows[q])) {
+        HIP_CHECK(hipMemsetAsync(d.p, 0, d.bytes, c->stream));
+        ps.rows = (int64_t)(d.bytes / ((size_t)ld * es));
+      }
+      ps.l = l;
+      ps.ld = ld;
     }
     Y[b] = p[0];
     Yn[b] = p[1];
@@ -667,9 +714,11 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
       panel_convert(c, nb, l, tmp.p, ld, SVDH_F64, Om[b], ld, pdt);
       HIP_CHECK(hipStreamSynchronize(c->stream));
     } else {
-      panel_gaussian(c, nb, l, sd, Om[b], ld, pdt);
+      g_rows.push_back(nb);
+      g_ptr.push_back(Om[b]);
     }
   }
+  panel_gaussian_batched(c, (int64_t)g_rows.size(), g_rows.data(), l, sd, g_ptr.data(), ld, pdt);
   std::vector<svdh_op *> opv(ops, ops + nops);
   spmm_batched(c, (int)nops, opv.data(), 0, pdt, Om.data(), ld, pdt, Y.data(), ld, pdt, l);  // Y = A Omega
 
@@ -751,22 +800,34 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
   }
   // B^T = Z_b; SVD(B) from its Gram, as in rsvd_solve
   if (!gz_done) grams(Z, nrows, [] {}, GZ);
-  for (int64_t b = 0; b < nops; ++b) {
-    svdh_op *op = ops[b];
-    const int64_t n = op->ncols, w = wq[b];
+  // every operator's small factors first (independent, on host threads; the
+  // GPU waits for them), then the outputs; device outputs take U and V^T of
+  // all operators in one matrix-core launch each (the per-operator arithmetic)
+  std::vector<std::vector<double>> MWs(nops), Wss(nops), svs(nops);
+  std::vector<int64_t> ks(nops);
+  parallel_for(nops, [&](int64_t b) {
+    const int64_t n = ops[b]->ncols, w = wq[b];
     std::vector<double> Gb(w * w), lam, W;
     const int64_t gld = gz_done ? w : l;
     for (int64_t i = 0; i < w; ++i)
       for (int64_t j = 0; j < w; ++j) Gb[i * w + j] = GZ[b][i * gld + j];
     desc_eig(w, Gb, lam, W);
     const int64_t k = std::min(target_rank, std::min(w, n));
-    std::vector<double> sv(k), MW(l * k, 0.0), Ws(w * k);
-    MW = mat_mul(Mq[b].data(), l, w, W.data(), w, k);
+    std::vector<double> sv(k), Ws(w * k);
+    MWs[b] = mat_mul(Mq[b].data(), l, w, W.data(), w, k);
     for (int64_t r = 0; r < k; ++r) {
       sv[r] = std::sqrt(std::max(lam[r], 0.0));
       const double inv = sv[r] > 0.0 ? 1.0 / sv[r] : 0.0;
       for (int64_t i = 0; i < w; ++i) Ws[i * k + r] = W[i * w + r] * inv;
     }
+    svs[b] = std::move(sv);
+    Wss[b] = std::move(Ws);
+    ks[b] = k;
+  });
+  for (int64_t b = 0; b < nops; ++b) {
+    svdh_op *op = ops[b];
+    const int64_t n = op->ncols, w = wq[b], k = ks[b];
+    const std::vector<double> &sv = svs[b], &MW = MWs[b], &Ws = Wss[b];
     svdh_result *out = &outs[b];
     out->d = k;
     out->u_rows = m;
@@ -779,8 +840,6 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
       for (int64_t r = 0; r < k; ++r) out->s[r] = sv[r];
       if (dev->ldu[b] < k || dev->ldvt[b] < n) fail(SVDH_ERR_ARG, "leading dimension too small for operator %lld",
                                                     (long long)b);
-      panel_apply(c, m, l, k, Y[b], ld, pdt, MW.data(), dev->u[b], dev->ldu[b], SVDH_F64);
-      panel_apply(c, n, w, k, Z[b], ld, pdt, Ws.data(), dev->vt[b], 1, SVDH_F64, dev->ldvt[b]);
     } else {
       double *ud = static_cast<double *>(ctx->workspace(WS_U, std::max<size_t>(m * k, 1) * 8));
       double *vtd = static_cast<double *>(ctx->workspace(WS_VT, std::max<size_t>(k * n, 1) * 8));
@@ -805,6 +864,34 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
     out->power_iters_run = iters_run;
     out->residual_estimate = -1.0;
     out->qr_fallbacks = fallbacks[b];
+  }
+  if (dev && nops > 0) {
+    const int64_t k0 = ks[0];
+    bool same_k = k0 <= 128 && (pdt == SVDH_F32 || pdt == SVDH_F64);
+    for (int64_t b = 1; b < nops; ++b) same_k = same_k && ks[b] == k0;
+    if (same_k) {
+      std::vector<int64_t> rows(nops, m), lis(nops, l), zcs1(nops, 1), lz1(nops, 1), vcs(nops), cols(nops);
+      std::vector<const void *> ys(nops), zs(nops);
+      std::vector<double *> us(nops), vts(nops);
+      for (int64_t b = 0; b < nops; ++b) {
+        ys[b] = Y[b];
+        zs[b] = Z[b];
+        us[b] = dev->u[b];
+        vts[b] = dev->vt[b];
+        vcs[b] = dev->ldvt[b];
+        cols[b] = ops[b]->ncols;
+      }
+      panel_apply_batched(c, nops, rows.data(), lis.data(), k0, ys.data(), ld, pdt, MWs.data(), us.data(),
+                          dev->ldu, zcs1.data());
+      panel_apply_batched(c, nops, cols.data(), wq.data(), k0, zs.data(), ld, pdt, Wss.data(), vts.data(),
+                          lz1.data(), vcs.data());
+    } else {
+      for (int64_t b = 0; b < nops; ++b) {
+        panel_apply(c, m, l, ks[b], Y[b], ld, pdt, MWs[b].data(), dev->u[b], dev->ldu[b], SVDH_F64);
+        panel_apply(c, ops[b]->ncols, wq[b], ks[b], Z[b], ld, pdt, Wss[b].data(), dev->vt[b], 1, SVDH_F64,
+                    dev->ldvt[b]);
+      }
+    }
   }
   HIP_CHECK(hipStreamSynchronize(c->stream));
   return SVDH_OK;

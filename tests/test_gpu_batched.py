@@ -41,6 +41,28 @@ def test_batched_equals_separate_solves(lib, dtype):
         assert r.diagnostics.iterations == 2 and r.extra["power_iters_run"] == 2
 
 
+def test_batched_calls_of_changing_widths_and_sizes_equal_separate_solves(lib):
+    # the batch panels are reused from call to call: their padding columns
+    # past the panel width must read as zero whatever an earlier call with
+    # another width, leading dimension or mask size left there
+    from single_svdlib.lanczos.masked import MaskedCSRMatrix
+    a = random_csr(2500, 1000, 0.02, seed=23).astype(np.float32)
+    base = lib.CsrMatrix.from_scipy(a)
+    # (rank, oversample) -> panel width l and leading dimension: 30/32,
+    # 26/32 (narrower, same ld), 30/32 again, 21/24, 30/32 with larger masks
+    plan = [((20, 10), [60, 200]), ((16, 10), [60, 200]), ((20, 10), [60, 200]), ((11, 10), [300, 90]),
+            ((20, 10), [900, 700, 400])]
+    for i, ((rank, over), sizes) in enumerate(plan):
+        cols = masks_of(1000, sizes, seed=40 + i)
+        views = [MaskedCSRMatrix.with_columns(base, c) for c in cols]
+        recs = lib.randomized.randomized_svd_batched(views, rank, over, 2, "QR", seed=11)
+        for v, r in zip(views, recs):
+            one = lib.randomized.randomized_svd(v, rank, over, 2, "QR", False, 11, False)
+            np.testing.assert_array_equal(r.s, one.s)
+            np.testing.assert_array_equal(r.u, one.u)
+            np.testing.assert_array_equal(r.vt, one.vt)
+
+
 @pytest.mark.parametrize("dtype,sv_tol,ang_tol", [(np.float64, 1e-10, 1e-8), (np.float32, 1e-5, 1e-3)])
 def test_batched_matches_oracle_on_physical_subsets(lib, dtype, sv_tol, ang_tol):
     # every mask against the FP64 oracle restatement of randomized_svd
@@ -110,15 +132,19 @@ def test_batched_rejects_mixed_widths_and_normalizers(lib):
         lib.randomized.randomized_svd_batched([big], 10, 5, 1, "LU", seed=1)
 
 
-def test_batched_device_outputs_equal_host_outputs(lib):
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batched_device_outputs_equal_host_outputs(lib, dtype):
+    # device outputs are FP64; an FP32 solve's host arrays are those values
+    # rounded to single precision
     import ctypes as C
     from single_svdlib import _ffi
     from single_svdlib.lanczos.masked import MaskedCSRMatrix
-    a = random_csr(2000, 800, 0.02, seed=29)
+    a = random_csr(2000, 800, 0.02, seed=29, dtype=dtype)
     base = lib.CsrMatrix.from_scipy(a)
     cols = masks_of(800, [150, 500, 60], seed=2)
     views = [MaskedCSRMatrix.with_columns(base, c) for c in cols]
     recs = lib.randomized.randomized_svd_batched(views, 6, 6, 2, "QR", seed=3)
+    prec = _ffi.PREC_F64 if dtype == np.float64 else _ffi.PREC_F32
     k = 6
     us = [_ffi.DeviceArray((2000, k + 2)) for _ in cols]
     vts = [_ffi.DeviceArray((k, len(c) + 3)) for c in cols]
@@ -130,14 +156,15 @@ def test_batched_device_outputs_equal_host_outputs(lib):
     ldu = (C.c_int64 * nb)(*[k + 2] * nb)
     ldvt = (C.c_int64 * nb)(*[len(c) + 3 for c in cols])
     _ffi.check(_ffi.lib().svdh_rsvd_batched_device(_ffi.context(0), nb, handles, 6, 6, 2, _ffi.NORM_QR, 3, 1,
-                                                   None, _ffi.PREC_F64, up, ldu, vp, ldvt, metas))
+                                                   None, prec, up, ldu, vp, ldvt, metas))
     for b in range(nb):
         assert metas[b].d == k and not metas[b].u and not metas[b].vt
         s = np.ctypeslib.as_array(metas[b].s, (k,)).copy()
         _ffi.lib().svdh_result_free(C.byref(metas[b]))
-        np.testing.assert_array_equal(s, recs[b].s)
-        np.testing.assert_array_equal(us[b].numpy()[:, :k], recs[b].u)
-        np.testing.assert_array_equal(vts[b].numpy()[:, :len(cols[b])], recs[b].vt)
+        assert recs[b].u.dtype == dtype and recs[b].vt.dtype == dtype
+        np.testing.assert_array_equal(s.astype(dtype), recs[b].s)
+        np.testing.assert_array_equal(us[b].numpy()[:, :k].astype(dtype), recs[b].u)
+        np.testing.assert_array_equal(vts[b].numpy()[:, :len(cols[b])].astype(dtype), recs[b].vt)
 
 
 def test_batched_persistent_autotuned_and_static_launches_agree(lib, tmp_path, monkeypatch):
