@@ -644,25 +644,32 @@ __device__ __forceinline__ void gram_mfma_body(int64_t bid, int64_t nbk, int64_t
   constexpr int G = 4;  // row steps per group; the next group is loaded during this one's MFMAs
   using Vec = typename GramVec<T, NB>::type;
   Vec qc[G], qd[G], nc[G], nd[G];
-  auto load = [&](int64_t st0, Vec (&xc)[G], Vec (&xd)[G]) {
+  // Every load is issued unconditionally at an in-bounds address (rows past
+  // the wave's range re-read its last row, columns past l read column 0) and
+  // the unused parts are zeroed by selects when the group is consumed: with
+  // no load behind a branch the compiler can count the loads in flight and
+  // wait only for the group it is about to use, so the next group's loads
+  // overlap this group's products.
+  const int64_t rlast = min<int64_t>(m, s1 * 4) - 1;
+  const int colc_a = okc ? colc : 0, cold_a = okd ? cold : 0;
+  auto load = [&](int64_t st0, Vec (&xc)[G], Vec (&xd)[G], bool (&ok)[G]) {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int64_t r = (st0 + g) * 4 + kr;
-      const bool okr = st0 + g < s1 && r < m;
-      if (okr && okc) xc[g].load(Y + r * ldy + colc); else xc[g].zero();
-      if (!SAME) {
-        if (okr && okd) xd[g].load(Y2 + r * ld2 + cold); else xd[g].zero();
-      }
+      ok[g] = st0 + g < s1 && r < m;
+      const int64_t ra = ok[g] ? r : rlast;
+      xc[g].load(Y + ra * ldy + colc_a);
+      if (!SAME) xd[g].load(Y2 + ra * ld2 + cold_a);
     }
   };
-  auto mfmas = [&](const Vec (&xc)[G], const Vec (&xd)[G]) {
+  auto mfmas = [&](const Vec (&xc)[G], const Vec (&xd)[G], const bool (&ok)[G]) {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       double a[NB], b[NB];
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
-        a[q] = q < limc ? xc[g].get(q) : 0.0;
-        b[q] = SAME ? a[q] : (q < limd ? xd[g].get(q) : 0.0);
+        a[q] = ok[g] && q < limc ? xc[g].get(q) : 0.0;
+        b[q] = SAME ? a[q] : (ok[g] && q < limd ? xd[g].get(q) : 0.0);
       }
 #pragma unroll
       for (int i = 0; i < NB; ++i)
@@ -671,14 +678,17 @@ __device__ __forceinline__ void gram_mfma_body(int64_t bid, int64_t nbk, int64_t
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   };
-  if (s0 < s1) load(s0, qc, qd);
-  for (int64_t st = s0; st < s1; st += 2 * G) {
-    const bool more = st + G < s1;
-    if (more) load(st + G, nc, nd);
-    mfmas(qc, qd);
-    if (!more) break;
-    if (st + 2 * G < s1) load(st + 2 * G, qc, qd);
-    mfmas(nc, nd);
+  if (s0 < s1) {
+    bool okq[G], okn[G];
+    load(s0, qc, qd, okq);
+    for (int64_t st = s0;; st += 2 * G) {
+      load(st + G, nc, nd, okn);
+      mfmas(qc, qd, okq);
+      if (st + G >= s1) break;
+      load(st + 2 * G, qc, qd, okq);
+      mfmas(nc, nd, okn);
+      if (st + 2 * G >= s1) break;
+    }
   }
   // sum the waves in order through LDS: red[(i * NB + j) * 256 + v * 64 + lane]
   for (int w = 0; w < W; ++w) {
@@ -694,17 +704,34 @@ __device__ __forceinline__ void gram_mfma_body(int64_t bid, int64_t nbk, int64_t
           }
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < NB * NB * 256; e += 64 * W) {
-    const int blk = e >> 8, v = (e >> 6) & 3, ln = e & 63;
-    const int b1 = blk / NB, b2 = blk % NB;
-    if (SAME && b2 < b1) continue;
-    const int gi = c0 + NB * ((ln >> 4) + 4 * v) + b1, gj = d0 + NB * (ln & 15) + b2;
-    if (gi < l && gj < l2) {
-      const double x = red[e];
-      out[(int64_t)gi * out_ld + gj] = x;
-      if (mirror) out[(int64_t)gj * out_ld + gi] = x;
+  // The tile out[c0 + ri][d0 + ci] (and its mirror out[d0 + ci][c0 + ri]
+  // when mirror is set and c0 != d0) in output order, so consecutive threads
+  // write consecutive entries. Entry (ri, ci) is column block b1 = ri % NB,
+  // b2 = ci % NB, lane 16 ((ri / NB) % 4) + ci / NB, v = ri / (4 NB) of the
+  // sums above; with SAME only blocks b1 <= b2 were formed, and (ri, ci) of
+  // block b1 > b2 reads its transpose.
+  auto raw = [](int ri, int ci) {
+    int b1 = ri % NB, b2 = ci % NB;
+    if (SAME && b2 < b1) {
+      const int t = ri;
+      ri = ci;
+      ci = t;
+      b1 = ri % NB;
+      b2 = ci % NB;
     }
+    const int rq = ri / NB;
+    return (b1 * NB + b2) * 256 + (rq >> 2) * 64 + (rq & 3) * 16 + ci / NB;
+  };
+  const int rows_t = min(16 * NB, l - c0), cols_t = min(16 * NB, l2 - d0);
+  for (int o = threadIdx.x; o < rows_t * cols_t; o += 64 * W) {
+    const int ri = o / cols_t, ci = o - ri * cols_t;
+    out[(int64_t)(c0 + ri) * out_ld + d0 + ci] = red[raw(ri, ci)];
   }
+  if (mirror && c0 != d0)
+    for (int o = threadIdx.x; o < rows_t * cols_t; o += 64 * W) {
+      const int ci = o / rows_t, ri = o - ci * rows_t;
+      out[(int64_t)(d0 + ci) * out_ld + c0 + ri] = red[raw(ri, ci)];
+    }
 }
 
 template <typename T, bool SAME, int NB = 4>
@@ -1181,8 +1208,9 @@ struct PanelTerm {
 template <int NP>
 __device__ __forceinline__ void panel_rows_load(const PanelTerm &t0, const PanelTerm &t1, int64_t rows, int64_t row,
                                                 int kq, f4_t (&ya)[NP][4]) {
-  // whole 16-byte loads at clamped addresses; out-of-range parts are zeroed
-  // by selects afterwards, so no load sits behind a branch
+  // whole 16-byte loads at clamped addresses, always in bounds; the parts
+  // outside the panel are zeroed by panel_rows_mask when the block is used,
+  // so neither a branch nor a select waits on the loads here
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
     const PanelTerm &t = p == 0 ? t0 : t1;
@@ -1192,9 +1220,24 @@ __device__ __forceinline__ void panel_rows_load(const PanelTerm &t0, const Panel
       const int kc = 16 * g + 4 * kq;
       const int kk = kc < t.li ? kc : 0;
       const float4 w = *reinterpret_cast<const float4 *>(t.y + rr * t.ld + kk);
+      ya[p][g] = f4_t{w.x, w.y, w.z, w.w};
+    }
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ void panel_rows_mask(const PanelTerm &t0, const PanelTerm &t1, int64_t rows, int64_t row,
+                                                int kq, f4_t (&ya)[NP][4]) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const PanelTerm &t = p == 0 ? t0 : t1;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int kc = 16 * g + 4 * kq;
       const bool ok = row < rows && kc < t.li;
-      ya[p][g] = f4_t{ok ? w.x : 0.f, ok && kc + 1 < t.li ? w.y : 0.f, ok && kc + 2 < t.li ? w.z : 0.f,
-                      ok && kc + 3 < t.li ? w.w : 0.f};
+      const f4_t w = ya[p][g];
+      ya[p][g] = f4_t{ok ? w[0] : 0.f, ok && kc + 1 < t.li ? w[1] : 0.f, ok && kc + 2 < t.li ? w[2] : 0.f,
+                      ok && kc + 3 < t.li ? w[3] : 0.f};
     }
   }
 }
@@ -1220,12 +1263,34 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
   // 16-byte read. The B operands are read from LDS at each step, which leaves
   // registers for more than one wave per SIMD.
   __shared__ float ms[NP][64 * kMsStride];
-  for (int p = 0; p < NP; ++p) {
-    const PanelTerm &t = p == 0 ? t0 : t1;
-    for (int e = threadIdx.x; e < 64 * 64; e += kPanelBlock) {
-      const int k = e >> 6, col = e & 63;
-      ms[p][k * kMsStride + 4 * (col & 15) + (col >> 4)] =
-          (k < t.li && col < lo) ? (float)(t.alpha * t.m[(int64_t)k * lo + col]) : 0.f;
+  if constexpr (NP == 2 && GRAM) {
+    // one element at a time: the batched loads below leave this variant's
+    // main loop short of registers (the compiler spills)
+    for (int p = 0; p < NP; ++p) {
+      const PanelTerm &t = p == 0 ? t0 : t1;
+      for (int e = threadIdx.x; e < 64 * 64; e += kPanelBlock) {
+        const int k = e >> 6, col = e & 63;
+        ms[p][k * kMsStride + col] = (k < t.li && col < lo) ? (float)(t.alpha * t.m[(int64_t)k * lo + col]) : 0.f;
+      }
+    }
+  } else {
+    // every load of the factors in flight before the first LDS write (a
+    // load-then-store loop waits out one memory latency per element)
+    constexpr int S = 64 * 64 / kPanelBlock;
+    for (int p = 0; p < NP; ++p) {
+      const PanelTerm &t = p == 0 ? t0 : t1;
+      double mv[S];
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const int e = threadIdx.x + i * kPanelBlock, k = e >> 6, col = e & 63;
+        const bool ok = k < t.li && col < lo;
+        mv[i] = t.li > 0 ? t.m[ok ? (int64_t)k * lo + col : 0] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < S; ++i) {
+        const int e = threadIdx.x + i * kPanelBlock, k = e >> 6, col = e & 63;
+        ms[p][k * kMsStride + col] = (k < t.li && col < lo) ? (float)(t.alpha * mv[i]) : 0.f;
+      }
     }
   }
   __syncthreads();
@@ -1258,6 +1323,7 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
   const bool use_zin = !RESID && beta != 0.f;
   if (wave < nblk) {
     panel_rows_load<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
+    panel_rows_mask<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
     if (use_zin) zin_load(wave * 16, zold);
   }
   for (int64_t b = wave; b < nblk; b += nwaves) {
@@ -1267,10 +1333,9 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
     asm volatile("v_mov_b32 %0, 0" : "=v"(lds0));
     const int64_t r0 = b * 16;
     const bool more = b + nwaves < nblk;
-    if (more) {  // next block in flight
-      panel_rows_load<NP>(t0, t1, rows, (b + nwaves) * 16 + rl, kq, yn);
-      if (use_zin) zin_load((b + nwaves) * 16, znext);
-    }
+    // the next block in flight (clamped rows past the end: in bounds, unused)
+    panel_rows_load<NP>(t0, t1, rows, (b + nwaves) * 16 + rl, kq, yn);
+    if (use_zin) zin_load((b + nwaves) * 16, znext);
     f4_t acc[4] = {f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f},
                    f4_t{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
@@ -1279,8 +1344,8 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
       for (int g = 0; g < 4; ++g)
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
-          const float4 b =
-              *reinterpret_cast<const float4 *>(&ms[p][(16 * g + 4 * kq + st) * kMsStride + 4 * rl + lds0]);
+          const float *mr = &ms[p][(16 * g + 4 * kq + st) * kMsStride + rl + lds0];
+          const float4 b = make_float4(mr[0], mr[16], mr[32], mr[48]);
           acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.x, acc[0], 0, 0, 0);
           acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.y, acc[1], 0, 0, 0);
           acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.z, acc[2], 0, 0, 0);
@@ -1316,6 +1381,7 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
                                                                 0);
     }
     if (more) {
+      panel_rows_mask<NP>(t0, t1, rows, (b + nwaves) * 16 + rl, kq, yn);
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
@@ -1344,18 +1410,20 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
             }
       __syncthreads();
     }
-    double *out = part;
-    for (int e = threadIdx.x; e < 16 * 4 * 64; e += kPanelBlock) {
-      const int blk = e >> 8, v = (e >> 6) & 3, ln = e & 63;
-      const int bi = blk >> 2, bj = blk & 3;
-      if (bj < bi) continue;
-      // lane ln holds D[ln / 16 + 4 v][ln % 16] of block (bi, bj)
-      const int gi = 16 * bi + (ln >> 4) + 4 * v, gj = 16 * bj + (ln & 15);
-      if (gi < lo && gj < lo) {
-        const double x = gred[e];
-        out[(int64_t)gi * lo + gj] = x;
-        out[(int64_t)gj * lo + gi] = x;
+    // lo x lo in output order (consecutive threads, consecutive entries):
+    // lane ln of block (bi, bj) holds D[ln / 16 + 4 v][ln % 16], so entry
+    // (gi, gj) is gred[((bi * 4 + bj) * 4 + v) * 64 + 16 ((gi % 16) % 4) +
+    // gj % 16] with v = (gi % 16) / 4; only blocks bi <= bj were formed, and
+    // an entry of block bi > bj reads its transpose
+    for (int o = threadIdx.x; o < lo * lo; o += kPanelBlock) {
+      int gi = o / lo, gj = o - gi * lo;
+      if ((gj >> 4) < (gi >> 4)) {
+        const int t = gi;
+        gi = gj;
+        gj = t;
       }
+      const int r = gi & 15;
+      part[o] = gred[(((gi >> 4) * 4 + (gj >> 4)) * 4 + (r >> 2)) * 64 + 16 * (r & 3) + (gj & 15)];
     }
   }
   if (RESID) {
