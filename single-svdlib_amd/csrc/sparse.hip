@@ -887,9 +887,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin(const int32_t *multi, const
 // thread per column adds the row's pieces in four interleaved chains (piece
 // i goes to chain i mod 4), combined in a fixed order.
 template <typename Y, typename Acc>
-__global__ __launch_bounds__(kBlock) void k_spmm_fin4(const int32_t *multi, int nrows, const Acc *partials,
-                                                     int64_t ldp, Y *y, int64_t ldy, int l) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), o = threadIdx.x & 63;
+__device__ __forceinline__ void spmm_fin4_rows(const int32_t *multi, int nrows, const Acc *partials, int64_t ldp,
+                                               Y *y, int64_t ldy, int l, int blk) {
+  const int row = blk * 4 + (threadIdx.x >> 6), o = threadIdx.x & 63;
   if (row >= nrows || o >= l) return;
   const int *md = multi + 3 * (int64_t)row;
   const int r = md[0], first = md[1], nch = md[2];
@@ -904,6 +904,32 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin4(const int32_t *multi, int 
   }
   for (; i < nch; ++i) s0 += pp[(int64_t)i * ldp];
   y[(int64_t)r * ldy + o] = static_cast<Y>((s0 + s1) + (s2 + s3));
+}
+
+template <typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm_fin4(const int32_t *multi, int nrows, const Acc *partials,
+                                                     int64_t ldp, Y *y, int64_t ldy, int l) {
+  spmm_fin4_rows<Y, Acc>(multi, nrows, partials, ldp, y, ldy, l, blockIdx.x);
+}
+
+// All piece sums of one A^T Y with staged columns in one launch (l <= 64):
+// the table holds nch chunked rows, then n4 rows of a few segments, then
+// nwide staged rows (one piece per matrix-core workgroup). Workgroups
+// [0, nwide) sum the staged rows and [nwide, nwide + nch) the chunked rows,
+// one row each as k_spmm_fin does; the rest take four rows of a few segments
+// each as k_spmm_fin4 does.
+template <typename Y, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_spmm_fin_merged(const int32_t *multi, int nch, int n4, int nwide,
+                                                           const Acc *partials, int64_t ldp, Y *y, int64_t ldy,
+                                                           int l) {
+  const int w = blockIdx.x;
+  if (w < nwide) {
+    spmm_fin_row<Y, Acc>(multi, partials, ldp, y, ldy, l, nch + n4 + w);
+  } else if (w < nwide + nch) {
+    spmm_fin_row<Y, Acc>(multi, partials, ldp, y, ldy, l, w - nwide);
+  } else {
+    spmm_fin4_rows<Y, Acc>(multi + 3 * (int64_t)nch, n4, partials, ldp, y, ldy, l, w - nwide - nch);
+  }
 }
 
 // The same sum for rows with hundreds of pieces (l <= 64): 1024 threads per
@@ -2331,10 +2357,21 @@ static bool spmm_t_hot(svdh_op *op, const void *vals_t, const void *X_, int64_t 
   HIP_CHECK(hipGetLastError());
   const Schedule &sp = op->sched_thot;
   const int ncold = sp.nlong_rows - sp.hot_multi;  // the staged rows come last in the table
-  if (ncold > 0) launch_fin<float, float>(c, sp, ncold, p.a.ldp, static_cast<float *>(Y_), ldy, (int)l);
-  hipLaunchKernelGGL((k_spmm_fin_wide<float, float>), sp.hot_multi, 1024, 0, c->stream,
-                     sp.counters.as<int32_t>() + 3 * (int64_t)ncold, (const float *)sp.partials.p, p.a.ldp,
-                     static_cast<float *>(Y_), ldy, (int)l);
+  static const bool merged = [] {
+    const char *e = std::getenv("SVDH_SPMM_FIN_MERGE");
+    return !e || std::atoi(e) != 0;
+  }();
+  if (merged) {
+    const int nch = std::min<int>(ncold, sp.chunk_multi), n4 = ncold - nch;
+    hipLaunchKernelGGL((k_spmm_fin_merged<float, float>), sp.hot_multi + nch + (n4 + 3) / 4, kBlock, 0, c->stream,
+                       sp.counters.as<int32_t>(), nch, n4, sp.hot_multi, (const float *)sp.partials.p, p.a.ldp,
+                       static_cast<float *>(Y_), ldy, (int)l);
+  } else {
+    if (ncold > 0) launch_fin<float, float>(c, sp, ncold, p.a.ldp, static_cast<float *>(Y_), ldy, (int)l);
+    hipLaunchKernelGGL((k_spmm_fin_wide<float, float>), sp.hot_multi, 1024, 0, c->stream,
+                       sp.counters.as<int32_t>() + 3 * (int64_t)ncold, (const float *)sp.partials.p, p.a.ldp,
+                       static_cast<float *>(Y_), ldy, (int)l);
+  }
   HIP_CHECK(hipGetLastError());
   return true;
 }
