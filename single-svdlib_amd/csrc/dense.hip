@@ -1187,9 +1187,6 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 #ifndef SVDH_PANEL_BLOCK
 #define SVDH_PANEL_BLOCK 512
 #endif
-#ifndef SVDH_PANEL_SERIAL_STAGE
-#define SVDH_PANEL_SERIAL_STAGE 0
-#endif
 constexpr int kPanelBlock = SVDH_PANEL_BLOCK;  // threads per workgroup of the panel products
 constexpr int kMsStride = 68;  // LDS row of the staged factors (64 + 4: rows 4 apart start 16 banks apart)
 
@@ -1266,7 +1263,7 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
   // 16-byte read. The B operands are read from LDS at each step, which leaves
   // registers for more than one wave per SIMD.
   __shared__ float ms[NP][64 * kMsStride];
-  if constexpr (NP == 2 && GRAM && SVDH_PANEL_SERIAL_STAGE) {
+  if constexpr (NP == 2 && GRAM) {
     // one element at a time: the batched loads below leave this variant's
     // main loop short of registers (the compiler spills)
     for (int p = 0; p < NP; ++p) {
