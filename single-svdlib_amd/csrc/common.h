@@ -429,9 +429,13 @@ void panel_gemm_dev(Ctx *c, int64_t m, int64_t li, int64_t lo, const void *Y, in
 // one or two terms (M_p li_p x lo FP64 row-major in device memory, lo, li_p
 // <= 64); Zin is Z unless given (leading dimension ldzin). Z may be Y[0] when
 // beta = 0.
+// tri: every M_p is upper triangular (zero below the diagonal), so the
+// products skip the factor blocks that hold only zeros (same result).
 void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
                 const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
-                double beta, const float *Zin = nullptr, int64_t ldzin = 0);
+                double beta, const float *Zin = nullptr, int64_t ldzin = 0, bool tri = false);
+// M (rows x cols, row-major) is zero below the diagonal.
+bool upper_triangular(const double *M, int64_t rows, int64_t cols);
 // Z_b (rows[b] x lo[b]) = Z_b (rows[b] x li) M_b in place for count FP32
 // panels (shared leading dimension ld), M_b li x lo[b] FP64 row-major on the
 // host: the factors travel in upload slots and each group that fits one slot
@@ -449,7 +453,8 @@ void panel_mfma_gram_batched(Ctx *c, int64_t count, const int64_t *rows, int64_t
 // gram_ev) without reading Z again.
 void panel_mfma_gram(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
                      const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
-                     double beta, const float *Zin, int64_t ldzin, size_t gram_off, hipEvent_t gram_ev = nullptr);
+                     double beta, const float *Zin, int64_t ldzin, size_t gram_off, hipEvent_t gram_ev = nullptr,
+                     bool tri = false);
 // C (k1 x k2, FP64 row-major, device) = X1^T X2 for row-major panels (k1, k2 <= 64).
 void gram_cross(Ctx *c, int64_t m, int64_t k1, const void *X1, int64_t ld1, int64_t k2, const void *X2, int64_t ld2,
                 int dt, double *C_dev);

@@ -1254,7 +1254,10 @@ __device__ __forceinline__ void panel_rows_mask(const PanelTerm &t0, const Panel
 // lo x lo partial (both triangles) to part[blockIdx.x * lo * lo ...].
 // Workgroup bid of nbk; part is this workgroup's partial (RESID: lo
 // entries, GRAM: lo x lo).
-template <int NP, bool RESID, bool GRAM>
+// TRI: every M_p is upper triangular (the inverse Cholesky factors and their
+// products), so the 16 x 16 factor blocks below the diagonal, which hold
+// exact zeros, are not multiplied: 10 of the 16 block products per term.
+template <int NP, bool RESID, bool GRAM, bool TRI = false>
 __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_t rows, int lo, const PanelTerm &t0,
                                                 const PanelTerm &t1, float *z, int64_t ldz, const float *zin,
                                                 int64_t ldzin, float beta, double *part) {
@@ -1345,11 +1348,18 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
           const float *mr = &ms[p][(16 * g + 4 * kq + st) * kMsStride + rl + lds0];
-          const float4 b = make_float4(mr[0], mr[16], mr[32], mr[48]);
-          acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.x, acc[0], 0, 0, 0);
-          acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.y, acc[1], 0, 0, 0);
-          acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.z, acc[2], 0, 0, 0);
-          acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.w, acc[3], 0, 0, 0);
+          if (TRI) {
+            // column tiles q >= g only (rows 16 g .. of M are zero left of column 16 g)
+#pragma unroll
+            for (int q = g; q < 4; ++q)
+              acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], mr[16 * q], acc[q], 0, 0, 0);
+          } else {
+            const float4 b = make_float4(mr[0], mr[16], mr[32], mr[48]);
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.x, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.y, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.z, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(ya[p][g][st], b.w, acc[3], 0, 0, 0);
+          }
         }
     if (RESID) {
       // rows past the end were loaded as zeros, so they add nothing
@@ -1441,12 +1451,12 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
   }
 }
 
-template <int NP, bool RESID, bool GRAM = false>
+template <int NP, bool RESID, bool GRAM = false, bool TRI = false>
 __global__ __launch_bounds__(kPanelBlock) __attribute__((amdgpu_waves_per_eu(2)))
 void k_panel_mfma(int64_t rows, int lo, PanelTerm t0, PanelTerm t1, float *z, int64_t ldz, const float *zin,
                   int64_t ldzin, float beta, double *part) {
   double *pw = part ? part + (int64_t)blockIdx.x * (GRAM ? (int64_t)lo * lo : (int64_t)lo) : nullptr;
-  panel_mfma_body<NP, RESID, GRAM>(blockIdx.x, gridDim.x, rows, lo, t0, t1, z, ldz, zin, ldzin, beta, pw);
+  panel_mfma_body<NP, RESID, GRAM, TRI>(blockIdx.x, gridDim.x, rows, lo, t0, t1, z, ldz, zin, ldzin, beta, pw);
 }
 
 // Z_b = Y_b M_b for many panels in one launch (one term, beta 0; Z_b may be
@@ -1467,13 +1477,13 @@ struct PanelJob {
   double *gram;
 };
 
-template <bool GRAM>
+template <bool GRAM, bool TRI = false>
 __global__ __launch_bounds__(kPanelBlock) __attribute__((amdgpu_waves_per_eu(2)))
 void k_panel_mfma_batched(const PanelJob *jobs) {
   const PanelJob j = jobs[blockIdx.y];
   if ((int64_t)blockIdx.x >= j.nblk) return;
   double *pw = GRAM ? j.part + (int64_t)blockIdx.x * j.lo * j.lo : nullptr;
-  panel_mfma_body<1, false, GRAM>(blockIdx.x, j.nblk, j.rows, j.lo, j.t, j.t, j.z, j.ldz, j.z, j.ldz, 0.f, pw);
+  panel_mfma_body<1, false, GRAM, TRI>(blockIdx.x, j.nblk, j.rows, j.lo, j.t, j.t, j.z, j.ldz, j.z, j.ldz, 0.f, pw);
 }
 
 // k_gram_reduce for job b = blockIdx.y (grid x: reduce_grid of the widest job).
@@ -1482,6 +1492,13 @@ __global__ __launch_bounds__(kBlock) void k_gram_reduce_jobs(const PanelJob *job
   const int64_t ll = (int64_t)j.lo * j.lo;
   if ((int64_t)blockIdx.x * 16 >= ll) return;  // the whole workgroup leaves together
   gram_reduce_body((int)j.nblk, ll, j.part, j.gram, blockIdx.x);
+}
+
+bool upper_triangular(const double *M, int64_t rows, int64_t cols) {
+  for (int64_t k = 1; k < rows; ++k)
+    for (int64_t j = 0; j < std::min(k, cols); ++j)
+      if (M[k * cols + j] != 0.0) return false;
+  return true;
 }
 
 // Workgroups per CU of the panel kernels (SVDH_PANEL_WG, default 1): the
@@ -1502,7 +1519,7 @@ static int panel_wg_per_cu() {
 static void panel_mfma_impl(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
                             const int64_t *li, const double *const *M_dev, const double *alpha, float *Z,
                             int64_t ldz, double beta, const float *Zin, int64_t ldzin, int64_t gram_off,
-                            hipEvent_t gram_ev) {
+                            hipEvent_t gram_ev, bool tri) {
   if (!Zin) {
     Zin = Z;
     ldzin = ldz;
@@ -1530,18 +1547,21 @@ static void panel_mfma_impl(Ctx *c, int64_t rows, int64_t lo, int nterms, const 
   }
   {
     KTimer tm(c, KC_DENSE, bytes);
-    if (nterms == 1 && !gram)
-      hipLaunchKernelGGL((k_panel_mfma<1, false>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z, ldz,
-                         Zin, ldzin, (float)beta, nullptr);
-    else if (nterms == 1)
-      hipLaunchKernelGGL((k_panel_mfma<1, false, true>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[0], Z,
-                         ldz, Zin, ldzin, (float)beta, part);
-    else if (!gram)
-      hipLaunchKernelGGL((k_panel_mfma<2, false>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z, ldz,
-                         Zin, ldzin, (float)beta, nullptr);
-    else
-      hipLaunchKernelGGL((k_panel_mfma<2, false, true>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0], t[1], Z,
-                         ldz, Zin, ldzin, (float)beta, part);
+    const int sel = (nterms == 2 ? 4 : 0) + (gram ? 2 : 0) + (tri ? 1 : 0);
+#define SVDH_PANEL_LAUNCH(NP, GR, TR)                                                                             \
+  hipLaunchKernelGGL((k_panel_mfma<NP, false, GR, TR>), g, kPanelBlock, 0, c->stream, rows, (int)lo, t[0],       \
+                     t[NP - 1], Z, ldz, Zin, ldzin, (float)beta, GR ? part : nullptr)
+    switch (sel) {
+      case 0: SVDH_PANEL_LAUNCH(1, false, false); break;
+      case 1: SVDH_PANEL_LAUNCH(1, false, true); break;
+      case 2: SVDH_PANEL_LAUNCH(1, true, false); break;
+      case 3: SVDH_PANEL_LAUNCH(1, true, true); break;
+      case 4: SVDH_PANEL_LAUNCH(2, false, false); break;
+      case 5: SVDH_PANEL_LAUNCH(2, false, true); break;
+      case 6: SVDH_PANEL_LAUNCH(2, true, false); break;
+      default: SVDH_PANEL_LAUNCH(2, true, true); break;
+    }
+#undef SVDH_PANEL_LAUNCH
     HIP_CHECK(hipGetLastError());
     if (gram) {
       const int64_t ll = lo * lo;
@@ -1555,20 +1575,20 @@ static void panel_mfma_impl(Ctx *c, int64_t rows, int64_t lo, int nterms, const 
 
 void panel_mfma(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
                 const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
-                double beta, const float *Zin, int64_t ldzin) {
+                double beta, const float *Zin, int64_t ldzin, bool tri) {
   if (rows == 0 || lo == 0) return;
-  panel_mfma_impl(c, rows, lo, nterms, Y, ldy, li, M_dev, alpha, Z, ldz, beta, Zin, ldzin, -1, nullptr);
+  panel_mfma_impl(c, rows, lo, nterms, Y, ldy, li, M_dev, alpha, Z, ldz, beta, Zin, ldzin, -1, nullptr, tri);
 }
 
 void panel_mfma_gram(Ctx *c, int64_t rows, int64_t lo, int nterms, const float *const *Y, const int64_t *ldy,
                      const int64_t *li, const double *const *M_dev, const double *alpha, float *Z, int64_t ldz,
-                     double beta, const float *Zin, int64_t ldzin, size_t gram_off, hipEvent_t gram_ev) {
+                     double beta, const float *Zin, int64_t ldzin, size_t gram_off, hipEvent_t gram_ev, bool tri) {
   if (rows == 0 || lo == 0) {
     gram_async(c, rows, lo, Z, ldz, SVDH_F32, gram_off, gram_ev);  // the empty Gram, through the usual path
     return;
   }
   panel_mfma_impl(c, rows, lo, nterms, Y, ldy, li, M_dev, alpha, Z, ldz, beta, Zin, ldzin, (int64_t)gram_off,
-                  gram_ev);
+                  gram_ev, tri);
 }
 
 static void panel_mfma_batched_impl(Ctx *c, int64_t count, const int64_t *rows, int64_t li, const int64_t *lo,
@@ -1593,7 +1613,11 @@ static void panel_mfma_batched_impl(Ctx *c, int64_t count, const int64_t *rows, 
     if (b1 == b0) fail(SVDH_ERR_ARG, "panel_mfma_batched: factor larger than the upload slot");
     std::vector<double> mall;
     mall.reserve(doubles);
-    for (int64_t b = b0; b < b1; ++b) mall.insert(mall.end(), M[b].begin(), M[b].end());
+    bool tri = true;  // every factor of the group upper triangular
+    for (int64_t b = b0; b < b1; ++b) {
+      mall.insert(mall.end(), M[b].begin(), M[b].end());
+      tri = tri && upper_triangular(M[b].data(), li, lo[b]);
+    }
     const double *dm = static_cast<const double *>(c->upload(mall.data(), mall.size() * sizeof(double)));
     std::vector<PanelJob> jobs(b1 - b0);
     int64_t maxblk = 1, maxll = 1;
@@ -1628,12 +1652,17 @@ static void panel_mfma_batched_impl(Ctx *c, int64_t count, const int64_t *rows, 
       KTimer tm(c, KC_DENSE, bytes);
       const dim3 grid((unsigned)maxblk, (unsigned)(b1 - b0));
       if (gram) {
-        hipLaunchKernelGGL(k_panel_mfma_batched<true>, grid, kPanelBlock, 0, c->stream, dj);
+        if (tri)
+          hipLaunchKernelGGL((k_panel_mfma_batched<true, true>), grid, kPanelBlock, 0, c->stream, dj);
+        else
+          hipLaunchKernelGGL((k_panel_mfma_batched<true, false>), grid, kPanelBlock, 0, c->stream, dj);
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_gram_reduce_jobs, dim3((unsigned)reduce_grid(maxll), (unsigned)(b1 - b0)), kBlock, 0,
                            c->stream, dj);
+      } else if (tri) {
+        hipLaunchKernelGGL((k_panel_mfma_batched<false, true>), grid, kPanelBlock, 0, c->stream, dj);
       } else {
-        hipLaunchKernelGGL(k_panel_mfma_batched<false>, grid, kPanelBlock, 0, c->stream, dj);
+        hipLaunchKernelGGL((k_panel_mfma_batched<false, false>), grid, kPanelBlock, 0, c->stream, dj);
       }
       HIP_CHECK(hipGetLastError());
     }

@@ -402,12 +402,14 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
         const int64_t ldin[1] = {Z.ld}, liin[1] = {Y.w};
         const double *min_[1] = {dMq};
         const double al[1] = {1.0};
+        const bool tri = upper_triangular(Mq.data(), Y.w, wq);
         if (want_gz) {  // the Gram of Z from the same kernel
           panel_mfma_gram(ctx, n, wq, 1, yin, ldin, liin, min_, al, static_cast<float *>(Z.p()), Z.ld, 0.0, nullptr,
-                          0, 0);
+                          0, 0, nullptr, tri);
           gz_done = true;
         } else {
-          panel_mfma(ctx, n, wq, 1, yin, ldin, liin, min_, al, static_cast<float *>(Z.p()), Z.ld, 0.0);
+          panel_mfma(ctx, n, wq, 1, yin, ldin, liin, min_, al, static_cast<float *>(Z.p()), Z.ld, 0.0, nullptr, 0,
+                     tri);
         }
       } else {
         // Z holds A^T Y; Z = A^T Q = (A^T Y) Mq
@@ -440,8 +442,9 @@ int32_t rsvd_solve(svdh_ctx *ctx, svdh_op *op, int64_t target_rank, int64_t n_ov
           const int64_t ldin[2] = {Y.ld, Yp.ld}, liin[2] = {Y.w, Yp.w};
           const double *min_[2] = {dMq, dPM};
           const double al[2] = {-shift, -coef * shift * shift};
+          const bool tri = upper_triangular(Mq.data(), Y.w, wq) && (!mom || upper_triangular(PM.data(), Yp.w, wq));
           panel_mfma_gram(ctx, m, wq, mom ? 2 : 1, yin, ldin, liin, min_, al, static_cast<float *>(next->p()),
-                          next->ld, 1.0, static_cast<const float *>(Yn.p()), Yn.ld, goff, gev);
+                          next->ld, 1.0, static_cast<const float *>(Yn.p()), Yn.ld, goff, gev, tri);
           return true;
         } else {
           if (next != &Yn)
@@ -783,7 +786,8 @@ int32_t rsvd_batched_solve(svdh_ctx *ctx, int64_t nops, svdh_op *const *ops, int
         const int64_t ldin[1] = {ld}, liin[1] = {l};
         const double *min_[1] = {dM};
         const double al[1] = {1.0};
-        panel_mfma(c, nrows[b], wq[b], 1, yin, ldin, liin, min_, al, static_cast<float *>(Z[b]), ld, 0.0);
+        panel_mfma(c, nrows[b], wq[b], 1, yin, ldin, liin, min_, al, static_cast<float *>(Z[b]), ld, 0.0, nullptr,
+                   0, upper_triangular(Mq[b].data(), l, wq[b]));
       } else {
         panel_apply(c, nrows[b], l, wq[b], Z[b], ld, pdt, Mq[b].data(), Z[b], ld, pdt);
       }
