@@ -403,38 +403,59 @@ __global__ __launch_bounds__(kTiledBlock) void k_spmv_tiled(TiledArgs<V, X, Y> a
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int t = rg.y + w; t < rg.z; t += kTiledBlock / 64) {
-    const int4 tk = a.tasks[t];
+  constexpr int WV = kTiledBlock / 64;
+  // A wave takes tasks t, t + WV, ... of the range. While it forms one task's
+  // products, the next task's row bounds and first step of entries are in
+  // flight (their descriptor was loaded one task earlier), so a task costs
+  // its arithmetic rather than a memory round trip. Every load is
+  // unconditional at a clamped index (the descriptor of the range's last
+  // task, entry 0), and entries past a task's end are dropped where they are
+  // used.
+  auto ld_task = [&](int t) { return a.tasks[t < rg.z ? t : rg.z - 1]; };
+  auto ld_first = [&](const int4 &tk, int &p0, int &p1, int (&c)[kTiledU], V (&v)[kTiledU]) {
+    const int nr = tk.y - tk.x;
+    const int lg = nr > 1 ? 32 - __clz(nr - 1) : 0;
+    const int gid = lane >> (6 - lg);
+    const int q = gid < nr ? tk.x + gid : tk.x;
+    p0 = a.rowptr[q];
+    p1 = a.rowptr[q + 1];
+#pragma unroll
+    for (int j = 0; j < kTiledU; ++j) {
+      const int k = tk.z + lane + 64 * j;
+      const int ks = k < tk.w ? k : 0;
+      c[j] = a.col[ks];
+      v[j] = a.vals[ks];
+    }
+  };
+  int t = rg.y + w;
+  int4 tk = ld_task(t), tkn = ld_task(t + WV);
+  int p0, p1, c[kTiledU];
+  V v[kTiledU];
+  ld_first(tk, p0, p1, c, v);
+  for (; t < rg.z; t += WV) {
+    // the next task's first step (and the descriptor after it) in flight
+    int q0n, q1n, cn[kTiledU];
+    V vn[kTiledU];
+    ld_first(tkn, q0n, q1n, cn, vn);
+    const int4 tknn = ld_task(t + 2 * WV);
     const int qa = tk.x, qb = tk.y, kb = tk.z, ke = tk.w;
     if (qb > qa) {
       const int nr = qb - qa;
       const int lg = nr > 1 ? 32 - __clz(nr - 1) : 0;
       const int sh = 6 - lg;
       const int gid = lane >> sh, sub = lane & ((1 << sh) - 1);
-      // lanes past the last row load row qa's bounds (unconditional loads)
-      // and then take an empty range
-      const int q = gid < nr ? qa + gid : qa;
-      const int p0 = a.rowptr[q], p1 = a.rowptr[q + 1];
+      // lanes past the last row loaded row qa's bounds and take an empty range
       const int rs = gid < nr ? p0 - kb : 0, re = gid < nr ? p1 - kb : 0;
-      int c[kTiledU];
-      V v[kTiledU];
-#pragma unroll
-      for (int j = 0; j < kTiledU; ++j) {
-        const int k = kb + lane + 64 * j;
-        const bool ok = k < ke;
-        const int ks = ok ? k : 0;  // unconditional loads, all in flight together
-        c[j] = a.col[ks];
-        v[j] = a.vals[ks];
-        if (!ok) c[j] = -1;
-      }
       // the previous task's reads of this wave's products come first
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-      for (int j = 0; j < kTiledU; ++j)
+      for (int j = 0; j < kTiledU; ++j) {
         // entries past the task end land in slots no row reads: no select
-        prod[w][lane + 64 * j] = ld_as<double>(v, j) * (double)xs[c[j] >= 0 ? c[j] : 0];
+        const int cj = kb + lane + 64 * j < ke ? c[j] : 0;
+        prod[w][lane + 64 * j] = ld_as<double>(v, j) * (double)xs[cj];
+      }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -448,24 +469,25 @@ __global__ __launch_bounds__(kTiledBlock) void k_spmv_tiled(TiledArgs<V, X, Y> a
           a.part[qa + gid] = acc;
       }
     } else {
+      // one long row: the first step from the prefetched entries, then the
+      // rest, each step's loads all in flight before the first is used
       double acc = 0.0;
       for (int k0 = kb; k0 < ke; k0 += kTiledStep) {
-        int c[kTiledU];
-        V v[kTiledU];
+        if (k0 > kb) {
 #pragma unroll
-        for (int j = 0; j < kTiledU; ++j) {
-          const int k = k0 + lane + 64 * j;
-          const bool ok = k < ke;
-          const int ks = ok ? k : 0;
-          c[j] = a.col[ks];
-          v[j] = a.vals[ks];
-          if (!ok) c[j] = -1;
+          for (int j = 0; j < kTiledU; ++j) {
+            const int k = k0 + lane + 64 * j;
+            const int ks = k < ke ? k : 0;
+            c[j] = a.col[ks];
+            v[j] = a.vals[ks];
+          }
         }
 #pragma unroll
         for (int j = 0; j < kTiledU; ++j) {
           // unconditional LDS read at a clamped index, then a select
-          const double p = ld_as<double>(v, j) * (double)xs[c[j] >= 0 ? c[j] : 0];
-          acc += c[j] >= 0 ? p : 0.0;
+          const bool ok = k0 + lane + 64 * j < ke;
+          const double p = ld_as<double>(v, j) * (double)xs[ok ? c[j] : 0];
+          acc += ok ? p : 0.0;
         }
       }
       acc = wave_sum(acc);
@@ -475,6 +497,15 @@ __global__ __launch_bounds__(kTiledBlock) void k_spmv_tiled(TiledArgs<V, X, Y> a
         else
           a.part[qa] = acc;
       }
+    }
+    tk = tkn;
+    tkn = tknn;
+    p0 = q0n;
+    p1 = q1n;
+#pragma unroll
+    for (int j = 0; j < kTiledU; ++j) {
+      c[j] = cn[j];
+      v[j] = vn[j];
     }
   }
 }
