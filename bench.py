@@ -533,19 +533,25 @@ def _probe_lib():
     return lib
 
 
-def gather_ceilings(a, h=256):
+def gather_ceilings(a, h_ax=256, h_aty=None):
     """Gather ceilings (ms) of the two SpMM directions of the headline
-    workload, FP32 rows of 60 values: A X's unstaged column stream (the h most
-    popular columns come from LDS) and A^T Y's unstaged row stream grouped by
-    8 source blocks (as its segment schedule runs), best of U in (4, 8) at 16
-    waves per CU (the configurations the full sweep of --workload
-    gather-ceiling selects)."""
+    workload, FP32 rows of 60 values: A X's unstaged column stream (the h_ax
+    most popular columns come from LDS) and A^T Y's unstaged row stream (the
+    h_aty most popular columns run on the matrix cores; the library's choice,
+    SVDH_SPMM_THOT_H, default 128) grouped by 8 source blocks (as its segment
+    schedule runs), best of U in (4, 8) at 16 waves per CU (the
+    configurations the full sweep of --workload gather-ceiling selects)."""
+    if h_aty is None:
+        h_aty = 256 if os.environ.get("SVDH_SPMM_THOT_H") == "256" else 128
     lib = _probe_lib()
     m, n = a.shape
     counts = np.bincount(a.indices, minlength=n)
+    order = np.argsort(-counts, kind="stable")
     is_hot = np.zeros(n, dtype=bool)
-    is_hot[np.argsort(-counts, kind="stable")[:h]] = True
+    is_hot[order[:h_ax]] = True
     ax = np.ascontiguousarray(a.indices[~is_hot[a.indices]].astype(np.int32))
+    is_hot[:] = False
+    is_hot[order[:h_aty]] = True
     csc = a.tocsc()
     keep = np.repeat(~is_hot, np.diff(csc.indptr))
     aty = csc.indices[keep].astype(np.int32)

@@ -265,6 +265,7 @@ struct svdh_op {
   std::vector<int64_t> row_hist[2];
   int thot_state = -1;
   int thot_g = 0;
+  int thot_h = 0;  // staged columns of that product (0: not decided)
   // pattern = true: the base matrix's own pattern (with the column remap),
   // never the compacted one (the adjoint writes one value per base nonzero).
   svdh::CsrView view(int direction, bool pattern = false) const;

@@ -1217,40 +1217,48 @@ __global__ __launch_bounds__(kHotBlock) void k_spmm_hot(SpmmArgs<V, X, Y, Acc> a
 // fin pass sums those in workgroup order. LDS rows are 272 (tile) and 80 (Y)
 // floats apart, 16 banks, so the operand reads are free of bank conflicts.
 // ------------------------------------------------------------------------
-constexpr int kTHotRows = 32, kTHotH = 256, kTHotStride = 272, kTHotYStride = 80, kTHotBlock = 512;
+constexpr int kTHotRows = 32, kTHotYStride = 80, kTHotBlock = 512;
+#ifndef SVDH_THOT_DEFAULT_H
+#define SVDH_THOT_DEFAULT_H 128
+#endif
+constexpr int kTHotDefaultH = SVDH_THOT_DEFAULT_H;
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
-// Ad[r][slot] += staged entries of row r, in CSR order (one thread per row).
+// Ad[r][slot] += staged entries of row r, in CSR order (one thread per row);
+// Ad is m x h.
 __global__ void k_hot_dense(int64_t m, const int32_t *rowptr, const int32_t *hcol, const float *hval,
-                            const int32_t *split, float *Ad) {
+                            const int32_t *split, float *Ad, int h) {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m; r += (int64_t)gridDim.x * blockDim.x) {
     const int kb = rowptr[r], ke = kb + split[r];
-    for (int k = kb; k < ke; ++k) Ad[r * kTHotH + (-1 - hcol[k])] += hval[k];
+    for (int k = kb; k < ke; ++k) Ad[r * h + (-1 - hcol[k])] += hval[k];
   }
 }
 
+// T slot tiles of 16 per wave: h = 128 T staged columns.
+template <int T>
 __global__ __launch_bounds__(kTHotBlock) void k_spmm_t_hot(int64_t m, const float *Ad, const float *Y, int64_t ldy,
                                                           int l, float *partials, int64_t ldp, int32_t slot_base) {
-  __shared__ float tile[kTHotRows * kTHotStride];
+  constexpr int H = 128 * T, STRIDE = H + 16, VPR = H / 4, NA = 32 * VPR / kTHotBlock;
+  __shared__ float tile[kTHotRows * STRIDE];
   __shared__ float ytile[kTHotRows * kTHotYStride];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int kq = lane >> 4, rl = lane & 15;
-  f4_t acc[2][4];
+  f4_t acc[T][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < T; ++a)
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[a][q] = f4_t{0.f, 0.f, 0.f, 0.f};
   const int64_t nblocks = (m + kTHotRows - 1) / kTHotRows;
-  // per thread: 4 float4 of the 32 x 256 Ad block (row e / 64, vector e % 64)
+  // per thread: NA float4 of the 32 x H Ad block (row e / VPR, vector e % VPR)
   // and one float4 of the 32 x 64 Y block (row tid / 16, vector tid % 16)
-  float4 ra[4], ry;
+  float4 ra[NA], ry;
   const int yr = tid >> 4, yq = tid & 15;
   auto fetch = [&](int64_t r0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < NA; ++i) {
       const int e = tid + kTHotBlock * i;
-      const int64_t r = r0 + (e >> 6);
-      ra[i] = r < m ? reinterpret_cast<const float4 *>(Ad + r * kTHotH)[e & 63] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const int64_t r = r0 + e / VPR;
+      ra[i] = r < m ? reinterpret_cast<const float4 *>(Ad + r * H)[e % VPR] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     const int64_t r = r0 + yr;
     const int c = 4 * yq;
@@ -1262,9 +1270,9 @@ __global__ __launch_bounds__(kTHotBlock) void k_spmm_t_hot(int64_t m, const floa
   for (int64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
     __syncthreads();  // the previous block's operands are consumed
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < NA; ++i) {
       const int e = tid + kTHotBlock * i;
-      *reinterpret_cast<float4 *>(tile + (e >> 6) * kTHotStride + 4 * (e & 63)) = ra[i];
+      *reinterpret_cast<float4 *>(tile + (e / VPR) * STRIDE + 4 * (e % VPR)) = ra[i];
     }
     *reinterpret_cast<float4 *>(ytile + yr * kTHotYStride + 4 * yq) = ry;
     __syncthreads();
@@ -1272,24 +1280,25 @@ __global__ __launch_bounds__(kTHotBlock) void k_spmm_t_hot(int64_t m, const floa
 #pragma unroll
     for (int s = 0; s < kTHotRows / 4; ++s) {
       const int row = 4 * s + kq;
-      const float a0 = tile[row * kTHotStride + 16 * (2 * wave) + rl];
-      const float a1 = tile[row * kTHotStride + 16 * (2 * wave + 1) + rl];
+      float av[T];
+#pragma unroll
+      for (int a = 0; a < T; ++a) av[a] = tile[row * STRIDE + 16 * (T * wave + a) + rl];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const float bv = ytile[row * kTHotYStride + 16 * q + rl];
-        acc[0][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, acc[0][q], 0, 0, 0);
-        acc[1][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, acc[1][q], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < T; ++a) acc[a][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv, acc[a][q], 0, 0, 0);
       }
     }
   }
   const int G = gridDim.x;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < T; ++a)
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int slot = 16 * (2 * wave + a) + 4 * kq + v;
+        const int slot = 16 * (T * wave + a) + 4 * kq + v;
         const int col = 16 * q + rl;
         if (col < l) partials[((int64_t)slot_base + (int64_t)slot * G + blockIdx.x) * ldp + col] = acc[a][q][v];
       }
@@ -2353,13 +2362,19 @@ static void launch_fin(Ctx *c, const Schedule &s, int n, int64_t ldp, Y *y, int6
 static bool spmm_t_hot(svdh_op *op, const void *vals_t, const void *X_, int64_t ldx, void *Y_, int64_t ldy,
                        int64_t l) {
   (void)vals_t;
-  constexpr int kH = kTHotH;
   if (op->base || l > 64 || ldx % 4 || op->thot_state == 0) return false;
   const char *env = std::getenv("SVDH_SPMM_THOT");
   if (env && std::atoi(env) == 0) return false;
   Ctx *c = op->ctx;
   const void *vals0 = op->values(0, SVDH_F32);
   if (!vals0) return false;
+  if (op->thot_h == 0) {
+    // staged columns on the matrix cores: 128 or 256 (the MFMA work and the
+    // dense block grow with h, the gathered entries shrink)
+    op->thot_h = kTHotDefaultH;
+    if (const char *eh = std::getenv("SVDH_SPMM_THOT_H")) op->thot_h = std::atoi(eh) == 256 ? 256 : 128;
+  }
+  const int kH = op->thot_h;
   Schedule::HotEnc *e = hot_columns(op, get_spmm_schedule(op, 0), kH);
   if (!e || e->h != kH) return false;
   if (!op->sched_thot.built) {
@@ -2374,15 +2389,21 @@ static bool spmm_t_hot(svdh_op *op, const void *vals_t, const void *X_, int64_t 
   if (!plan_spmm(op, 1, vals_t, X_, ldx, Y_, ldy, l, p, &op->sched_thot)) return false;
   if (!e->dense.p) {
     const float *hval = hot_values<float>(c, *e, static_cast<const float *>(vals0), op->nnz);
-    e->dense.alloc(std::max<size_t>((size_t)op->nrows * kTHotH * sizeof(float), 16));
+    e->dense.alloc(std::max<size_t>((size_t)op->nrows * kH * sizeof(float), 16));
     HIP_CHECK(hipMemsetAsync(e->dense.p, 0, e->dense.bytes, c->stream));
     hipLaunchKernelGGL(k_hot_dense, grid_for(op->nrows), kBlock, 0, c->stream, op->nrows, op->rowptr.as<int32_t>(),
-                       e->colidx.as<int32_t>(), hval, e->split.as<int32_t>(), e->dense.as<float>());
+                       e->colidx.as<int32_t>(), hval, e->split.as<int32_t>(), e->dense.as<float>(), kH);
     HIP_CHECK(hipGetLastError());
   }
   KTimer t(c, KC_SPMM_T, p.bytes);
-  hipLaunchKernelGGL(k_spmm_t_hot, op->thot_g, kTHotBlock, 0, c->stream, op->nrows, e->dense.as<float>(),
-                     static_cast<const float *>(X_), ldx, (int)l, p.a.partials, p.a.ldp, op->sched_thot.hot_slot_base);
+  if (kH == 256)
+    hipLaunchKernelGGL(k_spmm_t_hot<2>, op->thot_g, kTHotBlock, 0, c->stream, op->nrows, e->dense.as<float>(),
+                       static_cast<const float *>(X_), ldx, (int)l, p.a.partials, p.a.ldp,
+                       op->sched_thot.hot_slot_base);
+  else
+    hipLaunchKernelGGL(k_spmm_t_hot<1>, op->thot_g, kTHotBlock, 0, c->stream, op->nrows, e->dense.as<float>(),
+                       static_cast<const float *>(X_), ldx, (int)l, p.a.partials, p.a.ldp,
+                       op->sched_thot.hot_slot_base);
   HIP_CHECK(hipGetLastError());
   spmm_dispatch<false>(c, p.a, p.lgr, p.blocks);
   HIP_CHECK(hipGetLastError());

@@ -267,15 +267,18 @@ def zipf_rows(m, n, per_row, seed, dtype):
     return a.astype(dtype)
 
 
-def test_spmm_transposed_hot_columns_on_matrix_cores(svdlib):
-    # FP32 A^T Y with the popular columns as a dense tile product on the
-    # matrix cores and the rest by the gather kernel: equals the exact product
-    # within the FP error bound, as does the path without the split (whose
-    # rounding differs, which shows that the split ran); rows past the last
-    # full 32-row block and empty columns included
+@pytest.mark.parametrize("h", [128, 256])
+def test_spmm_transposed_hot_columns_on_matrix_cores(svdlib, h, monkeypatch):
+    # FP32 A^T Y with the h most popular columns as a dense tile product on
+    # the matrix cores (one or two 16-slot tiles per wave) and the rest by the
+    # gather kernel: equals the exact product within the FP error bound, as
+    # does the path without the split (whose rounding differs, which shows
+    # that the split ran); rows past the last full 32-row block and empty
+    # columns included. The operator takes h on its first A^T Y.
     import os
+    monkeypatch.setenv("SVDH_SPMM_THOT_H", str(h))
     a = zipf_rows(70_001, 3000, 20, seed=9, dtype=np.float32)
-    print(f"case zipf rows {a.shape} nnz={a.nnz}", flush=True)
+    print(f"case zipf rows {a.shape} nnz={a.nnz} h={h}", flush=True)
     op = svdlib.CsrMatrix.from_scipy(a)
     a64 = a.astype(np.float64)
     rng = np.random.default_rng(4)
