@@ -533,14 +533,19 @@ def _probe_lib():
     return lib
 
 
-def gather_ceilings(a, h_ax=256, h_aty=None):
+def gather_ceilings(a, h_ax=None, h_aty=None):
     """Gather ceilings (ms) of the two SpMM directions of the headline
     workload, FP32 rows of 60 values: A X's unstaged column stream (the h_ax
-    most popular columns come from LDS) and A^T Y's unstaged row stream (the
-    h_aty most popular columns run on the matrix cores; the library's choice,
-    SVDH_SPMM_THOT_H, default 128) grouped by 8 source blocks (as its segment
-    schedule runs), best of U in (4, 8) at 16 waves per CU (the
-    configurations the full sweep of --workload gather-ceiling selects)."""
+    most popular columns come from LDS: as many 256-byte panel rows as
+    SVDH_SPMM_HOT_LDS bytes hold, default 81,920 -> 320) and A^T Y's unstaged
+    row stream (the h_aty most popular columns run on the matrix cores; the
+    library's choice, SVDH_SPMM_THOT_H, default 128) grouped by 8 source
+    blocks (as its segment schedule runs), best of U in (4, 8) at 16 waves per
+    CU (the configurations the full sweep of --workload gather-ceiling
+    selects)."""
+    if h_ax is None:
+        lds = int(os.environ.get("SVDH_SPMM_HOT_LDS", "81920"))
+        h_ax = min(1024, max(4096, min(155648, lds)) // 256)
     if h_aty is None:
         h_aty = 256 if os.environ.get("SVDH_SPMM_THOT_H") == "256" else 128
     lib = _probe_lib()

@@ -2438,7 +2438,9 @@ static void launch_spmm(svdh_op *op, int direction, const void *vals, const void
   if (direction == 0 && !p.mc && (uint64_t)op->ncols * (uint64_t)ldx * sizeof(X) < (1ull << 31)) {
     static const int64_t hot_lds = [] {  // LDS bytes for the staged panel rows (SVDH_SPMM_HOT_LDS)
       const char *e = std::getenv("SVDH_SPMM_HOT_LDS");
-      return e ? std::max<int64_t>(4096, std::min<int64_t>(155648, std::atoll(e))) : 65536;
+      // 80 KB: two workgroups still fit a CU's 160 KB (320 rows of l = 60 FP32: A X 110.1 -> 106.9 us
+      // against 64 KB at 100,000 x 20,000; one workgroup per CU, 96 KB and more, is slower)
+      return e ? std::max<int64_t>(4096, std::min<int64_t>(155648, std::atoll(e))) : 81920;
     }();
     Schedule::HotEnc *e = hot_columns(op, *p.s, std::min<int>(1024, (int)(hot_lds / (ldx * (int64_t)sizeof(X)))));
     if (e) {
