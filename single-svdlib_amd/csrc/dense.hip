@@ -1189,6 +1189,12 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 #endif
 constexpr int kPanelBlock = SVDH_PANEL_BLOCK;  // threads per workgroup of the panel products
 constexpr int kMsStride = 68;  // LDS row of the staged factors (64 + 4: rows 4 apart start 16 banks apart)
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format;
+// with stride 0 an access at byte offset >= the resource's size is out of
+// range: a load returns 0, a store is dropped).
+constexpr int kBufferWord3 = 0x00020000;
+constexpr int kBufferDrop = 0x7ffffff0;  // a byte offset past every resource range used here
+constexpr int64_t kBufferMaxLd = 1 << 24;  // leading dimensions whose 16-row blocks stay below kBufferDrop
 
 struct PanelTerm {
   const float *y;
@@ -1261,46 +1267,36 @@ template <int NP, bool RESID, bool GRAM, bool TRI = false>
 __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_t rows, int lo, const PanelTerm &t0,
                                                 const PanelTerm &t1, float *z, int64_t ldz, const float *zin,
                                                 int64_t ldzin, float beta, double *part) {
-  // alpha_p M_p in LDS as [k][rl][q] (element M[k][16 q + rl] at
-  // k * kMsStride + 4 rl + q): a lane's four column blocks for one k are one
-  // 16-byte read. The B operands are read from LDS at each step, which leaves
-  // registers for more than one wave per SIMD.
+  // alpha_p M_p in LDS, row k at k * kMsStride (zero padded to 64 x 64): for
+  // contraction index k a lane reads columns 16 q + rl, q = 0..3. The B
+  // operands are read from LDS at each step, which leaves registers for more
+  // than one wave per SIMD.
   __shared__ float ms[NP][64 * kMsStride];
-  if constexpr (NP == 2 && GRAM) {
-    // one element at a time: the batched loads below leave this variant's
-    // main loop short of registers (the compiler spills)
-    for (int p = 0; p < NP; ++p) {
-      const PanelTerm &t = p == 0 ? t0 : t1;
-      for (int e = threadIdx.x; e < 64 * 64; e += kPanelBlock) {
-        const int k = e >> 6, col = e & 63;
-        ms[p][k * kMsStride + col] = (k < t.li && col < lo) ? (float)(t.alpha * t.m[(int64_t)k * lo + col]) : 0.f;
-      }
+  // every load of the factors in flight before the first LDS write (a
+  // load-then-store loop waits out one memory latency per element)
+  constexpr int S = 64 * 64 / kPanelBlock;
+  for (int p = 0; p < NP; ++p) {
+    const PanelTerm &t = p == 0 ? t0 : t1;
+    double mv[S];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const int e = threadIdx.x + i * kPanelBlock, k = e >> 6, col = e & 63;
+      const bool ok = k < t.li && col < lo;
+      mv[i] = t.li > 0 ? t.m[ok ? (int64_t)k * lo + col : 0] : 0.0;
     }
-  } else {
-    // every load of the factors in flight before the first LDS write (a
-    // load-then-store loop waits out one memory latency per element)
-    constexpr int S = 64 * 64 / kPanelBlock;
-    for (int p = 0; p < NP; ++p) {
-      const PanelTerm &t = p == 0 ? t0 : t1;
-      double mv[S];
 #pragma unroll
-      for (int i = 0; i < S; ++i) {
-        const int e = threadIdx.x + i * kPanelBlock, k = e >> 6, col = e & 63;
-        const bool ok = k < t.li && col < lo;
-        mv[i] = t.li > 0 ? t.m[ok ? (int64_t)k * lo + col : 0] : 0.0;
-      }
-#pragma unroll
-      for (int i = 0; i < S; ++i) {
-        const int e = threadIdx.x + i * kPanelBlock, k = e >> 6, col = e & 63;
-        ms[p][k * kMsStride + col] = (k < t.li && col < lo) ? (float)(t.alpha * mv[i]) : 0.f;
-      }
+    for (int i = 0; i < S; ++i) {
+      const int e = threadIdx.x + i * kPanelBlock, k = e >> 6, col = e & 63;
+      ms[p][k * kMsStride + col] = (k < t.li && col < lo) ? (float)(t.alpha * mv[i]) : 0.f;
     }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int kq = lane >> 4, rl = lane & 15;
   const int64_t nblk = (rows + 15) / 16;
-  const int64_t wave = bid * (kPanelBlock / 64) + (threadIdx.x >> 6);
+  // the wave index in a scalar register: the block loop, its bounds and the
+  // stores' buffer resource below are wave-uniform
+  const int64_t wave = bid * (kPanelBlock / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t nwaves = nbk * (kPanelBlock / 64);
   f4_t ya[NP][4], yn[NP][4];
   double csq[4] = {0.0, 0.0, 0.0, 0.0};  // RESID: columns 16 q + rl
@@ -1311,8 +1307,9 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
 #pragma unroll
       for (int j = i; j < 4; ++j) gacc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
   // the beta term's rows of a block (lane: rows 4 kq + v, column 16 q + rl),
-  // loaded one block ahead like the inputs
-  float zold[4][4], znext[4][4];
+  // loaded as the block starts: they are first used by its stores, after the
+  // products
+  float zold[4][4];
   auto zin_load = [&](int64_t r0, float (&dst)[4][4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -1327,7 +1324,6 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
   if (wave < nblk) {
     panel_rows_load<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
     panel_rows_mask<NP>(t0, t1, rows, wave * 16 + rl, kq, ya);
-    if (use_zin) zin_load(wave * 16, zold);
   }
   for (int64_t b = wave; b < nblk; b += nwaves) {
     // an offset of zero the compiler cannot see through: the B reads below
@@ -1336,9 +1332,14 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
     asm volatile("v_mov_b32 %0, 0" : "=v"(lds0));
     const int64_t r0 = b * 16;
     const bool more = b + nwaves < nblk;
-    // the next block in flight (clamped rows past the end: in bounds, unused)
+    // this block's beta rows, then the next block in flight (clamped rows
+    // past the end: in bounds, unused); in this order the wait for the beta
+    // rows leaves the next block's loads outstanding
+    if (use_zin) zin_load(r0, zold);
     panel_rows_load<NP>(t0, t1, rows, (b + nwaves) * 16 + rl, kq, yn);
-    if (use_zin) zin_load((b + nwaves) * 16, znext);
+    // the loads stay here, ahead of the products (the scheduler would
+    // otherwise sink them to their first use, after the products)
+    __builtin_amdgcn_sched_barrier(0);
     f4_t acc[4] = {f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f}, f4_t{0.f, 0.f, 0.f, 0.f},
                    f4_t{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
@@ -1368,6 +1369,16 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
 #pragma unroll
         for (int v = 0; v < 4; ++v) csq[q] += (double)acc[q][v] * (double)acc[q][v];
     } else {
+      // The block's stores go through a buffer resource over its rows of Z:
+      // an element outside the panel (a row past the end, a column from lo)
+      // gets an offset past the resource's range, and the hardware drops the
+      // store. No store sits behind a branch: the memory counter counts loads
+      // and stores in issue order, so one store that may or may not have been
+      // issued makes the wait for the next block's loads wait for every store
+      // of this block too.
+      const int64_t nrow = rows - r0 < 16 ? rows - r0 : 16;
+      const __amdgpu_buffer_rsrc_t zrs =
+          __builtin_amdgcn_make_buffer_rsrc(z + r0 * ldz, (short)0, (int)(nrow * ldz * 4), kBufferWord3);
       float zv[4][4];  // the stored values; zero outside the panel (GRAM)
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -1377,7 +1388,9 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
           const int c = 16 * q + rl;
           const float val = (use_zin ? beta * zold[q][v] : 0.f) + acc[q][v];
           const bool ok = gr < rows && c < lo;
-          if (ok) z[gr * ldz + c] = val;
+          const int off = ok ? (int)(((4 * kq + v) * ldz + c) * 4) : kBufferDrop;
+          // (the builtin's data operand is a 32-bit integer: the float's bits)
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, val), zrs, off, 0, 0);
           zv[q][v] = ok ? val : 0.f;
         }
       if (GRAM)
@@ -1396,11 +1409,6 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
       for (int p = 0; p < NP; ++p)
 #pragma unroll
         for (int g = 0; g < 4; ++g) ya[p][g] = yn[p][g];
-      if (use_zin)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) zold[q][v] = znext[q][v];
     }
   }
   if (GRAM) {
@@ -1536,6 +1544,7 @@ static void panel_mfma_impl(Ctx *c, int64_t rows, int64_t lo, int nterms, const 
   }
   for (int p = 0; p < nterms; ++p)
     if (ldy[p] % 4) fail(SVDH_ERR_ARG, "panel_mfma: leading dimensions must be multiples of 4");
+  if (ldz >= kBufferMaxLd) fail(SVDH_ERR_ARG, "panel_mfma: output leading dimension too large");
   const int64_t nblk = (rows + 15) / 16;
   constexpr int W = kPanelBlock / 64;
   const int g =
@@ -1595,6 +1604,7 @@ static void panel_mfma_batched_impl(Ctx *c, int64_t count, const int64_t *rows, 
                                     float *const *Z, int64_t ld, const std::vector<double> *M,
                                     std::vector<std::vector<double>> *G) {
   if (li > 64 || ld % 4) fail(SVDH_ERR_ARG, "panel_mfma_batched: li <= 64 and ld a multiple of 4");
+  if (ld >= kBufferMaxLd) fail(SVDH_ERR_ARG, "panel_mfma_batched: leading dimension too large");
   constexpr int W = kPanelBlock / 64;
   const bool gram = G != nullptr;
   if (gram) G->assign(count, {});
