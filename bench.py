@@ -533,6 +533,17 @@ def _probe_lib():
     return lib
 
 
+def staged_columns():
+    """The library's choice of popular columns for the headline workload
+    (FP32 panel rows of 60 values, 256 bytes apart): A X stages as many as
+    SVDH_SPMM_HOT_LDS bytes of LDS hold (default 81,920 -> 320); A^T Y runs
+    SVDH_SPMM_THOT_H of them on the matrix cores (128 unless 256)."""
+    lds = int(os.environ.get("SVDH_SPMM_HOT_LDS", "81920"))
+    h_ax = max(4096, min(155648, lds)) // 256
+    h_aty = 256 if os.environ.get("SVDH_SPMM_THOT_H") == "256" else 128
+    return h_ax, h_aty
+
+
 def gather_ceilings(a, h_ax=None, h_aty=None):
     """Gather ceilings (ms) of the two SpMM directions of the headline
     workload, FP32 rows of 60 values: A X's unstaged column stream (the h_ax
@@ -543,11 +554,9 @@ def gather_ceilings(a, h_ax=None, h_aty=None):
     blocks (as its segment schedule runs), best of U in (4, 8) at 16 waves per
     CU (the configurations the full sweep of --workload gather-ceiling
     selects)."""
-    if h_ax is None:
-        lds = int(os.environ.get("SVDH_SPMM_HOT_LDS", "81920"))
-        h_ax = min(1024, max(4096, min(155648, lds)) // 256)
-    if h_aty is None:
-        h_aty = 256 if os.environ.get("SVDH_SPMM_THOT_H") == "256" else 128
+    d_ax, d_aty = staged_columns()
+    h_ax = d_ax if h_ax is None else h_ax
+    h_aty = d_aty if h_aty is None else h_aty
     lib = _probe_lib()
     m, n = a.shape
     counts = np.bincount(a.indices, minlength=n)
@@ -600,16 +609,19 @@ def bench_gather_ceiling(args):
     a = synth.config2(np.float32)
     m, n = a.shape
     counts = np.bincount(a.indices, minlength=n)
-    hot = np.argsort(-counts, kind="stable")[:256]
-    is_hot = np.zeros(n, dtype=bool)
-    is_hot[hot] = True
+    order = np.argsort(-counts, kind="stable")
+    h_ax, h_aty = staged_columns()
+    hot_ax = np.zeros(n, dtype=bool)
+    hot_ax[order[:h_ax]] = True
+    hot_aty = np.zeros(n, dtype=bool)
+    hot_aty[order[:h_aty]] = True
     csc = a.tocsc()
     streams = {
         "ax_all": (a.indices.astype(np.int32), n),
-        "ax_unstaged": (a.indices[~is_hot[a.indices]].astype(np.int32), n),
+        "ax_unstaged": (a.indices[~hot_ax[a.indices]].astype(np.int32), n),
         "aty_all": (csc.indices.astype(np.int32), m),
         "aty_unstaged": (np.concatenate([csc.indices[csc.indptr[c]:csc.indptr[c + 1]] for c in range(n)
-                                         if not is_hot[c]]).astype(np.int32), m),
+                                         if not hot_aty[c]]).astype(np.int32), m),
     }
     # the same unstaged streams grouped by eight equal source blocks, each
     # block gathered by the workgroups of one XCD (the SpMM segment schedule)
