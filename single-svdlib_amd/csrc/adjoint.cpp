@@ -50,6 +50,33 @@ void upload_panel(Ctx *c, DevBuf &buf, int64_t rows, int64_t cols, int64_t ld, i
   HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
+// Row-major panel (rows x ld, dtype dt, columns past cols zero) of the first
+// `cols` singular vectors held on the host in a solve's result layout:
+// vectors as columns (src is rows x src_cols row-major) or as rows (src is
+// src_rows x rows row-major, vector j at src + j * rows). The host block goes
+// over in one contiguous copy (the whole rows x src_cols matrix, or the first
+// cols rows; a strided copy from pageable memory moves one row at a time)
+// and the device selects, converts and transposes.
+void upload_factor(Ctx *c, DevBuf &buf, int64_t rows, int64_t cols, int64_t ld, int dt, const void *src, int sdt,
+                   int64_t src_cols, bool vectors_as_rows) {
+  const size_t es = dtype_size(dt), ses = dtype_size(sdt);
+  buf.alloc(std::max<size_t>((size_t)rows * ld * es, 64));
+  if (rows == 0) return;
+  HIP_CHECK(hipMemsetAsync(buf.p, 0, (size_t)rows * ld * es, c->stream));
+  if (cols == 0) return;
+  if (vectors_as_rows) {
+    DevBuf tmp(std::max<size_t>((size_t)rows * cols * ses, 64));
+    HIP_CHECK(hipMemcpyAsync(tmp.p, src, (size_t)rows * cols * ses, hipMemcpyHostToDevice, c->stream));
+    panel_transpose_convert(c, cols, rows, tmp.p, rows, sdt, buf.p, ld, dt);
+    HIP_CHECK(hipStreamSynchronize(c->stream));  // tmp is released here
+  } else {
+    DevBuf tmp(std::max<size_t>((size_t)rows * src_cols * ses, 64));
+    HIP_CHECK(hipMemcpyAsync(tmp.p, src, (size_t)rows * src_cols * ses, hipMemcpyHostToDevice, c->stream));
+    panel_convert(c, rows, cols, tmp.p, src_cols, sdt, buf.p, ld, dt);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+}
+
 void to_host(Ctx *c, const double *dev, int64_t n, std::vector<double> &out) {
   out.resize(n);
   HIP_CHECK(hipMemcpyAsync(out.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -107,8 +134,10 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
     return u_rows_are_vectors ? u_at(j * m + i) : u_at(i * d + j);
   };
   DevBuf U, V, G;
-  upload_panel(c, U, m, k, ld, dt, uget);
-  upload_panel(c, V, n, k, ld, dt, [&](int64_t i, int64_t j) { return vt_at(j * n + i); });
+  upload_factor(c, U, m, k, ld, dt, r64 ? (const void *)res->u : (const void *)res->u32, r64 ? SVDH_F64 : SVDH_F32,
+                d, u_rows_are_vectors);
+  upload_factor(c, V, n, k, ld, dt, r64 ? (const void *)res->vt : (const void *)res->vt32, r64 ? SVDH_F64 : SVDH_F32,
+                n, true);
   const int64_t nnz = op->base ? op->base->nnz : op->nnz;
   G.alloc(std::max<size_t>((size_t)nnz * dtype_size(grad_dtype), 16));
   std::memset(info, 0, sizeof *info);

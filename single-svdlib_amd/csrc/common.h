@@ -463,6 +463,10 @@ void gram_cross(Ctx *c, int64_t m, int64_t k1, const void *X1, int64_t ld1, int6
 // row-major FP64 panel X (rows x cols); scale is a device vector or null.
 void panel_transpose_scale(Ctx *c, int64_t rows, int64_t cols, const double *X, int64_t ldx, const double *scale_dev,
                            double *out, int64_t ldo);
+// Y (cols x rows, leading dimension ldy, dtype ydt) = X^T for a row-major
+// device matrix X (rows x cols, dtype xdt), converted; FP32 / FP64.
+void panel_transpose_convert(Ctx *c, int64_t rows, int64_t cols, const void *X, int64_t ldx, int xdt, void *Y,
+                             int64_t ldy, int ydt);
 // out[c] = sum_i X[i, c] Y[i, c] for k columns (device FP64 vector).
 void panel_coldots(Ctx *c, int64_t m, int64_t k, const void *X, const void *Y, int64_t ld, int dt, double *out_dev);
 // W += alpha_c P, R -= alpha_c MP per column; rr[c] = |R[:, c]|^2 afterwards (device).

@@ -2317,6 +2317,47 @@ void panel_convert(Ctx *c, int64_t m, int64_t l, const void *X, int64_t ldx, int
   HIP_CHECK(hipGetLastError());
 }
 
+// Y[c * ldy + r] = X[r * ldx + c] (converted) for r < rows, c < cols: an
+// LDS-tiled 64 x 64 transpose.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void k_transpose_cvt(int64_t rows, int64_t cols, const TI *X, int64_t ldx,
+                                                          TO *Y, int64_t ldy) {
+  __shared__ TI t[64][65];
+  const int64_t r0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int rr = ty; rr < 64; rr += kBlock / 64) {
+    const int64_t r = r0 + rr, cc = c0 + tx;
+    t[rr][tx] = (r < rows && cc < cols) ? X[r * ldx + cc] : TI(0);
+  }
+  __syncthreads();
+  for (int cc = ty; cc < 64; cc += kBlock / 64) {
+    const int64_t col = c0 + cc, r = r0 + tx;
+    if (col < cols && r < rows) Y[col * ldy + r] = static_cast<TO>(t[tx][cc]);
+  }
+}
+
+void panel_transpose_convert(Ctx *c, int64_t rows, int64_t cols, const void *X, int64_t ldx, int xdt, void *Y,
+                             int64_t ldy, int ydt) {
+  if (rows == 0 || cols == 0) return;
+  if (rows > (int64_t)64 * 0x7fffffff || cols > (int64_t)64 * 65535) fail(SVDH_ERR_ARG, "transpose too large");
+  dim3 g((unsigned)((rows + 63) / 64), (unsigned)((cols + 63) / 64));
+  if (xdt == SVDH_F32 && ydt == SVDH_F32)
+    hipLaunchKernelGGL((k_transpose_cvt<float, float>), g, kBlock, 0, c->stream, rows, cols, (const float *)X, ldx,
+                       (float *)Y, ldy);
+  else if (xdt == SVDH_F64 && ydt == SVDH_F32)
+    hipLaunchKernelGGL((k_transpose_cvt<double, float>), g, kBlock, 0, c->stream, rows, cols, (const double *)X, ldx,
+                       (float *)Y, ldy);
+  else if (xdt == SVDH_F32 && ydt == SVDH_F64)
+    hipLaunchKernelGGL((k_transpose_cvt<float, double>), g, kBlock, 0, c->stream, rows, cols, (const float *)X, ldx,
+                       (double *)Y, ldy);
+  else if (xdt == SVDH_F64 && ydt == SVDH_F64)
+    hipLaunchKernelGGL((k_transpose_cvt<double, double>), g, kBlock, 0, c->stream, rows, cols, (const double *)X,
+                       ldx, (double *)Y, ldy);
+  else
+    fail(SVDH_ERR_ARG, "unsupported transpose %d -> %d", xdt, ydt);
+  HIP_CHECK(hipGetLastError());
+}
+
 void panel_fill(Ctx *c, int64_t m, int64_t l, double v, void *Y, int64_t ldy, int ydt) {
   const int g = grid_of(m * l);
   if (ydt == SVDH_F64)

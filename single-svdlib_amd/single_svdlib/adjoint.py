@@ -28,16 +28,24 @@ def backward(matrix, rec, loss="sigma_sum", k=None, y=None, precision=None, tol=
         prec = _ffi.PREC_F64 if op.dtype == np.float64 else _ffi.PREC_F32
     else:
         prec = {"f64": _ffi.PREC_F64, "f32": _ffi.PREC_F32, "bf16": _ffi.PREC_BF16}[precision]
-    u = np.ascontiguousarray(rec.u, dtype=np.float64)
-    s = np.ascontiguousarray(rec.s, dtype=np.float64)
-    vt = np.ascontiguousarray(rec.vt, dtype=np.float64)
+    # single-precision factors go over as they are (the library reads either)
+    single = all(np.asarray(a).dtype == np.float32 for a in (rec.u, rec.s, rec.vt))
+    fdt = np.float32 if single else np.float64
+    u = np.ascontiguousarray(rec.u, dtype=fdt)
+    s = np.ascontiguousarray(rec.s, dtype=fdt)
+    vt = np.ascontiguousarray(rec.vt, dtype=fdt)
     res = _ffi.Result_()
     res.d = int(rec.d)
     res.u_rows, res.u_cols = u.shape
     res.vt_rows, res.vt_cols = vt.shape
-    res.u = _ffi.ptr(u)
-    res.s = _ffi.ptr(s)
-    res.vt = _ffi.ptr(vt)
+    if single:
+        res.u32 = u.ctypes.data_as(C.POINTER(C.c_float))
+        res.s32 = s.ctypes.data_as(C.POINTER(C.c_float))
+        res.vt32 = vt.ctypes.data_as(C.POINTER(C.c_float))
+    else:
+        res.u = _ffi.ptr(u)
+        res.s = _ffi.ptr(s)
+        res.vt = _ffi.ptr(vt)
     yv = None
     if y is not None:
         yv = np.ascontiguousarray(y, dtype=np.float64)
