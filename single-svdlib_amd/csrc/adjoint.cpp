@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -75,6 +76,38 @@ void upload_factor(Ctx *c, DevBuf &buf, int64_t rows, int64_t cols, int64_t ld, 
     panel_convert(c, rows, cols, tmp.p, src_cols, sdt, buf.p, ld, dt);
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
+}
+
+// bytes of device memory to a host destination that may be pageable: one DMA
+// into a pinned buffer from the result pool, then up to 8 host threads copy
+// it out (the runtime's own pageable copy stages through one thread; the
+// first touch of a fresh destination's pages is what it waits on).
+void to_host_pageable(Ctx *c, void *dst, const void *src, size_t bytes) {
+  if (bytes < ((size_t)4 << 20)) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return;
+  }
+  char *stage = static_cast<char *>(result_alloc_bytes(bytes));
+  try {
+    HIP_CHECK(hipMemcpyAsync(stage, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    result_release(stage);
+    throw;
+  }
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min<size_t>(8, hw), bytes >> 20);
+  const size_t per = (bytes + nt - 1) / nt;
+  auto part = [&](size_t t) {
+    const size_t b = std::min(bytes, t * per), e = std::min(bytes, b + per);
+    if (e > b) std::memcpy(static_cast<char *>(dst) + b, stage + b, e - b);
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; ++t) pool.emplace_back(part, t);
+  part(0);
+  for (auto &t : pool) t.join();
+  result_release(stage);
 }
 
 void to_host(Ctx *c, const double *dev, int64_t n, std::vector<double> &out) {
@@ -242,8 +275,7 @@ int32_t backward_solve(svdh_ctx *ctx, svdh_op *op, const svdh_result *res, int32
     sddmm(op, k, sddmm_factor(W.p, m), sddmm_factor(V.p, n), s1_d, sddmm_factor(U.p, m), sddmm_factor(Z.p, n),
           s2_d, ld, ld, sdt, G.p, grad_dtype);
   }
-  HIP_CHECK(hipMemcpyAsync(grad, G.p, (size_t)nnz * dtype_size(grad_dtype), hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipStreamSynchronize(c->stream));
+  to_host_pageable(c, grad, G.p, (size_t)nnz * dtype_size(grad_dtype));
   return SVDH_OK;
 }
 
