@@ -51,31 +51,19 @@ void upload_panel(Ctx *c, DevBuf &buf, int64_t rows, int64_t cols, int64_t ld, i
   HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
-// Row-major panel (rows x ld, dtype dt, columns past cols zero) of the first
-// `cols` singular vectors held on the host in a solve's result layout:
-// vectors as columns (src is rows x src_cols row-major) or as rows (src is
-// src_rows x rows row-major, vector j at src + j * rows). The host block goes
-// over in one contiguous copy (the whole rows x src_cols matrix, or the first
-// cols rows; a strided copy from pageable memory moves one row at a time)
-// and the device selects, converts and transposes.
-void upload_factor(Ctx *c, DevBuf &buf, int64_t rows, int64_t cols, int64_t ld, int dt, const void *src, int sdt,
-                   int64_t src_cols, bool vectors_as_rows) {
-  const size_t es = dtype_size(dt), ses = dtype_size(sdt);
-  buf.alloc(std::max<size_t>((size_t)rows * ld * es, 64));
-  if (rows == 0) return;
-  HIP_CHECK(hipMemsetAsync(buf.p, 0, (size_t)rows * ld * es, c->stream));
-  if (cols == 0) return;
-  if (vectors_as_rows) {
-    DevBuf tmp(std::max<size_t>((size_t)rows * cols * ses, 64));
-    HIP_CHECK(hipMemcpyAsync(tmp.p, src, (size_t)rows * cols * ses, hipMemcpyHostToDevice, c->stream));
-    panel_transpose_convert(c, cols, rows, tmp.p, rows, sdt, buf.p, ld, dt);
-    HIP_CHECK(hipStreamSynchronize(c->stream));  // tmp is released here
-  } else {
-    DevBuf tmp(std::max<size_t>((size_t)rows * src_cols * ses, 64));
-    HIP_CHECK(hipMemcpyAsync(tmp.p, src, (size_t)rows * src_cols * ses, hipMemcpyHostToDevice, c->stream));
-    panel_convert(c, rows, cols, tmp.p, src_cols, sdt, buf.p, ld, dt);
-    HIP_CHECK(hipStreamSynchronize(c->stream));
-  }
+// memcpy by up to 8 host threads (1 MB or more each)
+void parallel_copy(void *dst, const void *src, size_t bytes) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::max<size_t>(1, std::min<size_t>(std::min<size_t>(8, hw), bytes >> 20));
+  const size_t per = (bytes + nt - 1) / nt;
+  auto part = [&](size_t t) {
+    const size_t b = std::min(bytes, t * per), e = std::min(bytes, b + per);
+    if (e > b) std::memcpy(static_cast<char *>(dst) + b, static_cast<const char *>(src) + b, e - b);
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; ++t) pool.emplace_back(part, t);
+  part(0);
+  for (auto &t : pool) t.join();
 }
 
 // bytes of device memory to a host destination that may be pageable: one DMA
@@ -96,18 +84,55 @@ void to_host_pageable(Ctx *c, void *dst, const void *src, size_t bytes) {
     result_release(stage);
     throw;
   }
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>(std::min<size_t>(8, hw), bytes >> 20);
-  const size_t per = (bytes + nt - 1) / nt;
-  auto part = [&](size_t t) {
-    const size_t b = std::min(bytes, t * per), e = std::min(bytes, b + per);
-    if (e > b) std::memcpy(static_cast<char *>(dst) + b, stage + b, e - b);
-  };
-  std::vector<std::thread> pool;
-  for (size_t t = 1; t < nt; ++t) pool.emplace_back(part, t);
-  part(0);
-  for (auto &t : pool) t.join();
+  parallel_copy(dst, stage, bytes);
   result_release(stage);
+}
+
+// The reverse: a host source that may be pageable copied into a pooled
+// pinned buffer by up to 8 host threads, then one DMA.
+void to_device_pageable(Ctx *c, void *dst, const void *src, size_t bytes) {
+  if (bytes < ((size_t)4 << 20)) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return;
+  }
+  char *stage = static_cast<char *>(result_alloc_bytes(bytes));
+  parallel_copy(stage, src, bytes);
+  try {
+    HIP_CHECK(hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  } catch (...) {
+    result_release(stage);
+    throw;
+  }
+  result_release(stage);
+}
+
+// Row-major panel (rows x ld, dtype dt, columns past cols zero) of the first
+// `cols` singular vectors held on the host in a solve's result layout:
+// vectors as columns (src is rows x src_cols row-major) or as rows (src is
+// src_rows x rows row-major, vector j at src + j * rows). The host block goes
+// over in one contiguous copy (the whole rows x src_cols matrix, or the first
+// cols rows; a strided copy from pageable memory moves one row at a time)
+// and the device selects, converts and transposes.
+void upload_factor(Ctx *c, DevBuf &buf, int64_t rows, int64_t cols, int64_t ld, int dt, const void *src, int sdt,
+                   int64_t src_cols, bool vectors_as_rows) {
+  const size_t es = dtype_size(dt), ses = dtype_size(sdt);
+  buf.alloc(std::max<size_t>((size_t)rows * ld * es, 64));
+  if (rows == 0) return;
+  HIP_CHECK(hipMemsetAsync(buf.p, 0, (size_t)rows * ld * es, c->stream));
+  if (cols == 0) return;
+  if (vectors_as_rows) {
+    DevBuf tmp(std::max<size_t>((size_t)rows * cols * ses, 64));
+    to_device_pageable(c, tmp.p, src, (size_t)rows * cols * ses);
+    panel_transpose_convert(c, cols, rows, tmp.p, rows, sdt, buf.p, ld, dt);
+    HIP_CHECK(hipStreamSynchronize(c->stream));  // tmp is released here
+  } else {
+    DevBuf tmp(std::max<size_t>((size_t)rows * src_cols * ses, 64));
+    to_device_pageable(c, tmp.p, src, (size_t)rows * src_cols * ses);
+    panel_convert(c, rows, cols, tmp.p, src_cols, sdt, buf.p, ld, dt);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
 }
 
 void to_host(Ctx *c, const double *dev, int64_t n, std::vector<double> &out) {
