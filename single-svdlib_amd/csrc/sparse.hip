@@ -950,6 +950,8 @@ __device__ __forceinline__ void spmm_chunk(const SpmmArgs<V, X, Y, Acc> &a, int 
 // adds the chunk partials in chunk order (4 interleaved chains per column,
 // combined in a fixed order). The kernel boundary makes the partials of the
 // first pass visible, so no fences or atomics are needed.
+// In a workgroup of more than kBlock threads the others only take part in
+// the barriers (the sums are the same bits as with kBlock threads).
 template <typename Y, typename Acc>
 __device__ __forceinline__ void spmm_fin_row(const int32_t *multi, const Acc *partials, int64_t ldp, Y *y,
                                              int64_t ldy, int l, int row) {
@@ -957,11 +959,12 @@ __device__ __forceinline__ void spmm_fin_row(const int32_t *multi, const Acc *pa
   const int *md = multi + 3 * (int64_t)row;
   const int r = md[0], first = md[1], nch = md[2];
   const int q = threadIdx.x >> 6, o = threadIdx.x & 63;
+  const bool act = threadIdx.x < kBlock;
   for (int c0 = 0; c0 < l; c0 += 64) {
     const int cc = c0 + o;
     // thread q sums pieces q, q + 4, ... in four interleaved chains
     Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
-    if (cc < l) {
+    if (act && cc < l) {
       const Acc *pp = partials + (int64_t)first * ldp + cc;
       int i = q;
       for (; i + 12 < nch; i += 16) {
@@ -974,7 +977,7 @@ __device__ __forceinline__ void spmm_fin_row(const int32_t *multi, const Acc *pa
     }
     const Acc sum = (s0 + s1) + (s2 + s3);
     __syncthreads();
-    red[q * 64 + o] = sum;
+    if (act) red[q * 64 + o] = sum;
     __syncthreads();
     if (q == 0 && cc < l) y[(int64_t)r * ldy + cc] = static_cast<Y>(((red[o] + red[64 + o]) + red[128 + o]) + red[192 + o]);
   }
@@ -989,10 +992,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin(const int32_t *multi, const
 // The same sum for panels of width l <= 64, four rows per workgroup: one
 // thread per column adds the row's pieces in four interleaved chains (piece
 // i goes to chain i mod 4), combined in a fixed order.
-template <typename Y, typename Acc>
+template <typename Y, typename Acc, int RPB = 4>
 __device__ __forceinline__ void spmm_fin4_rows(const int32_t *multi, int nrows, const Acc *partials, int64_t ldp,
                                                Y *y, int64_t ldy, int l, int blk) {
-  const int row = blk * 4 + (threadIdx.x >> 6), o = threadIdx.x & 63;
+  const int row = blk * RPB + (threadIdx.x >> 6), o = threadIdx.x & 63;
   if (row >= nrows || o >= l) return;
   const int *md = multi + 3 * (int64_t)row;
   const int r = md[0], first = md[1], nch = md[2];
@@ -1015,34 +1018,12 @@ __global__ __launch_bounds__(kBlock) void k_spmm_fin4(const int32_t *multi, int 
   spmm_fin4_rows<Y, Acc>(multi, nrows, partials, ldp, y, ldy, l, blockIdx.x);
 }
 
-// All piece sums of one A^T Y with staged columns in one launch (l <= 64):
-// the table holds nch chunked rows, then n4 rows of a few segments, then
-// nwide staged rows (one piece per matrix-core workgroup). Workgroups
-// [0, nwide) sum the staged rows and [nwide, nwide + nch) the chunked rows,
-// one row each as k_spmm_fin does; the rest take four rows of a few segments
-// each as k_spmm_fin4 does.
-template <typename Y, typename Acc>
-__global__ __launch_bounds__(kBlock) void k_spmm_fin_merged(const int32_t *multi, int nch, int n4, int nwide,
-                                                           const Acc *partials, int64_t ldp, Y *y, int64_t ldy,
-                                                           int l) {
-  const int w = blockIdx.x;
-  if (w < nwide) {
-    spmm_fin_row<Y, Acc>(multi, partials, ldp, y, ldy, l, nch + n4 + w);
-  } else if (w < nwide + nch) {
-    spmm_fin_row<Y, Acc>(multi, partials, ldp, y, ldy, l, w - nwide);
-  } else {
-    spmm_fin4_rows<Y, Acc>(multi + 3 * (int64_t)nch, n4, partials, ldp, y, ldy, l, w - nwide - nch);
-  }
-}
-
 // The same sum for rows with hundreds of pieces (l <= 64): 1024 threads per
 // row, thread q of a column summing pieces q, q + 16, ... in four chains,
-// the 16 thread sums then added in q order.
+// the 16 thread sums then added in q order. red holds 1024 sums.
 template <typename Y, typename Acc>
-__global__ __launch_bounds__(1024) void k_spmm_fin_wide(const int32_t *multi, const Acc *partials, int64_t ldp, Y *y,
-                                                       int64_t ldy, int l) {
-  __shared__ Acc red[1024];
-  const int *md = multi + 3 * (int64_t)blockIdx.x;
+__device__ __forceinline__ void spmm_fin_wide_row(const int32_t *md, const Acc *partials, int64_t ldp, Y *y,
+                                                  int64_t ldy, int l, Acc *red) {
   const int r = md[0], first = md[1], nch = md[2];
   const int q = threadIdx.x >> 6, o = threadIdx.x & 63;
   Acc s0 = Acc(0), s1 = Acc(0), s2 = Acc(0), s3 = Acc(0);
@@ -1063,6 +1044,36 @@ __global__ __launch_bounds__(1024) void k_spmm_fin_wide(const int32_t *multi, co
     Acc t = Acc(0);
     for (int k = 0; k < 16; ++k) t += red[k * 64 + o];
     y[(int64_t)r * ldy + o] = static_cast<Y>(t);
+  }
+}
+
+template <typename Y, typename Acc>
+__global__ __launch_bounds__(1024) void k_spmm_fin_wide(const int32_t *multi, const Acc *partials, int64_t ldp, Y *y,
+                                                       int64_t ldy, int l) {
+  __shared__ Acc red[1024];
+  spmm_fin_wide_row<Y, Acc>(multi + 3 * (int64_t)blockIdx.x, partials, ldp, y, ldy, l, red);
+}
+
+// All piece sums of one A^T Y with staged columns in one launch of
+// 1024-thread workgroups (l <= 64), each sum the same bits as in the
+// separate kernels: the table holds nch chunked rows, then n4 rows of a few
+// segments, then nwide staged rows (one piece per matrix-core workgroup).
+// Workgroups [0, nwide) take one staged row each as k_spmm_fin_wide does
+// (the staged rows have the most pieces: 16 threads per column),
+// [nwide, nwide + nch) one chunked row each as k_spmm_fin does (its first
+// kBlock threads), the rest 16 rows of a few segments each as k_spmm_fin4.
+template <typename Y, typename Acc>
+__global__ __launch_bounds__(1024) void k_spmm_fin_merged(const int32_t *multi, int nch, int n4, int nwide,
+                                                         const Acc *partials, int64_t ldp, Y *y, int64_t ldy,
+                                                         int l) {
+  __shared__ Acc red[1024];
+  const int w = blockIdx.x;
+  if (w < nwide) {
+    spmm_fin_wide_row<Y, Acc>(multi + 3 * (int64_t)(nch + n4 + w), partials, ldp, y, ldy, l, red);
+  } else if (w < nwide + nch) {
+    spmm_fin_row<Y, Acc>(multi, partials, ldp, y, ldy, l, w - nwide);
+  } else {
+    spmm_fin4_rows<Y, Acc, 16>(multi + 3 * (int64_t)nch, n4, partials, ldp, y, ldy, l, w - nwide - nch);
   }
 }
 
@@ -2495,7 +2506,7 @@ static bool spmm_t_hot(svdh_op *op, const void *vals_t, const void *X_, int64_t 
   }();
   if (merged) {
     const int nch = std::min<int>(ncold, sp.chunk_multi), n4 = ncold - nch;
-    hipLaunchKernelGGL((k_spmm_fin_merged<float, float>), sp.hot_multi + nch + (n4 + 3) / 4, kBlock, 0, c->stream,
+    hipLaunchKernelGGL((k_spmm_fin_merged<float, float>), sp.hot_multi + nch + (n4 + 15) / 16, 1024, 0, c->stream,
                        sp.counters.as<int32_t>(), nch, n4, sp.hot_multi, (const float *)sp.partials.p, p.a.ldp,
                        static_cast<float *>(Y_), ldy, (int)l);
   } else {
