@@ -1412,27 +1412,46 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
     }
   }
   if (GRAM) {
-    // waves in order into LDS: gred[(blk * 4 + v) * 64 + lane], blk = 4 i + j
-    __shared__ double gred[16 * 4 * 64];
+    // The waves' accumulators summed in a fixed tree through two LDS slots
+    // (block (i, j), i <= j, at (t * 4 + v) * 64 + lane, t = i (7 - i) / 2 + j):
+    // at each level h = W/2 .. 1, waves h + s and h + s + 1 store into the
+    // slots and waves s and s + 1 add them, for s = 0, 2, .. < h; wave 0 then
+    // stores the total. Eight waves: ((w0 + w4) + (w2 + w6)) + ((w1 + w5) +
+    // (w3 + w7)), four store/add phases instead of eight serial ones.
+    constexpr int W = kPanelBlock / 64;
+    static_assert((W & (W - 1)) == 0, "a power-of-two number of waves");
+    __shared__ double gsl[2][10 * 4 * 64];
     const int wv = threadIdx.x >> 6;
-    for (int w = 0; w < kPanelBlock / 64; ++w) {
-      if (wv == w)
+    auto put = [&](double *dst) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int j = i; j < 4; ++j)
+        for (int j = i; j < 4; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              double *q = gred + ((i * 4 + j) * 4 + v) * 64 + lane;
-              *q = (w == 0 ? 0.0 : *q) + gacc[i][j][v];
-            }
-      __syncthreads();
-    }
+          for (int v = 0; v < 4; ++v) dst[((i * (7 - i) / 2 + j) * 4 + v) * 64 + lane] = gacc[i][j][v];
+    };
+    auto add = [&](const double *src) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) gacc[i][j][v] += src[((i * (7 - i) / 2 + j) * 4 + v) * 64 + lane];
+    };
+    for (int h = W / 2; h >= 1; h /= 2)
+      for (int s0 = 0; s0 < h; s0 += 2) {
+        if (wv >= h + s0 && wv < h + s0 + 2 && wv < 2 * h) put(gsl[wv - h - s0]);
+        __syncthreads();
+        if (wv >= s0 && wv < s0 + 2 && wv + h < 2 * h && wv < h) add(gsl[wv - s0]);
+        __syncthreads();
+      }
+    if (wv == 0) put(gsl[0]);
+    __syncthreads();
     // lo x lo in output order (consecutive threads, consecutive entries):
     // lane ln of block (bi, bj) holds D[ln / 16 + 4 v][ln % 16], so entry
-    // (gi, gj) is gred[((bi * 4 + bj) * 4 + v) * 64 + 16 ((gi % 16) % 4) +
-    // gj % 16] with v = (gi % 16) / 4; only blocks bi <= bj were formed, and
-    // an entry of block bi > bj reads its transpose
+    // (gi, gj) is at (t(bi, bj) * 4 + v) * 64 + 16 ((gi % 16) % 4) + gj % 16
+    // with v = (gi % 16) / 4; only blocks bi <= bj were formed, and an entry
+    // of block bi > bj reads its transpose
     for (int o = threadIdx.x; o < lo * lo; o += kPanelBlock) {
       int gi = o / lo, gj = o - gi * lo;
       if ((gj >> 4) < (gi >> 4)) {
@@ -1440,8 +1459,8 @@ __device__ __forceinline__ void panel_mfma_body(int64_t bid, int64_t nbk, int64_
         gi = gj;
         gj = t;
       }
-      const int r = gi & 15;
-      part[o] = gred[(((gi >> 4) * 4 + (gj >> 4)) * 4 + (r >> 2)) * 64 + 16 * (r & 3) + (gj & 15)];
+      const int r = gi & 15, bi = gi >> 4, bj = gj >> 4;
+      part[o] = gsl[0][((bi * (7 - bi) / 2 + bj) * 4 + (r >> 2)) * 64 + 16 * (r & 3) + (gj & 15)];
     }
   }
   if (RESID) {
