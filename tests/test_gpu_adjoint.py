@@ -100,3 +100,38 @@ def test_backward_end_to_end_from_gpu_solves(lib):
     g2, _ = las.backward(op, loss="sigma_sum", k=k)
     ref2 = oracle.adjoint.gradient_on_pattern(a, "sigma_sum", k)
     assert np.max(np.abs(g2 - ref2)) <= 1e-8 * np.max(np.abs(ref2))
+
+
+@pytest.mark.parametrize("layout", ["columns", "rows"])
+@pytest.mark.parametrize("fdt,precision", [(np.float64, "f64"), (np.float32, "f32"), (np.float64, "f32"),
+                                           (np.float32, "f64")])
+def test_backward_factor_layouts_and_hand_over(lib, layout, fdt, precision):
+    # The sigma-sum gradient is the SDDMM sum_i u_i[r] v_i[c] of whatever
+    # factors the record holds, so any factors pin the factor upload (vectors
+    # as columns, m x d, or as rows, d x m as LAS2 returns them; FP64 or FP32
+    # arrays into FP64 or FP32 panels) and the gradient's hand-over. 140,000
+    # rows of 10 FP64 factors (11 MB) and 1.1 M nonzeros (8.8 MB of FP64
+    # gradient) take the staged paths; the direct paths are covered by the
+    # small cases above.
+    from single_svdlib.utils import Diagnostics, SvdRec
+    m, n, d, k = 140_000, 3_000, 10, 7
+    print(f"case layouts {layout} {np.dtype(fdt).name} {precision} {m}x{n} d={d} k={k}", flush=True)
+    rng = np.random.default_rng(17)
+    a = random_csr(m, n, 0.0026, seed=17)
+    u = rng.normal(size=(m, d)).astype(fdt)
+    vt = rng.normal(size=(d, n)).astype(fdt)
+    s = np.sort(rng.uniform(1.0, 2.0, size=d))[::-1].astype(fdt)
+    diag = Diagnostics(0, d, 0, False, 0, d, d, d, [0.0, 0.0], 0.0, 0)
+    uu = np.ascontiguousarray(u if layout == "columns" else u.T)
+    rec = SvdRec(d=d, u=uu, s=s, vt=np.ascontiguousarray(vt), diagnostics=diag)
+    mat = a if precision == "f64" else a.astype(np.float32)
+    g, info = lib.adjoint.backward(lib.CsrMatrix.from_scipy(mat), rec, loss="sigma_sum", k=k, precision=precision)
+    coo = a.tocoo()
+    u64, vt64 = u.astype(np.float64), vt.astype(np.float64)
+    ref = np.einsum("ij,ij->i", u64[coo.row, :k], vt64[:k, coo.col].T)
+    # the gradient in CSR order of the matrix (tocoo keeps it)
+    scale = np.max(np.abs(ref))
+    tol = 1e-12 if (precision == "f64" and fdt == np.float64) else 1e-5
+    assert g.shape == (a.nnz,)
+    assert np.max(np.abs(g - ref)) <= tol * scale, np.max(np.abs(g - ref)) / scale
+    assert np.isclose(info["loss"], float(np.sum(s[:k].astype(np.float64))), rtol=1e-6)
