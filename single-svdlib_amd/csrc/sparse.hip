@@ -3092,23 +3092,37 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(SddmmArgs<T, G> a) {
       }
       const int n = min(R, ke - base);
       A mine = A(0);
-      for (int j = 0; j < n; ++j) {
-        const int cj = __shfl(c, j, R);
-        A part = A(0);
-        if (colok && cj >= 0) {
-          A rv[NV];
-          ld_panel<A, T>(a.r1 + (int64_t)cj * a.ldr + col, rv);
+      // U nonzeros at a time: their right rows are gathered together (clamped
+      // to row 0 and column 0 where the entry or the lane's columns are
+      // absent, so no load waits behind a branch), then each product is
+      // formed and reduced as for a single nonzero (the same bits)
+      constexpr int U = 4;
+      for (int j0 = 0; j0 < n; j0 += U) {
+        int cj[U];
+        A rv1[U][NV], rv2[U][NV];
 #pragma unroll
-          for (int v = 0; v < NV; ++v) part += lv1[v] * rv[v];
-          if (two) {
-            ld_panel<A, T>(a.r2 + (int64_t)cj * a.ldr + col, rv);
+        for (int u = 0; u < U; ++u) cj[u] = __shfl(c, (j0 + u) & (R - 1), R);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) part += lv2[v] * rv[v];
-          }
+        for (int u = 0; u < U; ++u) {
+          const int64_t rr = cj[u] >= 0 ? (int64_t)cj[u] : 0;
+          const int cc = colok ? col : 0;
+          ld_panel<A, T>(a.r1 + rr * a.ldr + cc, rv1[u]);
+          if (two) ld_panel<A, T>(a.r2 + rr * a.ldr + cc, rv2[u]);
         }
 #pragma unroll
-        for (int off = R / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, R);
-        if (lane == j) mine = part;
+        for (int u = 0; u < U; ++u) {
+          A part = A(0);
+          if (colok && cj[u] >= 0 && j0 + u < n) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) part += lv1[v] * rv1[u][v];
+            if (two)
+#pragma unroll
+              for (int v = 0; v < NV; ++v) part += lv2[v] * rv2[u][v];
+          }
+#pragma unroll
+          for (int off = R / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, R);
+          if (lane == j0 + u) mine = part;
+        }
       }
       if (lane < n) a.grad[base + lane] = static_cast<G>(mine);
     }
